@@ -1,0 +1,213 @@
+/*
+ * pamtra_amd.h -- C-ABI of the MI355X passive forward operator.
+ *
+ * This is the drop-in boundary for the work the reference does inside
+ * `pypamtralib.run_pamtra()` (src/pyPamtraLib.f90:18-148) for passive-only
+ * runs: run_rt (src/run_rt.f90:1-208) -> hydrometeor_extinction / Mie /
+ * scatcnv -> rt4 / radtran4 -> collect_output.  The reference binds that
+ * routine through f2py module globals which python/pyPamtra/libWrapper.py:24-235
+ * fills from the `nmlSet`, descriptor-file and profile dictionaries.  Here the
+ * same information travels as flat C-contiguous arrays with explicit shapes
+ * (numpy's default C order of the pyPamtra dictionaries), strings are replaced
+ * by enums, and nothing is held in global state.
+ *
+ * Plain pointers and sizes only: no torch or HIP types appear in the
+ * signatures.  All arrays are C order (last index fastest).
+ */
+#ifndef PAMTRA_AMD_H
+#define PAMTRA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Quadrature and polarisation of the RT4 solver (src/settings.f90:12-31). */
+#define PAM_NUMMU 16
+#define PAM_NSTOKES 2
+#define PAM_NANGLES (2 * PAM_NUMMU)
+#define PAM_MAXNLEG 200
+
+/* Error status codes (src/report_module.f90:9-14). */
+enum pam_status { PAM_SUCCESS = 0, PAM_WARNING = 1, PAM_FATAL = 2, PAM_INFO = 3 };
+
+/* descriptor file column "dist_name" (src/make_dist_param.f90). */
+enum pam_dist_name {
+  PAM_DIST_DISABLED = 0,
+  PAM_DIST_MONO = 1,
+  PAM_DIST_EXP = 2,
+  PAM_DIST_EXP_FIELD_T = 3,
+  PAM_DIST_EXP_RYAN = 4,
+  PAM_DIST_MGAMMA = 5,
+  PAM_DIST_LOGN = 6
+};
+
+/* descriptor file column "scat_name" (src/scatProperties.f90:722-769). */
+enum pam_scat_name { PAM_SCAT_DISABLED = 0, PAM_SCAT_MIE_SPHERE = 1 };
+
+/* arithmetic of the device path */
+enum pam_precision { PAM_PREC_FP64 = 0, PAM_PREC_MIXED = 1 };
+
+/* nmlSet entries that reach the passive path (python/pyPamtra/core.py:88-157,
+ * defaults src/settings.f90:454-573).  gas_mod: 0 = 'R98'; liq_mod: 0 = 'TKC';
+ * outpol: 0 = 'VH', 1 = 'IQ'.  active must be 0 (the radar simulator is not
+ * part of this library and is refused with PAM_FATAL). */
+typedef struct pam_settings {
+  int32_t passive;
+  int32_t active;
+  int32_t lgas_extinction;
+  int32_t lhyd_absorption;
+  int32_t lhyd_scattering;
+  int32_t lhyd_emission;
+  int32_t hydro_includehydroinrhoair;
+  int32_t hydro_limit_density_area;
+  int32_t hydro_adaptive_grid;
+  int32_t conserve_mass_rescale_dsd;
+  int32_t gas_mod;
+  int32_t liq_mod;
+  int32_t outpol;
+  int32_t precision;
+  double hydro_threshold;
+  double hydro_softsphere_min_density;
+  double emissivity;
+} pam_settings;
+
+/* One row of the 20-column descriptor file
+ * (python/pyPamtra/descriptorFile.py:19-20).  hydro_name and vel_size_mod do
+ * not enter the passive path and are not carried. */
+typedef struct pam_hydro_desc {
+  double as_ratio;
+  int32_t liq_ice;
+  int32_t moment_in;
+  double rho_ms;
+  double a_ms;
+  double b_ms;
+  double alpha_as;
+  double beta_as;
+  int32_t nbin;
+  int32_t dist_name;
+  double p_1, p_2, p_3, p_4;
+  double d_1, d_2;
+  int32_t scat_name;
+  int32_t pad_;
+  double canting;
+} pam_hydro_desc;
+
+/* Optional per-(x,y,z,hydro) overrides of descriptor columns, the
+ * descriptorFile4D of libWrapper.py:112-116 (hydrometeor_extinction.f90:153-245).
+ * Each pointer is NULL or an array of shape (ngridx, ngridy, max_nlyrs, nhydro). */
+typedef struct pam_hydro_4d {
+  const double *as_ratio;
+  const double *rho_ms;
+  const double *a_ms;
+  const double *b_ms;
+  const double *alpha_as;
+  const double *beta_as;
+  const double *p_1, *p_2, *p_3, *p_4;
+  const double *d_1, *d_2;
+  const int32_t *nbin;
+} pam_hydro_4d;
+
+/* The profile dictionary `pam.p` (python/pyPamtra/core.py:756-976).  NaN
+ * marks a missing value that fillMissing_atmosphere_vars derives
+ * (src/vars_atmosphere.f90:671-901).  Layer arrays may be NULL (all missing).
+ * sfc_emissivity is an extension: a per-column emissivity for sfc_type=-9999
+ * columns (NULL: settings.emissivity for every column, the reference's only
+ * form, src/sfc_optics.f90:79-81). */
+typedef struct pam_profile {
+  int32_t ngridx, ngridy, max_nlyrs, noutlevels, nhydro;
+  int32_t pad_;
+  const int32_t *nlyrs;          /* (nx, ny) */
+  const double *hgt_lev;         /* (nx, ny, L+1) [m] */
+  const double *temp_lev;        /* (nx, ny, L+1) [K] */
+  const double *press_lev;       /* (nx, ny, L+1) [Pa] */
+  const double *relhum_lev;      /* (nx, ny, L+1) [%] */
+  const double *hgt;             /* (nx, ny, L) or NULL */
+  const double *temp;            /* (nx, ny, L) or NULL */
+  const double *press;           /* (nx, ny, L) or NULL */
+  const double *relhum;          /* (nx, ny, L) or NULL */
+  const double *hydro_q;         /* (nx, ny, L, nhydro) [kg/kg] */
+  const double *hydro_n;         /* (nx, ny, L, nhydro) [#/kg] */
+  const double *hydro_reff;      /* (nx, ny, L, nhydro) [m] */
+  const double *groundtemp;      /* (nx, ny) [K], NaN: temp_lev at the ground */
+  const double *wind10u;         /* (nx, ny) [m/s] */
+  const double *wind10v;         /* (nx, ny) [m/s] */
+  const double *sfc_salinity;    /* (nx, ny) [ppt] */
+  const double *sfc_emissivity;  /* (nx, ny) or NULL */
+  const int32_t *sfc_type;       /* (nx, ny): -9999 constant emissivity, 0 ocean */
+  const int32_t *sfc_model;      /* (nx, ny): -9999/0 TESSEM2 */
+  const char *sfc_refl;          /* (nx, ny) one byte each: 'S', 'F' or 'L' */
+  const double *obs_height;      /* (nx, ny, noutlevels) [m] */
+} pam_profile;
+
+/* Result arrays (vars_output.f90:134-141, keys of libWrapper.py:214-223).
+ * Every pointer except tb may be NULL.  Diagnostic per-layer arrays are
+ * indexed bottom-up like the profile. */
+typedef struct pam_results {
+  double *tb;            /* (nx, ny, noutlevels, 32, nfreq, 2) [K] */
+  double *emissivity;    /* (nx, ny, 2, nfreq, 16) */
+  double *angles_deg;    /* (32) */
+  int32_t *status;       /* (nx, ny, nfreq) */
+  double *kextatmo;      /* (nx, ny, L, nfreq) gas extinction [Np/m] */
+  double *kexttot;       /* (nx, ny, L, nfreq) hydrometeor extinction [1/m] */
+  double *salbedo;       /* (nx, ny, L, nfreq) single scattering albedo */
+  int32_t *nlegen;       /* (nx, ny, L, nfreq) Legendre order, -1 clear */
+  int32_t *ndoubles;     /* (nx, ny, L, nfreq) doublings, -1 clear layer */
+  char *msg;             /* first error message, NUL terminated */
+  int32_t msglen;
+  int32_t pad_;
+} pam_results;
+
+typedef struct pam_context pam_context;
+
+/* Library version string. */
+const char *pam_version(void);
+
+/* Number of HIP devices visible to the library (0 when there is none). */
+int pam_device_count(void);
+
+/* Creates a context bound to one HIP device.  Replaces the module-global
+ * state the reference resets in deallocate_everything (libWrapper.py:48).
+ * Returns PAM_SUCCESS or PAM_FATAL (no device, allocation failure). */
+int pam_context_create(pam_context **ctx, int device);
+
+/* Frees the device workspace and the context. */
+int pam_context_destroy(pam_context *ctx);
+
+/* Runs the passive forward operator for every (column, frequency):
+ * the replacement of run_pamtra (src/pyPamtraLib.f90:119-133), host arrays in
+ * and out.  freqs in GHz, sorted ascending (core.py:1536).  Returns the worst
+ * status over all columns: PAM_SUCCESS, PAM_WARNING or PAM_FATAL; never
+ * aborts the process (radmat.f90:180-189 STOPs, this returns PAM_FATAL). */
+int pam_run(pam_context *ctx, const pam_settings *settings,
+            const pam_hydro_desc *hydros, const pam_hydro_4d *hydro4d,
+            const pam_profile *profile, const double *freqs, int32_t nfreq,
+            pam_results *results);
+
+/* As pam_run, but every array in `profile`, `hydro4d` and `results` (except
+ * msg) is device memory already resident on the context's device, and the
+ * work is queued on `stream` (a hipStream_t, NULL for the null stream).
+ * freqs and hydros stay host arrays.  The call returns after the results are
+ * written (it synchronises `stream`). */
+int pam_run_device(pam_context *ctx, const pam_settings *settings,
+                   const pam_hydro_desc *hydros, const pam_hydro_4d *hydro4d,
+                   const pam_profile *profile, const double *freqs,
+                   int32_t nfreq, pam_results *results, void *stream);
+
+/* Kernel-level timings of the last call (milliseconds, measured with HIP
+ * events on the call's stream), in the order of pam_stage_names(). */
+int pam_last_stage_times(const pam_context *ctx, double *ms, int32_t n);
+const char *pam_stage_names(void);
+
+/* Work counters of the last call: [0] columns, [1] channels,
+ * [2] scattering (layer, channel) items, [3] total doubling steps,
+ * [4] scattering adding steps, [5] clear adding steps, [6] Mie bins,
+ * [7] sum over Mie bins of the series length. */
+int pam_last_work(const pam_context *ctx, int64_t *counts, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PAMTRA_AMD_H */

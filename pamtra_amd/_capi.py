@@ -1,0 +1,400 @@
+"""ctypes mirror of include/pamtra_amd.h and the marshalling of pyPamtra
+dictionaries into it.
+
+The reference pushes the same information into Fortran module globals through
+f2py (python/pyPamtra/libWrapper.py:24-235); here it becomes flat C-order
+arrays plus a few structs.  Arrays may be numpy arrays (host, for ``pam_run``)
+or torch tensors already on the device (for ``pam_run_device``).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+NUMMU = 16
+NSTOKES = 2
+NANGLES = 2 * NUMMU
+
+PAM_SUCCESS, PAM_WARNING, PAM_FATAL, PAM_INFO = 0, 1, 2, 3
+
+DIST_NAMES = {
+    "disabled": 0,
+    "mono": 1,
+    "exp": 2,
+    "exp_field_t": 3,
+    "exp_ryan": 4,
+    "mgamma": 5,
+    "logn": 6,
+}
+SCAT_NAMES = {"disabled": 0, "mie-sphere": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class PamSettings(C.Structure):
+    _fields_ = [
+        ("passive", C.c_int32),
+        ("active", C.c_int32),
+        ("lgas_extinction", C.c_int32),
+        ("lhyd_absorption", C.c_int32),
+        ("lhyd_scattering", C.c_int32),
+        ("lhyd_emission", C.c_int32),
+        ("hydro_includehydroinrhoair", C.c_int32),
+        ("hydro_limit_density_area", C.c_int32),
+        ("hydro_adaptive_grid", C.c_int32),
+        ("conserve_mass_rescale_dsd", C.c_int32),
+        ("gas_mod", C.c_int32),
+        ("liq_mod", C.c_int32),
+        ("outpol", C.c_int32),
+        ("precision", C.c_int32),
+        ("hydro_threshold", C.c_double),
+        ("hydro_softsphere_min_density", C.c_double),
+        ("emissivity", C.c_double),
+    ]
+
+
+class PamHydroDesc(C.Structure):
+    _fields_ = [
+        ("as_ratio", C.c_double),
+        ("liq_ice", C.c_int32),
+        ("moment_in", C.c_int32),
+        ("rho_ms", C.c_double),
+        ("a_ms", C.c_double),
+        ("b_ms", C.c_double),
+        ("alpha_as", C.c_double),
+        ("beta_as", C.c_double),
+        ("nbin", C.c_int32),
+        ("dist_name", C.c_int32),
+        ("p_1", C.c_double),
+        ("p_2", C.c_double),
+        ("p_3", C.c_double),
+        ("p_4", C.c_double),
+        ("d_1", C.c_double),
+        ("d_2", C.c_double),
+        ("scat_name", C.c_int32),
+        ("pad_", C.c_int32),
+        ("canting", C.c_double),
+    ]
+
+
+HYDRO4D_KEYS = ["as_ratio", "rho_ms", "a_ms", "b_ms", "alpha_as", "beta_as",
+                "p_1", "p_2", "p_3", "p_4", "d_1", "d_2"]
+
+
+class PamHydro4D(C.Structure):
+    _fields_ = [(k, _dp) for k in HYDRO4D_KEYS] + [("nbin", _ip)]
+
+
+PROFILE_DOUBLE_KEYS = ["hgt_lev", "temp_lev", "press_lev", "relhum_lev", "hgt", "temp",
+                       "press", "relhum", "hydro_q", "hydro_n", "hydro_reff", "groundtemp",
+                       "wind10u", "wind10v", "sfc_salinity", "sfc_emissivity"]
+
+
+class PamProfile(C.Structure):
+    _fields_ = [
+        ("ngridx", C.c_int32),
+        ("ngridy", C.c_int32),
+        ("max_nlyrs", C.c_int32),
+        ("noutlevels", C.c_int32),
+        ("nhydro", C.c_int32),
+        ("pad_", C.c_int32),
+        ("nlyrs", _ip),
+    ] + [(k, _dp) for k in PROFILE_DOUBLE_KEYS[:16]] + [
+        ("sfc_type", _ip),
+        ("sfc_model", _ip),
+        ("sfc_refl", C.c_char_p),
+        ("obs_height", _dp),
+    ]
+
+
+class PamResults(C.Structure):
+    _fields_ = [
+        ("tb", _dp),
+        ("emissivity", _dp),
+        ("angles_deg", _dp),
+        ("status", _ip),
+        ("kextatmo", _dp),
+        ("kexttot", _dp),
+        ("salbedo", _dp),
+        ("nlegen", _ip),
+        ("ndoubles", _ip),
+        ("msg", C.c_char_p),
+        ("msglen", C.c_int32),
+        ("pad_", C.c_int32),
+    ]
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _ptr(a, ctype):
+    """Pointer to the data of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return C.cast(None, C.POINTER(ctype))
+    if _is_torch(a):
+        return C.cast(C.c_void_p(a.data_ptr()), C.POINTER(ctype))
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _as_f64(a, shape=None):
+    if a is None:
+        return None
+    if _is_torch(a):
+        return a
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+    if shape is not None:
+        arr = arr.reshape(shape)
+    return arr
+
+
+def _as_i32(a, shape=None):
+    if a is None:
+        return None
+    if _is_torch(a):
+        return a
+    arr = np.ascontiguousarray(np.asarray(a).astype(np.int32))
+    if shape is not None:
+        arr = arr.reshape(shape)
+    return arr
+
+
+def _flag(v) -> int:
+    return 1 if bool(v) else 0
+
+
+def make_settings(nmlSets: dict, sets: dict | None = None) -> PamSettings:
+    """nmlSet -> pam_settings.  Options outside the passive Mie path fail
+    loudly, as the reference's Fortran would for an unknown model name."""
+    s = PamSettings()
+    g = nmlSets.get
+    s.passive = _flag(g("passive", True))
+    s.active = _flag(g("active", True))
+    s.lgas_extinction = _flag(g("lgas_extinction", True))
+    s.lhyd_absorption = _flag(g("lhyd_absorption", True))
+    s.lhyd_scattering = _flag(g("lhyd_scattering", True))
+    s.lhyd_emission = _flag(g("lhyd_emission", True))
+    s.hydro_includehydroinrhoair = _flag(g("hydro_includehydroinrhoair", True))
+    s.hydro_limit_density_area = _flag(g("hydro_limit_density_area", True))
+    s.hydro_adaptive_grid = _flag(g("hydro_adaptive_grid", True))
+    s.conserve_mass_rescale_dsd = _flag(g("conserve_mass_rescale_dsd", True))
+    gas = str(g("gas_mod", "R98")).strip()
+    if gas != "R98":
+        raise NotImplementedError(f"gas_mod {gas!r}: only 'R98' is implemented")
+    s.gas_mod = 0
+    liq = str(g("liq_mod", "TKC")).strip()
+    if liq != "TKC":
+        raise NotImplementedError(f"liq_mod {liq!r}: only 'TKC' is implemented")
+    s.liq_mod = 0
+    outpol = str(g("outpol", "VH")).strip()
+    if outpol not in ("VH", "IQ"):
+        raise ValueError(f"outpol {outpol!r} must be 'VH' or 'IQ'")
+    s.outpol = 0 if outpol == "VH" else 1
+    prec = (sets or {}).get("precision", "fp64")
+    if prec not in PRECISIONS:
+        raise ValueError(f"precision {prec!r} must be one of {sorted(PRECISIONS)}")
+    s.precision = PRECISIONS[prec]
+    s.hydro_threshold = float(g("hydro_threshold", 1e-10))
+    s.hydro_softsphere_min_density = float(g("hydro_softsphere_min_density", 10.0))
+    s.emissivity = float(g("emissivity", 0.6))
+    return s
+
+
+def make_hydros(descriptorFile) -> C.Array:
+    """descriptor recarray (descriptorFile.py:19-20 layout) -> pam_hydro_desc[]."""
+    n = len(descriptorFile)
+    arr = (PamHydroDesc * max(n, 1))()
+    names = descriptorFile.dtype.names
+    for i in range(n):
+        row = descriptorFile[i]
+        h = arr[i]
+
+        def num(key, default=-99.0):
+            return float(row[key]) if key in names else default
+
+        dist = str(row["dist_name"]).strip()
+        if dist not in DIST_NAMES:
+            raise NotImplementedError(f"dist_name {dist!r} is not implemented")
+        scat = str(row["scat_name"]).strip()
+        if scat not in SCAT_NAMES:
+            raise NotImplementedError(f"scat_name {scat!r}: only 'mie-sphere' is implemented")
+        h.as_ratio = num("as_ratio")
+        h.liq_ice = int(row["liq_ice"])
+        h.moment_in = int(row["moment_in"])
+        h.rho_ms = num("rho_ms")
+        h.a_ms = num("a_ms")
+        h.b_ms = num("b_ms")
+        h.alpha_as = num("alpha_as")
+        h.beta_as = num("beta_as")
+        h.nbin = int(row["nbin"]) if "nbin" in names else 0
+        h.dist_name = DIST_NAMES[dist]
+        h.p_1, h.p_2, h.p_3, h.p_4 = num("p_1"), num("p_2"), num("p_3"), num("p_4")
+        h.d_1, h.d_2 = num("d_1"), num("d_2")
+        h.scat_name = SCAT_NAMES[scat]
+        h.canting = num("canting")
+    return arr
+
+
+@dataclass
+class Marshalled:
+    settings: PamSettings
+    hydros: object
+    hydro4d: PamHydro4D
+    profile: PamProfile
+    nx: int
+    ny: int
+    max_nlyrs: int
+    noutlevels: int
+    nhydro: int
+    keep: list = field(default_factory=list)
+
+
+def marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile) -> Marshalled:
+    """Build the C structs from the pyPamtra dictionaries.  The returned
+    object keeps every array alive for the duration of the call."""
+    keep = []
+    nx, ny = int(profile["ngridx"]), int(profile["ngridy"])
+    L = int(profile["max_nlyrs"])
+    nout = int(profile.get("noutlevels", 2))
+    nh = len(descriptorFile)
+    settings = make_settings(nmlSets, sets)
+    hydros = make_hydros(descriptorFile)
+    keep.append(hydros)
+    h4 = PamHydro4D()
+    for key, arr in (descriptorFile4D or {}).items():
+        if key == "nbin":
+            a = _as_i32(arr, (nx, ny, L, nh))
+            keep.append(a)
+            h4.nbin = _ptr(a, C.c_int32)
+        elif key in HYDRO4D_KEYS:
+            a = _as_f64(arr, (nx, ny, L, nh))
+            keep.append(a)
+            setattr(h4, key, _ptr(a, C.c_double))
+        else:
+            raise NotImplementedError(f"4D descriptor field {key!r} is not supported")
+    p = PamProfile()
+    p.ngridx, p.ngridy, p.max_nlyrs, p.noutlevels, p.nhydro = nx, ny, L, nout, nh
+    shapes = {
+        "hgt_lev": (nx, ny, L + 1), "temp_lev": (nx, ny, L + 1), "press_lev": (nx, ny, L + 1),
+        "relhum_lev": (nx, ny, L + 1), "hgt": (nx, ny, L), "temp": (nx, ny, L),
+        "press": (nx, ny, L), "relhum": (nx, ny, L), "hydro_q": (nx, ny, L, nh),
+        "hydro_n": (nx, ny, L, nh), "hydro_reff": (nx, ny, L, nh), "groundtemp": (nx, ny),
+        "wind10u": (nx, ny), "wind10v": (nx, ny), "sfc_salinity": (nx, ny),
+        "sfc_emissivity": (nx, ny),
+    }
+    for key in PROFILE_DOUBLE_KEYS:
+        v = profile.get(key)
+        if v is None or (key.startswith("hydro_") and nh == 0):
+            continue
+        a = _as_f64(v, shapes[key])
+        keep.append(a)
+        setattr(p, key, _ptr(a, C.c_double))
+    nl = _as_i32(profile["nlyrs"], (nx, ny))
+    keep.append(nl)
+    p.nlyrs = _ptr(nl, C.c_int32)
+    for key in ("sfc_type", "sfc_model"):
+        v = profile.get(key)
+        if v is not None:
+            a = _as_i32(v, (nx, ny))
+            keep.append(a)
+            setattr(p, key, _ptr(a, C.c_int32))
+    refl = profile.get("sfc_refl")
+    if refl is not None:
+        if _is_torch(refl):
+            p.sfc_refl = C.cast(C.c_void_p(refl.data_ptr()), C.c_char_p)
+        else:
+            r = np.asarray(refl)
+            if r.dtype.kind in ("U", "S"):
+                flat = [str(x.decode() if isinstance(x, bytes) else x)[:1] or "S"
+                        for x in r.reshape(-1)]
+                buf = C.create_string_buffer("".join(flat).encode("ascii"), nx * ny + 1)
+            else:
+                buf = C.create_string_buffer(r.astype(np.uint8).tobytes(), nx * ny + 1)
+            keep.append(buf)
+            p.sfc_refl = C.cast(buf, C.c_char_p)
+    oh = profile.get("obs_height")
+    if oh is not None:
+        a = _as_f64(oh, (nx, ny, nout))
+        keep.append(a)
+        p.obs_height = _ptr(a, C.c_double)
+    return Marshalled(settings, hydros, h4, p, nx, ny, L, nout, nh, keep)
+
+
+@dataclass
+class HostResults:
+    struct: PamResults
+    arrays: dict
+    keep: list
+
+
+def alloc_results(m: Marshalled, nfreq: int, diagnostics: bool = False,
+                  msglen: int = 512) -> HostResults:
+    nx, ny, L, nout = m.nx, m.ny, m.max_nlyrs, m.noutlevels
+    arrs = {
+        "tb": np.full((nx, ny, nout, NANGLES, nfreq, NSTOKES), np.nan),
+        "emissivity": np.full((nx, ny, NSTOKES, nfreq, NUMMU), np.nan),
+        "angles_deg": np.full((NANGLES,), np.nan),
+        "status": np.zeros((nx, ny, nfreq), dtype=np.int32),
+    }
+    if diagnostics:
+        arrs["kextatmo"] = np.full((nx, ny, L, nfreq), np.nan)
+        arrs["kexttot"] = np.full((nx, ny, L, nfreq), np.nan)
+        arrs["salbedo"] = np.full((nx, ny, L, nfreq), np.nan)
+        arrs["nlegen"] = np.full((nx, ny, L, nfreq), -9999, dtype=np.int32)
+        arrs["ndoubles"] = np.full((nx, ny, L, nfreq), -9999, dtype=np.int32)
+    r = PamResults()
+    for k, a in arrs.items():
+        setattr(r, k, _ptr(a, C.c_int32 if a.dtype == np.int32 else C.c_double))
+    msg = C.create_string_buffer(msglen)
+    r.msg = C.cast(msg, C.c_char_p)
+    r.msglen = msglen
+    return HostResults(r, arrs, [msg])
+
+
+def results_message(hr: HostResults) -> str:
+    return hr.keep[0].value.decode(errors="replace")
+
+
+def freqs_array(freqs):
+    f = np.ascontiguousarray(np.asarray(freqs, dtype=np.float64).reshape(-1))
+    return f, f.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def declare_library(lib: C.CDLL) -> C.CDLL:
+    """Argument and return types of the product library's C-ABI."""
+    lib.pam_version.restype = C.c_char_p
+    lib.pam_version.argtypes = []
+    lib.pam_device_count.restype = C.c_int
+    lib.pam_device_count.argtypes = []
+    lib.pam_context_create.restype = C.c_int
+    lib.pam_context_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+    lib.pam_context_destroy.restype = C.c_int
+    lib.pam_context_destroy.argtypes = [C.c_void_p]
+    run_args = [C.c_void_p, C.POINTER(PamSettings), C.POINTER(PamHydroDesc),
+                C.POINTER(PamHydro4D), C.POINTER(PamProfile), C.POINTER(C.c_double),
+                C.c_int32, C.POINTER(PamResults)]
+    lib.pam_run.restype = C.c_int
+    lib.pam_run.argtypes = run_args
+    lib.pam_run_device.restype = C.c_int
+    lib.pam_run_device.argtypes = run_args + [C.c_void_p]
+    lib.pam_last_stage_times.restype = C.c_int
+    lib.pam_last_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int32]
+    lib.pam_stage_names.restype = C.c_char_p
+    lib.pam_stage_names.argtypes = []
+    lib.pam_last_work.restype = C.c_int
+    lib.pam_last_work.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32]
+    return lib
+
+
+EXPORTED_SYMBOLS = ["pam_version", "pam_device_count", "pam_context_create",
+                    "pam_context_destroy", "pam_run", "pam_run_device",
+                    "pam_last_stage_times", "pam_stage_names", "pam_last_work"]
+
+PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(PACKAGE_DIR, "libpamtra_amd.so")

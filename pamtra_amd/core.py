@@ -1,0 +1,299 @@
+"""The pyPamtra user API (python/pyPamtra/core.py:74-1568), passive path.
+
+Same names and dictionaries as the reference: ``pyPamtra()`` with ``nmlSet``,
+``set``, ``df`` (descriptor table), ``p`` (profile), ``r`` (results),
+``createProfile(**kwargs)`` and ``runPamtra(freqs)``.  ``runPamtra`` hands the
+dictionaries to ``libWrapper.PamtraFortranWrapper``, which runs the HIP
+library instead of the f2py Fortran module.
+"""
+
+from __future__ import annotations
+
+import time
+import warnings
+from copy import deepcopy
+
+import numpy as np
+
+from .descriptorFile import pamDescriptorFile
+from .libWrapper import PamtraFortranWrapper
+
+missingNumber = -9999.
+missingIntNumber = int(missingNumber)
+
+
+class pyPamtra(object):
+    """Settings, profile and results of a forward-operator run
+    (python/pyPamtra/core.py:74-274)."""
+
+    def __init__(self):
+        self.default_p_vars = [
+            "timestamp", "lat", "lon", "wind10u", "wind10v", "hgt", "press", "temp", "relhum",
+            "hgt_lev", "press_lev", "temp_lev", "relhum_lev", "q", "hydro_q", "hydro_n",
+            "hydro_reff", "obs_height", "ngridy", "ngridx", "max_nlyrs", "nlyrs", "model_i",
+            "model_j", "unixtime", "airturb", "radar_prop", "groundtemp", "wind_w", "wind_uv",
+            "turb_edr", "sfc_type", "sfc_model", "sfc_refl", "sfc_salinity", "sfc_slf",
+            "sfc_sif", "sfc_emissivity"]
+        n = dict()
+        n["hydro_threshold"] = 1.e-10
+        n["write_nc"] = True
+        n["data_path"] = '$PAMTRA_DATADIR'
+        n["save_psd"] = False
+        n["save_ssp"] = False
+        n["add_obs_height_to_layer"] = False
+        n["conserve_mass_rescale_dsd"] = True
+        n["outpol"] = 'VH'
+        n["file_desc"] = ''
+        n["creator"] = 'Pamtrauser'
+        n["active"] = True
+        n["passive"] = True
+        n["radar_mode"] = "simple"
+        n["randomseed"] = 0
+        n["emissivity"] = 0.6
+        n["lgas_extinction"] = True
+        n["gas_mod"] = 'R98'
+        n["lhyd_absorption"] = True
+        n["lhyd_scattering"] = True
+        n["lhyd_emission"] = True
+        n["liq_mod"] = "TKC"
+        n["hydro_includehydroinrhoair"] = True
+        n["hydro_fullspec"] = False
+        n["hydro_limit_density_area"] = True
+        n["hydro_softsphere_min_density"] = 10.
+        n["hydro_adaptive_grid"] = True
+        n["tmatrix_db"] = "none"
+        n["tmatrix_db_path"] = "database/"
+        n["radar_nfft"] = 256
+        n["radar_no_ave"] = 150
+        n["radar_max_v"] = 7.885
+        n["radar_min_v"] = -7.885
+        n["radar_pnoise0"] = -32.23
+        n["radar_allow_negative_dD_dU"] = False
+        n["radar_airmotion"] = False
+        n["radar_airmotion_model"] = "constant"
+        n["radar_airmotion_vmin"] = -4.e0
+        n["radar_airmotion_vmax"] = +4.e0
+        n["radar_airmotion_linear_steps"] = 30
+        n["radar_airmotion_step_vmin"] = 0.5e0
+        n["radar_peak_min_bins"] = 2
+        n["radar_aliasing_nyquist_interv"] = 1
+        n["radar_save_noise_corrected_spectra"] = False
+        n["radar_use_hildebrand"] = False
+        n["radar_peak_snr_definition"] = 'log'
+        n["radar_peak_min_snr"] = -10
+        n["radar_convolution_fft"] = True
+        n["radar_smooth_spectrum"] = True
+        n["radar_k2"] = 0.93
+        n["radar_npeaks"] = 1
+        n["radar_noise_distance_factor"] = 2.0
+        n["radar_receiver_uncertainty_std"] = 0.e0
+        n["radar_receiver_miscalibration"] = 0.e0
+        n["radar_attenuation"] = "disabled"
+        n["radar_polarisation"] = "NN"
+        n["radar_use_wider_peak"] = False
+        n["radar_integration_time"] = 1.4
+        n["radar_fwhr_beamwidth_deg"] = 0.31
+        n["liblapack"] = True
+        self.nmlSet = n
+
+        self.set = dict()
+        self.set["pyVerbose"] = 0
+        self.set["verbose"] = 0
+        self.set["freqs"] = []
+        self.set["namelist_file"] = "TMPFILE"
+        # extension: arithmetic of the device path, "fp64" or "mixed"
+        self.set["precision"] = "fp64"
+        self._nmlDefaultSet = deepcopy(self.nmlSet)
+
+        self.dimensions = {
+            "tb": ["gridx", "gridy", "outlevels", "angles", "frequency", "passive_npol"],
+            "hgt_lev": ["ngridx", "ngridy", "max_nlyrs+1"],
+            "hydro_q": ["ngridx", "ngridy", "max_nlyrs", "nhydro"],
+        }
+        self.units = {"tb": "K", "hgt_lev": "m", "temp_lev": "K", "p_lev": "Pa",
+                      "relhum_lev": "%", "hydro_q": "kg/kg"}
+        self._nstokes = 2
+        self._nangles = 16
+        self.p = dict()
+        self.r = dict()
+        self.p["ngridx"] = 0
+        self.p["ngridy"] = 0
+        self.p["max_nlyrs"] = 0
+        self._shape2D = (0, 0)
+        self._shape3D = (0, 0, 0)
+        self._shape4D = (0, 0, 0, 0)
+        self.df = pamDescriptorFile(self)
+
+    # ------------------------------------------------------------------
+    def createProfile(self, **kwargs):
+        """Builds ``self.p`` from level/layer fields (core.py:756-976).
+        hgt(_lev), temp(_lev), press(_lev) and relhum(_lev) are mandatory;
+        everything else gets the reference's defaults."""
+        for key in list(kwargs.keys()):
+            v = kwargs[key]
+            if isinstance(v, np.ma.MaskedArray):
+                v = v.filled(np.nan)
+            if isinstance(v, np.char.chararray):
+                continue
+            kwargs[key] = np.array(v)
+
+        if not self.df.data.shape[0] > 0:
+            warnings.warn("No descriptor file defined. Assuming dry profile without hydrometeors")
+        for qv in ("hydro_q", "hydro_n", "hydro_reff"):
+            if qv in kwargs:
+                assert self.df.nhydro == kwargs[qv].shape[-1]
+
+        for key in kwargs:
+            if (key not in self.default_p_vars and key.split("+")[0] not in self.default_p_vars) \
+                    or key in ("model_i", "model_j"):
+                raise TypeError("Could not parse " + key)
+        if not ((("hgt_lev" in kwargs) or ("hgt" in kwargs))
+                and (("temp_lev" in kwargs) or ("temp" in kwargs))
+                and (("press_lev" in kwargs) or ("press" in kwargs))
+                and (("relhum_lev" in kwargs) or ("relhum" in kwargs))):
+            raise TypeError("I need hgt(_lev) and temp(_lev) and press(_lev) and relhum(_lev)!")
+
+        if "hgt" not in kwargs:
+            kwargs["hgt"] = (kwargs["hgt_lev"][..., 1:] + kwargs["hgt_lev"][..., :-1]) / 2.
+
+        nd = kwargs["hgt"].ndim
+        if nd == 1:
+            self.p["ngridx"], self.p["ngridy"] = 1, 1
+        elif nd == 2:
+            self.p["ngridx"], self.p["ngridy"] = kwargs["hgt"].shape[0], 1
+        elif nd == 3:
+            self.p["ngridx"], self.p["ngridy"] = kwargs["hgt"].shape[0], kwargs["hgt"].shape[1]
+        else:
+            raise IOError("Too many dimensions!")
+        self.p["max_nlyrs"] = kwargs["hgt"].shape[-1]
+        self.p["nlyrs"] = np.array(np.sum(kwargs["hgt"] != missingNumber, axis=-1))
+        try:
+            self.p["noutlevels"] = np.shape(kwargs["obs_height"])[-1]
+        except (KeyError, IndexError):
+            self.p["noutlevels"] = 2
+
+        nx, ny, L = self.p["ngridx"], self.p["ngridy"], self.p["max_nlyrs"]
+        nh = self.df.nhydro
+        self._shape2D = (nx, ny)
+        self._shape3D = (nx, ny, L)
+        self._shape3Dout = (nx, ny, self.p["noutlevels"])
+        self._shape3Dplus = (nx, ny, L + 1)
+        self._shape4D = (nx, ny, L, nh)
+        self.p["nlyrs"] = self.p["nlyrs"].reshape(self._shape2D)
+
+        for key in ("hgt_lev", "temp_lev", "press_lev", "relhum_lev"):
+            if key in kwargs:
+                self.p[key] = kwargs[key].reshape(self._shape3Dplus)
+        for key in ("hgt", "temp", "press", "relhum"):
+            if key in kwargs:
+                self.p[key] = kwargs[key].reshape(self._shape3D)
+        if "hgt_lev" not in kwargs:
+            h = self.p["hgt"]
+            hl = np.empty(self._shape3Dplus)
+            hl[..., 0] = h[..., 0] - (h[..., 1] - h[..., 0]) * 0.5
+            hl[..., 1:-1] = 0.5 * (h[..., :-1] + h[..., 1:])
+            hl[..., -1] = h[..., -1] + (h[..., -1] - h[..., -2]) * 0.5
+            self.p["hgt_lev"] = hl
+
+        if "timestamp" not in kwargs:
+            self.p["unixtime"] = np.ones(self._shape2D) * int(time.time())
+        else:
+            self.p["unixtime"] = np.ones(self._shape2D) * np.asarray(kwargs["timestamp"])
+
+        for env, preset in (("lat", 50.938056), ("lon", 6.956944), ("wind10u", 0),
+                            ("wind10v", 0), ("groundtemp", np.nan), ("sfc_salinity", 33.),
+                            ("sfc_slf", 1.), ("sfc_sif", 0.)):
+            if env not in kwargs:
+                self.p[env] = np.ones(self._shape2D) * preset
+            else:
+                self.p[env] = (np.ones(self._shape2D) * kwargs[env]) if kwargs[env].ndim == 0 \
+                    else kwargs[env].reshape(self._shape2D)
+        for env, preset in (("sfc_type", -9999), ("sfc_model", -9999)):
+            if env not in kwargs:
+                self.p[env] = np.ones(self._shape2D, dtype=int) * preset
+            else:
+                self.p[env] = (np.ones(self._shape2D) * kwargs[env]) if kwargs[env].ndim == 0 \
+                    else kwargs[env].reshape(self._shape2D)
+        if "sfc_refl" not in kwargs:
+            self.p["sfc_refl"] = np.full(self._shape2D, "S", dtype="U1")
+        else:
+            r = np.asarray(kwargs["sfc_refl"]).astype("U1")
+            self.p["sfc_refl"] = np.full(self._shape2D, str(r), dtype="U1") if r.ndim == 0 \
+                else r.reshape(self._shape2D)
+        if "sfc_emissivity" in kwargs:
+            self.p["sfc_emissivity"] = kwargs["sfc_emissivity"].reshape(self._shape2D)
+        if "obs_height" not in kwargs:
+            self.p["obs_height"] = np.ones(self._shape3Dout) * np.array([833000., 0.])
+        else:
+            oh = kwargs["obs_height"]
+            self.p["obs_height"] = (np.ones(self._shape3Dout) * oh) if oh.ndim <= 1 \
+                else oh.reshape(self._shape3Dout)
+        for qv in ("hydro_q", "hydro_reff", "hydro_n"):
+            if qv not in kwargs and qv + "+no000" not in kwargs:
+                self.p[qv] = np.zeros(self._shape4D)
+            elif qv + "+no000" in kwargs:
+                self.p[qv] = np.zeros(self._shape4D)
+                for hh in range(nh):
+                    self.p[qv][..., hh] = kwargs[qv + "+no" + str(hh).zfill(3)].reshape(self._shape3D)
+            else:
+                self.p[qv] = kwargs[qv].reshape(self._shape4D)
+        for qv in ("airturb", "wind_w", "wind_uv", "turb_edr"):
+            self.p[qv] = kwargs[qv].reshape(self._shape3D) if qv in kwargs \
+                else np.zeros(self._shape3D) + np.nan
+
+    # ------------------------------------------------------------------
+    def _checkData(self):
+        """Range checks of core.py:979-999."""
+        maxLimits = {"relhum_lev": 200, "hydro_q": 0.05, "temp_lev": 320, "press_lev": 110000,
+                     "relhum": 200, "temp": 320, "press": 110000}
+        minLimits = {"relhum_lev": 0, "hydro_q": 0, "temp_lev": 160, "press_lev": 1,
+                     "relhum": 0, "temp": 160, "press": 1}
+        for key, v in self.p.items():
+            if not isinstance(v, np.ndarray) or key == "sfc_refl" or v.dtype.kind not in "fiu":
+                continue
+            d = v[~np.isnan(v)]
+            if key in maxLimits and len(d) > 0 and np.max(d) > maxLimits[key]:
+                raise ValueError("unrealistic value for " + key + ": " + str(np.max(d))
+                                 + ", maximum is " + str(maxLimits[key]))
+            if key in minLimits and len(d) > 0:
+                dd = d[d != missingNumber]
+                if len(dd) > 0 and np.min(dd) < minLimits[key]:
+                    raise ValueError("unrealistic value for " + key + ": " + str(np.min(d))
+                                     + ", minimum is " + str(minLimits[key]))
+        if np.any(self.p["sfc_refl"][self.p["sfc_type"] == 0] == "L"):
+            raise ValueError("It does not make sense to treat Ocean as Lambertian reflector "
+                             "(incompatible sfc_refl and sfc_type).")
+        if np.any(self.p["sfc_refl"][self.p["sfc_type"] == 1] == "F"):
+            raise ValueError("It does not make sense to simulate land with Fresnel reflection "
+                             "(incompatible sfc_refl and sfc_type).")
+
+    # ------------------------------------------------------------------
+    def runPamtra(self, freqs, checkData=True, failOnError=True):
+        """Runs the forward operator and fills ``self.r`` (core.py:1518-1568).
+        freqs [GHz] must be sorted ascending."""
+        tttt = time.time()
+        if isinstance(freqs, (int, float, np.integer, np.floating)):
+            freqs = [freqs]
+        assert np.array_equal(np.sort(freqs), freqs)
+        self.set["freqs"] = freqs
+        self.set["nfreqs"] = len(freqs)
+        self.set["radar_pol"] = self.nmlSet["radar_polarisation"].split(",")
+        self.set["radar_npol"] = len(self.set["radar_pol"])
+        self.set["att_pol"] = ["N"]
+        self.set["att_npol"] = 1
+        assert self.set["nfreqs"] > 0
+        assert self.p["noutlevels"] > 0
+        assert np.prod(self._shape2D) > 0
+        if checkData:
+            self._checkData()
+        if self.nmlSet["add_obs_height_to_layer"]:
+            raise NotImplementedError("add_obs_height_to_layer is not supported")
+        fortResults, self.fortError, fortObject = PamtraFortranWrapper(
+            self.set, self.nmlSet, self.df.data, self.df.data4D, self.df.dataFullSpec, self.p)
+        if failOnError and self.fortError != 0:
+            raise RuntimeError("Fortran code failed")
+        self.r = fortResults
+        self.fortObject = fortObject
+        self.r["nmlSettings"] = self.nmlSet
+        if self.set["pyVerbose"] > 0:
+            print("pyPamtra runtime:", time.time() - tttt)
