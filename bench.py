@@ -1,0 +1,188 @@
+"""Benchmark of the passive forward operator on one MI355X.
+
+Workload (default): the batched FP64 configuration -- 16384 columns with
+20-120 layers (tops 10-20 km), 7 channels (22.235-183.31 GHz), cloud
+(modified gamma) + rain (Marshall-Palmer) with 80 Mie bins each, NSTOKES=2,
+16 Lobatto angles, ocean (TESSEM2) or Lambertian land.  Synthetic seeded
+profiles; inputs resident in HBM before the timed region.  A step is one
+pam_run_device call over the whole batch (all columns x channels).
+
+Prints one JSON line (metric, value in columns*channels/s, roofline of the
+dominant kernel from its HIP-event stage time, CPU baseline of the oracle on
+a bounded sample of the same columns with its thread count).
+"""
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "columns·channels/s at ≤0.05 K TB error vs FP64; p50/p99 request latency"
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector)
+PEAK_HBM_GBS = 8000.0
+NN = 32
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def slice_columns(pam, n):
+    """The first n columns (along x) of a pyPamtra profile dict."""
+    p = {}
+    nx = pam.p["ngridx"]
+    for k, v in pam.p.items():
+        if isinstance(v, np.ndarray) and v.ndim >= 1 and v.shape[0] == nx:
+            p[k] = v[:n]
+        else:
+            p[k] = v
+    p["ngridx"] = n
+    return p
+
+
+def to_device(pam, torch):
+    from pamtra_amd import _capi
+    prof = {}
+    for k, v in pam.p.items():
+        if k in _capi.PROFILE_DOUBLE_KEYS or k == "obs_height":
+            prof[k] = torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64), device="cuda")
+        elif k in ("nlyrs", "sfc_type", "sfc_model"):
+            prof[k] = torch.as_tensor(np.ascontiguousarray(v).astype(np.int32), device="cuda")
+        elif k == "sfc_refl":
+            b = np.frombuffer("".join(np.asarray(v).reshape(-1)).encode(), dtype=np.uint8)
+            prof[k] = torch.as_tensor(b.copy(), device="cuda")
+        else:
+            prof[k] = v
+    return prof
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--ncol", type=int, default=16384)
+    ap.add_argument("--cpu-cols", type=int, default=256)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the columns are "
+              "independent, a multi-GPU run would shard them with no collective.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pamtra_amd import _capi, _lib, workloads
+
+    t0 = time.time()
+    pam, freqs = workloads.config2(ncol=args.ncol, seed=1002)
+    log(f"workload built in {time.time() - t0:.1f} s: {args.ncol} columns x {len(freqs)} ch")
+    lib = _lib.library()
+    ctx = _lib.get_context(0)
+    prof = to_device(pam, torch)
+    nml = dict(pam.nmlSet)
+    m = _capi.marshal(nml, pam.set, pam.df.data, pam.df.data4D, prof)
+    nx, ny, nout, nf = m.nx, m.ny, m.noutlevels, len(freqs)
+    tb = torch.zeros((nx, ny, nout, 32, nf, 2), dtype=torch.float64, device="cuda")
+    status = torch.zeros((nx, ny, nf), dtype=torch.int32, device="cuda")
+    r = _capi.PamResults()
+    r.tb = C.cast(C.c_void_p(tb.data_ptr()), C.POINTER(C.c_double))
+    r.status = C.cast(C.c_void_p(status.data_ptr()), C.POINTER(C.c_int32))
+    msg = C.create_string_buffer(512)
+    r.msg = C.cast(msg, C.c_char_p)
+    r.msglen = 512
+    f, fp = _capi.freqs_array(freqs)
+
+    def step():
+        st = lib.pam_run_device(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),
+                                C.byref(m.profile), fp, nf, C.byref(r), None)
+        if st == _capi.PAM_FATAL:
+            raise RuntimeError("pam_run_device: " + msg.value.decode())
+        return st
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i} done")
+    torch.cuda.synchronize()
+    stage_ms = np.zeros(5)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        ms = (C.c_double * 5)()
+        lib.pam_last_stage_times(ctx, ms, 5)
+        stage_ms += np.array(ms[:])
+        log(f"step {i} done")
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    stage_ms /= args.steps
+    work = (C.c_int64 * 9)()
+    lib.pam_last_work(ctx, work, 9)
+    work = list(work)
+    names = lib.pam_stage_names().decode().split(",")
+    value = nx * ny * nf / dt
+
+    # algorithmic flops per step of the RT stages
+    n3, n2 = NN ** 3, NN ** 2
+    f_doubling = work[3] * (14 * n3 + 24 * n2) + work[8] * 512 * 11
+    f_adding = work[4] * (20 * n3 + 16 * n2) + work[5] * (4 * n2 + 3 * n2) \
+        + nx * ny * nf * (4 * 2 * n3 + 2 * 2 * n3)
+    flops = {"doubling": f_doubling, "adding": f_adding}
+    dom = max(range(5), key=lambda i: stage_ms[i])
+    dom_name = names[dom]
+    roof = None
+    if dom_name in flops:
+        ach = flops[dom_name] / (stage_ms[dom] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": dom_name, "achieved": round(ach, 3),
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP64_TFLOPS, 4),
+                "traffic": None}
+
+    tb_h = tb.cpu().numpy()
+    st_h = status.cpu().numpy()
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "columns·channels/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batched FP64 drop-in: 16384 columns, 20-120 layers, tops "
+                               "10-20 km, 7 channels 22.235-183.31 GHz, cloud+rain Mie 80 bins, "
+                               "NSTOKES=2, 16 angles, ocean TESSEM2 / Lambertian land",
+                   "columns": nx * ny, "channels": nf, "precision": "fp64"},
+        "stage_ms": {n: round(v, 3) for n, v in zip(names, stage_ms)},
+        "work": {"scattering_items": work[2], "doubling_steps": work[3],
+                 "adding_scattering": work[4], "adding_clear": work[5]},
+        "status_nonzero": int((st_h != 0).sum()),
+        "roofline": roof,
+    }
+
+    if not args.no_cpu:
+        import oracle
+        n = min(args.cpu_cols, nx)
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sub = slice_columns(pam, n)
+        t0 = time.perf_counter()
+        ref, st, _ = oracle.run(nml, pam.set, pam.df.data, pam.df.data4D, sub, freqs,
+                                nthreads=threads)
+        tc = time.perf_counter() - t0
+        ok = np.isfinite(ref["tb"])
+        dmax = float(np.max(np.abs(tb_h[:n][ok] - ref["tb"][ok])))
+        out["cpu_baseline"] = {"value": round(n * nf / tc, 3), "unit": "columns·channels/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"first {n} of the {nx} columns x {nf} channels, "
+                                         f"{tc:.1f} s"}
+        out["max_abs_dtb_vs_oracle_K"] = dmax
+        log(f"cpu baseline {n * nf / tc:.2f} col*ch/s on {threads} threads; "
+            f"max |dTB| GPU vs oracle on the sample {dmax:.2e} K")
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

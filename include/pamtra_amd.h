@@ -203,7 +203,8 @@ const char *pam_stage_names(void);
 /* Work counters of the last call: [0] columns, [1] channels,
  * [2] scattering (layer, channel) items, [3] total doubling steps,
  * [4] scattering adding steps, [5] clear adding steps, [6] Mie bins,
- * [7] sum over Mie bins of the series length. */
+ * [7] sum over Mie bins of the series length, [8] sum over scattering items
+ * of (NUMPTS/2+1)*(nlegen+1), the phase-matrix work. */
 int pam_last_work(const pam_context *ctx, int64_t *counts, int32_t n);
 
 #ifdef __cplusplus

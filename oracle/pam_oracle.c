@@ -1793,6 +1793,9 @@ static int run_column_freq(const pam_settings *s, const pam_hydro_desc *hyd,
         }
         if (fabs(sum) > maxsum) maxsum = fabs(sum);
       }
+    if (getenv("PAM_ORACLE_DEBUG"))
+      fprintf(stderr, "z=%d kext=%g salb=%g nleg=%d maxsum=%g rel=%g gas=%g\n", z, lo.kexttot,
+              lo.salbedo, lo.nlegen, maxsum, maxsum / lo.kexttot, gas);
     if (maxsum > 5.0e-4) {
       snprintf(msg, 120, "Scattering function not normalized: %g", maxsum);
       st = PAM_FATAL;

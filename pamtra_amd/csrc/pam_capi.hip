@@ -1,0 +1,698 @@
+// pam_capi.hip -- the C-ABI of include/pamtra_amd.h and the host-side
+// scheduling of one call: the replacement of run_pamtra's frequency/column
+// loop nest (pyPamtraLib.f90:119-133).  Columns are processed in chunks
+// sized so that the per-layer optics of a chunk fit the device workspace
+// budget; within a chunk every stage is one kernel launch over all
+// (layer, channel) or (column, channel) units.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pam_internal.h"
+
+using namespace pam;
+
+#define HIP_OK(x)                                                              \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) {                                                    \
+      set_msg(res, std::string("HIP error: ") + hipGetErrorString(e_) + " at " \
+                       #x);                                                    \
+      return PAM_FATAL;                                                        \
+    }                                                                          \
+  } while (0)
+
+constexpr int PAM_NWORK = 9;
+static const char *k_stage_names = "columns,dsd,mie,doubling,adding";
+constexpr int NSTAGE = 5;
+
+// A device allocation that only grows.
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = n + n / 8 + 4096;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct pam_context {
+  int device = 0;
+  DevBuf colbuf, itembuf, iobuf, smallbuf;
+  double *d_mu = nullptr, *d_qw = nullptr;
+  double mu[NMU], qw[NMU];
+  double stage_ms[NSTAGE] = {0, 0, 0, 0, 0};
+  int64_t work[PAM_NWORK] = {};
+  size_t budget_bytes = 0;
+};
+
+static void set_msg(pam_results *res, const std::string &m) {
+  if (res && res->msg && res->msglen > 0) {
+    snprintf(res->msg, (size_t)res->msglen, "%s", m.c_str());
+  }
+}
+
+// Lobatto half-space quadrature (rt_utilities.f90:165-203), host side
+static void lobatto(int num, double *mu, double *w) {
+  int n = 2 * num, n1 = n - 1;
+  double ci = (n % 2 == 1) ? 1.0 : 0.5;
+  for (int j = 1; j <= num - 1; j++) {
+    double x = sin((double)3.141592654f * (j - ci) / (double)((float)n - 0.5f));
+    double pl = 0, pl1 = 0, pl2 = 0, dpl, d2pl, xp;
+    int it = 0;
+    do {
+      pl1 = 1.0;
+      pl = x;
+      for (int l = 2; l <= n1; l++) {
+        pl2 = pl1;
+        pl1 = pl;
+        pl = ((double)(2 * l - 1) * x * pl1 - (double)(l - 1) * pl2) / (double)l;
+      }
+      dpl = (double)n1 * (x * pl - pl1) / (x * x - 1.0);
+      d2pl = (2.0 * x * dpl - (double)(n1 * (n1 + 1)) * pl) / (1.0 - x * x);
+      xp = x;
+      x = xp - dpl / d2pl;
+      it++;
+    } while (std::fabs(x - xp) > 3.0e-14 && it < 10);
+    mu[j - 1] = x;
+    w[j - 1] = 2.0 / ((double)(n * n1) * pl * pl);
+  }
+  mu[num - 1] = 1.0;
+  w[num - 1] = 2.0 / (double)(n * n1);
+}
+
+static const char *err_text(int32_t code) {
+  switch (code & ~0xff) {
+    case E_NLYRS: return "nlyrs out of range or unsupported output level";
+    case E_NAN_LEVEL: return "found nan in level variables";
+    case E_NAN_LAYER: return "found nan in layer variables";
+    case E_HGT_ORDER: return "atmo_hgt_lev must be defined bottom-up";
+    case E_GAS: return "error in rosen98_gasabs (frequency, temperature or pressure out of range)";
+    case E_SURFACE: return "surface model not supported (FASTEM, land atlases)";
+    case E_MASS_SIZE: return "error in make_mass_size";
+    case E_DIST_PARAMS: return "error in make_dist_params";
+    case E_ADAPTIVE: return "make_dist: adaptive grid failed";
+    case E_MOMENT: return "calc_moment: invalid distribution";
+    case E_SOFT: return "make_soft_spheroid: invalid density (frequency dependent density unsupported)";
+    case E_MIE_INPUT: return "calc_mie_spheres: invalid bin input";
+    case E_MIE_TERMS: return "calc_mie_spheres: Mie series too long for the device kernel";
+    case E_MIE_RESULT: return "calc_mie_spheres: extinction/scatter/back_scatt must be positive";
+    case E_KEXT: return "rt_kexttot smaller than zero or nan";
+    case E_NLEG: return "nlegen_coef greater maxnleg";
+    case E_NORM: return "Scattering function not normalized";
+    case E_SINGULAR: return "Matrix inversion failed";
+    case E_REFL: return "Wrong or no ground_type selected";
+    case E_NAN_COEF: return "nan in Legendre coefficients";
+    default: return "error";
+  }
+}
+
+// ------------------------------------------------------------------
+// small kernels of the scheduler
+// ------------------------------------------------------------------
+
+// merge column and (column, channel) status into the result status array
+__global__ void k_status(Dims d, Work w, int32_t *out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.ncol * d.nf) return;
+  int col = i / d.nf;
+  int32_t a = w.col_status[col], b = w.cf_status[i];
+  int32_t v = ((a & 0xff) >= (b & 0xff)) ? a : b;
+  w.cf_status[i] = v;
+  if (out) out[i] = v & 0xff;
+}
+
+// per-layer diagnostics of one chunk (bottom-up layers)
+__global__ void k_diag(Dims d, Work w, double *kextatmo, double *kexttot, double *salbedo,
+                       int32_t *nlegen, int32_t *ndoubles) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t n = (size_t)d.ncol_chunk * d.ML * d.nf;
+  if (i >= n) return;
+  int f = i % d.nf;
+  int z = (i / d.nf) % d.ML;
+  int col = d.col0 + (int)(i / ((size_t)d.nf * d.ML));
+  size_t o = ((size_t)col * d.ML + z) * d.nf + f;
+  bool in = z < w.nl[col];
+  int item = in ? w.layer_item[(size_t)col * d.ML + z] : -1;
+  if (kextatmo) kextatmo[o] = in ? w.kgas[((size_t)col * d.nf + f) * d.ML + z] : __builtin_nan("");
+  if (item >= 0) {
+    size_t itf = (size_t)(item - d.item0) * d.nf + f;
+    if (kexttot) kexttot[o] = w.kext[itf];
+    if (salbedo) salbedo[o] = w.salb[itf];
+    if (nlegen) nlegen[o] = w.nleg[itf] > 0 ? w.nleg[itf] : -1;
+    if (ndoubles) ndoubles[o] = w.ndbl[itf];
+  } else {
+    if (kexttot) kexttot[o] = in ? 0.0 : __builtin_nan("");
+    if (salbedo) salbedo[o] = in ? 0.0 : __builtin_nan("");
+    if (nlegen) nlegen[o] = in ? -1 : -9999;
+    if (ndoubles) ndoubles[o] = in ? -1 : -9999;
+  }
+}
+
+// ------------------------------------------------------------------
+// C-ABI
+// ------------------------------------------------------------------
+
+extern "C" const char *pam_version(void) { return "pamtra_amd 0.1.0 (gfx950)"; }
+
+extern "C" int pam_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" int pam_context_create(pam_context **out, int device) {
+  if (!out) return PAM_FATAL;
+  *out = nullptr;
+  int n = pam_device_count();
+  if (device < 0 || device >= n) return PAM_FATAL;
+  if (hipSetDevice(device) != hipSuccess) return PAM_FATAL;
+  pam_context *c = new pam_context();
+  c->device = device;
+  lobatto(NMU, c->mu, c->qw);
+  if (hipMalloc(&c->d_mu, 2 * NMU * sizeof(double)) != hipSuccess) {
+    delete c;
+    return PAM_FATAL;
+  }
+  c->d_qw = c->d_mu + NMU;
+  double tmp[2 * NMU];
+  memcpy(tmp, c->mu, sizeof(c->mu));
+  memcpy(tmp + NMU, c->qw, sizeof(c->qw));
+  if (hipMemcpy(c->d_mu, tmp, sizeof(tmp), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(c->d_mu);
+    delete c;
+    return PAM_FATAL;
+  }
+  double gb = 32.0;
+  if (const char *e = getenv("PAM_WORKSPACE_GB")) gb = atof(e);
+  c->budget_bytes = (size_t)(gb * (1ull << 30));
+  *out = c;
+  return PAM_SUCCESS;
+}
+
+extern "C" int pam_context_destroy(pam_context *c) {
+  if (!c) return PAM_SUCCESS;
+  (void)hipSetDevice(c->device);
+  c->colbuf.release();
+  c->itembuf.release();
+  c->iobuf.release();
+  c->smallbuf.release();
+  if (c->d_mu) (void)hipFree(c->d_mu);
+  delete c;
+  return PAM_SUCCESS;
+}
+
+extern "C" const char *pam_stage_names(void) { return k_stage_names; }
+
+extern "C" int pam_last_stage_times(const pam_context *c, double *ms, int32_t n) {
+  if (!c || !ms) return PAM_FATAL;
+  for (int i = 0; i < n && i < NSTAGE; i++) ms[i] = c->stage_ms[i];
+  return PAM_SUCCESS;
+}
+
+extern "C" int pam_last_work(const pam_context *c, int64_t *counts, int32_t n) {
+  if (!c || !counts) return PAM_FATAL;
+  for (int i = 0; i < n && i < PAM_NWORK; i++) counts[i] = c->work[i];
+  return PAM_SUCCESS;
+}
+
+// carve aligned sub-arrays out of one allocation
+struct Carver {
+  char *base;
+  size_t off = 0;
+  explicit Carver(void *b) : base((char *)b) {}
+  template <class T>
+  T *take(size_t n) {
+    off = (off + 255) & ~(size_t)255;
+    T *p = (T *)(base + off);
+    off += n * sizeof(T);
+    return p;
+  }
+};
+
+static size_t col_bytes(const Dims &d) {
+  size_t n = 0;
+  auto add = [&](size_t b) { n += ((b + 255) & ~(size_t)255); };
+  size_t ncol = d.ncol, ML = d.ML, nf = d.nf;
+  add(2 * ncol * (ML + 1) * 8);
+  add(5 * ncol * ML * 8);
+  add(ncol * 8);
+  add(ncol * 4);
+  add(ncol * d.nout * 4);
+  add(ncol * 4);
+  add(ncol * ML * 4);
+  add(ncol * 4);
+  add(ncol * 4);
+  add(ncol * ML * 4);
+  add(ncol * nf * ML * 8);
+  add(2 * ncol * nf * 2 * NMU * 8);
+  add(ncol * nf * 4);
+  add(8 * 8);
+  for (int i = 0; i < 16; i++) add(0);
+  return n + 256 * 32;
+}
+
+static size_t item_bytes_per(const Dims &d) {
+  size_t per = 2 * 4 + (size_t)d.nh * d.nbmax * 4 * 8 + (size_t)d.nh * 4;
+  per += (size_t)d.nf * (2 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8);
+  return per;
+}
+
+static void carve_items(Work &w, void *base, const Dims &d, size_t nitems) {
+  Carver cv(base);
+  size_t nif = nitems * d.nf;
+  w.item_col = cv.take<int32_t>(nitems);
+  w.item_layer = cv.take<int32_t>(nitems);
+  size_t nb = nitems * d.nh * d.nbmax;
+  w.dsd_d = cv.take<double>(nb);
+  w.dsd_dd = cv.take<double>(nb);
+  w.dsd_nd = cv.take<double>(nb);
+  w.dsd_rho = cv.take<double>(nb);
+  w.dsd_flag = cv.take<int32_t>(nitems * d.nh);
+  w.kext = cv.take<double>(nif);
+  w.salb = cv.take<double>(nif);
+  w.nleg = cv.take<int32_t>(nif);
+  w.coef = cv.take<double>(nif * NCOEF * MAXNLEG);
+  w.ndbl = cv.take<int32_t>(nif);
+  w.itf_status = cv.take<int32_t>(nif);
+  w.R = cv.take<double>(nif * NN2);
+  w.T = cv.take<double>(nif * NN2);
+  w.Sp = cv.take<double>(nif * NN);
+  w.Sm = cv.take<double>(nif * NN);
+}
+
+static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_desc *hydros,
+                      const pam_hydro_4d *h4in, const pam_profile *pin, const double *freqs,
+                      int32_t nfreq, pam_results *res, hipStream_t stream) {
+  if (!c || !st || !pin || !freqs || !res || !res->tb) {
+    set_msg(res, "pam_run: NULL argument");
+    return PAM_FATAL;
+  }
+  if (!st->passive || st->active) {
+    set_msg(res, "only passive-only runs are supported (radar simulator not in this library)");
+    return PAM_FATAL;
+  }
+  if (st->gas_mod != 0 || st->liq_mod != 0) {
+    set_msg(res, "only gas_mod R98 and liq_mod TKC are implemented");
+    return PAM_FATAL;
+  }
+  if (pin->nhydro > MAX_HYDRO || nfreq <= 0 || pin->ngridx <= 0 || pin->ngridy <= 0 ||
+      pin->max_nlyrs <= 0 || pin->noutlevels <= 0) {
+    set_msg(res, "pam_run: invalid dimensions");
+    return PAM_FATAL;
+  }
+  for (int i = 1; i < nfreq; i++)
+    if (!(freqs[i] >= freqs[i - 1])) {
+      set_msg(res, "frequencies must be sorted ascending");
+      return PAM_FATAL;
+    }
+  HIP_OK(hipSetDevice(c->device));
+  for (int i = 0; i < NSTAGE; i++) c->stage_ms[i] = 0.0;
+  for (int i = 0; i < PAM_NWORK; i++) c->work[i] = 0;
+
+  Dims d;
+  d.nx = pin->ngridx;
+  d.ny = pin->ngridy;
+  d.ncol = d.nx * d.ny;
+  d.ML = pin->max_nlyrs;
+  d.nout = pin->noutlevels;
+  d.nh = pin->nhydro;
+  d.nf = nfreq;
+  d.nbmax = 1;
+  for (int h = 0; h < d.nh; h++) {
+    int nb = hydros[h].dist_name == PAM_DIST_MONO ? 2 : hydros[h].nbin;
+    d.nbmax = std::max(d.nbmax, nb);
+  }
+  if (h4in && h4in->nbin) {
+    std::vector<int32_t> hb((size_t)d.ncol * d.ML * d.nh);
+    HIP_OK(hipMemcpy(hb.data(), h4in->nbin, hb.size() * 4, hipMemcpyDeviceToHost));
+    for (int32_t v : hb) d.nbmax = std::max(d.nbmax, (int)v);
+  }
+  d.col0 = 0;
+  d.ncol_chunk = d.ncol;
+  d.nitems = 0;
+  d.item0 = 0;
+
+  Settings s;
+  s.lgas = st->lgas_extinction;
+  s.lhyd_abs = st->lhyd_absorption;
+  s.lhyd_scat = st->lhyd_scattering;
+  s.lhyd_emis = st->lhyd_emission;
+  s.incl_hydro_rho = st->hydro_includehydroinrhoair;
+  s.limit_density = st->hydro_limit_density_area;
+  s.adaptive = st->hydro_adaptive_grid;
+  s.conserve = st->conserve_mass_rescale_dsd;
+  s.outpol = st->outpol;
+  s.precision = st->precision;
+  s.threshold = st->hydro_threshold;
+  s.softsphere_min = st->hydro_softsphere_min_density;
+  s.emissivity = st->emissivity;
+
+  Profile p;
+  p.nlyrs = pin->nlyrs;
+  p.hgt_lev = pin->hgt_lev;
+  p.temp_lev = pin->temp_lev;
+  p.press_lev = pin->press_lev;
+  p.relhum_lev = pin->relhum_lev;
+  p.hgt = pin->hgt;
+  p.temp = pin->temp;
+  p.press = pin->press;
+  p.relhum = pin->relhum;
+  p.hydro_q = pin->hydro_q;
+  p.hydro_n = pin->hydro_n;
+  p.hydro_reff = pin->hydro_reff;
+  p.groundtemp = pin->groundtemp;
+  p.wind10u = pin->wind10u;
+  p.wind10v = pin->wind10v;
+  p.sfc_salinity = pin->sfc_salinity;
+  p.sfc_emissivity = pin->sfc_emissivity;
+  p.sfc_type = pin->sfc_type;
+  p.sfc_model = pin->sfc_model;
+  p.sfc_refl = pin->sfc_refl;
+  p.obs_height = pin->obs_height;
+  Hydro4D h4;
+  memset(&h4, 0, sizeof(h4));
+  if (h4in) {
+    h4.as_ratio = h4in->as_ratio;
+    h4.rho_ms = h4in->rho_ms;
+    h4.a_ms = h4in->a_ms;
+    h4.b_ms = h4in->b_ms;
+    h4.alpha_as = h4in->alpha_as;
+    h4.beta_as = h4in->beta_as;
+    h4.p_1 = h4in->p_1;
+    h4.p_2 = h4in->p_2;
+    h4.p_3 = h4in->p_3;
+    h4.p_4 = h4in->p_4;
+    h4.d_1 = h4in->d_1;
+    h4.d_2 = h4in->d_2;
+    h4.nbin = h4in->nbin;
+  }
+
+  // small buffers: hydro table, frequencies, counters
+  size_t small = 4096 + sizeof(pam_hydro_desc) * std::max(1, d.nh) + 8 * (size_t)nfreq + 64 * 8;
+  HIP_OK(c->smallbuf.ensure(small));
+  Carver sc(c->smallbuf.p);
+  pam_hydro_desc *d_hyd = sc.take<pam_hydro_desc>(std::max(1, d.nh));
+  double *d_freqs = sc.take<double>(nfreq);
+  unsigned long long *d_cnt = sc.take<unsigned long long>(PAM_NWORK);
+  if (d.nh > 0)
+    HIP_OK(hipMemcpyAsync(d_hyd, hydros, sizeof(pam_hydro_desc) * d.nh, hipMemcpyHostToDevice,
+                          stream));
+  HIP_OK(hipMemcpyAsync(d_freqs, freqs, 8 * (size_t)nfreq, hipMemcpyHostToDevice, stream));
+  HIP_OK(hipMemsetAsync(d_cnt, 0, 8 * PAM_NWORK, stream));
+
+  // per-column workspace
+  HIP_OK(c->colbuf.ensure(col_bytes(d)));
+  Work w;
+  memset(&w, 0, sizeof(w));
+  {
+    Carver cv(c->colbuf.p);
+    size_t ncol = d.ncol, ML = d.ML, nf = d.nf;
+    w.hgt_lev = cv.take<double>(ncol * (ML + 1));
+    w.temp_lev = cv.take<double>(ncol * (ML + 1));
+    w.temp = cv.take<double>(ncol * ML);
+    w.press = cv.take<double>(ncol * ML);
+    w.rho_v = cv.take<double>(ncol * ML);
+    w.q_hum = cv.take<double>(ncol * ML);
+    w.dz = cv.take<double>(ncol * ML);
+    w.groundtemp = cv.take<double>(ncol);
+    w.nl = cv.take<int32_t>(ncol);
+    w.outlev = cv.take<int32_t>(ncol * d.nout);
+    w.col_status = cv.take<int32_t>(ncol);
+    w.hmask = cv.take<uint32_t>(ncol * ML);
+    w.col_nitems = cv.take<int32_t>(ncol);
+    w.col_item0 = cv.take<int32_t>(ncol);
+    w.layer_item = cv.take<int32_t>(ncol * ML);
+    w.kgas = cv.take<double>(ncol * nf * ML);
+    w.sfc_emis = cv.take<double>(ncol * nf * 2 * NMU);
+    w.sfc_refl = cv.take<double>(ncol * nf * 2 * NMU);
+    w.cf_status = cv.take<int32_t>(ncol * nf);
+    HIP_OK(hipMemsetAsync(w.col_status, 0, ncol * 4, stream));
+    HIP_OK(hipMemsetAsync(w.cf_status, 0, ncol * nf * 4, stream));
+  }
+  w.mu = c->d_mu;
+  w.qw = c->d_qw;
+  w.counters = d_cnt;
+
+  std::vector<hipEvent_t> evs;
+  std::vector<int> ev_stage;
+  auto mark = [&](int stage) -> hipError_t {
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    evs.push_back(e);
+    ev_stage.push_back(stage);
+    return hipEventRecord(e, stream);
+  };
+  auto free_events = [&]() {
+    for (auto e : evs) (void)hipEventDestroy(e);
+    evs.clear();
+  };
+
+  HIP_OK(mark(-1));
+  launch_columns(d, s, p, d_hyd, h4, d_freqs, w, stream);
+  launch_surface(d, s, p, d_freqs, w, res->emissivity, stream);
+  HIP_OK(hipGetLastError());
+  HIP_OK(mark(0));
+
+  // scattering layers per column -> item offsets (host scan) and chunks
+  std::vector<int32_t> cnt(d.ncol), off(d.ncol);
+  HIP_OK(hipMemcpyAsync(cnt.data(), w.col_nitems, 4 * (size_t)d.ncol, hipMemcpyDeviceToHost,
+                        stream));
+  HIP_OK(hipStreamSynchronize(stream));
+  int64_t total = 0;
+  for (int i = 0; i < d.ncol; i++) {
+    off[i] = (int32_t)total;
+    total += cnt[i];
+  }
+  if (total > 0x7fffffff) {
+    free_events();
+    set_msg(res, "too many scattering layers in one call");
+    return PAM_FATAL;
+  }
+  HIP_OK(hipMemcpyAsync(w.col_item0, off.data(), 4 * (size_t)d.ncol, hipMemcpyHostToDevice,
+                        stream));
+  const size_t per_item = item_bytes_per(d);
+  size_t max_items = std::max<size_t>(1, c->budget_bytes / per_item);
+  // chunks of whole columns
+  std::vector<std::pair<int, int>> chunks;
+  {
+    int c0 = 0;
+    int64_t acc = 0;
+    for (int i = 0; i < d.ncol; i++) {
+      if (acc + cnt[i] > (int64_t)max_items && i > c0) {
+        chunks.push_back({c0, i});
+        c0 = i;
+        acc = 0;
+      }
+      acc += cnt[i];
+    }
+    chunks.push_back({c0, d.ncol});
+  }
+  size_t chunk_items_max = 0;
+  for (auto &ch : chunks) {
+    int64_t n = 0;
+    for (int i = ch.first; i < ch.second; i++) n += cnt[i];
+    chunk_items_max = std::max<size_t>(chunk_items_max, (size_t)n);
+  }
+  HIP_OK(c->itembuf.ensure(std::max<size_t>(1, chunk_items_max) * per_item + 64 * 256));
+  c->work[0] = d.ncol;
+  c->work[1] = d.nf;
+  c->work[2] = total * d.nf;
+
+  for (auto &ch : chunks) {
+    Dims dc = d;
+    dc.col0 = ch.first;
+    dc.ncol_chunk = ch.second - ch.first;
+    dc.item0 = off[ch.first];
+    int64_t n = 0;
+    for (int i = ch.first; i < ch.second; i++) n += cnt[i];
+    dc.nitems = (int)n;
+    carve_items(w, c->itembuf.p, dc, std::max<size_t>(1, (size_t)n));
+    if (n > 0)
+      HIP_OK(hipMemsetAsync(w.itf_status, 0, 4 * (size_t)n * d.nf, stream));
+    launch_items(dc, w, stream);
+    launch_dsd(dc, s, p, d_hyd, h4, w, stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(mark(1));
+    launch_mie(dc, s, d_hyd, h4, p, d_freqs, w, stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(mark(2));
+    launch_doubling(dc, s, d_freqs, w, stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(mark(3));
+    launch_adding(dc, s, d_freqs, pin->sfc_refl, w, res->tb, stream);
+    HIP_OK(hipGetLastError());
+    if (res->kextatmo || res->kexttot || res->salbedo || res->nlegen || res->ndoubles) {
+      size_t nd = (size_t)dc.ncol_chunk * d.ML * d.nf;
+      hipLaunchKernelGGL(k_diag, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, stream, dc, w,
+                         res->kextatmo, res->kexttot, res->salbedo, res->nlegen, res->ndoubles);
+    }
+    HIP_OK(mark(4));
+  }
+  hipLaunchKernelGGL(k_status, dim3((unsigned)((d.ncol * d.nf + 255) / 256)), dim3(256), 0,
+                     stream, d, w, res->status);
+  HIP_OK(hipGetLastError());
+  std::vector<int32_t> cf((size_t)d.ncol * d.nf);
+  unsigned long long cntv[PAM_NWORK];
+  HIP_OK(hipMemcpyAsync(cf.data(), w.cf_status, 4 * cf.size(), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync(cntv, d_cnt, sizeof(cntv), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipStreamSynchronize(stream));
+  // stage times
+  for (size_t i = 1; i < evs.size(); i++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, evs[i - 1], evs[i]) == hipSuccess && ev_stage[i] >= 0)
+      c->stage_ms[ev_stage[i]] += ms;
+  }
+  free_events();
+  for (int i = 3; i < PAM_NWORK; i++) c->work[i] = (int64_t)cntv[i];
+  int worst = PAM_SUCCESS;
+  int32_t first = 0;
+  for (int32_t v : cf) {
+    int sev = v & 0xff;
+    if (sev == PAM_INFO) continue;
+    if (sev > worst) {
+      worst = sev;
+      first = v;
+    }
+  }
+  if (worst != PAM_SUCCESS) set_msg(res, err_text(first));
+  if (res->angles_deg) {
+    double a[PAM_NANGLES];
+    for (int j = 0; j < NMU; j++) {
+      a[j] = 180.0 - (180. * std::acos(c->mu[NMU - 1 - j]) / PI);
+      a[NMU + j] = (180. * std::acos(c->mu[j]) / PI);
+    }
+    HIP_OK(hipMemcpy(res->angles_deg, a, sizeof(a), hipMemcpyHostToDevice));
+  }
+  return worst;
+}
+
+extern "C" int pam_run_device(pam_context *c, const pam_settings *st, const pam_hydro_desc *hyd,
+                              const pam_hydro_4d *h4, const pam_profile *p, const double *freqs,
+                              int32_t nfreq, pam_results *res, void *stream) {
+  return run_device(c, st, hyd, h4, p, freqs, nfreq, res, (hipStream_t)stream);
+}
+
+// Host arrays in, host arrays out: stage through one device allocation.
+extern "C" int pam_run(pam_context *c, const pam_settings *st, const pam_hydro_desc *hyd,
+                       const pam_hydro_4d *h4, const pam_profile *p, const double *freqs,
+                       int32_t nfreq, pam_results *res) {
+  if (!c || !p || !res) return PAM_FATAL;
+  if (hipSetDevice(c->device) != hipSuccess) return PAM_FATAL;
+  const size_t nx = p->ngridx, ny = p->ngridy, ncol = nx * ny, ML = p->max_nlyrs,
+               nh = p->nhydro, nout = p->noutlevels, nf = nfreq;
+  struct Arr {
+    const void *h;
+    size_t bytes;
+    const void **slot;
+  };
+  pam_profile dp = *p;
+  pam_hydro_4d d4;
+  memset(&d4, 0, sizeof(d4));
+  if (h4) d4 = *h4;
+  std::vector<Arr> in;
+  auto reg = [&](const void *const *field, size_t bytes) {
+    if (*field) in.push_back({*field, bytes, (const void **)field});
+  };
+  reg((const void *const *)&dp.nlyrs, ncol * 4);
+  reg((const void *const *)&dp.hgt_lev, ncol * (ML + 1) * 8);
+  reg((const void *const *)&dp.temp_lev, ncol * (ML + 1) * 8);
+  reg((const void *const *)&dp.press_lev, ncol * (ML + 1) * 8);
+  reg((const void *const *)&dp.relhum_lev, ncol * (ML + 1) * 8);
+  reg((const void *const *)&dp.hgt, ncol * ML * 8);
+  reg((const void *const *)&dp.temp, ncol * ML * 8);
+  reg((const void *const *)&dp.press, ncol * ML * 8);
+  reg((const void *const *)&dp.relhum, ncol * ML * 8);
+  reg((const void *const *)&dp.hydro_q, ncol * ML * nh * 8);
+  reg((const void *const *)&dp.hydro_n, ncol * ML * nh * 8);
+  reg((const void *const *)&dp.hydro_reff, ncol * ML * nh * 8);
+  reg((const void *const *)&dp.groundtemp, ncol * 8);
+  reg((const void *const *)&dp.wind10u, ncol * 8);
+  reg((const void *const *)&dp.wind10v, ncol * 8);
+  reg((const void *const *)&dp.sfc_salinity, ncol * 8);
+  reg((const void *const *)&dp.sfc_emissivity, ncol * 8);
+  reg((const void *const *)&dp.sfc_type, ncol * 4);
+  reg((const void *const *)&dp.sfc_model, ncol * 4);
+  reg((const void *const *)&dp.sfc_refl, ncol);
+  reg((const void *const *)&dp.obs_height, ncol * nout * 8);
+  const size_t n4 = ncol * ML * nh;
+  reg((const void *const *)&d4.as_ratio, n4 * 8);
+  reg((const void *const *)&d4.rho_ms, n4 * 8);
+  reg((const void *const *)&d4.a_ms, n4 * 8);
+  reg((const void *const *)&d4.b_ms, n4 * 8);
+  reg((const void *const *)&d4.alpha_as, n4 * 8);
+  reg((const void *const *)&d4.beta_as, n4 * 8);
+  reg((const void *const *)&d4.p_1, n4 * 8);
+  reg((const void *const *)&d4.p_2, n4 * 8);
+  reg((const void *const *)&d4.p_3, n4 * 8);
+  reg((const void *const *)&d4.p_4, n4 * 8);
+  reg((const void *const *)&d4.d_1, n4 * 8);
+  reg((const void *const *)&d4.d_2, n4 * 8);
+  reg((const void *const *)&d4.nbin, n4 * 4);
+  pam_results dr = *res;
+  struct Out {
+    void *h;
+    size_t bytes;
+    void **slot;
+  };
+  std::vector<Out> out;
+  auto oreg = [&](void **field, size_t bytes) {
+    if (*field) out.push_back({*field, bytes, field});
+  };
+  oreg((void **)&dr.tb, ncol * nout * PAM_NANGLES * nf * 2 * 8);
+  oreg((void **)&dr.emissivity, ncol * 2 * nf * NMU * 8);
+  oreg((void **)&dr.angles_deg, PAM_NANGLES * 8);
+  oreg((void **)&dr.status, ncol * nf * 4);
+  oreg((void **)&dr.kextatmo, ncol * ML * nf * 8);
+  oreg((void **)&dr.kexttot, ncol * ML * nf * 8);
+  oreg((void **)&dr.salbedo, ncol * ML * nf * 8);
+  oreg((void **)&dr.nlegen, ncol * ML * nf * 4);
+  oreg((void **)&dr.ndoubles, ncol * ML * nf * 4);
+  size_t total = 0;
+  for (auto &a : in) total += (a.bytes + 255) & ~(size_t)255;
+  for (auto &o : out) total += (o.bytes + 255) & ~(size_t)255;
+  if (c->iobuf.ensure(total + 4096) != hipSuccess) {
+    set_msg(res, "pam_run: device allocation failed");
+    return PAM_FATAL;
+  }
+  Carver cv(c->iobuf.p);
+  for (auto &a : in) {
+    void *dptr = cv.take<char>(a.bytes);
+    if (hipMemcpy(dptr, a.h, a.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      set_msg(res, "pam_run: copy to device failed");
+      return PAM_FATAL;
+    }
+    *a.slot = dptr;
+  }
+  for (auto &o : out) {
+    void *dptr = cv.take<char>(o.bytes);
+    *o.slot = dptr;
+  }
+  if (dr.tb) (void)hipMemset(dr.tb, 0, ncol * nout * PAM_NANGLES * nf * 2 * 8);
+  int rc = run_device(c, st, hyd, h4 ? &d4 : nullptr, &dp, freqs, nfreq, &dr, nullptr);
+  for (auto &o : out) {
+    if (hipMemcpy(o.h, *o.slot, o.bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+      set_msg(res, "pam_run: copy from device failed");
+      return PAM_FATAL;
+    }
+  }
+  return rc;
+}

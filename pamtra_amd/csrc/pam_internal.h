@@ -1,0 +1,207 @@
+// pam_internal.h -- device workspace layout, launch helpers and device math
+// shared by the HIP translation units of libpamtra_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pamtra_amd.h"
+
+namespace pam {
+
+constexpr int NMU = PAM_NUMMU;       // quadrature angles per hemisphere
+constexpr int NST = PAM_NSTOKES;     // Stokes parameters
+constexpr int NN = NMU * NST;        // order of the R / T matrices (32)
+constexpr int NN2 = NN * NN;
+constexpr int MAXNLEG = PAM_MAXNLEG; // Legendre coefficients per series
+constexpr int NCOEF = 3;             // P11, P12, P33 series
+constexpr int MAX_NQUAD = 256;       // Gauss nodes of the Mie phase integral
+constexpr int MAX_NTERMS = 160;      // Mie series length supported on device
+constexpr int MAX_HYDRO = 32;        // bits of the per-layer hydrometeor mask
+constexpr int MAX_NUMPTS = 204;      // 2*int((199+1)/2)+4
+constexpr int NUM_NP = MAX_NUMPTS / 2 - 1;  // NUMPTS values 4, 6, ..., 204
+
+constexpr double PI = 3.141592653589793;      // constants.f90:16
+constexpr double C_LIGHT = 299792458.0;       // constants.f90:15
+constexpr double R_D = 287.0596736665907;
+constexpr double R_V = 461.5249933083879;
+constexpr double SKY_TEMP = 2.73;
+constexpr double DELTA_D_MONO = 1.0e-8;
+
+// error codes carried in the status arrays (high bits) next to the
+// reference's severity (low byte)
+enum ErrCode : int32_t {
+  E_NONE = 0,
+  E_NLYRS = 1 << 8,
+  E_NAN_LEVEL = 2 << 8,
+  E_NAN_LAYER = 3 << 8,
+  E_HGT_ORDER = 4 << 8,
+  E_GAS = 5 << 8,
+  E_SURFACE = 6 << 8,
+  E_MASS_SIZE = 7 << 8,
+  E_DIST_PARAMS = 8 << 8,
+  E_ADAPTIVE = 9 << 8,
+  E_MOMENT = 10 << 8,
+  E_SOFT = 11 << 8,
+  E_MIE_INPUT = 12 << 8,
+  E_MIE_TERMS = 13 << 8,
+  E_MIE_RESULT = 14 << 8,
+  E_KEXT = 15 << 8,
+  E_NLEG = 16 << 8,
+  E_NORM = 17 << 8,
+  E_SINGULAR = 18 << 8,
+  E_REFL = 19 << 8,
+  E_NAN_COEF = 20 << 8,
+};
+
+// one call's shapes and settings, passed by value to every kernel
+struct Dims {
+  int nx, ny, ncol, ML, nout, nh, nf, nbmax;
+  int col0;         // first column of the current chunk (global index)
+  int ncol_chunk;   // columns in the chunk
+  int nitems;       // scattering layers in the chunk
+  int item0;        // global index of the chunk's first item
+};
+
+struct Settings {
+  int lgas, lhyd_abs, lhyd_scat, lhyd_emis, incl_hydro_rho, limit_density, adaptive,
+      conserve, outpol, precision;
+  double threshold, softsphere_min, emissivity;
+};
+
+// input profile (device pointers), see pam_profile
+struct Profile {
+  const int32_t *nlyrs;
+  const double *hgt_lev, *temp_lev, *press_lev, *relhum_lev;
+  const double *hgt, *temp, *press, *relhum;
+  const double *hydro_q, *hydro_n, *hydro_reff;
+  const double *groundtemp, *wind10u, *wind10v, *sfc_salinity, *sfc_emissivity;
+  const int32_t *sfc_type, *sfc_model;
+  const char *sfc_refl;
+  const double *obs_height;
+};
+
+struct Hydro4D {
+  const double *as_ratio, *rho_ms, *a_ms, *b_ms, *alpha_as, *beta_as, *p_1, *p_2, *p_3, *p_4,
+      *d_1, *d_2;
+  const int32_t *nbin;
+};
+
+// device workspace (all device pointers)
+struct Work {
+  // per column (global column index)
+  double *hgt_lev, *temp_lev;          // [ncol][ML+1]
+  double *temp, *press, *rho_v, *q_hum, *dz;  // [ncol][ML]
+  double *groundtemp;                  // [ncol]
+  int32_t *nl;                         // [ncol]
+  int32_t *outlev;                     // [ncol][nout]
+  int32_t *col_status;                 // [ncol]
+  uint32_t *hmask;                     // [ncol][ML]
+  int32_t *col_nitems;                 // [ncol]
+  int32_t *col_item0;                  // [ncol] exclusive prefix
+  int32_t *layer_item;                 // [ncol][ML], -1 clear
+  // per column and frequency
+  double *kgas;                        // [ncol][nf][ML]
+  double *sfc_emis, *sfc_refl;         // [ncol][nf][2][NMU]
+  int32_t *cf_status;                  // [ncol][nf]
+  // per item of the chunk
+  int32_t *item_col, *item_layer;      // [nitems]
+  double *dsd_d, *dsd_dd, *dsd_nd, *dsd_rho;  // [nitems][nh][nbmax]
+  int32_t *dsd_flag;                   // [nitems][nh]
+  // per (item, frequency) of the chunk
+  double *kext, *salb;                 // [nitems*nf]
+  int32_t *nleg;                       // [nitems*nf]
+  double *coef;                        // [nitems*nf][3][MAXNLEG]
+  int32_t *ndbl;                       // [nitems*nf]
+  int32_t *itf_status;                 // [nitems*nf]
+  double *R, *T;                       // [nitems*nf][NN2]
+  double *Sp, *Sm;                     // [nitems*nf][NN]
+  // phase-matrix weight tables: offsets per NUMPTS index, see pam_rt.hip
+  const double *wtab;
+  const int64_t *wtab_off;             // [NUM_NP]
+  const double *mu, *qw;               // Lobatto nodes/weights [NMU]
+  // work counters (atomics)
+  unsigned long long *counters;        // [9], see pam_last_work
+};
+
+// ------------------------------------------------------------------
+// device math helpers
+// ------------------------------------------------------------------
+
+__device__ __forceinline__ double sp(double x) { return (double)(float)x; }
+
+struct cd {
+  double re, im;
+};
+__device__ __forceinline__ cd mk(double r, double i) { return cd{r, i}; }
+__device__ __forceinline__ cd operator+(cd a, cd b) { return cd{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cd operator-(cd a, cd b) { return cd{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cd operator*(cd a, cd b) {
+  return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cd operator*(double s, cd a) { return cd{s * a.re, s * a.im}; }
+__device__ __forceinline__ cd operator*(cd a, double s) { return cd{a.re * s, a.im * s}; }
+__device__ __forceinline__ cd operator+(cd a, double s) { return cd{a.re + s, a.im}; }
+__device__ __forceinline__ cd operator-(cd a, double s) { return cd{a.re - s, a.im}; }
+__device__ __forceinline__ cd conj(cd a) { return cd{a.re, -a.im}; }
+// Smith's algorithm for complex division
+__device__ __forceinline__ cd cdiv(cd a, cd b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    double r = b.im / b.re, den = b.re + b.im * r;
+    return cd{(a.re + a.im * r) / den, (a.im - a.re * r) / den};
+  } else {
+    double r = b.re / b.im, den = b.re * r + b.im;
+    return cd{(a.re * r + a.im) / den, (a.im * r - a.re) / den};
+  }
+}
+__device__ __forceinline__ double cabs_(cd a) { return hypot(a.re, a.im); }
+__device__ __forceinline__ cd csqrt_(cd z) {
+  // principal square root
+  double m = hypot(z.re, z.im);
+  double r = sqrt(0.5 * (m + fabs(z.re)));
+  if (r == 0.0) return cd{0.0, 0.0};
+  if (z.re >= 0.0) return cd{r, z.im / (2.0 * r)};
+  return cd{fabs(z.im) / (2.0 * r), z.im >= 0.0 ? r : -r};
+}
+
+// Planck radiance [W m^-2 sr^-1 um^-1] (rt_utilities.f90:69-90, units 'r')
+__device__ __forceinline__ double planck(double temp, double wl) {
+  if (temp > 0.0) {
+    double w2 = wl * wl, w5 = w2 * w2 * wl;
+    return 1.1911e8 / w5 / (exp(1.4388e4 / (wl * temp)) - 1.0);
+  }
+  return 0.0;
+}
+
+// inverse Planck (CONVERT_OUTPUT, rt_utilities.f90:705-735)
+__device__ __forceinline__ double rad2tb(double rad, double wl) {
+  double w2 = wl * wl, w5 = w2 * w2 * wl;
+  if (rad > 0.0) return 1.4388e4 / (wl * log(1.0 + 1.1911e8 / (rad * w5)));
+  if (rad == 0.0) return 0.0;
+  return -1.4388e4 / (wl * log(1.0 + 1.1911e8 / (-rad * w5)));
+}
+
+__device__ __forceinline__ void set_err(int32_t *st, int32_t code) {
+  atomicMax(st, code | PAM_FATAL);
+}
+
+}  // namespace pam
+
+// ------------------------------------------------------------------
+// launchers (host side), one per translation unit
+// ------------------------------------------------------------------
+namespace pam {
+void launch_columns(const Dims &d, const Settings &s, const Profile &p, const pam_hydro_desc *hyd,
+                    const Hydro4D &h4, const double *freqs, Work &w, hipStream_t st);
+void launch_items(const Dims &d, Work &w, hipStream_t st);
+void launch_dsd(const Dims &d, const Settings &s, const Profile &p, const pam_hydro_desc *hyd,
+                const Hydro4D &h4, Work &w, hipStream_t st);
+void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, const Hydro4D &h4,
+                const Profile &p, const double *freqs, Work &w, hipStream_t st);
+void launch_surface(const Dims &d, const Settings &s, const Profile &p, const double *freqs,
+                    Work &w, double *emis_out, hipStream_t st);
+void launch_doubling(const Dims &d, const Settings &s, const double *freqs, Work &w,
+                     hipStream_t st);
+void launch_adding(const Dims &d, const Settings &s, const double *freqs, const char *sfc_refl,
+                   Work &w, double *tb, hipStream_t st);
+}  // namespace pam

@@ -1,0 +1,772 @@
+// pam_rt.hip -- the RT4 solver on the device.
+//
+//  k_doubling: one workgroup per (scattering layer, channel).  Builds the
+//    m=0 phase matrix from the Legendre series (scatcnv.f90:103-192 with the
+//    rt_utilities.f90:242-493 helpers), checks its normalisation
+//    (radscat4.f90:87-136), initialises the thin layer (radintg4.f90:1-73)
+//    and doubles it up (radintg4.f90:174-309).  Only the + quadrants are
+//    formed: for randomly oriented particles with NSTOKES=2 the (-,-) and
+//    (+,-) quadrants equal (+,+) and (-,+) exactly, so R- = R+ and T- = T+
+//    throughout the symmetric doubling.
+//  k_adding: one workgroup per (column, channel).  Streams the layers top
+//    down, combining each into the stack (radintg4.f90:447-513).  Clear
+//    layers (R = 0, T diagonal) take a closed form that performs the same
+//    floating-point operations as the full combine restricted to the
+//    non-zero terms.  Then the surface, the internal radiances at the
+//    output levels (radintg4.f90:119-172) and the Planck inversion to TB
+//    (collect_output.f90:31-88).
+//
+// All matrices live in LDS with a padded leading dimension; products are
+// 2x2 register blocks per thread, the inverse is Gauss-Jordan with partial
+// pivoting (the reference calls LAPACK DGETRF/DGETRI).
+#include "pam_internal.h"
+
+namespace pam {
+
+constexpr int RT_WG = 256;
+constexpr int LD = NN + 1;
+constexpr int MSZ = NN * LD;
+
+// ------------------------------------------------------------------
+// LDS dense helpers; every thread of the workgroup calls them
+// ------------------------------------------------------------------
+
+// C = A * B
+__device__ __forceinline__ void mm(const double *A, const double *B, double *C) {
+  const int t = threadIdx.x, r0 = (t >> 4) * 2, c0 = (t & 15) * 2;
+  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+#pragma unroll 8
+  for (int k = 0; k < NN; k++) {
+    double a0 = A[r0 * LD + k], a1 = A[(r0 + 1) * LD + k];
+    double b0 = B[k * LD + c0], b1 = B[k * LD + c0 + 1];
+    c00 += a0 * b0;
+    c01 += a0 * b1;
+    c10 += a1 * b0;
+    c11 += a1 * b1;
+  }
+  __syncthreads();
+  C[r0 * LD + c0] = c00;
+  C[r0 * LD + c0 + 1] = c01;
+  C[(r0 + 1) * LD + c0] = c10;
+  C[(r0 + 1) * LD + c0 + 1] = c11;
+  __syncthreads();
+}
+
+// C = D + A * B
+__device__ __forceinline__ void mm_add(const double *D, const double *A, const double *B,
+                                       double *C) {
+  const int t = threadIdx.x, r0 = (t >> 4) * 2, c0 = (t & 15) * 2;
+  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+#pragma unroll 8
+  for (int k = 0; k < NN; k++) {
+    double a0 = A[r0 * LD + k], a1 = A[(r0 + 1) * LD + k];
+    double b0 = B[k * LD + c0], b1 = B[k * LD + c0 + 1];
+    c00 += a0 * b0;
+    c01 += a0 * b1;
+    c10 += a1 * b0;
+    c11 += a1 * b1;
+  }
+  double d00 = D[r0 * LD + c0], d01 = D[r0 * LD + c0 + 1];
+  double d10 = D[(r0 + 1) * LD + c0], d11 = D[(r0 + 1) * LD + c0 + 1];
+  __syncthreads();
+  C[r0 * LD + c0] = d00 + c00;
+  C[r0 * LD + c0 + 1] = d01 + c01;
+  C[(r0 + 1) * LD + c0] = d10 + c10;
+  C[(r0 + 1) * LD + c0 + 1] = d11 + c11;
+  __syncthreads();
+}
+
+// y = A x for a 32-vector, by the 32 threads of group g (no barrier)
+__device__ __forceinline__ double mv_row(const double *A, const double *x, int i) {
+  double s = 0.0;
+#pragma unroll 8
+  for (int k = 0; k < NN; k++) s += A[i * LD + k] * x[k];
+  return s;
+}
+
+// G = inv(I - X); Y is scratch.  Returns false if singular.
+__device__ bool inv_i_minus(const double *X, double *Y, double *G, double *s_fac, int *s_piv) {
+  const int t = threadIdx.x;
+  for (int e = t; e < NN2; e += RT_WG) {
+    int i = e / NN, j = e % NN;
+    Y[i * LD + j] = (i == j ? 1.0 : 0.0) - X[i * LD + j];
+    G[i * LD + j] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  bool ok = true;
+  for (int k = 0; k < NN; k++) {
+    // pivot: first row of maximal |Y[i][k]|, i >= k (idamax order)
+    if (t < 64) {
+      double v = (t >= k && t < NN) ? fabs(Y[t * LD + k]) : -1.0;
+      int idx = t;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        double v2 = __shfl_xor(v, o, 64);
+        int i2 = __shfl_xor(idx, o, 64);
+        if (v2 > v || (v2 == v && i2 < idx)) {
+          v = v2;
+          idx = i2;
+        }
+      }
+      if (t == 0) s_piv[0] = idx;
+    }
+    __syncthreads();
+    const int p = s_piv[0];
+    if (p != k) {
+      if (t < 2 * NN) {
+        double *M = (t < NN) ? Y : G;
+        int j = t % NN;
+        double a = M[k * LD + j];
+        M[k * LD + j] = M[p * LD + j];
+        M[p * LD + j] = a;
+      }
+      __syncthreads();
+    }
+    const double piv = Y[k * LD + k];
+    if (piv == 0.0) ok = false;
+    if (t < NN) s_fac[t] = (t == k) ? 0.0 : Y[t * LD + k];
+    __syncthreads();
+    if (t < 2 * NN) {
+      double *M = (t < NN) ? Y : G;
+      int j = t % NN;
+      M[k * LD + j] = M[k * LD + j] / piv;
+    }
+    __syncthreads();
+    for (int e = t; e < 2 * NN2; e += RT_WG) {
+      int half = e / NN2, r = (e % NN2) / NN, j = e % NN;
+      if (r == k) continue;
+      double *M = half ? G : Y;
+      M[r * LD + j] = M[r * LD + j] - s_fac[r] * M[k * LD + j];
+    }
+    __syncthreads();
+  }
+  return ok;
+}
+
+// ------------------------------------------------------------------
+// phase matrix + doubling
+// ------------------------------------------------------------------
+
+__global__ void __launch_bounds__(RT_WG) k_doubling(Dims d, Settings s, const double *freqs,
+                                                    Work w) {
+  __shared__ double sR[MSZ], sT[MSZ], sG[MSZ], sX[MSZ], sY[MSZ];
+  __shared__ double sc[NCOEF][MAXNLEG];
+  __shared__ double sv[8][NN];  // S+, S-, C+, C-, chain scratch
+  __shared__ double sfac[NN];
+  __shared__ double snorm[NMU];
+  __shared__ int spiv[1];
+  __shared__ double smu[NMU], sqw[NMU];
+
+  const int t = threadIdx.x;
+  const int itf = blockIdx.x;
+  const int li = itf / d.nf, f = itf % d.nf;
+  const int col = w.item_col[li], z = w.item_layer[li];
+  if (t < NMU) {
+    smu[t] = w.mu[t];
+    sqw[t] = w.qw[t];
+  }
+  if (w.itf_status[itf] & 0xff) return;
+  const int nleg = w.nleg[itf];
+  double kext = w.kext[itf], salb = w.salb[itf];
+  if (nleg <= 0) {  // no Legendre series: no scatcnv, zero optics
+    kext = 0.0;
+    salb = 0.0;
+  }
+  const double cst = kext * salb / (4.0 * 3.1415926535897932384);
+  const double *cg = w.coef + (size_t)itf * NCOEF * MAXNLEG;
+  for (int e = t; e < NCOEF * MAXNLEG; e += RT_WG) {
+    int sidx = e / MAXNLEG, l = e % MAXNLEG;
+    sc[sidx][l] = (l <= nleg && nleg > 0) ? cst * cg[sidx * MAXNLEG + l] : 0.0;
+  }
+  __syncthreads();
+  // ---- scatcnv: m = 0 Fourier mode of the rotated phase matrix for the
+  // (+,+) quadrant (into sX) and the incoming-minus/outgoing-plus quadrant
+  // (into sY).  Each thread does 2 of the 512 (quadrant, mu2, mu1) pairs.
+  const int numpts = 2 * ((nleg + 1) / 2) + 4;
+  if (nleg > 0 && s.lhyd_scat) {
+    for (int pr = t; pr < 2 * NMU * NMU; pr += RT_WG) {
+      const int q = pr / (NMU * NMU), j2 = (pr / NMU) % NMU, j1 = pr % NMU;
+      const double mu1 = q ? -smu[j1] : smu[j1];
+      const double mu2 = smu[j2];
+      const double st1 = sqrt(1. - mu1 * mu1), st2 = sqrt(1. - mu2 * mu2);
+      double e11 = 0.0, e12 = 0.0, e21 = 0.0, e22 = 0.0;
+      for (int k = 1; k <= numpts / 2 + 1; k++) {
+        const double delphi = (2.0 * PI * (k - 1)) / numpts;
+        const double cosphi = cos(delphi), sinphi = sin(delphi);
+        const double x = mu1 * mu2 + sqrt((1. - mu1 * mu1) * (1. - mu2 * mu2)) * cosphi;
+        double a1 = 0.0, b1 = 0.0, a3 = 0.0, pl = 1.0, pl1 = 1.0, pl2 = 0.0;
+        for (int l = 0; l <= nleg; l++) {
+          if (l > 0) pl = (double)(2 * l - 1) * x * pl1 / l - (double)(l - 1) * pl2 / l;
+          a1 += sc[0][l] * pl;
+          b1 += sc[1][l] * pl;
+          a3 += sc[2][l] * pl;
+          pl2 = pl1;
+          pl1 = pl;
+        }
+        const double sin_scat = sqrt(fmax(0.0, 1. - x * x));
+        double sin1, sin2, cos1, cos2;
+        if (sin_scat == 0.0) {
+          sin1 = 0.0;
+          sin2 = 0.0;
+          cos1 = 1.0;
+          cos2 = -1.0;
+        } else {
+          sin1 = st2 * sinphi / sin_scat;
+          sin2 = st1 * sinphi / sin_scat;
+          cos1 = (st1 * mu2 - st2 * mu1 * cosphi) / sin_scat;
+          cos2 = (st2 * mu1 - st1 * mu2 * cosphi) / sin_scat;
+        }
+        const double sin21 = 2.0 * sin1 * cos1, cos21 = 1.0 - 2.0 * sin1 * sin1;
+        const double sin22 = 2.0 * sin2 * cos2, cos22 = 1.0 - 2.0 * sin2 * sin2;
+        const double wk = (k == 1 || k == numpts / 2 + 1) ? 1.0 : 2.0;
+        e11 += wk * a1;
+        e12 += wk * (b1 * cos21);
+        e21 += wk * (b1 * cos22);
+        e22 += wk * (a1 * cos21 * cos22 - a3 * sin21 * sin22);
+      }
+      double *M = q ? sY : sX;
+      const int r = NST * j2, c = NST * j1;
+      M[r * LD + c] = e11 / numpts;
+      M[r * LD + c + 1] = e12 / numpts;
+      M[(r + 1) * LD + c] = e21 / numpts;
+      M[(r + 1) * LD + c + 1] = e22 / numpts;
+    }
+  } else {
+    for (int e = t; e < NN2; e += RT_WG) {
+      int i = e / NN, j = e % NN;
+      sX[i * LD + j] = 0.0;
+      sY[i * LD + j] = 0.0;
+    }
+  }
+  __syncthreads();
+  const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
+  // ---- CHECK_NORM4
+  if (t < NMU) {
+    double sum = absorb - kext;
+    for (int j2 = 0; j2 < NMU; j2++)
+      sum = sum + 2.0 * PI * sqw[j2] * (sX[(NST * j2) * LD + NST * t] + sY[(NST * j2) * LD + NST * t]);
+    snorm[t] = fabs(sum);
+  }
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int j = 0; j < NMU; j++) mx = fmax(mx, snorm[j]);
+    if (mx > 5.0e-4) set_err(&w.itf_status[itf], E_NORM);
+  }
+  // ---- layer thickness, doubling count (radtran4.f90:311-329)
+  const double wl = C_LIGHT / (freqs[f] * 1.e3);
+  const double zdiff = fabs(w.hgt_lev[(size_t)col * (d.ML + 1) + z + 1] -
+                            w.hgt_lev[(size_t)col * (d.ML + 1) + z]);
+  const double gas = fmax(w.kgas[((size_t)col * d.nf + f) * d.ML + z], 0.0);
+  const double p0 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z + 1], wl);
+  const double p1 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z], wl);
+  const double extinct = kext + gas;
+  const double fnd = log(fmax(extinct * zdiff, 1.0e-7) / 1.0e-6) / (double)logf(2.0f);
+  const int nd = (fnd > 0.0) ? (int)fnd + 1 : 0;
+  const double nsub = (double)ldexpf(1.0f, nd);
+  const double dz = zdiff / nsub;
+  double linfac = (p0 == 0.0) ? 0.0 : (p1 / p0 - 1.0) / nsub;
+  // ---- INITIALIZE4 / INITIAL_SOURCE4
+  const double C2 = 2.0 * 3.1415926535897932384;
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int r = e / NN, c = e % NN;
+    const int i2 = r % NST, j2 = r / NST, i1 = c % NST, j1 = c / NST;
+    const double tmp = dz / smu[j2];
+    const double gext = (i1 == i2) ? gas : 0.0;
+    const double diag = (r == c) ? 1.0 : 0.0;
+    const double ext = (j1 == j2) ? ((i1 == i2) ? kext : 0.0) + gext : 0.0;
+    sR[r * LD + c] = C2 * tmp * sqw[j1] * sY[r * LD + c];
+    sT[r * LD + c] = diag - tmp * (ext - C2 * sqw[j1] * sX[r * LD + c]);
+  }
+  if (t < NN) {
+    const int i = t % NST, j = t / NST;
+    const double e = (i == 0) ? gas : 0.0;
+    const double tmp = p0 * dz / smu[j];
+    const double v = tmp * ((i == 0 ? absorb : 0.0) + e);
+    sv[0][t] = v;  // S+
+    sv[1][t] = v;  // S-
+    sv[2][t] = v;  // C+
+    sv[3][t] = v;  // C-
+  }
+  __syncthreads();
+  // ---- doubling (DOUBLING_INTEGRATION4, SYMMETRIC)
+  for (int it = 0; it < nd; it++) {
+    mm(sR, sR, sX);
+    if (!inv_i_minus(sX, sY, sG, sfac, spiv)) {
+      if (t == 0) set_err(&w.itf_status[itf], E_SINGULAR);
+      return;
+    }
+    // sources: four independent 3-matvec chains, one per 32-thread group
+    {
+      const int g = t >> 5, i = t & 31;
+      double a = 0.0;
+      if (g == 0) a = sv[3][i] * linfac + sv[1][i];  // Sp: S- + f C-
+      __syncthreads();
+      // stage 1: u = R * v_g
+      double u = 0.0;
+      if (g == 0) {
+        sv[4][i] = a;
+      }
+      __syncthreads();
+      if (g == 0) u = mv_row(sR, sv[4], i);        // R (S- + f C-)
+      if (g == 1) u = mv_row(sR, sv[3], i);        // R C-
+      if (g == 2) u = mv_row(sR, sv[0], i);        // R S+
+      if (g == 3) u = mv_row(sR, sv[2], i);        // R C+
+      __syncthreads();
+      if (g == 0) sv[4][i] = sv[0][i] + u;                       // S+ + R(...)
+      if (g == 1) sv[5][i] = sv[2][i] + u;                       // C+ + R C-
+      if (g == 2) sv[6][i] = (sv[1][i] + linfac * sv[3][i]) + u; // (S- + f C-) + R S+
+      if (g == 3) sv[7][i] = sv[3][i] + u;                       // C- + R C+
+      __syncthreads();
+      double v2 = (g < 4) ? mv_row(sG, sv[4 + g], i) : 0.0;
+      __syncthreads();
+      if (g < 4) sv[4 + g][i] = v2;
+      __syncthreads();
+      double v3 = (g < 4) ? mv_row(sT, sv[4 + g], i) : 0.0;
+      double nv = 0.0;
+      if (g == 0) nv = (sv[0][i] + v3) + linfac * sv[2][i];  // new S+
+      if (g == 1) nv = sv[2][i] + v3;                        // new C+
+      if (g == 2) nv = sv[1][i] + v3;                        // new S-
+      if (g == 3) nv = sv[3][i] + v3;                        // new C-
+      __syncthreads();
+      if (g < 4) sv[g == 0 ? 0 : (g == 1 ? 2 : (g == 2 ? 1 : 3))][i] = nv;
+      __syncthreads();
+    }
+    // R <- R + T (G (R T)); T <- T (G T)
+    mm(sR, sT, sX);
+    mm(sG, sX, sY);
+    mm_add(sR, sT, sY, sR);
+    mm(sG, sT, sX);
+    mm(sT, sX, sY);
+    for (int e = t; e < NN2; e += RT_WG) {
+      int i = e / NN, j = e % NN;
+      sT[i * LD + j] = sY[i * LD + j];
+    }
+    linfac = 2.0 * linfac;
+    __syncthreads();
+  }
+  // ---- store R, T, S+, S-
+  double *R = w.R + (size_t)itf * NN2, *T = w.T + (size_t)itf * NN2;
+  for (int e = t; e < NN2; e += RT_WG) {
+    int i = e / NN, j = e % NN;
+    R[e] = sR[i * LD + j];
+    T[e] = sT[i * LD + j];
+  }
+  if (t < NN) {
+    w.Sp[(size_t)itf * NN + t] = sv[0][t];
+    w.Sm[(size_t)itf * NN + t] = sv[1][t];
+  }
+  if (t == 0) {
+    w.ndbl[itf] = nd;
+    atomicAdd(&w.counters[3], (unsigned long long)nd);
+    if (nleg > 0 && s.lhyd_scat)
+      atomicAdd(&w.counters[8], (unsigned long long)(numpts / 2 + 1) * (nleg + 1));
+  }
+}
+
+
+// ------------------------------------------------------------------
+// adding, internal radiance, TB
+// ------------------------------------------------------------------
+
+// operator of the layer stack above the current level: R+, R-, T+, T- (LDS
+// matrices) and S+, S- (LDS vectors)
+struct Stack {
+  double *Rp, *Rm, *Tp, *Tm, *Sp, *Sm;
+};
+
+// v_out[i] = A v_in[i] chain helper for the 32 threads t < NN, with
+// barriers so that the shared vectors may be reused between stages
+__device__ __forceinline__ void mv_stage(const double *A, double *v) {
+  const int t = threadIdx.x;
+  double u = (t < NN) ? mv_row(A, v, t) : 0.0;
+  __syncthreads();
+  if (t < NN) v[t] = u;
+  __syncthreads();
+}
+
+// COMBINE_LAYERS4 (radintg4.f90:447-513): stack on top of a doubled layer
+// with R2+ = R2- = R2 and T2+ = T2- = T2.  The minus side goes first, since
+// it reads the stack's old R+ and T+.
+__device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, const double *S2p,
+                             const double *S2m, double *G, double *X, double *Y, double *vs,
+                             double *sfac, int *spiv) {
+  const int t = threadIdx.x;
+  mm(R2, s1.Rp, X);
+  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (t < NN) vs[t] = S2m[t] + mv_row(R2, s1.Sp, t);  // S2m + R2m S1p
+  __syncthreads();
+  mv_stage(G, vs);
+  mv_stage(s1.Tm, vs);
+  if (t < NN) s1.Sm[t] = s1.Sm[t] + vs[t];            // S1m + T1m G (...)
+  mm(R2, s1.Tp, X);
+  mm(G, X, Y);
+  mm_add(s1.Rm, s1.Tm, Y, s1.Rm);                     // R1m + T1m G R2m T1p
+  mm(G, T2, X);
+  mm(s1.Tm, X, s1.Tm);                                // T1m G T2m
+  mm(s1.Rp, R2, X);
+  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (t < NN) vs[t] = s1.Sp[t] + mv_row(s1.Rp, S2m, t);  // S1p + R1p S2m
+  __syncthreads();
+  mv_stage(G, vs);
+  mv_stage(T2, vs);
+  if (t < NN) s1.Sp[t] = S2p[t] + vs[t];              // S2p + T2p G (...)
+  mm(s1.Rp, T2, X);
+  mm(G, X, Y);
+  mm_add(R2, T2, Y, s1.Rp);                           // R2p + T2p G R1p T2m
+  mm(G, s1.Tp, X);
+  mm(T2, X, s1.Tp);                                   // T2p G T1p
+  __syncthreads();
+  return true;
+}
+
+// The same combine for a clear layer (NONSCATTER_LAYER4: R2 = 0,
+// T2 = diag(dg)).  With R2 = 0 both gammas are the identity and every
+// product reduces to a diagonal scaling or one matrix-vector product.
+__device__ void combine_clear(Stack &s1, const double *dg, const double *S2p, const double *S2m) {
+  const int t = threadIdx.x;
+  double u = 0.0, u2 = 0.0;
+  if (t < NN) {
+    u = mv_row(s1.Tm, S2m, t);              // T1m S2m
+    u2 = s1.Sp[t] + mv_row(s1.Rp, S2m, t);  // S1p + R1p S2m
+  }
+  __syncthreads();
+  if (t < NN) {
+    s1.Sm[t] = s1.Sm[t] + u;
+    s1.Sp[t] = S2p[t] + dg[t] * u2;
+  }
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int i = e / NN, j = e % NN;
+    s1.Rp[i * LD + j] = dg[i] * (s1.Rp[i * LD + j] * dg[j]);
+    s1.Tp[i * LD + j] = dg[i] * s1.Tp[i * LD + j];
+    s1.Tm[i * LD + j] = s1.Tm[i * LD + j] * dg[j];
+  }
+  __syncthreads();
+}
+
+// The combine with the surface "layer" of get_sfc_matrices
+// (sfc_matrices.f90:21-194): R2+ = 0, R2- = Rs, T2 = I, S2 = 0.
+__device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *X, double *Y,
+                                double *vs, double *sfac, int *spiv) {
+  const int t = threadIdx.x;
+  mm(Rs, s1.Rp, X);
+  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (t < NN) vs[t] = mv_row(Rs, s1.Sp, t);
+  __syncthreads();
+  mv_stage(G, vs);
+  mv_stage(s1.Tm, vs);
+  if (t < NN) s1.Sm[t] = s1.Sm[t] + vs[t];
+  mm(Rs, s1.Tp, X);
+  mm(G, X, Y);
+  mm_add(s1.Rm, s1.Tm, Y, s1.Rm);
+  mm(s1.Tm, G, s1.Tm);
+  mm(s1.Rp, Rs, X);
+  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (t < NN) vs[t] = s1.Sp[t];
+  __syncthreads();
+  mv_stage(G, vs);
+  if (t < NN) s1.Sp[t] = vs[t];
+  mm(G, s1.Rp, s1.Rp);
+  mm(G, s1.Tp, s1.Tp);
+  return true;
+}
+
+// Loads bottom-up layer z: a doubled layer into (R2, T2, S2p, S2m), or the
+// closed form of a clear layer into (dg, S2p, S2m).  Returns true when the
+// layer scatters.
+__device__ bool load_layer(const Dims &d, const Work &w, int col, int f, int z, double wl,
+                           const double *smu, double *R2, double *T2, double *dg, double *S2p,
+                           double *S2m, int *sbad) {
+  const int t = threadIdx.x;
+  const size_t oy = (size_t)col * d.ML + z;
+  const int item = w.layer_item[oy];
+  if (item >= 0) {
+    const size_t itf = (size_t)(item - d.item0) * d.nf + f;
+    if (t == 0 && (w.itf_status[itf] & 0xff)) atomicMax(sbad, w.itf_status[itf]);
+    const double *R = w.R + itf * NN2, *T = w.T + itf * NN2;
+    for (int e = t; e < NN2; e += RT_WG) {
+      const int i = e / NN, j = e % NN;
+      R2[i * LD + j] = R[e];
+      T2[i * LD + j] = T[e];
+    }
+    if (t < NN) {
+      S2p[t] = w.Sp[itf * NN + t];
+      S2m[t] = w.Sm[itf * NN + t];
+    }
+    __syncthreads();
+    return true;
+  }
+  const size_t ol = (size_t)col * (d.ML + 1) + z;
+  const double zdiff = fabs(w.hgt_lev[ol + 1] - w.hgt_lev[ol]);
+  const double gas = fmax(w.kgas[((size_t)col * d.nf + f) * d.ML + z], 0.0);
+  const double dtau = zdiff * gas;
+  const double p0 = planck(w.temp_lev[ol + 1], wl);  // top of the layer
+  const double p1 = planck(w.temp_lev[ol], wl);      // bottom
+  if (t < NN) {
+    const int i = t % NST, j = t / NST;
+    dg[t] = exp(-dtau / smu[j]);
+    double a = 0.0, b = 0.0;
+    if (i == 0 && dtau > 0.0) {
+      const double path = dtau / smu[j];
+      const double slope = (p1 - p0) / path;
+      a = p1 - slope - (p1 - slope * (1.0 + path)) * exp(-path);
+      b = p0 + slope - (p0 + slope * (1.0 + path)) * exp(-path);
+    }
+    S2p[t] = a;
+    S2m[t] = b;
+  }
+  __syncthreads();
+  return false;
+}
+
+// INTERNAL_RADIANCE4 at the surface level: up = stack of all layers, down =
+// the surface (R- = Rs, T- = I, S- = 0)
+__device__ bool radiance_at_surface(const Stack &up, const double *Rs, const double *sky,
+                                    const double *gnd, double *G, double *X, double *Y,
+                                    double *vs, double *ur, double *dr, double *sfac,
+                                    int *spiv) {
+  const int t = threadIdx.x;
+  mm(up.Rp, Rs, X);
+  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  double s_ = 0.0;
+  if (t < NN) {
+    s_ = mv_row(up.Rp, gnd, t);          // UPR (DNT INBOT)
+    s_ = mv_row(up.Tp, sky, t) + s_;     // + UPT INTOP
+    s_ = 0.0 + s_;                       // + UPR DNS (= 0)
+    s_ = up.Sp[t] + s_;
+  }
+  __syncthreads();
+  if (t < NN) vs[t] = s_;
+  __syncthreads();
+  if (t < NN) dr[t] = mv_row(G, vs, t);
+  __syncthreads();
+  mm(Rs, up.Rp, X);
+  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (t < NN) vs[t] = mv_row(up.Tp, sky, t);
+  __syncthreads();
+  if (t < NN) {
+    s_ = mv_row(Rs, vs, t);
+    s_ = gnd[t] + s_;
+    s_ = mv_row(Rs, up.Sp, t) + s_;
+    s_ = 0.0 + s_;
+  }
+  __syncthreads();
+  if (t < NN) vs[t] = s_;
+  __syncthreads();
+  if (t < NN) ur[t] = mv_row(G, vs, t);
+  __syncthreads();
+  return true;
+}
+
+// INTERNAL_RADIANCE4 at the top: up = nothing (R = 0, T = I, S = 0, both
+// gammas the identity), down = stack including the surface
+__device__ void radiance_at_top(const Stack &dn, const double *sky, const double *gnd,
+                                double *ur, double *dr) {
+  const int t = threadIdx.x;
+  if (t < NN) {
+    dr[t] = sky[t];
+    double s_ = mv_row(dn.Rm, sky, t);
+    s_ = mv_row(dn.Tm, gnd, t) + s_;
+    s_ = 0.0 + s_;
+    ur[t] = dn.Sm[t] + s_;
+  }
+  __syncthreads();
+}
+
+// CONVERT_OUTPUT + collect_output (rt_utilities.f90:687-737,
+// collect_output.f90:31-88) for one output level
+__device__ void write_tb(const Dims &d, const Settings &s, int col, int f, int io, double wl,
+                         const double *ur, const double *dr, double *tb) {
+  const int t = threadIdx.x;
+  if (t < NMU) {
+    const int j = t;
+    double o_up[2], o_dn[2];
+    const double iu = ur[NST * j], qu = ur[NST * j + 1];
+    const double idn = dr[NST * j], qdn = dr[NST * j + 1];
+    if (s.outpol == 0) {
+      o_up[0] = rad2tb(2.0 * (0.5 * (iu + qu)), wl);
+      o_up[1] = rad2tb(2.0 * (0.5 * (iu - qu)), wl);
+      o_dn[0] = rad2tb(2.0 * (0.5 * (idn + qdn)), wl);
+      o_dn[1] = rad2tb(2.0 * (0.5 * (idn - qdn)), wl);
+    } else {
+      o_up[0] = rad2tb(iu, wl);
+      o_up[1] = rad2tb(qu, wl);
+      o_dn[0] = rad2tb(idn, wl);
+      o_dn[1] = rad2tb(qdn, wl);
+    }
+    const int su = NMU - 1 - j, sd = NMU + j;
+    const size_t base = (size_t)col * d.nout + io;
+    for (int k = 0; k < 2; k++) {
+      tb[((base * PAM_NANGLES + su) * d.nf + f) * 2 + k] = o_up[k];
+      tb[((base * PAM_NANGLES + sd) * d.nf + f) * 2 + k] = o_dn[k];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const double *freqs,
+                                                  const char *sfc_refl, Work w, double *tb) {
+  __shared__ double M[9][MSZ];
+  __shared__ double sv[10][NN];
+  __shared__ double sfac[NN];
+  __shared__ int spiv[1];
+  __shared__ double smu[NMU], sqw[NMU];
+  __shared__ int sbad[1];
+
+  const int t = threadIdx.x;
+  const int cf = blockIdx.x;
+  const int cl = cf / d.nf, f = cf % d.nf;
+  const int col = d.col0 + cl;
+  if (t < NMU) {
+    smu[t] = w.mu[t];
+    sqw[t] = w.qw[t];
+  }
+  if (t == 0) sbad[0] = 0;
+  int32_t *cst = w.cf_status + (size_t)col * d.nf + f;
+  const int L = w.nl[col];
+  const int nout = d.nout;
+  const double wl = C_LIGHT / (freqs[f] * 1.e3);
+  const char refl = sfc_refl ? sfc_refl[col] : 'S';
+  __syncthreads();
+  auto fail = [&](int32_t code) {
+    if (t == 0 && code) atomicMax(cst, code);
+    for (int e = t; e < nout * PAM_NANGLES * 2; e += RT_WG) {
+      const int io = e / (PAM_NANGLES * 2), a = (e / 2) % PAM_NANGLES, k = e % 2;
+      tb[((((size_t)col * nout + io) * PAM_NANGLES + a) * d.nf + f) * 2 + k] =
+          __builtin_nan("");
+    }
+  };
+  if ((w.col_status[col] & 0xff) >= PAM_FATAL) {
+    fail(w.col_status[col]);
+    return;
+  }
+  if ((*cst & 0xff) >= PAM_FATAL) {
+    fail(0);
+    return;
+  }
+  if (refl != 'S' && refl != 'F' && refl != 'L') {
+    fail(E_REFL | PAM_WARNING);
+    return;
+  }
+  int lev_toa = -1, lev_sfc = -1;
+  for (int io = 0; io < nout; io++) {
+    int o = w.outlev[(size_t)col * nout + io];
+    o = max(1, min(o, L + 2));
+    if (o == 1)
+      lev_toa = io;
+    else if (o == L + 1)
+      lev_sfc = io;
+    else {
+      // a level inside the column needs a second stack: not supported
+      fail(E_NLYRS | PAM_FATAL);
+      return;
+    }
+  }
+  Stack stk = {M[0], M[1], M[2], M[3], sv[0], sv[1]};
+  double *R2 = M[4], *T2 = M[5], *G = M[6], *X = M[7], *Y = M[8];
+  double *S2p = sv[2], *S2m = sv[3], *dg = sv[4], *vs = sv[5];
+  double *sky = sv[6], *gnd = sv[7], *ur = sv[8], *dr = sv[9];
+  const double psky = planck(SKY_TEMP, wl);
+  const double pg = planck(w.groundtemp[col], wl);
+  const double *E = w.sfc_emis + ((size_t)col * d.nf + f) * 2 * NMU;
+  const double *Rf = w.sfc_refl + ((size_t)col * d.nf + f) * 2 * NMU;
+  if (t < NN) {
+    const int i = t % NST, j = t / NST;
+    sky[t] = (i == 0) ? (1.0 - 0.0) * psky : 0.0;
+    if (refl == 'L') {
+      gnd[t] = (i == 0) ? E[0] * pg : 0.0;
+    } else {
+      const double rv = Rf[j], rh = Rf[NMU + j];
+      const double r1 = (rv + rh) / 2.0, r2 = (rv - rh) / 2.0;
+      gnd[t] = (i == 0) ? (1.0 - r1) * pg : -r2 * pg;
+    }
+  }
+  __syncthreads();
+  // fold the layers top down (radtran4.f90:386-417 with prefix reuse: the
+  // stack above the surface level is the stack the top level continues)
+  unsigned long long nscat = 0, nclear = 0;
+  for (int l = 1; l <= L; l++) {
+    const int z = L - l;
+    const bool sc = load_layer(d, w, col, f, z, wl, smu, R2, T2, dg, S2p, S2m, sbad);
+    if (l == 1) {
+      for (int e = t; e < NN2; e += RT_WG) {
+        const int i = e / NN, j = e % NN;
+        const double r = sc ? R2[i * LD + j] : 0.0;
+        const double tt = sc ? T2[i * LD + j] : (i == j ? dg[i] : 0.0);
+        stk.Rp[i * LD + j] = r;
+        stk.Rm[i * LD + j] = r;
+        stk.Tp[i * LD + j] = tt;
+        stk.Tm[i * LD + j] = tt;
+      }
+      if (t < NN) {
+        stk.Sp[t] = S2p[t];
+        stk.Sm[t] = S2m[t];
+      }
+      __syncthreads();
+    } else if (sc) {
+      if (!combine_scat(stk, R2, T2, S2p, S2m, G, X, Y, vs, sfac, spiv)) {
+        fail(E_SINGULAR | PAM_FATAL);
+        return;
+      }
+      nscat++;
+    } else {
+      combine_clear(stk, dg, S2p, S2m);
+      nclear++;
+    }
+  }
+  if (sbad[0]) {
+    fail(sbad[0]);
+    return;
+  }
+  // surface reflection matrix into R2 (specular/Fresnel or Lambertian)
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int r = e / NN, c = e % NN;
+    const int i2 = r % NST, j2 = r / NST, i1 = c % NST, j1 = c / NST;
+    double v = 0.0;
+    if (refl == 'L') {
+      if (i2 == 0 && i1 == 0) v = 2.0 * Rf[0] * smu[j1] * sqw[j1];
+    } else if (j1 == j2) {
+      const double rv = Rf[j2], rh = Rf[NMU + j2];
+      const double r1 = (rv + rh) / 2.0, r2 = (rv - rh) / 2.0;
+      v = (i1 == i2) ? r1 : r2;
+    }
+    R2[r * LD + c] = v;
+  }
+  __syncthreads();
+  if (lev_sfc >= 0) {
+    if (!radiance_at_surface(stk, R2, sky, gnd, G, X, Y, vs, ur, dr, sfac, spiv)) {
+      fail(E_SINGULAR | PAM_FATAL);
+      return;
+    }
+    write_tb(d, s, col, f, lev_sfc, wl, ur, dr, tb);
+    __syncthreads();
+  }
+  if (lev_toa >= 0) {
+    if (!combine_surface(stk, R2, G, X, Y, vs, sfac, spiv)) {
+      fail(E_SINGULAR | PAM_FATAL);
+      return;
+    }
+    radiance_at_top(stk, sky, gnd, ur, dr);
+    write_tb(d, s, col, f, lev_toa, wl, ur, dr, tb);
+  }
+  if (t == 0) {
+    atomicAdd(&w.counters[4], nscat);
+    atomicAdd(&w.counters[5], nclear);
+  }
+}
+
+void launch_doubling(const Dims &d, const Settings &s, const double *freqs, Work &w,
+                     hipStream_t st) {
+  size_t n = (size_t)d.nitems * d.nf;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_doubling, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, w);
+}
+
+void launch_adding(const Dims &d, const Settings &s, const double *freqs, const char *sfc_refl,
+                   Work &w, double *tb, hipStream_t st) {
+  size_t n = (size_t)d.ncol_chunk * d.nf;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_adding, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
+                     tb);
+}
+
+}  // namespace pam

@@ -1,0 +1,163 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle and the
+reference's golden vectors.  FP64 bar: |dTB| <= 1e-3 K; discrete decisions
+(doubling counts, Legendre orders) identical."""
+
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from pamtra_amd import _capi, workloads
+from _scenario import FREQUENCIES, build_pamtra, passive_nml
+
+pytestmark = pytest.mark.gpu
+
+TB_TOL = 1e-3  # K, FP64 device path vs oracle
+
+
+def _gpu(pam, freqs, diagnostics=True):
+    from pamtra_amd import _lib
+    lib = _lib.library()
+    ctx = _lib.get_context(0)
+    m = _capi.marshal(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p)
+    f, fp = _capi.freqs_array(freqs)
+    hr = _capi.alloc_results(m, len(f), diagnostics=diagnostics)
+    print(f"pam_run ncol={m.nx * m.ny} L={m.max_nlyrs} nf={len(f)}", flush=True)
+    st = lib.pam_run(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d), C.byref(m.profile),
+                     fp, len(f), C.byref(hr.struct))
+    out = dict(hr.arrays)
+    out["msg"] = _capi.results_message(hr)
+    return out, st
+
+
+def _cpu(pam, freqs):
+    out, st, _ = oracle.run(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p, freqs,
+                            nthreads=8, diagnostics=True)
+    return out, st
+
+
+def _compare(g, c, tol=TB_TOL):
+    ok = np.isfinite(c["tb"])
+    assert np.array_equal(ok, np.isfinite(g["tb"]))
+    d = np.abs(g["tb"][ok] - c["tb"][ok])
+    print(f"max |dTB| = {d.max():.3e} K", flush=True)
+    assert d.max() <= tol
+    np.testing.assert_array_equal(g["ndoubles"], c["ndoubles"])
+    np.testing.assert_array_equal(g["nlegen"], c["nlegen"])
+    m = np.isfinite(c["kextatmo"])
+    np.testing.assert_allclose(g["kextatmo"][m], c["kextatmo"][m], rtol=1e-12)
+    m = np.isfinite(c["kexttot"])
+    np.testing.assert_allclose(g["kexttot"][m], c["kexttot"][m], rtol=1e-9, atol=1e-300)
+
+
+def test_golden_through_pypamtra():
+    """The reference's golden scenario through runPamtra on the GPU."""
+    pam = build_pamtra()
+    pam.nmlSet["active"] = False
+    pam.runPamtra(FREQUENCIES)
+    ref = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden",
+                                             "rain_snow_reference.npz"))["tb"]
+    np.testing.assert_allclose(pam.r["tb"], ref, rtol=1e-5)
+    assert pam.r["tb"].shape == (1, 1, 2, 32, 2, 2)
+    assert pam.r["Ze"] == -9999.
+    np.testing.assert_allclose(pam.r["angles_deg"][:16], 180 - pam.r["angles_deg"][16:][::-1])
+
+
+def test_golden_vs_oracle_with_diagnostics():
+    pam = build_pamtra()
+    g, st = _gpu(pam, FREQUENCIES)
+    c, st2 = _cpu(pam, FREQUENCIES)
+    assert st == 0 and st2 == 0
+    _compare(g, c)
+
+
+def test_config1_columns_vs_oracle():
+    pam, freqs = workloads.config1(ncol=24, seed=1001)
+    g, st = _gpu(pam, freqs)
+    c, st2 = _cpu(pam, freqs)
+    assert st == 0 and st2 == 0, (g["msg"], c["msg"])
+    _compare(g, c)
+
+
+def test_config2_columns_vs_oracle():
+    """Uneven layer counts, 7 channels, ocean (TESSEM2) and Lambertian land."""
+    pam, freqs = workloads.config2(ncol=24, seed=1002)
+    g, st = _gpu(pam, freqs)
+    c, st2 = _cpu(pam, freqs)
+    assert st == 0 and st2 == 0, (g["msg"], c["msg"])
+    np.testing.assert_allclose(g["emissivity"], c["emissivity"], rtol=1e-12)
+    _compare(g, c)
+
+
+def test_adaptive_grid_and_soft_spheres_vs_oracle():
+    """The golden's adaptive grid and exp_field_t soft spheres at more
+    channels and contents."""
+    pam = build_pamtra()
+    pam.p["hydro_q"][0, 0, 0, 0] = 3e-4
+    pam.p["hydro_q"][0, 0, 1, 1] = 2e-4
+    freqs = [10.65, 23.8, 36.5, 89.0, 150.0, 183.31]
+    g, st = _gpu(pam, freqs)
+    c, st2 = _cpu(pam, freqs)
+    assert st == 0 and st2 == 0
+    _compare(g, c)
+
+
+def test_bad_column_is_isolated():
+    """A NaN temperature fails its own column with PAM_FATAL, the others are
+    computed as if alone (status per column, no abort)."""
+    pam, freqs = workloads.config1(ncol=6, seed=7)
+    pam.p["temp_lev"][2, 0, :] = np.nan
+    g, st = _gpu(pam, freqs)
+    assert st == _capi.PAM_FATAL
+    assert "nan" in g["msg"]
+    assert (g["status"][2] == _capi.PAM_FATAL).all()
+    good = [0, 1, 3, 4, 5]
+    assert (g["status"][good] == 0).all()
+    assert np.isnan(g["tb"][2]).all()
+    pam2, _ = workloads.config1(ncol=6, seed=7)
+    g2, _ = _gpu(pam2, freqs)
+    np.testing.assert_array_equal(g["tb"][good], g2["tb"][good])
+
+
+def test_repeat_runs_are_bitwise_identical():
+    pam, freqs = workloads.config2(ncol=8, seed=5)
+    a, _ = _gpu(pam, freqs)
+    b, _ = _gpu(pam, freqs)
+    np.testing.assert_array_equal(a["tb"], b["tb"])
+
+
+def test_device_entry_point_matches_host_entry_point():
+    """pam_run_device on HBM-resident torch tensors gives pam_run's result."""
+    torch = pytest.importorskip("torch")
+    from pamtra_amd import _lib
+    pam, freqs = workloads.config2(ncol=16, seed=11)
+    host, _ = _gpu(pam, freqs, diagnostics=False)
+    lib = _lib.library()
+    ctx = _lib.get_context(0)
+    prof = {}
+    for k, v in pam.p.items():
+        if isinstance(v, np.ndarray) and v.dtype.kind in "fiu" and k in (
+                _capi.PROFILE_DOUBLE_KEYS + ["nlyrs", "sfc_type", "sfc_model", "obs_height"]):
+            dt = torch.int32 if k in ("nlyrs", "sfc_type", "sfc_model") else torch.float64
+            prof[k] = torch.as_tensor(np.ascontiguousarray(v), dtype=dt, device="cuda")
+        else:
+            prof[k] = v
+    refl = np.frombuffer("".join(pam.p["sfc_refl"].reshape(-1)).encode(), dtype=np.uint8)
+    prof["sfc_refl"] = torch.as_tensor(refl.copy(), device="cuda")
+    m = _capi.marshal(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, prof)
+    nx, ny, nout, nf = m.nx, m.ny, m.noutlevels, len(freqs)
+    tb = torch.zeros((nx, ny, nout, 32, nf, 2), dtype=torch.float64, device="cuda")
+    status = torch.zeros((nx, ny, nf), dtype=torch.int32, device="cuda")
+    r = _capi.PamResults()
+    r.tb = C.cast(C.c_void_p(tb.data_ptr()), C.POINTER(C.c_double))
+    r.status = C.cast(C.c_void_p(status.data_ptr()), C.POINTER(C.c_int32))
+    msg = C.create_string_buffer(256)
+    r.msg = C.cast(msg, C.c_char_p)
+    r.msglen = 256
+    f, fp = _capi.freqs_array(freqs)
+    st = lib.pam_run_device(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),
+                            C.byref(m.profile), fp, nf, C.byref(r), None)
+    assert st == 0, msg.value
+    np.testing.assert_array_equal(tb.cpu().numpy(), host["tb"])

@@ -1,0 +1,227 @@
+"""Pins the CPU oracle (oracle/pam_oracle.c) before it is used to check the
+device path: the reference's own golden vectors and known answers, closed
+forms, an independent Mie series, invariants and a brute-force matrix
+exponential."""
+
+import numpy as np
+import pytest
+from numpy.polynomial import legendre as npleg
+from scipy.linalg import expm
+
+import oracle
+from _scenario import FREQUENCIES, build_pamtra, passive_nml
+
+GOLDEN = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__),
+                                            "golden", "rain_snow_reference.npz"))
+
+
+def _run(pam, freqs, **kw):
+    out, st, _ = oracle.run(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p, freqs,
+                            **kw)
+    return out, st
+
+
+def test_golden_brightness_temperature():
+    """reference tests/test_regression.py:42-43 (rtol 1e-5)."""
+    out, st = _run(build_pamtra(), FREQUENCIES)
+    assert st == 0
+    np.testing.assert_allclose(out["tb"], GOLDEN["tb"], rtol=1e-5)
+    assert np.max(np.abs(out["tb"] - GOLDEN["tb"])) < 1e-4
+
+
+def test_golden_is_sensitive_to_the_profile():
+    """Negative control: the geometric-height standard atmosphere moves TB by
+    ~8 mK, which the golden comparison must reject."""
+    out, _ = _run(build_pamtra(geopotential=False), FREQUENCIES)
+    assert np.max(np.abs(out["tb"] - GOLDEN["tb"])) > 1e-3
+
+
+def test_tkc_known_answer():
+    """reference tests/test_fortran_functions.py:23-33."""
+    e = oracle.eps_water_tkc(20.0, 10.0)
+    np.testing.assert_allclose([e.real, e.imag], [60.6084, 32.9184], rtol=1e-3)
+
+
+def test_lobatto_nodes_are_roots_of_legendre_derivative():
+    mu, w = oracle.lobatto(16)
+    n = 32
+    # interior Lobatto nodes of the 2N-point rule: roots of P'_{n-1}
+    c = np.zeros(n)
+    c[-1] = 1.0
+    roots = np.sort(npleg.legroots(npleg.legder(c)))
+    pos = roots[roots > 0]
+    np.testing.assert_allclose(mu[:-1], pos, rtol=0, atol=1e-13)
+    assert mu[-1] == 1.0
+    pn1 = npleg.legval(mu, c)
+    np.testing.assert_allclose(w, 2.0 / (n * (n - 1) * pn1 ** 2), rtol=1e-11)
+    # the full rule integrates x^2 exactly over [-1, 1] (2/3)
+    assert abs(2 * np.sum(w * mu ** 2) - 2.0 / 3.0) < 1e-12
+
+
+@pytest.mark.parametrize("n", [5, 29, 45, 77, 121])
+def test_gauss_legendre_nodes(n):
+    x, w = oracle.gausquad(n)
+    xr, wr = npleg.leggauss(n)
+    np.testing.assert_allclose(x, xr, atol=1e-13)
+    np.testing.assert_allclose(w, wr, rtol=1e-9)
+
+
+def _bhmie(x, m, nstop):
+    """Independent Mie efficiencies (Bohren & Huffman 1983, appendix A),
+    full double precision, complex xi recursion.  The downward recursion
+    starts at nstop + 15 as in the reference (mie_scat_utilities.f90:46),
+    not at max(nstop, |mx|) + 15 as Bohren & Huffman recommend."""
+    mx = m * x
+    nmx = int(nstop + 15)
+    d = np.zeros(nmx + 1, dtype=complex)
+    for n in range(nmx, 0, -1):
+        d[n - 1] = n / mx - 1.0 / (d[n] + n / mx)
+    psi0, psi1 = np.cos(x), np.sin(x)
+    chi0, chi1 = -np.sin(x), np.cos(x)
+    xi1 = complex(psi1, -chi1)
+    qext = qsca = 0.0
+    for n in range(1, nstop + 1):
+        psi = (2 * n - 1) * psi1 / x - psi0
+        chi = (2 * n - 1) * chi1 / x - chi0
+        xi = complex(psi, -chi)
+        an = ((d[n] / m + n / x) * psi - psi1) / ((d[n] / m + n / x) * xi - xi1)
+        bn = ((m * d[n] + n / x) * psi - psi1) / ((m * d[n] + n / x) * xi - xi1)
+        qext += (2 * n + 1) * (an.real + bn.real)
+        qsca += (2 * n + 1) * (abs(an) ** 2 + abs(bn) ** 2)
+        psi0, psi1, chi0, chi1, xi1 = psi1, psi, chi1, chi, xi
+    return 2 * qext / x ** 2, 2 * qsca / x ** 2
+
+
+@pytest.mark.parametrize("x,m", [(0.05, 7.0 - 2.5j), (0.8, 5.9 - 2.9j), (3.5, 3.2 - 1.8j),
+                                 (12.0, 4.5 - 2.4j), (25.0, 1.78 - 0.003j), (40.0, 1.3 - 0.0j)])
+def test_mie_against_independent_series(x, m):
+    nt, qe, qs, qb = oracle.mie_single(x, m)
+    # the oracle's conj-convention index has a negative imaginary part
+    qe2, qs2 = _bhmie(x, m.conjugate(), nt)
+    # xi passes through single precision in the reference (miecalc)
+    np.testing.assert_allclose([qe, qs], [qe2, qs2], rtol=2e-6)
+
+
+def test_mie_real_index_conserves_energy():
+    nt, qe, qs, qb = oracle.mie_single(6.0, 1.33 + 0j)
+    assert abs(qe - qs) / qe < 1e-6
+
+
+def test_mie_rayleigh_limit():
+    x, m = 1e-3, 7.0 - 2.5j
+    nt, qe, qs, qb = oracle.mie_single(x, m)
+    K = (m ** 2 - 1) / (m ** 2 + 2)
+    np.testing.assert_allclose(qs, 8.0 / 3.0 * x ** 4 * abs(K) ** 2, rtol=1e-3)
+    np.testing.assert_allclose(qe, 4 * x * (-K.imag) + qs, rtol=1e-3)
+
+
+def _hg_coefs(nlegen, g):
+    l = np.arange(nlegen + 1)
+    c1 = (2 * l + 1) * g ** l
+    c2 = -0.2 * c1 * (l >= 2)
+    c3 = 0.9 * c1
+    return np.stack([c1, c2, c3])
+
+
+def test_scatcnv_normalisation_and_symmetry():
+    coef = _hg_coefs(24, 0.6)
+    kext, salb = 2e-3, 0.8
+    smat = oracle.scatcnv(24, coef, kext, salb)
+    mu, w = oracle.lobatto(16)
+    # quadrant symmetry: (-,-) = (+,+), (+,-) = (-,+)
+    np.testing.assert_array_equal(smat[3], smat[0])
+    np.testing.assert_array_equal(smat[2], smat[1])
+    # CHECK_NORM4: emission - extinction + 2 pi sum w (P++ + P-+) ~ 0
+    for j1 in range(16):
+        s = (1 - salb) * kext - kext + 2 * np.pi * np.sum(
+            w * (smat[0][0::2, 2 * j1] + smat[2][0::2, 2 * j1]))
+        assert abs(s) < 5e-4 * kext
+
+
+def _exact_layer(smat, kext, h):
+    """R and T of a homogeneous source-free layer from the exponential of
+    the two-stream-set generator (interaction principle)."""
+    mu, w = oracle.lobatto(16)
+    n = 32
+    muv = np.repeat(mu, 2)
+    wv = np.repeat(w, 2)
+    Minv = np.diag(1 / muv)
+    K = np.eye(n) * kext
+    A = Minv @ (K - 2 * np.pi * smat[0] * wv[None, :])
+    B = Minv @ (2 * np.pi * smat[1] * wv[None, :])
+    G = np.block([[-A, B], [-B, A]])
+    Phi = expm(h * G)
+    P21, P22 = Phi[n:, :n], Phi[n:, n:]
+    P22i = np.linalg.inv(P22)
+    return -P22i @ P21, P22i
+
+
+@pytest.mark.parametrize("tau,salb", [(0.05, 0.5), (0.4, 0.9), (1.2, 0.3)])
+def test_doubling_matches_matrix_exponential(tau, salb):
+    coef = _hg_coefs(16, 0.5)
+    kext = 1e-3
+    smat = oracle.scatcnv(16, coef, kext, salb)
+    h = tau / kext
+    st, R, T, sp, sm, nd = oracle.layer_doubling(smat, kext, salb, 0.0, h, 0.0, 0.0, 3000.0)
+    assert st == 0 and nd > 10
+    Rx, Tx = _exact_layer(smat, kext, h)
+    assert np.max(np.abs(R - Rx)) < 2e-5 * max(1.0, np.max(np.abs(Rx)))
+    assert np.max(np.abs(T - Tx)) < 2e-5
+
+
+def test_isothermal_layer_source_is_kirchhoff():
+    """An isothermal layer emits (I - R - T) B on the intensity components."""
+    coef = _hg_coefs(12, 0.4)
+    kext, salb, T0, wl = 2e-3, 0.6, 270.0, 3370.0
+    smat = oracle.scatcnv(12, coef, kext, salb)
+    st, R, T, sp, sm, nd = oracle.layer_doubling(smat, kext, salb, 1e-5, 400.0, T0, T0, wl)
+    B = oracle.planck(T0, wl)
+    b = np.zeros(32)
+    b[0::2] = B
+    expect = (np.eye(32) - R - T) @ b
+    # intensity components: exact up to the thin-layer start (radtran4.f90:
+    # 323-329, 1e-6 optical depth); the Q emission is zero in the reference
+    # (scatcnv.f90:90-98), so Q is not Kirchhoff-consistent and not checked
+    np.testing.assert_allclose(sp[0::2], expect[0::2], rtol=2e-4)
+    np.testing.assert_allclose(sm[0::2], expect[0::2], rtol=2e-4)
+
+
+def test_planck_inverse():
+    for t in (2.73, 150.0, 288.15, 320.0):
+        for wl in (923.0, 3370.0, 28150.0):
+            assert abs(oracle.rad2tb(oracle.planck(t, wl), wl) - t) < 1e-9 * t
+
+
+def test_isothermal_black_column():
+    """Isothermal atmosphere over a black surface at the same temperature:
+    without hydrometeors the upwelling TB equals that temperature at every
+    level and angle; with a scattering layer the layer reflects the 2.73 K
+    sky, so the top-of-atmosphere TB drops below it while the surface-level
+    upwelling (the black surface alone) stays exact."""
+    T0 = 275.0
+    for hq in (0.0, 2e-3):
+        pam = build_pamtra()
+        pam.p["temp_lev"][:] = T0
+        pam.p["groundtemp"] = np.full((1, 1), T0)
+        pam.nmlSet["emissivity"] = 1.0
+        pam.p["hydro_q"][:] = 0.0
+        pam.p["hydro_q"][0, 0, 1, 0] = hq
+        out, st = _run(pam, [35.5, 89.0])
+        assert st == 0
+        up = out["tb"][0, 0, :, :16, :, :]
+        if hq == 0.0:
+            np.testing.assert_allclose(up, T0, atol=1e-9)
+        else:
+            np.testing.assert_allclose(up[1], T0, atol=1e-9)
+            assert np.all(up[0] < T0) and np.all(up[0] > 100.0)
+
+
+def test_gas_absorption_dry_air_positive_and_monotone_in_pressure():
+    """Dry-air absorption grows with pressure (pressure-broadened O2 wings)
+    and water vapour only adds absorption.  H2O line strengths themselves are
+    parity unpinned (no reference fixture covers humid air)."""
+    a1 = oracle.gasabs(35.5, 280.0, 0.0, 50000.0)
+    a2 = oracle.gasabs(35.5, 280.0, 0.0, 100000.0)
+    a3 = oracle.gasabs(35.5, 280.0, 0.01, 100000.0)
+    assert 0 < a1 < a2 < a3
+    assert oracle.gasabs(900.0, 280.0, 0.0, 1e5) < 0  # invalid input is refused
