@@ -23,6 +23,14 @@ class PamLibraryError(RuntimeError):
 def library() -> C.CDLL:
     global _lib
     if _lib is None:
+        # One HIP runtime per process: torch bundles its own libamdhip64
+        # (soname libamdhip64.so.7) and loads it by a different file name, so
+        # it must be loaded first for this library's dependency to resolve to
+        # the same copy.  Importing torch does not initialise the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         path = _capi.LIBRARY_PATH
         if not os.path.exists(path):
             raise PamLibraryError(

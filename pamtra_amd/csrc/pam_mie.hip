@@ -93,8 +93,6 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
   const double T = w.temp[oy];
   const double freq = freqs[f];
   const double wavelength = C_LIGHT / (freq * 1e9);
-  const uint64_t *unused = nullptr;
-  (void)unused;
 
   for (int k = lane; k < NCOEF * (MAXNLEG + 1); k += MIE_WG) (&s_lc[0][0])[k] = 0.0;
   double kexttot = 0.0, salbedo = 0.0;
@@ -115,10 +113,8 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
     else
       m = csqrt_(eps_ice(T, freq));
     const cd m_ice = cd{m.re, -m.im};
-    const bool mix = (hd.liq_ice == -1) && (rho[0] != 917.0);
+    const bool is_ice = (hd.liq_ice == -1);
     // series length from the largest bin (mie_spheres.f90:139-158)
-    cd msph = mix ? eps_mix_air(m_ice, rho[nbin - 1]) : m_ice;
-    (void)msph;
     const double xmax = PI * dia[nbin - 1] / wavelength;
     const int nterms = mie_nstop(xmax);
     int nlegen = 2 * nterms;
@@ -173,7 +169,9 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
         double q3[3] = {0.0, 0.0, 0.0};
         int nmie = 0;
         if (nd[ir] > 0.0) {
-          cd msp = mix ? eps_mix_air(m_ice, rho[ir]) : m_ice;
+          // soft sphere: index of the ice-air mixture at the bin's density
+          // (mie_spheres.f90:187-192), decided per bin
+          cd msp = (is_ice && rho[ir] != 917.0) ? eps_mix_air(m_ice, rho[ir]) : m_ice;
           double x = PI * dia[ir] / wavelength;
           nmie = mie_nstop(x);
           if (nmie > nterms) {
