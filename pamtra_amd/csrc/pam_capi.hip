@@ -364,6 +364,10 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   s.threshold = st->hydro_threshold;
   s.softsphere_min = st->hydro_softsphere_min_density;
   s.emissivity = st->emissivity;
+  {
+    const char *inv = getenv("PAM_INVERSE");
+    s.inverse_gj = (inv && strcmp(inv, "gauss-jordan") == 0) ? 1 : 0;
+  }
 
   Profile p;
   p.nlyrs = pin->nlyrs;

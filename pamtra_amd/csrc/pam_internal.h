@@ -66,6 +66,7 @@ struct Dims {
 struct Settings {
   int lgas, lhyd_abs, lhyd_scat, lhyd_emis, incl_hydro_rho, limit_density, adaptive,
       conserve, outpol, precision;
+  int inverse_gj;  // 1: every (I - X)^-1 by Gauss-Jordan (PAM_INVERSE=gauss-jordan)
   double threshold, softsphere_min, emissivity;
 };
 

@@ -16,9 +16,10 @@
 //    output levels (radintg4.f90:119-172) and the Planck inversion to TB
 //    (collect_output.f90:31-88).
 //
-// All matrices live in LDS with a padded leading dimension; products are
-// 2x2 register blocks per thread, the inverse is Gauss-Jordan with partial
-// pivoting (the reference calls LAPACK DGETRF/DGETRI).
+// All matrices live in LDS with a padded leading dimension.  Products run on
+// the FP64 matrix cores (one 16x16 output tile per wave); the inverse is a
+// register-resident Gauss-Jordan with partial pivoting (the reference calls
+// LAPACK DGETRF/DGETRI).
 #include "pam_internal.h"
 
 namespace pam {
@@ -31,52 +32,63 @@ constexpr int MSZ = NN * LD;
 // LDS dense helpers; every thread of the workgroup calls them
 // ------------------------------------------------------------------
 
-// C = A * B
-__device__ __forceinline__ void mm(const double *A, const double *B, double *C) {
-  const int t = threadIdx.x, r0 = (t >> 4) * 2, c0 = (t & 15) * 2;
-  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
-#pragma unroll 8
-  for (int k = 0; k < NN; k++) {
-    double a0 = A[r0 * LD + k], a1 = A[(r0 + 1) * LD + k];
-    double b0 = B[k * LD + c0], b1 = B[k * LD + c0 + 1];
-    c00 += a0 * b0;
-    c01 += a0 * b1;
-    c10 += a1 * b0;
-    c11 += a1 * b1;
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+// NaN-propagating maximum (a NaN anywhere wins)
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = max_nan(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// C = D + A * B (D may be NULL) on FP64 matrix cores.  The four waves of the
+// workgroup each own one 16x16 tile of C and run 8 v_mfma_f64_16x16x4
+// steps over k.  Operand layout of the 16x16x4 f64 MFMA: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16] of the 16x4 / 4x16 blocks, and holds
+// D[4*r+l/16][l%16], r = 0..3, of the result (rows interleaved over the
+// four 16-lane groups).  Every lane has read all its operands before the
+// first write, so C may alias A, B or D.  With wmax (4 LDS doubles) the
+// return value is max |C_ij| (NaN if any entry is NaN), else 0.
+__device__ __forceinline__ double mm_core(const double *A, const double *B, const double *D,
+                                          double *C, double *wmax) {
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
+  const int li = l & 15, lk = l >> 4;
+  d4_t acc;
+#pragma unroll
+  for (int r = 0; r < 4; r++) acc[r] = D ? D[(I + 4 * r + lk) * LD + J + li] : 0.0;
+#pragma unroll
+  for (int s = 0; s < NN / 4; s++) {
+    const double a = A[(I + li) * LD + 4 * s + lk];
+    const double b = B[(4 * s + lk) * LD + J + li];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
   }
   __syncthreads();
-  C[r0 * LD + c0] = c00;
-  C[r0 * LD + c0 + 1] = c01;
-  C[(r0 + 1) * LD + c0] = c10;
-  C[(r0 + 1) * LD + c0 + 1] = c11;
+#pragma unroll
+  for (int r = 0; r < 4; r++) C[(I + 4 * r + lk) * LD + J + li] = acc[r];
+  if (wmax) {
+    double m = max_nan(max_nan(fabs(acc[0]), fabs(acc[1])), max_nan(fabs(acc[2]), fabs(acc[3])));
+    m = wave_max(m);
+    if (l == 0) wmax[wv] = m;
+  }
   __syncthreads();
+  return wmax ? max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.0;
+}
+
+// C = A * B
+__device__ __forceinline__ void mm(const double *A, const double *B, double *C) {
+  mm_core(A, B, nullptr, C, nullptr);
 }
 
 // C = D + A * B
 __device__ __forceinline__ void mm_add(const double *D, const double *A, const double *B,
                                        double *C) {
-  const int t = threadIdx.x, r0 = (t >> 4) * 2, c0 = (t & 15) * 2;
-  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
-#pragma unroll 8
-  for (int k = 0; k < NN; k++) {
-    double a0 = A[r0 * LD + k], a1 = A[(r0 + 1) * LD + k];
-    double b0 = B[k * LD + c0], b1 = B[k * LD + c0 + 1];
-    c00 += a0 * b0;
-    c01 += a0 * b1;
-    c10 += a1 * b0;
-    c11 += a1 * b1;
-  }
-  double d00 = D[r0 * LD + c0], d01 = D[r0 * LD + c0 + 1];
-  double d10 = D[(r0 + 1) * LD + c0], d11 = D[(r0 + 1) * LD + c0 + 1];
-  __syncthreads();
-  C[r0 * LD + c0] = d00 + c00;
-  C[r0 * LD + c0 + 1] = d01 + c01;
-  C[(r0 + 1) * LD + c0] = d10 + c10;
-  C[(r0 + 1) * LD + c0 + 1] = d11 + c11;
-  __syncthreads();
+  mm_core(A, B, D, C, nullptr);
 }
 
-// y = A x for a 32-vector, by the 32 threads of group g (no barrier)
+// row i of A x for a 32-vector (no barrier)
 __device__ __forceinline__ double mv_row(const double *A, const double *x, int i) {
   double s = 0.0;
 #pragma unroll 8
@@ -84,63 +96,137 @@ __device__ __forceinline__ double mv_row(const double *A, const double *x, int i
   return s;
 }
 
-// G = inv(I - X); Y is scratch.  Returns false if singular.
-__device__ bool inv_i_minus(const double *X, double *Y, double *G, double *s_fac, int *s_piv) {
-  const int t = threadIdx.x;
-  for (int e = t; e < NN2; e += RT_WG) {
-    int i = e / NN, j = e % NN;
-    Y[i * LD + j] = (i == j ? 1.0 : 0.0) - X[i * LD + j];
-    G[i * LD + j] = (i == j) ? 1.0 : 0.0;
+// LDS scratch of the inverse, and its mode (gj = 1: Gauss-Jordan only)
+struct InvLds {
+  int gj;
+  double wmax[4];
+  double red[4];
+  int redi[4];
+  double prow[2][2 * NN];
+};
+
+// G = inv(I - X) by Gauss-Jordan elimination with partial pivoting on the
+// augmented [I - X | I].  Rows stay in registers: thread t owns row t/8 and
+// the 8 augmented columns 8*(t%8)..+7.  Pivoting is implicit (no row swaps):
+// the pivot row of step k becomes row k of the inverse.  Two barriers per
+// pivot.  Returns false if a pivot is zero (singular matrix).  Kept out of
+// line: it is the rare fallback of inv_i_minus_prod, and inlined it would
+// raise the register count (and lower the occupancy) of its callers.
+__device__ __noinline__ bool inv_i_minus(const double *X, double *G, InvLds &sh) {
+  const int t = threadIdx.x, r = t >> 3, cg = t & 7, l = t & 63, wv = t >> 6;
+  double a[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int j = cg * 8 + c;
+    a[c] = (j < NN) ? ((r == j ? 1.0 : 0.0) - X[r * LD + j]) : ((r == j - NN) ? 1.0 : 0.0);
+  }
+  bool used = false, ok = true;
+  int my_step = -1;
+  for (int k = 0; k < NN; k++) {
+    const int kg = k >> 3, kc = k & 7;
+    double ck = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      if (c == kc) ck = a[c];
+    // entry of my row in column k (held by the lane of column group kg)
+    const double f = __shfl(ck, (l & ~7) | kg, 64);
+    // pivot: largest |entry| among rows not yet used, lowest row on ties
+    double v = (!used && cg == 0) ? fabs(f) : -1.0;
+    int idx = r;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double v2 = __shfl_xor(v, o, 64);
+      const int i2 = __shfl_xor(idx, o, 64);
+      if (v2 > v || (v2 == v && i2 < idx)) {
+        v = v2;
+        idx = i2;
+      }
+    }
+    if (l == 0) {
+      sh.red[wv] = v;
+      sh.redi[wv] = idx;
+    }
+    __syncthreads();
+    double bv = sh.red[0];
+    int p = sh.redi[0];
+#pragma unroll
+    for (int q = 1; q < 4; q++) {
+      const double v2 = sh.red[q];
+      const int i2 = sh.redi[q];
+      if (v2 > bv || (v2 == bv && i2 < p)) {
+        bv = v2;
+        p = i2;
+      }
+    }
+    if (!(bv > 0.0)) ok = false;
+    double *prow = sh.prow[k & 1];
+    if (r == p) {
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        a[c] = a[c] / f;
+        prow[cg * 8 + c] = a[c];
+      }
+      used = true;
+      my_step = k;
+    }
+    __syncthreads();
+    if (r != p) {
+#pragma unroll
+      for (int c = 0; c < 8; c++) a[c] = a[c] - f * prow[cg * 8 + c];
+    }
+  }
+  if (cg >= 4 && my_step >= 0) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) G[my_step * LD + (cg * 8 + c - NN)] = a[c];
   }
   __syncthreads();
-  bool ok = true;
-  for (int k = 0; k < NN; k++) {
-    // pivot: first row of maximal |Y[i][k]|, i >= k (idamax order)
-    if (t < 64) {
-      double v = (t >= k && t < NN) ? fabs(Y[t * LD + k]) : -1.0;
-      int idx = t;
-#pragma unroll
-      for (int o = 32; o >= 1; o >>= 1) {
-        double v2 = __shfl_xor(v, o, 64);
-        int i2 = __shfl_xor(idx, o, 64);
-        if (v2 > v || (v2 == v && i2 < idx)) {
-          v = v2;
-          idx = i2;
-        }
-      }
-      if (t == 0) s_piv[0] = idx;
-    }
-    __syncthreads();
-    const int p = s_piv[0];
-    if (p != k) {
-      if (t < 2 * NN) {
-        double *M = (t < NN) ? Y : G;
-        int j = t % NN;
-        double a = M[k * LD + j];
-        M[k * LD + j] = M[p * LD + j];
-        M[p * LD + j] = a;
-      }
-      __syncthreads();
-    }
-    const double piv = Y[k * LD + k];
-    if (piv == 0.0) ok = false;
-    if (t < NN) s_fac[t] = (t == k) ? 0.0 : Y[t * LD + k];
-    __syncthreads();
-    if (t < 2 * NN) {
-      double *M = (t < NN) ? Y : G;
-      int j = t % NN;
-      M[k * LD + j] = M[k * LD + j] / piv;
-    }
-    __syncthreads();
-    for (int e = t; e < 2 * NN2; e += RT_WG) {
-      int half = e / NN2, r = (e % NN2) / NN, j = e % NN;
-      if (r == k) continue;
-      double *M = half ? G : Y;
-      M[r * LD + j] = M[r * LD + j] - s_fac[r] * M[k * LD + j];
-    }
-    __syncthreads();
-  }
   return ok;
+}
+
+// Truncation threshold of the product-form series below: the omitted tail
+// X^(2^k) (I - X)^-1 has entries below 32 * 2^-56 ||G||, under the rounding
+// of the factors themselves.
+constexpr double NEU_TOL = 0x1p-56;
+constexpr int NEU_MAXIT = 20;
+
+// G = inv(I - A B) with X = A B formed in X (overwritten).  When the
+// spectral radius of X is below one, inv(I - X) = (I + X)(I + X^2)(I + X^4)...
+// and each factor costs two matrix-core products; the series stops as soon as
+// max |X^(2^k)| < NEU_TOL (one product for the thin layers at the start of
+// the doubling, where X is ~1e-12).  If it has not converged after NEU_MAXIT
+// factors (spectral radius within ~1e-6 of one) or X holds a NaN, the
+// register Gauss-Jordan inverts I - A B instead and reports a singular
+// matrix.
+__device__ bool inv_i_minus_prod(const double *A, const double *B, double *X, double *G,
+                                 InvLds &sh) {
+  const int t = threadIdx.x;
+  if (sh.gj) {
+    mm(A, B, X);
+    return inv_i_minus(X, G, sh);
+  }
+  double m = mm_core(A, B, nullptr, X, sh.wmax);
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int i = e / NN, j = e % NN;
+    G[i * LD + j] = (i == j ? 1.0 : 0.0) + X[i * LD + j];
+  }
+  __syncthreads();
+  bool conv = false;
+  for (int it = 0; it < NEU_MAXIT; it++) {
+    if (!(m < 0x1p64)) break;      // diverging, or a NaN: Gauss-Jordan decides
+    if (32.0 * m * m < NEU_TOL) {  // bound on max |X^2|
+      conv = true;
+      break;
+    }
+    m = mm_core(X, X, nullptr, X, sh.wmax);  // X <- X^2
+    if (m < NEU_TOL) {
+      conv = true;
+      break;
+    }
+    mm_add(G, G, X, G);  // G <- G (I + X)
+  }
+  if (conv) return true;
+  mm(A, B, X);
+  return inv_i_minus(X, G, sh);
 }
 
 // ------------------------------------------------------------------
@@ -149,13 +235,13 @@ __device__ bool inv_i_minus(const double *X, double *Y, double *G, double *s_fac
 
 __global__ void __launch_bounds__(RT_WG) k_doubling(Dims d, Settings s, const double *freqs,
                                                     Work w) {
-  __shared__ double sR[MSZ], sT[MSZ], sG[MSZ], sX[MSZ], sY[MSZ];
+  __shared__ double sR[MSZ], sT[MSZ], sG[MSZ], sX[MSZ];
   __shared__ double sc[NCOEF][MAXNLEG];
   __shared__ double sv[8][NN];  // S+, S-, C+, C-, chain scratch
-  __shared__ double sfac[NN];
   __shared__ double snorm[NMU];
-  __shared__ int spiv[1];
+  __shared__ InvLds sinv;
   __shared__ double smu[NMU], sqw[NMU];
+  double *sY = sG;  // quadrant (-,+) of the phase matrix until the init
 
   const int t = threadIdx.x;
   const int itf = blockIdx.x;
@@ -165,6 +251,7 @@ __global__ void __launch_bounds__(RT_WG) k_doubling(Dims d, Settings s, const do
     smu[t] = w.mu[t];
     sqw[t] = w.qw[t];
   }
+  if (t == 0) sinv.gj = s.inverse_gj;
   if (w.itf_status[itf] & 0xff) return;
   const int nleg = w.nleg[itf];
   double kext = w.kext[itf], salb = w.salb[itf];
@@ -291,8 +378,7 @@ __global__ void __launch_bounds__(RT_WG) k_doubling(Dims d, Settings s, const do
   __syncthreads();
   // ---- doubling (DOUBLING_INTEGRATION4, SYMMETRIC)
   for (int it = 0; it < nd; it++) {
-    mm(sR, sR, sX);
-    if (!inv_i_minus(sX, sY, sG, sfac, spiv)) {
+    if (!inv_i_minus_prod(sR, sR, sX, sG, sinv)) {
       if (t == 0) set_err(&w.itf_status[itf], E_SINGULAR);
       return;
     }
@@ -332,18 +418,13 @@ __global__ void __launch_bounds__(RT_WG) k_doubling(Dims d, Settings s, const do
       if (g < 4) sv[g == 0 ? 0 : (g == 1 ? 2 : (g == 2 ? 1 : 3))][i] = nv;
       __syncthreads();
     }
-    // R <- R + T (G (R T)); T <- T (G T)
-    mm(sR, sT, sX);
-    mm(sG, sX, sY);
-    mm_add(sR, sT, sY, sR);
-    mm(sG, sT, sX);
-    mm(sT, sX, sY);
-    for (int e = t; e < NN2; e += RT_WG) {
-      int i = e / NN, j = e % NN;
-      sT[i * LD + j] = sY[i * LD + j];
-    }
+    // R <- R + T G R T, T <- T G T.  G = inv(I - R R) commutes with R, so
+    // T G R T = (T R)(G T): four products instead of five.
+    mm(sT, sR, sX);          // X = T R
+    mm(sG, sT, sG);          // Y = G T (G is no longer needed)
+    mm(sT, sG, sT);          // T' = T Y
+    mm_add(sR, sX, sG, sR);  // R' = R + X Y
     linfac = 2.0 * linfac;
-    __syncthreads();
   }
   // ---- store R, T, S+, S-
   double *R = w.R + (size_t)itf * NN2, *T = w.T + (size_t)itf * NN2;
@@ -389,31 +470,28 @@ __device__ __forceinline__ void mv_stage(const double *A, double *v) {
 // with R2+ = R2- = R2 and T2+ = T2- = T2.  The minus side goes first, since
 // it reads the stack's old R+ and T+.
 __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, const double *S2p,
-                             const double *S2m, double *G, double *X, double *Y, double *vs,
-                             double *sfac, int *spiv) {
+                             const double *S2m, double *G, double *X, double *vs, InvLds &sinv) {
   const int t = threadIdx.x;
-  mm(R2, s1.Rp, X);
-  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (!inv_i_minus_prod(R2, s1.Rp, X, G, sinv)) return false;
   if (t < NN) vs[t] = S2m[t] + mv_row(R2, s1.Sp, t);  // S2m + R2m S1p
   __syncthreads();
   mv_stage(G, vs);
   mv_stage(s1.Tm, vs);
   if (t < NN) s1.Sm[t] = s1.Sm[t] + vs[t];            // S1m + T1m G (...)
   mm(R2, s1.Tp, X);
-  mm(G, X, Y);
-  mm_add(s1.Rm, s1.Tm, Y, s1.Rm);                     // R1m + T1m G R2m T1p
+  mm(G, X, X);
+  mm_add(s1.Rm, s1.Tm, X, s1.Rm);                     // R1m + T1m G R2m T1p
   mm(G, T2, X);
   mm(s1.Tm, X, s1.Tm);                                // T1m G T2m
-  mm(s1.Rp, R2, X);
-  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (!inv_i_minus_prod(s1.Rp, R2, X, G, sinv)) return false;
   if (t < NN) vs[t] = s1.Sp[t] + mv_row(s1.Rp, S2m, t);  // S1p + R1p S2m
   __syncthreads();
   mv_stage(G, vs);
   mv_stage(T2, vs);
   if (t < NN) s1.Sp[t] = S2p[t] + vs[t];              // S2p + T2p G (...)
   mm(s1.Rp, T2, X);
-  mm(G, X, Y);
-  mm_add(R2, T2, Y, s1.Rp);                           // R2p + T2p G R1p T2m
+  mm(G, X, X);
+  mm_add(R2, T2, X, s1.Rp);                           // R2p + T2p G R1p T2m
   mm(G, s1.Tp, X);
   mm(T2, X, s1.Tp);                                   // T2p G T1p
   __syncthreads();
@@ -446,22 +524,20 @@ __device__ void combine_clear(Stack &s1, const double *dg, const double *S2p, co
 
 // The combine with the surface "layer" of get_sfc_matrices
 // (sfc_matrices.f90:21-194): R2+ = 0, R2- = Rs, T2 = I, S2 = 0.
-__device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *X, double *Y,
-                                double *vs, double *sfac, int *spiv) {
+__device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *X, double *vs,
+                                InvLds &sinv) {
   const int t = threadIdx.x;
-  mm(Rs, s1.Rp, X);
-  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (!inv_i_minus_prod(Rs, s1.Rp, X, G, sinv)) return false;
   if (t < NN) vs[t] = mv_row(Rs, s1.Sp, t);
   __syncthreads();
   mv_stage(G, vs);
   mv_stage(s1.Tm, vs);
   if (t < NN) s1.Sm[t] = s1.Sm[t] + vs[t];
   mm(Rs, s1.Tp, X);
-  mm(G, X, Y);
-  mm_add(s1.Rm, s1.Tm, Y, s1.Rm);
+  mm(G, X, X);
+  mm_add(s1.Rm, s1.Tm, X, s1.Rm);
   mm(s1.Tm, G, s1.Tm);
-  mm(s1.Rp, Rs, X);
-  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (!inv_i_minus_prod(s1.Rp, Rs, X, G, sinv)) return false;
   if (t < NN) vs[t] = s1.Sp[t];
   __syncthreads();
   mv_stage(G, vs);
@@ -522,12 +598,10 @@ __device__ bool load_layer(const Dims &d, const Work &w, int col, int f, int z, 
 // INTERNAL_RADIANCE4 at the surface level: up = stack of all layers, down =
 // the surface (R- = Rs, T- = I, S- = 0)
 __device__ bool radiance_at_surface(const Stack &up, const double *Rs, const double *sky,
-                                    const double *gnd, double *G, double *X, double *Y,
-                                    double *vs, double *ur, double *dr, double *sfac,
-                                    int *spiv) {
+                                    const double *gnd, double *G, double *X, double *vs,
+                                    double *ur, double *dr, InvLds &sinv) {
   const int t = threadIdx.x;
-  mm(up.Rp, Rs, X);
-  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (!inv_i_minus_prod(up.Rp, Rs, X, G, sinv)) return false;
   double s_ = 0.0;
   if (t < NN) {
     s_ = mv_row(up.Rp, gnd, t);          // UPR (DNT INBOT)
@@ -540,8 +614,7 @@ __device__ bool radiance_at_surface(const Stack &up, const double *Rs, const dou
   __syncthreads();
   if (t < NN) dr[t] = mv_row(G, vs, t);
   __syncthreads();
-  mm(Rs, up.Rp, X);
-  if (!inv_i_minus(X, Y, G, sfac, spiv)) return false;
+  if (!inv_i_minus_prod(Rs, up.Rp, X, G, sinv)) return false;
   if (t < NN) vs[t] = mv_row(up.Tp, sky, t);
   __syncthreads();
   if (t < NN) {
@@ -605,10 +678,9 @@ __device__ void write_tb(const Dims &d, const Settings &s, int col, int f, int i
 
 __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const double *freqs,
                                                   const char *sfc_refl, Work w, double *tb) {
-  __shared__ double M[9][MSZ];
+  __shared__ double M[8][MSZ];
   __shared__ double sv[10][NN];
-  __shared__ double sfac[NN];
-  __shared__ int spiv[1];
+  __shared__ InvLds sinv;
   __shared__ double smu[NMU], sqw[NMU];
   __shared__ int sbad[1];
 
@@ -620,7 +692,10 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
     smu[t] = w.mu[t];
     sqw[t] = w.qw[t];
   }
-  if (t == 0) sbad[0] = 0;
+  if (t == 0) {
+    sbad[0] = 0;
+    sinv.gj = s.inverse_gj;
+  }
   int32_t *cst = w.cf_status + (size_t)col * d.nf + f;
   const int L = w.nl[col];
   const int nout = d.nout;
@@ -662,7 +737,7 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
     }
   }
   Stack stk = {M[0], M[1], M[2], M[3], sv[0], sv[1]};
-  double *R2 = M[4], *T2 = M[5], *G = M[6], *X = M[7], *Y = M[8];
+  double *R2 = M[4], *T2 = M[5], *G = M[6], *X = M[7];
   double *S2p = sv[2], *S2m = sv[3], *dg = sv[4], *vs = sv[5];
   double *sky = sv[6], *gnd = sv[7], *ur = sv[8], *dr = sv[9];
   const double psky = planck(SKY_TEMP, wl);
@@ -703,7 +778,7 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
       }
       __syncthreads();
     } else if (sc) {
-      if (!combine_scat(stk, R2, T2, S2p, S2m, G, X, Y, vs, sfac, spiv)) {
+      if (!combine_scat(stk, R2, T2, S2p, S2m, G, X, vs, sinv)) {
         fail(E_SINGULAR | PAM_FATAL);
         return;
       }
@@ -733,7 +808,7 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
   }
   __syncthreads();
   if (lev_sfc >= 0) {
-    if (!radiance_at_surface(stk, R2, sky, gnd, G, X, Y, vs, ur, dr, sfac, spiv)) {
+    if (!radiance_at_surface(stk, R2, sky, gnd, G, X, vs, ur, dr, sinv)) {
       fail(E_SINGULAR | PAM_FATAL);
       return;
     }
@@ -741,7 +816,7 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
     __syncthreads();
   }
   if (lev_toa >= 0) {
-    if (!combine_surface(stk, R2, G, X, Y, vs, sfac, spiv)) {
+    if (!combine_surface(stk, R2, G, X, vs, sinv)) {
       fail(E_SINGULAR | PAM_FATAL);
       return;
     }

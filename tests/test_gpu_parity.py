@@ -104,6 +104,38 @@ def test_adaptive_grid_and_soft_spheres_vs_oracle():
     _compare(g, c)
 
 
+def test_gauss_jordan_and_series_inverses_agree(monkeypatch):
+    """Every (I - X)^-1 of the doubling and adding is a product-form series
+    on the matrix cores, with Gauss-Jordan as its fallback.  Forcing
+    Gauss-Jordan everywhere (PAM_INVERSE=gauss-jordan) must give the same
+    TB to rounding (the FP64 noise floor of the whole chain, doubling from
+    1e-6 optical depth, is ~1e-7 K), and both must match the oracle."""
+    pam, freqs = workloads.config2(ncol=16, seed=21)
+    a, st = _gpu(pam, freqs)
+    monkeypatch.setenv("PAM_INVERSE", "gauss-jordan")
+    b, st2 = _gpu(pam, freqs)
+    monkeypatch.delenv("PAM_INVERSE")
+    assert st == 0 and st2 == 0
+    d = np.max(np.abs(a["tb"] - b["tb"]))
+    print(f"series vs Gauss-Jordan max |dTB| = {d:.3e} K", flush=True)
+    assert d <= 1e-6
+    c, _ = _cpu(pam, freqs)
+    _compare(b, c)
+
+
+def test_optically_thick_scattering_vs_oracle():
+    """Heavy rain and cloud (20x the contents) at 89-183 GHz: thick,
+    strongly scattering layers, where the inverse series needs the most
+    factors and the stack reflects most."""
+    pam, freqs = workloads.config1(ncol=8, seed=33)
+    pam.p["hydro_q"] *= 20.0
+    freqs = [89.0, 150.0, 183.31]
+    g, st = _gpu(pam, freqs)
+    c, st2 = _cpu(pam, freqs)
+    assert st == 0 and st2 == 0, (g["msg"], c["msg"])
+    _compare(g, c)
+
+
 def test_bad_column_is_isolated():
     """A NaN temperature fails its own column with PAM_FATAL, the others are
     computed as if alone (status per column, no abort)."""
