@@ -56,6 +56,10 @@ struct DevBuf {
 struct pam_context {
   int device = 0;
   DevBuf colbuf, itembuf, iobuf, smallbuf;
+  DevBuf atab;  // Mie angle tables + their offsets (built once)
+  const double *d_atab = nullptr;
+  const int64_t *d_atab_off = nullptr;
+  int atab_nmax = 0;
   double *d_mu = nullptr, *d_qw = nullptr;
   double mu[NMU], qw[NMU];
   double stage_ms[NSTAGE] = {0, 0, 0, 0, 0};
@@ -200,6 +204,33 @@ extern "C" int pam_context_create(pam_context **out, int device) {
     delete c;
     return PAM_FATAL;
   }
+  {
+    // Mie angle tables for every supported series length
+    int64_t off[MAX_NTERMS + 1];
+    int nmax = 0;
+    const int64_t n = angle_table_layout(off, &nmax);
+    const size_t tab_bytes = ((size_t)n * 8 + 255) & ~(size_t)255;
+    bool ok = c->atab.ensure(tab_bytes + sizeof(off)) == hipSuccess;
+    if (ok) {
+      double *tab = (double *)c->atab.p;
+      int64_t *doff = (int64_t *)((char *)c->atab.p + tab_bytes);
+      ok = hipMemcpy(doff, off, sizeof(int64_t) * (nmax + 1), hipMemcpyHostToDevice) ==
+           hipSuccess;
+      if (ok) {
+        build_angle_tables(tab, doff, nmax, nullptr);
+        ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+      }
+      c->d_atab = tab;
+      c->d_atab_off = doff;
+      c->atab_nmax = nmax;
+    }
+    if (!ok) {
+      c->atab.release();
+      (void)hipFree(c->d_mu);
+      delete c;
+      return PAM_FATAL;
+    }
+  }
   double gb = 32.0;
   if (const char *e = getenv("PAM_WORKSPACE_GB")) gb = atof(e);
   c->budget_bytes = (size_t)(gb * (1ull << 30));
@@ -214,6 +245,7 @@ extern "C" int pam_context_destroy(pam_context *c) {
   c->itembuf.release();
   c->iobuf.release();
   c->smallbuf.release();
+  c->atab.release();
   if (c->d_mu) (void)hipFree(c->d_mu);
   delete c;
   return PAM_SUCCESS;
@@ -453,6 +485,9 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   }
   w.mu = c->d_mu;
   w.qw = c->d_qw;
+  w.atab = c->d_atab;
+  w.atab_off = c->d_atab_off;
+  w.atab_nmax = c->atab_nmax;
   w.counters = d_cnt;
 
   std::vector<hipEvent_t> evs;

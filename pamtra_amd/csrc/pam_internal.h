@@ -117,9 +117,10 @@ struct Work {
   int32_t *itf_status;                 // [nitems*nf]
   double *R, *T;                       // [nitems*nf][NN2]
   double *Sp, *Sm;                     // [nitems*nf][NN]
-  // phase-matrix weight tables: offsets per NUMPTS index, see pam_rt.hip
-  const double *wtab;
-  const int64_t *wtab_off;             // [NUM_NP]
+  // Mie angle tables per series length N (see build_angle_tables)
+  const double *atab;
+  const int64_t *atab_off;             // [atab_nmax + 1]
+  int atab_nmax;
   const double *mu, *qw;               // Lobatto nodes/weights [NMU]
   // work counters (atomics)
   unsigned long long *counters;        // [9], see pam_last_work
@@ -197,6 +198,12 @@ void launch_columns(const Dims &d, const Settings &s, const Profile &p, const pa
 void launch_items(const Dims &d, Work &w, hipStream_t st);
 void launch_dsd(const Dims &d, const Settings &s, const Profile &p, const pam_hydro_desc *hyd,
                 const Hydro4D &h4, Work &w, hipStream_t st);
+// Mie angle tables: per series length N = 1..nmax the Gauss-Legendre nodes
+// and weights of the phase integral, pi_n/tau_n at the nodes, and the
+// Legendre polynomials at the nodes.  angle_table_layout fills off[0..nmax]
+// (doubles) and returns the total size; build_angle_tables fills the table.
+int64_t angle_table_layout(int64_t *off, int *nmax);
+void build_angle_tables(double *tab, const int64_t *d_off, int nmax, hipStream_t st);
 void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, const Hydro4D &h4,
                 const Profile &p, const double *freqs, Work &w, hipStream_t st);
 void launch_surface(const Dims &d, const Settings &s, const Profile &p, const double *freqs,

@@ -5,24 +5,31 @@
 // calc_scatProperties does (scatProperties.f90:920-946) and normalised as in
 // finalize_rt3_scatProperties (:993-1002).
 //
-// One 64-lane wavefront per (scattering layer, channel).  Lanes run the
-// per-bin Mie recurrences in parallel (bins in chunks, coefficients staged in
-// LDS), then own quadrature angles for the DSD-weighted phase-function sums,
-// which stay in registers; the Legendre projection is a wavefront reduction
-// per order.  Bin sums follow the reference's bin order.
+// One 256-thread workgroup per (scattering layer, channel), three phases per
+// hydrometeor:
+//   A. threads over size bins: the a_n, b_n recurrences (miecalc) and the
+//      efficiencies (miecross); coefficients staged in LDS, bins in chunks;
+//   B. threads over (quadrature angle, bin group): S1/S2 and the DSD-weighted
+//      phase-function sums (mieangle), pi_n/tau_n read from the angle table;
+//   C. threads over Legendre orders: the projection sum over the angles in
+//      ascending order, as the reference sums it.
+// The Gauss-Legendre nodes, pi_n(mu_k), tau_n(mu_k) and P_l(mu_k) depend
+// only on the series length N (nquad and nlegen are functions of N), so they
+// are tabulated once per context (build_angle_tables) with the recurrences
+// the reference evaluates per bin.
 #include "pam_internal.h"
 
 namespace pam {
 
-constexpr int MIE_WG = 64;
+constexpr int MIE_WG = 256;
 constexpr int AB_CAP = 1024;  // (a_n, b_n) pairs staged in LDS per chunk
-constexpr int MAXK = MAX_NQUAD / MIE_WG;
+static_assert(MAX_NQUAD <= MIE_WG && MAXNLEG <= MIE_WG, "one thread per angle / order");
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// nlegen and nquad of a series of length N (mie_spheres.f90:139-158)
+__host__ __device__ __forceinline__ int mie_nlegen(int n) {
+  return (2 * n > MAXNLEG - 1) ? MAXNLEG - 1 : 2 * n;
 }
+__host__ __device__ __forceinline__ int mie_nquad(int n) { return (mie_nlegen(n) + 2 * n + 2) / 2; }
 
 // TKC liquid water permittivity (eps_water.f90:653-757), T in degC, f GHz;
 // rounded to single precision like the reference's default-kind cmplx()
@@ -75,16 +82,92 @@ __device__ __forceinline__ int mie_nstop(double x) {
   return (int)(x + 4.0 * pow(x, sp(0.3334)) + 2.0);
 }
 
+// Angle table of series length N at tab + off[N]:
+//   mu[Q], wt[Q]          Gauss-Legendre nodes (ascending) and weights
+//   pi[N][Q], tau[N][Q]   pi_n(mu_k), tau_n(mu_k), n = 1..N
+//   P[Q][nlegen+1]        P_l(mu_k), l = 0..nlegen
+int64_t angle_table_layout(int64_t *off, int *nmax) {
+  int64_t o = 0;
+  int n = 1;
+  off[0] = 0;
+  for (; n <= MAX_NTERMS; n++) {
+    const int q = mie_nquad(n), nl = mie_nlegen(n);
+    if (q > MAX_NQUAD) break;
+    off[n] = o;
+    o += (int64_t)q * (2 + 2 * n + nl + 1);
+  }
+  *nmax = n - 1;
+  return o;
+}
+
+__global__ void __launch_bounds__(MIE_WG) k_angle_tables(double *tab, const int64_t *off) {
+  const int n = blockIdx.x + 1, t = threadIdx.x;
+  const int q = mie_nquad(n), nl = mie_nlegen(n);
+  double *mu = tab + off[n], *wt = mu + q, *pit = wt + q, *taut = pit + (size_t)n * q,
+         *pt = taut + (size_t)n * q;
+  // gausquad (scat_utilities.f90:1-44)
+  for (int j = t + 1; j <= (q + 1) / 2; j += MIE_WG) {
+    double x = cos(PI * (j - 0.25) / (q + 0.5));
+    double pl, pl1, pl2, dpl, xp;
+    int it = 0;
+    do {
+      pl1 = 1.0;
+      pl = x;
+      for (int l = 2; l <= q; l++) {
+        pl2 = pl1;
+        pl1 = pl;
+        pl = ((double)(2 * l - 1) * x * pl1 - (double)(l - 1) * pl2) / (double)l;
+      }
+      dpl = (double)q * (x * pl - pl1) / (x * x - 1.0);
+      xp = x;
+      x = xp - pl / dpl;
+      it++;
+    } while (fabs(x - xp) > 3.0e-13 && it < 10);
+    const double w = 2.0 / ((1.0 - x * x) * dpl * dpl);
+    mu[j - 1] = -x;
+    wt[j - 1] = w;
+    mu[q - j] = x;
+    wt[q - j] = w;
+  }
+  __syncthreads();
+  if (t < q) {
+    const double m = mu[t];
+    // pi_n, tau_n (mie_scat_utilities.f90:128-160)
+    double pin = 1.0, pim = 0.0;
+    for (int k = 1; k <= n; k++) {
+      const double taun = (double)k * m * pin - (double)(k + 1) * pim;
+      pit[(size_t)(k - 1) * q + t] = pin;
+      taut[(size_t)(k - 1) * q + t] = taun;
+      const double tmp = pin;
+      pin = ((double)(2 * k + 1) * m * pin - (double)(k + 1) * pim) / (double)k;
+      pim = tmp;
+    }
+    // Legendre polynomials of the projection (mie_spheres.f90:272-314)
+    double pl = 1.0, pl1 = 1.0, pl2 = 0.0;
+    for (int l = 0; l <= nl; l++) {
+      if (l > 0) pl = (double)(2 * l - 1) * m * pl1 / (double)l - (double)(l - 1) * pl2 / (double)l;
+      pt[(size_t)t * (nl + 1) + l] = pl;
+      pl2 = pl1;
+      pl1 = pl;
+    }
+  }
+}
+
+void build_angle_tables(double *tab, const int64_t *d_off, int nmax, hipStream_t st) {
+  hipLaunchKernelGGL(k_angle_tables, dim3((unsigned)nmax), dim3(MIE_WG), 0, st, tab, d_off);
+}
+
 __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hydro_desc *hyd,
                                                 const double *freqs, Work w) {
-  __shared__ double s_mu[MAX_NQUAD], s_wt[MAX_NQUAD];
   __shared__ cd s_a[AB_CAP], s_b[AB_CAP];
-  __shared__ double s_q[3][MIE_WG];
-  __shared__ int s_nm[MIE_WG];
-  __shared__ double s_leg[NCOEF][MAXNLEG];
+  __shared__ double s_q[3][MIE_WG];     // per-bin efficiencies; later the weighted sums
+  __shared__ double s_part[3][MIE_WG];  // phase sums per (bin group, angle)
   __shared__ double s_lc[NCOEF][MAXNLEG + 1];
+  __shared__ int s_nm[MIE_WG];
+  __shared__ double s_sum[3];
+  __shared__ int s_nlh, s_nan;
 
-  const int lane = threadIdx.x;
+  const int t = threadIdx.x;
   const int itf = blockIdx.x;
   const int li = itf / d.nf, f = itf % d.nf;
   const int col = w.item_col[li], z = w.item_layer[li];
@@ -94,7 +177,7 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
   const double freq = freqs[f];
   const double wavelength = C_LIGHT / (freq * 1e9);
 
-  for (int k = lane; k < NCOEF * (MAXNLEG + 1); k += MIE_WG) (&s_lc[0][0])[k] = 0.0;
+  for (int k = t; k < NCOEF * (MAXNLEG + 1); k += MIE_WG) (&s_lc[0][0])[k] = 0.0;
   double kexttot = 0.0, salbedo = 0.0;
   int nleg_coef = 0;
   long long work_bins = 0, work_series = 0;
@@ -117,55 +200,27 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
     // series length from the largest bin (mie_spheres.f90:139-158)
     const double xmax = PI * dia[nbin - 1] / wavelength;
     const int nterms = mie_nstop(xmax);
-    int nlegen = 2 * nterms;
-    if (nlegen > MAXNLEG - 1) nlegen = MAXNLEG - 1;
-    const int nquad = (nlegen + 2 * nterms + 2) / 2;
-    if (nterms > MAX_NTERMS || nquad > MAX_NQUAD) {
-      if (lane == 0) set_err(st, E_MIE_TERMS);
+    if (nterms < 1 || nterms > w.atab_nmax) {
+      if (t == 0) set_err(st, E_MIE_TERMS);
       return;
     }
-    // Gauss-Legendre nodes (scat_utilities.f90:1-44), ascending
-    for (int j = lane + 1; j <= (nquad + 1) / 2; j += MIE_WG) {
-      double x = cos(PI * (j - 0.25) / (nquad + 0.5));
-      double pl, pl1, pl2, dpl, xp;
-      int it = 0;
-      do {
-        pl1 = 1.0;
-        pl = x;
-        for (int l = 2; l <= nquad; l++) {
-          pl2 = pl1;
-          pl1 = pl;
-          pl = ((double)(2 * l - 1) * x * pl1 - (double)(l - 1) * pl2) / (double)l;
-        }
-        dpl = (double)nquad * (x * pl - pl1) / (x * x - 1.0);
-        xp = x;
-        x = xp - pl / dpl;
-        it++;
-      } while (fabs(x - xp) > 3.0e-13 && it < 10);
-      double wt = 2.0 / ((1.0 - x * x) * dpl * dpl);
-      s_mu[j - 1] = -x;
-      s_wt[j - 1] = wt;
-      s_mu[nquad - j] = x;
-      s_wt[nquad - j] = wt;
-    }
-    __syncthreads();
-    double mu_k[MAXK];
-#pragma unroll
-    for (int k = 0; k < MAXK; k++) {
-      int i = lane + MIE_WG * k;
-      mu_k[k] = (i < nquad) ? s_mu[i] : 0.0;
-    }
-    double sp1[MAXK], sp2[MAXK], sp3[MAXK];
-#pragma unroll
-    for (int k = 0; k < MAXK; k++) sp1[k] = sp2[k] = sp3[k] = 0.0;
+    const int nlegen = mie_nlegen(nterms);
+    const int nquad = mie_nquad(nterms);
+    const double *tmu = w.atab + w.atab_off[nterms], *twt = tmu + nquad, *tpi = twt + nquad,
+                 *ttau = tpi + (size_t)nterms * nquad, *tpl = ttau + (size_t)nterms * nquad;
+    // phase B layout: thread = (bin group g, angle a)
+    const int ngrp = MIE_WG / nquad;
+    const int g = t / nquad, a = t % nquad;
+    const bool angle_thread = g < ngrp;
+    double acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
     double sumqe = 0.0, sumqs = 0.0, sumqb = 0.0;
 
     const int chunk = min(MIE_WG, AB_CAP / nterms);
     for (int b0 = 0; b0 < nbin; b0 += chunk) {
       const int nb = min(chunk, nbin - b0);
-      // --- lanes over bins: a_n, b_n by miecalc, efficiencies by miecross
-      if (lane < nb) {
-        const int ir = b0 + lane;
+      // --- A: threads over bins (miecalc, miecross)
+      if (t < nb) {
+        const int ir = b0 + t;
         double q3[3] = {0.0, 0.0, 0.0};
         int nmie = 0;
         if (nd[ir] > 0.0) {
@@ -180,7 +235,7 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
           }
           cd mc = conj(msp);
           cd y = mc * x;
-          cd *A = s_a + lane * nterms, *B = s_b + lane * nterms;
+          cd *A = s_a + t * nterms, *B = s_b + t * nterms;
           // downward log-derivative recursion, D_n kept in the a-slot
           int nn = nmie + 15;
           cd dn = cd{0.0, 0.0};
@@ -222,130 +277,126 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
           q3[1] = qs * ar * nd[ir] * dd[ir];
           q3[2] = qb * ar * nd[ir] * dd[ir];
         }
-        s_q[0][lane] = q3[0];
-        s_q[1][lane] = q3[1];
-        s_q[2][lane] = q3[2];
-        s_nm[lane] = nmie;
+        s_q[0][t] = q3[0];
+        s_q[1][t] = q3[1];
+        s_q[2][t] = q3[2];
+        s_nm[t] = nmie;
       }
       __syncthreads();
-      // --- lanes over angles: DSD-weighted phase-function sums
-      for (int cb = 0; cb < nb; cb++) {
-        const int nmie = s_nm[cb];
-        if (nmie <= 0) continue;
-        const int ir = b0 + cb;
-        const double wgt = nd[ir] * dd[ir];
-        const cd *A = s_a + cb * nterms, *B = s_b + cb * nterms;
-#pragma unroll
-        for (int k = 0; k < MAXK; k++) {
-          if (lane + MIE_WG * k >= nquad) break;
-          const double mu = mu_k[k];
+      // --- B: threads over (bin group, angle): S1, S2 (mieangle) and the
+      // DSD-weighted phase-function sums
+      if (angle_thread) {
+        for (int cb = g; cb < nb; cb += ngrp) {
+          const int nmie = s_nm[cb];
+          if (nmie <= 0) continue;
+          const int ir = b0 + cb;
+          const double wgt = nd[ir] * dd[ir];
+          const cd *A = s_a + cb * nterms, *B = s_b + cb * nterms;
           cd S1 = cd{0.0, 0.0}, S2 = cd{0.0, 0.0};
-          double pin = 1.0, pim = 0.0;
           for (int n = 1; n <= nmie; n++) {
-            double taun = (double)n * mu * pin - (double)(n + 1) * pim;
-            double c = (double)((float)(2 * n + 1) / (float)(n * (n + 1)));
-            cd an = A[n - 1], bn = B[n - 1];
+            const double pin = tpi[(size_t)(n - 1) * nquad + a];
+            const double taun = ttau[(size_t)(n - 1) * nquad + a];
+            const double c = (double)((float)(2 * n + 1) / (float)(n * (n + 1)));
+            const cd an = A[n - 1], bn = B[n - 1];
             S1 = S1 + c * (an * pin + bn * taun);
             S2 = S2 + c * (bn * pin + an * taun);
-            double tmp = pin;
-            pin = ((double)(2 * n + 1) * mu * pin - (double)(n + 1) * pim) / (double)n;
-            pim = tmp;
           }
-          double a1 = cabs_(S1), a2 = cabs_(S2);
-          double p1 = 0.5 * (a2 * a2 + a1 * a1);
-          double p2 = 0.5 * (a2 * a2 - a1 * a1);
-          double p3 = (conj(S2) * S1).re;
-          sp1[k] = sp1[k] + p1 * wgt;
-          sp2[k] = sp2[k] + p2 * wgt;
-          sp3[k] = sp3[k] + p3 * wgt;
+          const double a1 = cabs_(S1), a2 = cabs_(S2);
+          const double p1 = 0.5 * (a2 * a2 + a1 * a1);
+          const double p2 = 0.5 * (a2 * a2 - a1 * a1);
+          const double p3 = (conj(S2) * S1).re;
+          acc1 = acc1 + p1 * wgt;
+          acc2 = acc2 + p2 * wgt;
+          acc3 = acc3 + p3 * wgt;
         }
-        if (lane == 0) {
+      }
+      if (t == 0) {  // efficiency sums in bin order
+        for (int cb = 0; cb < nb; cb++) {
+          if (s_nm[cb] <= 0) continue;
           sumqe = sumqe + s_q[0][cb];
           sumqs = sumqs + s_q[1][cb];
           sumqb = sumqb + s_q[2][cb];
           work_bins++;
-          work_series += nmie;
+          work_series += s_nm[cb];
         }
       }
       __syncthreads();
     }
-    // broadcast the ordered bin sums of lane 0
-    sumqe = __shfl(sumqe, 0, 64);
-    sumqs = __shfl(sumqs, 0, 64);
-    sumqb = __shfl(sumqb, 0, 64);
+    s_part[0][t] = acc1;
+    s_part[1][t] = acc2;
+    s_part[2][t] = acc3;
+    if (t == 0) {
+      s_sum[0] = sumqe;
+      s_sum[1] = sumqs;
+      s_sum[2] = sumqb;
+      s_nlh = 0;
+      s_nan = 0;
+    }
+    __syncthreads();
+    sumqe = s_sum[0];
+    sumqs = s_sum[1];
+    sumqb = s_sum[2];
     const double extinction = s.lhyd_abs ? sumqe : sumqe - sumqs;
     const double scatter = sumqs;
     const double albedo = scatter / extinction;
     if (!(extinction > 0.0) || !(scatter > 0.0) || !(sumqb > 0.0)) {
-      if (lane == 0) set_err(st, E_MIE_RESULT);
+      if (t == 0) set_err(st, E_MIE_RESULT);
       return;
     }
-    // Legendre projection (mie_spheres.f90:272-314)
+    // weighted phase sums per angle (groups combined in group order)
     const double tmp = wavelength * wavelength / (PI * scatter);
-    double pl[MAXK], pl1[MAXK], pl2[MAXK];
-#pragma unroll
-    for (int k = 0; k < MAXK; k++) {
-      int i = lane + MIE_WG * k;
-      double wt = (i < nquad) ? s_wt[i] : 0.0;
-      sp1[k] = tmp * sp1[k] * wt;
-      sp2[k] = tmp * sp2[k] * wt;
-      sp3[k] = tmp * sp3[k] * wt;
-      pl[k] = 1.0;
-      pl1[k] = 1.0;
-      pl2[k] = 0.0;
-    }
-    int nl_h = 0;
-    for (int l = 0; l <= nlegen; l++) {
-      double c1 = 0.0, c2 = 0.0, c3 = 0.0;
-#pragma unroll
-      for (int k = 0; k < MAXK; k++) {
-        if (l > 0)
-          pl[k] = (double)(2 * l - 1) * mu_k[k] * pl1[k] / (double)l -
-                  (double)(l - 1) * pl2[k] / (double)l;
-        c1 += sp1[k] * pl[k];
-        c2 += sp2[k] * pl[k];
-        c3 += sp3[k] * pl[k];
-        pl2[k] = pl1[k];
-        pl1[k] = pl[k];
+    if (t < nquad) {
+      double v1 = 0.0, v2 = 0.0, v3 = 0.0;
+      for (int gg = 0; gg < ngrp; gg++) {
+        v1 = v1 + s_part[0][gg * nquad + t];
+        v2 = v2 + s_part[1][gg * nquad + t];
+        v3 = v3 + s_part[2][gg * nquad + t];
       }
-      c1 = wave_sum(c1);
-      c2 = wave_sum(c2);
-      c3 = wave_sum(c3);
-      const double hl = (double)((float)(2 * l + 1) / 2.0f);
-      const double g1 = hl * c1;
-      if (lane == 0) {
-        s_leg[0][l] = g1;
-        s_leg[1][l] = hl * c2;
-        s_leg[2][l] = hl * c3;
-      }
-      if (g1 > sp(1.0e-7)) nl_h = l;
-      if (isnan(g1) || isnan(c2) || isnan(c3)) {
-        if (lane == 0) set_err(st, E_NAN_COEF);
-        return;
-      }
-    }
-    for (int l = nlegen + 1 + lane; l < MAXNLEG; l += MIE_WG) {
-      s_leg[0][l] = 0.0;
-      s_leg[1][l] = 0.0;
-      s_leg[2][l] = 0.0;
+      const double wt = twt[t];
+      s_q[0][t] = tmp * v1 * wt;
+      s_q[1][t] = tmp * v2 * wt;
+      s_q[2][t] = tmp * v3 * wt;
     }
     __syncthreads();
+    // --- C: Legendre projection (mie_spheres.f90:272-314), thread per order
+    double g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    if (t <= nlegen) {
+      double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+      for (int k = 0; k < nquad; k++) {
+        const double pl = tpl[(size_t)k * (nlegen + 1) + t];
+        c1 += s_q[0][k] * pl;
+        c2 += s_q[1][k] * pl;
+        c3 += s_q[2][k] * pl;
+      }
+      const double hl = (double)((float)(2 * t + 1) / 2.0f);
+      g1 = hl * c1;
+      g2 = hl * c2;
+      g3 = hl * c3;
+      if (g1 > sp(1.0e-7)) atomicMax(&s_nlh, t);
+      if (isnan(g1) || isnan(c2) || isnan(c3)) s_nan = 1;
+    }
+    __syncthreads();
+    if (s_nan) {
+      if (t == 0) set_err(st, E_NAN_COEF);
+      return;
+    }
     // accumulate over hydrometeors (scatProperties.f90:920-946)
+    const int nl_h = s_nlh;
     if (nl_h > nleg_coef) nleg_coef = nl_h;
     kexttot = kexttot + extinction;
     salbedo = (kexttot == 0.0) ? 0.0 : salbedo + (albedo * extinction);
-    for (int jj = lane; jj < nleg_coef; jj += MIE_WG) {
-      s_lc[0][jj] = s_lc[0][jj] + (s_leg[0][jj] * albedo * extinction);
-      s_lc[1][jj] = s_lc[1][jj] + (s_leg[1][jj] * albedo * extinction);
-      s_lc[2][jj] = s_lc[2][jj] + (s_leg[2][jj] * albedo * extinction);
+    if (t < nleg_coef && t <= nlegen) {
+      s_lc[0][t] = s_lc[0][t] + (g1 * albedo * extinction);
+      s_lc[1][t] = s_lc[1][t] + (g2 * albedo * extinction);
+      s_lc[2][t] = s_lc[2][t] + (g3 * albedo * extinction);
     }
     __syncthreads();
     if (kexttot < 0.0 || isnan(kexttot)) {
-      if (lane == 0) set_err(st, E_KEXT);
+      if (t == 0) set_err(st, E_KEXT);
       return;
     }
     if (nleg_coef > MAXNLEG - 1) {
-      if (lane == 0) set_err(st, E_NLEG);
+      if (t == 0) set_err(st, E_NLEG);
       return;
     }
   }
@@ -353,12 +404,12 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
   salbedo = salbedo / kexttot;
   const double norm = salbedo * kexttot;
   double *out = w.coef + (size_t)itf * NCOEF * MAXNLEG;
-  for (int l = lane; l <= nleg_coef && l < MAXNLEG; l += MIE_WG) {
+  for (int l = t; l <= nleg_coef && l < MAXNLEG; l += MIE_WG) {
     out[l] = s_lc[0][l] / norm;
     out[MAXNLEG + l] = s_lc[1][l] / norm;
     out[2 * MAXNLEG + l] = s_lc[2][l] / norm;
   }
-  if (lane == 0) {
+  if (t == 0) {
     w.kext[itf] = kexttot;
     w.salb[itf] = salbedo;
     w.nleg[itf] = nleg_coef;
