@@ -123,18 +123,25 @@ def main():
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     stage_ms /= args.steps
-    work = (C.c_int64 * 9)()
-    lib.pam_last_work(ctx, work, 9)
+    work = (C.c_int64 * 12)()
+    lib.pam_last_work(ctx, work, 12)
     work = list(work)
     names = lib.pam_stage_names().decode().split(",")
     value = nx * ny * nf / dt
 
-    # algorithmic flops per step of the RT stages
-    n3, n2 = NN ** 3, NN ** 2
-    f_doubling = work[3] * (14 * n3 + 24 * n2) + work[8] * 512 * 11
-    f_adding = work[4] * (20 * n3 + 16 * n2) + work[5] * (4 * n2 + 3 * n2) \
-        + nx * ny * nf * (4 * 2 * n3 + 2 * 2 * n3)
+    # Algorithmic flops of the RT stages, counted on the reference's algorithm
+    # (one FMA = 2 flops): a symmetric doubling step is 6 products and one
+    # inverse, 14 N^3 (radintg4.f90:212-297); the m=0 phase matrix is
+    # 4 NUMMU^2 (NUMPTS/2+1)(nlegen+1) terms of 11 flops (scatcnv.f90:103-192);
+    # adding with prefix reuse is 20 N^3 per layer combine plus 2 x 8 N^3 for
+    # the two internal radiances (radintg4.f90:119-172, 447-513).
+    n3 = NN ** 3
+    nummu = NN // 2
+    f_doubling = work[3] * 14 * n3 + work[8] * 4 * nummu * nummu * 11
+    f_adding = (work[4] + work[5] + nx * ny * nf) * 20 * n3 + nx * ny * nf * 16 * n3
     flops = {"doubling": f_doubling, "adding": f_adding}
+    # 32x32 products the matrix cores actually executed (series inverse included)
+    executed = {"doubling": work[9] * 2 * n3, "adding": work[10] * 2 * n3}
     dom = max(range(5), key=lambda i: stage_ms[i])
     dom_name = names[dom]
     roof = None
@@ -143,7 +150,9 @@ def main():
         roof = {"bound": "mfma", "kernel": dom_name, "achieved": round(ach, 3),
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_FP64_TFLOPS, 4),
-                "traffic": None}
+                "traffic": None,
+                "mfma_executed_tflops": round(executed[dom_name] / (stage_ms[dom] * 1e-3) / 1e12,
+                                              3)}
 
     tb_h = tb.cpu().numpy()
     st_h = status.cpu().numpy()
@@ -158,7 +167,9 @@ def main():
                    "columns": nx * ny, "channels": nf, "precision": "fp64"},
         "stage_ms": {n: round(v, 3) for n, v in zip(names, stage_ms)},
         "work": {"scattering_items": work[2], "doubling_steps": work[3],
-                 "adding_scattering": work[4], "adding_clear": work[5]},
+                 "adding_scattering": work[4], "adding_clear": work[5],
+                 "doubling_products": work[9], "adding_products": work[10],
+                 "gauss_jordan_units": work[11]},
         "status_nonzero": int((st_h != 0).sum()),
         "roofline": roof,
     }

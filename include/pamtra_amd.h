@@ -204,7 +204,9 @@ const char *pam_stage_names(void);
  * [2] scattering (layer, channel) items, [3] total doubling steps,
  * [4] scattering adding steps, [5] clear adding steps, [6] Mie bins,
  * [7] sum over Mie bins of the series length, [8] sum over scattering items
- * of (NUMPTS/2+1)*(nlegen+1), the phase-matrix work. */
+ * of (NUMPTS/2+1)*(nlegen+1), the phase-matrix work, [9] 32x32 matrix
+ * products executed by the doubling, [10] the same for the adding,
+ * [11] units recomputed by the Gauss-Jordan pass. */
 int pam_last_work(const pam_context *ctx, int64_t *counts, int32_t n);
 
 #ifdef __cplusplus

@@ -28,7 +28,7 @@ using namespace pam;
     }                                                                          \
   } while (0)
 
-constexpr int PAM_NWORK = 9;
+constexpr int PAM_NWORK = 12;
 static const char *k_stage_names = "columns,dsd,mie,doubling,adding";
 constexpr int NSTAGE = 5;
 
@@ -296,6 +296,7 @@ static size_t col_bytes(const Dims &d) {
   add(ncol * nf * ML * 8);
   add(2 * ncol * nf * 2 * NMU * 8);
   add(ncol * nf * 4);
+  add(ncol * nf * 4);
   add(8 * 8);
   for (int i = 0; i < 16; i++) add(0);
   return n + 256 * 32;
@@ -303,7 +304,7 @@ static size_t col_bytes(const Dims &d) {
 
 static size_t item_bytes_per(const Dims &d) {
   size_t per = 2 * 4 + (size_t)d.nh * d.nbmax * 4 * 8 + (size_t)d.nh * 4;
-  per += (size_t)d.nf * (2 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8);
+  per += (size_t)d.nf * (2 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8 + 4);
   return per;
 }
 
@@ -328,6 +329,7 @@ static void carve_items(Work &w, void *base, const Dims &d, size_t nitems) {
   w.T = cv.take<double>(nif * NN2);
   w.Sp = cv.take<double>(nif * NN);
   w.Sm = cv.take<double>(nif * NN);
+  w.redo_dbl = cv.take<int32_t>(nif);
 }
 
 static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_desc *hydros,
@@ -398,7 +400,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   s.emissivity = st->emissivity;
   {
     const char *inv = getenv("PAM_INVERSE");
-    s.inverse_gj = (inv && strcmp(inv, "gauss-jordan") == 0) ? 1 : 0;
+    s.inverse_gj = !inv ? 0 : strcmp(inv, "gauss-jordan") == 0 ? 1 : strcmp(inv, "queue-all") == 0 ? 2 : 0;
   }
 
   Profile p;
@@ -448,6 +450,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   pam_hydro_desc *d_hyd = sc.take<pam_hydro_desc>(std::max(1, d.nh));
   double *d_freqs = sc.take<double>(nfreq);
   unsigned long long *d_cnt = sc.take<unsigned long long>(PAM_NWORK);
+  uint32_t *d_redo_n = sc.take<uint32_t>(2);
   if (d.nh > 0)
     HIP_OK(hipMemcpyAsync(d_hyd, hydros, sizeof(pam_hydro_desc) * d.nh, hipMemcpyHostToDevice,
                           stream));
@@ -480,6 +483,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     w.sfc_emis = cv.take<double>(ncol * nf * 2 * NMU);
     w.sfc_refl = cv.take<double>(ncol * nf * 2 * NMU);
     w.cf_status = cv.take<int32_t>(ncol * nf);
+    w.redo_add = cv.take<int32_t>(ncol * nf);
     HIP_OK(hipMemsetAsync(w.col_status, 0, ncol * 4, stream));
     HIP_OK(hipMemsetAsync(w.cf_status, 0, ncol * nf * 4, stream));
   }
@@ -489,6 +493,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   w.atab_off = c->d_atab_off;
   w.atab_nmax = c->atab_nmax;
   w.counters = d_cnt;
+  w.redo_n = d_redo_n;
 
   std::vector<hipEvent_t> evs;
   std::vector<int> ev_stage;
@@ -574,6 +579,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     launch_mie(dc, s, d_hyd, h4, p, d_freqs, w, stream);
     HIP_OK(hipGetLastError());
     HIP_OK(mark(2));
+    HIP_OK(hipMemsetAsync(w.redo_n, 0, 2 * sizeof(uint32_t), stream));
     launch_doubling(dc, s, d_freqs, w, stream);
     HIP_OK(hipGetLastError());
     HIP_OK(mark(3));

@@ -66,7 +66,10 @@ struct Dims {
 struct Settings {
   int lgas, lhyd_abs, lhyd_scat, lhyd_emis, incl_hydro_rho, limit_density, adaptive,
       conserve, outpol, precision;
-  int inverse_gj;  // 1: every (I - X)^-1 by Gauss-Jordan (PAM_INVERSE=gauss-jordan)
+  // inverses: 0 series with a Gauss-Jordan pass for the units that do not
+  // converge; 1 Gauss-Jordan only (PAM_INVERSE=gauss-jordan); 2 every unit
+  // through the Gauss-Jordan queue (PAM_INVERSE=queue-all, a test mode)
+  int inverse_gj;
   double threshold, softsphere_min, emissivity;
 };
 
@@ -122,8 +125,12 @@ struct Work {
   const int64_t *atab_off;             // [atab_nmax + 1]
   int atab_nmax;
   const double *mu, *qw;               // Lobatto nodes/weights [NMU]
+  // units whose inverse series did not converge, for the Gauss-Jordan pass
+  uint32_t *redo_n;                    // [2]: doubling, adding (reset per chunk)
+  int32_t *redo_dbl;                   // [nitems*nf] of the chunk
+  int32_t *redo_add;                   // [ncol*nf]
   // work counters (atomics)
-  unsigned long long *counters;        // [9], see pam_last_work
+  unsigned long long *counters;        // [12], see pam_last_work
 };
 
 // ------------------------------------------------------------------

@@ -77,6 +77,46 @@ __device__ __forceinline__ double mm_core(const double *A, const double *B, cons
   return wmax ? max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.0;
 }
 
+// Two independent products in one pass: C1 = D1 + A1 B1, C2 = D2 + A2 B2
+// (D1, D2 may be NULL).  Each wave interleaves the two MFMA chains of its
+// tile; all operands are read before the first write, so any output may
+// alias any input.  With wmax the return value is max |C2_ij|.
+__device__ __forceinline__ double mm2(const double *A1, const double *B1, const double *D1,
+                                      double *C1, const double *A2, const double *B2,
+                                      const double *D2, double *C2, double *wmax) {
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
+  const int li = l & 15, lk = l >> 4;
+  d4_t acc1, acc2;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    acc1[r] = D1 ? D1[(I + 4 * r + lk) * LD + J + li] : 0.0;
+    acc2[r] = D2 ? D2[(I + 4 * r + lk) * LD + J + li] : 0.0;
+  }
+#pragma unroll
+  for (int s = 0; s < NN / 4; s++) {
+    const double a1 = A1[(I + li) * LD + 4 * s + lk];
+    const double b1 = B1[(4 * s + lk) * LD + J + li];
+    const double a2 = A2[(I + li) * LD + 4 * s + lk];
+    const double b2 = B2[(4 * s + lk) * LD + J + li];
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc1, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, acc2, 0, 0, 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    C1[(I + 4 * r + lk) * LD + J + li] = acc1[r];
+    C2[(I + 4 * r + lk) * LD + J + li] = acc2[r];
+  }
+  if (wmax) {
+    double m = max_nan(max_nan(fabs(acc2[0]), fabs(acc2[1])), max_nan(fabs(acc2[2]), fabs(acc2[3])));
+    m = wave_max(m);
+    if (l == 0) wmax[wv] = m;
+  }
+  __syncthreads();
+  return wmax ? max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.0;
+}
+
 // C = A * B
 __device__ __forceinline__ void mm(const double *A, const double *B, double *C) {
   mm_core(A, B, nullptr, C, nullptr);
@@ -96,9 +136,18 @@ __device__ __forceinline__ double mv_row(const double *A, const double *x, int i
   return s;
 }
 
-// LDS scratch of the inverse, and its mode (gj = 1: Gauss-Jordan only)
+// Row i of A x for a 32-vector, split over the two lanes t and t^1 (each
+// sums half of the row); both lanes return the full sum.  No barrier.
+__device__ __forceinline__ double mv_pair(const double *A, const double *x, int i) {
+  const int h = (threadIdx.x & 1) * (NN / 2);
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < NN / 2; k++) s += A[i * LD + h + k] * x[h + k];
+  return s + __shfl_xor(s, 1, 64);
+}
+
+// LDS scratch of the inverses
 struct InvLds {
-  int gj;
   double wmax[4];
   double red[4];
   int redi[4];
@@ -109,10 +158,10 @@ struct InvLds {
 // augmented [I - X | I].  Rows stay in registers: thread t owns row t/8 and
 // the 8 augmented columns 8*(t%8)..+7.  Pivoting is implicit (no row swaps):
 // the pivot row of step k becomes row k of the inverse.  Two barriers per
-// pivot.  Returns false if a pivot is zero (singular matrix).  Kept out of
-// line: it is the rare fallback of inv_i_minus_prod, and inlined it would
-// raise the register count (and lower the occupancy) of its callers.
-__device__ __noinline__ bool inv_i_minus(const double *X, double *G, InvLds &sh) {
+// pivot.  Returns false if a pivot is zero (singular matrix).  Only the
+// Gauss-Jordan kernels use it, so the series kernels keep a low register
+// count.
+__device__ bool inv_i_minus(const double *X, double *G, InvLds &sh) {
   const int t = threadIdx.x, r = t >> 3, cg = t & 7, l = t & 63, wv = t >> 6;
   double a[8];
 #pragma unroll
@@ -189,70 +238,89 @@ __device__ __noinline__ bool inv_i_minus(const double *X, double *G, InvLds &sh)
 constexpr double NEU_TOL = 0x1p-56;
 constexpr int NEU_MAXIT = 20;
 
-// G = inv(I - A B) with X = A B formed in X (overwritten).  When the
-// spectral radius of X is below one, inv(I - X) = (I + X)(I + X^2)(I + X^4)...
-// and each factor costs two matrix-core products; the series stops as soon as
-// max |X^(2^k)| < NEU_TOL (one product for the thin layers at the start of
-// the doubling, where X is ~1e-12).  If it has not converged after NEU_MAXIT
-// factors (spectral radius within ~1e-6 of one) or X holds a NaN, the
-// register Gauss-Jordan inverts I - A B instead and reports a singular
-// matrix.
-__device__ bool inv_i_minus_prod(const double *A, const double *B, double *X, double *G,
-                                 InvLds &sh) {
-  const int t = threadIdx.x;
-  if (sh.gj) {
+// G = inv(I - A B) with X = A B formed in X (overwritten).
+// GJ = false: when the spectral radius of X is below one,
+//   inv(I - X) = (I + X)(I + X^2)(I + X^4)...,
+// each factor costing two matrix-core products; the series stops as soon as
+// max |X^(2^k)| < NEU_TOL (a single product for the thin layers at the start
+// of the doubling, where X is ~1e-12).  Returns false if it has not
+// converged after NEU_MAXIT factors (spectral radius within ~1e-6 of one) or
+// X holds a NaN: the caller then queues its unit for the Gauss-Jordan pass.
+// GJ = true: Gauss-Jordan with partial pivoting; false means singular.
+// np counts the 32x32 products executed.
+template <bool GJ>
+__device__ __forceinline__ bool inv_i_minus_ab(const double *A, const double *B, double *X,
+                                               double *G, InvLds &sh, int &np) {
+  np++;
+  if constexpr (GJ) {
     mm(A, B, X);
     return inv_i_minus(X, G, sh);
-  }
-  double m = mm_core(A, B, nullptr, X, sh.wmax);
-  for (int e = t; e < NN2; e += RT_WG) {
-    const int i = e / NN, j = e % NN;
-    G[i * LD + j] = (i == j ? 1.0 : 0.0) + X[i * LD + j];
-  }
-  __syncthreads();
-  bool conv = false;
-  for (int it = 0; it < NEU_MAXIT; it++) {
-    if (!(m < 0x1p64)) break;      // diverging, or a NaN: Gauss-Jordan decides
-    if (32.0 * m * m < NEU_TOL) {  // bound on max |X^2|
-      conv = true;
-      break;
+  } else {
+    const int t = threadIdx.x;
+    double m = mm_core(A, B, nullptr, X, sh.wmax);
+    for (int e = t; e < NN2; e += RT_WG) {
+      const int i = e / NN, j = e % NN;
+      G[i * LD + j] = (i == j ? 1.0 : 0.0) + X[i * LD + j];
     }
-    m = mm_core(X, X, nullptr, X, sh.wmax);  // X <- X^2
-    if (m < NEU_TOL) {
-      conv = true;
-      break;
+    __syncthreads();
+    // G = I + P with P = X; then per factor: G <- G (I + P), P <- P^2, the
+    // two products fused while the next factor still matters
+    if (!(m < 0x1p64)) return false;       // NaN or overflow
+    if (32.0 * m * m < NEU_TOL) return true;  // bound on max |P^2|
+    m = mm_core(X, X, nullptr, X, sh.wmax);
+    np++;
+    for (int it = 0; it < NEU_MAXIT; it++) {
+      if (!(m < 0x1p64)) return false;
+      if (m < NEU_TOL) return true;
+      if (32.0 * m * m < NEU_TOL) {  // last factor
+        mm_add(G, G, X, G);
+        np++;
+        return true;
+      }
+      m = mm2(G, X, G, G, X, X, nullptr, X, sh.wmax);
+      np += 2;
     }
-    mm_add(G, G, X, G);  // G <- G (I + X)
+    return false;
   }
-  if (conv) return true;
-  mm(A, B, X);
-  return inv_i_minus(X, G, sh);
 }
 
 // ------------------------------------------------------------------
 // phase matrix + doubling
 // ------------------------------------------------------------------
 
-__global__ void __launch_bounds__(RT_WG) k_doubling(Dims d, Settings s, const double *freqs,
-                                                    Work w) {
-  __shared__ double sR[MSZ], sT[MSZ], sG[MSZ], sX[MSZ];
-  __shared__ double sc[NCOEF][MAXNLEG];
-  __shared__ double sv[8][NN];  // S+, S-, C+, C-, chain scratch
-  __shared__ double snorm[NMU];
-  __shared__ InvLds sinv;
-  __shared__ double smu[NMU], sqw[NMU];
-  double *sY = sG;  // quadrant (-,+) of the phase matrix until the init
+struct DblLds {
+  double R[MSZ], T[MSZ], G[MSZ], X[MSZ];
+  double v[12][NN];  // S+, S-, C+, C-, then source-chain scratch
+  double norm[NMU];
+  InvLds inv;
+  double mu[NMU], qw[NMU];
+};
+
+// One (scattering layer, channel) unit.  Returns false only in the series
+// variant (GJ = false) when an inverse did not converge; nothing has been
+// written for the unit then.
+template <bool GJ>
+__device__ bool doubling_unit(const Dims &d, const Settings &s, const double *freqs,
+                              const Work &w, int itf, DblLds &L) {
+  double *sR = L.R, *sT = L.T, *sG = L.G, *sX = L.X;
+  double(*sv)[NN] = L.v;
+  double *snorm = L.norm, *smu = L.mu, *sqw = L.qw;
+  InvLds &sinv = L.inv;
+  // until INITIALIZE4: sG holds the (-,+) quadrant of the phase matrix, sT
+  // the scaled Legendre series, sR the azimuth cosines and sines
+  static_assert(NCOEF * MAXNLEG <= MSZ && 2 * 128 <= MSZ, "scatcnv scratch");
+  double *sY = sG;
+  double(*sc)[MAXNLEG] = reinterpret_cast<double(*)[MAXNLEG]>(sT);
+  double *scos = sR, *ssin = sR + 128;
 
   const int t = threadIdx.x;
-  const int itf = blockIdx.x;
   const int li = itf / d.nf, f = itf % d.nf;
   const int col = w.item_col[li], z = w.item_layer[li];
   if (t < NMU) {
     smu[t] = w.mu[t];
     sqw[t] = w.qw[t];
   }
-  if (t == 0) sinv.gj = s.inverse_gj;
-  if (w.itf_status[itf] & 0xff) return;
+  if (w.itf_status[itf] & 0xff) return true;
   const int nleg = w.nleg[itf];
   double kext = w.kext[itf], salb = w.salb[itf];
   if (nleg <= 0) {  // no Legendre series: no scatcnv, zero optics
@@ -271,16 +339,22 @@ __global__ void __launch_bounds__(RT_WG) k_doubling(Dims d, Settings s, const do
   // (into sY).  Each thread does 2 of the 512 (quadrant, mu2, mu1) pairs.
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
   if (nleg > 0 && s.lhyd_scat) {
+    for (int k = t; k <= numpts / 2; k += RT_WG) {
+      const double delphi = (2.0 * PI * k) / numpts;
+      scos[k] = cos(delphi);
+      ssin[k] = sin(delphi);
+    }
+    __syncthreads();
     for (int pr = t; pr < 2 * NMU * NMU; pr += RT_WG) {
       const int q = pr / (NMU * NMU), j2 = (pr / NMU) % NMU, j1 = pr % NMU;
       const double mu1 = q ? -smu[j1] : smu[j1];
       const double mu2 = smu[j2];
       const double st1 = sqrt(1. - mu1 * mu1), st2 = sqrt(1. - mu2 * mu2);
+      const double sq12 = sqrt((1. - mu1 * mu1) * (1. - mu2 * mu2));
       double e11 = 0.0, e12 = 0.0, e21 = 0.0, e22 = 0.0;
       for (int k = 1; k <= numpts / 2 + 1; k++) {
-        const double delphi = (2.0 * PI * (k - 1)) / numpts;
-        const double cosphi = cos(delphi), sinphi = sin(delphi);
-        const double x = mu1 * mu2 + sqrt((1. - mu1 * mu1) * (1. - mu2 * mu2)) * cosphi;
+        const double cosphi = scos[k - 1], sinphi = ssin[k - 1];
+        const double x = mu1 * mu2 + sq12 * cosphi;
         double a1 = 0.0, b1 = 0.0, a3 = 0.0, pl = 1.0, pl1 = 1.0, pl2 = 0.0;
         for (int l = 0; l <= nleg; l++) {
           if (l > 0) pl = (double)(2 * l - 1) * x * pl1 / l - (double)(l - 1) * pl2 / l;
@@ -377,53 +451,36 @@ __global__ void __launch_bounds__(RT_WG) k_doubling(Dims d, Settings s, const do
   }
   __syncthreads();
   // ---- doubling (DOUBLING_INTEGRATION4, SYMMETRIC)
+  int nprod = 0;
   for (int it = 0; it < nd; it++) {
-    if (!inv_i_minus_prod(sR, sR, sX, sG, sinv)) {
+    if (!inv_i_minus_ab<GJ>(sR, sR, sX, sG, sinv, nprod)) {
+      if (!GJ) return false;
       if (t == 0) set_err(&w.itf_status[itf], E_SINGULAR);
-      return;
+      return true;
     }
-    // sources: four independent 3-matvec chains, one per 32-thread group
+    // R' = R + T G R T and T' = T G T with Z = T G and V = R T:
+    // R' = R + Z V, T' = Z T (four products, two fused passes).
+    mm2(sT, sG, nullptr, sG, sR, sT, nullptr, sX, nullptr);  // Z = T G (into G), V = R T
+    // sources: S+' = (S+ + Z (S+ + R (S- + f C-))) + f C+, C+' = C+ + Z (C+ + R C-),
+    // S-' = S- + Z ((S- + f C-) + R S+), C-' = C- + Z (C- + R C+)
     {
-      const int g = t >> 5, i = t & 31;
-      double a = 0.0;
-      if (g == 0) a = sv[3][i] * linfac + sv[1][i];  // Sp: S- + f C-
+      const int g = t >> 6, i = (t >> 1) & 31;
+      const bool lead = (t & 1) == 0;
+      if (t < NN) sv[4][t] = sv[1][t] + linfac * sv[3][t];  // S- + f C-
       __syncthreads();
-      // stage 1: u = R * v_g
-      double u = 0.0;
-      if (g == 0) {
-        sv[4][i] = a;
-      }
+      const double *v1 = g == 0 ? sv[4] : g == 1 ? sv[3] : g == 2 ? sv[0] : sv[2];
+      const double base1 = g == 0 ? sv[0][i] : g == 1 ? sv[2][i] : g == 2 ? sv[4][i] : sv[3][i];
+      const double u = base1 + mv_pair(sR, v1, i);
+      if (lead) sv[8 + g][i] = u;
       __syncthreads();
-      if (g == 0) u = mv_row(sR, sv[4], i);        // R (S- + f C-)
-      if (g == 1) u = mv_row(sR, sv[3], i);        // R C-
-      if (g == 2) u = mv_row(sR, sv[0], i);        // R S+
-      if (g == 3) u = mv_row(sR, sv[2], i);        // R C+
+      const double z = mv_pair(sG, sv[8 + g], i);
+      const double nv = g == 0 ? (sv[0][i] + z) + linfac * sv[2][i]
+                               : (g == 1 ? sv[2][i] : g == 2 ? sv[1][i] : sv[3][i]) + z;
       __syncthreads();
-      if (g == 0) sv[4][i] = sv[0][i] + u;                       // S+ + R(...)
-      if (g == 1) sv[5][i] = sv[2][i] + u;                       // C+ + R C-
-      if (g == 2) sv[6][i] = (sv[1][i] + linfac * sv[3][i]) + u; // (S- + f C-) + R S+
-      if (g == 3) sv[7][i] = sv[3][i] + u;                       // C- + R C+
-      __syncthreads();
-      double v2 = (g < 4) ? mv_row(sG, sv[4 + g], i) : 0.0;
-      __syncthreads();
-      if (g < 4) sv[4 + g][i] = v2;
-      __syncthreads();
-      double v3 = (g < 4) ? mv_row(sT, sv[4 + g], i) : 0.0;
-      double nv = 0.0;
-      if (g == 0) nv = (sv[0][i] + v3) + linfac * sv[2][i];  // new S+
-      if (g == 1) nv = sv[2][i] + v3;                        // new C+
-      if (g == 2) nv = sv[1][i] + v3;                        // new S-
-      if (g == 3) nv = sv[3][i] + v3;                        // new C-
-      __syncthreads();
-      if (g < 4) sv[g == 0 ? 0 : (g == 1 ? 2 : (g == 2 ? 1 : 3))][i] = nv;
-      __syncthreads();
+      if (lead) sv[g == 0 ? 0 : (g == 1 ? 2 : (g == 2 ? 1 : 3))][i] = nv;
     }
-    // R <- R + T G R T, T <- T G T.  G = inv(I - R R) commutes with R, so
-    // T G R T = (T R)(G T): four products instead of five.
-    mm(sT, sR, sX);          // X = T R
-    mm(sG, sT, sG);          // Y = G T (G is no longer needed)
-    mm(sT, sG, sT);          // T' = T Y
-    mm_add(sR, sX, sG, sR);  // R' = R + X Y
+    mm2(sG, sT, nullptr, sT, sG, sX, sR, sR, nullptr);  // T' = Z T, R' = R + Z V
+    nprod += 4;
     linfac = 2.0 * linfac;
   }
   // ---- store R, T, S+, S-
@@ -442,6 +499,32 @@ __global__ void __launch_bounds__(RT_WG) k_doubling(Dims d, Settings s, const do
     atomicAdd(&w.counters[3], (unsigned long long)nd);
     if (nleg > 0 && s.lhyd_scat)
       atomicAdd(&w.counters[8], (unsigned long long)(numpts / 2 + 1) * (nleg + 1));
+    atomicAdd(&w.counters[9], (unsigned long long)nprod);
+    if (GJ) atomicAdd(&w.counters[11], 1ull);
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(RT_WG) __attribute__((amdgpu_waves_per_eu(4))) k_doubling(Dims d, Settings s, const double *freqs,
+                                                    Work w) {
+  __shared__ DblLds L;
+  const int itf = blockIdx.x;
+  // inverse_gj == 2 (a test mode) queues every unit for the Gauss-Jordan pass
+  const bool done = s.inverse_gj != 2 && doubling_unit<false>(d, s, freqs, w, itf, L);
+  if (!done && threadIdx.x == 0) {
+    const unsigned q = atomicAdd(&w.redo_n[0], 1u);
+    w.redo_dbl[q] = itf;
+  }
+}
+
+// Gauss-Jordan pass over the queued units (all = 0) or every unit (all = 1)
+__global__ void __launch_bounds__(RT_WG) k_doubling_gj(Dims d, Settings s, const double *freqs,
+                                                       Work w, int all) {
+  __shared__ DblLds L;
+  const int n = all ? d.nitems * d.nf : (int)w.redo_n[0];
+  for (int q = blockIdx.x; q < n; q += gridDim.x) {
+    __syncthreads();
+    doubling_unit<true>(d, s, freqs, w, all ? q : w.redo_dbl[q], L);
   }
 }
 
@@ -469,10 +552,11 @@ __device__ __forceinline__ void mv_stage(const double *A, double *v) {
 // COMBINE_LAYERS4 (radintg4.f90:447-513): stack on top of a doubled layer
 // with R2+ = R2- = R2 and T2+ = T2- = T2.  The minus side goes first, since
 // it reads the stack's old R+ and T+.
+template <bool GJ>
 __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, const double *S2p,
-                             const double *S2m, double *G, double *X, double *vs, InvLds &sinv) {
+                             const double *S2m, double *G, double *X, double *vs, InvLds &sinv, int &np) {
   const int t = threadIdx.x;
-  if (!inv_i_minus_prod(R2, s1.Rp, X, G, sinv)) return false;
+  if (!inv_i_minus_ab<GJ>(R2, s1.Rp, X, G, sinv, np)) return false;
   if (t < NN) vs[t] = S2m[t] + mv_row(R2, s1.Sp, t);  // S2m + R2m S1p
   __syncthreads();
   mv_stage(G, vs);
@@ -483,7 +567,8 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
   mm_add(s1.Rm, s1.Tm, X, s1.Rm);                     // R1m + T1m G R2m T1p
   mm(G, T2, X);
   mm(s1.Tm, X, s1.Tm);                                // T1m G T2m
-  if (!inv_i_minus_prod(s1.Rp, R2, X, G, sinv)) return false;
+  np += 5;
+  if (!inv_i_minus_ab<GJ>(s1.Rp, R2, X, G, sinv, np)) return false;
   if (t < NN) vs[t] = s1.Sp[t] + mv_row(s1.Rp, S2m, t);  // S1p + R1p S2m
   __syncthreads();
   mv_stage(G, vs);
@@ -494,6 +579,7 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
   mm_add(R2, T2, X, s1.Rp);                           // R2p + T2p G R1p T2m
   mm(G, s1.Tp, X);
   mm(T2, X, s1.Tp);                                   // T2p G T1p
+  np += 5;
   __syncthreads();
   return true;
 }
@@ -524,10 +610,11 @@ __device__ void combine_clear(Stack &s1, const double *dg, const double *S2p, co
 
 // The combine with the surface "layer" of get_sfc_matrices
 // (sfc_matrices.f90:21-194): R2+ = 0, R2- = Rs, T2 = I, S2 = 0.
+template <bool GJ>
 __device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *X, double *vs,
-                                InvLds &sinv) {
+                                InvLds &sinv, int &np) {
   const int t = threadIdx.x;
-  if (!inv_i_minus_prod(Rs, s1.Rp, X, G, sinv)) return false;
+  if (!inv_i_minus_ab<GJ>(Rs, s1.Rp, X, G, sinv, np)) return false;
   if (t < NN) vs[t] = mv_row(Rs, s1.Sp, t);
   __syncthreads();
   mv_stage(G, vs);
@@ -537,13 +624,14 @@ __device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *
   mm(G, X, X);
   mm_add(s1.Rm, s1.Tm, X, s1.Rm);
   mm(s1.Tm, G, s1.Tm);
-  if (!inv_i_minus_prod(s1.Rp, Rs, X, G, sinv)) return false;
+  if (!inv_i_minus_ab<GJ>(s1.Rp, Rs, X, G, sinv, np)) return false;
   if (t < NN) vs[t] = s1.Sp[t];
   __syncthreads();
   mv_stage(G, vs);
   if (t < NN) s1.Sp[t] = vs[t];
   mm(G, s1.Rp, s1.Rp);
   mm(G, s1.Tp, s1.Tp);
+  np += 6;
   return true;
 }
 
@@ -597,11 +685,12 @@ __device__ bool load_layer(const Dims &d, const Work &w, int col, int f, int z, 
 
 // INTERNAL_RADIANCE4 at the surface level: up = stack of all layers, down =
 // the surface (R- = Rs, T- = I, S- = 0)
+template <bool GJ>
 __device__ bool radiance_at_surface(const Stack &up, const double *Rs, const double *sky,
                                     const double *gnd, double *G, double *X, double *vs,
-                                    double *ur, double *dr, InvLds &sinv) {
+                                    double *ur, double *dr, InvLds &sinv, int &np) {
   const int t = threadIdx.x;
-  if (!inv_i_minus_prod(up.Rp, Rs, X, G, sinv)) return false;
+  if (!inv_i_minus_ab<GJ>(up.Rp, Rs, X, G, sinv, np)) return false;
   double s_ = 0.0;
   if (t < NN) {
     s_ = mv_row(up.Rp, gnd, t);          // UPR (DNT INBOT)
@@ -614,7 +703,7 @@ __device__ bool radiance_at_surface(const Stack &up, const double *Rs, const dou
   __syncthreads();
   if (t < NN) dr[t] = mv_row(G, vs, t);
   __syncthreads();
-  if (!inv_i_minus_prod(Rs, up.Rp, X, G, sinv)) return false;
+  if (!inv_i_minus_ab<GJ>(Rs, up.Rp, X, G, sinv, np)) return false;
   if (t < NN) vs[t] = mv_row(up.Tp, sky, t);
   __syncthreads();
   if (t < NN) {
@@ -676,26 +765,35 @@ __device__ void write_tb(const Dims &d, const Settings &s, int col, int f, int i
   }
 }
 
-__global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const double *freqs,
-                                                  const char *sfc_refl, Work w, double *tb) {
-  __shared__ double M[8][MSZ];
-  __shared__ double sv[10][NN];
-  __shared__ InvLds sinv;
-  __shared__ double smu[NMU], sqw[NMU];
-  __shared__ int sbad[1];
+struct AddLds {
+  double M[8][MSZ];
+  double v[10][NN];
+  InvLds inv;
+  double mu[NMU], qw[NMU];
+  int bad[1];
+};
+
+// One (column, channel) unit of the chunk.  Returns false only in the
+// series variant (GJ = false) when an inverse did not converge; the unit is
+// then recomputed from scratch by the Gauss-Jordan pass.
+template <bool GJ>
+__device__ bool adding_unit(const Dims &d, const Settings &s, const double *freqs,
+                            const char *sfc_refl, const Work &w, double *tb, int cf, AddLds &LL) {
+  double(*M)[MSZ] = LL.M;
+  double(*sv)[NN] = LL.v;
+  InvLds &sinv = LL.inv;
+  double *smu = LL.mu, *sqw = LL.qw;
+  int *sbad = LL.bad;
+  (void)s;
 
   const int t = threadIdx.x;
-  const int cf = blockIdx.x;
   const int cl = cf / d.nf, f = cf % d.nf;
   const int col = d.col0 + cl;
   if (t < NMU) {
     smu[t] = w.mu[t];
     sqw[t] = w.qw[t];
   }
-  if (t == 0) {
-    sbad[0] = 0;
-    sinv.gj = s.inverse_gj;
-  }
+  if (t == 0) sbad[0] = 0;
   int32_t *cst = w.cf_status + (size_t)col * d.nf + f;
   const int L = w.nl[col];
   const int nout = d.nout;
@@ -712,15 +810,15 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
   };
   if ((w.col_status[col] & 0xff) >= PAM_FATAL) {
     fail(w.col_status[col]);
-    return;
+    return true;
   }
   if ((*cst & 0xff) >= PAM_FATAL) {
     fail(0);
-    return;
+    return true;
   }
   if (refl != 'S' && refl != 'F' && refl != 'L') {
     fail(E_REFL | PAM_WARNING);
-    return;
+    return true;
   }
   int lev_toa = -1, lev_sfc = -1;
   for (int io = 0; io < nout; io++) {
@@ -733,7 +831,7 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
     else {
       // a level inside the column needs a second stack: not supported
       fail(E_NLYRS | PAM_FATAL);
-      return;
+      return true;
     }
   }
   Stack stk = {M[0], M[1], M[2], M[3], sv[0], sv[1]};
@@ -759,6 +857,7 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
   // fold the layers top down (radtran4.f90:386-417 with prefix reuse: the
   // stack above the surface level is the stack the top level continues)
   unsigned long long nscat = 0, nclear = 0;
+  int nprod = 0;
   for (int l = 1; l <= L; l++) {
     const int z = L - l;
     const bool sc = load_layer(d, w, col, f, z, wl, smu, R2, T2, dg, S2p, S2m, sbad);
@@ -778,9 +877,10 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
       }
       __syncthreads();
     } else if (sc) {
-      if (!combine_scat(stk, R2, T2, S2p, S2m, G, X, vs, sinv)) {
+      if (!combine_scat<GJ>(stk, R2, T2, S2p, S2m, G, X, vs, sinv, nprod)) {
+        if (!GJ) return false;
         fail(E_SINGULAR | PAM_FATAL);
-        return;
+        return true;
       }
       nscat++;
     } else {
@@ -790,7 +890,7 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
   }
   if (sbad[0]) {
     fail(sbad[0]);
-    return;
+    return true;
   }
   // surface reflection matrix into R2 (specular/Fresnel or Lambertian)
   for (int e = t; e < NN2; e += RT_WG) {
@@ -808,17 +908,19 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
   }
   __syncthreads();
   if (lev_sfc >= 0) {
-    if (!radiance_at_surface(stk, R2, sky, gnd, G, X, vs, ur, dr, sinv)) {
+    if (!radiance_at_surface<GJ>(stk, R2, sky, gnd, G, X, vs, ur, dr, sinv, nprod)) {
+      if (!GJ) return false;
       fail(E_SINGULAR | PAM_FATAL);
-      return;
+      return true;
     }
     write_tb(d, s, col, f, lev_sfc, wl, ur, dr, tb);
     __syncthreads();
   }
   if (lev_toa >= 0) {
-    if (!combine_surface(stk, R2, G, X, vs, sinv)) {
+    if (!combine_surface<GJ>(stk, R2, G, X, vs, sinv, nprod)) {
+      if (!GJ) return false;
       fail(E_SINGULAR | PAM_FATAL);
-      return;
+      return true;
     }
     radiance_at_top(stk, sky, gnd, ur, dr);
     write_tb(d, s, col, f, lev_toa, wl, ur, dr, tb);
@@ -826,22 +928,62 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
   if (t == 0) {
     atomicAdd(&w.counters[4], nscat);
     atomicAdd(&w.counters[5], nclear);
+    atomicAdd(&w.counters[10], (unsigned long long)nprod);
+    if (GJ) atomicAdd(&w.counters[11], 1ull);
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const double *freqs,
+                                                  const char *sfc_refl, Work w, double *tb) {
+  __shared__ AddLds LL;
+  const int cf = blockIdx.x;
+  // inverse_gj == 2 (a test mode) queues every unit for the Gauss-Jordan pass
+  const bool done = s.inverse_gj != 2 && adding_unit<false>(d, s, freqs, sfc_refl, w, tb, cf, LL);
+  if (!done && threadIdx.x == 0) {
+    const unsigned q = atomicAdd(&w.redo_n[1], 1u);
+    w.redo_add[q] = cf;
   }
 }
+
+// Gauss-Jordan pass over the queued units (all = 0) or every unit (all = 1)
+__global__ void __launch_bounds__(RT_WG) k_adding_gj(Dims d, Settings s, const double *freqs,
+                                                     const char *sfc_refl, Work w, double *tb,
+                                                     int all) {
+  __shared__ AddLds LL;
+  const int n = all ? d.ncol_chunk * d.nf : (int)w.redo_n[1];
+  for (int q = blockIdx.x; q < n; q += gridDim.x) {
+    __syncthreads();
+    adding_unit<true>(d, s, freqs, sfc_refl, w, tb, all ? q : w.redo_add[q], LL);
+  }
+}
+
 
 void launch_doubling(const Dims &d, const Settings &s, const double *freqs, Work &w,
                      hipStream_t st) {
   size_t n = (size_t)d.nitems * d.nf;
   if (n == 0) return;
+  if (s.inverse_gj == 1) {
+    hipLaunchKernelGGL(k_doubling_gj, dim3((unsigned)std::min<size_t>(n, 16384)), dim3(RT_WG), 0,
+                       st, d, s, freqs, w, 1);
+    return;
+  }
   hipLaunchKernelGGL(k_doubling, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, w);
+  hipLaunchKernelGGL(k_doubling_gj, dim3(512), dim3(RT_WG), 0, st, d, s, freqs, w, 0);
 }
 
 void launch_adding(const Dims &d, const Settings &s, const double *freqs, const char *sfc_refl,
                    Work &w, double *tb, hipStream_t st) {
   size_t n = (size_t)d.ncol_chunk * d.nf;
   if (n == 0) return;
+  if (s.inverse_gj == 1) {
+    hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, 16384)), dim3(RT_WG), 0, st,
+                       d, s, freqs, sfc_refl, w, tb, 1);
+    return;
+  }
   hipLaunchKernelGGL(k_adding, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
                      tb);
+  hipLaunchKernelGGL(k_adding_gj, dim3(512), dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb, 0);
 }
 
 }  // namespace pam

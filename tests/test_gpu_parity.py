@@ -121,6 +121,13 @@ def test_gauss_jordan_and_series_inverses_agree(monkeypatch):
     assert d <= 1e-6
     c, _ = _cpu(pam, freqs)
     _compare(b, c)
+    # every unit through the queue of the Gauss-Jordan pass (the path of a
+    # series that does not converge) gives the Gauss-Jordan result
+    monkeypatch.setenv("PAM_INVERSE", "queue-all")
+    q, st3 = _gpu(pam, freqs)
+    monkeypatch.delenv("PAM_INVERSE")
+    assert st3 == 0
+    np.testing.assert_array_equal(q["tb"], b["tb"])
 
 
 def test_optically_thick_scattering_vs_oracle():
