@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
         L.pam_oracle_layer_doubling.argtypes = [dp] + [C.c_double] * 7 + [dp, dp, dp, dp,
                                                                           C.POINTER(C.c_int)]
         L.pam_oracle_scatcnv.argtypes = [C.c_int, dp, C.c_double, C.c_double, dp]
+        L.pam_oracle_esat.restype = C.c_double
+        L.pam_oracle_esat.argtypes = [C.c_double]
         L.pam_oracle_planck.restype = C.c_double
         L.pam_oracle_planck.argtypes = [C.c_double, C.c_double]
         L.pam_oracle_rad2tb.restype = C.c_double
@@ -151,6 +153,11 @@ def layer_doubling(smat, kext, salb, gas, zdiff, t_top, t_bot, wavelength_um):
                                          wavelength_um, _dp(R), _dp(T), _dp(sp), _dp(sm),
                                          C.byref(nd))
     return st, R, T, sp, sm, nd.value
+
+
+def esat(t_k):
+    """Goff-Gratch saturation vapour pressure over water [hPa]."""
+    return lib().pam_oracle_esat(t_k)
 
 
 def planck(t, wl):

@@ -13,14 +13,15 @@
  * several discrete decisions (Mie series length, doubling count, Legendre
  * trimming, adaptive size grid) depend on them.
  *
- * Parity pins (tests/test_oracle_*.py): the reference's golden end-to-end
+ * Parity pins (tests/test_oracle.py): the reference's golden end-to-end
  * brightness temperatures (tests/golden/rain_snow_reference.npz, rtol 1e-5),
  * the TKC permittivity known answer (tests/test_fortran_functions.py:23-33),
- * closed forms (Gauss/Lobatto nodes vs numpy, Planck inverse, Rayleigh-limit
- * and optical-theorem Mie identities, an independent Mie series), invariants
- * (isothermal black-surface column, normalisation, symmetry) and brute force
- * (doubling vs matrix exponential).  Functions none of these reach are named
- * "parity unpinned" in their comment.
+ * the Goff-Gratch known answer (tests/test_meteosi.py:6-10), closed forms
+ * (Gauss/Lobatto nodes vs numpy, Planck inverse, Rayleigh-limit and
+ * optical-theorem Mie identities, an independent Mie series, DSD moments),
+ * invariants (isothermal black-surface column, normalisation, symmetry) and
+ * brute force (doubling vs matrix exponential).  Functions none of these
+ * reach are named "parity unpinned" in their comment.
  */
 #include <complex.h>
 #include <math.h>
@@ -210,7 +211,9 @@ static double o2abs(double tk, double pres, double vapden, double f) {
   return 0.5034e12 * sum * presda * (th * th * th) / PI;
 }
 
-/* abh2o (gasabs_module.f90:214-333), Np/km */
+/* abh2o (gasabs_module.f90:214-333), Np/km.  Parity unpinned: the golden
+ * profile is dry (relhum 0); the tests only check the 22.235 GHz line peak
+ * and the growth with vapour density. */
 static double abh2o(double tk, double pres, double rho, double f) {
   if (rho <= 0.0) return 0.0;
   double pvap = rho * tk / 217.0;
@@ -254,7 +257,9 @@ double pam_oracle_gasabs(double f, double tk, double rhowv, double pres) {
 /* ------------------------------------------------------------------ */
 
 /* TESSEM2 network (src/tessem2.f90:1-137).  Weight tables are the model's
- * published coefficients, stored as single-precision literals there. */
+ * published coefficients, stored as single-precision literals there.
+ * Parity unpinned: the golden scenario uses a constant-emissivity surface;
+ * the tests check only bounds and the V/H ordering. */
 static const float TS_B1[2][15] = {
     {-4.425637f, -9.833792f, 1.338131f, -1.688713f, 0.121844f, -0.020075f, 0.983048f, -5.365536f,
      0.759206f, 1.705090f, -0.861439f, 2.639012f, 4.438483f, -1.376479f, 0.920075f},
@@ -2104,6 +2109,9 @@ void pam_oracle_scatcnv(int nlegen, const double *coef, double kext, double salb
     for (int l = 0; l <= nlegen; l++) c[k][l] = cst * coef[k * (nlegen + 1) + l];
   transform_scatter(mu, nlegen, c, smat);
 }
+
+/* Goff-Gratch saturation vapour pressure over water [hPa], for tests */
+double pam_oracle_esat(double T) { return e_sat_gg_water(T); }
 
 /* planck radiance and its inverse, for tests */
 double pam_oracle_planck(double temp, double wl) { return planck(temp, wl); }

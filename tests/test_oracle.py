@@ -225,3 +225,72 @@ def test_gas_absorption_dry_air_positive_and_monotone_in_pressure():
     a3 = oracle.gasabs(35.5, 280.0, 0.01, 100000.0)
     assert 0 < a1 < a2 < a3
     assert oracle.gasabs(900.0, 280.0, 0.0, 1e5) < 0  # invalid input is refused
+
+
+def _liquid_row(dist, nbin, d1, d2, p1=-99.0, p2=-99.0, p3=-99.0, p4=-99.0):
+    return {"hydro_name": "x", "as_ratio": -99.0, "liq_ice": 1, "rho_ms": -99.0,
+            "a_ms": -99.0, "b_ms": -99.0, "alpha_as": -99.0, "beta_as": -99.0,
+            "moment_in": 3, "nbin": nbin, "dist_name": dist, "p_1": p1, "p_2": p2,
+            "p_3": p3, "p_4": p4, "d_1": d1, "d_2": d2, "scat_name": "mie-sphere",
+            "vel_size_mod": "khvorostyanov01_drops", "canting": -99.0}
+
+
+@pytest.mark.parametrize("case", ["mgamma", "exp"])
+def test_dsd_parameters_reproduce_the_prescribed_moments(case):
+    """Closed form of make_dist_params (make_dist_param.f90): without the
+    mass rescale, the binned spectrum of a modified-gamma cloud (N_tot and q
+    given, mu=2, gamma=1) and of a Marshall-Palmer rain (N0 and q given)
+    integrates to the prescribed number and mass, for liquid spheres
+    m = pi/6 rho_w D^3.  A wrong exponent or Gamma argument in the parameter
+    derivation misses these by a large factor."""
+    nml = {"conserve_mass_rescale_dsd": False, "hydro_adaptive_grid": False}
+    a, b = np.pi / 6.0 * 1000.0, 3.0
+    if case == "mgamma":
+        q, ntot = 2.0e-4, 1.0e8
+        row = _liquid_row("mgamma", 4000, 1.0e-7, 2.0e-4, p1=ntot, p3=2.0, p4=1.0)
+    else:
+        q, n0 = 5.0e-4, 8.0e6
+        row = _liquid_row("exp", 4000, 1.0e-6, 2.0e-2, p3=n0)
+    st, (d, dd, n, rho, dscat) = oracle.dsd(row, nml, q, -99.0, -99.0, 280.0)
+    assert st == 0
+    mass = np.sum(a * d ** b * n)
+    np.testing.assert_allclose(mass, q, rtol=2e-3)
+    if case == "mgamma":
+        np.testing.assert_allclose(np.sum(n), ntot, rtol=2e-3)
+    else:
+        # N(D) = N0 exp(-lambda D) with lambda = (a N0 Gamma(b+1) / q)^(1/(b+1))
+        lam = (a * n0 * 6.0 / q) ** 0.25
+        keep = n > 1e-3 * n.max()
+        np.testing.assert_allclose(n[keep] / dd[keep], n0 * np.exp(-lam * d[keep]), rtol=2e-3)
+
+
+def test_goff_gratch_known_answers():
+    """e_sat_gg_water: the reference's known answer at 0 C
+    (tests/test_meteosi.py:6-10, 610.34 Pa) and the steam point the
+    Goff-Gratch formula is anchored at (1013.246 hPa at 373.16 K)."""
+    np.testing.assert_allclose(oracle.esat(273.15) * 100.0, 610.34, rtol=1e-3)
+    np.testing.assert_allclose(oracle.esat(373.16), 1013.246, rtol=1e-6)
+
+
+def test_water_vapour_line_peaks_at_22_235_ghz():
+    """abh2o: at 100 hPa, where the line is narrow, humid-air absorption
+    peaks at the 22.235 GHz line centre; absorption grows with vapour
+    density (the line table itself is not pinned)."""
+    f = np.linspace(20.0, 24.5, 451)
+    k = np.array([oracle.gasabs(x, 293.0, 7.5e-3, 10000.0) for x in f])
+    assert abs(f[np.argmax(k)] - 22.235) <= 0.02
+    assert oracle.gasabs(22.235, 293.0, 15e-3, 101325.0) > 1.5 * oracle.gasabs(
+        22.235, 293.0, 7.5e-3, 101325.0)
+
+
+def test_tessem2_bounds_and_polarisation_order():
+    """TESSEM2 ocean emissivity: within (0, 1), vertical above horizontal at
+    oblique angles, equal at nadir within the network's accuracy."""
+    for f in (10.65, 37.0, 89.0, 183.31):
+        for zen in (0.0, 30.0, 53.0, 70.0):
+            (ev, eh), _ = oracle.tessem2(f, zen, 5.0, 290.0, 35.0)
+            assert 0.0 < eh < 1.0 and 0.0 < ev < 1.0
+            if zen >= 30.0:
+                assert ev > eh
+        (ev0, eh0), _ = oracle.tessem2(f, 0.0, 5.0, 290.0, 35.0)
+        assert abs(ev0 - eh0) < 0.02
