@@ -63,6 +63,24 @@ def to_device(pam, torch):
     return prof
 
 
+# HBM bytes per launch of the dominant kernel on the default workload, from the
+# two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a copy of known
+# size; recorded in this summary with the commands that produced it.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batched_fp64_16384col_7ch_hbm_counters.md")
+
+
+def measured_traffic(kernel):
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                f = line.split()
+                if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+                    return int(f[2])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +90,10 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host-entry", action="store_true",
+                    help="skip the pam_run (host arrays, PCIe-inclusive) measurement")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one device copy of MIB MiB (to scale HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the columns are "
@@ -150,12 +172,34 @@ def main():
         roof = {"bound": "mfma", "kernel": dom_name, "achieved": round(ach, 3),
                 "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_FP64_TFLOPS, 4),
-                "traffic": None,
+                "traffic": (measured_traffic("k_" + dom_name)
+                            if args.ncol == 16384 else None),
                 "mfma_executed_tflops": round(executed[dom_name] / (stage_ms[dom] * 1e-3) / 1e12,
                                               3)}
 
     tb_h = tb.cpu().numpy()
     st_h = status.cpu().numpy()
+
+    host_rate = None
+    if not args.no_host_entry:
+        # the host-array entry: inputs staged to HBM and TB copied back inside the call
+        mh = _capi.marshal(nml, pam.set, pam.df.data, pam.df.data4D, pam.p)
+        hr = _capi.alloc_results(mh, nf)
+        t0 = time.perf_counter()
+        sth = lib.pam_run(ctx, C.byref(mh.settings), mh.hydros, C.byref(mh.hydro4d),
+                          C.byref(mh.profile), fp, nf, C.byref(hr.struct))
+        th = time.perf_counter() - t0
+        if sth == _capi.PAM_FATAL:
+            raise RuntimeError("pam_run: " + _capi.results_message(hr))
+        host_rate = nx * ny * nf / th
+        log(f"host-array entry (PCIe-inclusive): {host_rate:.0f} col*ch/s")
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration} MiB read + written")
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "columns·channels/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -171,6 +215,7 @@ def main():
                  "doubling_products": work[9], "adding_products": work[10],
                  "gauss_jordan_units": work[11]},
         "status_nonzero": int((st_h != 0).sum()),
+        "pcie_inclusive_value": None if host_rate is None else round(host_rate, 2),
         "roofline": roof,
     }
 

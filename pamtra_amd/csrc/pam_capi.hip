@@ -231,8 +231,12 @@ extern "C" int pam_context_create(pam_context **out, int device) {
       return PAM_FATAL;
     }
   }
+  // per-call workspace budget: half of the free HBM, at most 96 GiB
+  // (PAM_WORKSPACE_GB, read at each call, overrides it)
+  size_t freeb = 0, totalb = 0;
   double gb = 32.0;
-  if (const char *e = getenv("PAM_WORKSPACE_GB")) gb = atof(e);
+  if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && freeb > 0)
+    gb = std::min(96.0, 0.5 * (double)freeb / (double)(1ull << 30));
   c->budget_bytes = (size_t)(gb * (1ull << 30));
   *out = c;
   return PAM_SUCCESS;
@@ -534,14 +538,18 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   HIP_OK(hipMemcpyAsync(w.col_item0, off.data(), 4 * (size_t)d.ncol, hipMemcpyHostToDevice,
                         stream));
   const size_t per_item = item_bytes_per(d);
-  size_t max_items = std::max<size_t>(1, c->budget_bytes / per_item);
-  // chunks of whole columns
+  size_t budget = c->budget_bytes;
+  if (const char *e = getenv("PAM_WORKSPACE_GB")) budget = (size_t)(atof(e) * (1ull << 30));
+  size_t max_items = std::max<size_t>(1, budget / per_item);
+  // chunks of whole columns with about equal item counts (no small tail chunk)
   std::vector<std::pair<int, int>> chunks;
   {
+    const int64_t nchunk = std::max<int64_t>(1, (total + (int64_t)max_items - 1) / (int64_t)max_items);
+    const int64_t target = (total + nchunk - 1) / nchunk;
     int c0 = 0;
     int64_t acc = 0;
     for (int i = 0; i < d.ncol; i++) {
-      if (acc + cnt[i] > (int64_t)max_items && i > c0) {
+      if (acc + cnt[i] > target && i > c0) {
         chunks.push_back({c0, i});
         c0 = i;
         acc = 0;

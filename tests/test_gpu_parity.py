@@ -160,6 +160,19 @@ def test_bad_column_is_isolated():
     np.testing.assert_array_equal(g["tb"][good], g2["tb"][good])
 
 
+def test_column_chunking_is_bitwise_neutral(monkeypatch):
+    """A workspace budget of a few layers per chunk (many launches per stage)
+    gives the single-chunk result bit for bit."""
+    pam, freqs = workloads.config2(ncol=12, seed=17)
+    a, st = _gpu(pam, freqs)
+    monkeypatch.setenv("PAM_WORKSPACE_GB", "0.004")  # ~25 scattering layers per chunk
+    b, st2 = _gpu(pam, freqs)
+    monkeypatch.delenv("PAM_WORKSPACE_GB")
+    assert st == 0 and st2 == 0
+    np.testing.assert_array_equal(a["tb"], b["tb"])
+    np.testing.assert_array_equal(a["ndoubles"], b["ndoubles"])
+
+
 def test_repeat_runs_are_bitwise_identical():
     pam, freqs = workloads.config2(ncol=8, seed=5)
     a, _ = _gpu(pam, freqs)
