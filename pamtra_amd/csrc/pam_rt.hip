@@ -37,7 +37,11 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 // NaN-propagating maximum (a NaN anywhere wins)
 __device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
 
-__device__ __forceinline__ double wave_max(double v) {
+__device__ __forceinline__ float max_nan(float a, float b) { return (b > a || b != b) ? b : a; }
+
+// Wavefront maximum in single precision (the inverse series only compares it
+// against thresholds far from the float limits); NaN propagates.
+__device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v = max_nan(v, __shfl_xor(v, o, 64));
   return v;
@@ -49,10 +53,10 @@ __device__ __forceinline__ double wave_max(double v) {
 // A[l%16][l/16] and B[l/16][l%16] of the 16x4 / 4x16 blocks, and holds
 // D[4*r+l/16][l%16], r = 0..3, of the result (rows interleaved over the
 // four 16-lane groups).  Every lane has read all its operands before the
-// first write, so C may alias A, B or D.  With wmax (4 LDS doubles) the
+// first write, so C may alias A, B or D.  With wmax (4 LDS floats) the
 // return value is max |C_ij| (NaN if any entry is NaN), else 0.
 __device__ __forceinline__ double mm_core(const double *A, const double *B, const double *D,
-                                          double *C, double *wmax) {
+                                          double *C, float *wmax) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
   const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
   const int li = l & 15, lk = l >> 4;
@@ -69,12 +73,12 @@ __device__ __forceinline__ double mm_core(const double *A, const double *B, cons
 #pragma unroll
   for (int r = 0; r < 4; r++) C[(I + 4 * r + lk) * LD + J + li] = acc[r];
   if (wmax) {
-    double m = max_nan(max_nan(fabs(acc[0]), fabs(acc[1])), max_nan(fabs(acc[2]), fabs(acc[3])));
-    m = wave_max(m);
-    if (l == 0) wmax[wv] = m;
+    const double m = max_nan(max_nan(fabs(acc[0]), fabs(acc[1])), max_nan(fabs(acc[2]), fabs(acc[3])));
+    const float mf = wave_max((float)m);
+    if (l == 0) wmax[wv] = mf;
   }
   __syncthreads();
-  return wmax ? max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.0;
+  return wmax ? (double)max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.0;
 }
 
 // Two independent products in one pass: C1 = D1 + A1 B1, C2 = D2 + A2 B2
@@ -83,7 +87,7 @@ __device__ __forceinline__ double mm_core(const double *A, const double *B, cons
 // alias any input.  With wmax the return value is max |C2_ij|.
 __device__ __forceinline__ double mm2(const double *A1, const double *B1, const double *D1,
                                       double *C1, const double *A2, const double *B2,
-                                      const double *D2, double *C2, double *wmax) {
+                                      const double *D2, double *C2, float *wmax) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
   const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
   const int li = l & 15, lk = l >> 4;
@@ -109,12 +113,13 @@ __device__ __forceinline__ double mm2(const double *A1, const double *B1, const 
     C2[(I + 4 * r + lk) * LD + J + li] = acc2[r];
   }
   if (wmax) {
-    double m = max_nan(max_nan(fabs(acc2[0]), fabs(acc2[1])), max_nan(fabs(acc2[2]), fabs(acc2[3])));
-    m = wave_max(m);
-    if (l == 0) wmax[wv] = m;
+    const double m =
+        max_nan(max_nan(fabs(acc2[0]), fabs(acc2[1])), max_nan(fabs(acc2[2]), fabs(acc2[3])));
+    const float mf = wave_max((float)m);
+    if (l == 0) wmax[wv] = mf;
   }
   __syncthreads();
-  return wmax ? max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.0;
+  return wmax ? (double)max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.0;
 }
 
 // C = A * B
@@ -136,19 +141,23 @@ __device__ __forceinline__ double mv_row(const double *A, const double *x, int i
   return s;
 }
 
-// Row i of A x for a 32-vector, split over the two lanes t and t^1 (each
-// sums half of the row); both lanes return the full sum.  No barrier.
+// Row i of A x for a 32-vector, split over the two lanes t and t^1 (lane h
+// sums the columns 8h..8h+7 and 16+8h..16+8h+7, so the two lanes of a pair
+// read LDS banks 16 apart); both lanes return the full sum.  No barrier.
 __device__ __forceinline__ double mv_pair(const double *A, const double *x, int i) {
-  const int h = (threadIdx.x & 1) * (NN / 2);
+  const int h = (threadIdx.x & 1) * 8;
   double s = 0.0;
 #pragma unroll
-  for (int k = 0; k < NN / 2; k++) s += A[i * LD + h + k] * x[h + k];
+  for (int k = 0; k < NN / 2; k++) {
+    const int c = (k & 7) + h + 16 * (k >> 3);
+    s += A[i * LD + c] * x[c];
+  }
   return s + __shfl_xor(s, 1, 64);
 }
 
 // LDS scratch of the inverses
 struct InvLds {
-  double wmax[4];
+  float wmax[4];
   double red[4];
   int redi[4];
   double prow[2][2 * NN];
@@ -288,6 +297,21 @@ __device__ __forceinline__ bool inv_i_minus_ab(const double *A, const double *B,
 // phase matrix + doubling
 // ------------------------------------------------------------------
 
+// Legendre recurrence P_l = a_l x P_{l-1} - b_l P_{l-2} with a_l = (2l-1)/l,
+// b_l = (l-1)/l (rt_utilities.f90:242-269 divides by l at every step)
+struct LegRec {
+  double a[MAXNLEG + 1], b[MAXNLEG + 1];
+};
+constexpr LegRec make_legrec() {
+  LegRec r{};
+  for (int l = 1; l <= MAXNLEG; l++) {
+    r.a[l] = (2.0 * l - 1.0) / l;
+    r.b[l] = (l - 1.0) / l;
+  }
+  return r;
+}
+__constant__ LegRec c_leg = make_legrec();
+
 struct DblLds {
   double R[MSZ], T[MSZ], G[MSZ], X[MSZ];
   double v[12][NN];  // S+, S-, C+, C-, then source-chain scratch
@@ -357,7 +381,7 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
         const double x = mu1 * mu2 + sq12 * cosphi;
         double a1 = 0.0, b1 = 0.0, a3 = 0.0, pl = 1.0, pl1 = 1.0, pl2 = 0.0;
         for (int l = 0; l <= nleg; l++) {
-          if (l > 0) pl = (double)(2 * l - 1) * x * pl1 / l - (double)(l - 1) * pl2 / l;
+          if (l > 0) pl = (c_leg.a[l] * x) * pl1 - c_leg.b[l] * pl2;
           a1 += sc[0][l] * pl;
           b1 += sc[1][l] * pl;
           a3 += sc[2][l] * pl;
@@ -372,10 +396,11 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
           cos1 = 1.0;
           cos2 = -1.0;
         } else {
-          sin1 = st2 * sinphi / sin_scat;
-          sin2 = st1 * sinphi / sin_scat;
-          cos1 = (st1 * mu2 - st2 * mu1 * cosphi) / sin_scat;
-          cos2 = (st2 * mu1 - st1 * mu2 * cosphi) / sin_scat;
+          const double rs = 1.0 / sin_scat;
+          sin1 = st2 * sinphi * rs;
+          sin2 = st1 * sinphi * rs;
+          cos1 = (st1 * mu2 - st2 * mu1 * cosphi) * rs;
+          cos2 = (st2 * mu1 - st1 * mu2 * cosphi) * rs;
         }
         const double sin21 = 2.0 * sin1 * cos1, cos21 = 1.0 - 2.0 * sin1 * sin1;
         const double sin22 = 2.0 * sin2 * cos2, cos22 = 1.0 - 2.0 * sin2 * sin2;
