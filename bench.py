@@ -81,6 +81,34 @@ def measured_traffic(kernel):
     return None
 
 
+def serve(args):
+    """The serving mix at an offered Poisson rate: throughput and p50/p99
+    latency (arrival to result on the host, input staging included)."""
+    import numpy as np
+    from pamtra_amd import serving, workloads
+    rng = np.random.default_rng(1004)
+    t0 = time.time()
+    mix = workloads.serving_requests(args.serve_requests, seed=1004)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=len(mix)))
+    reqs = [serving.Request(pam=p, freqs=f, arrival=float(a), rid=i)
+            for i, ((p, f), a) in enumerate(zip(mix, arrivals))]
+    log(f"serving mix built in {time.time() - t0:.1f} s: {len(reqs)} requests")
+    with serving.ForwardServer(slots=args.serve_slots, hbm_budget_gb=32.0) as srv:
+        srv.run(reqs[:2], realtime=False)  # warm the slots' workspace and code
+        t0 = time.perf_counter()
+        res = srv.run(reqs, realtime=True)
+        wall = time.perf_counter() - t0
+        peak = srv.peak_reserved
+    summ = serving.summarize(res, reqs, wall)
+    summ.update({"offered_req_s": args.serve_rate, "slots": args.serve_slots,
+                 "hbm_budget_gib": 32.0, "peak_workspace_gib": round(peak / 2 ** 30, 3),
+                 "precision": "fp64",
+                 "mix": "1-4096 columns (log-uniform), 20-150 layers, 1-12 channels of "
+                        "10.65-325 GHz, 0-4 of cloud/ice/snow/graupel with 100 bins"})
+    log(f"serving: {summ}")
+    return summ
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +120,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-host-entry", action="store_true",
                     help="skip the pam_run (host arrays, PCIe-inclusive) measurement")
+    ap.add_argument("--serve-requests", type=int, default=100,
+                    help="requests of the serving mix (0: no serving run)")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="offered requests/s (Poisson)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving mix and print its summary")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (to scale HBM counters)")
     args = ap.parse_args()
@@ -103,6 +137,10 @@ def main():
 
     import torch
     from pamtra_amd import _capi, _lib, workloads
+
+    if args.serve_only:
+        print(json.dumps({"serving": serve(args)}))
+        return
 
     t0 = time.time()
     pam, freqs = workloads.config2(ncol=args.ncol, seed=1002)
@@ -218,6 +256,9 @@ def main():
         "pcie_inclusive_value": None if host_rate is None else round(host_rate, 2),
         "roofline": roof,
     }
+
+    if args.serve_requests > 0:
+        out["serving"] = serve(args)
 
     if not args.no_cpu:
         import oracle

@@ -179,7 +179,9 @@ int pam_context_destroy(pam_context *ctx);
  * the replacement of run_pamtra (src/pyPamtraLib.f90:119-133), host arrays in
  * and out.  freqs in GHz, sorted ascending (core.py:1536).  Returns the worst
  * status over all columns: PAM_SUCCESS, PAM_WARNING or PAM_FATAL; never
- * aborts the process (radmat.f90:180-189 STOPs, this returns PAM_FATAL). */
+ * aborts the process (radmat.f90:180-189 STOPs, this returns PAM_FATAL).
+ * The work runs on the context's own non-blocking stream: different
+ * contexts may run concurrently, one host thread per context. */
 int pam_run(pam_context *ctx, const pam_settings *settings,
             const pam_hydro_desc *hydros, const pam_hydro_4d *hydro4d,
             const pam_profile *profile, const double *freqs, int32_t nfreq,
@@ -208,6 +210,23 @@ const char *pam_stage_names(void);
  * products executed by the doubling, [10] the same for the adding,
  * [11] units recomputed by the Gauss-Jordan pass. */
 int pam_last_work(const pam_context *ctx, int64_t *counts, int32_t n);
+
+/* Device bytes one call needs for these dimensions, with `nitems` layers
+ * holding hydrometeors above hydro_threshold (an upper bound if nitems is).
+ * Used for admission by an HBM budget.  No counterpart in the reference,
+ * which runs one call per process (python/pyPamtra/core.py:1613-1647). */
+int64_t pam_workspace_bytes(int64_t ncol, int32_t max_nlyrs, int32_t nhydro,
+                            int32_t nbin_max, int32_t nfreq, int32_t noutlevels,
+                            int64_t nitems);
+
+/* Device bytes the context's grow-only workspace holds now.  No counterpart
+ * in the reference. */
+int64_t pam_context_bytes(const pam_context *ctx);
+
+/* Releases the context's grow-only workspace (the device's Mie tables stay).
+ * No counterpart in the reference (its module globals are reallocated per
+ * call, libWrapper.py:48). */
+int pam_context_trim(pam_context *ctx);
 
 #ifdef __cplusplus
 }

@@ -389,12 +389,20 @@ def declare_library(lib: C.CDLL) -> C.CDLL:
     lib.pam_stage_names.argtypes = []
     lib.pam_last_work.restype = C.c_int
     lib.pam_last_work.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int32]
+    lib.pam_workspace_bytes.restype = C.c_int64
+    lib.pam_workspace_bytes.argtypes = [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                        C.c_int32, C.c_int64]
+    lib.pam_context_bytes.restype = C.c_int64
+    lib.pam_context_bytes.argtypes = [C.c_void_p]
+    lib.pam_context_trim.restype = C.c_int
+    lib.pam_context_trim.argtypes = [C.c_void_p]
     return lib
 
 
 EXPORTED_SYMBOLS = ["pam_version", "pam_device_count", "pam_context_create",
                     "pam_context_destroy", "pam_run", "pam_run_device",
-                    "pam_last_stage_times", "pam_stage_names", "pam_last_work"]
+                    "pam_last_stage_times", "pam_stage_names", "pam_last_work",
+                    "pam_workspace_bytes", "pam_context_bytes", "pam_context_trim"]
 
 PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIBRARY_PATH = os.path.join(PACKAGE_DIR, "libpamtra_amd.so")

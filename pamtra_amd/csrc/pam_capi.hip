@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -53,10 +55,53 @@ struct DevBuf {
   }
 };
 
+// Mie angle tables of one device, shared by all its contexts and kept until
+// the process ends (66 MiB).
+struct AngleTables {
+  DevBuf buf;
+  const double *tab = nullptr;
+  const int64_t *off = nullptr;
+  int nmax = 0;
+};
+static std::mutex g_tab_mutex;
+static std::map<int, AngleTables *> g_tables;
+
+// Builds the tables of the current device once; nullptr on failure.
+static AngleTables *device_tables(int device) {
+  std::lock_guard<std::mutex> lock(g_tab_mutex);
+  auto it = g_tables.find(device);
+  if (it != g_tables.end()) return it->second;
+  AngleTables *t = new AngleTables();
+  int64_t off[MAX_NTERMS + 1];
+  int nmax = 0;
+  const int64_t n = angle_table_layout(off, &nmax);
+  const size_t tab_bytes = ((size_t)n * 8 + 255) & ~(size_t)255;
+  bool ok = t->buf.ensure(tab_bytes + sizeof(off)) == hipSuccess;
+  if (ok) {
+    double *tab = (double *)t->buf.p;
+    int64_t *doff = (int64_t *)((char *)t->buf.p + tab_bytes);
+    ok = hipMemcpy(doff, off, sizeof(int64_t) * (nmax + 1), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+      build_angle_tables(tab, doff, nmax, nullptr);
+      ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+    }
+    t->tab = tab;
+    t->off = doff;
+    t->nmax = nmax;
+  }
+  if (!ok) {
+    t->buf.release();
+    delete t;
+    return nullptr;
+  }
+  g_tables[device] = t;
+  return t;
+}
+
 struct pam_context {
   int device = 0;
+  hipStream_t stream = nullptr;  // the stream of pam_run (non-blocking)
   DevBuf colbuf, itembuf, iobuf, smallbuf;
-  DevBuf atab;  // Mie angle tables + their offsets (built once)
   const double *d_atab = nullptr;
   const int64_t *d_atab_off = nullptr;
   int atab_nmax = 0;
@@ -205,31 +250,15 @@ extern "C" int pam_context_create(pam_context **out, int device) {
     return PAM_FATAL;
   }
   {
-    // Mie angle tables for every supported series length
-    int64_t off[MAX_NTERMS + 1];
-    int nmax = 0;
-    const int64_t n = angle_table_layout(off, &nmax);
-    const size_t tab_bytes = ((size_t)n * 8 + 255) & ~(size_t)255;
-    bool ok = c->atab.ensure(tab_bytes + sizeof(off)) == hipSuccess;
-    if (ok) {
-      double *tab = (double *)c->atab.p;
-      int64_t *doff = (int64_t *)((char *)c->atab.p + tab_bytes);
-      ok = hipMemcpy(doff, off, sizeof(int64_t) * (nmax + 1), hipMemcpyHostToDevice) ==
-           hipSuccess;
-      if (ok) {
-        build_angle_tables(tab, doff, nmax, nullptr);
-        ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
-      }
-      c->d_atab = tab;
-      c->d_atab_off = doff;
-      c->atab_nmax = nmax;
-    }
-    if (!ok) {
-      c->atab.release();
+    AngleTables *t = device_tables(device);
+    if (!t || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
       (void)hipFree(c->d_mu);
       delete c;
       return PAM_FATAL;
     }
+    c->d_atab = t->tab;
+    c->d_atab_off = t->off;
+    c->atab_nmax = t->nmax;
   }
   // per-call workspace budget: half of the free HBM, at most 96 GiB
   // (PAM_WORKSPACE_GB, read at each call, overrides it)
@@ -249,7 +278,7 @@ extern "C" int pam_context_destroy(pam_context *c) {
   c->itembuf.release();
   c->iobuf.release();
   c->smallbuf.release();
-  c->atab.release();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->d_mu) (void)hipFree(c->d_mu);
   delete c;
   return PAM_SUCCESS;
@@ -266,6 +295,22 @@ extern "C" int pam_last_stage_times(const pam_context *c, double *ms, int32_t n)
 extern "C" int pam_last_work(const pam_context *c, int64_t *counts, int32_t n) {
   if (!c || !counts) return PAM_FATAL;
   for (int i = 0; i < n && i < PAM_NWORK; i++) counts[i] = c->work[i];
+  return PAM_SUCCESS;
+}
+
+extern "C" int64_t pam_context_bytes(const pam_context *c) {
+  if (!c) return -1;
+  return (int64_t)(c->colbuf.cap + c->itembuf.cap + c->iobuf.cap + c->smallbuf.cap);
+}
+
+extern "C" int pam_context_trim(pam_context *c) {
+  if (!c) return PAM_FATAL;
+  if (hipSetDevice(c->device) != hipSuccess) return PAM_FATAL;
+  if (c->stream && hipStreamSynchronize(c->stream) != hipSuccess) return PAM_FATAL;
+  c->colbuf.release();
+  c->itembuf.release();
+  c->iobuf.release();
+  c->smallbuf.release();
   return PAM_SUCCESS;
 }
 
@@ -334,6 +379,30 @@ static void carve_items(Work &w, void *base, const Dims &d, size_t nitems) {
   w.Sp = cv.take<double>(nif * NN);
   w.Sm = cv.take<double>(nif * NN);
   w.redo_dbl = cv.take<int32_t>(nif);
+}
+
+extern "C" int64_t pam_workspace_bytes(int64_t ncol, int32_t max_nlyrs, int32_t nhydro,
+                                       int32_t nbin_max, int32_t nfreq, int32_t noutlevels,
+                                       int64_t nitems) {
+  if (ncol <= 0 || max_nlyrs <= 0 || nhydro < 0 || nfreq <= 0 || noutlevels <= 0 || nitems < 0)
+    return -1;
+  Dims d;
+  memset(&d, 0, sizeof(d));
+  d.ncol = (int)ncol;
+  d.ML = max_nlyrs;
+  d.nh = nhydro;
+  d.nbmax = std::max(2, nbin_max);
+  d.nf = nfreq;
+  d.nout = noutlevels;
+  const size_t c = (size_t)ncol, L = max_nlyrs, nh = nhydro, nf = nfreq, no = noutlevels;
+  // pam_run staging: every profile and 4D descriptor array, and every result
+  size_t io = c * 4 * 4 + 4 * c * (L + 1) * 8 + 4 * c * L * 8 + 3 * c * L * nh * 8 + 5 * c * 8 +
+              c + c * no * 8 + 13 * c * L * nh * 8;
+  io += c * no * PAM_NANGLES * nf * 2 * 8 + c * 2 * nf * NMU * 8 + PAM_NANGLES * 8 + c * nf * 4 +
+        c * L * nf * (3 * 8 + 2 * 4);
+  const size_t small = 4096 + sizeof(pam_hydro_desc) * std::max<size_t>(1, nh) + 8 * nf + 64 * 8;
+  return (int64_t)(col_bytes(d) + (size_t)std::max<int64_t>(1, nitems) * item_bytes_per(d) +
+                   64 * 256 + io + 64 * 256 + small);
 }
 
 static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_desc *hydros,
@@ -633,7 +702,8 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
       a[j] = 180.0 - (180. * std::acos(c->mu[NMU - 1 - j]) / PI);
       a[NMU + j] = (180. * std::acos(c->mu[j]) / PI);
     }
-    HIP_OK(hipMemcpy(res->angles_deg, a, sizeof(a), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpyAsync(res->angles_deg, a, sizeof(a), hipMemcpyHostToDevice, stream));
+    HIP_OK(hipStreamSynchronize(stream));
   }
   return worst;
 }
@@ -726,10 +796,12 @@ extern "C" int pam_run(pam_context *c, const pam_settings *st, const pam_hydro_d
     set_msg(res, "pam_run: device allocation failed");
     return PAM_FATAL;
   }
+  // everything on the context's own non-blocking stream, so that contexts
+  // driven from different host threads overlap on the device
   Carver cv(c->iobuf.p);
   for (auto &a : in) {
     void *dptr = cv.take<char>(a.bytes);
-    if (hipMemcpy(dptr, a.h, a.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpyAsync(dptr, a.h, a.bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) {
       set_msg(res, "pam_run: copy to device failed");
       return PAM_FATAL;
     }
@@ -739,13 +811,17 @@ extern "C" int pam_run(pam_context *c, const pam_settings *st, const pam_hydro_d
     void *dptr = cv.take<char>(o.bytes);
     *o.slot = dptr;
   }
-  if (dr.tb) (void)hipMemset(dr.tb, 0, ncol * nout * PAM_NANGLES * nf * 2 * 8);
-  int rc = run_device(c, st, hyd, h4 ? &d4 : nullptr, &dp, freqs, nfreq, &dr, nullptr);
+  if (dr.tb) (void)hipMemsetAsync(dr.tb, 0, ncol * nout * PAM_NANGLES * nf * 2 * 8, c->stream);
+  int rc = run_device(c, st, hyd, h4 ? &d4 : nullptr, &dp, freqs, nfreq, &dr, c->stream);
   for (auto &o : out) {
-    if (hipMemcpy(o.h, *o.slot, o.bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    if (hipMemcpyAsync(o.h, *o.slot, o.bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) {
       set_msg(res, "pam_run: copy from device failed");
       return PAM_FATAL;
     }
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    set_msg(res, "pam_run: stream synchronisation failed");
+    return PAM_FATAL;
   }
   return rc;
 }

@@ -1,0 +1,282 @@
+"""Request loop of the passive forward operator on one device.
+
+The reference has no serving mode: each ``runPamtra`` call allocates its
+module globals, runs every (column, channel) and returns
+(``python/pyPamtra/core.py:1518-1568``); its only concurrency is one process
+per core (``core.py:1613-1647``).  Here a server keeps ``slots`` contexts of
+the C-ABI on one GPU, each with its own non-blocking stream and one host
+thread, so requests that arrive while others are on the device run
+alongside them:
+
+* admission: a request starts when a slot is free and the device workspace
+  of all slots, with this request's need (``pam_workspace_bytes``), fits the
+  HBM budget; idle slots give their workspace back (``pam_context_trim``)
+  when the budget is short;
+* the Mie angle tables are built once per device and shared by every slot;
+* each request is timed from its arrival to its result on the host
+  (latency), and the server reports throughput and latency percentiles.
+
+A request's result does not depend on what runs beside it: every unit of
+work is a (column, channel) of that request, computed by the same kernels on
+its own stream and workspace.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import queue
+import threading
+import time
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from . import _capi, _lib
+
+
+@dataclasses.dataclass
+class Request:
+    """One forward-operator call: a pyPamtra-like object (``p``, ``nmlSet``,
+    ``set``, ``df``) and its channels; ``arrival`` is seconds after the start
+    of ``ForwardServer.run``."""
+    pam: object
+    freqs: Sequence[float]
+    arrival: float = 0.0
+    rid: int = 0
+
+    @property
+    def columns(self) -> int:
+        return int(self.pam.p["ngridx"]) * int(self.pam.p["ngridy"])
+
+
+@dataclasses.dataclass
+class Result:
+    rid: int
+    status: int
+    message: str
+    tb: np.ndarray
+    arrival: float
+    start: float
+    end: float
+    slot: int
+    need_bytes: int
+
+    @property
+    def latency(self) -> float:
+        return self.end - self.arrival
+
+
+def scattering_layers_bound(pam) -> int:
+    """Upper bound of the layers that hold hydrometeors (the device's work
+    items): layers inside nlyrs with any positive content."""
+    p = pam.p
+    nl = np.asarray(p["nlyrs"]).reshape(-1)
+    L = int(p["max_nlyrs"])
+    inside = np.arange(L)[None, :] < nl[:, None]
+    any_h = np.zeros_like(inside)
+    for key in ("hydro_q", "hydro_n", "hydro_reff"):
+        a = p.get(key)
+        if isinstance(a, np.ndarray) and a.size:
+            a = a.reshape(nl.size, L, -1)
+            any_h |= np.any(a > 0.0, axis=-1)
+    return int(np.count_nonzero(any_h & inside))
+
+
+def workspace_bytes(lib, pam, nfreq: int) -> int:
+    """Device bytes of one call of this request (the library's own sizing)."""
+    p = pam.p
+    ncol = int(p["ngridx"]) * int(p["ngridy"])
+    nbins = [2 if str(r["dist_name"]).strip() == "mono" else int(r["nbin"])
+             for r in pam.df.data] or [2]
+    nb = max(nbins)
+    if "nbin" in pam.df.data4D:
+        nb = max(nb, int(np.max(pam.df.data4D["nbin"])))
+    n = lib.pam_workspace_bytes(ncol, int(p["max_nlyrs"]), len(pam.df.data), nb, nfreq,
+                                int(p["noutlevels"]), scattering_layers_bound(pam))
+    if n < 0:
+        raise ValueError("pam_workspace_bytes: invalid dimensions")
+    return int(n)
+
+
+def _run_on_context(lib, ctx, req: Request):
+    """One pam_run of the request on a slot's context; returns (status, msg, tb)."""
+    pam = req.pam
+    nml = dict(pam.nmlSet)
+    nml["active"] = False
+    m = _capi.marshal(nml, pam.set, pam.df.data, pam.df.data4D, pam.p)
+    f, fp = _capi.freqs_array(req.freqs)
+    hr = _capi.alloc_results(m, len(f))
+    st = lib.pam_run(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d), C.byref(m.profile),
+                     fp, len(f), C.byref(hr.struct))
+    return int(st), _capi.results_message(hr), hr.arrays["tb"]
+
+
+class ForwardServer:
+    """``slots`` concurrent contexts on ``device`` under an HBM budget.
+
+    ``runner(slot, request) -> (status, message, tb)`` replaces the device
+    call (used by the CPU tests of the admission logic); ``need(request)``
+    replaces the workspace sizing then.
+    """
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget_gb: float = 64.0,
+                 runner: Optional[Callable] = None, need: Optional[Callable] = None):
+        if slots < 1:
+            raise ValueError("slots must be >= 1")
+        self.slots = slots
+        self.budget = int(hbm_budget_gb * (1 << 30))
+        self._runner = runner
+        self._need = need
+        self._lib = None
+        self._ctx: List[C.c_void_p] = []
+        if runner is None:
+            self._lib = _lib.library()
+            if self._lib.pam_device_count() <= device:
+                raise _lib.PamLibraryError(f"no HIP device {device}")
+            for _ in range(slots):
+                h = C.c_void_p()
+                if self._lib.pam_context_create(C.byref(h), device) != 0 or not h.value:
+                    raise _lib.PamLibraryError("pam_context_create failed")
+                self._ctx.append(h)
+        # workspace each slot's context holds (grow-only until trimmed); the
+        # library rounds its allocations up by an eighth
+        self._held = [0] * slots
+        self.peak_reserved = 0
+
+    def close(self):
+        if self._lib is not None:
+            for h in self._ctx:
+                self._lib.pam_context_destroy(h)
+        self._ctx = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- sizing and the device call ------------------------------------
+    def need_bytes(self, req: Request) -> int:
+        if self._need is not None:
+            return int(self._need(req))
+        return workspace_bytes(self._lib, req.pam, len(req.freqs))
+
+    def _call(self, slot: int, req: Request):
+        if self._runner is not None:
+            return self._runner(slot, req)
+        return _run_on_context(self._lib, self._ctx[slot], req)
+
+    def _trim(self, slot: int):
+        if self._lib is not None:
+            self._lib.pam_context_trim(self._ctx[slot])
+        self._held[slot] = 0
+
+    def _admit(self, free: List[int], need: int) -> Optional[int]:
+        """A free slot whose workspace can grow to `need` within the budget
+        (idle slots give theirs back first), or None."""
+        slot = max(free, key=lambda s: self._held[s])  # reuse warm workspace
+
+        def total():
+            return sum(self._held) - self._held[slot] + max(self._held[slot], need)
+
+        for s in sorted(free, key=lambda s: -self._held[s]):
+            if total() <= self.budget:
+                break
+            if s != slot and self._held[s]:
+                self._trim(s)
+        if total() > self.budget and self._held[slot] > need:
+            self._trim(slot)
+        if total() > self.budget:
+            return None
+        self._held[slot] = max(self._held[slot], need)
+        self.peak_reserved = max(self.peak_reserved, sum(self._held))
+        return slot
+
+    # -- the loop ---------------------------------------------------------
+    def run(self, requests: Sequence[Request], realtime: bool = True) -> List[Result]:
+        """Serves `requests` (arrival times in seconds from now when
+        `realtime`, else all at once in order) and returns their results in
+        request order."""
+        order = sorted(range(len(requests)), key=lambda i: (requests[i].arrival, i))
+        needs = [self.need_bytes(r) for r in requests]
+        for i, n in enumerate(needs):
+            if n * 9 // 8 > self.budget:
+                raise ValueError(f"request {requests[i].rid} needs {n} B, over the "
+                                 f"HBM budget of {self.budget} B")
+        done: "queue.Queue" = queue.Queue()
+        jobs = [queue.Queue() for _ in range(self.slots)]
+        t0 = time.perf_counter()
+        now = (lambda: time.perf_counter() - t0) if realtime else (lambda: 0.0)
+
+        def worker(slot):
+            while True:
+                item = jobs[slot].get()
+                if item is None:
+                    return
+                i, t_admit = item
+                try:
+                    st, msg, tb = self._call(slot, requests[i])
+                except Exception as e:  # surfaced on the host, the loop goes on
+                    st, msg, tb = _capi.PAM_FATAL, f"{type(e).__name__}: {e}", None
+                done.put((i, slot, st, msg, tb, t_admit, now()))
+
+        threads = [threading.Thread(target=worker, args=(s,), daemon=True)
+                   for s in range(self.slots)]
+        for th in threads:
+            th.start()
+        free = list(range(self.slots))
+        busy = {}
+        results: List[Optional[Result]] = [None] * len(requests)
+        nxt = 0
+        try:
+            while nxt < len(order) or busy:
+                # admit in arrival order while a slot and memory are free
+                blocked = False
+                while nxt < len(order) and free:
+                    i = order[nxt]
+                    if realtime and requests[i].arrival > now():
+                        break
+                    slot = self._admit(free, needs[i] * 9 // 8)
+                    if slot is None:
+                        blocked = True  # wait for a running request to finish
+                        break
+                    free.remove(slot)
+                    busy[slot] = i
+                    jobs[slot].put((i, now()))
+                    nxt += 1
+                # wait for a completion, or for the next arrival
+                timeout = None
+                if nxt < len(order) and free and not blocked and realtime:
+                    timeout = max(0.0, requests[order[nxt]].arrival - now())
+                try:
+                    i, slot, st, msg, tb, t_admit, t_end = done.get(timeout=timeout)
+                except queue.Empty:
+                    continue
+                r = requests[i]
+                results[i] = Result(r.rid, st, msg, tb, r.arrival if realtime else 0.0,
+                                    t_admit, t_end, slot, needs[i])
+                if self._lib is not None:  # what the slot's context really holds now
+                    self._held[slot] = int(self._lib.pam_context_bytes(self._ctx[slot]))
+                del busy[slot]
+                free.append(slot)
+        finally:
+            for q in jobs:
+                q.put(None)
+            for th in threads:
+                th.join()
+        return results  # type: ignore[return-value]
+
+
+def summarize(results: Sequence[Result], requests: Sequence[Request], wall: float) -> dict:
+    """Throughput (columns*channels/s over the wall time) and latency
+    percentiles (ms) of a served set."""
+    lat = np.array([r.latency for r in results]) * 1e3
+    work = sum(q.columns * len(q.freqs) for q in requests)
+    return {"requests": len(results),
+            "columns_channels": int(work),
+            "wall_s": round(wall, 4),
+            "throughput": round(work / wall, 2) if wall > 0 else None,
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "failed": int(sum(1 for r in results if r.status >= _capi.PAM_FATAL))}

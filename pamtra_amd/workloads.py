@@ -7,6 +7,11 @@ device for ``pam_run_device``.
            (modified gamma) + rain (Marshall-Palmer), 50 bins, ocean.
   config2: 16384 columns, 20-120 layers, tops 10-20 km, 7 channels,
            cloud + rain with 80 bins, ocean (TESSEM2) or Lambertian land.
+  config3: the config2 shapes with four categories of 100 bins: cloud
+           liquid (modified gamma), cloud ice (917 kg/m3 spheres), snow (soft
+           spheres, 50-300 kg/m3 per column) and graupel (400 kg/m3).
+  serving_requests: the serving mix, requests of 1-4096 columns with
+           20-150 layers, 1-12 channels and 0-4 of the config3 categories.
 Random draws use numpy.random.default_rng(seed), fields in a fixed order.
 """
 
@@ -146,3 +151,105 @@ def config2(ncol=16384, seed=1002, nbin=80, freqs=None):
                             pam.p["hgt"]) if pam.p["hgt"].ndim == 3 else pam.p["hgt"]
     _common_nml(pam)
     return pam, list(CONFIG2_FREQS if freqs is None else freqs)
+
+
+CONFIG3_NBIN = 100
+SERVING_FREQS = [10.65, 18.7, 22.235, 23.8, 31.4, 36.5, 50.3, 89.0, 150.0, 166.0, 183.31, 325.0]
+
+
+def _descriptor4(pam, nbin):
+    """Cloud liquid, cloud ice, snow, graupel (mass from the density, so the
+    exponential spectra take N0 via p_3).  Large particles reach 13 mm, a
+    size parameter of ~25 at 183.31 GHz."""
+    pam.df.addHydrometeor(("cwc_q", -99.0, 1, -99.0, -99.0, -99.0, -99.0, -99.0, 3, nbin,
+                           "mgamma", 1.0e8, -99.0, 2.0, 1.0, 1e-6, 6e-3, "mie-sphere",
+                           "khvorostyanov01_drops", -99.0))
+    pam.df.addHydrometeor(("iwc_q", -99.0, -1, 917.0, -99.0, -99.0, -99.0, -99.0, 3, nbin,
+                           "exp", -99.0, -99.0, 1.0e7, -99.0, 1e-6, 1.3e-2, "mie-sphere",
+                           "heymsfield10_particles", -99.0))
+    pam.df.addHydrometeor(("swc_q", -99.0, -1, 100.0, -99.0, -99.0, -99.0, -99.0, 3, nbin,
+                           "exp", -99.0, -99.0, 3.0e6, -99.0, 1e-5, 1.3e-2, "mie-sphere",
+                           "heymsfield10_particles", -99.0))
+    pam.df.addHydrometeor(("gwc_q", -99.0, -1, 400.0, -99.0, -99.0, -99.0, -99.0, 3, nbin,
+                           "exp", -99.0, -99.0, 4.0e6, -99.0, 1e-5, 1.3e-2, "mie-sphere",
+                           "heymsfield10_particles", -99.0))
+
+
+def _hydro_q4(rng, hgt, t, p, ncol, max_nlyrs, nlyrs, present=None):
+    """Contents (kg/kg) of the four categories: cloud LWC log-U(1e-3, 0.5)
+    g/m3 in 1-3 km; ice IWC log-U(1e-4, 0.2) g/m3 in 6-11 km; snow log-U(1e-3,
+    0.5) g/m3 in 3-8 km; graupel log-U(1e-3, 0.5) g/m3 in 2-6 km.  `present`
+    (ncol, 4) switches categories off per column."""
+    rho = p / (R_D * t)
+    valid = np.arange(max_nlyrs)[None, :] < nlyrs[:, None]
+    spec = [(-3.0, np.log10(0.5), 1000.0, 3000.0), (-4.0, np.log10(0.2), 6000.0, 11000.0),
+            (-3.0, np.log10(0.5), 3000.0, 8000.0), (-3.0, np.log10(0.5), 2000.0, 6000.0)]
+    q = np.zeros((ncol, max_nlyrs, 4))
+    for h, (lo, hi, z0, z1) in enumerate(spec):
+        wc = 10.0 ** rng.uniform(lo, hi, size=(ncol, max_nlyrs)) * 1e-3
+        inside = (hgt >= z0) & (hgt <= z1) & valid
+        if present is not None:
+            inside &= present[:, h][:, None]
+        q[..., h] = np.where(inside, wc / rho, 0.0)
+    return q
+
+
+def _profile_columns(rng, ncol, nlyrs, tops, ML, nbin, present=None):
+    pam = pyPamtra()
+    _descriptor4(pam, nbin)
+    hgt_lev, t_lev, p_lev, rh_lev = _columns(rng, ncol, nlyrs, tops, ML)
+    hgt = 0.5 * (hgt_lev[:, 1:] + hgt_lev[:, :-1])
+    t = 0.5 * (t_lev[:, 1:] + t_lev[:, :-1])
+    p = np.exp(0.5 * (np.log(p_lev[:, 1:]) + np.log(p_lev[:, :-1])))
+    q = _hydro_q4(rng, hgt, t, p, ncol, ML, nlyrs, present)
+    land = rng.uniform(size=ncol) < 0.5
+    eps = np.round(rng.uniform(0.85, 0.95, size=ncol), 2)
+    sst = rng.uniform(280.0, 300.0, size=ncol)
+    rho_snow = rng.uniform(50.0, 300.0, size=ncol)
+    pam.createProfile(hgt_lev=hgt_lev, temp_lev=t_lev, press_lev=p_lev, relhum_lev=rh_lev,
+                      hydro_q=q, groundtemp=sst,
+                      sfc_type=np.where(land, -9999, 0), sfc_model=np.where(land, -9999, 0),
+                      sfc_refl=np.where(land, "L", "S"), sfc_emissivity=eps,
+                      sfc_salinity=np.full(ncol, 33.0), wind10u=np.zeros(ncol),
+                      wind10v=np.zeros(ncol), timestamp=0)
+    # per-column snow density (soft spheres) through the 4D descriptor
+    rho_ms = np.empty((ncol, 1, ML, 4))
+    rho_ms[..., 0] = -99.0
+    rho_ms[..., 1] = 917.0
+    rho_ms[..., 2] = rho_snow[:, None, None]
+    rho_ms[..., 3] = 400.0
+    pam.df.add4D("rho_ms", rho_ms)
+    _common_nml(pam)
+    return pam
+
+
+def config3(ncol=16384, seed=1003, nbin=CONFIG3_NBIN, freqs=None):
+    """Mixed-precision shapes: config2 grid and channels, four categories."""
+    rng = np.random.default_rng(seed)
+    ML = 120
+    nlyrs = rng.integers(20, 121, size=ncol)
+    tops = rng.uniform(10000.0, 20000.0, size=ncol)
+    pam = _profile_columns(rng, ncol, nlyrs, tops, ML, nbin)
+    return pam, list(CONFIG2_FREQS if freqs is None else freqs)
+
+
+def serving_requests(n, seed=1004, max_cols=4096, nbin=CONFIG3_NBIN):
+    """n requests of the serving mix: columns log-uniform in [1, max_cols],
+    20-150 layers per column, 1-12 channels drawn from SERVING_FREQS, 0-4
+    categories.  Returns a list of (pyPamtra, freqs)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        ncol = int(np.clip(np.round(np.exp(rng.uniform(0.0, np.log(max_cols)))), 1, max_cols))
+        nlyrs = rng.integers(20, 151, size=ncol)
+        ML = int(nlyrs.max())
+        tops = rng.uniform(10000.0, 20000.0, size=ncol)
+        nch = int(rng.integers(1, 13))
+        freqs = sorted(rng.choice(SERVING_FREQS, size=nch, replace=False).tolist())
+        ncat = int(rng.integers(0, 5))
+        cats = rng.choice(4, size=ncat, replace=False)
+        present = np.zeros((ncol, 4), dtype=bool)
+        present[:, cats] = True
+        pam = _profile_columns(rng, ncol, nlyrs, tops, ML, nbin, present)
+        out.append((pam, freqs))
+    return out
