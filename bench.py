@@ -93,7 +93,7 @@ def serve(args):
     reqs = [serving.Request(pam=p, freqs=f, arrival=float(a), rid=i)
             for i, ((p, f), a) in enumerate(zip(mix, arrivals))]
     log(f"serving mix built in {time.time() - t0:.1f} s: {len(reqs)} requests")
-    with serving.ForwardServer(slots=args.serve_slots, hbm_budget_gb=32.0) as srv:
+    with serving.ForwardServer(slots=args.serve_slots, hbm_budget_gb=64.0) as srv:
         srv.run(reqs[:2], realtime=False)  # warm the slots' workspace and code
         t0 = time.perf_counter()
         res = srv.run(reqs, realtime=True)
@@ -101,7 +101,7 @@ def serve(args):
         peak = srv.peak_reserved
     summ = serving.summarize(res, reqs, wall)
     summ.update({"offered_req_s": args.serve_rate, "slots": args.serve_slots,
-                 "hbm_budget_gib": 32.0, "peak_workspace_gib": round(peak / 2 ** 30, 3),
+                 "hbm_budget_gib": 64.0, "peak_workspace_gib": round(peak / 2 ** 30, 3),
                  "precision": "fp64",
                  "mix": "1-4096 columns (log-uniform), 20-150 layers, 1-12 channels of "
                         "10.65-325 GHz, 0-4 of cloud/ice/snow/graupel with 100 bins"})

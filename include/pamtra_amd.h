@@ -219,6 +219,12 @@ int64_t pam_workspace_bytes(int64_t ncol, int32_t max_nlyrs, int32_t nhydro,
                             int32_t nbin_max, int32_t nfreq, int32_t noutlevels,
                             int64_t nitems);
 
+/* Caps the per-layer optics workspace of each call on this context (the
+ * columns are then processed in chunks that fit).  Default: half of the free
+ * HBM at creation, at most 96 GiB; PAM_WORKSPACE_GB overrides both.  No
+ * counterpart in the reference. */
+int pam_context_set_workspace_limit(pam_context *ctx, int64_t bytes);
+
 /* Device bytes the context's grow-only workspace holds now.  No counterpart
  * in the reference. */
 int64_t pam_context_bytes(const pam_context *ctx);

@@ -392,6 +392,8 @@ def declare_library(lib: C.CDLL) -> C.CDLL:
     lib.pam_workspace_bytes.restype = C.c_int64
     lib.pam_workspace_bytes.argtypes = [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                         C.c_int32, C.c_int64]
+    lib.pam_context_set_workspace_limit.restype = C.c_int
+    lib.pam_context_set_workspace_limit.argtypes = [C.c_void_p, C.c_int64]
     lib.pam_context_bytes.restype = C.c_int64
     lib.pam_context_bytes.argtypes = [C.c_void_p]
     lib.pam_context_trim.restype = C.c_int
@@ -402,7 +404,8 @@ def declare_library(lib: C.CDLL) -> C.CDLL:
 EXPORTED_SYMBOLS = ["pam_version", "pam_device_count", "pam_context_create",
                     "pam_context_destroy", "pam_run", "pam_run_device",
                     "pam_last_stage_times", "pam_stage_names", "pam_last_work",
-                    "pam_workspace_bytes", "pam_context_bytes", "pam_context_trim"]
+                    "pam_workspace_bytes", "pam_context_set_workspace_limit", "pam_context_bytes",
+                    "pam_context_trim"]
 
 PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIBRARY_PATH = os.path.join(PACKAGE_DIR, "libpamtra_amd.so")

@@ -298,6 +298,12 @@ extern "C" int pam_last_work(const pam_context *c, int64_t *counts, int32_t n) {
   return PAM_SUCCESS;
 }
 
+extern "C" int pam_context_set_workspace_limit(pam_context *c, int64_t bytes) {
+  if (!c || bytes <= 0) return PAM_FATAL;
+  c->budget_bytes = (size_t)bytes;
+  return PAM_SUCCESS;
+}
+
 extern "C" int64_t pam_context_bytes(const pam_context *c) {
   if (!c) return -1;
   return (int64_t)(c->colbuf.cap + c->itembuf.cap + c->iobuf.cap + c->smallbuf.cap);

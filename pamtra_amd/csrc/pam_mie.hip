@@ -22,7 +22,6 @@
 namespace pam {
 
 constexpr int MIE_WG = 256;
-constexpr int AB_CAP = 1024;  // (a_n, b_n) pairs staged in LDS per chunk
 static_assert(MAX_NQUAD <= MIE_WG && MAXNLEG <= MIE_WG, "one thread per angle / order");
 
 // nlegen and nquad of a series of length N (mie_spheres.f90:139-158)
@@ -157,20 +156,56 @@ void build_angle_tables(double *tab, const int64_t *d_off, int nmax, hipStream_t
   hipLaunchKernelGGL(k_angle_tables, dim3((unsigned)nmax), dim3(MIE_WG), 0, st, tab, d_off);
 }
 
-__global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hydro_desc *hyd,
+// Phase B as a matrix product on the FP64 matrix cores.  For a chunk of Bc
+// bins the coefficient matrix U (2 Bc rows: real and imaginary parts; Kp =
+// 2 Np columns: c_n a_n for n < Np, then c_n b_n) times
+//   P1 = [pi; tau] and P2 = [tau; pi]   (Kp x nquad, from the angle table)
+// gives Re/Im of S1 and S2 at every Gauss angle: S1 = U P1, S2 = U P2.
+// U rows are ordered per 16-row tile as [re b0..b3, im b0..b3, re b4..b7,
+// im b4..b7], so that each lane's MFMA result holds the real and imaginary
+// parts of two bins at one angle.
+constexpr int U_CAP = 4096;       // doubles of U staged in LDS per chunk
+constexpr int MIE_MAXBIN = 512;   // size bins per hydrometeor supported by the chunk plan
+constexpr int MIE_MAXCHUNK = 160;
+
+__device__ __forceinline__ int u_row_re(int j) { return 16 * (j >> 3) + 8 * ((j & 7) >> 2) + (j & 3); }
+
+// 1/x from the hardware reciprocal and two Newton steps (to ~1 ulp)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// a / b for the Mie recurrences, with one reciprocal of |b|^2 (the operands
+// stay far from the overflow range that Smith's algorithm guards against)
+__device__ __forceinline__ cd cdiv_m(cd a, cd b) {
+  const double r = rcp_nr(b.re * b.re + b.im * b.im);
+  return cd{(a.re * b.re + a.im * b.im) * r, (a.im * b.re - a.re * b.im) * r};
+}
+
+typedef double d4m_t __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(3))) k_mie(Dims d, Settings s, const pam_hydro_desc *hyd,
                                                 const double *freqs, Work w) {
-  __shared__ cd s_a[AB_CAP], s_b[AB_CAP];
-  __shared__ double s_q[3][MIE_WG];     // per-bin efficiencies; later the weighted sums
-  __shared__ double s_part[3][MIE_WG];  // phase sums per (bin group, angle)
+  __shared__ double s_u[U_CAP];
+  __shared__ double s_q[3][MIE_WG];  // per-bin efficiencies of the chunk; later the phase sums
+  __shared__ double s_wb[MIE_WG];    // per-bin DSD weight n(D) dD of the chunk, 0 if skipped
+  __shared__ double s_ps[3][MAX_NQUAD];  // weighted phase sums per angle
   __shared__ double s_lc[NCOEF][MAXNLEG + 1];
-  __shared__ int s_nm[MIE_WG];
+  __shared__ short s_nmb[MIE_MAXBIN];  // series length of every bin (0: skipped)
+  __shared__ short s_cb[MIE_MAXCHUNK + 1], s_cnp[MIE_MAXCHUNK];
+  __shared__ int s_nc;
   __shared__ double s_sum[3];
   __shared__ int s_nlh, s_nan;
 
   const int t = threadIdx.x;
+  const int wv = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
   const int itf = blockIdx.x;
-  const int li = itf / d.nf, f = itf % d.nf;
-  const int col = w.item_col[li], z = w.item_layer[li];
+  const int li_item = itf / d.nf, f = itf % d.nf;
+  const int col = w.item_col[li_item], z = w.item_layer[li_item];
   int32_t *st = w.itf_status + itf;
   const size_t oy = (size_t)col * d.ML + z;
   const double T = w.temp[oy];
@@ -183,10 +218,10 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
   long long work_bins = 0, work_series = 0;
 
   for (int hh = 0; hh < d.nh; hh++) {
-    const int nbin = w.dsd_flag[(size_t)li * d.nh + hh];
+    const int nbin = w.dsd_flag[(size_t)li_item * d.nh + hh];
     if (nbin <= 0) continue;
     const pam_hydro_desc &hd = hyd[hh];
-    const size_t ob = ((size_t)li * d.nh + hh) * d.nbmax;
+    const size_t ob = ((size_t)li_item * d.nh + hh) * d.nbmax;
     const double *dia = w.dsd_d + ob, *dd = w.dsd_dd + ob, *nd = w.dsd_nd + ob,
                  *rho = w.dsd_rho + ob;
     // refractive index (scatProperties.f90:227-243)
@@ -200,7 +235,8 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
     // series length from the largest bin (mie_spheres.f90:139-158)
     const double xmax = PI * dia[nbin - 1] / wavelength;
     const int nterms = mie_nstop(xmax);
-    if (nterms < 1 || nterms > w.atab_nmax) {
+    if (nterms < 1 || nterms > w.atab_nmax || 16 * (2 * ((nterms + 3) & ~3) + 1) > U_CAP ||
+        nbin > MIE_MAXBIN) {
       if (t == 0) set_err(st, E_MIE_TERMS);
       return;
     }
@@ -208,21 +244,52 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
     const int nquad = mie_nquad(nterms);
     const double *tmu = w.atab + w.atab_off[nterms], *twt = tmu + nquad, *tpi = twt + nquad,
                  *ttau = tpi + (size_t)nterms * nquad, *tpl = ttau + (size_t)nterms * nquad;
-    // phase B layout: thread = (bin group g, angle a)
-    const int ngrp = MIE_WG / nquad;
-    const int g = t / nquad, a = t % nquad;
-    const bool angle_thread = g < ngrp;
-    double acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+    (void)tmu;
+    const int nct = (nquad + 15) >> 4;
+    for (int k = t; k < 3 * MAX_NQUAD; k += MIE_WG) (&s_ps[0][0])[k] = 0.0;
     double sumqe = 0.0, sumqs = 0.0, sumqb = 0.0;
+    // chunk plan: consecutive bins (ascending size) as long as their rows of
+    // U, with the chunk's own series length, fit the LDS stage
+    for (int e = t; e < nbin; e += MIE_WG)
+      s_nmb[e] = nd[e] > 0.0 ? (short)min(mie_nstop(PI * dia[e] / wavelength), nterms) : 0;
+    __syncthreads();
+    if (t == 0) {
+      int nc = 0, b = 0;
+      while (b < nbin && nc < MIE_MAXCHUNK) {
+        int nmax = 0, e = b;
+        while (e < nbin && e - b < MIE_WG) {
+          const int nm2 = max(nmax, (int)s_nmb[e]);
+          const int npc = max(4, (nm2 + 3) & ~3);
+          if (16 * ((e - b + 8) >> 3) * (2 * npc + 1) > U_CAP) break;
+          nmax = nm2;
+          e++;
+        }
+        s_cb[nc] = (short)b;
+        s_cnp[nc] = (short)max(4, (nmax + 3) & ~3);
+        nc++;
+        b = e;
+      }
+      s_cb[nc] = (short)b;
+      s_nc = (b < nbin) ? -1 : nc;
+    }
+    __syncthreads();
+    if (s_nc < 0) {
+      if (t == 0) set_err(st, E_MIE_TERMS);
+      return;
+    }
 
-    const int chunk = min(MIE_WG, AB_CAP / nterms);
-    for (int b0 = 0; b0 < nbin; b0 += chunk) {
-      const int nb = min(chunk, nbin - b0);
+    for (int c = 0; c < s_nc; c++) {
+      const int b0 = s_cb[c], nb = s_cb[c + 1] - b0;
+      const int np = s_cnp[c], kp = 2 * np, ldu = kp + 1;
+      const int nrt = (nb + 7) >> 3;
+      for (int e = t; e < 16 * nrt * ldu; e += MIE_WG) s_u[e] = 0.0;
+      __syncthreads();
       // --- A: threads over bins (miecalc, miecross)
       if (t < nb) {
         const int ir = b0 + t;
         double q3[3] = {0.0, 0.0, 0.0};
         int nmie = 0;
+        double wgt = 0.0;
         if (nd[ir] > 0.0) {
           // soft sphere: index of the ice-air mixture at the bin's density
           // (mie_spheres.f90:187-192), decided per bin
@@ -235,39 +302,51 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
           }
           cd mc = conj(msp);
           cd y = mc * x;
-          cd *A = s_a + t * nterms, *B = s_b + t * nterms;
-          // downward log-derivative recursion, D_n kept in the a-slot
+          const cd yinv = cdiv_m(cd{1.0, 0.0}, y), mcinv = cdiv_m(cd{1.0, 0.0}, mc);
+          const double xinv = rcp_nr(x);
+          double *Ure = s_u + u_row_re(t) * ldu, *Uim = Ure + 4 * ldu;
+          // downward log-derivative recursion (mie_scat_utilities.f90:44-50),
+          // D_n kept in the a-columns of the bin's rows
           int nn = nmie + 15;
           cd dn = cd{0.0, 0.0};
           for (int n = nn; n >= 2; n--) {
-            cd ny = cdiv(cd{(double)n, 0.0}, y);
-            dn = ny - cdiv(cd{1.0, 0.0}, dn + ny);
-            if (n - 1 <= nmie) A[n - 2] = dn;
+            cd ny = (double)n * yinv;
+            dn = ny - cdiv_m(cd{1.0, 0.0}, dn + ny);
+            if (n - 1 <= nmie) {
+              Ure[n - 2] = dn.re;
+              Uim[n - 2] = dn.im;
+            }
           }
           double psim = cos(x), psin = sin(x), chim = -sin(x), chin = cos(x);
           double s1 = 0.0, s2 = 0.0;
           cd s3 = cd{0.0, 0.0};
           for (int n = 1; n <= nmie; n++) {
+            const double fn = (double)(2 * n - 1) * xinv;
             double tmp = psin;
-            psin = (double)(2 * n - 1) / x * psin - psim;
+            psin = fn * psin - psim;
             psim = tmp;
             tmp = chin;
-            chin = (double)(2 * n - 1) / x * chin - chim;
+            chin = fn * chin - chim;
             chim = tmp;
             cd xin = cd{sp(psin), sp(-chin)};
             cd xim = cd{sp(psim), sp(-chim)};
-            cd Dn = A[n - 1];
-            cd ct = cdiv(Dn, mc) + (double)n / x;
-            cd an = cdiv(ct * psin - psim, ct * xin - xim);
-            ct = mc * Dn + (double)n / x;
-            cd bn = cdiv(ct * psin - psim, ct * xin - xim);
-            A[n - 1] = an;
-            B[n - 1] = bn;
+            cd Dn = cd{Ure[n - 1], Uim[n - 1]};
+            const double nx = (double)n * xinv;
+            cd ct = Dn * mcinv + nx;
+            cd an = cdiv_m(ct * psin - psim, ct * xin - xim);
+            ct = mc * Dn + nx;
+            cd bn = cdiv_m(ct * psin - psim, ct * xin - xim);
             double wn = (double)(2 * n + 1);
             s1 = s1 + wn * (an.re + bn.re);
             s2 = s2 + wn * ((an * conj(an)).re + (bn * conj(bn)).re);
             double sg = (n & 1) ? -1.0 : 1.0;
             s3 = s3 + (wn * sg) * (an - bn);
+            // c_n in single precision (mie_scat_utilities.f90:144)
+            const double c = (double)((float)(2 * n + 1) / (float)(n * (n + 1)));
+            Ure[n - 1] = c * an.re;
+            Uim[n - 1] = c * an.im;
+            Ure[np + n - 1] = c * bn.re;
+            Uim[np + n - 1] = c * bn.im;
           }
           double r = dia[ir] / 2.0;
           double ar = (r * r) * PI;
@@ -276,55 +355,75 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
           q3[0] = qe * ar * nd[ir] * dd[ir];
           q3[1] = qs * ar * nd[ir] * dd[ir];
           q3[2] = qb * ar * nd[ir] * dd[ir];
+          wgt = nd[ir] * dd[ir];
         }
         s_q[0][t] = q3[0];
         s_q[1][t] = q3[1];
         s_q[2][t] = q3[2];
-        s_nm[t] = nmie;
+        s_wb[t] = nmie > 0 ? wgt : 0.0;
+      } else if (t < 8 * nrt) {
+        s_wb[t] = 0.0;
       }
       __syncthreads();
-      // --- B: threads over (bin group, angle): S1, S2 (mieangle) and the
-      // DSD-weighted phase-function sums
-      if (angle_thread) {
-        for (int cb = g; cb < nb; cb += ngrp) {
-          const int nmie = s_nm[cb];
-          if (nmie <= 0) continue;
-          const int ir = b0 + cb;
-          const double wgt = nd[ir] * dd[ir];
-          const cd *A = s_a + cb * nterms, *B = s_b + cb * nterms;
-          cd S1 = cd{0.0, 0.0}, S2 = cd{0.0, 0.0};
-          for (int n = 1; n <= nmie; n++) {
-            const double pin = tpi[(size_t)(n - 1) * nquad + a];
-            const double taun = ttau[(size_t)(n - 1) * nquad + a];
-            const double c = (double)((float)(2 * n + 1) / (float)(n * (n + 1)));
-            const cd an = A[n - 1], bn = B[n - 1];
-            S1 = S1 + c * (an * pin + bn * taun);
-            S2 = S2 + c * (bn * pin + an * taun);
+      // --- B: S1, S2 of every (bin, angle) on the matrix cores and the
+      // DSD-weighted phase-function sums (mieangle, mie_spheres.f90:238-247).
+      // Wave wv owns the angle tiles ct = wv, wv + 4, ...
+      for (int ct = wv; ct < nct; ct += 4) {
+        const int ang = 16 * ct + li;
+        const bool ang_ok = ang < nquad;
+        double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+        for (int rt = 0; rt < nrt; rt++) {
+          d4m_t acc1 = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+          const double *urow = s_u + (16 * rt + li) * ldu;
+          for (int sk = 0; sk < kp / 4; sk++) {
+            const int kk = 4 * sk + lk;
+            const bool ablk = kk < np;
+            const int n = ablk ? kk : kk - np;
+            double pv = 0.0, tv = 0.0;
+            if (ang_ok && n < nterms) {
+              pv = tpi[(size_t)n * nquad + ang];
+              tv = ttau[(size_t)n * nquad + ang];
+            }
+            const double a = urow[kk];
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? pv : tv, acc1, 0, 0, 0);
+            acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? tv : pv, acc2, 0, 0, 0);
           }
-          const double a1 = cabs_(S1), a2 = cabs_(S2);
-          const double p1 = 0.5 * (a2 * a2 + a1 * a1);
-          const double p2 = 0.5 * (a2 * a2 - a1 * a1);
-          const double p3 = (conj(S2) * S1).re;
-          acc1 = acc1 + p1 * wgt;
-          acc2 = acc2 + p2 * wgt;
-          acc3 = acc3 + p3 * wgt;
+#pragma unroll
+          for (int h2 = 0; h2 < 2; h2++) {
+            const double wgt = s_wb[8 * rt + 4 * h2 + lk];
+            const double s1r = acc1[2 * h2], s1i = acc1[2 * h2 + 1];
+            const double s2r = acc2[2 * h2], s2i = acc2[2 * h2 + 1];
+            const double a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
+            p1 += (0.5 * (a2 + a1)) * wgt;
+            p2 += (0.5 * (a2 - a1)) * wgt;
+            p3 += (s2r * s1r + s2i * s1i) * wgt;
+          }
+        }
+        // the four lane groups hold different bins of the same angle
+        p1 += __shfl_xor(p1, 16, 64);
+        p1 += __shfl_xor(p1, 32, 64);
+        p2 += __shfl_xor(p2, 16, 64);
+        p2 += __shfl_xor(p2, 32, 64);
+        p3 += __shfl_xor(p3, 16, 64);
+        p3 += __shfl_xor(p3, 32, 64);
+        if (lk == 0 && ang_ok) {  // this wave alone owns the angle
+          s_ps[0][ang] += p1;
+          s_ps[1][ang] += p2;
+          s_ps[2][ang] += p3;
         }
       }
       if (t == 0) {  // efficiency sums in bin order
         for (int cb = 0; cb < nb; cb++) {
-          if (s_nm[cb] <= 0) continue;
+          if (s_nmb[b0 + cb] <= 0) continue;
           sumqe = sumqe + s_q[0][cb];
           sumqs = sumqs + s_q[1][cb];
           sumqb = sumqb + s_q[2][cb];
           work_bins++;
-          work_series += s_nm[cb];
+          work_series += s_nmb[b0 + cb];
         }
       }
       __syncthreads();
     }
-    s_part[0][t] = acc1;
-    s_part[1][t] = acc2;
-    s_part[2][t] = acc3;
     if (t == 0) {
       s_sum[0] = sumqe;
       s_sum[1] = sumqs;
@@ -343,19 +442,13 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
       if (t == 0) set_err(st, E_MIE_RESULT);
       return;
     }
-    // weighted phase sums per angle (groups combined in group order)
+    // weighted phase sums per angle, normalised (mie_spheres.f90:272-280)
     const double tmp = wavelength * wavelength / (PI * scatter);
     if (t < nquad) {
-      double v1 = 0.0, v2 = 0.0, v3 = 0.0;
-      for (int gg = 0; gg < ngrp; gg++) {
-        v1 = v1 + s_part[0][gg * nquad + t];
-        v2 = v2 + s_part[1][gg * nquad + t];
-        v3 = v3 + s_part[2][gg * nquad + t];
-      }
       const double wt = twt[t];
-      s_q[0][t] = tmp * v1 * wt;
-      s_q[1][t] = tmp * v2 * wt;
-      s_q[2][t] = tmp * v3 * wt;
+      s_q[0][t] = tmp * s_ps[0][t] * wt;
+      s_q[1][t] = tmp * s_ps[1][t] * wt;
+      s_q[2][t] = tmp * s_ps[2][t] * wt;
     }
     __syncthreads();
     // --- C: Legendre projection (mie_spheres.f90:272-314), thread per order

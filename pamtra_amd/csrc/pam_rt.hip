@@ -155,8 +155,10 @@ __device__ __forceinline__ double mv_pair(const double *A, const double *x, int 
   return s + __shfl_xor(s, 1, 64);
 }
 
-// LDS scratch of the inverses
+// LDS scratch of the inverses; gj_only: Gauss-Jordan without trying the
+// series first (set by the queue pass in the Gauss-Jordan test mode)
 struct InvLds {
+  int gj_only;
   float wmax[4];
   double red[4];
   int redi[4];
@@ -247,49 +249,59 @@ __device__ bool inv_i_minus(const double *X, double *G, InvLds &sh) {
 constexpr double NEU_TOL = 0x1p-56;
 constexpr int NEU_MAXIT = 20;
 
-// G = inv(I - A B) with X = A B formed in X (overwritten).
-// GJ = false: when the spectral radius of X is below one,
+// G = inv(I - A B) by the product-form series, X = A B formed in X
+// (overwritten): when the spectral radius of X is below one,
 //   inv(I - X) = (I + X)(I + X^2)(I + X^4)...,
 // each factor costing two matrix-core products; the series stops as soon as
 // max |X^(2^k)| < NEU_TOL (a single product for the thin layers at the start
 // of the doubling, where X is ~1e-12).  Returns false if it has not
 // converged after NEU_MAXIT factors (spectral radius within ~1e-6 of one) or
-// X holds a NaN: the caller then queues its unit for the Gauss-Jordan pass.
-// GJ = true: Gauss-Jordan with partial pivoting; false means singular.
-// np counts the 32x32 products executed.
+// X holds a NaN.  np counts the 32x32 products executed.
+__device__ __forceinline__ bool series_inv(const double *A, const double *B, double *X, double *G,
+                                           InvLds &sh, int &np) {
+  const int t = threadIdx.x;
+  np++;
+  double m = mm_core(A, B, nullptr, X, sh.wmax);
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int i = e / NN, j = e % NN;
+    G[i * LD + j] = (i == j ? 1.0 : 0.0) + X[i * LD + j];
+  }
+  __syncthreads();
+  // G = I + P with P = X; then per factor: G <- G (I + P), P <- P^2, the
+  // two products fused while the next factor still matters
+  if (!(m < 0x1p64)) return false;          // NaN or overflow
+  if (32.0 * m * m < NEU_TOL) return true;  // bound on max |P^2|
+  m = mm_core(X, X, nullptr, X, sh.wmax);
+  np++;
+  for (int it = 0; it < NEU_MAXIT; it++) {
+    if (!(m < 0x1p64)) return false;
+    if (m < NEU_TOL) return true;
+    if (32.0 * m * m < NEU_TOL) {  // last factor
+      mm_add(G, G, X, G);
+      np++;
+      return true;
+    }
+    m = mm2(G, X, G, G, X, X, nullptr, X, sh.wmax);
+    np += 2;
+  }
+  return false;
+}
+
+// G = inv(I - A B).  GJ = false (the hot kernels): the series only; false
+// means it did not converge and the caller queues its unit.  GJ = true (the
+// queue pass): the series, and Gauss-Jordan with partial pivoting where the
+// series does not converge (Gauss-Jordan only when sh.gj_only); false means
+// a singular matrix.
 template <bool GJ>
 __device__ __forceinline__ bool inv_i_minus_ab(const double *A, const double *B, double *X,
                                                double *G, InvLds &sh, int &np) {
-  np++;
   if constexpr (GJ) {
+    if (!sh.gj_only && series_inv(A, B, X, G, sh, np)) return true;
+    np++;
     mm(A, B, X);
     return inv_i_minus(X, G, sh);
   } else {
-    const int t = threadIdx.x;
-    double m = mm_core(A, B, nullptr, X, sh.wmax);
-    for (int e = t; e < NN2; e += RT_WG) {
-      const int i = e / NN, j = e % NN;
-      G[i * LD + j] = (i == j ? 1.0 : 0.0) + X[i * LD + j];
-    }
-    __syncthreads();
-    // G = I + P with P = X; then per factor: G <- G (I + P), P <- P^2, the
-    // two products fused while the next factor still matters
-    if (!(m < 0x1p64)) return false;       // NaN or overflow
-    if (32.0 * m * m < NEU_TOL) return true;  // bound on max |P^2|
-    m = mm_core(X, X, nullptr, X, sh.wmax);
-    np++;
-    for (int it = 0; it < NEU_MAXIT; it++) {
-      if (!(m < 0x1p64)) return false;
-      if (m < NEU_TOL) return true;
-      if (32.0 * m * m < NEU_TOL) {  // last factor
-        mm_add(G, G, X, G);
-        np++;
-        return true;
-      }
-      m = mm2(G, X, G, G, X, X, nullptr, X, sh.wmax);
-      np += 2;
-    }
-    return false;
+    return series_inv(A, B, X, G, sh, np);
   }
 }
 
@@ -547,6 +559,7 @@ __global__ void __launch_bounds__(RT_WG) k_doubling_gj(Dims d, Settings s, const
                                                        Work w, int all) {
   __shared__ DblLds L;
   const int n = all ? d.nitems * d.nf : (int)w.redo_n[0];
+  if (threadIdx.x == 0) L.inv.gj_only = all;
   for (int q = blockIdx.x; q < n; q += gridDim.x) {
     __syncthreads();
     doubling_unit<true>(d, s, freqs, w, all ? q : w.redo_dbl[q], L);
@@ -977,6 +990,7 @@ __global__ void __launch_bounds__(RT_WG) k_adding_gj(Dims d, Settings s, const d
                                                      int all) {
   __shared__ AddLds LL;
   const int n = all ? d.ncol_chunk * d.nf : (int)w.redo_n[1];
+  if (threadIdx.x == 0) LL.inv.gj_only = all;
   for (int q = blockIdx.x; q < n; q += gridDim.x) {
     __syncthreads();
     adding_unit<true>(d, s, freqs, sfc_refl, w, tb, all ? q : w.redo_add[q], LL);
@@ -994,7 +1008,8 @@ void launch_doubling(const Dims &d, const Settings &s, const double *freqs, Work
     return;
   }
   hipLaunchKernelGGL(k_doubling, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, w);
-  hipLaunchKernelGGL(k_doubling_gj, dim3(512), dim3(RT_WG), 0, st, d, s, freqs, w, 0);
+  hipLaunchKernelGGL(k_doubling_gj, dim3((unsigned)std::min<size_t>(n, 2048)), dim3(RT_WG), 0, st,
+                     d, s, freqs, w, 0);
 }
 
 void launch_adding(const Dims &d, const Settings &s, const double *freqs, const char *sfc_refl,
@@ -1008,7 +1023,8 @@ void launch_adding(const Dims &d, const Settings &s, const double *freqs, const 
   }
   hipLaunchKernelGGL(k_adding, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
                      tb);
-  hipLaunchKernelGGL(k_adding_gj, dim3(512), dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb, 0);
+  hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, 2048)), dim3(RT_WG), 0, st, d,
+                     s, freqs, sfc_refl, w, tb, 0);
 }
 
 }  // namespace pam

@@ -83,8 +83,10 @@ def scattering_layers_bound(pam) -> int:
     return int(np.count_nonzero(any_h & inside))
 
 
-def workspace_bytes(lib, pam, nfreq: int) -> int:
-    """Device bytes of one call of this request (the library's own sizing)."""
+def workspace_bytes(lib, pam, nfreq: int, optics_limit: Optional[int] = None) -> int:
+    """Device bytes of one call of this request (the library's own sizing).
+    With `optics_limit` (the context's workspace limit) the layers are
+    processed in chunks of at most that many bytes of optics."""
     p = pam.p
     ncol = int(p["ngridx"]) * int(p["ngridy"])
     nbins = [2 if str(r["dist_name"]).strip() == "mono" else int(r["nbin"])
@@ -92,8 +94,12 @@ def workspace_bytes(lib, pam, nfreq: int) -> int:
     nb = max(nbins)
     if "nbin" in pam.df.data4D:
         nb = max(nb, int(np.max(pam.df.data4D["nbin"])))
-    n = lib.pam_workspace_bytes(ncol, int(p["max_nlyrs"]), len(pam.df.data), nb, nfreq,
-                                int(p["noutlevels"]), scattering_layers_bound(pam))
+    dims = (ncol, int(p["max_nlyrs"]), len(pam.df.data), nb, nfreq, int(p["noutlevels"]))
+    nitems = scattering_layers_bound(pam)
+    if optics_limit is not None:
+        per_item = lib.pam_workspace_bytes(*dims, 2) - lib.pam_workspace_bytes(*dims, 1)
+        nitems = min(nitems, max(1, optics_limit // max(1, per_item)))
+    n = lib.pam_workspace_bytes(*dims, nitems)
     if n < 0:
         raise ValueError("pam_workspace_bytes: invalid dimensions")
     return int(n)
@@ -126,6 +132,9 @@ class ForwardServer:
             raise ValueError("slots must be >= 1")
         self.slots = slots
         self.budget = int(hbm_budget_gb * (1 << 30))
+        # each slot processes a request's layers in chunks of at most this
+        # much optics, so that every request fits a slot's share
+        self.optics_limit = int(0.75 * self.budget / slots)
         self._runner = runner
         self._need = need
         self._lib = None
@@ -138,6 +147,7 @@ class ForwardServer:
                 h = C.c_void_p()
                 if self._lib.pam_context_create(C.byref(h), device) != 0 or not h.value:
                     raise _lib.PamLibraryError("pam_context_create failed")
+                self._lib.pam_context_set_workspace_limit(h, self.optics_limit)
                 self._ctx.append(h)
         # workspace each slot's context holds (grow-only until trimmed); the
         # library rounds its allocations up by an eighth
@@ -160,7 +170,7 @@ class ForwardServer:
     def need_bytes(self, req: Request) -> int:
         if self._need is not None:
             return int(self._need(req))
-        return workspace_bytes(self._lib, req.pam, len(req.freqs))
+        return workspace_bytes(self._lib, req.pam, len(req.freqs), self.optics_limit)
 
     def _call(self, slot: int, req: Request):
         if self._runner is not None:
