@@ -81,7 +81,7 @@ def measured_traffic(kernel):
     return None
 
 
-def serve(args):
+def serve(args, precision="fp64"):
     """The serving mix at an offered Poisson rate: throughput and p50/p99
     latency (arrival to result on the host, input staging included)."""
     import numpy as np
@@ -90,6 +90,8 @@ def serve(args):
     t0 = time.time()
     mix = workloads.serving_requests(args.serve_requests, seed=1004)
     arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=len(mix)))
+    for p, _ in mix:
+        p.set["precision"] = precision
     reqs = [serving.Request(pam=p, freqs=f, arrival=float(a), rid=i)
             for i, ((p, f), a) in enumerate(zip(mix, arrivals))]
     log(f"serving mix built in {time.time() - t0:.1f} s: {len(reqs)} requests")
@@ -102,7 +104,7 @@ def serve(args):
     summ = serving.summarize(res, reqs, wall)
     summ.update({"offered_req_s": args.serve_rate, "slots": args.serve_slots,
                  "hbm_budget_gib": 64.0, "peak_workspace_gib": round(peak / 2 ** 30, 3),
-                 "precision": "fp64",
+                 "precision": precision,
                  "mix": "1-4096 columns (log-uniform), 20-150 layers, 1-12 channels of "
                         "10.65-325 GHz, 0-4 of cloud/ice/snow/graupel with 100 bins"})
     log(f"serving: {summ}")
@@ -118,6 +120,7 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=256)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--no-host-entry", action="store_true",
                     help="skip the pam_run (host arrays, PCIe-inclusive) measurement")
     ap.add_argument("--serve-requests", type=int, default=100,
@@ -139,7 +142,7 @@ def main():
     from pamtra_amd import _capi, _lib, workloads
 
     if args.serve_only:
-        print(json.dumps({"serving": serve(args)}))
+        print(json.dumps({"serving": {"fp64": serve(args), "mixed": serve(args, "mixed")}}))
         return
 
     t0 = time.time()
@@ -218,6 +221,41 @@ def main():
     tb_h = tb.cpu().numpy()
     st_h = status.cpu().numpy()
 
+    # the mixed-precision path on the same workload: throughput and its
+    # per-channel TB error against the FP64 result above
+    mixed = None
+    if not args.no_mixed:
+        sets_m = dict(pam.set)
+        sets_m["precision"] = "mixed"
+        mm = _capi.marshal(nml, sets_m, pam.df.data, pam.df.data4D, prof)
+
+        def step_m():
+            st = lib.pam_run_device(ctx, C.byref(mm.settings), mm.hydros, C.byref(mm.hydro4d),
+                                    C.byref(mm.profile), fp, nf, C.byref(r), None)
+            if st == _capi.PAM_FATAL:
+                raise RuntimeError("pam_run_device (mixed): " + msg.value.decode())
+
+        for _ in range(args.warmup):
+            step_m()
+        torch.cuda.synchronize()
+        sms = np.zeros(5)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            step_m()
+            ms = (C.c_double * 5)()
+            lib.pam_last_stage_times(ctx, ms, 5)
+            sms += np.array(ms[:])
+        torch.cuda.synchronize()
+        dtm = (time.perf_counter() - t0) / args.steps
+        tbm = tb.cpu().numpy()
+        dch = np.max(np.abs(tbm - tb_h), axis=(0, 1, 2, 3, 5))
+        mixed = {"value": round(nx * ny * nf / dtm, 2), "ms_per_step": round(dtm * 1e3, 3),
+                 "stage_ms": {n: round(v / args.steps, 3) for n, v in zip(names, sms)},
+                 "speedup_vs_fp64": round(dt / dtm, 3),
+                 "max_abs_dtb_vs_fp64_K": {f"{fr:g}": float(v) for fr, v in zip(freqs, dch)},
+                 "dtype": "f32 doubling products (f64 thin-layer start, Mie, adding)"}
+        log(f"mixed: {mixed}")
+
     host_rate = None
     if not args.no_host_entry:
         # the host-array entry: inputs staged to HBM and TB copied back inside the call
@@ -254,11 +292,14 @@ def main():
                  "gauss_jordan_units": work[11]},
         "status_nonzero": int((st_h != 0).sum()),
         "pcie_inclusive_value": None if host_rate is None else round(host_rate, 2),
+        "mixed_precision": mixed,
         "roofline": roof,
     }
 
     if args.serve_requests > 0:
-        out["serving"] = serve(args)
+        out["serving"] = {"fp64": serve(args)}
+        if not args.no_mixed:
+            out["serving"]["mixed"] = serve(args, "mixed")
 
     if not args.no_cpu:
         import oracle

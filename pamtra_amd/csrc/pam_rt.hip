@@ -324,57 +324,17 @@ constexpr LegRec make_legrec() {
 }
 __constant__ LegRec c_leg = make_legrec();
 
-struct DblLds {
-  double R[MSZ], T[MSZ], G[MSZ], X[MSZ];
-  double v[12][NN];  // S+, S-, C+, C-, then source-chain scratch
-  double norm[NMU];
-  InvLds inv;
-  double mu[NMU], qw[NMU];
-};
-
-// One (scattering layer, channel) unit.  Returns false only in the series
-// variant (GJ = false) when an inverse did not converge; nothing has been
-// written for the unit then.
-template <bool GJ>
-__device__ bool doubling_unit(const Dims &d, const Settings &s, const double *freqs,
-                              const Work &w, int itf, DblLds &L) {
-  double *sR = L.R, *sT = L.T, *sG = L.G, *sX = L.X;
-  double(*sv)[NN] = L.v;
-  double *snorm = L.norm, *smu = L.mu, *sqw = L.qw;
-  InvLds &sinv = L.inv;
-  // until INITIALIZE4: sG holds the (-,+) quadrant of the phase matrix, sT
-  // the scaled Legendre series, sR the azimuth cosines and sines
-  static_assert(NCOEF * MAXNLEG <= MSZ && 2 * 128 <= MSZ, "scatcnv scratch");
-  double *sY = sG;
-  double(*sc)[MAXNLEG] = reinterpret_cast<double(*)[MAXNLEG]>(sT);
-  double *scos = sR, *ssin = sR + 128;
-
+// The m = 0 Fourier mode of the rotated phase matrix (scatcnv.f90:103-192
+// with the rt_utilities.f90:242-493 helpers) for the (+,+) quadrant (into
+// sX) and the incoming-minus/outgoing-plus quadrant (into sY), from the
+// scaled Legendre series sc.  Each thread does 2 of the 512 (quadrant, mu2,
+// mu1) pairs, in double precision; T is the storage type of the matrices.
+template <typename T>
+__device__ void phase_matrix(int nleg, int scat, const double (*sc)[MAXNLEG], const double *smu,
+                             double *scos, double *ssin, T *sX, T *sY) {
   const int t = threadIdx.x;
-  const int li = itf / d.nf, f = itf % d.nf;
-  const int col = w.item_col[li], z = w.item_layer[li];
-  if (t < NMU) {
-    smu[t] = w.mu[t];
-    sqw[t] = w.qw[t];
-  }
-  if (w.itf_status[itf] & 0xff) return true;
-  const int nleg = w.nleg[itf];
-  double kext = w.kext[itf], salb = w.salb[itf];
-  if (nleg <= 0) {  // no Legendre series: no scatcnv, zero optics
-    kext = 0.0;
-    salb = 0.0;
-  }
-  const double cst = kext * salb / (4.0 * 3.1415926535897932384);
-  const double *cg = w.coef + (size_t)itf * NCOEF * MAXNLEG;
-  for (int e = t; e < NCOEF * MAXNLEG; e += RT_WG) {
-    int sidx = e / MAXNLEG, l = e % MAXNLEG;
-    sc[sidx][l] = (l <= nleg && nleg > 0) ? cst * cg[sidx * MAXNLEG + l] : 0.0;
-  }
-  __syncthreads();
-  // ---- scatcnv: m = 0 Fourier mode of the rotated phase matrix for the
-  // (+,+) quadrant (into sX) and the incoming-minus/outgoing-plus quadrant
-  // (into sY).  Each thread does 2 of the 512 (quadrant, mu2, mu1) pairs.
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
-  if (nleg > 0 && s.lhyd_scat) {
+  if (nleg > 0 && scat) {
     for (int k = t; k <= numpts / 2; k += RT_WG) {
       const double delphi = (2.0 * PI * k) / numpts;
       scos[k] = cos(delphi);
@@ -422,20 +382,71 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
         e21 += wk * (b1 * cos22);
         e22 += wk * (a1 * cos21 * cos22 - a3 * sin21 * sin22);
       }
-      double *M = q ? sY : sX;
+      T *M = q ? sY : sX;
       const int r = NST * j2, c = NST * j1;
-      M[r * LD + c] = e11 / numpts;
-      M[r * LD + c + 1] = e12 / numpts;
-      M[(r + 1) * LD + c] = e21 / numpts;
-      M[(r + 1) * LD + c + 1] = e22 / numpts;
+      M[r * LD + c] = (T)(e11 / numpts);
+      M[r * LD + c + 1] = (T)(e12 / numpts);
+      M[(r + 1) * LD + c] = (T)(e21 / numpts);
+      M[(r + 1) * LD + c + 1] = (T)(e22 / numpts);
     }
   } else {
     for (int e = t; e < NN2; e += RT_WG) {
       int i = e / NN, j = e % NN;
-      sX[i * LD + j] = 0.0;
-      sY[i * LD + j] = 0.0;
+      sX[i * LD + j] = (T)0;
+      sY[i * LD + j] = (T)0;
     }
   }
+}
+
+struct DblLds {
+  double R[MSZ], T[MSZ], G[MSZ], X[MSZ];
+  double v[12][NN];  // S+, S-, C+, C-, then source-chain scratch
+  double norm[NMU];
+  InvLds inv;
+  double mu[NMU], qw[NMU];
+};
+
+// One (scattering layer, channel) unit.  Returns false only in the series
+// variant (GJ = false) when an inverse did not converge; nothing has been
+// written for the unit then.
+template <bool GJ>
+__device__ bool doubling_unit(const Dims &d, const Settings &s, const double *freqs,
+                              const Work &w, int itf, DblLds &L) {
+  double *sR = L.R, *sT = L.T, *sG = L.G, *sX = L.X;
+  double(*sv)[NN] = L.v;
+  double *snorm = L.norm, *smu = L.mu, *sqw = L.qw;
+  InvLds &sinv = L.inv;
+  // until INITIALIZE4: sG holds the (-,+) quadrant of the phase matrix, sT
+  // the scaled Legendre series, sR the azimuth cosines and sines
+  static_assert(NCOEF * MAXNLEG <= MSZ && 2 * 128 <= MSZ, "scatcnv scratch");
+  double *sY = sG;
+  double(*sc)[MAXNLEG] = reinterpret_cast<double(*)[MAXNLEG]>(sT);
+  double *scos = sR, *ssin = sR + 128;
+
+  const int t = threadIdx.x;
+  const int li = itf / d.nf, f = itf % d.nf;
+  const int col = w.item_col[li], z = w.item_layer[li];
+  if (t < NMU) {
+    smu[t] = w.mu[t];
+    sqw[t] = w.qw[t];
+  }
+  if (w.itf_status[itf] & 0xff) return true;
+  const int nleg = w.nleg[itf];
+  double kext = w.kext[itf], salb = w.salb[itf];
+  if (nleg <= 0) {  // no Legendre series: no scatcnv, zero optics
+    kext = 0.0;
+    salb = 0.0;
+  }
+  const double cst = kext * salb / (4.0 * 3.1415926535897932384);
+  const double *cg = w.coef + (size_t)itf * NCOEF * MAXNLEG;
+  for (int e = t; e < NCOEF * MAXNLEG; e += RT_WG) {
+    int sidx = e / MAXNLEG, l = e % MAXNLEG;
+    sc[sidx][l] = (l <= nleg && nleg > 0) ? cst * cg[sidx * MAXNLEG + l] : 0.0;
+  }
+  __syncthreads();
+  // ---- scatcnv
+  const int numpts = 2 * ((nleg + 1) / 2) + 4;
+  phase_matrix<double>(nleg, s.lhyd_scat, sc, smu, scos, ssin, sX, sY);
   __syncthreads();
   const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
   // ---- CHECK_NORM4
@@ -566,6 +577,294 @@ __global__ void __launch_bounds__(RT_WG) k_doubling_gj(Dims d, Settings s, const
   }
 }
 
+
+// ------------------------------------------------------------------
+// mixed precision: the doubling on the FP32 matrix cores
+// ------------------------------------------------------------------
+//
+// T and G = inv(I - R R) stay close to the identity for thin layers, so they
+// are carried as deviations T~ = T - I and G~ = G - I: every entry then has
+// single-precision relative accuracy however thin the layer.  With Z~ = T~ +
+// (I + T~) G~ and V = R + R T~ a doubling step is
+//   T~' = Z~ + (I + Z~) T~,   R' = R + (I + Z~) V,
+// four products as in the FP64 kernel; the identity is added to the A
+// operand while it is loaded.  The phase matrix, its normalisation check,
+// the thin-layer start and the doubling count are computed in FP64; the
+// layer operators are written back in FP64 for the adding kernel.
+
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// C = D + (A + iA I) B on the FP32 matrix cores.  16x16x4 f32 layout: lane l
+// supplies A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+r][l%16].
+// With wmax the return value is max |C_ij| (NaN propagates).
+__device__ __forceinline__ float mmf(const float *A, bool iA, const float *B, const float *D,
+                                     float *C, float *wmax) {
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
+  const int li = l & 15, lk = l >> 4;
+  f4_t acc;
+#pragma unroll
+  for (int r = 0; r < 4; r++) acc[r] = D ? D[(I + 4 * lk + r) * LD + J + li] : 0.f;
+#pragma unroll
+  for (int s = 0; s < NN / 4; s++) {
+    const int kc = 4 * s + lk;
+    float a = A[(I + li) * LD + kc];
+    if (iA && I + li == kc) a += 1.f;
+    const float b = B[kc * LD + J + li];
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; r++) C[(I + 4 * lk + r) * LD + J + li] = acc[r];
+  if (wmax) {
+    const float m = max_nan(max_nan(fabsf(acc[0]), fabsf(acc[1])), max_nan(fabsf(acc[2]), fabsf(acc[3])));
+    const float mf = wave_max(m);
+    if (l == 0) wmax[wv] = mf;
+  }
+  __syncthreads();
+  return wmax ? max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.f;
+}
+
+// Two independent FP32 products in one pass (outputs may alias inputs);
+// with wmax the return value is max |C2_ij|.
+__device__ __forceinline__ float mmf2(const float *A1, bool iA1, const float *B1, const float *D1,
+                                      float *C1, const float *A2, bool iA2, const float *B2,
+                                      const float *D2, float *C2, float *wmax) {
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
+  const int li = l & 15, lk = l >> 4;
+  f4_t acc1, acc2;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    acc1[r] = D1 ? D1[(I + 4 * lk + r) * LD + J + li] : 0.f;
+    acc2[r] = D2 ? D2[(I + 4 * lk + r) * LD + J + li] : 0.f;
+  }
+#pragma unroll
+  for (int s = 0; s < NN / 4; s++) {
+    const int kc = 4 * s + lk;
+    const bool dg = I + li == kc;
+    float a1 = A1[(I + li) * LD + kc], a2 = A2[(I + li) * LD + kc];
+    if (iA1 && dg) a1 += 1.f;
+    if (iA2 && dg) a2 += 1.f;
+    const float b1 = B1[kc * LD + J + li], b2 = B2[kc * LD + J + li];
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc1, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc2, 0, 0, 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    C1[(I + 4 * lk + r) * LD + J + li] = acc1[r];
+    C2[(I + 4 * lk + r) * LD + J + li] = acc2[r];
+  }
+  if (wmax) {
+    const float m =
+        max_nan(max_nan(fabsf(acc2[0]), fabsf(acc2[1])), max_nan(fabsf(acc2[2]), fabsf(acc2[3])));
+    const float mf = wave_max(m);
+    if (l == 0) wmax[wv] = mf;
+  }
+  __syncthreads();
+  return wmax ? max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.f;
+}
+
+// Row i of A x in FP32 over the lane pair t, t^1 (as mv_pair).
+__device__ __forceinline__ float mvf_pair(const float *A, const float *x, int i) {
+  const int h = (threadIdx.x & 1) * 8;
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < NN / 2; k++) {
+    const int c = (k & 7) + h + 16 * (k >> 3);
+    s += A[i * LD + c] * x[c];
+  }
+  return s + __shfl_xor(s, 1, 64);
+}
+
+// Truncation of the FP32 series: the omitted tail is below the single-
+// precision resolution of G = I + G~.
+constexpr float NEU_TOL_F = 0x1p-28f;
+
+// G~ = inv(I - R R) - I by the product-form series in FP32 (X scratch).
+// false: not converged (the unit goes to the FP64 Gauss-Jordan pass).
+__device__ bool series_f32(const float *R, float *X, float *Gt, float *wmax, int &np) {
+  const int t = threadIdx.x;
+  np++;
+  float m = mmf(R, false, R, nullptr, X, wmax);
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int i = e / NN, j = e % NN;
+    Gt[i * LD + j] = X[i * LD + j];
+  }
+  __syncthreads();
+  if (!(m < 0x1p64f)) return false;
+  if (32.f * m * m < NEU_TOL_F) return true;
+  m = mmf(X, false, X, nullptr, X, wmax);
+  np++;
+  for (int it = 0; it < NEU_MAXIT; it++) {
+    if (!(m < 0x1p64f)) return false;
+    if (m < NEU_TOL_F) return true;
+    if (32.f * m * m < NEU_TOL_F) {  // last factor: G~ <- G~ + (I + G~) P
+      mmf(Gt, true, X, Gt, Gt, nullptr);
+      np++;
+      return true;
+    }
+    m = mmf2(Gt, true, X, Gt, Gt, X, false, X, nullptr, X, wmax);
+    np += 2;
+  }
+  return false;
+}
+
+struct DblLdsF {
+  float R[MSZ], T[MSZ], G[MSZ], X[MSZ];
+  union {
+    float v[16][NN];    // S+, S-, C+, C-, then source-chain scratch
+    double cs[2][128];  // azimuth cosines and sines of the phase matrix
+  } u;
+  double norm[NMU];
+  float wmax[4];
+  double mu[NMU], qw[NMU];
+};
+
+// One (scattering layer, channel) unit of the mixed-precision path; false
+// when the FP32 series did not converge (nothing written; the unit is
+// recomputed in FP64 by the Gauss-Jordan pass).
+__device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double *freqs,
+                                  const Work &w, int itf, DblLdsF &L) {
+  float *sR = L.R, *sT = L.T, *sG = L.G, *sX = L.X;
+  float(*sv)[NN] = L.u.v;
+  double *smu = L.mu, *sqw = L.qw, *snorm = L.norm;
+  // until the thin-layer start: the scaled Legendre series in R and T
+  static_assert(NCOEF * MAXNLEG * sizeof(double) <= 2 * MSZ * sizeof(float), "series scratch");
+  double(*sc)[MAXNLEG] = reinterpret_cast<double(*)[MAXNLEG]>(L.R);
+
+  const int t = threadIdx.x;
+  const int li = itf / d.nf, f = itf % d.nf;
+  const int col = w.item_col[li], z = w.item_layer[li];
+  if (t < NMU) {
+    smu[t] = w.mu[t];
+    sqw[t] = w.qw[t];
+  }
+  if (w.itf_status[itf] & 0xff) return true;
+  const int nleg = w.nleg[itf];
+  double kext = w.kext[itf], salb = w.salb[itf];
+  if (nleg <= 0) {
+    kext = 0.0;
+    salb = 0.0;
+  }
+  const double cst = kext * salb / (4.0 * 3.1415926535897932384);
+  const double *cg = w.coef + (size_t)itf * NCOEF * MAXNLEG;
+  for (int e = t; e < NCOEF * MAXNLEG; e += RT_WG) {
+    int sidx = e / MAXNLEG, l = e % MAXNLEG;
+    sc[sidx][l] = (l <= nleg && nleg > 0) ? cst * cg[sidx * MAXNLEG + l] : 0.0;
+  }
+  __syncthreads();
+  const int numpts = 2 * ((nleg + 1) / 2) + 4;
+  phase_matrix<float>(nleg, s.lhyd_scat, sc, smu, L.u.cs[0], L.u.cs[1], sX, sG);
+  __syncthreads();
+  const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
+  if (t < NMU) {  // CHECK_NORM4
+    double sum = absorb - kext;
+    for (int j2 = 0; j2 < NMU; j2++)
+      sum = sum + 2.0 * PI * sqw[j2] *
+                      ((double)sX[(NST * j2) * LD + NST * t] + (double)sG[(NST * j2) * LD + NST * t]);
+    snorm[t] = fabs(sum);
+  }
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int j = 0; j < NMU; j++) mx = fmax(mx, snorm[j]);
+    if (mx > 5.0e-4) set_err(&w.itf_status[itf], E_NORM);
+  }
+  const double wl = C_LIGHT / (freqs[f] * 1.e3);
+  const double zdiff = fabs(w.hgt_lev[(size_t)col * (d.ML + 1) + z + 1] -
+                            w.hgt_lev[(size_t)col * (d.ML + 1) + z]);
+  const double gas = fmax(w.kgas[((size_t)col * d.nf + f) * d.ML + z], 0.0);
+  const double p0 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z + 1], wl);
+  const double p1 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z], wl);
+  const double extinct = kext + gas;
+  const double fnd = log(fmax(extinct * zdiff, 1.0e-7) / 1.0e-6) / (double)logf(2.0f);
+  const int nd = (fnd > 0.0) ? (int)fnd + 1 : 0;
+  const double nsub = (double)ldexpf(1.0f, nd);
+  const double dz = zdiff / nsub;
+  double linfac = (p0 == 0.0) ? 0.0 : (p1 / p0 - 1.0) / nsub;
+  // thin-layer start in FP64 (INITIALIZE4 / INITIAL_SOURCE4), stored as
+  // R and T~ = T - I in FP32
+  const double C2 = 2.0 * 3.1415926535897932384;
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int r = e / NN, c = e % NN;
+    const int i2 = r % NST, j2 = r / NST, i1 = c % NST, j1 = c / NST;
+    const double tmp = dz / smu[j2];
+    const double gext = (i1 == i2) ? gas : 0.0;
+    const double ext = (j1 == j2) ? ((i1 == i2) ? kext : 0.0) + gext : 0.0;
+    const double pm = (double)sG[r * LD + c], pp = (double)sX[r * LD + c];
+    sR[r * LD + c] = (float)(C2 * tmp * sqw[j1] * pm);
+    sT[r * LD + c] = (float)(-tmp * (ext - C2 * sqw[j1] * pp));
+  }
+  __syncthreads();  // the cos/sin tables share storage with the sources
+  if (t < NN) {
+    const int i = t % NST, j = t / NST;
+    const double e = (i == 0) ? gas : 0.0;
+    const double tmp = p0 * dz / smu[j];
+    const float v = (float)(tmp * ((i == 0 ? absorb : 0.0) + e));
+    sv[0][t] = v;
+    sv[1][t] = v;
+    sv[2][t] = v;
+    sv[3][t] = v;
+  }
+  __syncthreads();
+  int nprod = 0;
+  for (int it = 0; it < nd; it++) {
+    if (!series_f32(sR, sX, sG, L.wmax, nprod)) return false;
+    // Z~ = T~ + (I + T~) G~ (into G), V = R + R T~
+    mmf2(sT, true, sG, sT, sG, sR, false, sT, sR, sX, nullptr);
+    {
+      const float lf = (float)linfac;
+      const int g = t >> 6, i = (t >> 1) & 31;
+      const bool lead = (t & 1) == 0;
+      if (t < NN) sv[4][t] = sv[1][t] + lf * sv[3][t];
+      __syncthreads();
+      const float *v1 = g == 0 ? sv[4] : g == 1 ? sv[3] : g == 2 ? sv[0] : sv[2];
+      const float base1 = g == 0 ? sv[0][i] : g == 1 ? sv[2][i] : g == 2 ? sv[4][i] : sv[3][i];
+      const float u = base1 + mvf_pair(sR, v1, i);
+      if (lead) sv[8 + g][i] = u;
+      __syncthreads();
+      const float zu = u + mvf_pair(sG, sv[8 + g], i);  // Z u = u + Z~ u
+      const float nv = g == 0 ? (sv[0][i] + zu) + lf * sv[2][i]
+                              : (g == 1 ? sv[2][i] : g == 2 ? sv[1][i] : sv[3][i]) + zu;
+      __syncthreads();
+      if (lead) sv[g == 0 ? 0 : (g == 1 ? 2 : (g == 2 ? 1 : 3))][i] = nv;
+    }
+    // T~' = Z~ + (I + Z~) T~, R' = R + (I + Z~) V
+    mmf2(sG, true, sT, sG, sT, sG, true, sX, sR, sR, nullptr);
+    nprod += 4;
+    linfac = 2.0 * linfac;
+  }
+  double *R = w.R + (size_t)itf * NN2, *T = w.T + (size_t)itf * NN2;
+  for (int e = t; e < NN2; e += RT_WG) {
+    int i = e / NN, j = e % NN;
+    R[e] = (double)sR[i * LD + j];
+    T[e] = (i == j ? 1.0 : 0.0) + (double)sT[i * LD + j];
+  }
+  if (t < NN) {
+    w.Sp[(size_t)itf * NN + t] = (double)sv[0][t];
+    w.Sm[(size_t)itf * NN + t] = (double)sv[1][t];
+  }
+  if (t == 0) {
+    w.ndbl[itf] = nd;
+    atomicAdd(&w.counters[3], (unsigned long long)nd);
+    if (nleg > 0 && s.lhyd_scat)
+      atomicAdd(&w.counters[8], (unsigned long long)(numpts / 2 + 1) * (nleg + 1));
+    atomicAdd(&w.counters[9], (unsigned long long)nprod);
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(RT_WG) k_doubling_mixed(Dims d, Settings s, const double *freqs,
+                                                          Work w) {
+  __shared__ DblLdsF L;
+  const int itf = blockIdx.x;
+  if (!doubling_unit_f32(d, s, freqs, w, itf, L) && threadIdx.x == 0) {
+    const unsigned q = atomicAdd(&w.redo_n[0], 1u);
+    w.redo_dbl[q] = itf;
+  }
+}
 
 // ------------------------------------------------------------------
 // adding, internal radiance, TB
@@ -1007,7 +1306,10 @@ void launch_doubling(const Dims &d, const Settings &s, const double *freqs, Work
                        st, d, s, freqs, w, 1);
     return;
   }
-  hipLaunchKernelGGL(k_doubling, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, w);
+  if (s.precision == PAM_PREC_MIXED && s.inverse_gj == 0)
+    hipLaunchKernelGGL(k_doubling_mixed, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, w);
+  else
+    hipLaunchKernelGGL(k_doubling, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, w);
   hipLaunchKernelGGL(k_doubling_gj, dim3((unsigned)std::min<size_t>(n, 2048)), dim3(RT_WG), 0, st,
                      d, s, freqs, w, 0);
 }
