@@ -175,12 +175,14 @@ def main():
         step()
         log(f"warmup {i} done")
     torch.cuda.synchronize()
-    stage_ms = np.zeros(5)
+    names = lib.pam_stage_names().decode().split(",")
+    nst = len(names)
+    stage_ms = np.zeros(nst)
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
-        ms = (C.c_double * 5)()
-        lib.pam_last_stage_times(ctx, ms, 5)
+        ms = (C.c_double * nst)()
+        lib.pam_last_stage_times(ctx, ms, nst)
         stage_ms += np.array(ms[:])
         log(f"step {i} done")
     torch.cuda.synchronize()
@@ -189,7 +191,6 @@ def main():
     work = (C.c_int64 * 12)()
     lib.pam_last_work(ctx, work, 12)
     work = list(work)
-    names = lib.pam_stage_names().decode().split(",")
     value = nx * ny * nf / dt
 
     # Algorithmic flops of the RT stages, counted on the reference's algorithm
@@ -200,12 +201,13 @@ def main():
     # the two internal radiances (radintg4.f90:119-172, 447-513).
     n3 = NN ** 3
     nummu = NN // 2
-    f_doubling = work[3] * 14 * n3 + work[8] * 4 * nummu * nummu * 11
+    f_doubling = work[3] * 14 * n3
+    f_phase = work[8] * 4 * nummu * nummu * 11
     f_adding = (work[4] + work[5] + nx * ny * nf) * 20 * n3 + nx * ny * nf * 16 * n3
-    flops = {"doubling": f_doubling, "adding": f_adding}
+    flops = {"doubling": f_doubling, "phase": f_phase, "adding": f_adding}
     # 32x32 products the matrix cores actually executed (series inverse included)
     executed = {"doubling": work[9] * 2 * n3, "adding": work[10] * 2 * n3}
-    dom = max(range(5), key=lambda i: stage_ms[i])
+    dom = max(range(nst), key=lambda i: stage_ms[i])
     dom_name = names[dom]
     roof = None
     if dom_name in flops:
@@ -215,8 +217,8 @@ def main():
                 "frac": round(ach / PEAK_FP64_TFLOPS, 4),
                 "traffic": (measured_traffic("k_" + dom_name)
                             if args.ncol == 16384 else None),
-                "mfma_executed_tflops": round(executed[dom_name] / (stage_ms[dom] * 1e-3) / 1e12,
-                                              3)}
+                "mfma_executed_tflops": (round(executed[dom_name] / (stage_ms[dom] * 1e-3) / 1e12, 3)
+                                         if dom_name in executed else None)}
 
     tb_h = tb.cpu().numpy()
     st_h = status.cpu().numpy()
@@ -238,12 +240,12 @@ def main():
         for _ in range(args.warmup):
             step_m()
         torch.cuda.synchronize()
-        sms = np.zeros(5)
+        sms = np.zeros(nst)
         t0 = time.perf_counter()
         for i in range(args.steps):
             step_m()
-            ms = (C.c_double * 5)()
-            lib.pam_last_stage_times(ctx, ms, 5)
+            ms = (C.c_double * nst)()
+            lib.pam_last_stage_times(ctx, ms, nst)
             sms += np.array(ms[:])
         torch.cuda.synchronize()
         dtm = (time.perf_counter() - t0) / args.steps

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpamtra_amd.so")
-SOURCES = ["pam_column.hip", "pam_dsd.hip", "pam_mie.hip", "pam_rt.hip", "pam_capi.hip"]
+SOURCES = ["pam_column.hip", "pam_dsd.hip", "pam_mie.hip", "pam_phase.hip", "pam_rt.hip",
+           "pam_capi.hip"]
 ARCH = os.environ.get("PAM_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -31,8 +31,8 @@ using namespace pam;
   } while (0)
 
 constexpr int PAM_NWORK = 12;
-static const char *k_stage_names = "columns,dsd,mie,doubling,adding";
-constexpr int NSTAGE = 5;
+static const char *k_stage_names = "columns,dsd,mie,phase,doubling,adding";
+constexpr int NSTAGE = 6;
 
 // A device allocation that only grows.
 struct DevBuf {
@@ -58,11 +58,14 @@ struct DevBuf {
 // Mie angle tables of one device, shared by all its contexts and kept until
 // the process ends (66 MiB).
 struct AngleTables {
-  DevBuf buf;
+  DevBuf buf, pbuf;
   const double *tab = nullptr;
   const int64_t *off = nullptr;
   int nmax = 0;
+  const double *ptab = nullptr;  // phase-matrix tables (about 200 MiB)
+  const int64_t *poff = nullptr;
 };
+static void lobatto(int num, double *mu, double *w);
 static std::mutex g_tab_mutex;
 static std::map<int, AngleTables *> g_tables;
 
@@ -89,8 +92,30 @@ static AngleTables *device_tables(int device) {
     t->off = doff;
     t->nmax = nmax;
   }
+  if (ok) {
+    int64_t poff[PH_NBUCKET + 1];
+    const int64_t pn = phase_table_layout(poff);
+    const size_t pbytes = ((size_t)pn * 8 + 255) & ~(size_t)255;
+    ok = t->pbuf.ensure(pbytes + sizeof(poff) + 256 + NMU * sizeof(double)) == hipSuccess;
+    if (ok) {
+      double *ptab = (double *)t->pbuf.p;
+      int64_t *dpoff = (int64_t *)((char *)t->pbuf.p + pbytes);
+      double *dmu = (double *)((char *)dpoff + ((sizeof(poff) + 255) & ~(size_t)255));
+      double mu[NMU], qw[NMU];
+      lobatto(NMU, mu, qw);
+      ok = hipMemcpy(dpoff, poff, sizeof(poff), hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(dmu, mu, sizeof(mu), hipMemcpyHostToDevice) == hipSuccess;
+      if (ok) {
+        build_phase_tables(ptab, dpoff, dmu, nullptr);
+        ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+      }
+      t->ptab = ptab;
+      t->poff = dpoff;
+    }
+  }
   if (!ok) {
     t->buf.release();
+    t->pbuf.release();
     delete t;
     return nullptr;
   }
@@ -105,9 +130,11 @@ struct pam_context {
   const double *d_atab = nullptr;
   const int64_t *d_atab_off = nullptr;
   int atab_nmax = 0;
+  const double *d_ptab = nullptr;
+  const int64_t *d_ptab_off = nullptr;
   double *d_mu = nullptr, *d_qw = nullptr;
   double mu[NMU], qw[NMU];
-  double stage_ms[NSTAGE] = {0, 0, 0, 0, 0};
+  double stage_ms[NSTAGE] = {0, 0, 0, 0, 0, 0};
   int64_t work[PAM_NWORK] = {};
   size_t budget_bytes = 0;
 };
@@ -259,6 +286,8 @@ extern "C" int pam_context_create(pam_context **out, int device) {
     c->d_atab = t->tab;
     c->d_atab_off = t->off;
     c->atab_nmax = t->nmax;
+    c->d_ptab = t->ptab;
+    c->d_ptab_off = t->poff;
   }
   // per-call workspace budget: half of the free HBM, at most 96 GiB
   // (PAM_WORKSPACE_GB, read at each call, overrides it)
@@ -359,7 +388,7 @@ static size_t col_bytes(const Dims &d) {
 
 static size_t item_bytes_per(const Dims &d) {
   size_t per = 2 * 4 + (size_t)d.nh * d.nbmax * 4 * 8 + (size_t)d.nh * 4;
-  per += (size_t)d.nf * (2 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8 + 4);
+  per += (size_t)d.nf * (2 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8 + 4 + 8);
   return per;
 }
 
@@ -385,6 +414,8 @@ static void carve_items(Work &w, void *base, const Dims &d, size_t nitems) {
   w.Sp = cv.take<double>(nif * NN);
   w.Sm = cv.take<double>(nif * NN);
   w.redo_dbl = cv.take<int32_t>(nif);
+  w.ph_rank = cv.take<int32_t>(nif);
+  w.ph_perm = cv.take<int32_t>(nif);
 }
 
 extern "C" int64_t pam_workspace_bytes(int64_t ncol, int32_t max_nlyrs, int32_t nhydro,
@@ -406,7 +437,8 @@ extern "C" int64_t pam_workspace_bytes(int64_t ncol, int32_t max_nlyrs, int32_t 
               c + c * no * 8 + 13 * c * L * nh * 8;
   io += c * no * PAM_NANGLES * nf * 2 * 8 + c * 2 * nf * NMU * 8 + PAM_NANGLES * 8 + c * nf * 4 +
         c * L * nf * (3 * 8 + 2 * 4);
-  const size_t small = 4096 + sizeof(pam_hydro_desc) * std::max<size_t>(1, nh) + 8 * nf + 64 * 8;
+  const size_t small = 4096 + sizeof(pam_hydro_desc) * std::max<size_t>(1, nh) + 8 * nf + 64 * 8 +
+                       3 * 4 * (PH_NBUCKET + 1) + 256;
   return (int64_t)(col_bytes(d) + (size_t)std::max<int64_t>(1, nitems) * item_bytes_per(d) +
                    64 * 256 + io + 64 * 256 + small);
 }
@@ -523,13 +555,15 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   }
 
   // small buffers: hydro table, frequencies, counters
-  size_t small = 4096 + sizeof(pam_hydro_desc) * std::max(1, d.nh) + 8 * (size_t)nfreq + 64 * 8;
+  size_t small = 4096 + sizeof(pam_hydro_desc) * std::max(1, d.nh) + 8 * (size_t)nfreq + 64 * 8 +
+                 3 * 4 * (PH_NBUCKET + 1) + 256;
   HIP_OK(c->smallbuf.ensure(small));
   Carver sc(c->smallbuf.p);
   pam_hydro_desc *d_hyd = sc.take<pam_hydro_desc>(std::max(1, d.nh));
   double *d_freqs = sc.take<double>(nfreq);
   unsigned long long *d_cnt = sc.take<unsigned long long>(PAM_NWORK);
   uint32_t *d_redo_n = sc.take<uint32_t>(2);
+  uint32_t *d_ph = sc.take<uint32_t>(3 * (PH_NBUCKET + 1));
   if (d.nh > 0)
     HIP_OK(hipMemcpyAsync(d_hyd, hydros, sizeof(pam_hydro_desc) * d.nh, hipMemcpyHostToDevice,
                           stream));
@@ -571,6 +605,11 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   w.atab = c->d_atab;
   w.atab_off = c->d_atab_off;
   w.atab_nmax = c->atab_nmax;
+  w.ptab = c->d_ptab;
+  w.ptab_off = c->d_ptab_off;
+  w.ph_cnt = d_ph;
+  w.ph_off = d_ph + (PH_NBUCKET + 1);
+  w.ph_toff = d_ph + 2 * (PH_NBUCKET + 1);
   w.counters = d_cnt;
   w.redo_n = d_redo_n;
 
@@ -662,10 +701,13 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     launch_mie(dc, s, d_hyd, h4, p, d_freqs, w, stream);
     HIP_OK(hipGetLastError());
     HIP_OK(mark(2));
+    launch_phase(dc, s, w, stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(mark(3));
     HIP_OK(hipMemsetAsync(w.redo_n, 0, 2 * sizeof(uint32_t), stream));
     launch_doubling(dc, s, d_freqs, w, stream);
     HIP_OK(hipGetLastError());
-    HIP_OK(mark(3));
+    HIP_OK(mark(4));
     launch_adding(dc, s, d_freqs, pin->sfc_refl, w, res->tb, stream);
     HIP_OK(hipGetLastError());
     if (res->kextatmo || res->kexttot || res->salbedo || res->nlegen || res->ndoubles) {
@@ -673,7 +715,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
       hipLaunchKernelGGL(k_diag, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, stream, dc, w,
                          res->kextatmo, res->kexttot, res->salbedo, res->nlegen, res->ndoubles);
     }
-    HIP_OK(mark(4));
+    HIP_OK(mark(5));
   }
   hipLaunchKernelGGL(k_status, dim3((unsigned)((d.ncol * d.nf + 255) / 256)), dim3(256), 0,
                      stream, d, w, res->status);

@@ -19,7 +19,7 @@ constexpr int MAX_NQUAD = 256;       // Gauss nodes of the Mie phase integral
 constexpr int MAX_NTERMS = 160;      // Mie series length supported on device
 constexpr int MAX_HYDRO = 32;        // bits of the per-layer hydrometeor mask
 constexpr int MAX_NUMPTS = 204;      // 2*int((199+1)/2)+4
-constexpr int NUM_NP = MAX_NUMPTS / 2 - 1;  // NUMPTS values 4, 6, ..., 204
+constexpr int PH_NBUCKET = MAX_NUMPTS / 2 - 1;  // NUMPTS values 4, 6, ..., 204
 
 constexpr double PI = 3.141592653589793;      // constants.f90:16
 constexpr double C_LIGHT = 299792458.0;       // constants.f90:15
@@ -124,6 +124,12 @@ struct Work {
   const double *atab;
   const int64_t *atab_off;             // [atab_nmax + 1]
   int atab_nmax;
+  // phase-matrix tables per NUMPTS (see build_phase_tables) and the units
+  // of the chunk gathered by NUMPTS
+  const double *ptab;
+  const int64_t *ptab_off;             // [PH_NBUCKET + 1]
+  uint32_t *ph_cnt, *ph_off, *ph_toff; // [PH_NBUCKET + 1]
+  int32_t *ph_rank, *ph_perm;          // [nitems*nf]
   const double *mu, *qw;               // Lobatto nodes/weights [NMU]
   // units whose inverse series did not converge, for the Gauss-Jordan pass
   uint32_t *redo_n;                    // [2]: doubling, adding (reset per chunk)
@@ -211,6 +217,12 @@ void launch_dsd(const Dims &d, const Settings &s, const Profile &p, const pam_hy
 // (doubles) and returns the total size; build_angle_tables fills the table.
 int64_t angle_table_layout(int64_t *off, int *nmax);
 void build_angle_tables(double *tab, const int64_t *d_off, int nmax, hipStream_t st);
+// phase-matrix tables: off[0..PH_NBUCKET] (doubles), total size returned;
+// d_mu holds the NMU Lobatto nodes
+int64_t phase_table_layout(int64_t *off);
+void build_phase_tables(double *tab, const int64_t *d_off, const double *d_mu, hipStream_t st);
+// phase matrices of the chunk's scattering units into their R/T slots
+void launch_phase(const Dims &d, const Settings &s, Work &w, hipStream_t st);
 void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, const Hydro4D &h4,
                 const Profile &p, const double *freqs, Work &w, hipStream_t st);
 void launch_surface(const Dims &d, const Settings &s, const Profile &p, const double *freqs,

@@ -1,8 +1,8 @@
 // pam_rt.hip -- the RT4 solver on the device.
 //
-//  k_doubling: one workgroup per (scattering layer, channel).  Builds the
-//    m=0 phase matrix from the Legendre series (scatcnv.f90:103-192 with the
-//    rt_utilities.f90:242-493 helpers), checks its normalisation
+//  k_doubling: one workgroup per (scattering layer, channel).  Takes the
+//    m=0 phase matrix that k_phase (pam_phase.hip) formed in the unit's
+//    R/T slots (scatcnv.f90:103-192), checks its normalisation
 //    (radscat4.f90:87-136), initialises the thin layer (radintg4.f90:1-73)
 //    and doubles it up (radintg4.f90:174-309).  Only the + quadrants are
 //    formed: for randomly oriented particles with NSTOKES=2 the (-,-) and
@@ -309,95 +309,6 @@ __device__ __forceinline__ bool inv_i_minus_ab(const double *A, const double *B,
 // phase matrix + doubling
 // ------------------------------------------------------------------
 
-// Legendre recurrence P_l = a_l x P_{l-1} - b_l P_{l-2} with a_l = (2l-1)/l,
-// b_l = (l-1)/l (rt_utilities.f90:242-269 divides by l at every step)
-struct LegRec {
-  double a[MAXNLEG + 1], b[MAXNLEG + 1];
-};
-constexpr LegRec make_legrec() {
-  LegRec r{};
-  for (int l = 1; l <= MAXNLEG; l++) {
-    r.a[l] = (2.0 * l - 1.0) / l;
-    r.b[l] = (l - 1.0) / l;
-  }
-  return r;
-}
-__constant__ LegRec c_leg = make_legrec();
-
-// The m = 0 Fourier mode of the rotated phase matrix (scatcnv.f90:103-192
-// with the rt_utilities.f90:242-493 helpers) for the (+,+) quadrant (into
-// sX) and the incoming-minus/outgoing-plus quadrant (into sY), from the
-// scaled Legendre series sc.  Each thread does 2 of the 512 (quadrant, mu2,
-// mu1) pairs, in double precision; T is the storage type of the matrices.
-template <typename T>
-__device__ void phase_matrix(int nleg, int scat, const double (*sc)[MAXNLEG], const double *smu,
-                             double *scos, double *ssin, T *sX, T *sY) {
-  const int t = threadIdx.x;
-  const int numpts = 2 * ((nleg + 1) / 2) + 4;
-  if (nleg > 0 && scat) {
-    for (int k = t; k <= numpts / 2; k += RT_WG) {
-      const double delphi = (2.0 * PI * k) / numpts;
-      scos[k] = cos(delphi);
-      ssin[k] = sin(delphi);
-    }
-    __syncthreads();
-    for (int pr = t; pr < 2 * NMU * NMU; pr += RT_WG) {
-      const int q = pr / (NMU * NMU), j2 = (pr / NMU) % NMU, j1 = pr % NMU;
-      const double mu1 = q ? -smu[j1] : smu[j1];
-      const double mu2 = smu[j2];
-      const double st1 = sqrt(1. - mu1 * mu1), st2 = sqrt(1. - mu2 * mu2);
-      const double sq12 = sqrt((1. - mu1 * mu1) * (1. - mu2 * mu2));
-      double e11 = 0.0, e12 = 0.0, e21 = 0.0, e22 = 0.0;
-      for (int k = 1; k <= numpts / 2 + 1; k++) {
-        const double cosphi = scos[k - 1], sinphi = ssin[k - 1];
-        const double x = mu1 * mu2 + sq12 * cosphi;
-        double a1 = 0.0, b1 = 0.0, a3 = 0.0, pl = 1.0, pl1 = 1.0, pl2 = 0.0;
-        for (int l = 0; l <= nleg; l++) {
-          if (l > 0) pl = (c_leg.a[l] * x) * pl1 - c_leg.b[l] * pl2;
-          a1 += sc[0][l] * pl;
-          b1 += sc[1][l] * pl;
-          a3 += sc[2][l] * pl;
-          pl2 = pl1;
-          pl1 = pl;
-        }
-        const double sin_scat = sqrt(fmax(0.0, 1. - x * x));
-        double sin1, sin2, cos1, cos2;
-        if (sin_scat == 0.0) {
-          sin1 = 0.0;
-          sin2 = 0.0;
-          cos1 = 1.0;
-          cos2 = -1.0;
-        } else {
-          const double rs = 1.0 / sin_scat;
-          sin1 = st2 * sinphi * rs;
-          sin2 = st1 * sinphi * rs;
-          cos1 = (st1 * mu2 - st2 * mu1 * cosphi) * rs;
-          cos2 = (st2 * mu1 - st1 * mu2 * cosphi) * rs;
-        }
-        const double sin21 = 2.0 * sin1 * cos1, cos21 = 1.0 - 2.0 * sin1 * sin1;
-        const double sin22 = 2.0 * sin2 * cos2, cos22 = 1.0 - 2.0 * sin2 * sin2;
-        const double wk = (k == 1 || k == numpts / 2 + 1) ? 1.0 : 2.0;
-        e11 += wk * a1;
-        e12 += wk * (b1 * cos21);
-        e21 += wk * (b1 * cos22);
-        e22 += wk * (a1 * cos21 * cos22 - a3 * sin21 * sin22);
-      }
-      T *M = q ? sY : sX;
-      const int r = NST * j2, c = NST * j1;
-      M[r * LD + c] = (T)(e11 / numpts);
-      M[r * LD + c + 1] = (T)(e12 / numpts);
-      M[(r + 1) * LD + c] = (T)(e21 / numpts);
-      M[(r + 1) * LD + c + 1] = (T)(e22 / numpts);
-    }
-  } else {
-    for (int e = t; e < NN2; e += RT_WG) {
-      int i = e / NN, j = e % NN;
-      sX[i * LD + j] = (T)0;
-      sY[i * LD + j] = (T)0;
-    }
-  }
-}
-
 struct DblLds {
   double R[MSZ], T[MSZ], G[MSZ], X[MSZ];
   double v[12][NN];  // S+, S-, C+, C-, then source-chain scratch
@@ -416,12 +327,8 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
   double(*sv)[NN] = L.v;
   double *snorm = L.norm, *smu = L.mu, *sqw = L.qw;
   InvLds &sinv = L.inv;
-  // until INITIALIZE4: sG holds the (-,+) quadrant of the phase matrix, sT
-  // the scaled Legendre series, sR the azimuth cosines and sines
-  static_assert(NCOEF * MAXNLEG <= MSZ && 2 * 128 <= MSZ, "scatcnv scratch");
+  // until INITIALIZE4: sG holds the (-,+) quadrant of the phase matrix
   double *sY = sG;
-  double(*sc)[MAXNLEG] = reinterpret_cast<double(*)[MAXNLEG]>(sT);
-  double *scos = sR, *ssin = sR + 128;
 
   const int t = threadIdx.x;
   const int li = itf / d.nf, f = itf % d.nf;
@@ -437,16 +344,18 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
     kext = 0.0;
     salb = 0.0;
   }
-  const double cst = kext * salb / (4.0 * 3.1415926535897932384);
-  const double *cg = w.coef + (size_t)itf * NCOEF * MAXNLEG;
-  for (int e = t; e < NCOEF * MAXNLEG; e += RT_WG) {
-    int sidx = e / MAXNLEG, l = e % MAXNLEG;
-    sc[sidx][l] = (l <= nleg && nleg > 0) ? cst * cg[sidx * MAXNLEG + l] : 0.0;
-  }
-  __syncthreads();
-  // ---- scatcnv
+  // ---- scatcnv: the phase matrix, formed by k_phase in the unit's T
+  // ((+,+) quadrant) and R ((-,+) quadrant) slots
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
-  phase_matrix<double>(nleg, s.lhyd_scat, sc, smu, scos, ssin, sX, sY);
+  {
+    const bool has = nleg > 0 && s.lhyd_scat;
+    const double *P0 = w.T + (size_t)itf * NN2, *P1 = w.R + (size_t)itf * NN2;
+    for (int e = t; e < NN2; e += RT_WG) {
+      const int i = e / NN, j = e % NN;
+      sX[i * LD + j] = has ? P0[e] : 0.0;
+      sY[i * LD + j] = has ? P1[e] : 0.0;
+    }
+  }
   __syncthreads();
   const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
   // ---- CHECK_NORM4
@@ -713,9 +622,8 @@ __device__ bool series_f32(const float *R, float *X, float *Gt, float *wmax, int
 
 struct DblLdsF {
   float R[MSZ], T[MSZ], G[MSZ], X[MSZ];
-  union {
-    float v[16][NN];    // S+, S-, C+, C-, then source-chain scratch
-    double cs[2][128];  // azimuth cosines and sines of the phase matrix
+  struct {
+    float v[16][NN];  // S+, S-, C+, C-, then source-chain scratch
   } u;
   double norm[NMU];
   float wmax[4];
@@ -730,9 +638,6 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   float *sR = L.R, *sT = L.T, *sG = L.G, *sX = L.X;
   float(*sv)[NN] = L.u.v;
   double *smu = L.mu, *sqw = L.qw, *snorm = L.norm;
-  // until the thin-layer start: the scaled Legendre series in R and T
-  static_assert(NCOEF * MAXNLEG * sizeof(double) <= 2 * MSZ * sizeof(float), "series scratch");
-  double(*sc)[MAXNLEG] = reinterpret_cast<double(*)[MAXNLEG]>(L.R);
 
   const int t = threadIdx.x;
   const int li = itf / d.nf, f = itf % d.nf;
@@ -748,15 +653,17 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     kext = 0.0;
     salb = 0.0;
   }
-  const double cst = kext * salb / (4.0 * 3.1415926535897932384);
-  const double *cg = w.coef + (size_t)itf * NCOEF * MAXNLEG;
-  for (int e = t; e < NCOEF * MAXNLEG; e += RT_WG) {
-    int sidx = e / MAXNLEG, l = e % MAXNLEG;
-    sc[sidx][l] = (l <= nleg && nleg > 0) ? cst * cg[sidx * MAXNLEG + l] : 0.0;
-  }
-  __syncthreads();
+  // the phase matrix, formed by k_phase in the unit's T and R slots
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
-  phase_matrix<float>(nleg, s.lhyd_scat, sc, smu, L.u.cs[0], L.u.cs[1], sX, sG);
+  {
+    const bool has = nleg > 0 && s.lhyd_scat;
+    const double *P0 = w.T + (size_t)itf * NN2, *P1 = w.R + (size_t)itf * NN2;
+    for (int e = t; e < NN2; e += RT_WG) {
+      const int i = e / NN, j = e % NN;
+      sX[i * LD + j] = has ? (float)P0[e] : 0.f;
+      sG[i * LD + j] = has ? (float)P1[e] : 0.f;
+    }
+  }
   __syncthreads();
   const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
   if (t < NMU) {  // CHECK_NORM4
@@ -797,7 +704,6 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     sR[r * LD + c] = (float)(C2 * tmp * sqw[j1] * pm);
     sT[r * LD + c] = (float)(-tmp * (ext - C2 * sqw[j1] * pp));
   }
-  __syncthreads();  // the cos/sin tables share storage with the sources
   if (t < NN) {
     const int i = t % NST, j = t / NST;
     const double e = (i == 0) ? gas : 0.0;
