@@ -782,13 +782,27 @@ struct Stack {
   double *Rp, *Rm, *Tp, *Tm, *Sp, *Sm;
 };
 
-// v_out[i] = A v_in[i] chain helper for the 32 threads t < NN, with
-// barriers so that the shared vectors may be reused between stages
+// Row i = t/8 of A x on all 256 threads: lane t%8 sums the columns
+// 4 (t%8) .. 4 (t%8) + 3 (conflict-free LDS banks), the eight lanes of a row
+// combine by shuffles; all eight return the full sum.  No barrier.
+__device__ __forceinline__ double mv_row8(const double *A, const double *x, int i) {
+  const int c0 = 4 * (threadIdx.x & 7);
+  double s = A[i * LD + c0] * x[c0];
+#pragma unroll
+  for (int k = 1; k < 4; k++) s += A[i * LD + c0 + k] * x[c0 + k];
+  s += __shfl_xor(s, 1, 64);
+  s += __shfl_xor(s, 2, 64);
+  s += __shfl_xor(s, 4, 64);
+  return s;
+}
+
+// v <- A v on all threads, with barriers so that the shared vector may be
+// reused between stages
 __device__ __forceinline__ void mv_stage(const double *A, double *v) {
   const int t = threadIdx.x;
-  double u = (t < NN) ? mv_row(A, v, t) : 0.0;
+  const double u = mv_row8(A, v, t >> 3);
   __syncthreads();
-  if (t < NN) v[t] = u;
+  if ((t & 7) == 0) v[t >> 3] = u;
   __syncthreads();
 }
 
@@ -800,7 +814,10 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
                              const double *S2m, double *G, double *X, double *vs, InvLds &sinv, int &np) {
   const int t = threadIdx.x;
   if (!inv_i_minus_ab<GJ>(R2, s1.Rp, X, G, sinv, np)) return false;
-  if (t < NN) vs[t] = S2m[t] + mv_row(R2, s1.Sp, t);  // S2m + R2m S1p
+  {
+    const double u = mv_row8(R2, s1.Sp, t >> 3);
+    if ((t & 7) == 0) vs[t >> 3] = S2m[t >> 3] + u;  // S2m + R2m S1p
+  }
   __syncthreads();
   mv_stage(G, vs);
   mv_stage(s1.Tm, vs);
@@ -812,7 +829,10 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
   mm(s1.Tm, X, s1.Tm);                                // T1m G T2m
   np += 5;
   if (!inv_i_minus_ab<GJ>(s1.Rp, R2, X, G, sinv, np)) return false;
-  if (t < NN) vs[t] = s1.Sp[t] + mv_row(s1.Rp, S2m, t);  // S1p + R1p S2m
+  {
+    const double u = mv_row8(s1.Rp, S2m, t >> 3);
+    if ((t & 7) == 0) vs[t >> 3] = s1.Sp[t >> 3] + u;  // S1p + R1p S2m
+  }
   __syncthreads();
   mv_stage(G, vs);
   mv_stage(T2, vs);
@@ -831,16 +851,13 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
 // T2 = diag(dg)).  With R2 = 0 both gammas are the identity and every
 // product reduces to a diagonal scaling or one matrix-vector product.
 __device__ void combine_clear(Stack &s1, const double *dg, const double *S2p, const double *S2m) {
-  const int t = threadIdx.x;
-  double u = 0.0, u2 = 0.0;
-  if (t < NN) {
-    u = mv_row(s1.Tm, S2m, t);              // T1m S2m
-    u2 = s1.Sp[t] + mv_row(s1.Rp, S2m, t);  // S1p + R1p S2m
-  }
+  const int t = threadIdx.x, i8 = t >> 3;
+  const double u = mv_row8(s1.Tm, S2m, i8);                // T1m S2m
+  const double u2 = s1.Sp[i8] + mv_row8(s1.Rp, S2m, i8);  // S1p + R1p S2m
   __syncthreads();
-  if (t < NN) {
-    s1.Sm[t] = s1.Sm[t] + u;
-    s1.Sp[t] = S2p[t] + dg[t] * u2;
+  if ((t & 7) == 0) {
+    s1.Sm[i8] = s1.Sm[i8] + u;
+    s1.Sp[i8] = S2p[i8] + dg[i8] * u2;
   }
   for (int e = t; e < NN2; e += RT_WG) {
     const int i = e / NN, j = e % NN;
@@ -858,7 +875,10 @@ __device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *
                                 InvLds &sinv, int &np) {
   const int t = threadIdx.x;
   if (!inv_i_minus_ab<GJ>(Rs, s1.Rp, X, G, sinv, np)) return false;
-  if (t < NN) vs[t] = mv_row(Rs, s1.Sp, t);
+  {
+    const double u = mv_row8(Rs, s1.Sp, t >> 3);
+    if ((t & 7) == 0) vs[t >> 3] = u;
+  }
   __syncthreads();
   mv_stage(G, vs);
   mv_stage(s1.Tm, vs);
@@ -878,37 +898,72 @@ __device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *
   return true;
 }
 
-// Loads bottom-up layer z: a doubled layer into (R2, T2, S2p, S2m), or the
-// closed form of a clear layer into (dg, S2p, S2m).  Returns true when the
-// layer scatters.
-__device__ bool load_layer(const Dims &d, const Work &w, int col, int f, int z, double wl,
-                           const double *smu, double *R2, double *T2, double *dg, double *S2p,
-                           double *S2m, int *sbad) {
+// The next layer's inputs, fetched into registers while the current layer
+// is combined (the barriers of the combine do not wait for global loads).
+struct LayerFetch {
+  int itf;                       // scattering unit, or -1 for a clear layer
+  int st;                        // its status (thread 0)
+  double r[NN2 / RT_WG], tt[NN2 / RT_WG];  // R, T elements t + RT_WG k
+  double sp, sm;                 // S+, S- (t < NN)
+  double h0, h1, kg, tp0, tp1;   // clear layer: levels, gas, temperatures
+};
+
+__device__ __forceinline__ void fetch_layer(const Dims &d, const Work &w, int col, int f, int z,
+                                            LayerFetch &F) {
   const int t = threadIdx.x;
-  const size_t oy = (size_t)col * d.ML + z;
-  const int item = w.layer_item[oy];
+  const int item = w.layer_item[(size_t)col * d.ML + z];
   if (item >= 0) {
     const size_t itf = (size_t)(item - d.item0) * d.nf + f;
-    if (t == 0 && (w.itf_status[itf] & 0xff)) atomicMax(sbad, w.itf_status[itf]);
+    F.itf = (int)itf;
     const double *R = w.R + itf * NN2, *T = w.T + itf * NN2;
-    for (int e = t; e < NN2; e += RT_WG) {
-      const int i = e / NN, j = e % NN;
-      R2[i * LD + j] = R[e];
-      T2[i * LD + j] = T[e];
+#pragma unroll
+    for (int k = 0; k < NN2 / RT_WG; k++) {
+      F.r[k] = R[t + RT_WG * k];
+      F.tt[k] = T[t + RT_WG * k];
     }
     if (t < NN) {
-      S2p[t] = w.Sp[itf * NN + t];
-      S2m[t] = w.Sm[itf * NN + t];
+      F.sp = w.Sp[itf * NN + t];
+      F.sm = w.Sm[itf * NN + t];
+    }
+    if (t == 0) F.st = w.itf_status[itf];
+  } else {
+    F.itf = -1;
+    const size_t ol = (size_t)col * (d.ML + 1) + z;
+    F.h0 = w.hgt_lev[ol];
+    F.h1 = w.hgt_lev[ol + 1];
+    F.kg = w.kgas[((size_t)col * d.nf + f) * d.ML + z];
+    F.tp0 = w.temp_lev[ol];
+    F.tp1 = w.temp_lev[ol + 1];
+  }
+}
+
+// Stages a fetched layer: a doubled layer into (R2, T2, S2p, S2m), or the
+// closed form of a clear layer into (dg, S2p, S2m).  Returns true when the
+// layer scatters.
+__device__ bool stage_layer(const LayerFetch &F, double wl, const double *smu, double *R2,
+                            double *T2, double *dg, double *S2p, double *S2m, int *sbad) {
+  const int t = threadIdx.x;
+  if (F.itf >= 0) {
+    if (t == 0 && (F.st & 0xff)) atomicMax(sbad, F.st);
+#pragma unroll
+    for (int k = 0; k < NN2 / RT_WG; k++) {
+      const int e = t + RT_WG * k, i = e / NN, j = e % NN;
+      R2[i * LD + j] = F.r[k];
+      T2[i * LD + j] = F.tt[k];
+    }
+    if (t < NN) {
+      S2p[t] = F.sp;
+      S2m[t] = F.sm;
     }
     __syncthreads();
     return true;
   }
-  const size_t ol = (size_t)col * (d.ML + 1) + z;
-  const double zdiff = fabs(w.hgt_lev[ol + 1] - w.hgt_lev[ol]);
-  const double gas = fmax(w.kgas[((size_t)col * d.nf + f) * d.ML + z], 0.0);
+  // NONSCATTER_LAYER4 (radintg4.f90:79-117)
+  const double zdiff = fabs(F.h1 - F.h0);
+  const double gas = fmax(F.kg, 0.0);
   const double dtau = zdiff * gas;
-  const double p0 = planck(w.temp_lev[ol + 1], wl);  // top of the layer
-  const double p1 = planck(w.temp_lev[ol], wl);      // bottom
+  const double p0 = planck(F.tp1, wl);  // top of the layer
+  const double p1 = planck(F.tp0, wl);  // bottom
   if (t < NN) {
     const int i = t % NST, j = t / NST;
     dg[t] = exp(-dtau / smu[j]);
@@ -1101,9 +1156,12 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
   // stack above the surface level is the stack the top level continues)
   unsigned long long nscat = 0, nclear = 0;
   int nprod = 0;
+  LayerFetch F;
+  fetch_layer(d, w, col, f, L - 1, F);
   for (int l = 1; l <= L; l++) {
     const int z = L - l;
-    const bool sc = load_layer(d, w, col, f, z, wl, smu, R2, T2, dg, S2p, S2m, sbad);
+    const bool sc = stage_layer(F, wl, smu, R2, T2, dg, S2p, S2m, sbad);
+    if (l < L) fetch_layer(d, w, col, f, z - 1, F);
     if (l == 1) {
       for (int e = t; e < NN2; e += RT_WG) {
         const int i = e / NN, j = e % NN;
