@@ -163,8 +163,8 @@ __global__ void k_phase_scatter(int nif, Settings s, Work w) {
 
 typedef double d4p_t __attribute__((ext_vector_type(4)));
 
-// One tile: up to 16 units of one NUMPTS.  Each wave computes 32 of the 128
-// output tiles (16 pairs of one entry e11, e12, e21 or e22).  16x16x4 f64
+// One tile: up to 16 units of one NUMPTS.  Each wave computes 16 of the 64
+// output tile pairs (16 angle pairs, entries e11/e12 or e21/e22).  16x16x4 f64
 // MFMA layout: lane l supplies A[l%16][l/16], B[l/16][l%16] and holds
 // D[4*r+l/16][l%16].
 __global__ void __launch_bounds__(PH_WG) k_phase(Work w) {
@@ -207,35 +207,38 @@ __global__ void __launch_bounds__(PH_WG) k_phase(Work w) {
   }
   __syncthreads();
   const double *Q = w.ptab + w.ptab_off[m];
-  for (int T = wv; T < 4 * (PH_NPAIR / 16); T += 4) {
-    const int ent = T / (PH_NPAIR / 16), pt = T % (PH_NPAIR / 16);
+  // entries (e11, e12) and (e21, e22) of 16 pairs per iteration, so that a
+  // lane stores two adjacent doubles of its row
+  for (int T = wv; T < 2 * (PH_NPAIR / 16); T += 4) {
+    const int hi = T / (PH_NPAIR / 16), pt = T % (PH_NPAIR / 16);
     const int pair = 16 * pt + li;
-    d4p_t acc = {0.0, 0.0, 0.0, 0.0};
-    // entry -> (coefficient set, Q component) of the first and, for e22,
-    // the second sum
-    const int ca = ent == 0 ? 0 : (ent == 3 ? 0 : 1);
-    const int qa = ent == 0 ? 0 : (ent == 1 ? 1 : (ent == 2 ? 2 : 3));
+    d4p_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    // hi = 0: e11 = c1 Q1, e12 = c2 Q12; hi = 1: e21 = c2 Q21, e22 = c1 Qa - c3 Qb
+    const int ca0 = hi ? 1 : 0, qa0 = hi ? 2 : 0, ca1 = hi ? 0 : 1, qa1 = hi ? 3 : 1;
     for (int sk = 0; sk < L1p / 4; sk++) {
       const int l = 4 * sk + lk;
-      const double a = sA[ca][li][l];
-      const double b = l < L1 ? Q[((size_t)qa * L1 + l) * PH_NPAIR + pair] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      const bool in = l < L1;
+      const double b0 = in ? Q[((size_t)qa0 * L1 + l) * PH_NPAIR + pair] : 0.0;
+      const double b1 = in ? Q[((size_t)qa1 * L1 + l) * PH_NPAIR + pair] : 0.0;
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[ca0][li][l], b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[ca1][li][l], b1, acc1, 0, 0, 0);
     }
-    if (ent == 3) {  // - sum_l c3_l Qb_l
+    if (hi) {  // e22: - sum_l c3_l Qb_l
       for (int sk = 0; sk < L1p / 4; sk++) {
         const int l = 4 * sk + lk;
-        const double a = sA[2][li][l];
         const double b = l < L1 ? -Q[((size_t)4 * L1 + l) * PH_NPAIR + pair] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[2][li][l], b, acc1, 0, 0, 0);
       }
     }
     const int q = pair / (NMU * NMU), j2 = (pair / NMU) % NMU, j1 = pair % NMU;
-    const int r = NST * j2 + (ent >> 1), c = NST * j1 + (ent & 1);
+    const int r = NST * j2 + hi, c = NST * j1;
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
-      const int i = 4 * rr + lk;
-      const int itf = s_unit[i];
-      if (itf >= 0) (q ? w.R : w.T)[(size_t)itf * NN2 + r * NN + c] = acc[rr];
+      const int itf = s_unit[4 * rr + lk];
+      if (itf >= 0) {
+        double2 *dst = reinterpret_cast<double2 *>((q ? w.R : w.T) + (size_t)itf * NN2 + r * NN + c);
+        *dst = make_double2(acc0[rr], acc1[rr]);
+      }
     }
   }
 }
