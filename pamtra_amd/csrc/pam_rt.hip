@@ -498,8 +498,9 @@ __global__ void __launch_bounds__(RT_WG) k_doubling_gj(Dims d, Settings s, const
 //   T~' = Z~ + (I + Z~) T~,   R' = R + (I + Z~) V,
 // four products as in the FP64 kernel; the identity is added to the A
 // operand while it is loaded.  The phase matrix, its normalisation check,
-// the thin-layer start and the doubling count are computed in FP64; the
-// layer operators are written back in FP64 for the adding kernel.
+// the start layer (a higher-order start, see doubling_unit_f32) and the
+// doubling count are computed in FP64; the layer operators are written back
+// in FP64 for the adding kernel.
 
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
@@ -620,23 +621,82 @@ __device__ bool series_f32(const float *R, float *X, float *Gt, float *wmax, int
   return false;
 }
 
-struct DblLdsF {
-  float R[MSZ], T[MSZ], G[MSZ], X[MSZ];
-  struct {
-    float v[16][NN];  // S+, S-, C+, C-, then source-chain scratch
+// G = inv(I - X) for a given X (overwritten by its powers) by the product-
+// form series, truncated when max |X^(2^k)| < tol.  false: not converged
+// after NEU_MAXIT factors, or a NaN.
+__device__ bool series_inv_given(double *X, double *G, float *wmax, double tol, int &np) {
+  const int t = threadIdx.x;
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int i = e / NN, j = e % NN;
+    G[i * LD + j] = (i == j ? 1.0 : 0.0) + X[i * LD + j];
+  }
+  double m = mm_core(X, X, nullptr, X, wmax);  // its barriers order the G writes
+  np++;
+  for (int it = 0; it < NEU_MAXIT; it++) {
+    if (!(m < 0x1p64)) return false;
+    if (m < tol) return true;
+    if (32.0 * m * m < tol) {
+      mm_add(G, G, X, G);
+      np++;
+      return true;
+    }
+    m = mm2(G, X, G, G, X, X, nullptr, X, wmax);
+    np += 2;
+  }
+  return false;
+}
+
+// Start layer of the mixed-precision path.
+//
+// The reference doubles from a first-order layer of optical depth <= 1e-6
+// (radtran4.f90:323-345, INITIALIZE4 radintg4.f90:1-45), so ~16 doublings
+// per layer, and its result carries that start's truncation error (~1e-5
+// relative in R).  Here the start layer is the exact solution of the same
+// discrete-ordinate system over a sub-layer zdiff / 2^nd, computed in FP64
+// to ~1e-8, with nd the smallest count that makes ||H dz||_inf <= HS_THETA;
+// that removes about 13 doublings.
+//
+// With the first-order operators R1 = dz A, T1 = I - dz B (A, B per unit
+// length, from the phase matrix and the extinction) the intensities of the
+// two hemispheres obey d(I+, I-)/dt = H (I+, I-) + (e, -e) b(t),
+// H = [[-B, A], [-A, B]].  In p = I+ + I-, m = I+ - I- the generator is
+// [[0, -S], [-D, 0]] with S = A + B, D = B - A, whose exponential over dz
+// has the blocks c(X), s(X) S, s(Y) D, c(Y) with X = S D dz^2,
+// Y = D S dz^2, c(Z) = sum Z^k / (2k)!, s(Z) = sum Z^k / (2k+1)!.  The
+// transmission and reflection of the sub-layer are then
+//   U = c(X) + dz s(X) S,  V = c(Y) + dz s(Y) D,
+//   T = ((U + V) / 2)^-1,  R = ((U - V) / 2) T.
+// For the Planck function b0 (1 + f t / dz) the particular solution is
+// g = sum_n H^n (e, -e) dz^(n+1) [1/(n+1)! + f/(n+2)!], and the sources are
+// S+ = g+ - R g-, S- = -T g-.  Doubling from there with the reference's
+// linear-source recursion (linfac = f) gives the layer's R, T, S.
+// c and s are truncated after Z^2 and the particular sum after n = 7: the
+// start's relative error is ~theta^6/720.
+constexpr double HS_THETA = 0.25;
+constexpr double HS_TOL = 0x1p-40;  // series truncation of the start's inverse
+
+struct MixLds {
+  union {
+    double M[4][MSZ];  // FP64 start
+    struct {
+      float R[MSZ], T[MSZ], G[MSZ], X[MSZ];
+      float v[16][NN];  // S+, S-, C+, C-, then source-chain scratch
+    } f;                // FP32 doubling
   } u;
+  double g[4][NN];  // particular sums: constant p, m; ramp p, m
+  double c[2][NN];  // chain vectors, then g- of the two sources
+  double hrow[NN];
   double norm[NMU];
   float wmax[4];
   double mu[NMU], qw[NMU];
 };
 
 // One (scattering layer, channel) unit of the mixed-precision path; false
-// when the FP32 series did not converge (nothing written; the unit is
+// when an inverse series did not converge (nothing written; the unit is
 // recomputed in FP64 by the Gauss-Jordan pass).
 __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double *freqs,
-                                  const Work &w, int itf, DblLdsF &L) {
-  float *sR = L.R, *sT = L.T, *sG = L.G, *sX = L.X;
-  float(*sv)[NN] = L.u.v;
+                                  const Work &w, int itf, MixLds &L) {
+  double *M0 = L.u.M[0], *M1 = L.u.M[1], *M2 = L.u.M[2], *M3 = L.u.M[3];
   double *smu = L.mu, *sqw = L.qw, *snorm = L.norm;
 
   const int t = threadIdx.x;
@@ -653,25 +713,41 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     kext = 0.0;
     salb = 0.0;
   }
-  // the phase matrix, formed by k_phase in the unit's T and R slots
+  // the phase matrix, formed by k_phase in the unit's T and R slots:
+  // M0 = (+,+) quadrant, M1 = (-,+) quadrant
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
   {
     const bool has = nleg > 0 && s.lhyd_scat;
     const double *P0 = w.T + (size_t)itf * NN2, *P1 = w.R + (size_t)itf * NN2;
     for (int e = t; e < NN2; e += RT_WG) {
       const int i = e / NN, j = e % NN;
-      sX[i * LD + j] = has ? (float)P0[e] : 0.f;
-      sG[i * LD + j] = has ? (float)P1[e] : 0.f;
+      M0[i * LD + j] = has ? P0[e] : 0.0;
+      M1[i * LD + j] = has ? P1[e] : 0.0;
     }
   }
   __syncthreads();
   const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
+  const double wl = C_LIGHT / (freqs[f] * 1.e3);
+  const double zdiff = fabs(w.hgt_lev[(size_t)col * (d.ML + 1) + z + 1] -
+                            w.hgt_lev[(size_t)col * (d.ML + 1) + z]);
+  const double gas = fmax(w.kgas[((size_t)col * d.nf + f) * d.ML + z], 0.0);
+  const double p0 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z + 1], wl);
+  const double p1 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z], wl);
+  const double ext = kext + gas;
+  const double C2 = 2.0 * 3.1415926535897932384;
   if (t < NMU) {  // CHECK_NORM4
     double sum = absorb - kext;
     for (int j2 = 0; j2 < NMU; j2++)
-      sum = sum + 2.0 * PI * sqw[j2] *
-                      ((double)sX[(NST * j2) * LD + NST * t] + (double)sG[(NST * j2) * LD + NST * t]);
+      sum = sum + 2.0 * PI * sqw[j2] * (M0[(NST * j2) * LD + NST * t] + M1[(NST * j2) * LD + NST * t]);
     snorm[t] = fabs(sum);
+  } else if (t >= 64 && t < 64 + NN) {  // row r of |A| + |B|, per unit length
+    const int r = t - 64;
+    double sum = 0.0;
+    for (int c = 0; c < NN; c++) {
+      const double wc = C2 * sqw[c / NST];
+      sum += fabs(wc * M1[r * LD + c]) + fabs((c == r ? ext : 0.0) - wc * M0[r * LD + c]);
+    }
+    L.hrow[r] = sum / smu[r / NST];
   }
   __syncthreads();
   if (t == 0) {
@@ -679,43 +755,125 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     for (int j = 0; j < NMU; j++) mx = fmax(mx, snorm[j]);
     if (mx > 5.0e-4) set_err(&w.itf_status[itf], E_NORM);
   }
-  const double wl = C_LIGHT / (freqs[f] * 1.e3);
-  const double zdiff = fabs(w.hgt_lev[(size_t)col * (d.ML + 1) + z + 1] -
-                            w.hgt_lev[(size_t)col * (d.ML + 1) + z]);
-  const double gas = fmax(w.kgas[((size_t)col * d.nf + f) * d.ML + z], 0.0);
-  const double p0 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z + 1], wl);
-  const double p1 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z], wl);
-  const double extinct = kext + gas;
-  const double fnd = log(fmax(extinct * zdiff, 1.0e-7) / 1.0e-6) / (double)logf(2.0f);
-  const int nd = (fnd > 0.0) ? (int)fnd + 1 : 0;
-  const double nsub = (double)ldexpf(1.0f, nd);
-  const double dz = zdiff / nsub;
-  double linfac = (p0 == 0.0) ? 0.0 : (p1 / p0 - 1.0) / nsub;
-  // thin-layer start in FP64 (INITIALIZE4 / INITIAL_SOURCE4), stored as
-  // R and T~ = T - I in FP32
-  const double C2 = 2.0 * 3.1415926535897932384;
+  double hn = 0.0;
+  for (int r = 0; r < NN; r++) hn = max_nan(hn, L.hrow[r]);
+  int nd = 0;
+  while (nd < 64 && hn * zdiff > ldexp(HS_THETA, nd)) nd++;
+  const double dz = ldexp(zdiff, -nd);
+  double linfac = (p0 == 0.0) ? 0.0 : ldexp(p1 / p0 - 1.0, -nd);
+  // dz S into M0, dz D into M1; dz e into the chain vector
   for (int e = t; e < NN2; e += RT_WG) {
     const int r = e / NN, c = e % NN;
-    const int i2 = r % NST, j2 = r / NST, i1 = c % NST, j1 = c / NST;
-    const double tmp = dz / smu[j2];
-    const double gext = (i1 == i2) ? gas : 0.0;
-    const double ext = (j1 == j2) ? ((i1 == i2) ? kext : 0.0) + gext : 0.0;
-    const double pm = (double)sG[r * LD + c], pp = (double)sX[r * LD + c];
-    sR[r * LD + c] = (float)(C2 * tmp * sqw[j1] * pm);
-    sT[r * LD + c] = (float)(-tmp * (ext - C2 * sqw[j1] * pp));
+    const double sc = dz / smu[r / NST], wc = C2 * sqw[c / NST];
+    const double a = sc * wc * M1[r * LD + c];
+    const double b = sc * ((c == r ? ext : 0.0) - wc * M0[r * LD + c]);
+    M0[r * LD + c] = b + a;
+    M1[r * LD + c] = b - a;
   }
-  if (t < NN) {
-    const int i = t % NST, j = t / NST;
-    const double e = (i == 0) ? gas : 0.0;
-    const double tmp = p0 * dz / smu[j];
-    const float v = (float)(tmp * ((i == 0 ? absorb : 0.0) + e));
-    sv[0][t] = v;
-    sv[1][t] = v;
-    sv[2][t] = v;
-    sv[3][t] = v;
+  double *cv = L.c[0], *cw = L.c[1];
+  if (t < NN) cv[t] = (t % NST == 0) ? dz * (absorb + gas) / smu[t / NST] : 0.0;
+  __syncthreads();
+  // particular sums with v_0 = dz e, w_k = dz S v_k, v_k+1 = dz D w_k:
+  //   constant: m = 2 sum v_k / (2k+1)!,  p = -2 sum w_k / (2k+2)!
+  //   ramp:     m = 2 sum v_k / (2k+2)!,  p = -2 sum w_k / (2k+3)!
+  {
+    constexpr double ifact[10] = {1.0,         1.0,          0.5,           1.0 / 6.0,
+                                  1.0 / 24.0,  1.0 / 120.0,  1.0 / 720.0,   1.0 / 5040.0,
+                                  1.0 / 40320.0, 1.0 / 362880.0};
+    double gcp = 0.0, gcm = 0.0, grp = 0.0, grm = 0.0;
+    const int i = (t >> 1) & 31;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (t < 64) {
+        const double v = cv[i];
+        gcm += 2.0 * v * ifact[2 * k + 1];
+        grm += 2.0 * v * ifact[2 * k + 2];
+        const double wk = mv_pair(M0, cv, i);
+        gcp -= 2.0 * wk * ifact[2 * k + 2];
+        grp -= 2.0 * wk * ifact[2 * k + 3];
+        if ((t & 1) == 0) cw[i] = wk;
+      }
+      __syncthreads();
+      if (k < 3) {
+        if (t < 64) {
+          const double v = mv_pair(M1, cw, i);
+          if ((t & 1) == 0) cv[i] = v;
+        }
+        __syncthreads();
+      }
+    }
+    if (t < 64 && (t & 1) == 0) {
+      L.g[0][i] = gcp;
+      L.g[1][i] = gcm;
+      L.g[2][i] = grp;
+      L.g[3][i] = grm;
+    }
+  }
+  // (the next product's barriers order these writes)
+  int nprod = 7;
+  mm(M0, M1, M2);  // X
+  mm(M2, M2, M3);  // X^2
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int i = e / NN, j = e % NN;
+    const double id = (i == j) ? 1.0 : 0.0, x = M2[i * LD + j], x2 = M3[i * LD + j];
+    M2[i * LD + j] = id + x * (1.0 / 6.0) + x2 * (1.0 / 120.0);  // s(X)
+    M3[i * LD + j] = id + x * 0.5 + x2 * (1.0 / 24.0);           // c(X)
   }
   __syncthreads();
-  int nprod = 0;
+  mm_add(M3, M2, M0, M3);  // U = c(X) + s(X) dz S
+  mm(M1, M0, M2);          // Y
+  mm(M2, M2, M0);          // Y^2
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int i = e / NN, j = e % NN;
+    const double id = (i == j) ? 1.0 : 0.0, y = M2[i * LD + j], y2 = M0[i * LD + j];
+    M2[i * LD + j] = id + y * (1.0 / 6.0) + y2 * (1.0 / 120.0);  // s(Y)
+    M0[i * LD + j] = id + y * 0.5 + y2 * (1.0 / 24.0);           // c(Y)
+  }
+  __syncthreads();
+  mm_add(M0, M2, M1, M0);  // V = c(Y) + s(Y) dz D
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int i = e / NN, j = e % NN;
+    const double u = M3[i * LD + j], v = M0[i * LD + j];
+    M1[i * LD + j] = (i == j ? 1.0 : 0.0) - 0.5 * (u + v);  // I - (U + V) / 2
+    M3[i * LD + j] = 0.5 * (u - v);
+  }
+  if (t < NN) {  // g- of the linear and of the constant source
+    const double gcmin = 0.5 * (L.g[0][t] - L.g[1][t]), grmin = 0.5 * (L.g[2][t] - L.g[3][t]);
+    cv[t] = p0 * (gcmin + linfac * grmin);
+    cw[t] = p0 * gcmin;
+  }
+  __syncthreads();
+  if (!series_inv_given(M1, M2, L.wmax, HS_TOL, nprod)) return false;  // T in M2
+  mm(M3, M2, M0);                                                       // R
+  nprod++;
+  // S+ = g+ - R g-, S- = -T g-: linear source (groups 0, 1), constant (2, 3)
+  float src;
+  {
+    const int g = t >> 6, i = (t >> 1) & 31;
+    const double *A = (g & 1) ? M2 : M0;
+    const double mv = mv_pair(A, g < 2 ? cv : cw, i);
+    const double gcplus = 0.5 * (L.g[0][i] + L.g[1][i]), grplus = 0.5 * (L.g[2][i] + L.g[3][i]);
+    const double gplus = p0 * (g < 2 ? gcplus + linfac * grplus : gcplus);
+    src = (float)((g & 1) ? -mv : gplus - mv);
+  }
+  float rr[4], tt[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int e = t + q * RT_WG, i = e / NN, j = e % NN;
+    rr[q] = (float)M0[i * LD + j];
+    tt[q] = (float)(M2[i * LD + j] - (i == j ? 1.0 : 0.0));
+  }
+  __syncthreads();
+  float *sR = L.u.f.R, *sT = L.u.f.T, *sG = L.u.f.G, *sX = L.u.f.X;
+  float(*sv)[NN] = L.u.f.v;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int e = t + q * RT_WG, i = e / NN, j = e % NN;
+    sR[i * LD + j] = rr[q];
+    sT[i * LD + j] = tt[q];
+  }
+  if ((t & 1) == 0) sv[t >> 6][(t >> 1) & 31] = src;
+  __syncthreads();
   for (int it = 0; it < nd; it++) {
     if (!series_f32(sR, sX, sG, L.wmax, nprod)) return false;
     // Z~ = T~ + (I + T~) G~ (into G), V = R + R T~
@@ -764,7 +922,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
 
 __global__ void __launch_bounds__(RT_WG) k_doubling_mixed(Dims d, Settings s, const double *freqs,
                                                           Work w) {
-  __shared__ DblLdsF L;
+  __shared__ MixLds L;
   const int itf = blockIdx.x;
   if (!doubling_unit_f32(d, s, freqs, w, itf, L) && threadIdx.x == 0) {
     const unsigned q = atomicAdd(&w.redo_n[0], 1u);

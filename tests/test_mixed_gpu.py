@@ -44,7 +44,12 @@ def test_mixed_matches_fp64_per_channel(cfg, ncol):
     a, st = _run(pam, freqs, "fp64")
     b, st2 = _run(pam, freqs, "mixed")
     assert st == 0 and st2 == 0
-    np.testing.assert_array_equal(a["ndoubles"], b["ndoubles"])
+    # the same layers scatter; the mixed path's higher-order start layer
+    # needs fewer doublings than the reference's first-order one
+    na, nb = a["ndoubles"], b["ndoubles"]
+    np.testing.assert_array_equal(na < 0, nb < 0)
+    assert np.all(nb[na >= 0] <= na[na >= 0])
+    assert np.mean(nb[na >= 0]) < np.mean(na[na >= 0]) - 8
     per = _per_channel(a, b, freqs)
     assert max(per) <= MIXED_TOL
 
