@@ -188,15 +188,34 @@ __device__ __forceinline__ cd cdiv_m(cd a, cd b) {
 
 typedef double d4m_t __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(3))) k_mie(Dims d, Settings s, const pam_hydro_desc *hyd,
+// Angle-table steps of phase B held in registers (k < 4 * MIE_BREG); longer
+// series read the table inside the product loop.
+constexpr int MIE_BREG = 8;
+
+// Sum over the 64 lanes of a wave by a fixed butterfly (the same order on
+// every call, so the result is deterministic).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hydro_desc *hyd,
                                                 const double *freqs, Work w) {
-  __shared__ double s_u[U_CAP];
-  __shared__ double s_q[3][MIE_WG];  // per-bin efficiencies of the chunk; later the phase sums
+  __shared__ double s_u[U_CAP];      // U of the chunk; at the end the normalised phase sums
+  __shared__ double s_bin[3][MIE_WG];  // per bin of the chunk: conj(m) (re, im) and 1/x
   __shared__ double s_wb[MIE_WG];    // per-bin DSD weight n(D) dD of the chunk, 0 if skipped
   __shared__ double s_ps[3][MAX_NQUAD];  // weighted phase sums per angle
   __shared__ double s_lc[NCOEF][MAXNLEG + 1];
+  __shared__ double s_red[4][3];     // per-wave efficiency sums of the chunk
+  __shared__ int s_redi[4][2];       // per-wave bin and series counts
   __shared__ short s_nmb[MIE_MAXBIN];  // series length of every bin (0: skipped)
-  __shared__ short s_cb[MIE_MAXCHUNK + 1], s_cnp[MIE_MAXCHUNK];
+  __shared__ short s_cb[MIE_MAXCHUNK + 1], s_cnp[MIE_MAXCHUNK], s_cnm[MIE_MAXCHUNK];
   __shared__ int s_nc;
   __shared__ double s_sum[3];
   __shared__ int s_nlh, s_nan;
@@ -248,29 +267,59 @@ __global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(3))
     const int nct = (nquad + 15) >> 4;
     for (int k = t; k < 3 * MAX_NQUAD; k += MIE_WG) (&s_ps[0][0])[k] = 0.0;
     double sumqe = 0.0, sumqs = 0.0, sumqb = 0.0;
-    // chunk plan: consecutive bins (ascending size) as long as their rows of
-    // U, with the chunk's own series length, fit the LDS stage
-    for (int e = t; e < nbin; e += MIE_WG)
-      s_nmb[e] = nd[e] > 0.0 ? (short)min(mie_nstop(PI * dia[e] / wavelength), nterms) : 0;
+    // series length of every bin (mie_scat_utilities.f90:34)
+    for (int e = t; e < nbin; e += MIE_WG) {
+      int v = 0;
+      if (nd[e] > 0.0) {
+        const int ns = mie_nstop(PI * dia[e] / wavelength);
+        if (ns > nterms) set_err(st, E_MIE_TERMS);
+        v = min(ns, nterms);
+      }
+      s_nmb[e] = (short)v;
+    }
     __syncthreads();
-    if (t == 0) {
+    // chunk plan (wave 0): consecutive bins (ascending size) as long as their
+    // rows of U, with the chunk's own series length, fit the LDS stage.  A
+    // window of 64 bins at a time: inclusive prefix max of the series
+    // lengths, then the first bin that no longer fits ends the chunk.
+    if (wv == 0) {
       int nc = 0, b = 0;
       while (b < nbin && nc < MIE_MAXCHUNK) {
         int nmax = 0, e = b;
-        while (e < nbin && e - b < MIE_WG) {
-          const int nm2 = max(nmax, (int)s_nmb[e]);
-          const int npc = max(4, (nm2 + 3) & ~3);
-          if (16 * ((e - b + 8) >> 3) * (2 * npc + 1) > U_CAP) break;
-          nmax = nm2;
-          e++;
+        for (;;) {
+          const int idx = e + lane;
+          int v = max(nmax, idx < nbin ? (int)s_nmb[idx] : 0);
+#pragma unroll
+          for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(v, o, 64);
+            if (lane >= o) v = max(v, u);
+          }
+          const int npc = max(4, (v + 3) & ~3);
+          const bool ok = idx < nbin && idx - b < MIE_WG &&
+                          16 * ((idx - b + 8) >> 3) * (2 * npc + 1) <= U_CAP;
+          const unsigned long long bad = __ballot(!ok);
+          if (bad) {
+            const int fl = __ffsll((long long)bad) - 1;
+            const int vprev = __shfl(v, fl > 0 ? fl - 1 : 0, 64);
+            if (fl > 0) nmax = vprev;
+            e += fl;
+            break;
+          }
+          nmax = __shfl(v, 63, 64);
+          e += 64;
         }
-        s_cb[nc] = (short)b;
-        s_cnp[nc] = (short)max(4, (nmax + 3) & ~3);
+        if (lane == 0) {
+          s_cb[nc] = (short)b;
+          s_cnp[nc] = (short)max(4, (nmax + 3) & ~3);
+          s_cnm[nc] = (short)nmax;
+        }
         nc++;
         b = e;
       }
-      s_cb[nc] = (short)b;
-      s_nc = (b < nbin) ? -1 : nc;
+      if (lane == 0) {
+        s_cb[nc] = (short)b;
+        s_nc = (b < nbin) ? -1 : nc;
+      }
     }
     __syncthreads();
     if (s_nc < 0) {
@@ -280,46 +329,41 @@ __global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(3))
 
     for (int c = 0; c < s_nc; c++) {
       const int b0 = s_cb[c], nb = s_cb[c + 1] - b0;
-      const int np = s_cnp[c], kp = 2 * np, ldu = kp + 1;
+      const int np = s_cnp[c], kp = 2 * np, ldu = kp + 1, nlev = s_cnm[c];
       const int nrt = (nb + 7) >> 3;
       for (int e = t; e < 16 * nrt * ldu; e += MIE_WG) s_u[e] = 0.0;
       __syncthreads();
-      // --- A: threads over bins (miecalc, miecross)
+      // --- A1: threads over bins: the two recurrences of miecalc
+      // (mie_scat_utilities.f90:44-72).  Row re/im of bin t: D_n in the
+      // a-columns n-1, psi_n / chi_n in the b-columns n-1, psi_0 / chi_0 in
+      // the pad column kp.
       if (t < nb) {
         const int ir = b0 + t;
-        double q3[3] = {0.0, 0.0, 0.0};
-        int nmie = 0;
-        double wgt = 0.0;
-        if (nd[ir] > 0.0) {
+        const int nmie = s_nmb[ir];
+        if (nmie > 0) {
           // soft sphere: index of the ice-air mixture at the bin's density
           // (mie_spheres.f90:187-192), decided per bin
-          cd msp = (is_ice && rho[ir] != 917.0) ? eps_mix_air(m_ice, rho[ir]) : m_ice;
-          double x = PI * dia[ir] / wavelength;
-          nmie = mie_nstop(x);
-          if (nmie > nterms) {
-            set_err(st, E_MIE_TERMS);
-            nmie = nterms;
-          }
-          cd mc = conj(msp);
-          cd y = mc * x;
-          const cd yinv = cdiv_m(cd{1.0, 0.0}, y), mcinv = cdiv_m(cd{1.0, 0.0}, mc);
+          const cd msp = (is_ice && rho[ir] != 917.0) ? eps_mix_air(m_ice, rho[ir]) : m_ice;
+          const double x = PI * dia[ir] / wavelength;
+          const cd mc = conj(msp);
+          const cd y = mc * x;
+          const cd yinv = cdiv_m(cd{1.0, 0.0}, y);
           const double xinv = rcp_nr(x);
           double *Ure = s_u + u_row_re(t) * ldu, *Uim = Ure + 4 * ldu;
-          // downward log-derivative recursion (mie_scat_utilities.f90:44-50),
-          // D_n kept in the a-columns of the bin's rows
-          int nn = nmie + 15;
+          // downward log-derivative recursion from n = nmie + 15
           cd dn = cd{0.0, 0.0};
-          for (int n = nn; n >= 2; n--) {
-            cd ny = (double)n * yinv;
+          for (int n = nmie + 15; n >= 2; n--) {
+            const cd ny = (double)n * yinv;
             dn = ny - cdiv_m(cd{1.0, 0.0}, dn + ny);
             if (n - 1 <= nmie) {
               Ure[n - 2] = dn.re;
               Uim[n - 2] = dn.im;
             }
           }
+          // upward Riccati-Bessel recursion
           double psim = cos(x), psin = sin(x), chim = -sin(x), chin = cos(x);
-          double s1 = 0.0, s2 = 0.0;
-          cd s3 = cd{0.0, 0.0};
+          Ure[kp] = psin;
+          Uim[kp] = chin;
           for (int n = 1; n <= nmie; n++) {
             const double fn = (double)(2 * n - 1) * xinv;
             double tmp = psin;
@@ -328,75 +372,169 @@ __global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(3))
             tmp = chin;
             chin = fn * chin - chim;
             chim = tmp;
-            cd xin = cd{sp(psin), sp(-chin)};
-            cd xim = cd{sp(psim), sp(-chim)};
-            cd Dn = cd{Ure[n - 1], Uim[n - 1]};
-            const double nx = (double)n * xinv;
-            cd ct = Dn * mcinv + nx;
-            cd an = cdiv_m(ct * psin - psim, ct * xin - xim);
-            ct = mc * Dn + nx;
-            cd bn = cdiv_m(ct * psin - psim, ct * xin - xim);
-            double wn = (double)(2 * n + 1);
-            s1 = s1 + wn * (an.re + bn.re);
-            s2 = s2 + wn * ((an * conj(an)).re + (bn * conj(bn)).re);
-            double sg = (n & 1) ? -1.0 : 1.0;
-            s3 = s3 + (wn * sg) * (an - bn);
-            // c_n in single precision (mie_scat_utilities.f90:144)
-            const double c = (double)((float)(2 * n + 1) / (float)(n * (n + 1)));
-            Ure[n - 1] = c * an.re;
-            Uim[n - 1] = c * an.im;
-            Ure[np + n - 1] = c * bn.re;
-            Uim[np + n - 1] = c * bn.im;
+            Ure[np + n - 1] = psin;
+            Uim[np + n - 1] = chin;
           }
-          double r = dia[ir] / 2.0;
-          double ar = (r * r) * PI;
-          double qe = 2.0 / (x * x) * s1, qs = 2.0 / (x * x) * s2;
-          double qb = ((1.0 / (x * x) * s3) * conj(s3)).re;
-          q3[0] = qe * ar * nd[ir] * dd[ir];
-          q3[1] = qs * ar * nd[ir] * dd[ir];
-          q3[2] = qb * ar * nd[ir] * dd[ir];
-          wgt = nd[ir] * dd[ir];
+          s_bin[0][t] = mc.re;
+          s_bin[1][t] = mc.im;
+          s_bin[2][t] = xinv;
         }
-        s_q[0][t] = q3[0];
-        s_q[1][t] = q3[1];
-        s_q[2][t] = q3[2];
-        s_wb[t] = nmie > 0 ? wgt : 0.0;
+        s_wb[t] = nmie > 0 ? nd[ir] * dd[ir] : 0.0;
       } else if (t < 8 * nrt) {
         s_wb[t] = 0.0;
       }
       __syncthreads();
+      // --- A2: threads over (bin, n): a_n, b_n (mie_scat_utilities.f90:55-72),
+      // stored as c_n a_n, c_n b_n over the recurrence values.  Items run in
+      // descending n (level-major), and every item reads its inputs before
+      // the barrier that precedes the writes, so the b-column n-1 (psi_n,
+      // also read by item n+1) is overwritten only after its last reader.
+      for (int k0 = 0; k0 < nb * nlev; k0 += MIE_WG) {
+        const int k = k0 + t;
+        const int bl = k % nb, n = nlev - k / nb;
+        const bool ok = k < nb * nlev && n <= s_nmb[b0 + bl];
+        cd an = cd{0.0, 0.0}, bn = cd{0.0, 0.0};
+        double *Ure = s_u + u_row_re(bl) * ldu, *Uim = Ure + 4 * ldu;
+        if (ok) {
+          const cd mc = cd{s_bin[0][bl], s_bin[1][bl]};
+          const cd mcinv = cdiv_m(cd{1.0, 0.0}, mc);
+          const double xinv = s_bin[2][bl];
+          const double psin = Ure[np + n - 1], chin = Uim[np + n - 1];
+          const double psim = (n == 1) ? Ure[kp] : Ure[np + n - 2];
+          const double chim = (n == 1) ? Uim[kp] : Uim[np + n - 2];
+          const cd xin = cd{sp(psin), sp(-chin)};
+          const cd xim = cd{sp(psim), sp(-chim)};
+          const cd Dn = cd{Ure[n - 1], Uim[n - 1]};
+          const double nx = (double)n * xinv;
+          cd ct = Dn * mcinv + nx;
+          an = cdiv_m(ct * psin - psim, ct * xin - xim);
+          ct = mc * Dn + nx;
+          bn = cdiv_m(ct * psin - psim, ct * xin - xim);
+        }
+        __syncthreads();
+        if (ok) {
+          // c_n in single precision (mie_scat_utilities.f90:144)
+          const double cn = (double)((float)(2 * n + 1) / (float)(n * (n + 1)));
+          Ure[n - 1] = cn * an.re;
+          Uim[n - 1] = cn * an.im;
+          Ure[np + n - 1] = cn * bn.re;
+          Uim[np + n - 1] = cn * bn.im;
+        }
+      }
+      __syncthreads();
+      // --- efficiencies (miecross, mie_scat_utilities.f90:77-116) of bin t,
+      // summed over n in ascending order from the stored c_n a_n, c_n b_n
+      {
+        double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+        int nbw = 0, nsw = 0;
+        if (t < nb && s_nmb[b0 + t] > 0) {
+          const int ir = b0 + t, nmie = s_nmb[ir];
+          const double *Ure = s_u + u_row_re(t) * ldu, *Uim = Ure + 4 * ldu;
+          double s1 = 0.0, s2 = 0.0;
+          cd s3 = cd{0.0, 0.0};
+          for (int n = 1; n <= nmie; n++) {
+            const double cn = (double)((float)(2 * n + 1) / (float)(n * (n + 1)));
+            const double wn = (double)(2 * n + 1), r = wn / cn;
+            const double ar = Ure[n - 1], ai = Uim[n - 1], br = Ure[np + n - 1],
+                         bi = Uim[np + n - 1];
+            s1 = s1 + r * (ar + br);
+            s2 = s2 + (r / cn) * ((ar * ar + ai * ai) + (br * br + bi * bi));
+            const double rs = (n & 1) ? -r : r;
+            s3 = s3 + rs * cd{ar - br, ai - bi};
+          }
+          const double x = PI * dia[ir] / wavelength;
+          const double rr = dia[ir] / 2.0;
+          const double ar = (rr * rr) * PI;
+          const double qe = 2.0 / (x * x) * s1, qs = 2.0 / (x * x) * s2;
+          const double qb = ((1.0 / (x * x) * s3) * conj(s3)).re;
+          q0 = qe * ar * nd[ir] * dd[ir];
+          q1 = qs * ar * nd[ir] * dd[ir];
+          q2 = qb * ar * nd[ir] * dd[ir];
+          nbw = 1;
+          nsw = nmie;
+        }
+        q0 = wave_sum(q0);
+        q1 = wave_sum(q1);
+        q2 = wave_sum(q2);
+        nbw = wave_sum(nbw);
+        nsw = wave_sum(nsw);
+        if (lane == 0) {
+          s_red[wv][0] = q0;
+          s_red[wv][1] = q1;
+          s_red[wv][2] = q2;
+          s_redi[wv][0] = nbw;
+          s_redi[wv][1] = nsw;
+        }
+      }
       // --- B: S1, S2 of every (bin, angle) on the matrix cores and the
       // DSD-weighted phase-function sums (mieangle, mie_spheres.f90:238-247).
       // Wave wv owns the angle tiles ct = wv, wv + 4, ...
       for (int ct = wv; ct < nct; ct += 4) {
         const int ang = 16 * ct + li;
         const bool ang_ok = ang < nquad;
+        const int nsk = kp >> 2;
         double p1 = 0.0, p2 = 0.0, p3 = 0.0;
-        for (int rt = 0; rt < nrt; rt++) {
-          d4m_t acc1 = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
-          const double *urow = s_u + (16 * rt + li) * ldu;
-          for (int sk = 0; sk < kp / 4; sk++) {
-            const int kk = 4 * sk + lk;
-            const bool ablk = kk < np;
-            const int n = ablk ? kk : kk - np;
-            double pv = 0.0, tv = 0.0;
-            if (ang_ok && n < nterms) {
-              pv = tpi[(size_t)n * nquad + ang];
-              tv = ttau[(size_t)n * nquad + ang];
-            }
-            const double a = urow[kk];
-            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? pv : tv, acc1, 0, 0, 0);
-            acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? tv : pv, acc2, 0, 0, 0);
-          }
+        if (nsk <= MIE_BREG) {
+          // the tile's pi_n, tau_n for every k, loaded once for all row tiles
+          double bp[MIE_BREG], bt[MIE_BREG];
 #pragma unroll
-          for (int h2 = 0; h2 < 2; h2++) {
-            const double wgt = s_wb[8 * rt + 4 * h2 + lk];
-            const double s1r = acc1[2 * h2], s1i = acc1[2 * h2 + 1];
-            const double s2r = acc2[2 * h2], s2i = acc2[2 * h2 + 1];
-            const double a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
-            p1 += (0.5 * (a2 + a1)) * wgt;
-            p2 += (0.5 * (a2 - a1)) * wgt;
-            p3 += (s2r * s1r + s2i * s1i) * wgt;
+          for (int sk = 0; sk < MIE_BREG; sk++) {
+            const int kk = 4 * sk + lk;
+            const int n = kk < np ? kk : kk - np;
+            const bool v = sk < nsk && ang_ok && n < nterms;
+            bp[sk] = v ? tpi[(size_t)n * nquad + ang] : 0.0;
+            bt[sk] = v ? ttau[(size_t)n * nquad + ang] : 0.0;
+          }
+          for (int rt = 0; rt < nrt; rt++) {
+            d4m_t acc1 = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+            const double *urow = s_u + (16 * rt + li) * ldu;
+#pragma unroll
+            for (int sk = 0; sk < MIE_BREG; sk++) {
+              if (sk < nsk) {
+                const bool ablk = 4 * sk < np;
+                const double a = urow[4 * sk + lk];
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? bp[sk] : bt[sk], acc1, 0, 0, 0);
+                acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? bt[sk] : bp[sk], acc2, 0, 0, 0);
+              }
+            }
+#pragma unroll
+            for (int h2 = 0; h2 < 2; h2++) {
+              const double wgt = s_wb[8 * rt + 4 * h2 + lk];
+              const double s1r = acc1[2 * h2], s1i = acc1[2 * h2 + 1];
+              const double s2r = acc2[2 * h2], s2i = acc2[2 * h2 + 1];
+              const double a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
+              p1 += (0.5 * (a2 + a1)) * wgt;
+              p2 += (0.5 * (a2 - a1)) * wgt;
+              p3 += (s2r * s1r + s2i * s1i) * wgt;
+            }
+          }
+        } else {
+          for (int rt = 0; rt < nrt; rt++) {
+            d4m_t acc1 = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+            const double *urow = s_u + (16 * rt + li) * ldu;
+            for (int sk = 0; sk < nsk; sk++) {
+              const int kk = 4 * sk + lk;
+              const bool ablk = kk < np;
+              const int n = ablk ? kk : kk - np;
+              double pv = 0.0, tv = 0.0;
+              if (ang_ok && n < nterms) {
+                pv = tpi[(size_t)n * nquad + ang];
+                tv = ttau[(size_t)n * nquad + ang];
+              }
+              const double a = urow[kk];
+              acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? pv : tv, acc1, 0, 0, 0);
+              acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? tv : pv, acc2, 0, 0, 0);
+            }
+#pragma unroll
+            for (int h2 = 0; h2 < 2; h2++) {
+              const double wgt = s_wb[8 * rt + 4 * h2 + lk];
+              const double s1r = acc1[2 * h2], s1i = acc1[2 * h2 + 1];
+              const double s2r = acc2[2 * h2], s2i = acc2[2 * h2 + 1];
+              const double a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
+              p1 += (0.5 * (a2 + a1)) * wgt;
+              p2 += (0.5 * (a2 - a1)) * wgt;
+              p3 += (s2r * s1r + s2i * s1i) * wgt;
+            }
           }
         }
         // the four lane groups hold different bins of the same angle
@@ -412,17 +550,16 @@ __global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(3))
           s_ps[2][ang] += p3;
         }
       }
-      if (t == 0) {  // efficiency sums in bin order
-        for (int cb = 0; cb < nb; cb++) {
-          if (s_nmb[b0 + cb] <= 0) continue;
-          sumqe = sumqe + s_q[0][cb];
-          sumqs = sumqs + s_q[1][cb];
-          sumqb = sumqb + s_q[2][cb];
-          work_bins++;
-          work_series += s_nmb[b0 + cb];
+      __syncthreads();
+      if (t == 0) {  // the chunk's efficiency sums, waves in order
+        for (int q = 0; q < 4; q++) {
+          sumqe = sumqe + s_red[q][0];
+          sumqs = sumqs + s_red[q][1];
+          sumqb = sumqb + s_red[q][2];
+          work_bins += s_redi[q][0];
+          work_series += s_redi[q][1];
         }
       }
-      __syncthreads();
     }
     if (t == 0) {
       s_sum[0] = sumqe;
@@ -442,24 +579,27 @@ __global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(3))
       if (t == 0) set_err(st, E_MIE_RESULT);
       return;
     }
-    // weighted phase sums per angle, normalised (mie_spheres.f90:272-280)
+    // weighted phase sums per angle, normalised (mie_spheres.f90:272-280),
+    // into the U stage
+    double *s_q = s_u;
     const double tmp = wavelength * wavelength / (PI * scatter);
     if (t < nquad) {
       const double wt = twt[t];
-      s_q[0][t] = tmp * s_ps[0][t] * wt;
-      s_q[1][t] = tmp * s_ps[1][t] * wt;
-      s_q[2][t] = tmp * s_ps[2][t] * wt;
+      s_q[t] = tmp * s_ps[0][t] * wt;
+      s_q[MAX_NQUAD + t] = tmp * s_ps[1][t] * wt;
+      s_q[2 * MAX_NQUAD + t] = tmp * s_ps[2][t] * wt;
     }
     __syncthreads();
     // --- C: Legendre projection (mie_spheres.f90:272-314), thread per order
     double g1 = 0.0, g2 = 0.0, g3 = 0.0;
     if (t <= nlegen) {
       double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+#pragma unroll 8
       for (int k = 0; k < nquad; k++) {
         const double pl = tpl[(size_t)k * (nlegen + 1) + t];
-        c1 += s_q[0][k] * pl;
-        c2 += s_q[1][k] * pl;
-        c3 += s_q[2][k] * pl;
+        c1 += s_q[k] * pl;
+        c2 += s_q[MAX_NQUAD + k] * pl;
+        c3 += s_q[2 * MAX_NQUAD + k] * pl;
       }
       const double hl = (double)((float)(2 * t + 1) / 2.0f);
       g1 = hl * c1;

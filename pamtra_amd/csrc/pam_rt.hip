@@ -966,7 +966,11 @@ __device__ __forceinline__ void mv_stage(const double *A, double *v) {
 
 // COMBINE_LAYERS4 (radintg4.f90:447-513): stack on top of a doubled layer
 // with R2+ = R2- = R2 and T2+ = T2- = T2.  The minus side goes first, since
-// it reads the stack's old R+ and T+.
+// it reads the stack's old R+ and T+.  Each side is grouped as
+//   A = T1m G,  R1m' = R1m + A (R2 T1p),  T1m' = A T2,  S1m' = S1m + A (...)
+//   B = T2 G',  R1p' = R2 + B (R1p T2),   T1p' = B T1p,  S1p' = S2p + B (...)
+// four products in two fused passes per side (the reference's order of the
+// products differs only by association).
 template <bool GJ>
 __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, const double *S2p,
                              const double *S2m, double *G, double *X, double *vs, InvLds &sinv, int &np) {
@@ -976,31 +980,21 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
     const double u = mv_row8(R2, s1.Sp, t >> 3);
     if ((t & 7) == 0) vs[t >> 3] = S2m[t >> 3] + u;  // S2m + R2m S1p
   }
-  __syncthreads();
-  mv_stage(G, vs);
-  mv_stage(s1.Tm, vs);
-  if (t < NN) s1.Sm[t] = s1.Sm[t] + vs[t];            // S1m + T1m G (...)
-  mm(R2, s1.Tp, X);
-  mm(G, X, X);
-  mm_add(s1.Rm, s1.Tm, X, s1.Rm);                     // R1m + T1m G R2m T1p
-  mm(G, T2, X);
-  mm(s1.Tm, X, s1.Tm);                                // T1m G T2m
-  np += 5;
+  mm2(s1.Tm, G, nullptr, X, R2, s1.Tp, nullptr, G, nullptr);  // X = A, G = R2 T1p
+  mv_stage(X, vs);
+  if (t < NN) s1.Sm[t] = s1.Sm[t] + vs[t];                     // S1m + A (...)
+  mm2(X, G, s1.Rm, s1.Rm, X, T2, nullptr, s1.Tm, nullptr);      // R1m', T1m'
+  np += 4;
   if (!inv_i_minus_ab<GJ>(s1.Rp, R2, X, G, sinv, np)) return false;
   {
     const double u = mv_row8(s1.Rp, S2m, t >> 3);
     if ((t & 7) == 0) vs[t >> 3] = s1.Sp[t >> 3] + u;  // S1p + R1p S2m
   }
-  __syncthreads();
-  mv_stage(G, vs);
-  mv_stage(T2, vs);
-  if (t < NN) s1.Sp[t] = S2p[t] + vs[t];              // S2p + T2p G (...)
-  mm(s1.Rp, T2, X);
-  mm(G, X, X);
-  mm_add(R2, T2, X, s1.Rp);                           // R2p + T2p G R1p T2m
-  mm(G, s1.Tp, X);
-  mm(T2, X, s1.Tp);                                   // T2p G T1p
-  np += 5;
+  mm2(T2, G, nullptr, X, s1.Rp, T2, nullptr, G, nullptr);       // X = B, G = R1p T2
+  mv_stage(X, vs);
+  if (t < NN) s1.Sp[t] = S2p[t] + vs[t];                       // S2p + B (...)
+  mm2(X, G, R2, s1.Rp, X, s1.Tp, nullptr, s1.Tp, nullptr);      // R1p', T1p'
+  np += 4;
   __syncthreads();
   return true;
 }
@@ -1314,9 +1308,87 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
   // stack above the surface level is the stack the top level continues)
   unsigned long long nscat = 0, nclear = 0;
   int nprod = 0;
+  // The clear layers above the highest scattering layer.  While the stack
+  // holds only clear layers it is R = 0, T = diag(td), and the lanes of
+  // wave 0 fold those layers in registers with the operations
+  // combine_clear performs on the non-zero entries.  Lane k of each window
+  // of 64 layers loads the bottom level of layer L-1-k and its Planck
+  // radiance; the top level is the previous layer's bottom.
+  int npre = L;
+  for (int b = 0; b < L; b += 64) {
+    const int k = b + (t & 63);
+    const bool scat = k >= L || w.layer_item[(size_t)col * d.ML + (L - 1 - k)] >= 0;
+    const unsigned long long m = __ballot(scat);
+    if (m) {
+      npre = min(L, b + __ffsll((long long)m) - 1);
+      break;
+    }
+  }
+  if (npre > 0) {
+    if (t < 64) {
+      const size_t ol = (size_t)col * (d.ML + 1);
+      const double *kgc = w.kgas + ((size_t)col * d.nf + f) * d.ML;
+      const int ii = t % NST;
+      const double muj = smu[(t % NN) / NST];
+      double hprev = w.hgt_lev[ol + L], pprev = planck(w.temp_lev[ol + L], wl);
+      double td = 1.0, spv = 0.0, smv = 0.0;
+      for (int b = 0; b < npre; b += 64) {
+        const int k = b + t;
+        double hk = 0.0, pk = 0.0, gk = 0.0;
+        if (k < npre) {
+          const int z = L - 1 - k;
+          hk = w.hgt_lev[ol + z];
+          pk = planck(w.temp_lev[ol + z], wl);
+          gk = kgc[z];
+        }
+        const int cnt = min(64, npre - b);
+        for (int q = 0; q < cnt; q++) {
+          // NONSCATTER_LAYER4 (radintg4.f90:79-117), top hprev/pprev
+          const double h0 = __shfl(hk, q, 64), p1 = __shfl(pk, q, 64), kg = __shfl(gk, q, 64);
+          const double p0 = pprev;
+          const double dtau = fabs(hprev - h0) * fmax(kg, 0.0);
+          const double dgl = exp(-dtau / muj);
+          double a = 0.0, bs = 0.0;
+          if (ii == 0 && dtau > 0.0) {
+            const double path = dtau / muj;
+            const double slope = (p1 - p0) / path;
+            a = p1 - slope - (p1 - slope * (1.0 + path)) * dgl;
+            bs = p0 + slope - (p0 + slope * (1.0 + path)) * dgl;
+          }
+          if (b + q == 0) {
+            td = dgl;
+            spv = a;
+            smv = bs;
+          } else {
+            smv = smv + td * bs;
+            spv = a + dgl * spv;
+            td = dgl * td;
+          }
+          hprev = h0;
+          pprev = p1;
+        }
+      }
+      if (t < NN) {
+        stk.Sp[t] = spv;
+        stk.Sm[t] = smv;
+        dg[t] = td;
+      }
+    }
+    __syncthreads();
+    for (int e = t; e < NN2; e += RT_WG) {
+      const int i = e / NN, j = e % NN;
+      const double tt = (i == j) ? dg[i] : 0.0;
+      stk.Rp[i * LD + j] = 0.0;
+      stk.Rm[i * LD + j] = 0.0;
+      stk.Tp[i * LD + j] = tt;
+      stk.Tm[i * LD + j] = tt;
+    }
+    __syncthreads();
+    nclear += (unsigned long long)(npre - 1);
+  }
   LayerFetch F;
-  fetch_layer(d, w, col, f, L - 1, F);
-  for (int l = 1; l <= L; l++) {
+  if (npre < L) fetch_layer(d, w, col, f, L - 1 - npre, F);
+  for (int l = npre + 1; l <= L; l++) {
     const int z = L - l;
     const bool sc = stage_layer(F, wl, smu, R2, T2, dg, S2p, S2m, sbad);
     if (l < L) fetch_layer(d, w, col, f, z - 1, F);
