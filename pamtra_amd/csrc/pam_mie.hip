@@ -83,7 +83,9 @@ __device__ __forceinline__ int mie_nstop(double x) {
 
 // Angle table of series length N at tab + off[N]:
 //   mu[Q], wt[Q]          Gauss-Legendre nodes (ascending) and weights
-//   pi[N][Q], tau[N][Q]   pi_n(mu_k), tau_n(mu_k), n = 1..N
+//   pi[N][Q], tau[N][Q]   c_n pi_n(mu_k), c_n tau_n(mu_k), n = 1..N, with
+//                         c_n = (2n+1)/(n(n+1)) in single precision
+//                         (mie_scat_utilities.f90:144)
 //   P[Q][nlegen+1]        P_l(mu_k), l = 0..nlegen
 int64_t angle_table_layout(int64_t *off, int *nmax) {
   int64_t o = 0;
@@ -135,8 +137,9 @@ __global__ void __launch_bounds__(MIE_WG) k_angle_tables(double *tab, const int6
     double pin = 1.0, pim = 0.0;
     for (int k = 1; k <= n; k++) {
       const double taun = (double)k * m * pin - (double)(k + 1) * pim;
-      pit[(size_t)(k - 1) * q + t] = pin;
-      taut[(size_t)(k - 1) * q + t] = taun;
+      const double cn = (double)((float)(2 * k + 1) / (float)(k * (k + 1)));
+      pit[(size_t)(k - 1) * q + t] = cn * pin;
+      taut[(size_t)(k - 1) * q + t] = cn * taun;
       const double tmp = pin;
       pin = ((double)(2 * k + 1) * m * pin - (double)(k + 1) * pim) / (double)k;
       pim = tmp;
@@ -188,9 +191,9 @@ __device__ __forceinline__ cd cdiv_m(cd a, cd b) {
 
 typedef double d4m_t __attribute__((ext_vector_type(4)));
 
-// Angle-table steps of phase B held in registers (k < 4 * MIE_BREG); longer
-// series read the table inside the product loop.
-constexpr int MIE_BREG = 8;
+// LDS stage of the angle table (each of c_n pi_n and c_n tau_n, series
+// length x angles with mu >= 0); longer tables are read from memory.
+constexpr int MIE_TAB = 1488;
 
 // Sum over the 64 lanes of a wave by a fixed butterfly (the same order on
 // every call, so the result is deterministic).
@@ -205,15 +208,30 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
-__global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hydro_desc *hyd,
+// Column of U and series index n (1-based) of position k of a parity group
+// (group 0: a_n for odd n, then b_n for even n; group 1: a_n for even n,
+// then b_n for odd n); `apart` tells an a-column.
+__device__ __forceinline__ void mie_kmap(int g, int k, int np, int &colu, int &n, bool &apart) {
+  const int h = np >> 1;
+  apart = k < h;
+  const int m = apart ? k : k - h;
+  n = apart ? 2 * m + 1 + g : 2 * m + 2 - g;
+  colu = apart ? n - 1 : np + n - 1;
+}
+
+__global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(2))) k_mie(Dims d, Settings s, const pam_hydro_desc *hyd,
                                                 const double *freqs, Work w) {
   __shared__ double s_u[U_CAP];      // U of the chunk; at the end the normalised phase sums
   __shared__ double s_bin[3][MIE_WG];  // per bin of the chunk: conj(m) (re, im) and 1/x
   __shared__ double s_wb[MIE_WG];    // per-bin DSD weight n(D) dD of the chunk, 0 if skipped
   __shared__ double s_ps[3][MAX_NQUAD];  // weighted phase sums per angle
+  __shared__ double s_tab[2 * MIE_TAB];  // angle table stage
+  __shared__ double s_pw[4][6 * 16];     // per-wave phase sums of shared angle tiles
   __shared__ double s_lc[NCOEF][MAXNLEG + 1];
   __shared__ double s_red[4][3];     // per-wave efficiency sums of the chunk
   __shared__ int s_redi[4][2];       // per-wave bin and series counts
+  __shared__ double s_hm[MAX_HYDRO][2];  // per hydrometeor: refractive index
+  __shared__ int s_hnb[MAX_HYDRO], s_hnt[MAX_HYDRO];  // bins, series length (0: none)
   __shared__ short s_nmb[MIE_MAXBIN];  // series length of every bin (0: skipped)
   __shared__ short s_cb[MIE_MAXCHUNK + 1], s_cnp[MIE_MAXCHUNK], s_cnm[MIE_MAXCHUNK];
   __shared__ int s_nc;
@@ -227,33 +245,45 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
   const int col = w.item_col[li_item], z = w.item_layer[li_item];
   int32_t *st = w.itf_status + itf;
   const size_t oy = (size_t)col * d.ML + z;
-  const double T = w.temp[oy];
   const double freq = freqs[f];
   const double wavelength = C_LIGHT / (freq * 1e9);
 
   for (int k = t; k < NCOEF * (MAXNLEG + 1); k += MIE_WG) (&s_lc[0][0])[k] = 0.0;
+  // per hydrometeor, one lane each: bins, refractive index
+  // (scatProperties.f90:227-243) and the series length of the largest bin
+  // (mie_spheres.f90:139-158)
+  if (t < d.nh) {
+    const int nbin = w.dsd_flag[(size_t)li_item * d.nh + t];
+    int nterms = 0;
+    cd m = cd{0.0, 0.0};
+    if (nbin > 0) {
+      const double T = w.temp[oy];
+      if (hyd[t].liq_ice == 1)
+        m = csqrt_(eps_water_tkc(T - sp(273.15), freq));
+      else
+        m = csqrt_(eps_ice(T, freq));
+      const double *dia = w.dsd_d + ((size_t)li_item * d.nh + t) * d.nbmax;
+      nterms = mie_nstop(PI * dia[nbin - 1] / wavelength);
+    }
+    s_hnb[t] = nbin;
+    s_hnt[t] = nterms;
+    s_hm[t][0] = m.re;
+    s_hm[t][1] = m.im;
+  }
+  __syncthreads();
   double kexttot = 0.0, salbedo = 0.0;
   int nleg_coef = 0;
   long long work_bins = 0, work_series = 0;
 
   for (int hh = 0; hh < d.nh; hh++) {
-    const int nbin = w.dsd_flag[(size_t)li_item * d.nh + hh];
+    const int nbin = s_hnb[hh];
     if (nbin <= 0) continue;
-    const pam_hydro_desc &hd = hyd[hh];
     const size_t ob = ((size_t)li_item * d.nh + hh) * d.nbmax;
     const double *dia = w.dsd_d + ob, *dd = w.dsd_dd + ob, *nd = w.dsd_nd + ob,
                  *rho = w.dsd_rho + ob;
-    // refractive index (scatProperties.f90:227-243)
-    cd m;
-    if (hd.liq_ice == 1)
-      m = csqrt_(eps_water_tkc(T - sp(273.15), freq));
-    else
-      m = csqrt_(eps_ice(T, freq));
-    const cd m_ice = cd{m.re, -m.im};
-    const bool is_ice = (hd.liq_ice == -1);
-    // series length from the largest bin (mie_spheres.f90:139-158)
-    const double xmax = PI * dia[nbin - 1] / wavelength;
-    const int nterms = mie_nstop(xmax);
+    const cd m_ice = cd{s_hm[hh][0], -s_hm[hh][1]};
+    const bool is_ice = (hyd[hh].liq_ice == -1);
+    const int nterms = s_hnt[hh];
     if (nterms < 1 || nterms > w.atab_nmax || 16 * (2 * ((nterms + 3) & ~3) + 1) > U_CAP ||
         nbin > MIE_MAXBIN) {
       if (t == 0) set_err(st, E_MIE_TERMS);
@@ -264,7 +294,10 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
     const double *tmu = w.atab + w.atab_off[nterms], *twt = tmu + nquad, *tpi = twt + nquad,
                  *ttau = tpi + (size_t)nterms * nquad, *tpl = ttau + (size_t)nterms * nquad;
     (void)tmu;
-    const int nct = (nquad + 15) >> 4;
+    // the Gauss nodes come in pairs -mu, +mu (the middle node is mu = 0 when
+    // nquad is odd): phase B works on the nh2 angles with mu >= 0
+    const int nh2 = (nquad + 1) >> 1, j0 = nquad - nh2;
+    const int nct = (nh2 + 15) >> 4;
     for (int k = t; k < 3 * MAX_NQUAD; k += MIE_WG) (&s_ps[0][0])[k] = 0.0;
     double sumqe = 0.0, sumqs = 0.0, sumqb = 0.0;
     // series length of every bin (mie_scat_utilities.f90:34)
@@ -277,6 +310,14 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
       }
       s_nmb[e] = (short)v;
     }
+    // c_n pi_n, c_n tau_n at the angles mu >= 0 into LDS when they fit
+    const bool tab_lds = nterms * nh2 <= MIE_TAB;
+    if (tab_lds)
+      for (int e = t; e < nterms * nh2; e += MIE_WG) {
+        const int n = e / nh2, h = e - n * nh2;
+        s_tab[e] = tpi[(size_t)n * nquad + j0 + h];
+        s_tab[MIE_TAB + e] = ttau[(size_t)n * nquad + j0 + h];
+      }
     __syncthreads();
     // chunk plan (wave 0): consecutive bins (ascending size) as long as their
     // rows of U, with the chunk's own series length, fit the LDS stage.  A
@@ -384,11 +425,11 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
         s_wb[t] = 0.0;
       }
       __syncthreads();
-      // --- A2: threads over (bin, n): a_n, b_n (mie_scat_utilities.f90:55-72),
-      // stored as c_n a_n, c_n b_n over the recurrence values.  Items run in
-      // descending n (level-major), and every item reads its inputs before
-      // the barrier that precedes the writes, so the b-column n-1 (psi_n,
-      // also read by item n+1) is overwritten only after its last reader.
+      // --- A2: threads over (bin, n): a_n, b_n (mie_scat_utilities.f90:55-72)
+      // over the recurrence values.  Items run in descending n (level-major),
+      // and every item reads its inputs before the barrier that precedes the
+      // writes, so the b-column n-1 (psi_n, also read by item n+1) is
+      // overwritten only after its last reader.
       for (int k0 = 0; k0 < nb * nlev; k0 += MIE_WG) {
         const int k = k0 + t;
         const int bl = k % nb, n = nlev - k / nb;
@@ -413,17 +454,15 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
         }
         __syncthreads();
         if (ok) {
-          // c_n in single precision (mie_scat_utilities.f90:144)
-          const double cn = (double)((float)(2 * n + 1) / (float)(n * (n + 1)));
-          Ure[n - 1] = cn * an.re;
-          Uim[n - 1] = cn * an.im;
-          Ure[np + n - 1] = cn * bn.re;
-          Uim[np + n - 1] = cn * bn.im;
+          Ure[n - 1] = an.re;
+          Uim[n - 1] = an.im;
+          Ure[np + n - 1] = bn.re;
+          Uim[np + n - 1] = bn.im;
         }
       }
       __syncthreads();
       // --- efficiencies (miecross, mie_scat_utilities.f90:77-116) of bin t,
-      // summed over n in ascending order from the stored c_n a_n, c_n b_n
+      // its sums over n in ascending order
       {
         double q0 = 0.0, q1 = 0.0, q2 = 0.0;
         int nbw = 0, nsw = 0;
@@ -433,14 +472,13 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
           double s1 = 0.0, s2 = 0.0;
           cd s3 = cd{0.0, 0.0};
           for (int n = 1; n <= nmie; n++) {
-            const double cn = (double)((float)(2 * n + 1) / (float)(n * (n + 1)));
-            const double wn = (double)(2 * n + 1), r = wn / cn;
+            const double wn = (double)(2 * n + 1);
             const double ar = Ure[n - 1], ai = Uim[n - 1], br = Ure[np + n - 1],
                          bi = Uim[np + n - 1];
-            s1 = s1 + r * (ar + br);
-            s2 = s2 + (r / cn) * ((ar * ar + ai * ai) + (br * br + bi * bi));
-            const double rs = (n & 1) ? -r : r;
-            s3 = s3 + rs * cd{ar - br, ai - bi};
+            s1 = s1 + wn * (ar + br);
+            s2 = s2 + wn * ((ar * ar + ai * ai) + (br * br + bi * bi));
+            const double ws = (n & 1) ? -wn : wn;
+            s3 = s3 + ws * cd{ar - br, ai - bi};
           }
           const double x = PI * dia[ir] / wavelength;
           const double rr = dia[ir] / 2.0;
@@ -468,73 +506,75 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
       }
       // --- B: S1, S2 of every (bin, angle) on the matrix cores and the
       // DSD-weighted phase-function sums (mieangle, mie_spheres.f90:238-247).
-      // Wave wv owns the angle tiles ct = wv, wv + 4, ...
-      for (int ct = wv; ct < nct; ct += 4) {
-        const int ang = 16 * ct + li;
-        const bool ang_ok = ang < nquad;
-        const int nsk = kp >> 2;
-        double p1 = 0.0, p2 = 0.0, p3 = 0.0;
-        if (nsk <= MIE_BREG) {
-          // the tile's pi_n, tau_n for every k, loaded once for all row tiles
-          double bp[MIE_BREG], bt[MIE_BREG];
+      // With pi_n(-mu) = (-1)^(n-1) pi_n(mu) and tau_n(-mu) = (-1)^n tau_n(mu)
+      // the series splits by the parity of n: over the a_n of odd n and the
+      // b_n of even n (group 0) against [pi; tau] gives X0 and against
+      // [tau; pi] gives Y0, group 1 likewise X1, Y1; then
+      //   S1(+mu) = X0 + X1, S1(-mu) = X0 - X1,
+      //   S2(+mu) = Y1 + Y0, S2(-mu) = Y1 - Y0,
+      // half the products of evaluating both angles.  With four or more
+      // tiles of angles (mu >= 0) wave wv owns the tiles ct = wv, wv + 4, ...
+      // with every row tile; with fewer, wpc = 4 / nct waves share a tile,
+      // each over every wpc-th row tile, and their sums are added in wave
+      // order after the barrier.
+      const int wpc = nct >= 4 ? 1 : 4 / nct;
+      for (int ct = (wpc > 1 ? wv / wpc : wv); ct < nct; ct += (wpc > 1 ? nct : 4)) {
+        const int sub = wpc > 1 ? wv % wpc : 0;
+        const int hix = 16 * ct + li;
+        const bool ang_ok = hix < nh2;
+        const int ang = j0 + hix, angm = nquad - 1 - ang;
+        const int nsk = np >> 2;  // steps per group
+        double p1 = 0.0, p2 = 0.0, p3 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+        for (int rt = sub; rt < nrt; rt += wpc) {
+          d4m_t x0 = {0.0, 0.0, 0.0, 0.0}, y0 = x0, x1 = x0, y1 = x0;
+          const double *urow = s_u + (16 * rt + li) * ldu;
+          for (int sk = 0; sk < nsk; sk++) {
 #pragma unroll
-          for (int sk = 0; sk < MIE_BREG; sk++) {
-            const int kk = 4 * sk + lk;
-            const int n = kk < np ? kk : kk - np;
-            const bool v = sk < nsk && ang_ok && n < nterms;
-            bp[sk] = v ? tpi[(size_t)n * nquad + ang] : 0.0;
-            bt[sk] = v ? ttau[(size_t)n * nquad + ang] : 0.0;
-          }
-          for (int rt = 0; rt < nrt; rt++) {
-            d4m_t acc1 = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
-            const double *urow = s_u + (16 * rt + li) * ldu;
-#pragma unroll
-            for (int sk = 0; sk < MIE_BREG; sk++) {
-              if (sk < nsk) {
-                const bool ablk = 4 * sk < np;
-                const double a = urow[4 * sk + lk];
-                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? bp[sk] : bt[sk], acc1, 0, 0, 0);
-                acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? bt[sk] : bp[sk], acc2, 0, 0, 0);
-              }
-            }
-#pragma unroll
-            for (int h2 = 0; h2 < 2; h2++) {
-              const double wgt = s_wb[8 * rt + 4 * h2 + lk];
-              const double s1r = acc1[2 * h2], s1i = acc1[2 * h2 + 1];
-              const double s2r = acc2[2 * h2], s2i = acc2[2 * h2 + 1];
-              const double a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
-              p1 += (0.5 * (a2 + a1)) * wgt;
-              p2 += (0.5 * (a2 - a1)) * wgt;
-              p3 += (s2r * s1r + s2i * s1i) * wgt;
-            }
-          }
-        } else {
-          for (int rt = 0; rt < nrt; rt++) {
-            d4m_t acc1 = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
-            const double *urow = s_u + (16 * rt + li) * ldu;
-            for (int sk = 0; sk < nsk; sk++) {
-              const int kk = 4 * sk + lk;
-              const bool ablk = kk < np;
-              const int n = ablk ? kk : kk - np;
+            for (int g = 0; g < 2; g++) {
+              int cu, n;
+              bool ap;
+              mie_kmap(g, 4 * sk + lk, np, cu, n, ap);
               double pv = 0.0, tv = 0.0;
-              if (ang_ok && n < nterms) {
-                pv = tpi[(size_t)n * nquad + ang];
-                tv = ttau[(size_t)n * nquad + ang];
+              if (ang_ok && n <= nterms) {
+                if (tab_lds) {
+                  pv = s_tab[(n - 1) * nh2 + hix];
+                  tv = s_tab[MIE_TAB + (n - 1) * nh2 + hix];
+                } else {
+                  pv = tpi[(size_t)(n - 1) * nquad + ang];
+                  tv = ttau[(size_t)(n - 1) * nquad + ang];
+                }
               }
-              const double a = urow[kk];
-              acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? pv : tv, acc1, 0, 0, 0);
-              acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ablk ? tv : pv, acc2, 0, 0, 0);
+              const double a = urow[cu];
+              if (g == 0) {
+                x0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? pv : tv, x0, 0, 0, 0);
+                y0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? tv : pv, y0, 0, 0, 0);
+              } else {
+                x1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? pv : tv, x1, 0, 0, 0);
+                y1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? tv : pv, y1, 0, 0, 0);
+              }
             }
+          }
 #pragma unroll
-            for (int h2 = 0; h2 < 2; h2++) {
-              const double wgt = s_wb[8 * rt + 4 * h2 + lk];
-              const double s1r = acc1[2 * h2], s1i = acc1[2 * h2 + 1];
-              const double s2r = acc2[2 * h2], s2i = acc2[2 * h2 + 1];
-              const double a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
-              p1 += (0.5 * (a2 + a1)) * wgt;
-              p2 += (0.5 * (a2 - a1)) * wgt;
-              p3 += (s2r * s1r + s2i * s1i) * wgt;
-            }
+          for (int h2 = 0; h2 < 2; h2++) {
+            const double wgt = s_wb[8 * rt + 4 * h2 + lk];
+            const int re = 2 * h2, im = 2 * h2 + 1;
+            // +mu
+            double s1r = x0[re] + x1[re], s1i = x0[im] + x1[im];
+            double s2r = y1[re] + y0[re], s2i = y1[im] + y0[im];
+            double a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
+            p1 += (0.5 * (a2 + a1)) * wgt;
+            p2 += (0.5 * (a2 - a1)) * wgt;
+            p3 += (s2r * s1r + s2i * s1i) * wgt;
+            // -mu
+            s1r = x0[re] - x1[re];
+            s1i = x0[im] - x1[im];
+            s2r = y1[re] - y0[re];
+            s2i = y1[im] - y0[im];
+            a1 = s1r * s1r + s1i * s1i;
+            a2 = s2r * s2r + s2i * s2i;
+            m1 += (0.5 * (a2 + a1)) * wgt;
+            m2 += (0.5 * (a2 - a1)) * wgt;
+            m3 += (s2r * s1r + s2i * s1i) * wgt;
           }
         }
         // the four lane groups hold different bins of the same angle
@@ -544,13 +584,54 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
         p2 += __shfl_xor(p2, 32, 64);
         p3 += __shfl_xor(p3, 16, 64);
         p3 += __shfl_xor(p3, 32, 64);
-        if (lk == 0 && ang_ok) {  // this wave alone owns the angle
-          s_ps[0][ang] += p1;
-          s_ps[1][ang] += p2;
-          s_ps[2][ang] += p3;
+        m1 += __shfl_xor(m1, 16, 64);
+        m1 += __shfl_xor(m1, 32, 64);
+        m2 += __shfl_xor(m2, 16, 64);
+        m2 += __shfl_xor(m2, 32, 64);
+        m3 += __shfl_xor(m3, 16, 64);
+        m3 += __shfl_xor(m3, 32, 64);
+        if (lk == 0 && ang_ok) {
+          if (wpc == 1) {  // this wave alone owns the pair
+            s_ps[0][ang] += p1;
+            s_ps[1][ang] += p2;
+            s_ps[2][ang] += p3;
+            if (angm != ang) {
+              s_ps[0][angm] += m1;
+              s_ps[1][angm] += m2;
+              s_ps[2][angm] += m3;
+            }
+          } else {
+            double *pw = s_pw[wv];
+            pw[li] = p1;
+            pw[16 + li] = p2;
+            pw[32 + li] = p3;
+            pw[48 + li] = m1;
+            pw[64 + li] = m2;
+            pw[80 + li] = m3;
+          }
         }
       }
       __syncthreads();
+      if (wpc > 1 && t < 16 * nct) {  // the shared tiles, waves in order
+        const int ct = t >> 4, l16 = t & 15, hix = t;
+        if (hix < nh2) {
+          const int ang = j0 + hix, angm = nquad - 1 - ang;
+          double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+          for (int q = 0; q < wpc; q++) {
+            const double *pw = s_pw[ct * wpc + q];
+#pragma unroll
+            for (int k = 0; k < 6; k++) v[k] += pw[16 * k + l16];
+          }
+          s_ps[0][ang] += v[0];
+          s_ps[1][ang] += v[1];
+          s_ps[2][ang] += v[2];
+          if (angm != ang) {
+            s_ps[0][angm] += v[3];
+            s_ps[1][angm] += v[4];
+            s_ps[2][angm] += v[5];
+          }
+        }
+      }
       if (t == 0) {  // the chunk's efficiency sums, waves in order
         for (int q = 0; q < 4; q++) {
           sumqe = sumqe + s_red[q][0];
@@ -590,16 +671,25 @@ __global__ void __launch_bounds__(MIE_WG) k_mie(Dims d, Settings s, const pam_hy
       s_q[2 * MAX_NQUAD + t] = tmp * s_ps[2][t] * wt;
     }
     __syncthreads();
-    // --- C: Legendre projection (mie_spheres.f90:272-314), thread per order
+    // --- C: Legendre projection (mie_spheres.f90:272-314), thread per
+    // order, over the pairs of angles: P_l(-mu) = (-1)^l P_l(mu)
     double g1 = 0.0, g2 = 0.0, g3 = 0.0;
     if (t <= nlegen) {
       double c1 = 0.0, c2 = 0.0, c3 = 0.0;
-#pragma unroll 8
-      for (int k = 0; k < nquad; k++) {
+      const double sg = (t & 1) ? -1.0 : 1.0;
+#pragma unroll 4
+      for (int h = 0; h < nh2; h++) {
+        const int k = j0 + h, km = nquad - 1 - k;
         const double pl = tpl[(size_t)k * (nlegen + 1) + t];
-        c1 += s_q[k] * pl;
-        c2 += s_q[MAX_NQUAD + k] * pl;
-        c3 += s_q[2 * MAX_NQUAD + k] * pl;
+        double q1 = s_q[k], q2 = s_q[MAX_NQUAD + k], q3 = s_q[2 * MAX_NQUAD + k];
+        if (km != k) {
+          q1 += sg * s_q[km];
+          q2 += sg * s_q[MAX_NQUAD + km];
+          q3 += sg * s_q[2 * MAX_NQUAD + km];
+        }
+        c1 += q1 * pl;
+        c2 += q2 * pl;
+        c3 += q3 * pl;
       }
       const double hl = (double)((float)(2 * t + 1) / 2.0f);
       g1 = hl * c1;
