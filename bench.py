@@ -81,7 +81,7 @@ def measured_traffic(kernel):
     return None
 
 
-def serve(args, precision="fp64"):
+def serve(args, precision="fp64", tuning=None):
     """The serving mix at an offered Poisson rate: throughput and p50/p99
     latency (arrival to result on the host, input staging included)."""
     import numpy as np
@@ -95,7 +95,13 @@ def serve(args, precision="fp64"):
     reqs = [serving.Request(pam=p, freqs=f, arrival=float(a), rid=i)
             for i, ((p, f), a) in enumerate(zip(mix, arrivals))]
     log(f"serving mix built in {time.time() - t0:.1f} s: {len(reqs)} requests")
-    with serving.ForwardServer(slots=args.serve_slots, hbm_budget_gb=64.0) as srv:
+    shapes = None
+    if tuning:
+        # persistent grids only: a graph is not replayed across requests of
+        # different shapes
+        shapes = {k: v for k, v in tuning.items() if not k.startswith("_") and k != "graph"}
+    with serving.ForwardServer(slots=args.serve_slots, hbm_budget_gb=64.0,
+                               tuning=shapes) as srv:
         srv.run(reqs[:2], realtime=False)  # warm the slots' workspace and code
         t0 = time.perf_counter()
         res = srv.run(reqs, realtime=True)
@@ -104,7 +110,7 @@ def serve(args, precision="fp64"):
     summ = serving.summarize(res, reqs, wall)
     summ.update({"offered_req_s": args.serve_rate, "slots": args.serve_slots,
                  "hbm_budget_gib": 64.0, "peak_workspace_gib": round(peak / 2 ** 30, 3),
-                 "precision": precision,
+                 "precision": precision, "launch_tuning": shapes,
                  "mix": "1-4096 columns (log-uniform), 20-150 layers, 1-12 channels of "
                         "10.65-325 GHz, 0-4 of cloud/ice/snow/graupel with 100 bins"})
     log(f"serving: {summ}")
@@ -129,6 +135,10 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving mix and print its summary")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes (no autotuning)")
+    ap.add_argument("--tune-cols", type=int, default=4096,
+                    help="columns of the workload slice the autotuner measures on")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (to scale HBM counters)")
     args = ap.parse_args()
@@ -171,23 +181,65 @@ def main():
             raise RuntimeError("pam_run_device: " + msg.value.decode())
         return st
 
+    def slice_runner(precision):
+        """One synchronous call on the first --tune-cols columns (the
+        autotuner's sample of the workload)."""
+        ns = min(args.tune_cols, nx)
+        sub = type("Sub", (), {})()
+        sub.p = slice_columns(pam, ns)
+        ps = to_device(sub, torch)
+        sets_s = dict(pam.set)
+        sets_s["precision"] = precision
+        ms_ = _capi.marshal(nml, sets_s, pam.df.data, pam.df.data4D, ps)
+        tb_s = torch.zeros((ns, ny, nout, 32, nf, 2), dtype=torch.float64, device="cuda")
+        st_s = torch.zeros((ns, ny, nf), dtype=torch.int32, device="cuda")
+        rs = _capi.PamResults()
+        rs.tb = C.cast(C.c_void_p(tb_s.data_ptr()), C.POINTER(C.c_double))
+        rs.status = C.cast(C.c_void_p(st_s.data_ptr()), C.POINTER(C.c_int32))
+        keep = (ps, ms_, tb_s, st_s)
+
+        def run_once():
+            _ = keep
+            if lib.pam_run_device(ctx, C.byref(ms_.settings), ms_.hydros, C.byref(ms_.hydro4d),
+                                  C.byref(ms_.profile), fp, nf, C.byref(rs), None) == _capi.PAM_FATAL:
+                raise RuntimeError("pam_run_device (tuning sample) failed")
+            torch.cuda.synchronize()
+        return run_once
+
+    def timed(fn, steps):
+        names_ = lib.pam_stage_names().decode().split(",")
+        acc = np.zeros(len(names_))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            fn()
+            ms = (C.c_double * len(names_))()
+            lib.pam_last_stage_times(ctx, ms, len(names_))
+            acc += np.array(ms[:])
+            log(f"step {i} done")
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, acc / steps
+
     for i in range(args.warmup):
         step()
         log(f"warmup {i} done")
     torch.cuda.synchronize()
     names = lib.pam_stage_names().decode().split(",")
     nst = len(names)
-    stage_ms = np.zeros(nst)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-        ms = (C.c_double * nst)()
-        lib.pam_last_stage_times(ctx, ms, nst)
-        stage_ms += np.array(ms[:])
-        log(f"step {i} done")
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    stage_ms /= args.steps
+    tuning = None
+    if not args.no_tune:
+        from pamtra_amd import autotune
+        dt0, stage0 = timed(step, args.steps)
+        log(f"default launch shapes: {dt0 * 1e3:.1f} ms/step")
+        t0 = time.time()
+        ch = autotune.tune(lib, ctx, slice_runner("fp64"))
+        log(f"autotuned in {time.time() - t0:.1f} s: {ch}")
+        tuning = {"fp64": ch, "default_shapes_value": round(nx * ny * nf / dt0, 2),
+                  "default_shapes_stage_ms": {n: round(v, 3) for n, v in zip(names, stage0)},
+                  "sample_columns": min(args.tune_cols, nx)}
+        for i in range(args.warmup):
+            step()
+    dt, stage_ms = timed(step, args.steps)
     work = (C.c_int64 * 12)()
     lib.pam_last_work(ctx, work, 12)
     work = list(work)
@@ -237,6 +289,11 @@ def main():
             if st == _capi.PAM_FATAL:
                 raise RuntimeError("pam_run_device (mixed): " + msg.value.decode())
 
+        if not args.no_tune:
+            from pamtra_amd import autotune
+            chm = autotune.tune(lib, ctx, slice_runner("mixed"))
+            log(f"autotuned (mixed): {chm}")
+            tuning["mixed"] = chm
         for _ in range(args.warmup):
             step_m()
         torch.cuda.synchronize()
@@ -295,13 +352,14 @@ def main():
         "status_nonzero": int((st_h != 0).sum()),
         "pcie_inclusive_value": None if host_rate is None else round(host_rate, 2),
         "mixed_precision": mixed,
+        "launch_tuning": tuning,
         "roofline": roof,
     }
 
     if args.serve_requests > 0:
-        out["serving"] = {"fp64": serve(args)}
+        out["serving"] = {"fp64": serve(args, tuning=tuning and tuning.get("fp64"))}
         if not args.no_mixed:
-            out["serving"]["mixed"] = serve(args, "mixed")
+            out["serving"]["mixed"] = serve(args, "mixed", tuning=tuning and tuning.get("mixed"))
 
     if not args.no_cpu:
         import oracle

@@ -234,6 +234,21 @@ int64_t pam_context_bytes(const pam_context *ctx);
  * call, libWrapper.py:48). */
 int pam_context_trim(pam_context *ctx);
 
+/* Launch shapes of the context, chosen by pamtra_amd.autotune or set by the
+ * caller; results are bitwise the same for every value.  Keys
+ * (pam_tuning_keys, comma-separated):
+ *   mie_persistent, doubling_persistent, adding_persistent: 0 launches one
+ *     workgroup per unit; k in 1..64 launches k workgroups per compute unit
+ *     that take units from a work queue;
+ *   graph: 1 captures the kernels of a single-chunk call in a HIP graph and
+ *     replays it while the launch arguments are unchanged.
+ * pam_context_set_tuning returns PAM_FATAL for an unknown key or value;
+ * pam_context_get_tuning returns -1 for an unknown key.  No counterpart in
+ * the reference (its Fortran loops have no launch shape). */
+const char *pam_tuning_keys(void);
+int pam_context_set_tuning(pam_context *ctx, const char *key, int64_t value);
+int64_t pam_context_get_tuning(const pam_context *ctx, const char *key);
+
 #ifdef __cplusplus
 }
 #endif

@@ -398,6 +398,12 @@ def declare_library(lib: C.CDLL) -> C.CDLL:
     lib.pam_context_bytes.argtypes = [C.c_void_p]
     lib.pam_context_trim.restype = C.c_int
     lib.pam_context_trim.argtypes = [C.c_void_p]
+    lib.pam_tuning_keys.restype = C.c_char_p
+    lib.pam_tuning_keys.argtypes = []
+    lib.pam_context_set_tuning.restype = C.c_int
+    lib.pam_context_set_tuning.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
+    lib.pam_context_get_tuning.restype = C.c_int64
+    lib.pam_context_get_tuning.argtypes = [C.c_void_p, C.c_char_p]
     return lib
 
 
@@ -405,7 +411,8 @@ EXPORTED_SYMBOLS = ["pam_version", "pam_device_count", "pam_context_create",
                     "pam_context_destroy", "pam_run", "pam_run_device",
                     "pam_last_stage_times", "pam_stage_names", "pam_last_work",
                     "pam_workspace_bytes", "pam_context_set_workspace_limit", "pam_context_bytes",
-                    "pam_context_trim"]
+                    "pam_context_trim", "pam_tuning_keys", "pam_context_set_tuning",
+                    "pam_context_get_tuning"]
 
 PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIBRARY_PATH = os.path.join(PACKAGE_DIR, "libpamtra_amd.so")

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+
 #include "pam_internal.h"
 
 using namespace pam;
@@ -33,6 +34,7 @@ using namespace pam;
 constexpr int PAM_NWORK = 12;
 static const char *k_stage_names = "columns,dsd,mie,phase,doubling,adding";
 constexpr int NSTAGE = 6;
+constexpr int PAM_NTUNE = 4;  // entries of k_tuning_keys
 
 // A device allocation that only grows.
 struct DevBuf {
@@ -137,7 +139,38 @@ struct pam_context {
   double stage_ms[NSTAGE] = {0, 0, 0, 0, 0, 0};
   int64_t work[PAM_NWORK] = {};
   size_t budget_bytes = 0;
+  int ncu = 0;                     // compute units of the device
+  int64_t tune[PAM_NTUNE] = {};    // launch shapes, see pam_context_set_tuning
+  // graph of the last single-chunk call and the launch arguments it was
+  // captured with (replayed while they are unchanged)
+  hipGraphExec_t gexec[5] = {};   // one graph per timed stage of the chunk
+  std::vector<unsigned char> gkey;
 };
+
+static const char *k_tuning_keys = "mie_persistent,doubling_persistent,adding_persistent,graph";
+
+static int tuning_index(const char *key) {
+  if (!key) return -1;
+  int i = 0;
+  const char *p = k_tuning_keys;
+  while (*p) {
+    const char *q = strchr(p, ',');
+    const size_t len = q ? (size_t)(q - p) : strlen(p);
+    if (strlen(key) == len && strncmp(key, p, len) == 0) return i;
+    if (!q) break;
+    p = q + 1;
+    i++;
+  }
+  return -1;
+}
+
+static void drop_graph(pam_context *c) {
+  for (auto &g : c->gexec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  c->gkey.clear();
+}
 
 static void set_msg(pam_results *res, const std::string &m) {
   if (res && res->msg && res->msglen > 0) {
@@ -296,6 +329,8 @@ extern "C" int pam_context_create(pam_context **out, int device) {
   if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && freeb > 0)
     gb = std::min(96.0, 0.5 * (double)freeb / (double)(1ull << 30));
   c->budget_bytes = (size_t)(gb * (1ull << 30));
+  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    c->ncu = 0;
   *out = c;
   return PAM_SUCCESS;
 }
@@ -307,6 +342,7 @@ extern "C" int pam_context_destroy(pam_context *c) {
   c->itembuf.release();
   c->iobuf.release();
   c->smallbuf.release();
+  drop_graph(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->d_mu) (void)hipFree(c->d_mu);
   delete c;
@@ -327,6 +363,23 @@ extern "C" int pam_last_work(const pam_context *c, int64_t *counts, int32_t n) {
   return PAM_SUCCESS;
 }
 
+extern "C" const char *pam_tuning_keys(void) { return k_tuning_keys; }
+
+extern "C" int pam_context_set_tuning(pam_context *c, const char *key, int64_t value) {
+  const int i = tuning_index(key);
+  if (!c || i < 0 || value < 0 || value > 64) return PAM_FATAL;
+  if (i == 3 && value > 1) return PAM_FATAL;
+  c->tune[i] = value;
+  if (i == 3 && !value) drop_graph(c);
+  return PAM_SUCCESS;
+}
+
+extern "C" int64_t pam_context_get_tuning(const pam_context *c, const char *key) {
+  const int i = tuning_index(key);
+  if (!c || i < 0) return -1;
+  return c->tune[i];
+}
+
 extern "C" int pam_context_set_workspace_limit(pam_context *c, int64_t bytes) {
   if (!c || bytes <= 0) return PAM_FATAL;
   c->budget_bytes = (size_t)bytes;
@@ -342,6 +395,7 @@ extern "C" int pam_context_trim(pam_context *c) {
   if (!c) return PAM_FATAL;
   if (hipSetDevice(c->device) != hipSuccess) return PAM_FATAL;
   if (c->stream && hipStreamSynchronize(c->stream) != hipSuccess) return PAM_FATAL;
+  drop_graph(c);
   c->colbuf.release();
   c->itembuf.release();
   c->iobuf.release();
@@ -469,10 +523,22 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
       return PAM_FATAL;
     }
   HIP_OK(hipSetDevice(c->device));
+  if (!stream && c->tune[3]) {
+    // the legacy stream cannot be captured: the call runs on the context's
+    // stream, after everything queued before it on the legacy stream (the
+    // call synchronises before it returns)
+    hipEvent_t e;
+    HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(e, nullptr));
+    HIP_OK(hipStreamWaitEvent(c->stream, e, 0));
+    (void)hipEventDestroy(e);
+    stream = c->stream;
+  }
   for (int i = 0; i < NSTAGE; i++) c->stage_ms[i] = 0.0;
   for (int i = 0; i < PAM_NWORK; i++) c->work[i] = 0;
 
   Dims d;
+  memset(&d, 0, sizeof(d));
   d.nx = pin->ngridx;
   d.ny = pin->ngridy;
   d.ncol = d.nx * d.ny;
@@ -496,6 +562,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   d.item0 = 0;
 
   Settings s;
+  memset(&s, 0, sizeof(s));
   s.lgas = st->lgas_extinction;
   s.lhyd_abs = st->lhyd_absorption;
   s.lhyd_scat = st->lhyd_scattering;
@@ -513,6 +580,10 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     const char *inv = getenv("PAM_INVERSE");
     s.inverse_gj = !inv ? 0 : strcmp(inv, "gauss-jordan") == 0 ? 1 : strcmp(inv, "queue-all") == 0 ? 2 : 0;
   }
+  s.pers_mie = (int)c->tune[0];
+  s.pers_dbl = (int)c->tune[1];
+  s.pers_add = (int)c->tune[2];
+  s.ncu = c->ncu;
 
   Profile p;
   p.nlyrs = pin->nlyrs;
@@ -563,6 +634,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   double *d_freqs = sc.take<double>(nfreq);
   unsigned long long *d_cnt = sc.take<unsigned long long>(PAM_NWORK);
   uint32_t *d_redo_n = sc.take<uint32_t>(2);
+  uint32_t *d_queue = sc.take<uint32_t>(4);
   uint32_t *d_ph = sc.take<uint32_t>(3 * (PH_NBUCKET + 1));
   if (d.nh > 0)
     HIP_OK(hipMemcpyAsync(d_hyd, hydros, sizeof(pam_hydro_desc) * d.nh, hipMemcpyHostToDevice,
@@ -612,6 +684,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   w.ph_toff = d_ph + 2 * (PH_NBUCKET + 1);
   w.counters = d_cnt;
   w.redo_n = d_redo_n;
+  w.queue = d_queue;
 
   std::vector<hipEvent_t> evs;
   std::vector<int> ev_stage;
@@ -683,8 +756,45 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   c->work[1] = d.nf;
   c->work[2] = total * d.nf;
 
+  // The kernels of one chunk in five segments, one per timed stage (dsd,
+  // mie, phase, doubling, adding).  Counters and status words are cleared by
+  // a fill kernel (not a memset), so that each segment can be captured in a
+  // graph.
+  auto enqueue_segment = [&](int seg, const Dims &dc, Work &wc) -> hipError_t {
+    switch (seg) {
+      case 1:
+        launch_fill_u32(wc.queue, 4, 0, stream);
+        launch_fill_u32(wc.redo_n, 2, 0, stream);
+        if (dc.nitems > 0)
+          launch_fill_u32((uint32_t *)wc.itf_status, (size_t)dc.nitems * d.nf, 0, stream);
+        launch_items(dc, wc, stream);
+        launch_dsd(dc, s, p, d_hyd, h4, wc, stream);
+        break;
+      case 2:
+        launch_mie(dc, s, d_hyd, h4, p, d_freqs, wc, stream);
+        break;
+      case 3:
+        launch_phase(dc, s, wc, stream);
+        break;
+      case 4:
+        launch_doubling(dc, s, d_freqs, wc, stream);
+        break;
+      default:
+        launch_adding(dc, s, d_freqs, pin->sfc_refl, wc, res->tb, stream);
+        if (res->kextatmo || res->kexttot || res->salbedo || res->nlegen || res->ndoubles) {
+          size_t nd = (size_t)dc.ncol_chunk * d.ML * d.nf;
+          hipLaunchKernelGGL(k_diag, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, stream, dc,
+                             wc, res->kextatmo, res->kexttot, res->salbedo, res->nlegen,
+                             res->ndoubles);
+        }
+        break;
+    }
+    return hipGetLastError();
+  };
+  const bool use_graph = c->tune[3] && chunks.size() == 1;
   for (auto &ch : chunks) {
-    Dims dc = d;
+    Dims dc;
+    memcpy(&dc, &d, sizeof(d));
     dc.col0 = ch.first;
     dc.ncol_chunk = ch.second - ch.first;
     dc.item0 = off[ch.first];
@@ -692,30 +802,52 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     for (int i = ch.first; i < ch.second; i++) n += cnt[i];
     dc.nitems = (int)n;
     carve_items(w, c->itembuf.p, dc, std::max<size_t>(1, (size_t)n));
-    if (n > 0)
-      HIP_OK(hipMemsetAsync(w.itf_status, 0, 4 * (size_t)n * d.nf, stream));
-    launch_items(dc, w, stream);
-    launch_dsd(dc, s, p, d_hyd, h4, w, stream);
-    HIP_OK(hipGetLastError());
-    HIP_OK(mark(1));
-    launch_mie(dc, s, d_hyd, h4, p, d_freqs, w, stream);
-    HIP_OK(hipGetLastError());
-    HIP_OK(mark(2));
-    launch_phase(dc, s, w, stream);
-    HIP_OK(hipGetLastError());
-    HIP_OK(mark(3));
-    HIP_OK(hipMemsetAsync(w.redo_n, 0, 2 * sizeof(uint32_t), stream));
-    launch_doubling(dc, s, d_freqs, w, stream);
-    HIP_OK(hipGetLastError());
-    HIP_OK(mark(4));
-    launch_adding(dc, s, d_freqs, pin->sfc_refl, w, res->tb, stream);
-    HIP_OK(hipGetLastError());
-    if (res->kextatmo || res->kexttot || res->salbedo || res->nlegen || res->ndoubles) {
-      size_t nd = (size_t)dc.ncol_chunk * d.ML * d.nf;
-      hipLaunchKernelGGL(k_diag, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, stream, dc, w,
-                         res->kextatmo, res->kexttot, res->salbedo, res->nlegen, res->ndoubles);
+    if (use_graph) {
+      // the launch arguments the graphs bake in: shapes, settings,
+      // workspace and the caller's device pointers
+      std::vector<unsigned char> key(sizeof(Dims) + sizeof(Settings) + sizeof(Work) +
+                                     sizeof(Profile) + sizeof(Hydro4D) + 9 * sizeof(void *));
+      unsigned char *kp = key.data();
+      const void *ptrs[9] = {d_hyd, d_freqs, pin->sfc_refl, res->tb, res->kextatmo,
+                             res->kexttot, res->salbedo, res->nlegen, res->ndoubles};
+      memcpy(kp, &dc, sizeof(Dims));
+      kp += sizeof(Dims);
+      memcpy(kp, &s, sizeof(Settings));
+      kp += sizeof(Settings);
+      memcpy(kp, &w, sizeof(Work));
+      kp += sizeof(Work);
+      memcpy(kp, &p, sizeof(Profile));
+      kp += sizeof(Profile);
+      memcpy(kp, &h4, sizeof(Hydro4D));
+      kp += sizeof(Hydro4D);
+      memcpy(kp, ptrs, sizeof(ptrs));
+      if (!(c->gexec[0] && c->gkey == key)) {
+        drop_graph(c);
+        bool ok = true;
+        for (int seg = 1; seg <= 5 && ok; seg++) {
+          hipGraph_t g = nullptr;
+          ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+          if (!ok) break;
+          const hipError_t e1 = enqueue_segment(seg, dc, w);
+          const hipError_t e2 = hipStreamEndCapture(stream, &g);
+          ok = e1 == hipSuccess && e2 == hipSuccess && g;
+          if (ok) ok = hipGraphInstantiate(&c->gexec[seg - 1], g, nullptr, nullptr, 0) == hipSuccess;
+          if (g) (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+        if (ok)
+          c->gkey = key;
+        else
+          drop_graph(c);  // not capturable here: direct launches
+      }
     }
-    HIP_OK(mark(5));
+    for (int seg = 1; seg <= 5; seg++) {
+      if (use_graph && c->gexec[0])
+        HIP_OK(hipGraphLaunch(c->gexec[seg - 1], stream));
+      else
+        HIP_OK(enqueue_segment(seg, dc, w));
+      HIP_OK(mark(seg));
+    }
   }
   hipLaunchKernelGGL(k_status, dim3((unsigned)((d.ncol * d.nf + 255) / 256)), dim3(256), 0,
                      stream, d, w, res->status);

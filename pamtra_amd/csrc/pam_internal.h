@@ -70,6 +70,10 @@ struct Settings {
   // converge; 1 Gauss-Jordan only (PAM_INVERSE=gauss-jordan); 2 every unit
   // through the Gauss-Jordan queue (PAM_INVERSE=queue-all, a test mode)
   int inverse_gj;
+  // launch shape: workgroups per compute unit of a persistent grid that
+  // takes units from a work queue (0: one workgroup per unit)
+  int pers_mie, pers_dbl, pers_add;
+  int ncu;  // compute units of the device
   double threshold, softsphere_min, emissivity;
 };
 
@@ -137,6 +141,9 @@ struct Work {
   int32_t *redo_add;                   // [ncol*nf]
   // work counters (atomics)
   unsigned long long *counters;        // [12], see pam_last_work
+  // work-queue counters of the persistent grids: mie, doubling, adding
+  // (cleared per chunk)
+  uint32_t *queue;                     // [4]
 };
 
 // ------------------------------------------------------------------
@@ -200,6 +207,25 @@ __device__ __forceinline__ void set_err(int32_t *st, int32_t code) {
   atomicMax(st, code | PAM_FATAL);
 }
 
+// Next unit of a persistent workgroup: thread 0 takes an index from the
+// queue counter and the workgroup reads it from LDS (the first barrier ends
+// the previous unit's use of LDS).  Every workgroup leaves its loop once the
+// counter has passed the unit count.
+__device__ __forceinline__ int next_unit(uint32_t *ctr, int *slot) {
+  __syncthreads();
+  if (threadIdx.x == 0) *slot = (int)atomicAdd(ctr, 1u);
+  __syncthreads();
+  return *slot;
+}
+
+// Grid of a launch over n units: one workgroup per unit, or per_cu
+// workgroups per compute unit (at most n) for a persistent grid.
+inline size_t persistent_grid(size_t n, int per_cu, int ncu) {
+  if (per_cu <= 0 || ncu <= 0) return n;
+  const size_t g = (size_t)per_cu * (size_t)ncu;
+  return g < n ? g : n;
+}
+
 }  // namespace pam
 
 // ------------------------------------------------------------------
@@ -231,4 +257,7 @@ void launch_doubling(const Dims &d, const Settings &s, const double *freqs, Work
                      hipStream_t st);
 void launch_adding(const Dims &d, const Settings &s, const double *freqs, const char *sfc_refl,
                    Work &w, double *tb, hipStream_t st);
+// n 32-bit words at p set to v (a kernel, so that it can sit in a captured
+// graph)
+void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t st);
 }  // namespace pam

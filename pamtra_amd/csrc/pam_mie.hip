@@ -219,28 +219,49 @@ __device__ __forceinline__ void mie_kmap(int g, int k, int np, int &colu, int &n
   colu = apart ? n - 1 : np + n - 1;
 }
 
-__global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(2))) k_mie(Dims d, Settings s, const pam_hydro_desc *hyd,
-                                                const double *freqs, Work w) {
-  __shared__ double s_u[U_CAP];      // U of the chunk; at the end the normalised phase sums
-  __shared__ double s_bin[3][MIE_WG];  // per bin of the chunk: conj(m) (re, im) and 1/x
-  __shared__ double s_wb[MIE_WG];    // per-bin DSD weight n(D) dD of the chunk, 0 if skipped
-  __shared__ double s_ps[3][MAX_NQUAD];  // weighted phase sums per angle
-  __shared__ double s_tab[2 * MIE_TAB];  // angle table stage
-  __shared__ double s_pw[4][6 * 16];     // per-wave phase sums of shared angle tiles
-  __shared__ double s_lc[NCOEF][MAXNLEG + 1];
-  __shared__ double s_red[4][3];     // per-wave efficiency sums of the chunk
-  __shared__ int s_redi[4][2];       // per-wave bin and series counts
-  __shared__ double s_hm[MAX_HYDRO][2];  // per hydrometeor: refractive index
-  __shared__ int s_hnb[MAX_HYDRO], s_hnt[MAX_HYDRO];  // bins, series length (0: none)
-  __shared__ short s_nmb[MIE_MAXBIN];  // series length of every bin (0: skipped)
-  __shared__ short s_cb[MIE_MAXCHUNK + 1], s_cnp[MIE_MAXCHUNK], s_cnm[MIE_MAXCHUNK];
-  __shared__ int s_nc;
-  __shared__ double s_sum[3];
-  __shared__ int s_nlh, s_nan;
+struct MieLds {
+  double u[U_CAP];             // U of the chunk; at the end the normalised phase sums
+  double bin[3][MIE_WG];       // per bin of the chunk: conj(m) (re, im) and 1/x
+  double wb[MIE_WG];           // per-bin DSD weight n(D) dD of the chunk, 0 if skipped
+  double ps[3][MAX_NQUAD];     // weighted phase sums per angle
+  double tab[2 * MIE_TAB];     // angle table stage
+  double pw[4][6 * 16];        // per-wave phase sums of shared angle tiles
+  double lc[NCOEF][MAXNLEG + 1];
+  double red[4][3];            // per-wave efficiency sums of the chunk
+  int redi[4][2];              // per-wave bin and series counts
+  double hm[MAX_HYDRO][2];     // per hydrometeor: refractive index
+  int hnb[MAX_HYDRO], hnt[MAX_HYDRO];  // bins, series length (0: none)
+  short nmb[MIE_MAXBIN];       // series length of every bin (0: skipped)
+  short cb[MIE_MAXCHUNK + 1], cnp[MIE_MAXCHUNK], cnm[MIE_MAXCHUNK];
+  int nc;
+  double sum[3];
+  int nlh, nan_;
+  int slot;                    // unit index of a persistent workgroup
+};
+
+// One (scattering layer, channel) unit: Mie of every hydrometeor of the
+// layer and their accumulation into the layer's Legendre series.
+__device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
+                                         const pam_hydro_desc *hyd, const double *freqs,
+                                         const Work &w, int itf, MieLds &L) {
+  double *s_u = L.u;
+  double(*s_bin)[MIE_WG] = L.bin;
+  double *s_wb = L.wb;
+  double(*s_ps)[MAX_NQUAD] = L.ps;
+  double *s_tab = L.tab;
+  double(*s_pw)[6 * 16] = L.pw;
+  double(*s_lc)[MAXNLEG + 1] = L.lc;
+  double(*s_red)[3] = L.red;
+  int(*s_redi)[2] = L.redi;
+  double(*s_hm)[2] = L.hm;
+  int *s_hnb = L.hnb, *s_hnt = L.hnt;
+  short *s_nmb = L.nmb, *s_cb = L.cb, *s_cnp = L.cnp, *s_cnm = L.cnm;
+  int &s_nc = L.nc;
+  double *s_sum = L.sum;
+  int &s_nlh = L.nlh, &s_nan = L.nan_;
 
   const int t = threadIdx.x;
   const int wv = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
-  const int itf = blockIdx.x;
   const int li_item = itf / d.nf, f = itf % d.nf;
   const int col = w.item_col[li_item], z = w.item_layer[li_item];
   int32_t *st = w.itf_status + itf;
@@ -741,13 +762,32 @@ __global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(2))
   }
 }
 
+// One unit per workgroup, or (PERS) a persistent grid taking units from a
+// work queue (separate instantiations).
+template <bool PERS>
+__global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(2))) k_mie(
+    Dims d, Settings s, const pam_hydro_desc *hyd, const double *freqs, Work w) {
+  __shared__ MieLds L;
+  const int n = d.nitems * d.nf;
+  int itf = PERS ? next_unit(&w.queue[0], &L.slot) : (int)blockIdx.x;
+  while (itf < n) {
+    mie_unit(d, s, hyd, freqs, w, itf, L);
+    if (!PERS) break;
+    itf = next_unit(&w.queue[0], &L.slot);
+  }
+}
+
 void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, const Hydro4D &h4,
                 const Profile &p, const double *freqs, Work &w, hipStream_t st) {
   (void)h4;
   (void)p;
   size_t n = (size_t)d.nitems * d.nf;
   if (n == 0) return;
-  hipLaunchKernelGGL(k_mie, dim3((unsigned)n), dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+  const dim3 grid((unsigned)persistent_grid(n, s.pers_mie, s.ncu));
+  if (grid.x < n)
+    hipLaunchKernelGGL(k_mie<true>, grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+  else
+    hipLaunchKernelGGL(k_mie<false>, grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
 }
 
 }  // namespace pam

@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(PH_WG) k_phase(Work w) {
 void launch_phase(const Dims &d, const Settings &s, Work &w, hipStream_t st) {
   const int nif = d.nitems * d.nf;
   if (nif == 0 || !s.lhyd_scat) return;
-  (void)hipMemsetAsync(w.ph_cnt, 0, sizeof(uint32_t) * (PH_NBUCKET + 1), st);
+  launch_fill_u32(w.ph_cnt, PH_NBUCKET + 1, 0, st);
   const int nb = (nif + 255) / 256;
   hipLaunchKernelGGL(k_phase_count, dim3(nb), dim3(256), 0, st, nif, s, w);
   hipLaunchKernelGGL(k_phase_scan, dim3(1), dim3(64), 0, st, w);

@@ -315,6 +315,7 @@ struct DblLds {
   double norm[NMU];
   InvLds inv;
   double mu[NMU], qw[NMU];
+  int slot;  // unit index of a persistent workgroup
 };
 
 // One (scattering layer, channel) unit.  Returns false only in the series
@@ -462,15 +463,24 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
   return true;
 }
 
+// One unit per workgroup, or (PERS) a persistent grid taking units from a
+// work queue.  The two are separate instantiations, so that the loop does
+// not cost the one-unit kernel registers.
+template <bool PERS>
 __global__ void __launch_bounds__(RT_WG) __attribute__((amdgpu_waves_per_eu(4))) k_doubling(Dims d, Settings s, const double *freqs,
                                                     Work w) {
   __shared__ DblLds L;
-  const int itf = blockIdx.x;
-  // inverse_gj == 2 (a test mode) queues every unit for the Gauss-Jordan pass
-  const bool done = s.inverse_gj != 2 && doubling_unit<false>(d, s, freqs, w, itf, L);
-  if (!done && threadIdx.x == 0) {
-    const unsigned q = atomicAdd(&w.redo_n[0], 1u);
-    w.redo_dbl[q] = itf;
+  const int n = d.nitems * d.nf;
+  int itf = PERS ? next_unit(&w.queue[1], &L.slot) : (int)blockIdx.x;
+  while (itf < n) {
+    // inverse_gj == 2 (a test mode) queues every unit for the Gauss-Jordan pass
+    const bool done = s.inverse_gj != 2 && doubling_unit<false>(d, s, freqs, w, itf, L);
+    if (!done && threadIdx.x == 0) {
+      const unsigned q = atomicAdd(&w.redo_n[0], 1u);
+      w.redo_dbl[q] = itf;
+    }
+    if (!PERS) break;
+    itf = next_unit(&w.queue[1], &L.slot);
   }
 }
 
@@ -689,6 +699,7 @@ struct MixLds {
   double norm[NMU];
   float wmax[4];
   double mu[NMU], qw[NMU];
+  int slot;  // unit index of a persistent workgroup
 };
 
 // One (scattering layer, channel) unit of the mixed-precision path; false
@@ -920,13 +931,19 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   return true;
 }
 
+template <bool PERS>
 __global__ void __launch_bounds__(RT_WG) k_doubling_mixed(Dims d, Settings s, const double *freqs,
                                                           Work w) {
   __shared__ MixLds L;
-  const int itf = blockIdx.x;
-  if (!doubling_unit_f32(d, s, freqs, w, itf, L) && threadIdx.x == 0) {
-    const unsigned q = atomicAdd(&w.redo_n[0], 1u);
-    w.redo_dbl[q] = itf;
+  const int n = d.nitems * d.nf;
+  int itf = PERS ? next_unit(&w.queue[1], &L.slot) : (int)blockIdx.x;
+  while (itf < n) {
+    if (!doubling_unit_f32(d, s, freqs, w, itf, L) && threadIdx.x == 0) {
+      const unsigned q = atomicAdd(&w.redo_n[0], 1u);
+      w.redo_dbl[q] = itf;
+    }
+    if (!PERS) break;
+    itf = next_unit(&w.queue[1], &L.slot);
   }
 }
 
@@ -1221,6 +1238,7 @@ struct AddLds {
   InvLds inv;
   double mu[NMU], qw[NMU];
   int bad[1];
+  int slot;  // unit index of a persistent workgroup
 };
 
 // One (column, channel) unit of the chunk.  Returns false only in the
@@ -1465,15 +1483,22 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
   return true;
 }
 
+template <bool PERS>
 __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const double *freqs,
                                                   const char *sfc_refl, Work w, double *tb) {
   __shared__ AddLds LL;
-  const int cf = blockIdx.x;
-  // inverse_gj == 2 (a test mode) queues every unit for the Gauss-Jordan pass
-  const bool done = s.inverse_gj != 2 && adding_unit<false>(d, s, freqs, sfc_refl, w, tb, cf, LL);
-  if (!done && threadIdx.x == 0) {
-    const unsigned q = atomicAdd(&w.redo_n[1], 1u);
-    w.redo_add[q] = cf;
+  const int n = d.ncol_chunk * d.nf;
+  int cf = PERS ? next_unit(&w.queue[2], &LL.slot) : (int)blockIdx.x;
+  while (cf < n) {
+    // inverse_gj == 2 (a test mode) queues every unit for the Gauss-Jordan pass
+    const bool done =
+        s.inverse_gj != 2 && adding_unit<false>(d, s, freqs, sfc_refl, w, tb, cf, LL);
+    if (!done && threadIdx.x == 0) {
+      const unsigned q = atomicAdd(&w.redo_n[1], 1u);
+      w.redo_add[q] = cf;
+    }
+    if (!PERS) break;
+    cf = next_unit(&w.queue[2], &LL.slot);
   }
 }
 
@@ -1500,10 +1525,19 @@ void launch_doubling(const Dims &d, const Settings &s, const double *freqs, Work
                        st, d, s, freqs, w, 1);
     return;
   }
-  if (s.precision == PAM_PREC_MIXED && s.inverse_gj == 0)
-    hipLaunchKernelGGL(k_doubling_mixed, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, w);
-  else
-    hipLaunchKernelGGL(k_doubling, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, w);
+  const dim3 grid((unsigned)persistent_grid(n, s.pers_dbl, s.ncu));
+  const bool pers = grid.x < n;
+  if (s.precision == PAM_PREC_MIXED && s.inverse_gj == 0) {
+    if (pers)
+      hipLaunchKernelGGL(k_doubling_mixed<true>, grid, dim3(RT_WG), 0, st, d, s, freqs, w);
+    else
+      hipLaunchKernelGGL(k_doubling_mixed<false>, grid, dim3(RT_WG), 0, st, d, s, freqs, w);
+  } else {
+    if (pers)
+      hipLaunchKernelGGL(k_doubling<true>, grid, dim3(RT_WG), 0, st, d, s, freqs, w);
+    else
+      hipLaunchKernelGGL(k_doubling<false>, grid, dim3(RT_WG), 0, st, d, s, freqs, w);
+  }
   hipLaunchKernelGGL(k_doubling_gj, dim3((unsigned)std::min<size_t>(n, 2048)), dim3(RT_WG), 0, st,
                      d, s, freqs, w, 0);
 }
@@ -1517,10 +1551,23 @@ void launch_adding(const Dims &d, const Settings &s, const double *freqs, const 
                        d, s, freqs, sfc_refl, w, tb, 1);
     return;
   }
-  hipLaunchKernelGGL(k_adding, dim3((unsigned)n), dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
-                     tb);
+  const dim3 grid((unsigned)persistent_grid(n, s.pers_add, s.ncu));
+  if (grid.x < n)
+    hipLaunchKernelGGL(k_adding<true>, grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
+  else
+    hipLaunchKernelGGL(k_adding<false>, grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
   hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, 2048)), dim3(RT_WG), 0, st, d,
                      s, freqs, sfc_refl, w, tb, 0);
+}
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
 
 }  // namespace pam

@@ -123,11 +123,13 @@ class ForwardServer:
 
     ``runner(slot, request) -> (status, message, tb)`` replaces the device
     call (used by the CPU tests of the admission logic); ``need(request)``
-    replaces the workspace sizing then.
+    replaces the workspace sizing then.  ``tuning`` (pam_context_set_tuning
+    keys, e.g. from pamtra_amd.autotune) is applied to every slot's context.
     """
 
     def __init__(self, device: int = 0, slots: int = 4, hbm_budget_gb: float = 64.0,
-                 runner: Optional[Callable] = None, need: Optional[Callable] = None):
+                 runner: Optional[Callable] = None, need: Optional[Callable] = None,
+                 tuning: Optional[dict] = None):
         if slots < 1:
             raise ValueError("slots must be >= 1")
         self.slots = slots
@@ -149,6 +151,9 @@ class ForwardServer:
                     raise _lib.PamLibraryError("pam_context_create failed")
                 self._lib.pam_context_set_workspace_limit(h, self.optics_limit)
                 self._ctx.append(h)
+                if tuning:
+                    from . import autotune
+                    autotune.apply(self._lib, h, tuning)
         # workspace each slot's context holds (grow-only until trimmed); the
         # library rounds its allocations up by an eighth
         self._held = [0] * slots
