@@ -137,10 +137,13 @@ def main():
                     help="run only the serving mix and print its summary")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes (no autotuning)")
-    ap.add_argument("--tune-cols", type=int, default=4096,
-                    help="columns of the workload slice the autotuner measures on")
+    ap.add_argument("--tune-cols", type=int, default=0,
+                    help="columns of the workload slice the autotuner measures on "
+                         "(0: the whole workload, so that every launch of a kernel has "
+                         "the timed run's shape)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="also run one device copy of MIB MiB (to scale HBM counters)")
+                    help="also fill and then sum a buffer of MIB MiB (kernels of known "
+                         "HBM traffic, to scale the HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the columns are "
@@ -184,7 +187,7 @@ def main():
     def slice_runner(precision):
         """One synchronous call on the first --tune-cols columns (the
         autotuner's sample of the workload)."""
-        ns = min(args.tune_cols, nx)
+        ns = min(args.tune_cols, nx) if args.tune_cols > 0 else nx
         sub = type("Sub", (), {})()
         sub.p = slice_columns(pam, ns)
         ps = to_device(sub, torch)
@@ -236,7 +239,7 @@ def main():
         log(f"autotuned in {time.time() - t0:.1f} s: {ch}")
         tuning = {"fp64": ch, "default_shapes_value": round(nx * ny * nf / dt0, 2),
                   "default_shapes_stage_ms": {n: round(v, 3) for n, v in zip(names, stage0)},
-                  "sample_columns": min(args.tune_cols, nx)}
+                  "sample_columns": min(args.tune_cols, nx) if args.tune_cols > 0 else nx}
         for i in range(args.warmup):
             step()
     dt, stage_ms = timed(step, args.steps)
@@ -329,12 +332,14 @@ def main():
         host_rate = nx * ny * nf / th
         log(f"host-array entry (PCIe-inclusive): {host_rate:.0f} col*ch/s")
     if args.copy_calibration:
+        # kernels of known traffic for the HBM counters: a fill writes the
+        # buffer once, a sum reads it once
         n = args.copy_calibration * (1 << 20) // 8
         a = torch.ones(n, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
+        total = float(a.sum())
         torch.cuda.synchronize()
-        log(f"copy calibration: {args.copy_calibration} MiB read + written")
+        log(f"counter calibration: {args.copy_calibration} MiB written (fill), "
+            f"then read (sum = {total:.0f})")
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "columns·channels/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
