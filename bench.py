@@ -81,7 +81,7 @@ def measured_traffic(kernel):
     return None
 
 
-def serve(args, precision="fp64", tuning=None):
+def serve(args, precision="fp64", tuning=None, rate=50.0):
     """The serving mix at an offered Poisson rate: throughput and p50/p99
     latency (arrival to result on the host, input staging included)."""
     import numpy as np
@@ -89,7 +89,7 @@ def serve(args, precision="fp64", tuning=None):
     rng = np.random.default_rng(1004)
     t0 = time.time()
     mix = workloads.serving_requests(args.serve_requests, seed=1004)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, size=len(mix)))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(mix)))
     for p, _ in mix:
         p.set["precision"] = precision
     reqs = [serving.Request(pam=p, freqs=f, arrival=float(a), rid=i)
@@ -108,7 +108,7 @@ def serve(args, precision="fp64", tuning=None):
         wall = time.perf_counter() - t0
         peak = srv.peak_reserved
     summ = serving.summarize(res, reqs, wall)
-    summ.update({"offered_req_s": args.serve_rate, "slots": args.serve_slots,
+    summ.update({"offered_req_s": rate, "slots": args.serve_slots,
                  "hbm_budget_gib": 64.0, "peak_workspace_gib": round(peak / 2 ** 30, 3),
                  "precision": precision, "launch_tuning": shapes,
                  "mix": "1-4096 columns (log-uniform), 20-150 layers, 1-12 channels of "
@@ -131,7 +131,8 @@ def main():
                     help="skip the pam_run (host arrays, PCIe-inclusive) measurement")
     ap.add_argument("--serve-requests", type=int, default=100,
                     help="requests of the serving mix (0: no serving run)")
-    ap.add_argument("--serve-rate", type=float, default=50.0, help="offered requests/s (Poisson)")
+    ap.add_argument("--serve-rate", type=str, default="10,50",
+                    help="offered requests/s (Poisson), comma-separated: one serving run each")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving mix and print its summary")
@@ -154,8 +155,10 @@ def main():
     import torch
     from pamtra_amd import _capi, _lib, workloads
 
+    rates = [float(x) for x in str(args.serve_rate).split(",") if x.strip()]
     if args.serve_only:
-        print(json.dumps({"serving": {"fp64": serve(args), "mixed": serve(args, "mixed")}}))
+        print(json.dumps({"serving": {"fp64": [serve(args, rate=r) for r in rates],
+                                      "mixed": [serve(args, "mixed", rate=r) for r in rates]}}))
         return
 
     t0 = time.time()
@@ -315,7 +318,8 @@ def main():
                  "stage_ms": {n: round(v / args.steps, 3) for n, v in zip(names, sms)},
                  "speedup_vs_fp64": round(dt / dtm, 3),
                  "max_abs_dtb_vs_fp64_K": {f"{fr:g}": float(v) for fr, v in zip(freqs, dch)},
-                 "dtype": "f32 doubling products (f64 thin-layer start, Mie, adding)"}
+                 "dtype": "f32 products in the doubling and adding (f64: start layer, adding "
+                          "inverses, Mie, phase matrix)"}
         log(f"mixed: {mixed}")
 
     host_rate = None
@@ -362,9 +366,11 @@ def main():
     }
 
     if args.serve_requests > 0:
-        out["serving"] = {"fp64": serve(args, tuning=tuning and tuning.get("fp64"))}
+        out["serving"] = {"fp64": [serve(args, tuning=tuning and tuning.get("fp64"), rate=r)
+                                   for r in rates]}
         if not args.no_mixed:
-            out["serving"]["mixed"] = serve(args, "mixed", tuning=tuning and tuning.get("mixed"))
+            out["serving"]["mixed"] = [serve(args, "mixed", tuning=tuning and tuning.get("mixed"),
+                                             rate=r) for r in rates]
 
     if not args.no_cpu:
         import oracle

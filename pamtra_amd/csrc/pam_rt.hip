@@ -981,14 +981,61 @@ __device__ __forceinline__ void mv_stage(const double *A, double *v) {
   __syncthreads();
 }
 
+// Two independent products C = D + A B on the FP32 matrix cores with FP64
+// operands and results (the mixed-precision adding): operands are rounded to
+// single precision as they are loaded, products accumulate in FP32, and D is
+// added in FP64.  Layout of 16x16x4 f32: lane l holds C[4*(l/16)+r][l%16].
+// All operands are read before the first write (outputs may alias inputs).
+__device__ __forceinline__ void mm2x(const double *A1, const double *B1, const double *D1,
+                                     double *C1, const double *A2, const double *B2,
+                                     const double *D2, double *C2) {
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
+  const int li = l & 15, lk = l >> 4;
+  f4_t acc1 = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
+  double d1[4], d2[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    d1[r] = D1 ? D1[(I + 4 * lk + r) * LD + J + li] : 0.0;
+    d2[r] = D2 ? D2[(I + 4 * lk + r) * LD + J + li] : 0.0;
+  }
+#pragma unroll
+  for (int s = 0; s < NN / 4; s++) {
+    const int kc = 4 * s + lk;
+    const float a1 = (float)A1[(I + li) * LD + kc], a2 = (float)A2[(I + li) * LD + kc];
+    const float b1 = (float)B1[kc * LD + J + li], b2 = (float)B2[kc * LD + J + li];
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc1, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc2, 0, 0, 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    C1[(I + 4 * lk + r) * LD + J + li] = d1[r] + (double)acc1[r];
+    C2[(I + 4 * lk + r) * LD + J + li] = d2[r] + (double)acc2[r];
+  }
+  __syncthreads();
+}
+
+// mm2 in FP64, or mm2x in the mixed path (MX)
+template <bool MX>
+__device__ __forceinline__ void mm2p(const double *A1, const double *B1, const double *D1,
+                                     double *C1, const double *A2, const double *B2,
+                                     const double *D2, double *C2) {
+  if constexpr (MX)
+    mm2x(A1, B1, D1, C1, A2, B2, D2, C2);
+  else
+    mm2(A1, B1, D1, C1, A2, B2, D2, C2, nullptr);
+}
+
 // COMBINE_LAYERS4 (radintg4.f90:447-513): stack on top of a doubled layer
 // with R2+ = R2- = R2 and T2+ = T2- = T2.  The minus side goes first, since
 // it reads the stack's old R+ and T+.  Each side is grouped as
 //   A = T1m G,  R1m' = R1m + A (R2 T1p),  T1m' = A T2,  S1m' = S1m + A (...)
 //   B = T2 G',  R1p' = R2 + B (R1p T2),   T1p' = B T1p,  S1p' = S2p + B (...)
 // four products in two fused passes per side (the reference's order of the
-// products differs only by association).
-template <bool GJ>
+// products differs only by association).  MX: the mixed path's FP32
+// products (the inverses stay FP64).
+template <bool GJ, bool MX>
 __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, const double *S2p,
                              const double *S2m, double *G, double *X, double *vs, InvLds &sinv, int &np) {
   const int t = threadIdx.x;
@@ -997,20 +1044,20 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
     const double u = mv_row8(R2, s1.Sp, t >> 3);
     if ((t & 7) == 0) vs[t >> 3] = S2m[t >> 3] + u;  // S2m + R2m S1p
   }
-  mm2(s1.Tm, G, nullptr, X, R2, s1.Tp, nullptr, G, nullptr);  // X = A, G = R2 T1p
+  mm2p<MX>(s1.Tm, G, nullptr, X, R2, s1.Tp, nullptr, G);  // X = A, G = R2 T1p
   mv_stage(X, vs);
   if (t < NN) s1.Sm[t] = s1.Sm[t] + vs[t];                     // S1m + A (...)
-  mm2(X, G, s1.Rm, s1.Rm, X, T2, nullptr, s1.Tm, nullptr);      // R1m', T1m'
+  mm2p<MX>(X, G, s1.Rm, s1.Rm, X, T2, nullptr, s1.Tm);      // R1m', T1m'
   np += 4;
   if (!inv_i_minus_ab<GJ>(s1.Rp, R2, X, G, sinv, np)) return false;
   {
     const double u = mv_row8(s1.Rp, S2m, t >> 3);
     if ((t & 7) == 0) vs[t >> 3] = s1.Sp[t >> 3] + u;  // S1p + R1p S2m
   }
-  mm2(T2, G, nullptr, X, s1.Rp, T2, nullptr, G, nullptr);       // X = B, G = R1p T2
+  mm2p<MX>(T2, G, nullptr, X, s1.Rp, T2, nullptr, G);       // X = B, G = R1p T2
   mv_stage(X, vs);
   if (t < NN) s1.Sp[t] = S2p[t] + vs[t];                       // S2p + B (...)
-  mm2(X, G, R2, s1.Rp, X, s1.Tp, nullptr, s1.Tp, nullptr);      // R1p', T1p'
+  mm2p<MX>(X, G, R2, s1.Rp, X, s1.Tp, nullptr, s1.Tp);      // R1p', T1p'
   np += 4;
   __syncthreads();
   return true;
@@ -1243,8 +1290,9 @@ struct AddLds {
 
 // One (column, channel) unit of the chunk.  Returns false only in the
 // series variant (GJ = false) when an inverse did not converge; the unit is
-// then recomputed from scratch by the Gauss-Jordan pass.
-template <bool GJ>
+// then recomputed from scratch by the Gauss-Jordan pass (in FP64).  MX: the
+// combines' products on the FP32 matrix cores (mixed precision).
+template <bool GJ, bool MX = false>
 __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freqs,
                             const char *sfc_refl, const Work &w, double *tb, int cf, AddLds &LL) {
   double(*M)[MSZ] = LL.M;
@@ -1426,7 +1474,7 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
       }
       __syncthreads();
     } else if (sc) {
-      if (!combine_scat<GJ>(stk, R2, T2, S2p, S2m, G, X, vs, sinv, nprod)) {
+      if (!combine_scat<GJ, MX>(stk, R2, T2, S2p, S2m, G, X, vs, sinv, nprod)) {
         if (!GJ) return false;
         fail(E_SINGULAR | PAM_FATAL);
         return true;
@@ -1483,7 +1531,7 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
   return true;
 }
 
-template <bool PERS>
+template <bool PERS, bool MX>
 __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const double *freqs,
                                                   const char *sfc_refl, Work w, double *tb) {
   __shared__ AddLds LL;
@@ -1492,7 +1540,7 @@ __global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const doub
   while (cf < n) {
     // inverse_gj == 2 (a test mode) queues every unit for the Gauss-Jordan pass
     const bool done =
-        s.inverse_gj != 2 && adding_unit<false>(d, s, freqs, sfc_refl, w, tb, cf, LL);
+        s.inverse_gj != 2 && adding_unit<false, MX>(d, s, freqs, sfc_refl, w, tb, cf, LL);
     if (!done && threadIdx.x == 0) {
       const unsigned q = atomicAdd(&w.redo_n[1], 1u);
       w.redo_add[q] = cf;
@@ -1552,10 +1600,15 @@ void launch_adding(const Dims &d, const Settings &s, const double *freqs, const 
     return;
   }
   const dim3 grid((unsigned)persistent_grid(n, s.pers_add, s.ncu));
-  if (grid.x < n)
-    hipLaunchKernelGGL(k_adding<true>, grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
+  const bool pers = grid.x < n, mx = s.precision == PAM_PREC_MIXED;
+  if (pers && mx)
+    hipLaunchKernelGGL((k_adding<true, true>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
+  else if (pers)
+    hipLaunchKernelGGL((k_adding<true, false>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
+  else if (mx)
+    hipLaunchKernelGGL((k_adding<false, true>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
   else
-    hipLaunchKernelGGL(k_adding<false>, grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
+    hipLaunchKernelGGL((k_adding<false, false>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
   hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, 2048)), dim3(RT_WG), 0, st, d,
                      s, freqs, sfc_refl, w, tb, 0);
 }

@@ -214,3 +214,17 @@ def test_device_entry_point_matches_host_entry_point():
                             C.byref(m.profile), fp, nf, C.byref(r), None)
     assert st == 0, msg.value
     np.testing.assert_array_equal(tb.cpu().numpy(), host["tb"])
+
+
+def test_large_particles_high_frequency():
+    """Four categories up to 13 mm at 183 and 325 GHz: series of up to ~60
+    terms, whose angle tables no longer fit the Mie kernel's LDS stage and are
+    read from memory, and the longest Legendre series."""
+    pam, _ = workloads.config3(ncol=6, seed=91)
+    freqs = [183.31, 325.0]
+    g, st = _gpu(pam, freqs)
+    c, st2 = _cpu(pam, freqs)
+    assert st == st2, g["msg"]
+    print("max nlegen", int(np.max(c["nlegen"])), flush=True)
+    assert np.max(c["nlegen"]) > 80
+    _compare(g, c)
