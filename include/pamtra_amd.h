@@ -49,10 +49,15 @@ enum pam_scat_name { PAM_SCAT_DISABLED = 0, PAM_SCAT_MIE_SPHERE = 1 };
 /* arithmetic of the device path */
 enum pam_precision { PAM_PREC_FP64 = 0, PAM_PREC_MIXED = 1 };
 
-/* nmlSet entries that reach the passive path (python/pyPamtra/core.py:88-157,
+/* nmlSet entries that reach the forward operator (python/pyPamtra/core.py:88-157,
  * defaults src/settings.f90:454-573).  gas_mod: 0 = 'R98'; liq_mod: 0 = 'TKC';
- * outpol: 0 = 'VH', 1 = 'IQ'.  active must be 0 (the radar simulator is not
- * part of this library and is refused with PAM_FATAL). */
+ * outpol: 0 = 'VH', 1 = 'IQ'.  active = 1 adds the radar simulator in its
+ * "simple" mode (src/radar_simulator.f90:94-170): reflectivity Ze from the
+ * Mie backscattering, hydrometeor and gas attenuation, radar heights.
+ * radar_attenuation: 0 'disabled', 1 'top-down', 2 'bottom-up';
+ * radar_pol[0..radar_npol-1]: 0 'NN', 1 'HH', 2 'VV', 3 'HV', 4 'VH';
+ * radar_k2: |K|^2 of the radar equation per frequency (nfreq values, NULL:
+ * 0.93 for every frequency, src/settings.f90:530). */
 typedef struct pam_settings {
   int32_t passive;
   int32_t active;
@@ -71,6 +76,11 @@ typedef struct pam_settings {
   double hydro_threshold;
   double hydro_softsphere_min_density;
   double emissivity;
+  int32_t radar_attenuation;
+  int32_t radar_npol;
+  int32_t radar_pol[5];
+  int32_t radar_pad_;
+  const double *radar_k2;
 } pam_settings;
 
 /* One row of the 20-column descriptor file
@@ -157,6 +167,13 @@ typedef struct pam_results {
   char *msg;             /* first error message, NUL terminated */
   int32_t msglen;
   int32_t pad_;
+  /* radar outputs (settings.active = 1; vars_output.f90:144-155): -9999 in
+   * Ze and radar_hgt where the reference leaves its fill value, NaN where a
+   * (column, frequency) failed */
+  double *ze;            /* (nx, ny, L, nfreq, radar_npol, 1) [dBz] */
+  double *att_hydro;     /* (nx, ny, L, nfreq, 1) [dB] one way */
+  double *att_atmo;      /* (nx, ny, L, nfreq) [dB] one way */
+  double *radar_hgt;     /* (nx, ny, L) [m] */
 } pam_results;
 
 typedef struct pam_context pam_context;

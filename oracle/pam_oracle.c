@@ -1551,6 +1551,7 @@ static double hydro_field(const pam_hydro_desc *hd, const double *arr4d, double 
 typedef struct {
   int present;       /* rt_hydros_present */
   double kexttot, salbedo;
+  double back;       /* rt_back of the co-polar channels (scatProperties.f90:743-757, 922) */
   int nlegen;        /* -1 when no scatcnv */
   double coef[3][MAXNLEG + 1];
 } layer_optics;
@@ -1640,6 +1641,7 @@ static int layer_hydrometeors(const pam_settings *s, const pam_hydro_desc *hyd,
       work[1] += mr.sum_series;
     }
     if (mr.nlegen > nleg_coef) nleg_coef = mr.nlegen;
+    lo->back = lo->back + mr.back_scatt;
     kexttot = kexttot + mr.extinction;
     if (kexttot == 0.0)
       salbedo = 0.0;
@@ -1680,8 +1682,8 @@ static int run_column_freq(const pam_settings *s, const pam_hydro_desc *hyd,
                            const pam_hydro_4d *h4, const pam_profile *p, int ix, int iy,
                            const column *c, double freq, const double *mu, const double *qw,
                            double *tb_out, double *emis_out, double *d_kgas, double *d_kext,
-                           double *d_salb, int32_t *d_nleg, int32_t *d_nd, int64_t *work,
-                           char *msg) {
+                           double *d_salb, int32_t *d_nleg, int32_t *d_nd, double *d_back,
+                           int32_t *d_pres, int64_t *work, char *msg) {
   int L = c->L, nout = p->noutlevels;
   size_t col = (size_t)ix * p->ngridy + iy;
   double wavelength = C_LIGHT / (freq * 1.e3);
@@ -1760,6 +1762,8 @@ static int run_column_freq(const pam_settings *s, const pam_hydro_desc *hyd,
     if (d_kext) d_kext[z] = lo.present ? lo.kexttot : 0.0;
     if (d_salb) d_salb[z] = lo.present ? lo.salbedo : 0.0;
     if (d_nleg) d_nleg[z] = lo.present ? lo.nlegen : -1;
+    if (d_back) d_back[z] = lo.present ? lo.back : 0.0;
+    if (d_pres) d_pres[z] = lo.present;
     if (!lo.present) {
       nonscatter4(zdiff * gas, mu, p0, p1, &ops[lr]);
       if (d_nd) d_nd[z] = -1;
@@ -1963,11 +1967,65 @@ static int run_column_freq(const pam_settings *s, const pam_hydro_desc *hyd,
 
 /* run_pamtra (src/pyPamtraLib.f90:18-148) on the CPU: the oracle's version
  * of pam_run.  OpenMP over (column, frequency) when compiled with -fopenmp. */
+/* The radar simulator in its "simple" mode for one (column, frequency)
+ * (radar_simulator.f90:94-170, with the attenuation of run_rt.f90:118-123):
+ * per layer z < L the one-way gas and hydrometeor attenuation [dB], and for
+ * every layer with hydrometeors Ze = 10 log10(1e18 back lambda^4 /
+ * (|K|^2 pi^5) / PIA) [dBz], -9999 when back = 0.  The layers are visited
+ * bottom-up as in hydrometeor_extinction, so a top-down PIA sees the
+ * hydrometeor attenuation of the layers above as not yet set (0). */
+static void radar_simple(const pam_settings *s, double freq_ghz, double k2, int L,
+                         const column *c, const double *kgas, const double *kext,
+                         const double *back, const int32_t *pres, double *att_atmo,
+                         double *att_hydro, double *ze /* [L][npol] */) {
+  const double wl = C_LIGHT / (freq_ghz * 1.e9);
+  for (int z = 0; z < L; z++) {
+    const double dz = c->hgt_lev[z + 1] - c->hgt_lev[z];
+    att_atmo[z] = 10.0 * log10(exp(kgas[z] * dz));
+    att_hydro[z] = 0.0;
+    for (int ip = 0; ip < s->radar_npol; ip++) ze[z * s->radar_npol + ip] = -9999.0;
+  }
+  for (int z = 0; z < L; z++) {
+    if (!pres[z]) continue;
+    const double dz = c->hgt_lev[z + 1] - c->hgt_lev[z];
+    att_hydro[z] = 10.0 * log10(exp(kext[z] * dz));
+    for (int ip = 0; ip < s->radar_npol; ip++) {
+      const int pol = s->radar_pol[ip];
+      const double b = (pol <= 2) ? back[z] : 0.0; /* NN, HH, VV; HV, VH: 0 */
+      double zb = 1.e18 * (1.0 / (k2 * pow(PI, 5))) * b * pow(wl, 4);
+      double pia = 0.0;
+      if (s->radar_attenuation == 1) { /* top-down */
+        if (z < L - 1) {
+          /* the hydrometeor attenuation of the layers above is still the
+           * fill value 0 when layer z is processed */
+          double sa = 0.0;
+          for (int k = L - 1; k >= z + 1; k--) sa += att_atmo[k];
+          pia = 2 * (0.0 + sa);
+        }
+        pia = pia + att_hydro[z] + att_atmo[z];
+      } else if (s->radar_attenuation == 2) { /* bottom-up */
+        if (z > 0) {
+          double sh = 0.0, sa = 0.0;
+          for (int k = 0; k <= z - 1; k++) {
+            sh += att_hydro[k];
+            sa += att_atmo[k];
+          }
+          pia = 2 * (sh + sa);
+        }
+        pia = pia + att_hydro[z] + att_atmo[z];
+      }
+      pia = pow(10.0, 0.1 * pia);
+      zb = zb / pia;
+      ze[z * s->radar_npol + ip] = (zb == 0.0) ? -9999.0 : 10 * log10(zb);
+    }
+  }
+}
+
 int pam_oracle_run(const pam_settings *s, const pam_hydro_desc *hyd, const pam_hydro_4d *h4,
                    const pam_profile *p, const double *freqs, int nfreq, pam_results *r,
                    int nthreads, int64_t *work_out) {
-  if (!s->passive || s->active) {
-    if (r->msg) snprintf(r->msg, r->msglen, "only passive-only runs are supported");
+  if (!s->passive) {
+    if (r->msg) snprintf(r->msg, r->msglen, "passive=False is not supported");
     return PAM_FATAL;
   }
   double mu[NMU], qw[NMU];
@@ -1997,20 +2055,43 @@ int pam_oracle_run(const pam_settings *s, const pam_hydro_desc *hyd, const pam_h
     int L = c.L;
     double *dk = NULL, *dx = NULL, *ds = NULL;
     int32_t *dn = NULL, *dd = NULL;
-    double *diagbuf = NULL;
-    int32_t *idiag = NULL;
+    double *diagbuf = NULL, *db = NULL;
+    int32_t *idiag = NULL, *dp = NULL;
     int64_t work[4] = {0, 0, 0, 0};
+    const int fill_ok = (st == PAM_SUCCESS);
     if (st == PAM_SUCCESS) {
-      diagbuf = (double *)malloc(sizeof(double) * 3 * L);
-      idiag = (int32_t *)malloc(sizeof(int32_t) * 2 * L);
+      diagbuf = (double *)malloc(sizeof(double) * 4 * L);
+      idiag = (int32_t *)malloc(sizeof(int32_t) * 3 * L);
       dk = diagbuf;
       dx = diagbuf + L;
       ds = diagbuf + 2 * L;
+      db = diagbuf + 3 * L;
       dn = idiag;
       dd = idiag + L;
+      dp = idiag + 2 * L;
       for (int i = 0; i < nout * PAM_NANGLES * 2; i++) tbp[i] = 0.0;
       st = run_column_freq(s, hyd, h4, p, ix, iy, &c, freqs[f], mu, qw, tbp, emis, dk, dx, ds, dn,
-                           dd, work, msg);
+                           dd, db, dp, work, msg);
+    }
+    if (s->active && (r->ze || r->att_hydro || r->att_atmo || r->radar_hgt)) {
+      const int npol = s->radar_npol;
+      double *ra = (double *)malloc(sizeof(double) * (2 + npol) * (L > 0 ? L : 1));
+      double *aa = ra, *ah = ra + L, *zz = ra + 2 * L;
+      if (st == PAM_SUCCESS)
+        radar_simple(s, freqs[f], s->radar_k2 ? s->radar_k2[f] : 0.93, L, &c, dk, dx, db, dp, aa,
+                     ah, zz);
+      for (int z = 0; z < ML; z++) {
+        const size_t o = ((size_t)colid * ML + z) * nfreq + f;
+        const int in = z < L;
+        if (r->att_atmo) r->att_atmo[o] = !in ? 0.0 : (st == PAM_SUCCESS ? aa[z] : NAN);
+        if (r->att_hydro) r->att_hydro[o] = !in ? 0.0 : (st == PAM_SUCCESS ? ah[z] : NAN);
+        if (r->ze)
+          for (int ip = 0; ip < npol; ip++)
+            r->ze[o * npol + ip] = !in ? -9999.0 : (st == PAM_SUCCESS ? zz[z * npol + ip] : NAN);
+        if (r->radar_hgt && f == 0)
+          r->radar_hgt[(size_t)colid * ML + z] = !in ? -9999.0 : (fill_ok ? c.hgt[z] : NAN);
+      }
+      free(ra);
     }
     if (r->tb)
       for (int io = 0; io < nout; io++)

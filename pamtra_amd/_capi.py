@@ -56,7 +56,16 @@ class PamSettings(C.Structure):
         ("hydro_threshold", C.c_double),
         ("hydro_softsphere_min_density", C.c_double),
         ("emissivity", C.c_double),
+        ("radar_attenuation", C.c_int32),
+        ("radar_npol", C.c_int32),
+        ("radar_pol", C.c_int32 * 5),
+        ("radar_pad_", C.c_int32),
+        ("radar_k2", _dp),
     ]
+
+
+RADAR_ATTENUATION = {"disabled": 0, "top-down": 1, "bottom-up": 2}
+RADAR_POL = {"NN": 0, "HH": 1, "VV": 2, "HV": 3, "VH": 4}
 
 
 class PamHydroDesc(C.Structure):
@@ -127,6 +136,10 @@ class PamResults(C.Structure):
         ("msg", C.c_char_p),
         ("msglen", C.c_int32),
         ("pad_", C.c_int32),
+        ("ze", _dp),
+        ("att_hydro", _dp),
+        ("att_atmo", _dp),
+        ("radar_hgt", _dp),
     ]
 
 
@@ -203,6 +216,25 @@ def make_settings(nmlSets: dict, sets: dict | None = None) -> PamSettings:
     s.hydro_threshold = float(g("hydro_threshold", 1e-10))
     s.hydro_softsphere_min_density = float(g("hydro_softsphere_min_density", 10.0))
     s.emissivity = float(g("emissivity", 0.6))
+    # radar simulator (src/radar_simulator.f90): the "simple" mode only
+    if s.active:
+        mode = str(g("radar_mode", "simple")).strip()
+        if mode != "simple":
+            raise NotImplementedError(f"radar_mode {mode!r}: only 'simple' is implemented")
+        if int(g("radar_npeaks", 1)) != 1:
+            raise ValueError("radar_mode 'simple' needs radar_npeaks=1 "
+                             "(radar_simulator.f90:161-163)")
+    att = str(g("radar_attenuation", "disabled")).strip()
+    if att not in RADAR_ATTENUATION:
+        raise ValueError(f"radar_attenuation {att!r} must be one of {sorted(RADAR_ATTENUATION)}")
+    s.radar_attenuation = RADAR_ATTENUATION[att]
+    pols = [x.strip() for x in str(g("radar_polarisation", "NN")).split(",") if x.strip()]
+    if not 1 <= len(pols) <= 5 or any(x not in RADAR_POL for x in pols):
+        raise ValueError(f"radar_polarisation {g('radar_polarisation')!r}: 1-5 of "
+                         f"{sorted(RADAR_POL)}")
+    s.radar_npol = len(pols)
+    for i, x in enumerate(pols):
+        s.radar_pol[i] = RADAR_POL[x]
     return s
 
 
@@ -253,6 +285,7 @@ class Marshalled:
     noutlevels: int
     nhydro: int
     keep: list = field(default_factory=list)
+    radar_k2: object = 0.93  # nmlSet radar_k2: one value or one per frequency
 
 
 def marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile) -> Marshalled:
@@ -323,7 +356,8 @@ def marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile) -> Marshal
         a = _as_f64(oh, (nx, ny, nout))
         keep.append(a)
         p.obs_height = _ptr(a, C.c_double)
-    return Marshalled(settings, hydros, h4, p, nx, ny, L, nout, nh, keep)
+    return Marshalled(settings, hydros, h4, p, nx, ny, L, nout, nh, keep,
+                      (nmlSets or {}).get("radar_k2", 0.93))
 
 
 @dataclass
@@ -342,6 +376,20 @@ def alloc_results(m: Marshalled, nfreq: int, diagnostics: bool = False,
         "angles_deg": np.full((NANGLES,), np.nan),
         "status": np.zeros((nx, ny, nfreq), dtype=np.int32),
     }
+    if m.settings.active:
+        npol = int(m.settings.radar_npol)
+        arrs["ze"] = np.full((nx, ny, L, nfreq, npol, 1), -9999.0)
+        arrs["att_hydro"] = np.zeros((nx, ny, L, nfreq, 1))
+        arrs["att_atmo"] = np.zeros((nx, ny, L, nfreq))
+        arrs["radar_hgt"] = np.full((nx, ny, L), -9999.0)
+        k2 = np.asarray(m.radar_k2, dtype=np.float64).reshape(-1)
+        if k2.size == 1:
+            k2 = np.full(nfreq, float(k2[0]))
+        if k2.size < nfreq:
+            raise ValueError(f"radar_k2 has {k2.size} values for {nfreq} frequencies")
+        k2 = np.ascontiguousarray(k2[:nfreq])
+        m.keep.append(k2)
+        m.settings.radar_k2 = k2.ctypes.data_as(C.POINTER(C.c_double))
     if diagnostics:
         arrs["kextatmo"] = np.full((nx, ny, L, nfreq), np.nan)
         arrs["kexttot"] = np.full((nx, ny, L, nfreq), np.nan)

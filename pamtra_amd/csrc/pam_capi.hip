@@ -422,7 +422,7 @@ static size_t col_bytes(const Dims &d) {
   auto add = [&](size_t b) { n += ((b + 255) & ~(size_t)255); };
   size_t ncol = d.ncol, ML = d.ML, nf = d.nf;
   add(2 * ncol * (ML + 1) * 8);
-  add(5 * ncol * ML * 8);
+  add(6 * ncol * ML * 8);
   add(ncol * 8);
   add(ncol * 4);
   add(ncol * d.nout * 4);
@@ -442,7 +442,7 @@ static size_t col_bytes(const Dims &d) {
 
 static size_t item_bytes_per(const Dims &d) {
   size_t per = 2 * 4 + (size_t)d.nh * d.nbmax * 4 * 8 + (size_t)d.nh * 4;
-  per += (size_t)d.nf * (2 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8 + 4 + 8);
+  per += (size_t)d.nf * (3 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8 + 4 + 8);
   return per;
 }
 
@@ -459,6 +459,7 @@ static void carve_items(Work &w, void *base, const Dims &d, size_t nitems) {
   w.dsd_flag = cv.take<int32_t>(nitems * d.nh);
   w.kext = cv.take<double>(nif);
   w.salb = cv.take<double>(nif);
+  w.back = cv.take<double>(nif);
   w.nleg = cv.take<int32_t>(nif);
   w.coef = cv.take<double>(nif * NCOEF * MAXNLEG);
   w.ndbl = cv.take<int32_t>(nif);
@@ -491,7 +492,7 @@ extern "C" int64_t pam_workspace_bytes(int64_t ncol, int32_t max_nlyrs, int32_t 
               c + c * no * 8 + 13 * c * L * nh * 8;
   io += c * no * PAM_NANGLES * nf * 2 * 8 + c * 2 * nf * NMU * 8 + PAM_NANGLES * 8 + c * nf * 4 +
         c * L * nf * (3 * 8 + 2 * 4);
-  const size_t small = 4096 + sizeof(pam_hydro_desc) * std::max<size_t>(1, nh) + 8 * nf + 64 * 8 +
+  const size_t small = 4096 + sizeof(pam_hydro_desc) * std::max<size_t>(1, nh) + 16 * nf + 256 + 64 * 8 +
                        3 * 4 * (PH_NBUCKET + 1) + 256;
   return (int64_t)(col_bytes(d) + (size_t)std::max<int64_t>(1, nitems) * item_bytes_per(d) +
                    64 * 256 + io + 64 * 256 + small);
@@ -504,8 +505,13 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     set_msg(res, "pam_run: NULL argument");
     return PAM_FATAL;
   }
-  if (!st->passive || st->active) {
-    set_msg(res, "only passive-only runs are supported (radar simulator not in this library)");
+  if (!st->passive) {
+    set_msg(res, "passive=False is not supported");
+    return PAM_FATAL;
+  }
+  if (st->active && (st->radar_npol < 1 || st->radar_npol > 5 || st->radar_attenuation < 0 ||
+                     st->radar_attenuation > 2)) {
+    set_msg(res, "invalid radar settings");
     return PAM_FATAL;
   }
   if (st->gas_mod != 0 || st->liq_mod != 0) {
@@ -580,6 +586,10 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     const char *inv = getenv("PAM_INVERSE");
     s.inverse_gj = !inv ? 0 : strcmp(inv, "gauss-jordan") == 0 ? 1 : strcmp(inv, "queue-all") == 0 ? 2 : 0;
   }
+  s.active = st->active ? 1 : 0;
+  s.radar_att = st->radar_attenuation;
+  s.radar_npol = st->active ? st->radar_npol : 0;
+  for (int i = 0; i < 5; i++) s.radar_pol[i] = st->radar_pol[i];
   s.pers_mie = (int)c->tune[0];
   s.pers_dbl = (int)c->tune[1];
   s.pers_add = (int)c->tune[2];
@@ -626,7 +636,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   }
 
   // small buffers: hydro table, frequencies, counters
-  size_t small = 4096 + sizeof(pam_hydro_desc) * std::max(1, d.nh) + 8 * (size_t)nfreq + 64 * 8 +
+  size_t small = 4096 + sizeof(pam_hydro_desc) * std::max(1, d.nh) + 16 * (size_t)nfreq + 256 + 64 * 8 +
                  3 * 4 * (PH_NBUCKET + 1) + 256;
   HIP_OK(c->smallbuf.ensure(small));
   Carver sc(c->smallbuf.p);
@@ -635,11 +645,14 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   unsigned long long *d_cnt = sc.take<unsigned long long>(PAM_NWORK);
   uint32_t *d_redo_n = sc.take<uint32_t>(2);
   uint32_t *d_queue = sc.take<uint32_t>(4);
+  double *d_k2 = sc.take<double>(nfreq);
   uint32_t *d_ph = sc.take<uint32_t>(3 * (PH_NBUCKET + 1));
   if (d.nh > 0)
     HIP_OK(hipMemcpyAsync(d_hyd, hydros, sizeof(pam_hydro_desc) * d.nh, hipMemcpyHostToDevice,
                           stream));
   HIP_OK(hipMemcpyAsync(d_freqs, freqs, 8 * (size_t)nfreq, hipMemcpyHostToDevice, stream));
+  if (st->active && st->radar_k2)
+    HIP_OK(hipMemcpyAsync(d_k2, st->radar_k2, 8 * (size_t)nfreq, hipMemcpyHostToDevice, stream));
   HIP_OK(hipMemsetAsync(d_cnt, 0, 8 * PAM_NWORK, stream));
 
   // per-column workspace
@@ -656,6 +669,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     w.rho_v = cv.take<double>(ncol * ML);
     w.q_hum = cv.take<double>(ncol * ML);
     w.dz = cv.take<double>(ncol * ML);
+    w.hgt = cv.take<double>(ncol * ML);
     w.groundtemp = cv.take<double>(ncol);
     w.nl = cv.take<int32_t>(ncol);
     w.outlev = cv.take<int32_t>(ncol * d.nout);
@@ -685,6 +699,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   w.counters = d_cnt;
   w.redo_n = d_redo_n;
   w.queue = d_queue;
+  w.radar_k2 = (st->active && st->radar_k2) ? d_k2 : nullptr;
 
   std::vector<hipEvent_t> evs;
   std::vector<int> ev_stage;
@@ -781,6 +796,9 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
         break;
       default:
         launch_adding(dc, s, d_freqs, pin->sfc_refl, wc, res->tb, stream);
+        if (s.active)
+          launch_radar(dc, s, d_freqs, wc, res->ze, res->att_hydro, res->att_atmo, res->radar_hgt,
+                       stream);
         if (res->kextatmo || res->kexttot || res->salbedo || res->nlegen || res->ndoubles) {
           size_t nd = (size_t)dc.ncol_chunk * d.ML * d.nf;
           hipLaunchKernelGGL(k_diag, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, stream, dc,
@@ -806,10 +824,11 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
       // the launch arguments the graphs bake in: shapes, settings,
       // workspace and the caller's device pointers
       std::vector<unsigned char> key(sizeof(Dims) + sizeof(Settings) + sizeof(Work) +
-                                     sizeof(Profile) + sizeof(Hydro4D) + 9 * sizeof(void *));
+                                     sizeof(Profile) + sizeof(Hydro4D) + 13 * sizeof(void *));
       unsigned char *kp = key.data();
       const void *ptrs[9] = {d_hyd, d_freqs, pin->sfc_refl, res->tb, res->kextatmo,
                              res->kexttot, res->salbedo, res->nlegen, res->ndoubles};
+      const void *rptrs[4] = {res->ze, res->att_hydro, res->att_atmo, res->radar_hgt};
       memcpy(kp, &dc, sizeof(Dims));
       kp += sizeof(Dims);
       memcpy(kp, &s, sizeof(Settings));
@@ -821,6 +840,8 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
       memcpy(kp, &h4, sizeof(Hydro4D));
       kp += sizeof(Hydro4D);
       memcpy(kp, ptrs, sizeof(ptrs));
+      kp += sizeof(ptrs);
+      memcpy(kp, rptrs, sizeof(rptrs));
       if (!(c->gexec[0] && c->gkey == key)) {
         drop_graph(c);
         bool ok = true;
@@ -969,6 +990,15 @@ extern "C" int pam_run(pam_context *c, const pam_settings *st, const pam_hydro_d
   oreg((void **)&dr.salbedo, ncol * ML * nf * 8);
   oreg((void **)&dr.nlegen, ncol * ML * nf * 4);
   oreg((void **)&dr.ndoubles, ncol * ML * nf * 4);
+  if (st && st->active) {
+    const size_t npol = (size_t)std::max(1, std::min(5, (int)st->radar_npol));
+    oreg((void **)&dr.ze, ncol * ML * nf * npol * 8);
+    oreg((void **)&dr.att_hydro, ncol * ML * nf * 8);
+    oreg((void **)&dr.att_atmo, ncol * ML * nf * 8);
+    oreg((void **)&dr.radar_hgt, ncol * ML * 8);
+  } else {
+    dr.ze = dr.att_hydro = dr.att_atmo = dr.radar_hgt = nullptr;
+  }
   size_t total = 0;
   for (auto &a : in) total += (a.bytes + 255) & ~(size_t)255;
   for (auto &o : out) total += (o.bytes + 255) & ~(size_t)255;

@@ -196,6 +196,7 @@ __global__ void k_fill_columns(Dims d, Profile p, Work w) {
     double rho_v = vp / (t[z] * R_V);
     double q = R_V / R_D * vp / (pr[z] - (1.0 - R_V / R_D) * vp);
     w.dz[oy + z] = hl[z + 1] - hl[z];
+    w.hgt[oy + z] = h[z];
     rl[z] = rho_v;  // w.rho_v
     h[z] = q;       // w.q_hum
     if (isnan(t[z]) || isnan(pr[z]) || isnan(rho_v) || isnan(q)) st = E_NAN_LAYER | PAM_FATAL;
@@ -470,6 +471,91 @@ void launch_surface(const Dims &d, const Settings &s, const Profile &p, const do
 
 void launch_items(const Dims &d, Work &w, hipStream_t st) {
   hipLaunchKernelGGL(k_items, dim3(nblk(d.ncol_chunk, 128)), dim3(128), 0, st, d, w);
+}
+
+// The radar simulator in its "simple" mode (radar_simulator.f90:94-170) for
+// one (column, channel) of the chunk, with the attenuation run_rt stores
+// (run_rt.f90:118-123): one-way gas and hydrometeor attenuation [dB] per
+// layer, and for each layer with hydrometeors (a scattering unit)
+// Ze = 10 log10(1e18 back lambda^4 / (|K|^2 pi^5) / PIA) [dBz], the
+// fill value -9999 when back = 0.  hydrometeor_extinction visits the layers
+// bottom-up, so a top-down PIA sees the hydrometeor attenuation above the
+// layer as not yet set (0); a bottom-up PIA sums the layers below.
+__global__ void k_radar(Dims d, Settings s, const double *freqs, Work w, double *ze,
+                        double *att_hydro, double *att_atmo, double *radar_hgt) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)d.ncol_chunk * d.nf) return;
+  const int f = (int)(idx % d.nf), col = d.col0 + (int)(idx / d.nf);
+  const int ML = d.ML, np = s.radar_npol;
+  const int L = min(max(w.nl[col], 0), ML);
+  const bool col_ok = (w.col_status[col] & 0xff) < PAM_FATAL;
+  const bool ok = col_ok && (w.cf_status[(size_t)col * d.nf + f] & 0xff) < PAM_FATAL;
+  const double wl = C_LIGHT / (freqs[f] * 1.e9);
+  const double k2 = w.radar_k2 ? w.radar_k2[f] : 0.93;
+  const double zconst = 1.e18 * (1.0 / (k2 * pow(PI, 5)));
+  const double wl4 = pow(wl, 4);
+  const size_t oy = (size_t)col * ML;
+  const double *kg = w.kgas + ((size_t)col * d.nf + f) * ML;
+  auto out = [&](int z) { return ((size_t)col * ML + z) * d.nf + f; };
+  // fill values above the column, and everything of a failed unit
+  for (int z = 0; z < ML; z++) {
+    const bool in = z < L;
+    att_atmo[out(z)] = !in ? 0.0 : (ok ? 10.0 * log10(exp(kg[z] * w.dz[oy + z])) : __builtin_nan(""));
+    att_hydro[out(z)] = !in ? 0.0 : (ok ? 0.0 : __builtin_nan(""));
+    for (int ip = 0; ip < np; ip++)
+      ze[out(z) * np + ip] = (!in || ok) ? -9999.0 : __builtin_nan("");
+    if (f == 0) radar_hgt[oy + z] = !in ? -9999.0 : (col_ok ? w.hgt[oy + z] : __builtin_nan(""));
+  }
+  if (!ok) return;
+  // hydrometeor attenuation of every scattering layer
+  for (int z = 0; z < L; z++) {
+    const int item = w.layer_item[oy + z];
+    if (item < 0) continue;
+    const size_t itf = (size_t)(item - d.item0) * d.nf + f;
+    att_hydro[out(z)] = 10.0 * log10(exp(w.kext[itf] * w.dz[oy + z]));
+  }
+  // reflectivities; the running sums follow the order of the reference's
+  // sums over the layers above (top-down) or below (bottom-up)
+  const bool td = s.radar_att == 1;
+  double sh = 0.0, sa = 0.0;
+  for (int i = 0; i < L; i++) {
+    const int z = td ? L - 1 - i : i;
+    const int item = w.layer_item[oy + z];
+    if (item >= 0) {
+      const size_t itf = (size_t)(item - d.item0) * d.nf + f;
+      const double ah = att_hydro[out(z)], aa = att_atmo[out(z)];
+      double pia = 0.0;
+      if (s.radar_att == 1) {
+        if (z < L - 1) pia = 2 * (0.0 + sa);
+        pia = pia + ah + aa;
+      } else if (s.radar_att == 2) {
+        if (z > 0) pia = 2 * (sh + sa);
+        pia = pia + ah + aa;
+      }
+      pia = pow(10.0, 0.1 * pia);
+      for (int ip = 0; ip < np; ip++) {
+        const double b = (s.radar_pol[ip] <= 2) ? w.back[itf] : 0.0;
+        double zb = zconst * b * wl4;
+        zb = zb / pia;
+        ze[out(z) * np + ip] = (zb == 0.0) ? -9999.0 : 10 * log10(zb);
+      }
+    }
+    if (td) {
+      sa += att_atmo[out(z)];
+    } else {
+      sh += att_hydro[out(z)];
+      sa += att_atmo[out(z)];
+    }
+  }
+}
+
+void launch_radar(const Dims &d, const Settings &s, const double *freqs, const Work &w,
+                  double *ze, double *att_hydro, double *att_atmo, double *radar_hgt,
+                  hipStream_t st) {
+  const size_t n = (size_t)d.ncol_chunk * d.nf;
+  if (n == 0 || !ze || !att_hydro || !att_atmo || !radar_hgt) return;
+  hipLaunchKernelGGL(k_radar, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, d, s, freqs, w,
+                     ze, att_hydro, att_atmo, radar_hgt);
 }
 
 }  // namespace pam

@@ -74,6 +74,9 @@ struct Settings {
   // takes units from a work queue (0: one workgroup per unit)
   int pers_mie, pers_dbl, pers_add;
   int ncu;  // compute units of the device
+  // radar simulator, "simple" mode (active): attenuation 0 disabled,
+  // 1 top-down, 2 bottom-up; polarisations 0 NN, 1 HH, 2 VV, 3 HV, 4 VH
+  int active, radar_att, radar_npol, radar_pol[5];
   double threshold, softsphere_min, emissivity;
 };
 
@@ -101,6 +104,7 @@ struct Work {
   double *hgt_lev, *temp_lev;          // [ncol][ML+1]
   double *temp, *press, *rho_v, *q_hum, *dz;  // [ncol][ML]
   double *groundtemp;                  // [ncol]
+  double *hgt;                         // [ncol][ML] layer heights
   int32_t *nl;                         // [ncol]
   int32_t *outlev;                     // [ncol][nout]
   int32_t *col_status;                 // [ncol]
@@ -118,6 +122,7 @@ struct Work {
   int32_t *dsd_flag;                   // [nitems][nh]
   // per (item, frequency) of the chunk
   double *kext, *salb;                 // [nitems*nf]
+  double *back;                        // [nitems*nf] co-polar backscattering [m2/m3]
   int32_t *nleg;                       // [nitems*nf]
   double *coef;                        // [nitems*nf][3][MAXNLEG]
   int32_t *ndbl;                       // [nitems*nf]
@@ -135,6 +140,7 @@ struct Work {
   uint32_t *ph_cnt, *ph_off, *ph_toff; // [PH_NBUCKET + 1]
   int32_t *ph_rank, *ph_perm;          // [nitems*nf]
   const double *mu, *qw;               // Lobatto nodes/weights [NMU]
+  const double *radar_k2;              // [nf] |K|^2 of the radar equation
   // units whose inverse series did not converge, for the Gauss-Jordan pass
   uint32_t *redo_n;                    // [2]: doubling, adding (reset per chunk)
   int32_t *redo_dbl;                   // [nitems*nf] of the chunk
@@ -257,6 +263,11 @@ void launch_doubling(const Dims &d, const Settings &s, const double *freqs, Work
                      hipStream_t st);
 void launch_adding(const Dims &d, const Settings &s, const double *freqs, const char *sfc_refl,
                    Work &w, double *tb, hipStream_t st);
+// radar "simple" mode of the chunk's columns (after the adding, which
+// settles each (column, channel)'s status)
+void launch_radar(const Dims &d, const Settings &s, const double *freqs, const Work &w,
+                  double *ze, double *att_hydro, double *att_atmo, double *radar_hgt,
+                  hipStream_t st);
 // n 32-bit words at p set to v (a kernel, so that it can sit in a captured
 // graph)
 void launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t st);

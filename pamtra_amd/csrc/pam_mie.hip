@@ -292,7 +292,7 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
     s_hm[t][1] = m.im;
   }
   __syncthreads();
-  double kexttot = 0.0, salbedo = 0.0;
+  double kexttot = 0.0, salbedo = 0.0, backtot = 0.0;
   int nleg_coef = 0;
   long long work_bins = 0, work_series = 0;
 
@@ -728,6 +728,7 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
     const int nl_h = s_nlh;
     if (nl_h > nleg_coef) nleg_coef = nl_h;
     kexttot = kexttot + extinction;
+    backtot = backtot + sumqb;  // rt_back (scatProperties.f90:743-757, 922)
     salbedo = (kexttot == 0.0) ? 0.0 : salbedo + (albedo * extinction);
     if (t < nleg_coef && t <= nlegen) {
       s_lc[0][t] = s_lc[0][t] + (g1 * albedo * extinction);
@@ -755,6 +756,7 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
   }
   if (t == 0) {
     w.kext[itf] = kexttot;
+    w.back[itf] = backtot;
     w.salb[itf] = salbedo;
     w.nleg[itf] = nleg_coef;
     atomicAdd(&w.counters[6], (unsigned long long)work_bins);

@@ -19,6 +19,9 @@ import numpy as np
 from . import _capi
 from ._lib import get_context, library
 
+# radar outputs of the "simple" mode (results key -> pam_results array)
+RADAR_KEYS = {"Ze": "ze", "Att_hydro": "att_hydro", "Att_atmo": "att_atmo",
+              "radar_hgt": "radar_hgt"}
 RESULT_KEYS = ["tb", "Ze", "emissivity", "Att_hydro", "Att_atmo", "radar_hgt", "radar_moments",
                "radar_edges", "radar_slopes", "radar_quality", "radar_snr", "radar_spectra",
                "radar_vel", "psd_d", "psd_deltad", "psd_n", "psd_mass", "psd_area", "psd_bscat",
@@ -44,10 +47,6 @@ def PamtraFortranWrapper(sets, nmlSets, descriptorFile, descriptorFile4D, descri
     if nmlSets.get("hydro_fullspec", False) or descriptorFileFS:
         raise NotImplementedError("hydro_fullspec is not supported")
     nml = dict(nmlSets)
-    if nml.get("active", False):
-        warnings.warn("the radar simulator is not part of this library: running passive only, "
-                      "active outputs are filled with -9999", RuntimeWarning)
-        nml["active"] = False
     if not nml.get("passive", True):
         raise NotImplementedError("passive=False leaves nothing for this library to compute")
     lib = library()
@@ -67,6 +66,8 @@ def PamtraFortranWrapper(sets, nmlSets, descriptorFile, descriptorFile4D, descri
             results[key] = copy.deepcopy(hr.arrays[key])
         elif key == "kextatmo" and "kextatmo" in hr.arrays:
             results[key] = copy.deepcopy(hr.arrays[key])
+        elif key in RADAR_KEYS and RADAR_KEYS[key] in hr.arrays:
+            results[key] = copy.deepcopy(hr.arrays[RADAR_KEYS[key]])
         elif key == "radar_quality":
             results[key] = -9999
         else:
