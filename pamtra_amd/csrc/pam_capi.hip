@@ -209,7 +209,7 @@ static void lobatto(int num, double *mu, double *w) {
 
 static const char *err_text(int32_t code) {
   switch (code & ~0xff) {
-    case E_NLYRS: return "nlyrs out of range or unsupported output level";
+    case E_NLYRS: return "nlyrs out of range";
     case E_NAN_LEVEL: return "found nan in level variables";
     case E_NAN_LAYER: return "found nan in layer variables";
     case E_HGT_ORDER: return "atmo_hgt_lev must be defined bottom-up";
@@ -435,6 +435,7 @@ static size_t col_bytes(const Dims &d) {
   add(2 * ncol * nf * 2 * NMU * 8);
   add(ncol * nf * 4);
   add(ncol * nf * 4);
+  add(ncol * nf * 4);
   add(8 * 8);
   for (int i = 0; i < 16; i++) add(0);
   return n + 256 * 32;
@@ -643,7 +644,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   pam_hydro_desc *d_hyd = sc.take<pam_hydro_desc>(std::max(1, d.nh));
   double *d_freqs = sc.take<double>(nfreq);
   unsigned long long *d_cnt = sc.take<unsigned long long>(PAM_NWORK);
-  uint32_t *d_redo_n = sc.take<uint32_t>(2);
+  uint32_t *d_redo_n = sc.take<uint32_t>(3);
   uint32_t *d_queue = sc.take<uint32_t>(4);
   double *d_k2 = sc.take<double>(nfreq);
   uint32_t *d_ph = sc.take<uint32_t>(3 * (PH_NBUCKET + 1));
@@ -683,6 +684,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     w.sfc_refl = cv.take<double>(ncol * nf * 2 * NMU);
     w.cf_status = cv.take<int32_t>(ncol * nf);
     w.redo_add = cv.take<int32_t>(ncol * nf);
+    w.redo_lev = cv.take<int32_t>(ncol * nf);
     HIP_OK(hipMemsetAsync(w.col_status, 0, ncol * 4, stream));
     HIP_OK(hipMemsetAsync(w.cf_status, 0, ncol * nf * 4, stream));
   }
@@ -779,7 +781,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     switch (seg) {
       case 1:
         launch_fill_u32(wc.queue, 4, 0, stream);
-        launch_fill_u32(wc.redo_n, 2, 0, stream);
+        launch_fill_u32(wc.redo_n, 3, 0, stream);
         if (dc.nitems > 0)
           launch_fill_u32((uint32_t *)wc.itf_status, (size_t)dc.nitems * d.nf, 0, stream);
         launch_items(dc, wc, stream);

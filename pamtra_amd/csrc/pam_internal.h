@@ -142,9 +142,10 @@ struct Work {
   const double *mu, *qw;               // Lobatto nodes/weights [NMU]
   const double *radar_k2;              // [nf] |K|^2 of the radar equation
   // units whose inverse series did not converge, for the Gauss-Jordan pass
-  uint32_t *redo_n;                    // [2]: doubling, adding (reset per chunk)
+  uint32_t *redo_n;                    // [3]: doubling, adding, levels (reset per chunk)
   int32_t *redo_dbl;                   // [nitems*nf] of the chunk
   int32_t *redo_add;                   // [ncol*nf]
+  int32_t *redo_lev;                   // [ncol*nf]: units with an interior output level
   // work counters (atomics)
   unsigned long long *counters;        // [12], see pam_last_work
   // work-queue counters of the persistent grids: mie, doubling, adding

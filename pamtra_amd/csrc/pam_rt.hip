@@ -1279,6 +1279,47 @@ __device__ void write_tb(const Dims &d, const Settings &s, int col, int f, int i
   }
 }
 
+// The surface's reflection matrix R- (get_sfc_matrices, sfc_matrices.f90:88-194):
+// specular/Fresnel 2x2 blocks on the diagonal, or the rank-one Lambertian
+// 2 r mu_j w_j on the I component
+__device__ void surface_matrix(char refl, const double *Rf, const double *smu, const double *sqw,
+                               double *Rs) {
+  for (int e = threadIdx.x; e < NN2; e += RT_WG) {
+    const int r = e / NN, c = e % NN;
+    const int i2 = r % NST, j2 = r / NST, i1 = c % NST, j1 = c / NST;
+    double v = 0.0;
+    if (refl == 'L') {
+      if (i2 == 0 && i1 == 0) v = 2.0 * Rf[0] * smu[j1] * sqw[j1];
+    } else if (j1 == j2) {
+      const double rv = Rf[j2], rh = Rf[NMU + j2];
+      const double r1 = (rv + rh) / 2.0, r2 = (rv - rh) / 2.0;
+      v = (i1 == i2) ? r1 : r2;
+    }
+    Rs[r * LD + c] = v;
+  }
+  __syncthreads();
+}
+
+// The radiances entering the column: sky (blackbody at SKY_TEMP, I only)
+// at the top and the surface's emission at the bottom (thermal_radiance,
+// surface.f90:169-195; get_sfc_matrices for the specular/Lambertian emission)
+__device__ void boundary_radiances(char refl, const double *E, const double *Rf, double psky,
+                                   double pg, double *sky, double *gnd) {
+  const int t = threadIdx.x;
+  if (t < NN) {
+    const int i = t % NST, j = t / NST;
+    sky[t] = (i == 0) ? (1.0 - 0.0) * psky : 0.0;
+    if (refl == 'L') {
+      gnd[t] = (i == 0) ? E[0] * pg : 0.0;
+    } else {
+      const double rv = Rf[j], rh = Rf[NMU + j];
+      const double r1 = (rv + rh) / 2.0, r2 = (rv - rh) / 2.0;
+      gnd[t] = (i == 0) ? (1.0 - r1) * pg : -r2 * pg;
+    }
+  }
+  __syncthreads();
+}
+
 struct AddLds {
   double M[8][MSZ];
   double v[10][NN];
@@ -1337,18 +1378,24 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
     return true;
   }
   int lev_toa = -1, lev_sfc = -1;
+  bool general = false;
   for (int io = 0; io < nout; io++) {
-    int o = w.outlev[(size_t)col * nout + io];
-    o = max(1, min(o, L + 2));
-    if (o == 1)
+    const int o = max(1, min(w.outlev[(size_t)col * nout + io], L + 1));
+    if (o == 1 && lev_toa < 0)
       lev_toa = io;
-    else if (o == L + 1)
+    else if (o == L + 1 && lev_sfc < 0)
       lev_sfc = io;
-    else {
-      // a level inside the column needs a second stack: not supported
-      fail(E_NLYRS | PAM_FATAL);
-      return true;
+    else
+      general = true;
+  }
+  if (general) {
+    // a level inside the column (or a level asked for twice) needs the
+    // stacks above and below it: the unit goes to k_adding_levels
+    if (t == 0) {
+      const unsigned q = atomicAdd(&w.redo_n[2], 1u);
+      w.redo_lev[q] = cf;
     }
+    return true;
   }
   Stack stk = {M[0], M[1], M[2], M[3], sv[0], sv[1]};
   double *R2 = M[4], *T2 = M[5], *G = M[6], *X = M[7];
@@ -1358,18 +1405,7 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
   const double pg = planck(w.groundtemp[col], wl);
   const double *E = w.sfc_emis + ((size_t)col * d.nf + f) * 2 * NMU;
   const double *Rf = w.sfc_refl + ((size_t)col * d.nf + f) * 2 * NMU;
-  if (t < NN) {
-    const int i = t % NST, j = t / NST;
-    sky[t] = (i == 0) ? (1.0 - 0.0) * psky : 0.0;
-    if (refl == 'L') {
-      gnd[t] = (i == 0) ? E[0] * pg : 0.0;
-    } else {
-      const double rv = Rf[j], rh = Rf[NMU + j];
-      const double r1 = (rv + rh) / 2.0, r2 = (rv - rh) / 2.0;
-      gnd[t] = (i == 0) ? (1.0 - r1) * pg : -r2 * pg;
-    }
-  }
-  __syncthreads();
+  boundary_radiances(refl, E, Rf, psky, pg, sky, gnd);
   // fold the layers top down (radtran4.f90:386-417 with prefix reuse: the
   // stack above the surface level is the stack the top level continues)
   unsigned long long nscat = 0, nclear = 0;
@@ -1489,21 +1525,7 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
     fail(sbad[0]);
     return true;
   }
-  // surface reflection matrix into R2 (specular/Fresnel or Lambertian)
-  for (int e = t; e < NN2; e += RT_WG) {
-    const int r = e / NN, c = e % NN;
-    const int i2 = r % NST, j2 = r / NST, i1 = c % NST, j1 = c / NST;
-    double v = 0.0;
-    if (refl == 'L') {
-      if (i2 == 0 && i1 == 0) v = 2.0 * Rf[0] * smu[j1] * sqw[j1];
-    } else if (j1 == j2) {
-      const double rv = Rf[j2], rh = Rf[NMU + j2];
-      const double r1 = (rv + rh) / 2.0, r2 = (rv - rh) / 2.0;
-      v = (i1 == i2) ? r1 : r2;
-    }
-    R2[r * LD + c] = v;
-  }
-  __syncthreads();
+  surface_matrix(refl, Rf, smu, sqw, R2);
   if (lev_sfc >= 0) {
     if (!radiance_at_surface<GJ>(stk, R2, sky, gnd, G, X, vs, ur, dr, sinv, nprod)) {
       if (!GJ) return false;
@@ -1563,6 +1585,210 @@ __global__ void __launch_bounds__(RT_WG) k_adding_gj(Dims d, Settings s, const d
   }
 }
 
+// INTERNAL_RADIANCE4 (radintg4.f90:119-172) between the stack above a level
+// (its R+, T+, S+: URp, UTp, USp) and the stack below it (dn: R-, T-, S-),
+// in the reference's order of the vector sums
+__device__ bool radiance_internal(const double *URp, const double *UTp, const double *USp,
+                                  const Stack &dn, const double *sky, const double *gnd,
+                                  double *G, double *X, double *vs, double *ur, double *dr,
+                                  InvLds &sinv, int &np) {
+  const int t = threadIdx.x;
+  if (!inv_i_minus_ab<true>(URp, dn.Rm, X, G, sinv, np)) return false;  // gamma+
+  if (t < NN) vs[t] = mv_row(dn.Tm, gnd, t);                            // DNT INBOT
+  __syncthreads();
+  double s_ = 0.0;
+  if (t < NN) {
+    s_ = mv_row(URp, vs, t);
+    s_ = mv_row(UTp, sky, t) + s_;
+    s_ = mv_row(URp, dn.Sm, t) + s_;
+    s_ = USp[t] + s_;
+  }
+  __syncthreads();
+  if (t < NN) vs[t] = s_;
+  __syncthreads();
+  if (t < NN) dr[t] = mv_row(G, vs, t);
+  __syncthreads();
+  if (!inv_i_minus_ab<true>(dn.Rm, URp, X, G, sinv, np)) return false;  // gamma-
+  if (t < NN) vs[t] = mv_row(UTp, sky, t);
+  __syncthreads();
+  if (t < NN) {
+    s_ = mv_row(dn.Rm, vs, t);
+    s_ = mv_row(dn.Tm, gnd, t) + s_;
+    s_ = mv_row(dn.Rm, USp, t) + s_;
+    s_ = dn.Sm[t] + s_;
+  }
+  __syncthreads();
+  if (t < NN) vs[t] = s_;
+  __syncthreads();
+  if (t < NN) ur[t] = mv_row(G, vs, t);
+  __syncthreads();
+  return true;
+}
+
+// Folds the layers lo..hi (rt4's numbering, 1 = top) into stk, top down, the
+// first one copied (radtran4.f90:386-417).  FP64 products; inverses by the
+// series with the Gauss-Jordan fallback.  False: a singular matrix.
+__device__ bool fold_layers(const Dims &d, const Work &w, int col, int f, int L, int lo, int hi,
+                            double wl, const double *smu, Stack &stk, double *R2, double *T2,
+                            double *dg, double *S2p, double *S2m, double *G, double *X, double *vs,
+                            InvLds &sinv, int *sbad, int &np, unsigned long long &nscat,
+                            unsigned long long &nclear) {
+  const int t = threadIdx.x;
+  LayerFetch F;
+  fetch_layer(d, w, col, f, L - lo, F);
+  for (int l = lo; l <= hi; l++) {
+    const bool sc = stage_layer(F, wl, smu, R2, T2, dg, S2p, S2m, sbad);
+    if (l < hi) fetch_layer(d, w, col, f, L - l - 1, F);
+    if (l == lo) {
+      for (int e = t; e < NN2; e += RT_WG) {
+        const int i = e / NN, j = e % NN;
+        const double r = sc ? R2[i * LD + j] : 0.0;
+        const double tt = sc ? T2[i * LD + j] : (i == j ? dg[i] : 0.0);
+        stk.Rp[i * LD + j] = r;
+        stk.Rm[i * LD + j] = r;
+        stk.Tp[i * LD + j] = tt;
+        stk.Tm[i * LD + j] = tt;
+      }
+      if (t < NN) {
+        stk.Sp[t] = S2p[t];
+        stk.Sm[t] = S2m[t];
+      }
+      __syncthreads();
+    } else if (sc) {
+      if (!combine_scat<true, false>(stk, R2, T2, S2p, S2m, G, X, vs, sinv, np)) return false;
+      nscat++;
+    } else {
+      combine_clear(stk, dg, S2p, S2m);
+      nclear++;
+    }
+  }
+  return true;
+}
+
+struct LevLds {
+  double M[10][MSZ];
+  double v[11][NN];
+  InvLds inv;
+  double mu[NMU], qw[NMU];
+  int bad[1];
+};
+
+// One (column, channel) unit with an output level inside the column (or a
+// level asked for twice).  As radtran4.f90:375-422 does for every level, each
+// output level gets the stack of the layers above it and the stack of the
+// layers below it with the surface, joined by INTERNAL_RADIANCE4.  FP64 in
+// both precisions.  Output levels are 1..L+1 (run_rt.f90:166-184).
+__device__ void levels_unit(const Dims &d, const Settings &s, const double *freqs,
+                            const char *sfc_refl, const Work &w, double *tb, int cf, LevLds &LL) {
+  double(*M)[MSZ] = LL.M;
+  double(*sv)[NN] = LL.v;
+  const int t = threadIdx.x;
+  const int cl = cf / d.nf, f = cf % d.nf;
+  const int col = d.col0 + cl;
+  if (t < NMU) {
+    LL.mu[t] = w.mu[t];
+    LL.qw[t] = w.qw[t];
+  }
+  if (t == 0) {
+    LL.bad[0] = 0;
+    LL.inv.gj_only = s.inverse_gj == 1;
+  }
+  int32_t *cst = w.cf_status + (size_t)col * d.nf + f;
+  const int L = w.nl[col];
+  const int nout = d.nout;
+  const double wl = C_LIGHT / (freqs[f] * 1.e3);
+  const char refl = sfc_refl ? sfc_refl[col] : 'S';
+  __syncthreads();
+  auto fail = [&](int32_t code) {
+    if (t == 0 && code) atomicMax(cst, code);
+    for (int e = t; e < nout * PAM_NANGLES * 2; e += RT_WG) {
+      const int io = e / (PAM_NANGLES * 2), a = (e / 2) % PAM_NANGLES, k = e % 2;
+      tb[((((size_t)col * nout + io) * PAM_NANGLES + a) * d.nf + f) * 2 + k] =
+          __builtin_nan("");
+    }
+  };
+  Stack stk = {M[0], M[1], M[2], M[3], sv[0], sv[1]};
+  double *URp = M[4], *UTp = M[5], *R2 = M[6], *T2 = M[7], *G = M[8], *X = M[9];
+  double *USp = sv[2], *S2p = sv[3], *S2m = sv[4], *dg = sv[5], *vs = sv[6];
+  double *sky = sv[7], *gnd = sv[8], *ur = sv[9], *dr = sv[10];
+  const double *E = w.sfc_emis + ((size_t)col * d.nf + f) * 2 * NMU;
+  const double *Rf = w.sfc_refl + ((size_t)col * d.nf + f) * 2 * NMU;
+  boundary_radiances(refl, E, Rf, planck(SKY_TEMP, wl), planck(w.groundtemp[col], wl), sky, gnd);
+  unsigned long long nscat = 0, nclear = 0;
+  int nprod = 0;
+  for (int io = 0; io < nout; io++) {
+    const int o = max(1, min(w.outlev[(size_t)col * nout + io], L + 1));
+    // the layers above the level: none at the top (R = 0, T = I, S = 0)
+    if (o > 1) {
+      if (!fold_layers(d, w, col, f, L, 1, o - 1, wl, LL.mu, stk, R2, T2, dg, S2p, S2m, G, X, vs,
+                       LL.inv, LL.bad, nprod, nscat, nclear)) {
+        fail(E_SINGULAR | PAM_FATAL);
+        return;
+      }
+    }
+    for (int e = t; e < NN2; e += RT_WG) {
+      const int i = e / NN, j = e % NN;
+      URp[i * LD + j] = o > 1 ? stk.Rp[i * LD + j] : 0.0;
+      UTp[i * LD + j] = o > 1 ? stk.Tp[i * LD + j] : (i == j ? 1.0 : 0.0);
+    }
+    if (t < NN) USp[t] = o > 1 ? stk.Sp[t] : 0.0;
+    __syncthreads();
+    // the layers below the level, then the surface (R+ = 0, R- = Rs, T = I,
+    // S = 0); at the surface level the surface alone
+    if (o <= L) {
+      if (!fold_layers(d, w, col, f, L, o, L, wl, LL.mu, stk, R2, T2, dg, S2p, S2m, G, X, vs,
+                       LL.inv, LL.bad, nprod, nscat, nclear)) {
+        fail(E_SINGULAR | PAM_FATAL);
+        return;
+      }
+      surface_matrix(refl, Rf, LL.mu, LL.qw, R2);
+      if (!combine_surface<true>(stk, R2, G, X, vs, LL.inv, nprod)) {
+        fail(E_SINGULAR | PAM_FATAL);
+        return;
+      }
+    } else {
+      surface_matrix(refl, Rf, LL.mu, LL.qw, stk.Rm);
+      for (int e = t; e < NN2; e += RT_WG) {
+        const int i = e / NN, j = e % NN;
+        stk.Rp[i * LD + j] = 0.0;
+        stk.Tp[i * LD + j] = i == j ? 1.0 : 0.0;
+        stk.Tm[i * LD + j] = i == j ? 1.0 : 0.0;
+      }
+      if (t < NN) {
+        stk.Sp[t] = 0.0;
+        stk.Sm[t] = 0.0;
+      }
+      __syncthreads();
+    }
+    if (LL.bad[0]) {
+      fail(LL.bad[0]);
+      return;
+    }
+    if (!radiance_internal(URp, UTp, USp, stk, sky, gnd, G, X, vs, ur, dr, LL.inv, nprod)) {
+      fail(E_SINGULAR | PAM_FATAL);
+      return;
+    }
+    write_tb(d, s, col, f, io, wl, ur, dr, tb);
+    __syncthreads();
+  }
+  if (t == 0) {
+    atomicAdd(&w.counters[4], nscat);
+    atomicAdd(&w.counters[5], nclear);
+    atomicAdd(&w.counters[10], (unsigned long long)nprod);
+  }
+}
+
+// The units queued by k_adding for output levels inside the column
+__global__ void __launch_bounds__(RT_WG) k_adding_levels(Dims d, Settings s, const double *freqs,
+                                                         const char *sfc_refl, Work w, double *tb) {
+  __shared__ LevLds LL;
+  const int n = (int)w.redo_n[2];
+  for (int q = blockIdx.x; q < n; q += gridDim.x) {
+    __syncthreads();
+    levels_unit(d, s, freqs, sfc_refl, w, tb, w.redo_lev[q], LL);
+  }
+}
+
 
 void launch_doubling(const Dims &d, const Settings &s, const double *freqs, Work &w,
                      hipStream_t st) {
@@ -1597,20 +1823,27 @@ void launch_adding(const Dims &d, const Settings &s, const double *freqs, const 
   if (s.inverse_gj == 1) {
     hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, 16384)), dim3(RT_WG), 0, st,
                        d, s, freqs, sfc_refl, w, tb, 1);
-    return;
+  } else {
+    const dim3 grid((unsigned)persistent_grid(n, s.pers_add, s.ncu));
+    const bool pers = grid.x < n, mx = s.precision == PAM_PREC_MIXED;
+    if (pers && mx)
+      hipLaunchKernelGGL((k_adding<true, true>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
+                         tb);
+    else if (pers)
+      hipLaunchKernelGGL((k_adding<true, false>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
+                         tb);
+    else if (mx)
+      hipLaunchKernelGGL((k_adding<false, true>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
+                         tb);
+    else
+      hipLaunchKernelGGL((k_adding<false, false>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
+                         tb);
+    hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, 2048)), dim3(RT_WG), 0, st, d,
+                       s, freqs, sfc_refl, w, tb, 0);
   }
-  const dim3 grid((unsigned)persistent_grid(n, s.pers_add, s.ncu));
-  const bool pers = grid.x < n, mx = s.precision == PAM_PREC_MIXED;
-  if (pers && mx)
-    hipLaunchKernelGGL((k_adding<true, true>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
-  else if (pers)
-    hipLaunchKernelGGL((k_adding<true, false>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
-  else if (mx)
-    hipLaunchKernelGGL((k_adding<false, true>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
-  else
-    hipLaunchKernelGGL((k_adding<false, false>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
-  hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, 2048)), dim3(RT_WG), 0, st, d,
-                     s, freqs, sfc_refl, w, tb, 0);
+  // units with output levels inside the column (none with the default levels)
+  hipLaunchKernelGGL(k_adding_levels, dim3((unsigned)std::min<size_t>(n, 1024)), dim3(RT_WG), 0, st,
+                     d, s, freqs, sfc_refl, w, tb);
 }
 
 __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {

@@ -228,3 +228,57 @@ def test_large_particles_high_frequency():
     print("max nlegen", int(np.max(c["nlegen"])), flush=True)
     assert np.max(c["nlegen"]) > 80
     _compare(g, c)
+
+
+def _levels(pam, heights):
+    nx, ny = pam.p["ngridx"], pam.p["ngridy"]
+    pam.p["noutlevels"] = len(heights[0])
+    pam.p["obs_height"] = np.asarray(heights, dtype=float).reshape(nx, ny, len(heights[0]))
+    return pam
+
+
+@pytest.mark.parametrize("mode", [None, "gauss-jordan", "queue-all"])
+def test_interior_output_levels_vs_oracle(monkeypatch, mode):
+    """Output levels inside the column (radtran4.f90:375-422): those units
+    go through k_adding_levels, which folds the stacks above and below each
+    level.  Device vs oracle, and the top and surface levels of those units
+    against the standard path's."""
+    pam, freqs = workloads.config2(ncol=20, seed=91)
+    hl = pam.p["hgt_lev"][:, 0, :]
+    nl = pam.p["nlyrs"][:, 0]
+    assert pam.p["obs_height"].shape[-1] == 2
+    ref, st0 = _gpu(pam, freqs)
+    assert st0 == 0
+    heights = [[833000.0, 0.0, hl[c, nl[c] // 2], 0.5 * (hl[c, 2] + hl[c, 3]) + 1.0, 1500.0]
+               for c in range(20)]
+    pam = _levels(pam, heights)
+    if mode:
+        monkeypatch.setenv("PAM_INVERSE", mode)
+    g, st = _gpu(pam, freqs)
+    c, st2 = _cpu(pam, freqs)
+    assert st == 0 and st2 == 0, (g["msg"], c["msg"])
+    _compare(g, c)
+    d = np.max(np.abs(g["tb"][..., :2, :, :, :] - ref["tb"]))
+    print(f"top/surface levels through the level pass vs the standard path: {d:.2e} K", flush=True)
+    assert d <= 1e-7
+
+
+def test_level_asked_twice_and_mixed_precision():
+    """Two observation heights above the column (both the top level) and a
+    mixed-precision run with an interior level."""
+    pam, freqs = workloads.config2(ncol=6, seed=92)
+    ref, _ = _gpu(pam, freqs)
+    pam = _levels(pam, [[833000.0, 0.0, 900000.0]] * 6)
+    g, st = _gpu(pam, freqs)
+    assert st == 0
+    np.testing.assert_allclose(g["tb"][..., 2, :, :, :], ref["tb"][..., 0, :, :, :], rtol=0,
+                               atol=1e-7)
+    np.testing.assert_allclose(g["tb"][..., :2, :, :, :], ref["tb"], rtol=0, atol=1e-7)
+    hl = pam.p["hgt_lev"][:, 0, :]
+    pam = _levels(pam, [[833000.0, 0.0, hl[c, 7]] for c in range(6)])
+    a, _ = _gpu(pam, freqs)
+    pam.set["precision"] = "mixed"
+    b, st3 = _gpu(pam, freqs)
+    pam.set["precision"] = "fp64"
+    assert st3 == 0
+    assert np.max(np.abs(a["tb"] - b["tb"])) <= 0.05

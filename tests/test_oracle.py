@@ -356,3 +356,100 @@ def test_radar_attenuation_modes_and_polarisations():
         np.testing.assert_allclose(aa[ix, 0, :L], 10 * np.log10(np.exp(kg[ix, 0, :L] * dz[ix, 0, :L, None])),
                                    rtol=1e-12, atol=1e-14)
         assert np.all(aa[ix, 0, L:] == 0.0) and np.all(base["radar_hgt"][ix, 0, L:] == -9999.0)
+
+
+def _planck(t, wl):
+    """Planck radiance [W m-2 sr-1 um-1] at wavelength wl [um] with the
+    constants of rt_utilities.f90:69-90."""
+    return 1.1911e8 / wl ** 5 / (np.exp(1.4388e4 / (wl * t)) - 1.0)
+
+
+def _crossing(i0, taus, b_in, b_out, mu):
+    """Radiance after crossing clear layers in order, each with the Planck
+    radiance linear in optical depth from b_in (entry side) to b_out: the
+    solution of mu dI/dtau = B - I by adaptive quadrature."""
+    from scipy.integrate import quad
+    i = i0
+    for tau, b0, b1 in zip(taus, b_in, b_out):
+        if tau <= 0.0:
+            continue
+        src = quad(lambda x: (b0 + (b1 - b0) * x / tau) * np.exp(-(tau - x) / mu) / mu, 0.0, tau,
+                   epsabs=0.0, epsrel=1e-13, limit=200)[0]
+        i = i * np.exp(-tau / mu) + src
+    return i
+
+
+def _with_levels(pam, heights):
+    nx, ny = pam.p["ngridx"], pam.p["ngridy"]
+    pam.p["noutlevels"] = len(heights[0])
+    pam.p["obs_height"] = np.asarray(heights, dtype=float).reshape(nx, ny, len(heights[0]))
+    return pam
+
+
+def test_interior_output_levels_clear_column_closed_form():
+    """Output levels inside the column (radtran4.f90:375-422, run_rt.f90:166-184):
+    a clear, gas-absorbing column over a specular surface of emissivity 0.7.
+    Up- and downwelling TB at every level against the radiative transfer
+    equation integrated by quadrature, with the level nearest to each
+    observation height (ties to the upper level)."""
+    pam, freqs = workloads.config1(ncol=2, seed=81)
+    pam.p["hydro_q"][:] = 0.0
+    pam.p["sfc_type"][:] = -9999
+    pam.p["sfc_refl"][:] = "S"
+    pam.nmlSet["emissivity"] = 0.7
+    hl = pam.p["hgt_lev"][:, 0, :]
+    dzl = hl[:, 1:] - hl[:, :-1]
+    heights, expect_lev = [], []
+    for c in range(2):
+        h = hl[c]
+        obs = [833000.0, 0.0, h[10], h[5] + 0.3 * dzl[c, 5], h[20] + 0.8 * dzl[c, 20],
+               h[15] + 0.5 * dzl[c, 15], 1e9, -5.0]
+        heights.append(obs)
+        expect_lev.append([30, 0, 10, 5, 21, 16, 30, 0])
+    pam = _with_levels(pam, heights)
+    out, st, _ = oracle.run(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p, freqs,
+                            nthreads=2, diagnostics=True)
+    assert st == 0
+    mu, _ = oracle.lobatto()
+    wl = 299792458.0 / (freqs[0] * 1e3)
+    for c in range(2):
+        t_lev = pam.p["temp_lev"][c, 0]
+        L = int(pam.p["nlyrs"][c, 0])
+        tau = dzl[c, :L] * np.maximum(out["kextatmo"][c, 0, :L, 0], 0.0)
+        assert abs(_planck(t_lev[3], wl) - oracle.planck(t_lev[3], wl)) < 1e-13 * _planck(t_lev[3], wl)
+        bl = _planck(t_lev[:L + 1], wl)
+        bg = _planck(pam.p["groundtemp"][c, 0], wl)
+        for j in range(16):
+            m = mu[j]
+            down = [_planck(2.73, wl)]  # downwelling at level L, L-1, ..., 0
+            for z in range(L - 1, -1, -1):
+                down.append(_crossing(down[-1], [tau[z]], [bl[z + 1]], [bl[z]], m))
+            down = down[::-1]  # index = level
+            up = [0.7 * bg + 0.3 * down[0]]
+            for z in range(L):
+                up.append(_crossing(up[-1], [tau[z]], [bl[z]], [bl[z + 1]], m))
+            for io, k in enumerate(expect_lev[c]):
+                tb_up = oracle.rad2tb(up[k], wl)
+                tb_dn = oracle.rad2tb(down[k], wl)
+                got = out["tb"][c, 0, io, :, 0, :]
+                np.testing.assert_allclose(got[15 - j], tb_up, atol=2e-7, rtol=0)
+                np.testing.assert_allclose(got[16 + j], tb_dn, atol=2e-7, rtol=0)
+
+
+def test_interior_level_above_transparent_layers_sees_the_surface_level():
+    """With no gas extinction the layers below a level are the identity, so
+    with hydrometeors only above it the level's radiances equal the surface
+    level's: the joint of the two stacks (both gammas non-trivial) at the
+    interior level against the surface-alone joint."""
+    pam, freqs = workloads.config1(ncol=3, seed=82)
+    hl = pam.p["hgt_lev"][:, 0, :]
+    pam.p["hydro_q"][:, :, :12, :] = 0.0
+    pam.p["hydro_q"][:, :, 12:18, 0] = 3e-4
+    pam.nmlSet["lgas_extinction"] = False
+    pam = _with_levels(pam, [[833000.0, 0.0, hl[c, 12], hl[c, 6]] for c in range(3)])
+    out, st = _run(pam, freqs, nthreads=3)
+    assert st == 0
+    tb = out["tb"][:, 0]
+    np.testing.assert_allclose(tb[:, 2], tb[:, 1], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(tb[:, 3], tb[:, 1], rtol=0, atol=1e-9)
+    assert np.all(np.abs(tb[:, 0, :16] - tb[:, 1, :16]) > 1e-3)  # scattering above the level
