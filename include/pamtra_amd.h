@@ -23,8 +23,14 @@
 extern "C" {
 #endif
 
-/* Quadrature and polarisation of the RT4 solver (src/settings.f90:12-31). */
+/* Quadrature and polarisation of the RT4 solver (src/settings.f90:12-31).
+ * The reference fixes nummu = 16 at compile time (settings.f90:16) and so
+ * does libpamtra_amd.so; libpamtra_amd_n64.so is the same source built with
+ * PAM_NUMMU 32 (32 angles per hemisphere, 64x64 operators), and a caller of
+ * that library compiles this header with -DPAM_NUMMU=32. */
+#ifndef PAM_NUMMU
 #define PAM_NUMMU 16
+#endif
 #define PAM_NSTOKES 2
 #define PAM_NANGLES (2 * PAM_NUMMU)
 #define PAM_MAXNLEG 200
@@ -180,6 +186,11 @@ typedef struct pam_context pam_context;
 
 /* Library version string. */
 const char *pam_version(void);
+
+/* PAM_NUMMU of the library (16 or 32): the angle count of its results.  No
+ * counterpart in the reference, whose count is a compile-time PARAMETER
+ * (settings.f90:16) that pyPamtra mirrors as _nangles (core.py:263). */
+int32_t pam_nummu(void);
 
 /* Number of HIP devices visible to the library (0 when there is none). */
 int pam_device_count(void);

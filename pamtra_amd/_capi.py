@@ -15,9 +15,11 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-NUMMU = 16
+NUMMU = 16  # the reference's compile-time nummu (settings.f90:16)
 NSTOKES = 2
 NANGLES = 2 * NUMMU
+# angles per hemisphere -> the library built for it (pyPamtra's _nangles)
+LIBRARIES = {16: "libpamtra_amd.so", 32: "libpamtra_amd_n64.so"}
 
 PAM_SUCCESS, PAM_WARNING, PAM_FATAL, PAM_INFO = 0, 1, 2, 3
 
@@ -286,6 +288,7 @@ class Marshalled:
     nhydro: int
     keep: list = field(default_factory=list)
     radar_k2: object = 0.93  # nmlSet radar_k2: one value or one per frequency
+    nummu: int = NUMMU       # quadrature angles per hemisphere (sets["_nangles"])
 
 
 def marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile) -> Marshalled:
@@ -357,7 +360,7 @@ def marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile) -> Marshal
         keep.append(a)
         p.obs_height = _ptr(a, C.c_double)
     return Marshalled(settings, hydros, h4, p, nx, ny, L, nout, nh, keep,
-                      (nmlSets or {}).get("radar_k2", 0.93))
+                      (nmlSets or {}).get("radar_k2", 0.93), nangles(sets))
 
 
 @dataclass
@@ -370,10 +373,11 @@ class HostResults:
 def alloc_results(m: Marshalled, nfreq: int, diagnostics: bool = False,
                   msglen: int = 512) -> HostResults:
     nx, ny, L, nout = m.nx, m.ny, m.max_nlyrs, m.noutlevels
+    nmu = m.nummu
     arrs = {
-        "tb": np.full((nx, ny, nout, NANGLES, nfreq, NSTOKES), np.nan),
-        "emissivity": np.full((nx, ny, NSTOKES, nfreq, NUMMU), np.nan),
-        "angles_deg": np.full((NANGLES,), np.nan),
+        "tb": np.full((nx, ny, nout, 2 * nmu, nfreq, NSTOKES), np.nan),
+        "emissivity": np.full((nx, ny, NSTOKES, nfreq, nmu), np.nan),
+        "angles_deg": np.full((2 * nmu,), np.nan),
         "status": np.zeros((nx, ny, nfreq), dtype=np.int32),
     }
     if m.settings.active:
@@ -418,6 +422,8 @@ def declare_library(lib: C.CDLL) -> C.CDLL:
     """Argument and return types of the product library's C-ABI."""
     lib.pam_version.restype = C.c_char_p
     lib.pam_version.argtypes = []
+    lib.pam_nummu.restype = C.c_int32
+    lib.pam_nummu.argtypes = []
     lib.pam_device_count.restype = C.c_int
     lib.pam_device_count.argtypes = []
     lib.pam_context_create.restype = C.c_int
@@ -460,7 +466,22 @@ EXPORTED_SYMBOLS = ["pam_version", "pam_device_count", "pam_context_create",
                     "pam_last_stage_times", "pam_stage_names", "pam_last_work",
                     "pam_workspace_bytes", "pam_context_set_workspace_limit", "pam_context_bytes",
                     "pam_context_trim", "pam_tuning_keys", "pam_context_set_tuning",
-                    "pam_context_get_tuning"]
+                    "pam_context_get_tuning", "pam_nummu"]
 
 PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))
-LIBRARY_PATH = os.path.join(PACKAGE_DIR, "libpamtra_amd.so")
+LIBRARY_PATH = os.path.join(PACKAGE_DIR, LIBRARIES[NUMMU])
+
+
+def library_path(nummu: int = NUMMU) -> str:
+    if nummu not in LIBRARIES:
+        raise NotImplementedError(
+            f"{nummu} quadrature angles: the library is built for {sorted(LIBRARIES)}")
+    return os.path.join(PACKAGE_DIR, LIBRARIES[nummu])
+
+
+def nangles(sets) -> int:
+    """Angles per hemisphere of a run: pyPamtra's _nangles (core.py:263),
+    passed as sets["_nangles"]; 16 by default."""
+    n = int((sets or {}).get("_nangles", NUMMU))
+    library_path(n)
+    return n

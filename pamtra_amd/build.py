@@ -1,4 +1,7 @@
-"""Builds libpamtra_amd.so in-tree with hipcc for gfx950.
+"""Builds the library in-tree with hipcc for gfx950, in two variants of the
+same sources: libpamtra_amd.so (16 angles per hemisphere, the reference's
+compile-time nummu) and libpamtra_amd_n64.so (PAM_NUMMU=32: 32 angles,
+64x64 operators).
 
 Usage: python -m pamtra_amd.build   (or __graft_entry__.build())
 """
@@ -15,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpamtra_amd.so")
+VARIANTS = {16: "libpamtra_amd.so", 32: "libpamtra_amd_n64.so"}
 SOURCES = ["pam_column.hip", "pam_dsd.hip", "pam_mie.hip", "pam_phase.hip", "pam_rt.hip",
            "pam_capi.hip"]
 ARCH = os.environ.get("PAM_OFFLOAD_ARCH", "gfx950")
@@ -35,12 +39,13 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(job) -> str:
+    src, nummu = job
+    obj = os.path.join(BUILD, f"{os.path.splitext(src)[0]}_mu{nummu}.o")
+    cmd = [hipcc(), *FLAGS, f"-DPAM_NUMMU={nummu}", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
+        raise RuntimeError(f"hipcc failed for {src} (PAM_NUMMU={nummu}):\n{r.stderr}")
     if r.stderr.strip():
         lines = [ln for ln in r.stderr.splitlines() if "packed-fp32-ops" not in ln
                  and "not a recognized feature" not in ln]
@@ -50,17 +55,24 @@ def _compile(src: str) -> str:
 
 
 def build(verbose: bool = True) -> str:
+    """Builds both variants; returns the path of the 16-angle library."""
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
-    if verbose:
-        print(f"built {LIB}")
+    jobs = [(src, mu) for mu in VARIANTS for src in SOURCES]
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(_compile, jobs))
+    for k, (mu, name) in enumerate(VARIANTS.items()):
+        lib = os.path.join(HERE, name)
+        tmp = lib + ".tmp"
+        part = objs[k * len(SOURCES):(k + 1) * len(SOURCES)]
+        # -Bsymbolic: each variant binds its own symbols when both are loaded
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", *part,
+               "-o", tmp]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed:\n{r.stderr}")
+        os.replace(tmp, lib)
+        if verbose:
+            print(f"built {lib}")
     return LIB
 
 

@@ -288,8 +288,11 @@ class pyPamtra(object):
             self._checkData()
         if self.nmlSet["add_obs_height_to_layer"]:
             raise NotImplementedError("add_obs_height_to_layer is not supported")
+        # the angle count is the reference's compile-time nummu (settings.f90:16),
+        # mirrored as _nangles: 16, or 32 for the 32-angle library
+        sets = dict(self.set, _nangles=self._nangles)
         fortResults, self.fortError, fortObject = PamtraFortranWrapper(
-            self.set, self.nmlSet, self.df.data, self.df.data4D, self.df.dataFullSpec, self.p)
+            sets, self.nmlSet, self.df.data, self.df.data4D, self.df.dataFullSpec, self.p)
         if failOnError and self.fortError != 0:
             raise RuntimeError("Fortran code failed")
         self.r = fortResults

@@ -129,6 +129,7 @@ struct pam_context {
   int device = 0;
   hipStream_t stream = nullptr;  // the stream of pam_run (non-blocking)
   DevBuf colbuf, itembuf, iobuf, smallbuf;
+  DevBuf rtbuf;  // global scratch of the adding kernels at N = 64
   const double *d_atab = nullptr;
   const int64_t *d_atab_off = nullptr;
   int atab_nmax = 0;
@@ -281,6 +282,8 @@ __global__ void k_diag(Dims d, Work w, double *kextatmo, double *kexttot, double
 
 extern "C" const char *pam_version(void) { return "pamtra_amd 0.1.0 (gfx950)"; }
 
+extern "C" int32_t pam_nummu(void) { return PAM_NUMMU; }
+
 extern "C" int pam_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -342,6 +345,7 @@ extern "C" int pam_context_destroy(pam_context *c) {
   c->itembuf.release();
   c->iobuf.release();
   c->smallbuf.release();
+  c->rtbuf.release();
   drop_graph(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->d_mu) (void)hipFree(c->d_mu);
@@ -388,7 +392,7 @@ extern "C" int pam_context_set_workspace_limit(pam_context *c, int64_t bytes) {
 
 extern "C" int64_t pam_context_bytes(const pam_context *c) {
   if (!c) return -1;
-  return (int64_t)(c->colbuf.cap + c->itembuf.cap + c->iobuf.cap + c->smallbuf.cap);
+  return (int64_t)(c->colbuf.cap + c->itembuf.cap + c->iobuf.cap + c->smallbuf.cap + c->rtbuf.cap);
 }
 
 extern "C" int pam_context_trim(pam_context *c) {
@@ -400,6 +404,7 @@ extern "C" int pam_context_trim(pam_context *c) {
   c->itembuf.release();
   c->iobuf.release();
   c->smallbuf.release();
+  c->rtbuf.release();
   return PAM_SUCCESS;
 }
 
@@ -700,6 +705,10 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   w.ph_toff = d_ph + 2 * (PH_NBUCKET + 1);
   w.counters = d_cnt;
   w.redo_n = d_redo_n;
+  if (RT_SLOT_DOUBLES > 0) {
+    HIP_OK(c->rtbuf.ensure(RT_SLOT_DOUBLES * 8 * (size_t)std::max(1, c->ncu)));
+    w.rt_scratch = (double *)c->rtbuf.p;
+  }
   w.queue = d_queue;
   w.radar_k2 = (st->active && st->radar_k2) ? d_k2 : nullptr;
 

@@ -13,6 +13,10 @@ constexpr int NMU = PAM_NUMMU;       // quadrature angles per hemisphere
 constexpr int NST = PAM_NSTOKES;     // Stokes parameters
 constexpr int NN = NMU * NST;        // order of the R / T matrices (32)
 constexpr int NN2 = NN * NN;
+// global scratch of the adding kernels at N = 64, per workgroup slot (one
+// slot per CU): RT_GMATS padded NN x (NN + 1) FP64 matrices (none at N = 32)
+constexpr int RT_GMATS = (NN > 32) ? 6 : 0;
+constexpr size_t RT_SLOT_DOUBLES = (size_t)RT_GMATS * NN * (NN + 1);
 constexpr int MAXNLEG = PAM_MAXNLEG; // Legendre coefficients per series
 constexpr int NCOEF = 3;             // P11, P12, P33 series
 constexpr int MAX_NQUAD = 256;       // Gauss nodes of the Mie phase integral
@@ -146,6 +150,7 @@ struct Work {
   int32_t *redo_dbl;                   // [nitems*nf] of the chunk
   int32_t *redo_add;                   // [ncol*nf]
   int32_t *redo_lev;                   // [ncol*nf]: units with an interior output level
+  double *rt_scratch;                  // [ncu][RT_SLOT_DOUBLES] at N = 64, else null
   // work counters (atomics)
   unsigned long long *counters;        // [12], see pam_last_work
   // work-queue counters of the persistent grids: mie, doubling, adding

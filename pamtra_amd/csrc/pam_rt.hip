@@ -16,10 +16,13 @@
 //    output levels (radintg4.f90:119-172) and the Planck inversion to TB
 //    (collect_output.f90:31-88).
 //
-// All matrices live in LDS with a padded leading dimension.  Products run on
-// the FP64 matrix cores (one 16x16 output tile per wave); the inverse is a
+// Matrices live in LDS with a padded leading dimension (at NUMMU = 32 the
+// adding kernels keep their working matrices in a global scratch per
+// workgroup, see AddMats).  Products run on the FP64 matrix cores (16x16
+// output tiles, NN/32 squared of them per wave); the inverse is a
 // register-resident Gauss-Jordan with partial pivoting (the reference calls
-// LAPACK DGETRF/DGETRI).
+// LAPACK DGETRF/DGETRI).  The same source serves NUMMU = 16 (N = 32) and
+// NUMMU = 32 (N = 64).
 #include "pam_internal.h"
 
 namespace pam {
@@ -27,6 +30,19 @@ namespace pam {
 constexpr int RT_WG = 256;
 constexpr int LD = NN + 1;
 constexpr int MSZ = NN * LD;
+static_assert(NN == 32 || NN == 64, "the tilings below cover N = 32 and N = 64");
+constexpr int NT = NN / 16;        // 16x16 tiles per matrix dimension
+constexpr int TPW = NT * NT / 4;   // output tiles per wave (1 at N = 32, 4 at N = 64)
+// wave wv's tile q: k = wv + 4 q, rows 16 (k / NT), columns 16 (k % NT)
+__device__ __forceinline__ int tile_i(int wv, int q) { return ((wv + 4 * q) / NT) * 16; }
+__device__ __forceinline__ int tile_j(int wv, int q) { return ((wv + 4 * q) % NT) * 16; }
+// matrix-vector mappings: VL lanes per row when the workgroup runs four
+// vectors at once (4 x NN rows), RPL lanes per row for one vector (NN rows)
+constexpr int VL = RT_WG / (4 * NN);
+constexpr int RPL = RT_WG / NN;
+// unrolling of the k loops of the products: complete at N = 32; by 2 at
+// N = 64, where TPW = 4 tiles per wave already give 4 independent chains
+constexpr int KUNR = (NN == 32) ? NN / 4 : 2;
 
 // ------------------------------------------------------------------
 // LDS dense helpers; every thread of the workgroup calls them
@@ -47,9 +63,9 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// C = D + A * B (D may be NULL) on FP64 matrix cores.  The four waves of the
-// workgroup each own one 16x16 tile of C and run 8 v_mfma_f64_16x16x4
-// steps over k.  Operand layout of the 16x16x4 f64 MFMA: lane l supplies
+// C = D + A * B (D may be NULL) on FP64 matrix cores.  Each wave of the
+// workgroup owns TPW 16x16 tiles of C and runs NN/4 v_mfma_f64_16x16x4 steps
+// over k for each.  Operand layout of the 16x16x4 f64 MFMA: lane l supplies
 // A[l%16][l/16] and B[l/16][l%16] of the 16x4 / 4x16 blocks, and holds
 // D[4*r+l/16][l%16], r = 0..3, of the result (rows interleaved over the
 // four 16-lane groups).  Every lane has read all its operands before the
@@ -58,22 +74,35 @@ __device__ __forceinline__ float wave_max(float v) {
 __device__ __forceinline__ double mm_core(const double *A, const double *B, const double *D,
                                           double *C, float *wmax) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
-  const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
   const int li = l & 15, lk = l >> 4;
-  d4_t acc;
+  d4_t acc[TPW];
 #pragma unroll
-  for (int r = 0; r < 4; r++) acc[r] = D ? D[(I + 4 * r + lk) * LD + J + li] : 0.0;
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
 #pragma unroll
+    for (int r = 0; r < 4; r++) acc[q][r] = D ? D[(I + 4 * r + lk) * LD + J + li] : 0.0;
+  }
+#pragma unroll KUNR
   for (int s = 0; s < NN / 4; s++) {
-    const double a = A[(I + li) * LD + 4 * s + lk];
-    const double b = B[(4 * s + lk) * LD + J + li];
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+      const double a = A[(I + li) * LD + 4 * s + lk];
+      const double b = B[(4 * s + lk) * LD + J + li];
+      acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+    }
   }
   __syncthreads();
+  double m = 0.0;
 #pragma unroll
-  for (int r = 0; r < 4; r++) C[(I + 4 * r + lk) * LD + J + li] = acc[r];
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) C[(I + 4 * r + lk) * LD + J + li] = acc[q][r];
+    m = max_nan(m, max_nan(max_nan(fabs(acc[q][0]), fabs(acc[q][1])),
+                           max_nan(fabs(acc[q][2]), fabs(acc[q][3]))));
+  }
   if (wmax) {
-    const double m = max_nan(max_nan(fabs(acc[0]), fabs(acc[1])), max_nan(fabs(acc[2]), fabs(acc[3])));
     const float mf = wave_max((float)m);
     if (l == 0) wmax[wv] = mf;
   }
@@ -83,38 +112,50 @@ __device__ __forceinline__ double mm_core(const double *A, const double *B, cons
 
 // Two independent products in one pass: C1 = D1 + A1 B1, C2 = D2 + A2 B2
 // (D1, D2 may be NULL).  Each wave interleaves the two MFMA chains of its
-// tile; all operands are read before the first write, so any output may
+// tiles; all operands are read before the first write, so any output may
 // alias any input.  With wmax the return value is max |C2_ij|.
 __device__ __forceinline__ double mm2(const double *A1, const double *B1, const double *D1,
                                       double *C1, const double *A2, const double *B2,
                                       const double *D2, double *C2, float *wmax) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
-  const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
   const int li = l & 15, lk = l >> 4;
-  d4_t acc1, acc2;
+  d4_t acc1[TPW], acc2[TPW];
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    acc1[r] = D1 ? D1[(I + 4 * r + lk) * LD + J + li] : 0.0;
-    acc2[r] = D2 ? D2[(I + 4 * r + lk) * LD + J + li] : 0.0;
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      acc1[q][r] = D1 ? D1[(I + 4 * r + lk) * LD + J + li] : 0.0;
+      acc2[q][r] = D2 ? D2[(I + 4 * r + lk) * LD + J + li] : 0.0;
+    }
   }
-#pragma unroll
+#pragma unroll KUNR
   for (int s = 0; s < NN / 4; s++) {
-    const double a1 = A1[(I + li) * LD + 4 * s + lk];
-    const double b1 = B1[(4 * s + lk) * LD + J + li];
-    const double a2 = A2[(I + li) * LD + 4 * s + lk];
-    const double b2 = B2[(4 * s + lk) * LD + J + li];
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, acc2, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+      const double a1 = A1[(I + li) * LD + 4 * s + lk];
+      const double b1 = B1[(4 * s + lk) * LD + J + li];
+      const double a2 = A2[(I + li) * LD + 4 * s + lk];
+      const double b2 = B2[(4 * s + lk) * LD + J + li];
+      acc1[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc1[q], 0, 0, 0);
+      acc2[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, acc2[q], 0, 0, 0);
+    }
   }
   __syncthreads();
+  double m = 0.0;
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    C1[(I + 4 * r + lk) * LD + J + li] = acc1[r];
-    C2[(I + 4 * r + lk) * LD + J + li] = acc2[r];
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      C1[(I + 4 * r + lk) * LD + J + li] = acc1[q][r];
+      C2[(I + 4 * r + lk) * LD + J + li] = acc2[q][r];
+    }
+    m = max_nan(m, max_nan(max_nan(fabs(acc2[q][0]), fabs(acc2[q][1])),
+                           max_nan(fabs(acc2[q][2]), fabs(acc2[q][3]))));
   }
   if (wmax) {
-    const double m =
-        max_nan(max_nan(fabs(acc2[0]), fabs(acc2[1])), max_nan(fabs(acc2[2]), fabs(acc2[3])));
     const float mf = wave_max((float)m);
     if (l == 0) wmax[wv] = mf;
   }
@@ -155,6 +196,17 @@ __device__ __forceinline__ double mv_pair(const double *A, const double *x, int 
   return s + __shfl_xor(s, 1, 64);
 }
 
+// Row i of A x in the four-vector mapping: a lane pair per row at N = 32,
+// one lane per row at N = 64 (i = (t % 64) / VL, lead lane t % VL == 0)
+__device__ __forceinline__ double mv_vl(const double *A, const double *x, int i) {
+  if constexpr (VL == 2)
+    return mv_pair(A, x, i);
+  else
+    return mv_row(A, x, i);
+}
+__device__ __forceinline__ int vl_row() { return (threadIdx.x & 63) / VL; }
+__device__ __forceinline__ bool vl_lead() { return (threadIdx.x % VL) == 0; }
+
 // LDS scratch of the inverses; gj_only: Gauss-Jordan without trying the
 // series first (set by the queue pass in the Gauss-Jordan test mode)
 struct InvLds {
@@ -166,30 +218,31 @@ struct InvLds {
 };
 
 // G = inv(I - X) by Gauss-Jordan elimination with partial pivoting on the
-// augmented [I - X | I].  Rows stay in registers: thread t owns row t/8 and
-// the 8 augmented columns 8*(t%8)..+7.  Pivoting is implicit (no row swaps):
-// the pivot row of step k becomes row k of the inverse.  Two barriers per
-// pivot.  Returns false if a pivot is zero (singular matrix).  Only the
-// Gauss-Jordan kernels use it, so the series kernels keep a low register
-// count.
+// augmented [I - X | I].  Rows stay in registers: thread t owns row t/RPL
+// and the GJC augmented columns GJC*(t%RPL)..+GJC-1 (8 of 64 at N = 32, 32
+// of 128 at N = 64).  Pivoting is implicit (no row swaps): the pivot row of
+// step k becomes row k of the inverse.  Two barriers per pivot.  Returns
+// false if a pivot is zero (singular matrix).  Only the Gauss-Jordan kernels
+// use it, so the series kernels keep a low register count.
+constexpr int GJC = 2 * NN / RPL;
 __device__ bool inv_i_minus(const double *X, double *G, InvLds &sh) {
-  const int t = threadIdx.x, r = t >> 3, cg = t & 7, l = t & 63, wv = t >> 6;
-  double a[8];
+  const int t = threadIdx.x, r = t / RPL, cg = t % RPL, l = t & 63, wv = t >> 6;
+  double a[GJC];
 #pragma unroll
-  for (int c = 0; c < 8; c++) {
-    const int j = cg * 8 + c;
+  for (int c = 0; c < GJC; c++) {
+    const int j = cg * GJC + c;
     a[c] = (j < NN) ? ((r == j ? 1.0 : 0.0) - X[r * LD + j]) : ((r == j - NN) ? 1.0 : 0.0);
   }
   bool used = false, ok = true;
   int my_step = -1;
   for (int k = 0; k < NN; k++) {
-    const int kg = k >> 3, kc = k & 7;
+    const int kg = k / GJC, kc = k % GJC;
     double ck = 0.0;
 #pragma unroll
-    for (int c = 0; c < 8; c++)
+    for (int c = 0; c < GJC; c++)
       if (c == kc) ck = a[c];
     // entry of my row in column k (held by the lane of column group kg)
-    const double f = __shfl(ck, (l & ~7) | kg, 64);
+    const double f = __shfl(ck, (l & ~(RPL - 1)) | kg, 64);
     // pivot: largest |entry| among rows not yet used, lowest row on ties
     double v = (!used && cg == 0) ? fabs(f) : -1.0;
     int idx = r;
@@ -222,9 +275,9 @@ __device__ bool inv_i_minus(const double *X, double *G, InvLds &sh) {
     double *prow = sh.prow[k & 1];
     if (r == p) {
 #pragma unroll
-      for (int c = 0; c < 8; c++) {
+      for (int c = 0; c < GJC; c++) {
         a[c] = a[c] / f;
-        prow[cg * 8 + c] = a[c];
+        prow[cg * GJC + c] = a[c];
       }
       used = true;
       my_step = k;
@@ -232,19 +285,19 @@ __device__ bool inv_i_minus(const double *X, double *G, InvLds &sh) {
     __syncthreads();
     if (r != p) {
 #pragma unroll
-      for (int c = 0; c < 8; c++) a[c] = a[c] - f * prow[cg * 8 + c];
+      for (int c = 0; c < GJC; c++) a[c] = a[c] - f * prow[cg * GJC + c];
     }
   }
-  if (cg >= 4 && my_step >= 0) {
+  if (cg >= RPL / 2 && my_step >= 0) {
 #pragma unroll
-    for (int c = 0; c < 8; c++) G[my_step * LD + (cg * 8 + c - NN)] = a[c];
+    for (int c = 0; c < GJC; c++) G[my_step * LD + (cg * GJC + c - NN)] = a[c];
   }
   __syncthreads();
   return ok;
 }
 
 // Truncation threshold of the product-form series below: the omitted tail
-// X^(2^k) (I - X)^-1 has entries below 32 * 2^-56 ||G||, under the rounding
+// X^(2^k) (I - X)^-1 has entries below N * 2^-56 ||G||, under the rounding
 // of the factors themselves.
 constexpr double NEU_TOL = 0x1p-56;
 constexpr int NEU_MAXIT = 20;
@@ -270,13 +323,13 @@ __device__ __forceinline__ bool series_inv(const double *A, const double *B, dou
   // G = I + P with P = X; then per factor: G <- G (I + P), P <- P^2, the
   // two products fused while the next factor still matters
   if (!(m < 0x1p64)) return false;          // NaN or overflow
-  if (32.0 * m * m < NEU_TOL) return true;  // bound on max |P^2|
+  if ((double)NN * m * m < NEU_TOL) return true;  // bound on max |P^2|
   m = mm_core(X, X, nullptr, X, sh.wmax);
   np++;
   for (int it = 0; it < NEU_MAXIT; it++) {
     if (!(m < 0x1p64)) return false;
     if (m < NEU_TOL) return true;
-    if (32.0 * m * m < NEU_TOL) {  // last factor
+    if ((double)NN * m * m < NEU_TOL) {  // last factor
       mm_add(G, G, X, G);
       np++;
       return true;
@@ -422,16 +475,16 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
     // sources: S+' = (S+ + Z (S+ + R (S- + f C-))) + f C+, C+' = C+ + Z (C+ + R C-),
     // S-' = S- + Z ((S- + f C-) + R S+), C-' = C- + Z (C- + R C+)
     {
-      const int g = t >> 6, i = (t >> 1) & 31;
-      const bool lead = (t & 1) == 0;
+      const int g = t >> 6, i = vl_row();
+      const bool lead = vl_lead();
       if (t < NN) sv[4][t] = sv[1][t] + linfac * sv[3][t];  // S- + f C-
       __syncthreads();
       const double *v1 = g == 0 ? sv[4] : g == 1 ? sv[3] : g == 2 ? sv[0] : sv[2];
       const double base1 = g == 0 ? sv[0][i] : g == 1 ? sv[2][i] : g == 2 ? sv[4][i] : sv[3][i];
-      const double u = base1 + mv_pair(sR, v1, i);
+      const double u = base1 + mv_vl(sR, v1, i);
       if (lead) sv[8 + g][i] = u;
       __syncthreads();
-      const double z = mv_pair(sG, sv[8 + g], i);
+      const double z = mv_vl(sG, sv[8 + g], i);
       const double nv = g == 0 ? (sv[0][i] + z) + linfac * sv[2][i]
                                : (g == 1 ? sv[2][i] : g == 2 ? sv[1][i] : sv[3][i]) + z;
       __syncthreads();
@@ -466,8 +519,11 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
 // One unit per workgroup, or (PERS) a persistent grid taking units from a
 // work queue.  The two are separate instantiations, so that the loop does
 // not cost the one-unit kernel registers.
+// four waves per SIMD at N = 32; at N = 64 the 141 KB of LDS admit one
+// workgroup per CU, and the registers are left to the larger tiles
+constexpr int DBL_WAVES = (NN == 32) ? 4 : 1;
 template <bool PERS>
-__global__ void __launch_bounds__(RT_WG) __attribute__((amdgpu_waves_per_eu(4))) k_doubling(Dims d, Settings s, const double *freqs,
+__global__ void __launch_bounds__(RT_WG) __attribute__((amdgpu_waves_per_eu(DBL_WAVES))) k_doubling(Dims d, Settings s, const double *freqs,
                                                     Work w) {
   __shared__ DblLds L;
   const int n = d.nitems * d.nf;
@@ -515,29 +571,43 @@ __global__ void __launch_bounds__(RT_WG) k_doubling_gj(Dims d, Settings s, const
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
 // C = D + (A + iA I) B on the FP32 matrix cores.  16x16x4 f32 layout: lane l
-// supplies A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+r][l%16].
-// With wmax the return value is max |C_ij| (NaN propagates).
+// supplies A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+r][l%16];
+// each wave owns TPW output tiles.  With wmax the return value is max |C_ij|
+// (NaN propagates).
 __device__ __forceinline__ float mmf(const float *A, bool iA, const float *B, const float *D,
                                      float *C, float *wmax) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
-  const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
   const int li = l & 15, lk = l >> 4;
-  f4_t acc;
+  f4_t acc[TPW];
 #pragma unroll
-  for (int r = 0; r < 4; r++) acc[r] = D ? D[(I + 4 * lk + r) * LD + J + li] : 0.f;
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
 #pragma unroll
+    for (int r = 0; r < 4; r++) acc[q][r] = D ? D[(I + 4 * lk + r) * LD + J + li] : 0.f;
+  }
+#pragma unroll KUNR
   for (int s = 0; s < NN / 4; s++) {
     const int kc = 4 * s + lk;
-    float a = A[(I + li) * LD + kc];
-    if (iA && I + li == kc) a += 1.f;
-    const float b = B[kc * LD + J + li];
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+      float a = A[(I + li) * LD + kc];
+      if (iA && I + li == kc) a += 1.f;
+      const float b = B[kc * LD + J + li];
+      acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[q], 0, 0, 0);
+    }
   }
   __syncthreads();
+  float m = 0.f;
 #pragma unroll
-  for (int r = 0; r < 4; r++) C[(I + 4 * lk + r) * LD + J + li] = acc[r];
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) C[(I + 4 * lk + r) * LD + J + li] = acc[q][r];
+    m = max_nan(m, max_nan(max_nan(fabsf(acc[q][0]), fabsf(acc[q][1])),
+                           max_nan(fabsf(acc[q][2]), fabsf(acc[q][3]))));
+  }
   if (wmax) {
-    const float m = max_nan(max_nan(fabsf(acc[0]), fabsf(acc[1])), max_nan(fabsf(acc[2]), fabsf(acc[3])));
     const float mf = wave_max(m);
     if (l == 0) wmax[wv] = mf;
   }
@@ -551,34 +621,46 @@ __device__ __forceinline__ float mmf2(const float *A1, bool iA1, const float *B1
                                       float *C1, const float *A2, bool iA2, const float *B2,
                                       const float *D2, float *C2, float *wmax) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
-  const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
   const int li = l & 15, lk = l >> 4;
-  f4_t acc1, acc2;
+  f4_t acc1[TPW], acc2[TPW];
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    acc1[r] = D1 ? D1[(I + 4 * lk + r) * LD + J + li] : 0.f;
-    acc2[r] = D2 ? D2[(I + 4 * lk + r) * LD + J + li] : 0.f;
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      acc1[q][r] = D1 ? D1[(I + 4 * lk + r) * LD + J + li] : 0.f;
+      acc2[q][r] = D2 ? D2[(I + 4 * lk + r) * LD + J + li] : 0.f;
+    }
   }
-#pragma unroll
+#pragma unroll KUNR
   for (int s = 0; s < NN / 4; s++) {
     const int kc = 4 * s + lk;
-    const bool dg = I + li == kc;
-    float a1 = A1[(I + li) * LD + kc], a2 = A2[(I + li) * LD + kc];
-    if (iA1 && dg) a1 += 1.f;
-    if (iA2 && dg) a2 += 1.f;
-    const float b1 = B1[kc * LD + J + li], b2 = B2[kc * LD + J + li];
-    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc2, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+      const bool dg = I + li == kc;
+      float a1 = A1[(I + li) * LD + kc], a2 = A2[(I + li) * LD + kc];
+      if (iA1 && dg) a1 += 1.f;
+      if (iA2 && dg) a2 += 1.f;
+      const float b1 = B1[kc * LD + J + li], b2 = B2[kc * LD + J + li];
+      acc1[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc1[q], 0, 0, 0);
+      acc2[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc2[q], 0, 0, 0);
+    }
   }
   __syncthreads();
+  float m = 0.f;
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    C1[(I + 4 * lk + r) * LD + J + li] = acc1[r];
-    C2[(I + 4 * lk + r) * LD + J + li] = acc2[r];
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      C1[(I + 4 * lk + r) * LD + J + li] = acc1[q][r];
+      C2[(I + 4 * lk + r) * LD + J + li] = acc2[q][r];
+    }
+    m = max_nan(m, max_nan(max_nan(fabsf(acc2[q][0]), fabsf(acc2[q][1])),
+                           max_nan(fabsf(acc2[q][2]), fabsf(acc2[q][3]))));
   }
   if (wmax) {
-    const float m =
-        max_nan(max_nan(fabsf(acc2[0]), fabsf(acc2[1])), max_nan(fabsf(acc2[2]), fabsf(acc2[3])));
     const float mf = wave_max(m);
     if (l == 0) wmax[wv] = mf;
   }
@@ -598,6 +680,18 @@ __device__ __forceinline__ float mvf_pair(const float *A, const float *x, int i)
   return s + __shfl_xor(s, 1, 64);
 }
 
+// Row i of A x in FP32 in the four-vector mapping (as mv_vl)
+__device__ __forceinline__ float mvf_vl(const float *A, const float *x, int i) {
+  if constexpr (VL == 2) {
+    return mvf_pair(A, x, i);
+  } else {
+    float s = 0.f;
+#pragma unroll 8
+    for (int k = 0; k < NN; k++) s += A[i * LD + k] * x[k];
+    return s;
+  }
+}
+
 // Truncation of the FP32 series: the omitted tail is below the single-
 // precision resolution of G = I + G~.
 constexpr float NEU_TOL_F = 0x1p-28f;
@@ -614,13 +708,13 @@ __device__ bool series_f32(const float *R, float *X, float *Gt, float *wmax, int
   }
   __syncthreads();
   if (!(m < 0x1p64f)) return false;
-  if (32.f * m * m < NEU_TOL_F) return true;
+  if ((float)NN * m * m < NEU_TOL_F) return true;
   m = mmf(X, false, X, nullptr, X, wmax);
   np++;
   for (int it = 0; it < NEU_MAXIT; it++) {
     if (!(m < 0x1p64f)) return false;
     if (m < NEU_TOL_F) return true;
-    if (32.f * m * m < NEU_TOL_F) {  // last factor: G~ <- G~ + (I + G~) P
+    if ((float)NN * m * m < NEU_TOL_F) {  // last factor: G~ <- G~ + (I + G~) P
       mmf(Gt, true, X, Gt, Gt, nullptr);
       np++;
       return true;
@@ -645,7 +739,7 @@ __device__ bool series_inv_given(double *X, double *G, float *wmax, double tol, 
   for (int it = 0; it < NEU_MAXIT; it++) {
     if (!(m < 0x1p64)) return false;
     if (m < tol) return true;
-    if (32.0 * m * m < tol) {
+    if ((double)NN * m * m < tol) {
       mm_add(G, G, X, G);
       np++;
       return true;
@@ -792,28 +886,28 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
                                   1.0 / 24.0,  1.0 / 120.0,  1.0 / 720.0,   1.0 / 5040.0,
                                   1.0 / 40320.0, 1.0 / 362880.0};
     double gcp = 0.0, gcm = 0.0, grp = 0.0, grm = 0.0;
-    const int i = (t >> 1) & 31;
+    const int i = vl_row();
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       if (t < 64) {
         const double v = cv[i];
         gcm += 2.0 * v * ifact[2 * k + 1];
         grm += 2.0 * v * ifact[2 * k + 2];
-        const double wk = mv_pair(M0, cv, i);
+        const double wk = mv_vl(M0, cv, i);
         gcp -= 2.0 * wk * ifact[2 * k + 2];
         grp -= 2.0 * wk * ifact[2 * k + 3];
-        if ((t & 1) == 0) cw[i] = wk;
+        if (vl_lead()) cw[i] = wk;
       }
       __syncthreads();
       if (k < 3) {
         if (t < 64) {
-          const double v = mv_pair(M1, cw, i);
-          if ((t & 1) == 0) cv[i] = v;
+          const double v = mv_vl(M1, cw, i);
+          if (vl_lead()) cv[i] = v;
         }
         __syncthreads();
       }
     }
-    if (t < 64 && (t & 1) == 0) {
+    if (t < 64 && vl_lead()) {
       L.g[0][i] = gcp;
       L.g[1][i] = gcm;
       L.g[2][i] = grp;
@@ -860,16 +954,16 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   // S+ = g+ - R g-, S- = -T g-: linear source (groups 0, 1), constant (2, 3)
   float src;
   {
-    const int g = t >> 6, i = (t >> 1) & 31;
+    const int g = t >> 6, i = vl_row();
     const double *A = (g & 1) ? M2 : M0;
-    const double mv = mv_pair(A, g < 2 ? cv : cw, i);
+    const double mv = mv_vl(A, g < 2 ? cv : cw, i);
     const double gcplus = 0.5 * (L.g[0][i] + L.g[1][i]), grplus = 0.5 * (L.g[2][i] + L.g[3][i]);
     const double gplus = p0 * (g < 2 ? gcplus + linfac * grplus : gcplus);
     src = (float)((g & 1) ? -mv : gplus - mv);
   }
-  float rr[4], tt[4];
+  float rr[NN2 / RT_WG], tt[NN2 / RT_WG];
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
+  for (int q = 0; q < NN2 / RT_WG; q++) {
     const int e = t + q * RT_WG, i = e / NN, j = e % NN;
     rr[q] = (float)M0[i * LD + j];
     tt[q] = (float)(M2[i * LD + j] - (i == j ? 1.0 : 0.0));
@@ -878,12 +972,12 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   float *sR = L.u.f.R, *sT = L.u.f.T, *sG = L.u.f.G, *sX = L.u.f.X;
   float(*sv)[NN] = L.u.f.v;
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
+  for (int q = 0; q < NN2 / RT_WG; q++) {
     const int e = t + q * RT_WG, i = e / NN, j = e % NN;
     sR[i * LD + j] = rr[q];
     sT[i * LD + j] = tt[q];
   }
-  if ((t & 1) == 0) sv[t >> 6][(t >> 1) & 31] = src;
+  if (vl_lead()) sv[t >> 6][vl_row()] = src;
   __syncthreads();
   for (int it = 0; it < nd; it++) {
     if (!series_f32(sR, sX, sG, L.wmax, nprod)) return false;
@@ -891,16 +985,16 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     mmf2(sT, true, sG, sT, sG, sR, false, sT, sR, sX, nullptr);
     {
       const float lf = (float)linfac;
-      const int g = t >> 6, i = (t >> 1) & 31;
-      const bool lead = (t & 1) == 0;
+      const int g = t >> 6, i = vl_row();
+      const bool lead = vl_lead();
       if (t < NN) sv[4][t] = sv[1][t] + lf * sv[3][t];
       __syncthreads();
       const float *v1 = g == 0 ? sv[4] : g == 1 ? sv[3] : g == 2 ? sv[0] : sv[2];
       const float base1 = g == 0 ? sv[0][i] : g == 1 ? sv[2][i] : g == 2 ? sv[4][i] : sv[3][i];
-      const float u = base1 + mvf_pair(sR, v1, i);
+      const float u = base1 + mvf_vl(sR, v1, i);
       if (lead) sv[8 + g][i] = u;
       __syncthreads();
-      const float zu = u + mvf_pair(sG, sv[8 + g], i);  // Z u = u + Z~ u
+      const float zu = u + mvf_vl(sG, sv[8 + g], i);  // Z u = u + Z~ u
       const float nv = g == 0 ? (sv[0][i] + zu) + lf * sv[2][i]
                               : (g == 1 ? sv[2][i] : g == 2 ? sv[1][i] : sv[3][i]) + zu;
       __syncthreads();
@@ -957,27 +1051,29 @@ struct Stack {
   double *Rp, *Rm, *Tp, *Tm, *Sp, *Sm;
 };
 
-// Row i = t/8 of A x on all 256 threads: lane t%8 sums the columns
-// 4 (t%8) .. 4 (t%8) + 3 (conflict-free LDS banks), the eight lanes of a row
-// combine by shuffles; all eight return the full sum.  No barrier.
+// Row i = t/RPL of A x on all 256 threads: lane t%RPL sums the NN/RPL
+// columns from (NN/RPL)(t%RPL) on (4 at N = 32, conflict-free LDS banks; 16
+// at N = 64), the RPL lanes of a row combine by shuffles; all of them return
+// the full sum.  No barrier.
+constexpr int MVC = NN / RPL;
 __device__ __forceinline__ double mv_row8(const double *A, const double *x, int i) {
-  const int c0 = 4 * (threadIdx.x & 7);
+  const int c0 = MVC * (threadIdx.x % RPL);
   double s = A[i * LD + c0] * x[c0];
 #pragma unroll
-  for (int k = 1; k < 4; k++) s += A[i * LD + c0 + k] * x[c0 + k];
-  s += __shfl_xor(s, 1, 64);
-  s += __shfl_xor(s, 2, 64);
-  s += __shfl_xor(s, 4, 64);
+  for (int k = 1; k < MVC; k++) s += A[i * LD + c0 + k] * x[c0 + k];
+#pragma unroll
+  for (int o = 1; o < RPL; o <<= 1) s += __shfl_xor(s, o, 64);
   return s;
 }
+__device__ __forceinline__ int r8_row() { return threadIdx.x / RPL; }
+__device__ __forceinline__ bool r8_lead() { return (threadIdx.x % RPL) == 0; }
 
 // v <- A v on all threads, with barriers so that the shared vector may be
 // reused between stages
 __device__ __forceinline__ void mv_stage(const double *A, double *v) {
-  const int t = threadIdx.x;
-  const double u = mv_row8(A, v, t >> 3);
+  const double u = mv_row8(A, v, r8_row());
   __syncthreads();
-  if ((t & 7) == 0) v[t >> 3] = u;
+  if (r8_lead()) v[r8_row()] = u;
   __syncthreads();
 }
 
@@ -990,28 +1086,47 @@ __device__ __forceinline__ void mm2x(const double *A1, const double *B1, const d
                                      double *C1, const double *A2, const double *B2,
                                      const double *D2, double *C2) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
-  const int I = (wv >> 1) * 16, J = (wv & 1) * 16;
   const int li = l & 15, lk = l >> 4;
-  f4_t acc1 = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
-  double d1[4], d2[4];
+  f4_t acc1[TPW], acc2[TPW];
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    d1[r] = D1 ? D1[(I + 4 * lk + r) * LD + J + li] : 0.0;
-    d2[r] = D2 ? D2[(I + 4 * lk + r) * LD + J + li] : 0.0;
+  for (int q = 0; q < TPW; q++) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      acc1[q][r] = 0.f;
+      acc2[q][r] = 0.f;
+    }
   }
-#pragma unroll
+#pragma unroll KUNR
   for (int s = 0; s < NN / 4; s++) {
     const int kc = 4 * s + lk;
-    const float a1 = (float)A1[(I + li) * LD + kc], a2 = (float)A2[(I + li) * LD + kc];
-    const float b1 = (float)B1[kc * LD + J + li], b2 = (float)B2[kc * LD + J + li];
-    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc2, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+      const float a1 = (float)A1[(I + li) * LD + kc], a2 = (float)A2[(I + li) * LD + kc];
+      const float b1 = (float)B1[kc * LD + J + li], b2 = (float)B2[kc * LD + J + li];
+      acc1[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc1[q], 0, 0, 0);
+      acc2[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc2[q], 0, 0, 0);
+    }
+  }
+  double d1[TPW][4], d2[TPW][4];
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      d1[q][r] = D1 ? D1[(I + 4 * lk + r) * LD + J + li] : 0.0;
+      d2[q][r] = D2 ? D2[(I + 4 * lk + r) * LD + J + li] : 0.0;
+    }
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    C1[(I + 4 * lk + r) * LD + J + li] = d1[r] + (double)acc1[r];
-    C2[(I + 4 * lk + r) * LD + J + li] = d2[r] + (double)acc2[r];
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      C1[(I + 4 * lk + r) * LD + J + li] = d1[q][r] + (double)acc1[q][r];
+      C2[(I + 4 * lk + r) * LD + J + li] = d2[q][r] + (double)acc2[q][r];
+    }
   }
   __syncthreads();
 }
@@ -1041,8 +1156,8 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
   const int t = threadIdx.x;
   if (!inv_i_minus_ab<GJ>(R2, s1.Rp, X, G, sinv, np)) return false;
   {
-    const double u = mv_row8(R2, s1.Sp, t >> 3);
-    if ((t & 7) == 0) vs[t >> 3] = S2m[t >> 3] + u;  // S2m + R2m S1p
+    const double u = mv_row8(R2, s1.Sp, r8_row());
+    if (r8_lead()) vs[r8_row()] = S2m[r8_row()] + u;  // S2m + R2m S1p
   }
   mm2p<MX>(s1.Tm, G, nullptr, X, R2, s1.Tp, nullptr, G);  // X = A, G = R2 T1p
   mv_stage(X, vs);
@@ -1051,8 +1166,8 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
   np += 4;
   if (!inv_i_minus_ab<GJ>(s1.Rp, R2, X, G, sinv, np)) return false;
   {
-    const double u = mv_row8(s1.Rp, S2m, t >> 3);
-    if ((t & 7) == 0) vs[t >> 3] = s1.Sp[t >> 3] + u;  // S1p + R1p S2m
+    const double u = mv_row8(s1.Rp, S2m, r8_row());
+    if (r8_lead()) vs[r8_row()] = s1.Sp[r8_row()] + u;  // S1p + R1p S2m
   }
   mm2p<MX>(T2, G, nullptr, X, s1.Rp, T2, nullptr, G);       // X = B, G = R1p T2
   mv_stage(X, vs);
@@ -1067,11 +1182,11 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
 // T2 = diag(dg)).  With R2 = 0 both gammas are the identity and every
 // product reduces to a diagonal scaling or one matrix-vector product.
 __device__ void combine_clear(Stack &s1, const double *dg, const double *S2p, const double *S2m) {
-  const int t = threadIdx.x, i8 = t >> 3;
+  const int t = threadIdx.x, i8 = r8_row();
   const double u = mv_row8(s1.Tm, S2m, i8);                // T1m S2m
   const double u2 = s1.Sp[i8] + mv_row8(s1.Rp, S2m, i8);  // S1p + R1p S2m
   __syncthreads();
-  if ((t & 7) == 0) {
+  if (r8_lead()) {
     s1.Sm[i8] = s1.Sm[i8] + u;
     s1.Sp[i8] = S2p[i8] + dg[i8] * u2;
   }
@@ -1092,8 +1207,8 @@ __device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *
   const int t = threadIdx.x;
   if (!inv_i_minus_ab<GJ>(Rs, s1.Rp, X, G, sinv, np)) return false;
   {
-    const double u = mv_row8(Rs, s1.Sp, t >> 3);
-    if ((t & 7) == 0) vs[t >> 3] = u;
+    const double u = mv_row8(Rs, s1.Sp, r8_row());
+    if (r8_lead()) vs[r8_row()] = u;
   }
   __syncthreads();
   mv_stage(G, vs);
@@ -1116,10 +1231,15 @@ __device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *
 
 // The next layer's inputs, fetched into registers while the current layer
 // is combined (the barriers of the combine do not wait for global loads).
+// At N = 32 the R and T elements come ahead into registers (8 per thread);
+// at N = 64 (32 per thread) stage_layer copies them from HBM itself.
+constexpr bool FETCH_RT = (NN == 32);
+constexpr int FETCH_N = FETCH_RT ? NN2 / RT_WG : 1;
 struct LayerFetch {
   int itf;                       // scattering unit, or -1 for a clear layer
   int st;                        // its status (thread 0)
-  double r[NN2 / RT_WG], tt[NN2 / RT_WG];  // R, T elements t + RT_WG k
+  double r[FETCH_N], tt[FETCH_N];  // R, T elements t + RT_WG k (FETCH_RT)
+  const double *gR, *gT;         // the unit's R and T in HBM (N = 64)
   double sp, sm;                 // S+, S- (t < NN)
   double h0, h1, kg, tp0, tp1;   // clear layer: levels, gas, temperatures
 };
@@ -1132,10 +1252,14 @@ __device__ __forceinline__ void fetch_layer(const Dims &d, const Work &w, int co
     const size_t itf = (size_t)(item - d.item0) * d.nf + f;
     F.itf = (int)itf;
     const double *R = w.R + itf * NN2, *T = w.T + itf * NN2;
+    F.gR = R;
+    F.gT = T;
+    if constexpr (FETCH_RT) {
 #pragma unroll
-    for (int k = 0; k < NN2 / RT_WG; k++) {
-      F.r[k] = R[t + RT_WG * k];
-      F.tt[k] = T[t + RT_WG * k];
+      for (int k = 0; k < NN2 / RT_WG; k++) {
+        F.r[k] = R[t + RT_WG * k];
+        F.tt[k] = T[t + RT_WG * k];
+      }
     }
     if (t < NN) {
       F.sp = w.Sp[itf * NN + t];
@@ -1161,11 +1285,20 @@ __device__ bool stage_layer(const LayerFetch &F, double wl, const double *smu, d
   const int t = threadIdx.x;
   if (F.itf >= 0) {
     if (t == 0 && (F.st & 0xff)) atomicMax(sbad, F.st);
+    if constexpr (FETCH_RT) {
 #pragma unroll
-    for (int k = 0; k < NN2 / RT_WG; k++) {
-      const int e = t + RT_WG * k, i = e / NN, j = e % NN;
-      R2[i * LD + j] = F.r[k];
-      T2[i * LD + j] = F.tt[k];
+      for (int k = 0; k < NN2 / RT_WG; k++) {
+        const int e = t + RT_WG * k, i = e / NN, j = e % NN;
+        R2[i * LD + j] = F.r[k];
+        T2[i * LD + j] = F.tt[k];
+      }
+    } else {
+#pragma unroll 4
+      for (int k = 0; k < NN2 / RT_WG; k++) {
+        const int e = t + RT_WG * k, i = e / NN, j = e % NN;
+        R2[i * LD + j] = F.gR[e];
+        T2[i * LD + j] = F.gT[e];
+      }
     }
     if (t < NN) {
       S2p[t] = F.sp;
@@ -1320,8 +1453,22 @@ __device__ void boundary_radiances(char refl, const double *E, const double *Rf,
   __syncthreads();
 }
 
+// The adding kernels' matrices.  At N = 32 all of them live in LDS.  At
+// N = 64 (33 KB each) only the stack R+, R-, T+, T- stays in LDS; the
+// working matrices live in a global scratch of RT_GMATS matrices per
+// workgroup slot (w.rt_scratch; the kernels then run at most one workgroup
+// per slot, so blockIdx.x is the slot).  Generic pointers serve both.
+constexpr int ADD_LDS = (NN == 32) ? 8 : 4;   // of the 8 matrices of k_adding
+constexpr int LEV_LDS = (NN == 32) ? 10 : 4;  // of the 10 of k_adding_levels
+static_assert(8 - ADD_LDS <= RT_GMATS && 10 - LEV_LDS <= RT_GMATS, "scratch per slot");
+
+__device__ __forceinline__ double *add_mat(double (*lds)[MSZ], int nlds, const Work &w, int k) {
+  if (k < nlds) return lds[k];
+  return w.rt_scratch + ((size_t)blockIdx.x * RT_GMATS + (k - nlds)) * MSZ;
+}
+
 struct AddLds {
-  double M[8][MSZ];
+  double M[ADD_LDS][MSZ];
   double v[10][NN];
   InvLds inv;
   double mu[NMU], qw[NMU];
@@ -1336,7 +1483,6 @@ struct AddLds {
 template <bool GJ, bool MX = false>
 __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freqs,
                             const char *sfc_refl, const Work &w, double *tb, int cf, AddLds &LL) {
-  double(*M)[MSZ] = LL.M;
   double(*sv)[NN] = LL.v;
   InvLds &sinv = LL.inv;
   double *smu = LL.mu, *sqw = LL.qw;
@@ -1397,8 +1543,10 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
     }
     return true;
   }
-  Stack stk = {M[0], M[1], M[2], M[3], sv[0], sv[1]};
-  double *R2 = M[4], *T2 = M[5], *G = M[6], *X = M[7];
+  Stack stk = {add_mat(LL.M, ADD_LDS, w, 0), add_mat(LL.M, ADD_LDS, w, 1),
+               add_mat(LL.M, ADD_LDS, w, 2), add_mat(LL.M, ADD_LDS, w, 3), sv[0], sv[1]};
+  double *R2 = add_mat(LL.M, ADD_LDS, w, 4), *T2 = add_mat(LL.M, ADD_LDS, w, 5);
+  double *G = add_mat(LL.M, ADD_LDS, w, 6), *X = add_mat(LL.M, ADD_LDS, w, 7);
   double *S2p = sv[2], *S2m = sv[3], *dg = sv[4], *vs = sv[5];
   double *sky = sv[6], *gnd = sv[7], *ur = sv[8], *dr = sv[9];
   const double psky = planck(SKY_TEMP, wl);
@@ -1666,7 +1814,7 @@ __device__ bool fold_layers(const Dims &d, const Work &w, int col, int f, int L,
 }
 
 struct LevLds {
-  double M[10][MSZ];
+  double M[LEV_LDS][MSZ];
   double v[11][NN];
   InvLds inv;
   double mu[NMU], qw[NMU];
@@ -1680,7 +1828,6 @@ struct LevLds {
 // both precisions.  Output levels are 1..L+1 (run_rt.f90:166-184).
 __device__ void levels_unit(const Dims &d, const Settings &s, const double *freqs,
                             const char *sfc_refl, const Work &w, double *tb, int cf, LevLds &LL) {
-  double(*M)[MSZ] = LL.M;
   double(*sv)[NN] = LL.v;
   const int t = threadIdx.x;
   const int cl = cf / d.nf, f = cf % d.nf;
@@ -1707,8 +1854,11 @@ __device__ void levels_unit(const Dims &d, const Settings &s, const double *freq
           __builtin_nan("");
     }
   };
-  Stack stk = {M[0], M[1], M[2], M[3], sv[0], sv[1]};
-  double *URp = M[4], *UTp = M[5], *R2 = M[6], *T2 = M[7], *G = M[8], *X = M[9];
+  Stack stk = {add_mat(LL.M, LEV_LDS, w, 0), add_mat(LL.M, LEV_LDS, w, 1),
+               add_mat(LL.M, LEV_LDS, w, 2), add_mat(LL.M, LEV_LDS, w, 3), sv[0], sv[1]};
+  double *URp = add_mat(LL.M, LEV_LDS, w, 4), *UTp = add_mat(LL.M, LEV_LDS, w, 5);
+  double *R2 = add_mat(LL.M, LEV_LDS, w, 6), *T2 = add_mat(LL.M, LEV_LDS, w, 7);
+  double *G = add_mat(LL.M, LEV_LDS, w, 8), *X = add_mat(LL.M, LEV_LDS, w, 9);
   double *USp = sv[2], *S2p = sv[3], *S2m = sv[4], *dg = sv[5], *vs = sv[6];
   double *sky = sv[7], *gnd = sv[8], *ur = sv[9], *dr = sv[10];
   const double *E = w.sfc_emis + ((size_t)col * d.nf + f) * 2 * NMU;
@@ -1820,11 +1970,15 @@ void launch_adding(const Dims &d, const Settings &s, const double *freqs, const 
                    Work &w, double *tb, hipStream_t st) {
   size_t n = (size_t)d.ncol_chunk * d.nf;
   if (n == 0) return;
+  // at N = 64 the kernels use the global scratch of their slot: at most one
+  // workgroup per CU, so blockIdx.x names the slot
+  const size_t slots = RT_GMATS > 0 ? (size_t)std::max(1, s.ncu) : (size_t)1 << 30;
   if (s.inverse_gj == 1) {
-    hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, 16384)), dim3(RT_WG), 0, st,
-                       d, s, freqs, sfc_refl, w, tb, 1);
+    hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, std::min<size_t>(16384, slots))),
+                       dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb, 1);
   } else {
-    const dim3 grid((unsigned)persistent_grid(n, s.pers_add, s.ncu));
+    dim3 grid((unsigned)persistent_grid(n, s.pers_add, s.ncu));
+    if (RT_GMATS > 0) grid.x = (unsigned)std::min(n, slots);
     const bool pers = grid.x < n, mx = s.precision == PAM_PREC_MIXED;
     if (pers && mx)
       hipLaunchKernelGGL((k_adding<true, true>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
@@ -1838,12 +1992,12 @@ void launch_adding(const Dims &d, const Settings &s, const double *freqs, const 
     else
       hipLaunchKernelGGL((k_adding<false, false>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
                          tb);
-    hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, 2048)), dim3(RT_WG), 0, st, d,
-                       s, freqs, sfc_refl, w, tb, 0);
+    hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, std::min<size_t>(2048, slots))),
+                       dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb, 0);
   }
   // units with output levels inside the column (none with the default levels)
-  hipLaunchKernelGGL(k_adding_levels, dim3((unsigned)std::min<size_t>(n, 1024)), dim3(RT_WG), 0, st,
-                     d, s, freqs, sfc_refl, w, tb);
+  hipLaunchKernelGGL(k_adding_levels, dim3((unsigned)std::min<size_t>(n, std::min<size_t>(1024, slots))),
+                     dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb);
 }
 
 __global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v) {

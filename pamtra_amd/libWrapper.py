@@ -49,9 +49,9 @@ def PamtraFortranWrapper(sets, nmlSets, descriptorFile, descriptorFile4D, descri
     nml = dict(nmlSets)
     if not nml.get("passive", True):
         raise NotImplementedError("passive=False leaves nothing for this library to compute")
-    lib = library()
-    ctx = get_context(0)
     m = _capi.marshal(nml, sets, descriptorFile, descriptorFile4D, profile)
+    lib = library(m.nummu)
+    ctx = get_context(0, m.nummu)
     freqs, fp = _capi.freqs_array(sets["freqs"])
     hr = _capi.alloc_results(m, len(freqs), diagnostics=bool(nml.get("save_ssp", False)))
     err = lib.pam_run(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),

@@ -12,6 +12,10 @@ device for ``pam_run_device``.
            spheres, 50-300 kg/m3 per column) and graupel (400 kg/m3).
   serving_requests: the serving mix, requests of 1-4096 columns with
            20-150 layers, 1-12 channels and 0-4 of the config3 categories.
+  config5: the large domain (1024 x 1024 columns in full) of 150 layers, 12
+           channels 10.65-325 GHz, the config3 categories, 32 quadrature
+           angles per hemisphere (64x64 operators); built for a column
+           sample of any size.
 Random draws use numpy.random.default_rng(seed), fields in a fixed order.
 """
 
@@ -253,3 +257,15 @@ def serving_requests(n, seed=1004, max_cols=4096, nbin=CONFIG3_NBIN):
         pam = _profile_columns(rng, ncol, nlyrs, tops, ML, nbin, present)
         out.append((pam, freqs))
     return out
+
+
+def config5(ncol=1024 * 1024, seed=1005, nbin=CONFIG3_NBIN, freqs=None):
+    """Large domain: 150 layers, tops 10-20 km, 12 channels, four categories,
+    32 angles per hemisphere (pyPamtra's _nangles = 32)."""
+    rng = np.random.default_rng(seed)
+    ML = 150
+    nlyrs = np.full(ncol, ML)
+    tops = rng.uniform(10000.0, 20000.0, size=ncol)
+    pam = _profile_columns(rng, ncol, nlyrs, tops, ML, nbin)
+    pam._nangles = 32
+    return pam, list(SERVING_FREQS if freqs is None else freqs)

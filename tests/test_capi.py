@@ -24,21 +24,37 @@ def test_header_declares_the_exported_set():
     assert _declared_functions() == sorted(_capi.EXPORTED_SYMBOLS)
 
 
-def test_library_loads_and_exports_every_symbol():
-    assert os.path.exists(_capi.LIBRARY_PATH), "build the library first"
-    lib = C.CDLL(_capi.LIBRARY_PATH)
+@pytest.mark.parametrize("nummu", sorted(_capi.LIBRARIES))
+def test_library_loads_and_exports_every_symbol(nummu):
+    path = _capi.library_path(nummu)
+    assert os.path.exists(path), "build the library first"
+    lib = C.CDLL(path)
     for name in _declared_functions():
         assert hasattr(lib, name), name
-    out = subprocess.run(["nm", "-D", "--defined-only", _capi.LIBRARY_PATH],
+    out = subprocess.run(["nm", "-D", "--defined-only", path],
                          capture_output=True, text=True, check=True).stdout
     for name in _declared_functions():
         assert re.search(r"\sT\s" + name + r"$", out, re.M), name
 
 
-def test_version_and_device_count_without_gpu():
-    lib = _capi.declare_library(C.CDLL(_capi.LIBRARY_PATH))
+@pytest.mark.parametrize("nummu", sorted(_capi.LIBRARIES))
+def test_version_and_device_count_without_gpu(nummu):
+    lib = _capi.declare_library(C.CDLL(_capi.library_path(nummu)))
     assert lib.pam_version().startswith(b"pamtra_amd")
     assert lib.pam_device_count() >= 0
+    assert lib.pam_nummu() == nummu
+
+
+def test_angle_count_selects_the_library_and_result_shapes():
+    from pamtra_amd import workloads
+    pam, freqs = workloads.config5(ncol=2)
+    m = _capi.marshal(pam.nmlSet, dict(pam.set, _nangles=pam._nangles), pam.df.data,
+                      pam.df.data4D, pam.p)
+    hr = _capi.alloc_results(m, len(freqs))
+    assert m.nummu == 32 and hr.arrays["tb"].shape == (2, 1, 2, 64, 12, 2)
+    assert hr.arrays["emissivity"].shape == (2, 1, 2, 12, 32)
+    with pytest.raises(NotImplementedError):
+        _capi.marshal(pam.nmlSet, dict(pam.set, _nangles=24), pam.df.data, pam.df.data4D, pam.p)
 
 
 STRUCTS = {

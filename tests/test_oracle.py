@@ -43,9 +43,10 @@ def test_tkc_known_answer():
     np.testing.assert_allclose([e.real, e.imag], [60.6084, 32.9184], rtol=1e-3)
 
 
-def test_lobatto_nodes_are_roots_of_legendre_derivative():
-    mu, w = oracle.lobatto(16)
-    n = 32
+@pytest.mark.parametrize("nummu", [16, 32])
+def test_lobatto_nodes_are_roots_of_legendre_derivative(nummu):
+    mu, w = oracle.lobatto(nummu)
+    n = 2 * nummu
     # interior Lobatto nodes of the 2N-point rule: roots of P'_{n-1}
     c = np.zeros(n)
     c[-1] = 1.0
@@ -142,8 +143,8 @@ def test_scatcnv_normalisation_and_symmetry():
 def _exact_layer(smat, kext, h):
     """R and T of a homogeneous source-free layer from the exponential of
     the two-stream-set generator (interaction principle)."""
-    mu, w = oracle.lobatto(16)
-    n = 32
+    n = smat.shape[-1]
+    mu, w = oracle.lobatto(n // 2)
     muv = np.repeat(mu, 2)
     wv = np.repeat(w, 2)
     Minv = np.diag(1 / muv)
@@ -157,11 +158,14 @@ def _exact_layer(smat, kext, h):
     return -P22i @ P21, P22i
 
 
-@pytest.mark.parametrize("tau,salb", [(0.05, 0.5), (0.4, 0.9), (1.2, 0.3)])
-def test_doubling_matches_matrix_exponential(tau, salb):
+# At 32 angles the smallest node is mu = 0.0126, so exp(tau / mu) of the
+# generator exceeds the double range of the brute force beyond tau ~ 0.5.
+@pytest.mark.parametrize("tau,salb,nummu", [(0.05, 0.5, 16), (0.4, 0.9, 16), (1.2, 0.3, 16),
+                                            (0.05, 0.5, 32), (0.4, 0.9, 32), (0.3, 0.3, 32)])
+def test_doubling_matches_matrix_exponential(tau, salb, nummu):
     coef = _hg_coefs(16, 0.5)
     kext = 1e-3
-    smat = oracle.scatcnv(16, coef, kext, salb)
+    smat = oracle.scatcnv(16, coef, kext, salb, nummu)
     h = tau / kext
     st, R, T, sp, sm, nd = oracle.layer_doubling(smat, kext, salb, 0.0, h, 0.0, 0.0, 3000.0)
     assert st == 0 and nd > 10
@@ -386,7 +390,8 @@ def _with_levels(pam, heights):
     return pam
 
 
-def test_interior_output_levels_clear_column_closed_form():
+@pytest.mark.parametrize("nummu", [16, 32])
+def test_interior_output_levels_clear_column_closed_form(nummu):
     """Output levels inside the column (radtran4.f90:375-422, run_rt.f90:166-184):
     a clear, gas-absorbing column over a specular surface of emissivity 0.7.
     Up- and downwelling TB at every level against the radiative transfer
@@ -407,10 +412,10 @@ def test_interior_output_levels_clear_column_closed_form():
         heights.append(obs)
         expect_lev.append([30, 0, 10, 5, 21, 16, 30, 0])
     pam = _with_levels(pam, heights)
-    out, st, _ = oracle.run(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p, freqs,
-                            nthreads=2, diagnostics=True)
+    out, st, _ = oracle.run(passive_nml(pam), dict(pam.set, _nangles=nummu), pam.df.data,
+                            pam.df.data4D, pam.p, freqs, nthreads=2, diagnostics=True)
     assert st == 0
-    mu, _ = oracle.lobatto()
+    mu, _ = oracle.lobatto(nummu)
     wl = 299792458.0 / (freqs[0] * 1e3)
     for c in range(2):
         t_lev = pam.p["temp_lev"][c, 0]
@@ -419,7 +424,7 @@ def test_interior_output_levels_clear_column_closed_form():
         assert abs(_planck(t_lev[3], wl) - oracle.planck(t_lev[3], wl)) < 1e-13 * _planck(t_lev[3], wl)
         bl = _planck(t_lev[:L + 1], wl)
         bg = _planck(pam.p["groundtemp"][c, 0], wl)
-        for j in range(16):
+        for j in range(nummu):
             m = mu[j]
             down = [_planck(2.73, wl)]  # downwelling at level L, L-1, ..., 0
             for z in range(L - 1, -1, -1):
@@ -432,8 +437,8 @@ def test_interior_output_levels_clear_column_closed_form():
                 tb_up = oracle.rad2tb(up[k], wl)
                 tb_dn = oracle.rad2tb(down[k], wl)
                 got = out["tb"][c, 0, io, :, 0, :]
-                np.testing.assert_allclose(got[15 - j], tb_up, atol=2e-7, rtol=0)
-                np.testing.assert_allclose(got[16 + j], tb_dn, atol=2e-7, rtol=0)
+                np.testing.assert_allclose(got[nummu - 1 - j], tb_up, atol=2e-7, rtol=0)
+                np.testing.assert_allclose(got[nummu + j], tb_dn, atol=2e-7, rtol=0)
 
 
 def test_interior_level_above_transparent_layers_sees_the_surface_level():
@@ -453,3 +458,29 @@ def test_interior_level_above_transparent_layers_sees_the_surface_level():
     np.testing.assert_allclose(tb[:, 2], tb[:, 1], rtol=0, atol=1e-9)
     np.testing.assert_allclose(tb[:, 3], tb[:, 1], rtol=0, atol=1e-9)
     assert np.all(np.abs(tb[:, 0, :16] - tb[:, 1, :16]) > 1e-3)  # scattering above the level
+
+
+def test_thirty_two_angles_converge_at_nadir():
+    """The 32-angle build (nummu = 32, as a recompiled reference would run)
+    on the golden scenario: every TB at 64 angles is finite, the two nadir
+    TB (mu = 1, up at the top, down at the surface) agree with the 16-angle
+    build's within the quadrature's convergence, and an isothermal column
+    over a black surface at the same temperature gives that temperature at
+    all 64 angles."""
+    pam = build_pamtra()
+    a, st = _run(pam, FREQUENCIES)
+    b, st2, _ = oracle.run(passive_nml(pam), dict(pam.set, _nangles=32), pam.df.data,
+                           pam.df.data4D, pam.p, FREQUENCIES)
+    assert st == st2 == 0 and b["tb"].shape == (1, 1, 2, 64, 2, 2)
+    assert np.all(np.isfinite(b["tb"]))
+    np.testing.assert_allclose(b["tb"][0, 0, 0, 0], a["tb"][0, 0, 0, 0], atol=0.05)
+    np.testing.assert_allclose(b["tb"][0, 0, 1, 63], a["tb"][0, 0, 1, 31], atol=0.05)
+    T0 = 265.0
+    pam.p["temp_lev"][:] = T0
+    pam.p["groundtemp"] = np.full((1, 1), T0)
+    pam.nmlSet["emissivity"] = 1.0
+    pam.p["hydro_q"][:] = 0.0
+    c, st3, _ = oracle.run(passive_nml(pam), dict(pam.set, _nangles=32), pam.df.data,
+                           pam.df.data4D, pam.p, [35.5])
+    assert st3 == 0
+    np.testing.assert_allclose(c["tb"][0, 0, :, :32], T0, atol=1e-9)
