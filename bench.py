@@ -63,6 +63,16 @@ def to_device(pam, torch):
     return prof
 
 
+def to_device_4d(pam, torch, ncol=None):
+    """The 4D descriptor arrays (nx, ny, L, nhydro) in HBM: pam_run_device
+    reads every array from the device."""
+    out = {}
+    for k, v in pam.df.data4D.items():
+        a = np.ascontiguousarray(v, dtype=np.int32 if k == "nbin" else np.float64)
+        out[k] = torch.as_tensor(a if ncol is None else a[:ncol].copy(), device="cuda")
+    return out
+
+
 # HBM bytes per launch of the dominant kernel on the default workload, from the
 # two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a copy of known
 # size; recorded in this summary with the commands that produced it.
@@ -117,6 +127,71 @@ def serve(args, precision="fp64", tuning=None, rate=50.0):
     return summ
 
 
+def large_domain(args, torch):
+    """The large-domain configuration (150 layers, 12 channels 10.65-325 GHz,
+    four categories of 100 bins, 32 angles per hemisphere: 64x64 operators)
+    on a column sample, FP64 and mixed; the full 1024 x 1024 domain streams
+    the same calls in HBM-admitted column chunks."""
+    from pamtra_amd import _capi, _lib, workloads
+    t0 = time.time()
+    pam, freqs = workloads.config5(ncol=args.large_cols, seed=1005)
+    log(f"large-domain sample built in {time.time() - t0:.1f} s: {args.large_cols} columns")
+    lib = _lib.library(32)
+    ctx = _lib.get_context(0, 32)
+    prof = to_device(pam, torch)
+    d4 = to_device_4d(pam, torch)
+    nml = dict(pam.nmlSet)
+    nf = len(freqs)
+    f, fp = _capi.freqs_array(freqs)
+    res = {}
+    tbs = {}
+    for precision in ("fp64", "mixed"):
+        sets = dict(pam.set, _nangles=32, precision=precision)
+        m = _capi.marshal(nml, sets, pam.df.data, d4, prof)
+        nx, ny, nout = m.nx, m.ny, m.noutlevels
+        tb = torch.zeros((nx, ny, nout, 64, nf, 2), dtype=torch.float64, device="cuda")
+        status = torch.zeros((nx, ny, nf), dtype=torch.int32, device="cuda")
+        r = _capi.PamResults()
+        r.tb = C.cast(C.c_void_p(tb.data_ptr()), C.POINTER(C.c_double))
+        r.status = C.cast(C.c_void_p(status.data_ptr()), C.POINTER(C.c_int32))
+        msg = C.create_string_buffer(512)
+        r.msg = C.cast(msg, C.c_char_p)
+        r.msglen = 512
+
+        def call():
+            if lib.pam_run_device(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),
+                                  C.byref(m.profile), fp, nf, C.byref(r), None) == _capi.PAM_FATAL:
+                raise RuntimeError("pam_run_device (32 angles): " + msg.value.decode())
+
+        call()
+        torch.cuda.synchronize()
+        names = lib.pam_stage_names().decode().split(",")
+        acc = np.zeros(len(names))
+        t0 = time.perf_counter()
+        for i in range(args.large_steps):
+            call()
+            ms = (C.c_double * len(names))()
+            lib.pam_last_stage_times(ctx, ms, len(names))
+            acc += np.array(ms[:])
+            log(f"large-domain {precision} step {i} done")
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.large_steps
+        tbs[precision] = tb.cpu().numpy()
+        res[precision] = {"value": round(nx * ny * nf / dt, 2), "ms_per_step": round(dt * 1e3, 3),
+                          "stage_ms": {n: round(v / args.large_steps, 3)
+                                       for n, v in zip(names, acc)},
+                          "status_nonzero": int((status.cpu().numpy() != 0).sum())}
+    dch = np.max(np.abs(tbs["mixed"] - tbs["fp64"]), axis=(0, 1, 2, 3, 5))
+    res["mixed"]["max_abs_dtb_vs_fp64_K"] = {f"{fr:g}": float(v) for fr, v in zip(freqs, dch)}
+    res["mixed"]["speedup_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+    res["workload"] = (f"large domain sample: {args.large_cols} of the 1024x1024 columns, 150 "
+                       "layers, 12 channels 10.65-325 GHz, cloud/ice/snow/graupel 100 bins, "
+                       "NSTOKES=2, 32 angles (64x64 operators)")
+    res["full_domain_s_fp64"] = round(1024 * 1024 * nf / res["fp64"]["value"], 1)
+    log(f"large domain: {res}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +220,9 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also fill and then sum a buffer of MIB MiB (kernels of known "
                          "HBM traffic, to scale the HBM counters)")
+    ap.add_argument("--large-cols", type=int, default=2048,
+                    help="columns of the large-domain (32-angle) sample (0: skip it)")
+    ap.add_argument("--large-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the columns are "
@@ -167,8 +245,9 @@ def main():
     lib = _lib.library()
     ctx = _lib.get_context(0)
     prof = to_device(pam, torch)
+    d4 = to_device_4d(pam, torch)
     nml = dict(pam.nmlSet)
-    m = _capi.marshal(nml, pam.set, pam.df.data, pam.df.data4D, prof)
+    m = _capi.marshal(nml, pam.set, pam.df.data, d4, prof)
     nx, ny, nout, nf = m.nx, m.ny, m.noutlevels, len(freqs)
     tb = torch.zeros((nx, ny, nout, 32, nf, 2), dtype=torch.float64, device="cuda")
     status = torch.zeros((nx, ny, nf), dtype=torch.int32, device="cuda")
@@ -196,7 +275,7 @@ def main():
         ps = to_device(sub, torch)
         sets_s = dict(pam.set)
         sets_s["precision"] = precision
-        ms_ = _capi.marshal(nml, sets_s, pam.df.data, pam.df.data4D, ps)
+        ms_ = _capi.marshal(nml, sets_s, pam.df.data, to_device_4d(pam, torch, ns), ps)
         tb_s = torch.zeros((ns, ny, nout, 32, nf, 2), dtype=torch.float64, device="cuda")
         st_s = torch.zeros((ns, ny, nf), dtype=torch.int32, device="cuda")
         rs = _capi.PamResults()
@@ -287,7 +366,7 @@ def main():
     if not args.no_mixed:
         sets_m = dict(pam.set)
         sets_m["precision"] = "mixed"
-        mm = _capi.marshal(nml, sets_m, pam.df.data, pam.df.data4D, prof)
+        mm = _capi.marshal(nml, sets_m, pam.df.data, d4, prof)
 
         def step_m():
             st = lib.pam_run_device(ctx, C.byref(mm.settings), mm.hydros, C.byref(mm.hydro4d),
@@ -371,6 +450,9 @@ def main():
         if not args.no_mixed:
             out["serving"]["mixed"] = [serve(args, "mixed", tuning=tuning and tuning.get("mixed"),
                                              rate=r) for r in rates]
+
+    if args.large_cols > 0:
+        out["large_domain"] = large_domain(args, torch)
 
     if not args.no_cpu:
         import oracle

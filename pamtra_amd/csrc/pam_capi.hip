@@ -920,9 +920,75 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   return worst;
 }
 
+// True when the device can dereference ptr: memory from hipMalloc (torch's
+// allocator included), managed memory, or pinned host memory mapped into the
+// device's address space.  A plain host pointer reads as unregistered; the
+// error that query leaves is cleared.
+static bool device_accessible(const void *ptr) {
+  hipPointerAttribute_t a;
+  memset(&a, 0, sizeof(a));
+  const hipError_t e = hipPointerGetAttributes(&a, ptr);
+  (void)hipGetLastError();
+  if (e != hipSuccess) return false;
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged ||
+         (a.type == hipMemoryTypeHost && a.devicePointer != nullptr);
+}
+
+// The arrays of a pam_run_device call that the kernels read or write: every
+// one must be device memory, or the call is refused before any launch (a
+// host array would fault the GPU).  Returns the offending field or NULL.
+static const char *first_host_array(const pam_hydro_4d *h4, const pam_profile *p,
+                                    const pam_results *r) {
+  const struct {
+    const void *ptr;
+    const char *name;
+  } arrays[] = {
+      {p->nlyrs, "profile.nlyrs"},           {p->hgt_lev, "profile.hgt_lev"},
+      {p->temp_lev, "profile.temp_lev"},     {p->press_lev, "profile.press_lev"},
+      {p->relhum_lev, "profile.relhum_lev"}, {p->hgt, "profile.hgt"},
+      {p->temp, "profile.temp"},             {p->press, "profile.press"},
+      {p->relhum, "profile.relhum"},         {p->hydro_q, "profile.hydro_q"},
+      {p->hydro_n, "profile.hydro_n"},       {p->hydro_reff, "profile.hydro_reff"},
+      {p->groundtemp, "profile.groundtemp"}, {p->wind10u, "profile.wind10u"},
+      {p->wind10v, "profile.wind10v"},       {p->sfc_salinity, "profile.sfc_salinity"},
+      {p->sfc_emissivity, "profile.sfc_emissivity"},
+      {p->sfc_type, "profile.sfc_type"},     {p->sfc_model, "profile.sfc_model"},
+      {p->sfc_refl, "profile.sfc_refl"},     {p->obs_height, "profile.obs_height"},
+      {h4 ? h4->as_ratio : nullptr, "hydro4d.as_ratio"},
+      {h4 ? h4->rho_ms : nullptr, "hydro4d.rho_ms"},
+      {h4 ? h4->a_ms : nullptr, "hydro4d.a_ms"},
+      {h4 ? h4->b_ms : nullptr, "hydro4d.b_ms"},
+      {h4 ? h4->alpha_as : nullptr, "hydro4d.alpha_as"},
+      {h4 ? h4->beta_as : nullptr, "hydro4d.beta_as"},
+      {h4 ? h4->p_1 : nullptr, "hydro4d.p_1"},     {h4 ? h4->p_2 : nullptr, "hydro4d.p_2"},
+      {h4 ? h4->p_3 : nullptr, "hydro4d.p_3"},     {h4 ? h4->p_4 : nullptr, "hydro4d.p_4"},
+      {h4 ? h4->d_1 : nullptr, "hydro4d.d_1"},     {h4 ? h4->d_2 : nullptr, "hydro4d.d_2"},
+      {h4 ? h4->nbin : nullptr, "hydro4d.nbin"},
+      {r->tb, "results.tb"},                 {r->emissivity, "results.emissivity"},
+      {r->angles_deg, "results.angles_deg"}, {r->status, "results.status"},
+      {r->kextatmo, "results.kextatmo"},     {r->kexttot, "results.kexttot"},
+      {r->salbedo, "results.salbedo"},       {r->nlegen, "results.nlegen"},
+      {r->ndoubles, "results.ndoubles"},     {r->ze, "results.ze"},
+      {r->att_hydro, "results.att_hydro"},   {r->att_atmo, "results.att_atmo"},
+      {r->radar_hgt, "results.radar_hgt"},
+  };
+  for (const auto &a : arrays)
+    if (a.ptr && !device_accessible(a.ptr)) return a.name;
+  return nullptr;
+}
+
 extern "C" int pam_run_device(pam_context *c, const pam_settings *st, const pam_hydro_desc *hyd,
                               const pam_hydro_4d *h4, const pam_profile *p, const double *freqs,
                               int32_t nfreq, pam_results *res, void *stream) {
+  if (c && p && res) {
+    if (hipSetDevice(c->device) != hipSuccess) return PAM_FATAL;
+    if (const char *bad = first_host_array(h4, p, res)) {
+      char m[160];
+      snprintf(m, sizeof(m), "pam_run_device: %s is not device memory", bad);
+      set_msg(res, m);
+      return PAM_FATAL;
+    }
+  }
   return run_device(c, st, hyd, h4, p, freqs, nfreq, res, (hipStream_t)stream);
 }
 

@@ -282,3 +282,73 @@ def test_level_asked_twice_and_mixed_precision():
     pam.set["precision"] = "fp64"
     assert st3 == 0
     assert np.max(np.abs(a["tb"] - b["tb"])) <= 0.05
+
+
+class _PtrAttr(C.Structure):
+    _fields_ = [("type", C.c_int), ("device", C.c_int), ("devicePointer", C.c_void_p),
+                ("hostPointer", C.c_void_p), ("isManaged", C.c_int), ("allocationFlags", C.c_uint)]
+
+
+def _device_profile(pam, torch):
+    prof = {}
+    for k, v in pam.p.items():
+        if isinstance(v, np.ndarray) and v.dtype.kind in "fiu" and k in (
+                _capi.PROFILE_DOUBLE_KEYS + ["nlyrs", "sfc_type", "sfc_model", "obs_height"]):
+            dt = torch.int32 if k in ("nlyrs", "sfc_type", "sfc_model") else torch.float64
+            prof[k] = torch.as_tensor(np.ascontiguousarray(v), dtype=dt, device="cuda")
+        else:
+            prof[k] = v
+    refl = np.frombuffer("".join(pam.p["sfc_refl"].reshape(-1)).encode(), dtype=np.uint8)
+    prof["sfc_refl"] = torch.as_tensor(refl.copy(), device="cuda")
+    return prof
+
+
+def _run_device(lib, ctx, m, freqs, torch):
+    nx, ny, nout, nf = m.nx, m.ny, m.noutlevels, len(freqs)
+    tb = torch.zeros((nx, ny, nout, 32, nf, 2), dtype=torch.float64, device="cuda")
+    status = torch.zeros((nx, ny, nf), dtype=torch.int32, device="cuda")
+    r = _capi.PamResults()
+    r.tb = C.cast(C.c_void_p(tb.data_ptr()), C.POINTER(C.c_double))
+    r.status = C.cast(C.c_void_p(status.data_ptr()), C.POINTER(C.c_int32))
+    msg = C.create_string_buffer(256)
+    r.msg = C.cast(msg, C.c_char_p)
+    r.msglen = 256
+    f, fp = _capi.freqs_array(freqs)
+    st = lib.pam_run_device(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),
+                            C.byref(m.profile), fp, nf, C.byref(r), None)
+    return st, tb, msg.value
+
+
+def test_device_entry_with_4d_descriptors_and_host_array_refusal():
+    """The 4D descriptor (per-column snow density) read from HBM gives
+    pam_run's result; the same call with that array left in host memory is
+    refused before any launch.  The refusal is exercised only after the HIP
+    runtime has been seen to classify the host array as host memory."""
+    torch = pytest.importorskip("torch")
+    from pamtra_amd import _lib
+    pam, freqs = workloads.config3(ncol=6, seed=13)
+    freqs = freqs[:3]
+    assert "rho_ms" in pam.df.data4D
+    host, _ = _gpu(pam, freqs, diagnostics=False)
+    lib = _lib.library()
+    ctx = _lib.get_context(0)
+    prof = _device_profile(pam, torch)
+    d4 = {k: torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64), device="cuda")
+          for k, v in pam.df.data4D.items()}
+    m = _capi.marshal(passive_nml(pam), pam.set, pam.df.data, d4, prof)
+    print("pam_run_device with HBM 4D descriptors", flush=True)
+    st, tb, msg = _run_device(lib, ctx, m, freqs, torch)
+    assert st == 0, msg
+    np.testing.assert_array_equal(tb.cpu().numpy(), host["tb"])
+    h4 = np.ascontiguousarray(pam.df.data4D["rho_ms"], dtype=np.float64)
+    hip = C.CDLL("libamdhip64.so.7")
+    a = _PtrAttr()
+    e = hip.hipPointerGetAttributes(C.byref(a), C.c_void_p(h4.ctypes.data))
+    hip.hipGetLastError()
+    if e == 0 and (a.type in (2, 3) or (a.type == 1 and a.devicePointer)):
+        pytest.skip("the HIP runtime reports this host array as device-accessible")
+    mh = _capi.marshal(passive_nml(pam), pam.set, pam.df.data, {"rho_ms": h4}, prof)
+    print("pam_run_device with a host 4D array (refused before launch)", flush=True)
+    st2, _, msg2 = _run_device(lib, ctx, mh, freqs, torch)
+    assert st2 == _capi.PAM_FATAL
+    assert b"hydro4d.rho_ms is not device memory" in msg2
