@@ -177,10 +177,22 @@ def large_domain(args, torch):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.large_steps
         tbs[precision] = tb.cpu().numpy()
+        work = (C.c_int64 * 12)()
+        lib.pam_last_work(ctx, work, 12)
+        stage = {n: v / args.large_steps for n, v in zip(names, acc)}
         res[precision] = {"value": round(nx * ny * nf / dt, 2), "ms_per_step": round(dt * 1e3, 3),
-                          "stage_ms": {n: round(v / args.large_steps, 3)
-                                       for n, v in zip(names, acc)},
-                          "status_nonzero": int((status.cpu().numpy() != 0).sum())}
+                          "stage_ms": {n: round(v, 3) for n, v in stage.items()},
+                          "status_nonzero": int((status.cpu().numpy() != 0).sum()),
+                          "doubling_steps": work[3], "doubling_products": work[9],
+                          "adding_products": work[10]}
+        if precision == "fp64":
+            # the reference's algorithm: 14 N^3 flops per doubling step, N = 64
+            ach = work[3] * 14 * 64 ** 3 / (stage["doubling"] * 1e-3) / 1e12
+            res[precision]["doubling_roofline"] = {
+                "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP64_TFLOPS, 4),
+                "mfma_executed_tflops": round(work[9] * 2 * 64 ** 3 / (stage["doubling"] * 1e-3)
+                                              / 1e12, 3)}
     dch = np.max(np.abs(tbs["mixed"] - tbs["fp64"]), axis=(0, 1, 2, 3, 5))
     res["mixed"]["max_abs_dtb_vs_fp64_K"] = {f"{fr:g}": float(v) for fr, v in zip(freqs, dch)}
     res["mixed"]["speedup_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
@@ -223,6 +235,8 @@ def main():
     ap.add_argument("--large-cols", type=int, default=2048,
                     help="columns of the large-domain (32-angle) sample (0: skip it)")
     ap.add_argument("--large-steps", type=int, default=2)
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the large-domain (32-angle) sample and print its summary")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the columns are "
@@ -233,6 +247,9 @@ def main():
     import torch
     from pamtra_amd import _capi, _lib, workloads
 
+    if args.large_only:
+        print(json.dumps({"large_domain": large_domain(args, torch)}))
+        return
     rates = [float(x) for x in str(args.serve_rate).split(",") if x.strip()]
     if args.serve_only:
         print(json.dumps({"serving": {"fp64": [serve(args, rate=r) for r in rates],

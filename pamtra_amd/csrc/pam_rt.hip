@@ -40,9 +40,13 @@ __device__ __forceinline__ int tile_j(int wv, int q) { return ((wv + 4 * q) % NT
 // vectors at once (4 x NN rows), RPL lanes per row for one vector (NN rows)
 constexpr int VL = RT_WG / (4 * NN);
 constexpr int RPL = RT_WG / NN;
-// unrolling of the k loops of the products: complete at N = 32; by 2 at
-// N = 64, where TPW = 4 tiles per wave already give 4 independent chains
+// unrolling of the k loops of the products: complete at N = 32.  At N = 64,
+// where TPW = 4 tiles per wave already give 4 independent chains, by 8 in the
+// doubling kernels (one workgroup per CU: the loads of the next steps must
+// be in flight) and by 2 in the adding kernels (their other registers leave
+// no room for more)
 constexpr int KUNR = (NN == 32) ? NN / 4 : 2;
+constexpr int KUNR_DBL = (NN == 32) ? NN / 4 : 8;
 
 // ------------------------------------------------------------------
 // LDS dense helpers; every thread of the workgroup calls them
@@ -71,6 +75,7 @@ __device__ __forceinline__ float wave_max(float v) {
 // four 16-lane groups).  Every lane has read all its operands before the
 // first write, so C may alias A, B or D.  With wmax (4 LDS floats) the
 // return value is max |C_ij| (NaN if any entry is NaN), else 0.
+template <int U = KUNR>
 __device__ __forceinline__ double mm_core(const double *A, const double *B, const double *D,
                                           double *C, float *wmax) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
@@ -82,7 +87,7 @@ __device__ __forceinline__ double mm_core(const double *A, const double *B, cons
 #pragma unroll
     for (int r = 0; r < 4; r++) acc[q][r] = D ? D[(I + 4 * r + lk) * LD + J + li] : 0.0;
   }
-#pragma unroll KUNR
+#pragma unroll U
   for (int s = 0; s < NN / 4; s++) {
 #pragma unroll
     for (int q = 0; q < TPW; q++) {
@@ -114,6 +119,7 @@ __device__ __forceinline__ double mm_core(const double *A, const double *B, cons
 // (D1, D2 may be NULL).  Each wave interleaves the two MFMA chains of its
 // tiles; all operands are read before the first write, so any output may
 // alias any input.  With wmax the return value is max |C2_ij|.
+template <int U = KUNR>
 __device__ __forceinline__ double mm2(const double *A1, const double *B1, const double *D1,
                                       double *C1, const double *A2, const double *B2,
                                       const double *D2, double *C2, float *wmax) {
@@ -129,7 +135,7 @@ __device__ __forceinline__ double mm2(const double *A1, const double *B1, const 
       acc2[q][r] = D2 ? D2[(I + 4 * r + lk) * LD + J + li] : 0.0;
     }
   }
-#pragma unroll KUNR
+#pragma unroll U
   for (int s = 0; s < NN / 4; s++) {
 #pragma unroll
     for (int q = 0; q < TPW; q++) {
@@ -164,14 +170,16 @@ __device__ __forceinline__ double mm2(const double *A1, const double *B1, const 
 }
 
 // C = A * B
+template <int U = KUNR>
 __device__ __forceinline__ void mm(const double *A, const double *B, double *C) {
-  mm_core(A, B, nullptr, C, nullptr);
+  mm_core<U>(A, B, nullptr, C, nullptr);
 }
 
 // C = D + A * B
+template <int U = KUNR>
 __device__ __forceinline__ void mm_add(const double *D, const double *A, const double *B,
                                        double *C) {
-  mm_core(A, B, D, C, nullptr);
+  mm_core<U>(A, B, D, C, nullptr);
 }
 
 // row i of A x for a 32-vector (no barrier)
@@ -310,11 +318,12 @@ constexpr int NEU_MAXIT = 20;
 // of the doubling, where X is ~1e-12).  Returns false if it has not
 // converged after NEU_MAXIT factors (spectral radius within ~1e-6 of one) or
 // X holds a NaN.  np counts the 32x32 products executed.
+template <int U = KUNR>
 __device__ __forceinline__ bool series_inv(const double *A, const double *B, double *X, double *G,
                                            InvLds &sh, int &np) {
   const int t = threadIdx.x;
   np++;
-  double m = mm_core(A, B, nullptr, X, sh.wmax);
+  double m = mm_core<U>(A, B, nullptr, X, sh.wmax);
   for (int e = t; e < NN2; e += RT_WG) {
     const int i = e / NN, j = e % NN;
     G[i * LD + j] = (i == j ? 1.0 : 0.0) + X[i * LD + j];
@@ -324,17 +333,17 @@ __device__ __forceinline__ bool series_inv(const double *A, const double *B, dou
   // two products fused while the next factor still matters
   if (!(m < 0x1p64)) return false;          // NaN or overflow
   if ((double)NN * m * m < NEU_TOL) return true;  // bound on max |P^2|
-  m = mm_core(X, X, nullptr, X, sh.wmax);
+  m = mm_core<U>(X, X, nullptr, X, sh.wmax);
   np++;
   for (int it = 0; it < NEU_MAXIT; it++) {
     if (!(m < 0x1p64)) return false;
     if (m < NEU_TOL) return true;
     if ((double)NN * m * m < NEU_TOL) {  // last factor
-      mm_add(G, G, X, G);
+      mm_add<U>(G, G, X, G);
       np++;
       return true;
     }
-    m = mm2(G, X, G, G, X, X, nullptr, X, sh.wmax);
+    m = mm2<U>(G, X, G, G, X, X, nullptr, X, sh.wmax);
     np += 2;
   }
   return false;
@@ -345,16 +354,16 @@ __device__ __forceinline__ bool series_inv(const double *A, const double *B, dou
 // queue pass): the series, and Gauss-Jordan with partial pivoting where the
 // series does not converge (Gauss-Jordan only when sh.gj_only); false means
 // a singular matrix.
-template <bool GJ>
+template <bool GJ, int U = KUNR>
 __device__ __forceinline__ bool inv_i_minus_ab(const double *A, const double *B, double *X,
                                                double *G, InvLds &sh, int &np) {
   if constexpr (GJ) {
-    if (!sh.gj_only && series_inv(A, B, X, G, sh, np)) return true;
+    if (!sh.gj_only && series_inv<U>(A, B, X, G, sh, np)) return true;
     np++;
-    mm(A, B, X);
+    mm<U>(A, B, X);
     return inv_i_minus(X, G, sh);
   } else {
-    return series_inv(A, B, X, G, sh, np);
+    return series_inv<U>(A, B, X, G, sh, np);
   }
 }
 
@@ -464,14 +473,14 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
   // ---- doubling (DOUBLING_INTEGRATION4, SYMMETRIC)
   int nprod = 0;
   for (int it = 0; it < nd; it++) {
-    if (!inv_i_minus_ab<GJ>(sR, sR, sX, sG, sinv, nprod)) {
+    if (!inv_i_minus_ab<GJ, KUNR_DBL>(sR, sR, sX, sG, sinv, nprod)) {
       if (!GJ) return false;
       if (t == 0) set_err(&w.itf_status[itf], E_SINGULAR);
       return true;
     }
     // R' = R + T G R T and T' = T G T with Z = T G and V = R T:
     // R' = R + Z V, T' = Z T (four products, two fused passes).
-    mm2(sT, sG, nullptr, sG, sR, sT, nullptr, sX, nullptr);  // Z = T G (into G), V = R T
+    mm2<KUNR_DBL>(sT, sG, nullptr, sG, sR, sT, nullptr, sX, nullptr);  // Z = T G (into G), V = R T
     // sources: S+' = (S+ + Z (S+ + R (S- + f C-))) + f C+, C+' = C+ + Z (C+ + R C-),
     // S-' = S- + Z ((S- + f C-) + R S+), C-' = C- + Z (C- + R C+)
     {
@@ -490,7 +499,7 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
       __syncthreads();
       if (lead) sv[g == 0 ? 0 : (g == 1 ? 2 : (g == 2 ? 1 : 3))][i] = nv;
     }
-    mm2(sG, sT, nullptr, sT, sG, sX, sR, sR, nullptr);  // T' = Z T, R' = R + Z V
+    mm2<KUNR_DBL>(sG, sT, nullptr, sT, sG, sX, sR, sR, nullptr);  // T' = Z T, R' = R + Z V
     nprod += 4;
     linfac = 2.0 * linfac;
   }
@@ -574,6 +583,7 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 // supplies A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+r][l%16];
 // each wave owns TPW output tiles.  With wmax the return value is max |C_ij|
 // (NaN propagates).
+template <int U = KUNR_DBL>
 __device__ __forceinline__ float mmf(const float *A, bool iA, const float *B, const float *D,
                                      float *C, float *wmax) {
   const int t = threadIdx.x, wv = t >> 6, l = t & 63;
@@ -585,7 +595,7 @@ __device__ __forceinline__ float mmf(const float *A, bool iA, const float *B, co
 #pragma unroll
     for (int r = 0; r < 4; r++) acc[q][r] = D ? D[(I + 4 * lk + r) * LD + J + li] : 0.f;
   }
-#pragma unroll KUNR
+#pragma unroll U
   for (int s = 0; s < NN / 4; s++) {
     const int kc = 4 * s + lk;
 #pragma unroll
@@ -617,6 +627,7 @@ __device__ __forceinline__ float mmf(const float *A, bool iA, const float *B, co
 
 // Two independent FP32 products in one pass (outputs may alias inputs);
 // with wmax the return value is max |C2_ij|.
+template <int U = KUNR_DBL>
 __device__ __forceinline__ float mmf2(const float *A1, bool iA1, const float *B1, const float *D1,
                                       float *C1, const float *A2, bool iA2, const float *B2,
                                       const float *D2, float *C2, float *wmax) {
@@ -632,7 +643,7 @@ __device__ __forceinline__ float mmf2(const float *A1, bool iA1, const float *B1
       acc2[q][r] = D2 ? D2[(I + 4 * lk + r) * LD + J + li] : 0.f;
     }
   }
-#pragma unroll KUNR
+#pragma unroll U
   for (int s = 0; s < NN / 4; s++) {
     const int kc = 4 * s + lk;
 #pragma unroll
@@ -729,22 +740,23 @@ __device__ bool series_f32(const float *R, float *X, float *Gt, float *wmax, int
 // form series, truncated when max |X^(2^k)| < tol.  false: not converged
 // after NEU_MAXIT factors, or a NaN.
 __device__ bool series_inv_given(double *X, double *G, float *wmax, double tol, int &np) {
+  constexpr int U = KUNR_DBL;
   const int t = threadIdx.x;
   for (int e = t; e < NN2; e += RT_WG) {
     const int i = e / NN, j = e % NN;
     G[i * LD + j] = (i == j ? 1.0 : 0.0) + X[i * LD + j];
   }
-  double m = mm_core(X, X, nullptr, X, wmax);  // its barriers order the G writes
+  double m = mm_core<U>(X, X, nullptr, X, wmax);  // its barriers order the G writes
   np++;
   for (int it = 0; it < NEU_MAXIT; it++) {
     if (!(m < 0x1p64)) return false;
     if (m < tol) return true;
     if ((double)NN * m * m < tol) {
-      mm_add(G, G, X, G);
+      mm_add<U>(G, G, X, G);
       np++;
       return true;
     }
-    m = mm2(G, X, G, G, X, X, nullptr, X, wmax);
+    m = mm2<U>(G, X, G, G, X, X, nullptr, X, wmax);
     np += 2;
   }
   return false;
@@ -916,8 +928,8 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   }
   // (the next product's barriers order these writes)
   int nprod = 7;
-  mm(M0, M1, M2);  // X
-  mm(M2, M2, M3);  // X^2
+  mm<KUNR_DBL>(M0, M1, M2);  // X
+  mm<KUNR_DBL>(M2, M2, M3);  // X^2
   for (int e = t; e < NN2; e += RT_WG) {
     const int i = e / NN, j = e % NN;
     const double id = (i == j) ? 1.0 : 0.0, x = M2[i * LD + j], x2 = M3[i * LD + j];
@@ -925,9 +937,9 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     M3[i * LD + j] = id + x * 0.5 + x2 * (1.0 / 24.0);           // c(X)
   }
   __syncthreads();
-  mm_add(M3, M2, M0, M3);  // U = c(X) + s(X) dz S
-  mm(M1, M0, M2);          // Y
-  mm(M2, M2, M0);          // Y^2
+  mm_add<KUNR_DBL>(M3, M2, M0, M3);  // U = c(X) + s(X) dz S
+  mm<KUNR_DBL>(M1, M0, M2);          // Y
+  mm<KUNR_DBL>(M2, M2, M0);          // Y^2
   for (int e = t; e < NN2; e += RT_WG) {
     const int i = e / NN, j = e % NN;
     const double id = (i == j) ? 1.0 : 0.0, y = M2[i * LD + j], y2 = M0[i * LD + j];
@@ -935,7 +947,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     M0[i * LD + j] = id + y * 0.5 + y2 * (1.0 / 24.0);           // c(Y)
   }
   __syncthreads();
-  mm_add(M0, M2, M1, M0);  // V = c(Y) + s(Y) dz D
+  mm_add<KUNR_DBL>(M0, M2, M1, M0);  // V = c(Y) + s(Y) dz D
   for (int e = t; e < NN2; e += RT_WG) {
     const int i = e / NN, j = e % NN;
     const double u = M3[i * LD + j], v = M0[i * LD + j];
@@ -949,7 +961,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   }
   __syncthreads();
   if (!series_inv_given(M1, M2, L.wmax, HS_TOL, nprod)) return false;  // T in M2
-  mm(M3, M2, M0);                                                       // R
+  mm<KUNR_DBL>(M3, M2, M0);                                             // R
   nprod++;
   // S+ = g+ - R g-, S- = -T g-: linear source (groups 0, 1), constant (2, 3)
   float src;
