@@ -27,21 +27,24 @@
 
 namespace pam {
 
-constexpr int RT_WG = 256;
+// 256 threads at N = 32; at N = 64, where the LDS admits one workgroup per
+// CU, 512 threads, so that every SIMD holds two waves
+constexpr int RT_WG = (NN == 32) ? 256 : 512;
+constexpr int NWAVE = RT_WG / 64;
 constexpr int LD = NN + 1;
 constexpr int MSZ = NN * LD;
 static_assert(NN == 32 || NN == 64, "the tilings below cover N = 32 and N = 64");
-constexpr int NT = NN / 16;        // 16x16 tiles per matrix dimension
-constexpr int TPW = NT * NT / 4;   // output tiles per wave (1 at N = 32, 4 at N = 64)
-// wave wv's tile q: k = wv + 4 q, rows 16 (k / NT), columns 16 (k % NT)
-__device__ __forceinline__ int tile_i(int wv, int q) { return ((wv + 4 * q) / NT) * 16; }
-__device__ __forceinline__ int tile_j(int wv, int q) { return ((wv + 4 * q) % NT) * 16; }
+constexpr int NT = NN / 16;            // 16x16 tiles per matrix dimension
+constexpr int TPW = NT * NT / NWAVE;   // output tiles per wave (1 at N = 32, 2 at N = 64)
+// wave wv's tile q: k = wv + NWAVE q, rows 16 (k / NT), columns 16 (k % NT)
+__device__ __forceinline__ int tile_i(int wv, int q) { return ((wv + NWAVE * q) / NT) * 16; }
+__device__ __forceinline__ int tile_j(int wv, int q) { return ((wv + NWAVE * q) % NT) * 16; }
 // matrix-vector mappings: VL lanes per row when the workgroup runs four
 // vectors at once (4 x NN rows), RPL lanes per row for one vector (NN rows)
 constexpr int VL = RT_WG / (4 * NN);
 constexpr int RPL = RT_WG / NN;
 // unrolling of the k loops of the products: complete at N = 32.  At N = 64,
-// where TPW = 4 tiles per wave already give 4 independent chains, by 8 in the
+// where TPW = 2 tiles per wave give independent chains, by 8 in the
 // doubling kernels (one workgroup per CU: the loads of the next steps must
 // be in flight) and by 2 in the adding kernels (their other registers leave
 // no room for more)
@@ -58,6 +61,14 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
 
 __device__ __forceinline__ float max_nan(float a, float b) { return (b > a || b != b) ? b : a; }
+
+// maximum of the per-wave maxima (NaN propagates)
+__device__ __forceinline__ float wmax_all(const float *w) {
+  float m = max_nan(max_nan(w[0], w[1]), max_nan(w[2], w[3]));
+#pragma unroll
+  for (int q = 4; q < NWAVE; q++) m = max_nan(m, w[q]);
+  return m;
+}
 
 // Wavefront maximum in single precision (the inverse series only compares it
 // against thresholds far from the float limits); NaN propagates.
@@ -112,7 +123,7 @@ __device__ __forceinline__ double mm_core(const double *A, const double *B, cons
     if (l == 0) wmax[wv] = mf;
   }
   __syncthreads();
-  return wmax ? (double)max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.0;
+  return wmax ? (double)wmax_all(wmax) : 0.0;
 }
 
 // Two independent products in one pass: C1 = D1 + A1 B1, C2 = D2 + A2 B2
@@ -166,7 +177,7 @@ __device__ __forceinline__ double mm2(const double *A1, const double *B1, const 
     if (l == 0) wmax[wv] = mf;
   }
   __syncthreads();
-  return wmax ? (double)max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.0;
+  return wmax ? (double)wmax_all(wmax) : 0.0;
 }
 
 // C = A * B
@@ -212,16 +223,18 @@ __device__ __forceinline__ double mv_vl(const double *A, const double *x, int i)
   else
     return mv_row(A, x, i);
 }
-__device__ __forceinline__ int vl_row() { return (threadIdx.x & 63) / VL; }
+// the four vectors are the groups of RT_WG / 4 threads
+__device__ __forceinline__ int vl_group() { return threadIdx.x / (RT_WG / 4); }
+__device__ __forceinline__ int vl_row() { return (threadIdx.x % (RT_WG / 4)) / VL; }
 __device__ __forceinline__ bool vl_lead() { return (threadIdx.x % VL) == 0; }
 
 // LDS scratch of the inverses; gj_only: Gauss-Jordan without trying the
 // series first (set by the queue pass in the Gauss-Jordan test mode)
 struct InvLds {
   int gj_only;
-  float wmax[4];
-  double red[4];
-  int redi[4];
+  float wmax[NWAVE];
+  double red[NWAVE];
+  int redi[NWAVE];
   double prow[2][2 * NN];
 };
 
@@ -271,7 +284,7 @@ __device__ bool inv_i_minus(const double *X, double *G, InvLds &sh) {
     double bv = sh.red[0];
     int p = sh.redi[0];
 #pragma unroll
-    for (int q = 1; q < 4; q++) {
+    for (int q = 1; q < NWAVE; q++) {
       const double v2 = sh.red[q];
       const int i2 = sh.redi[q];
       if (v2 > bv || (v2 == bv && i2 < p)) {
@@ -484,7 +497,7 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
     // sources: S+' = (S+ + Z (S+ + R (S- + f C-))) + f C+, C+' = C+ + Z (C+ + R C-),
     // S-' = S- + Z ((S- + f C-) + R S+), C-' = C- + Z (C- + R C+)
     {
-      const int g = t >> 6, i = vl_row();
+      const int g = vl_group(), i = vl_row();
       const bool lead = vl_lead();
       if (t < NN) sv[4][t] = sv[1][t] + linfac * sv[3][t];  // S- + f C-
       __syncthreads();
@@ -530,7 +543,7 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
 // not cost the one-unit kernel registers.
 // four waves per SIMD at N = 32; at N = 64 the 141 KB of LDS admit one
 // workgroup per CU, and the registers are left to the larger tiles
-constexpr int DBL_WAVES = (NN == 32) ? 4 : 1;
+constexpr int DBL_WAVES = (NN == 32) ? 4 : 2;
 template <bool PERS>
 __global__ void __launch_bounds__(RT_WG) __attribute__((amdgpu_waves_per_eu(DBL_WAVES))) k_doubling(Dims d, Settings s, const double *freqs,
                                                     Work w) {
@@ -622,7 +635,7 @@ __device__ __forceinline__ float mmf(const float *A, bool iA, const float *B, co
     if (l == 0) wmax[wv] = mf;
   }
   __syncthreads();
-  return wmax ? max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.f;
+  return wmax ? wmax_all(wmax) : 0.f;
 }
 
 // Two independent FP32 products in one pass (outputs may alias inputs);
@@ -676,7 +689,7 @@ __device__ __forceinline__ float mmf2(const float *A1, bool iA1, const float *B1
     if (l == 0) wmax[wv] = mf;
   }
   __syncthreads();
-  return wmax ? max_nan(max_nan(wmax[0], wmax[1]), max_nan(wmax[2], wmax[3])) : 0.f;
+  return wmax ? wmax_all(wmax) : 0.f;
 }
 
 // Row i of A x in FP32 over the lane pair t, t^1 (as mv_pair).
@@ -803,7 +816,7 @@ struct MixLds {
   double c[2][NN];  // chain vectors, then g- of the two sources
   double hrow[NN];
   double norm[NMU];
-  float wmax[4];
+  float wmax[NWAVE];
   double mu[NMU], qw[NMU];
   int slot;  // unit index of a persistent workgroup
 };
@@ -901,7 +914,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     const int i = vl_row();
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      if (t < 64) {
+      if (t < RT_WG / 4) {  // the first vector group: one chain
         const double v = cv[i];
         gcm += 2.0 * v * ifact[2 * k + 1];
         grm += 2.0 * v * ifact[2 * k + 2];
@@ -912,14 +925,14 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
       }
       __syncthreads();
       if (k < 3) {
-        if (t < 64) {
+        if (t < RT_WG / 4) {
           const double v = mv_vl(M1, cw, i);
           if (vl_lead()) cv[i] = v;
         }
         __syncthreads();
       }
     }
-    if (t < 64 && vl_lead()) {
+    if (t < RT_WG / 4 && vl_lead()) {
       L.g[0][i] = gcp;
       L.g[1][i] = gcm;
       L.g[2][i] = grp;
@@ -966,7 +979,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   // S+ = g+ - R g-, S- = -T g-: linear source (groups 0, 1), constant (2, 3)
   float src;
   {
-    const int g = t >> 6, i = vl_row();
+    const int g = vl_group(), i = vl_row();
     const double *A = (g & 1) ? M2 : M0;
     const double mv = mv_vl(A, g < 2 ? cv : cw, i);
     const double gcplus = 0.5 * (L.g[0][i] + L.g[1][i]), grplus = 0.5 * (L.g[2][i] + L.g[3][i]);
@@ -989,7 +1002,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     sR[i * LD + j] = rr[q];
     sT[i * LD + j] = tt[q];
   }
-  if (vl_lead()) sv[t >> 6][vl_row()] = src;
+  if (vl_lead()) sv[vl_group()][vl_row()] = src;
   __syncthreads();
   for (int it = 0; it < nd; it++) {
     if (!series_f32(sR, sX, sG, L.wmax, nprod)) return false;
@@ -997,7 +1010,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     mmf2(sT, true, sG, sT, sG, sR, false, sT, sR, sX, nullptr);
     {
       const float lf = (float)linfac;
-      const int g = t >> 6, i = vl_row();
+      const int g = vl_group(), i = vl_row();
       const bool lead = vl_lead();
       if (t < NN) sv[4][t] = sv[1][t] + lf * sv[3][t];
       __syncthreads();
@@ -1244,7 +1257,8 @@ __device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *
 // The next layer's inputs, fetched into registers while the current layer
 // is combined (the barriers of the combine do not wait for global loads).
 // At N = 32 the R and T elements come ahead into registers (8 per thread);
-// at N = 64 (32 per thread) stage_layer copies them from HBM itself.
+// at N = 64 (16 per thread of 512, in kernels already at the register limit)
+// stage_layer copies them from HBM itself.
 constexpr bool FETCH_RT = (NN == 32);
 constexpr int FETCH_N = FETCH_RT ? NN2 / RT_WG : 1;
 struct LayerFetch {
