@@ -18,8 +18,8 @@
 //
 // Matrices live in LDS with a padded leading dimension (at NUMMU = 32 the
 // adding kernels keep their working matrices in a global scratch per
-// workgroup, see AddMats).  Products run on the FP64 matrix cores (16x16
-// output tiles, NN/32 squared of them per wave); the inverse is a
+// workgroup, see add_mat).  Products run on the FP64 matrix cores (16x16
+// output tiles, one per wave at N = 32, two at N = 64); the inverse is a
 // register-resident Gauss-Jordan with partial pivoting (the reference calls
 // LAPACK DGETRF/DGETRI).  The same source serves NUMMU = 16 (N = 32) and
 // NUMMU = 32 (N = 64).
