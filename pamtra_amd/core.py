@@ -242,6 +242,145 @@ class pyPamtra(object):
                 else np.zeros(self._shape3D) + np.nan
 
     # ------------------------------------------------------------------
+    # ASCII profile files (core.py:339-504, 651-749).  File layout, per
+    # grid point: "year month day hhmm nheights i j", the observation
+    # heights, "lat lon lfrac wind10u wind10v groundtemp hgt_lev0", the
+    # integrated values, then one line per layer (.lay) or level (.lev, with
+    # the ground level first): height [m], pressure [Pa], temperature [K],
+    # relative humidity [%] and the hydrometeor moments named by each
+    # category's moment_in.
+    _MOMENT_KEYS = {0: (), 1: ("hydro_n",), 2: ("hydro_reff",), 3: ("hydro_q",),
+                    12: ("hydro_n", "hydro_reff"), 13: ("hydro_n", "hydro_q"),
+                    23: ("hydro_reff", "hydro_q")}
+
+    def _moment_keys(self, hh):
+        mi = int(self.df.data["moment_in"][hh])
+        if mi not in self._MOMENT_KEYS:
+            raise IOError('Did not understand df.data["moment_in"]')
+        return self._MOMENT_KEYS[mi]
+
+    def readPamtraProfile(self, inputFile):
+        """Reads a .lay or .lev profile (core.py:339-504); the descriptor
+        file must be defined first.  The classic format is not supported."""
+        import calendar
+        import datetime
+        assert self.df.nhydro > 0
+        kind = inputFile.split(".")[-1]
+        if kind not in ("lay", "lev"):
+            raise NotImplementedError("only the .lay and .lev profile formats are supported")
+        with open(inputFile) as fh:
+            rows = (line.split() for line in fh if line.strip())
+            nx, ny, L, nout = (int(v) for v in next(rows)[:4])
+            self.p["ngridx"], self.p["ngridy"], self.p["max_nlyrs"], self.p["noutlevels"] = \
+                nx, ny, L, nout
+            nh = self.df.nhydro
+            self._shape2D = (nx, ny)
+            self._shape3D = (nx, ny, L)
+            self._shape3Dplus = (nx, ny, L + 1)
+            self._shape3Dout = (nx, ny, nout)
+            self._shape4D = (nx, ny, L, nh)
+            p = self.p
+            for k in ("lat", "lon", "wind10u", "wind10v", "groundtemp", "sfc_salinity", "iwv",
+                      "sfc_slf", "sfc_sif"):
+                p[k] = np.full(self._shape2D, np.nan)
+            for k in ("model_i", "model_j", "unixtime", "nlyrs", "sfc_type", "sfc_model"):
+                p[k] = np.full(self._shape2D, missingIntNumber, dtype=int)
+            p["sfc_refl"] = np.full(self._shape2D, "S", dtype="U1")
+            p["obs_height"] = np.full(self._shape3Dout, np.nan)
+            for k in ("hydro_q", "hydro_n", "hydro_reff"):
+                p[k] = np.full(self._shape4D, np.nan)
+            for k in ("hgt", "temp", "press", "relhum", "airturb", "wind_w", "wind_uv",
+                      "turb_edr"):
+                p[k] = np.full(self._shape3D, np.nan)
+            for k in ("hgt_lev", "temp_lev", "press_lev", "relhum_lev"):
+                p[k] = np.full(self._shape3Dplus, np.nan)
+            for x in range(nx):
+                for y in range(ny):
+                    year, mon, day, hhmm, nhgt, mi, mj = next(rows)[:7]
+                    p["unixtime"][x, y] = calendar.timegm(datetime.datetime(
+                        int(year), int(mon), int(day), int(hhmm[0:2]), int(hhmm[2:4])).timetuple())
+                    p["model_i"][x, y], p["model_j"][x, y] = int(mi), int(mj)
+                    p["obs_height"][x, y, :] = [float(v) for v in next(rows)[:nout]]
+                    lat, lon, lfrac, u, v, tg, h0 = (float(t) for t in next(rows)[:7])
+                    p["lat"][x, y], p["lon"][x, y] = lat, lon
+                    p["wind10u"][x, y], p["wind10v"][x, y], p["groundtemp"][x, y] = u, v, tg
+                    p["hgt_lev"][x, y, 0] = h0
+                    # lfrac (deprecated) -> surface type: 0 ocean (Fresnel), else land
+                    p["sfc_type"][x, y] = int(np.around(lfrac))
+                    if p["sfc_type"][x, y] == 0:
+                        p["sfc_refl"][x, y] = "F"
+                        p["sfc_salinity"][x, y] = 33.0
+                    p["iwv"][x, y] = float(next(rows)[0])
+                    n = int(nhgt)
+                    if kind == "lev":
+                        vals = [float(t) for t in next(rows)]
+                        (p["hgt_lev"][x, y, 0], p["press_lev"][x, y, 0], p["temp_lev"][x, y, 0],
+                         p["relhum_lev"][x, y, 0]) = vals[:4]
+                        n -= 1  # the file counts levels
+                    p["nlyrs"][x, y] = n
+                    for z in range(n):
+                        vals = [float(t) for t in next(rows)]
+                        if kind == "lay":
+                            (p["hgt"][x, y, z], p["press"][x, y, z], p["temp"][x, y, z],
+                             p["relhum"][x, y, z]) = vals[:4]
+                        else:
+                            (p["hgt_lev"][x, y, z + 1], p["press_lev"][x, y, z + 1],
+                             p["temp_lev"][x, y, z + 1], p["relhum_lev"][x, y, z + 1]) = vals[:4]
+                        k = 4
+                        for hh in range(nh):
+                            for key in self._moment_keys(hh):
+                                p[key][x, y, z, hh] = vals[k]
+                                k += 1
+                        if k != len(vals):
+                            raise IOError(f"{inputFile}: {len(vals)} values on a line, {k} expected")
+        if kind == "lay":
+            p["hgt_lev"][..., 1:-1] = 0.5 * (p["hgt"][..., :-1] + p["hgt"][..., 1:])
+            p["hgt_lev"][..., -1] = p["hgt"][..., -1] + (p["hgt"][..., -1] - p["hgt"][..., -2]) * 0.5
+
+    def writePamtraProfile(self, profileFile):
+        """Writes the profile as .lay or .lev, by the file extension
+        (core.py:651-749), with the reference's number formats."""
+        import datetime
+        kind = profileFile.split(".")[-1]
+        if kind not in ("lay", "lev"):
+            raise IOError("Did not understand lay/lev: " + kind)
+        p = self.p
+        nx, ny, L = self._shape3D
+        nout = self._shape3Dout[2]
+        t0 = datetime.datetime.fromtimestamp(int(np.ravel(p["unixtime"])[0]), datetime.timezone.utc)
+        stamp = f"{t0.year} {t0.month:02d} {t0.day:02d} {t0:%H%M}"
+        iwv = p.get("iwv", np.full(self._shape2D, missingNumber))
+        out = [f"{nx} {ny} {L} {nout}"]
+        for x in range(nx):
+            for y in range(ny):
+                out.append(f"{stamp} {L + 1 if kind == 'lev' else L} {x + 1} {y + 1}")
+                out.append(" ".join("%9e" % h for h in p["obs_height"][x, y, :]))
+                out.append("%3.2f %3.2f %s %s %s %s %s" % (
+                    p["lat"][x, y], p["lon"][x, y], p["sfc_type"][x, y], p["wind10u"][x, y],
+                    p["wind10v"][x, y], p["groundtemp"][x, y], p["hgt_lev"][x, y, 0]))
+                line = str(iwv[x, y] if np.isfinite(iwv[x, y]) else missingNumber)
+                for hh in range(self.df.nhydro):
+                    line += "".join(" %9e" % missingNumber for _ in self._moment_keys(hh))
+                out.append(line)
+                if kind == "lev":
+                    out.append("%6.1f %6.1f %3.2f %1.4f" % (
+                        p["hgt_lev"][x, y, 0], p["press_lev"][x, y, 0], p["temp_lev"][x, y, 0],
+                        p["relhum_lev"][x, y, 0]))
+                for z in range(L):
+                    if kind == "lev":
+                        f = (p["hgt_lev"][x, y, z + 1], p["press_lev"][x, y, z + 1],
+                             p["temp_lev"][x, y, z + 1], p["relhum_lev"][x, y, z + 1])
+                    else:
+                        f = (p["hgt"][x, y, z], p["press"][x, y, z], p["temp"][x, y, z],
+                             p["relhum"][x, y, z])
+                    line = "%6.1f %6.1f %3.2f %1.4f " % f
+                    for hh in range(self.df.nhydro):
+                        line += "".join("%9e " % p[key][x, y, z, hh] for key in self._moment_keys(hh))
+                    out.append(line)
+        with open(profileFile, "w") as fh:
+            fh.write("\n".join(out) + "\n")
+
+    # ------------------------------------------------------------------
     def _checkData(self):
         """Range checks of core.py:979-999."""
         maxLimits = {"relhum_lev": 200, "hydro_q": 0.05, "temp_lev": 320, "press_lev": 110000,

@@ -1,0 +1,85 @@
+"""The pyPamtra ASCII profile formats (.lev levels, .lay layers;
+reference python/pyPamtra/core.py:339-504 reads them, :651-749 writes
+them): a written profile reads back to the values at the file's number
+formats, and the forward operator gives the same TB on it."""
+
+import numpy as np
+import pytest
+
+import oracle
+import pamtra_amd
+from pamtra_amd import workloads
+from _scenario import passive_nml
+
+
+def _same_descriptor(pam):
+    q = pamtra_amd.pyPamtra()
+    for row in pam.df.data:
+        q.df.addHydrometeor(tuple(row.tolist()))
+    q.nmlSet.update(pam.nmlSet)
+    q.set.update(pam.set)
+    return q
+
+
+def _layer_profile(ncol=3, seed=96):
+    """config1-like columns given as layer values (the .lay content)."""
+    pam, freqs = workloads.config1(ncol=ncol, seed=seed)
+    q = _same_descriptor(pam)
+    hl, tl, pl, rl = (pam.p[k][:, 0] for k in ("hgt_lev", "temp_lev", "press_lev", "relhum_lev"))
+    lay = lambda a: 0.5 * (a[:, 1:] + a[:, :-1])
+    q.createProfile(hgt=lay(hl), temp=lay(tl), press=np.sqrt(pl[:, 1:] * pl[:, :-1]),
+                    relhum=lay(rl), hydro_q=pam.p["hydro_q"][:, 0], groundtemp=pam.p["groundtemp"][:, 0],
+                    sfc_type=np.zeros(ncol, dtype=int), sfc_refl=np.full(ncol, "F"),
+                    sfc_salinity=np.full(ncol, 33.0), timestamp=1370908800)
+    q.p["hgt_lev"][..., 0] = hl[:, None, 0] if q.p["hgt_lev"].ndim == 3 else hl[:, 0]
+    return q, freqs
+
+
+def _tb(pam, freqs):
+    out, st, _ = oracle.run(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p, freqs,
+                            nthreads=4)
+    assert st == 0, out["msg"]
+    return out["tb"]
+
+
+def test_lev_profile_round_trip(tmp_path):
+    pam, freqs = workloads.config1(ncol=3, seed=95)
+    pam.p["sfc_refl"][:] = "F"  # what lfrac = 0 reads back as
+    f = str(tmp_path / "profile.lev")
+    pam.writePamtraProfile(f)
+    q = _same_descriptor(pam)
+    q.readPamtraProfile(f)
+    assert q.p["max_nlyrs"] == pam.p["max_nlyrs"] and q._shape3D == pam._shape3D
+    np.testing.assert_array_equal(q.p["nlyrs"], pam.p["nlyrs"])
+    np.testing.assert_allclose(q.p["obs_height"], pam.p["obs_height"], rtol=1e-6)
+    np.testing.assert_allclose(q.p["hgt_lev"], pam.p["hgt_lev"], atol=0.051)
+    np.testing.assert_allclose(q.p["temp_lev"], pam.p["temp_lev"], atol=0.0051)
+    np.testing.assert_allclose(q.p["press_lev"], pam.p["press_lev"], atol=0.051)
+    np.testing.assert_allclose(q.p["relhum_lev"], pam.p["relhum_lev"], atol=5.1e-5)
+    np.testing.assert_allclose(q.p["hydro_q"], pam.p["hydro_q"], rtol=5.1e-6, atol=0)
+    np.testing.assert_array_equal(q.p["sfc_type"], 0)
+    np.testing.assert_array_equal(q.p["sfc_refl"], "F")
+    np.testing.assert_allclose(q.p["groundtemp"], pam.p["groundtemp"], rtol=1e-12)
+    d = np.max(np.abs(_tb(q, freqs) - _tb(pam, freqs)))
+    assert d < 0.02, d  # from the file's 0.01 K / 0.1 m / 0.1 Pa rounding
+
+
+def test_lay_profile_round_trip(tmp_path):
+    pam, freqs = _layer_profile()
+    f = str(tmp_path / "profile.lay")
+    pam.writePamtraProfile(f)
+    q = _same_descriptor(pam)
+    q.readPamtraProfile(f)
+    for k, tol in (("hgt", 0.051), ("temp", 0.0051), ("press", 0.051), ("relhum", 5.1e-5)):
+        np.testing.assert_allclose(q.p[k], pam.p[k], atol=tol)
+    assert np.all(np.isnan(q.p["temp_lev"]))  # derived by fillMissing_atmosphere_vars
+    d = np.max(np.abs(_tb(q, freqs) - _tb(pam, freqs)))
+    assert d < 0.02, d
+
+
+def test_profile_reader_refuses_other_formats(tmp_path):
+    pam, _ = workloads.config1(ncol=1)
+    f = tmp_path / "profile.dat"
+    f.write_text("1 1 2 2\n")
+    with pytest.raises(NotImplementedError):
+        pam.readPamtraProfile(str(f))
