@@ -381,6 +381,102 @@ class pyPamtra(object):
             fh.write("\n".join(out) + "\n")
 
     # ------------------------------------------------------------------
+    def rescaleHeights(self, new_hgt_lev, new_hgt=[]):
+        """Moves the profile to a new height grid (core.py:1161-1288): every
+        level and layer field is interpolated linearly in height, pressure
+        in its logarithm, and continued linearly beyond the old grid;
+        negative hydrometeor moments become 0, other negative values
+        missing.  wind_w, which the reference leaves out, is reset to
+        missing on the new grid."""
+        def extrap(x, xp, yp):
+            y = np.interp(x, xp, yp)
+            lo, hi = x < xp[0], x > xp[-1]
+            y[lo] = yp[0] + (x[lo] - xp[0]) * (yp[0] - yp[1]) / (xp[0] - xp[1])
+            y[hi] = yp[-1] + (x[hi] - xp[-1]) * (yp[-1] - yp[-2]) / (xp[-1] - xp[-2])
+            return y
+
+        new_hgt_lev = np.asarray(new_hgt_lev, dtype=float)
+        old_hgt_lev, old_hgt = self.p["hgt_lev"], self.p["hgt"]
+        if len(new_hgt) == 0:
+            new_hgt = 0.5 * (new_hgt_lev[..., 1:] + new_hgt_lev[..., :-1])
+        nx, ny = self._shape2D
+        L = new_hgt_lev.shape[2] - 1
+        nh = self.df.nhydro
+        self.p["max_nlyrs"] = L
+        self._shape3D = (nx, ny, L)
+        self._shape3Dplus = (nx, ny, L + 1)
+        self._shape4D = (nx, ny, L, nh)
+
+        def regrid(field, new_z, old_z, shape, log=False):
+            out = np.full(shape, missingNumber)
+            for x in range(nx):
+                for y in range(ny):
+                    if len(shape) == 4:
+                        for h in range(nh):
+                            out[x, y, :, h] = extrap(new_z[x, y], old_z[x, y], field[x, y, :, h])
+                    elif log:
+                        out[x, y] = np.exp(extrap(new_z[x, y], old_z[x, y], np.log(field[x, y])))
+                    else:
+                        out[x, y] = extrap(new_z[x, y], old_z[x, y], field[x, y])
+            return out
+
+        for key in ("hydro_q", "hydro_n", "hydro_reff"):
+            if key in self.p:
+                self.p[key] = regrid(self.p[key], new_hgt, old_hgt, self._shape4D)
+                self.p[key][self.p[key] < 0.0] = 0.0
+        for key in list(self.df.data4D):
+            self.df.data4D[key] = regrid(self.df.data4D[key], new_hgt, old_hgt, self._shape4D)
+        for key in ("hgt_lev", "temp_lev", "relhum_lev"):
+            if key in self.p:
+                self.p[key] = regrid(self.p[key], new_hgt_lev, old_hgt_lev, self._shape3Dplus)
+                if key != "hgt_lev":
+                    self.p[key][self.p[key] < -1.e-6] = missingNumber
+        for key in ("airturb", "hgt", "temp", "relhum", "wind_uv", "turb_edr"):
+            if key in self.p:
+                self.p[key] = regrid(self.p[key], new_hgt, old_hgt, self._shape3D)
+                if key != "hgt":
+                    self.p[key][self.p[key] < -1.e-6] = missingNumber
+        if "press_lev" in self.p:
+            self.p["press_lev"] = regrid(self.p["press_lev"], new_hgt_lev, old_hgt_lev,
+                                         self._shape3Dplus, log=True)
+            self.p["press_lev"][self.p["press_lev"] < -1.e-6] = missingNumber
+        if "press" in self.p:
+            self.p["press"] = regrid(self.p["press"], new_hgt, old_hgt, self._shape3D, log=True)
+            self.p["press"][self.p["press"] < -1.e-6] = missingNumber
+        if "wind_w" in self.p:
+            self.p["wind_w"] = np.full(self._shape3D, np.nan)
+        self.p["nlyrs"] = np.sum(self.p["hgt_lev"] != missingNumber, axis=-1) - 1
+
+    def _addObservationLevels(self):
+        """Adds the observation heights to the level grid of every column
+        (core.py:1908-1951), then rescales the profile.  As in the
+        reference, the first observation level is taken as the top and the
+        last as the ground when all columns have them there, and a height
+        on an existing level, at or above the top, or at or below the ground
+        is moved 1 m into the column."""
+        oh = self.p["obs_height"]
+        top, ground = self.p["hgt_lev"][:, :, -1], self.p["hgt_lev"][:, :, 0]
+        nobs = oh.shape[2]
+        skip_top = 1 if any(np.all(oh[:, :, i] >= top) for i in range(nobs)) else 0
+        skip_bot = 1 if any(np.all(oh[:, :, i] <= ground) for i in range(nobs)) else 0
+        nx, ny, nl = self._shape3Dplus
+        new = np.full((nx, ny, nl + nobs - skip_top - skip_bot), np.nan)
+        for x in range(nx):
+            for y in range(ny):
+                lev = self.p["hgt_lev"][x, y, :]
+                for k in range(skip_top, nobs - skip_bot):
+                    h = oh[x, y, k]
+                    if np.any(h == lev[1:-1]):
+                        h = h + 1.0
+                    elif h >= lev[-1]:
+                        h = lev[-1] - 1.0
+                    elif h <= lev[0]:
+                        h = lev[0] + 1.0
+                    lev = np.sort(np.append(lev, h))
+                new[x, y, :] = lev
+        self.rescaleHeights(new)
+
+    # ------------------------------------------------------------------
     def _checkData(self):
         """Range checks of core.py:979-999."""
         maxLimits = {"relhum_lev": 200, "hydro_q": 0.05, "temp_lev": 320, "press_lev": 110000,
@@ -426,7 +522,7 @@ class pyPamtra(object):
         if checkData:
             self._checkData()
         if self.nmlSet["add_obs_height_to_layer"]:
-            raise NotImplementedError("add_obs_height_to_layer is not supported")
+            self._addObservationLevels()
         # the angle count is the reference's compile-time nummu (settings.f90:16),
         # mirrored as _nangles: 16, or 32 for the 32-angle library
         sets = dict(self.set, _nangles=self._nangles)

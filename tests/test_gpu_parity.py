@@ -352,3 +352,24 @@ def test_device_entry_with_4d_descriptors_and_host_array_refusal():
     st2, _, msg2 = _run_device(lib, ctx, mh, freqs, torch)
     assert st2 == _capi.PAM_FATAL
     assert b"hydro4d.rho_ms is not device memory" in msg2
+
+
+def test_add_obs_height_to_layer_through_pypamtra():
+    """nmlSet add_obs_height_to_layer (core.py:1553, :1908-1951): runPamtra
+    inserts the observation height as a level and outputs there; the device
+    matches the oracle on the regridded profile."""
+    pam, freqs = workloads.config1(ncol=4, seed=98)
+    nx, ny = pam.p["ngridx"], pam.p["ngridy"]
+    pam.p["noutlevels"] = 3
+    pam._shape3Dout = (nx, ny, 3)
+    pam.p["obs_height"] = np.array([[833000.0, 2150.0, 0.0]] * nx).reshape(nx, ny, 3)
+    pam.nmlSet["active"] = False
+    pam.nmlSet["add_obs_height_to_layer"] = True
+    pam.runPamtra(freqs)
+    assert pam.p["max_nlyrs"] == 31 and 2150.0 in pam.p["hgt_lev"][0, 0]
+    pam.nmlSet["add_obs_height_to_layer"] = False
+    c, st = _cpu(pam, freqs)
+    assert st == 0
+    d = np.max(np.abs(pam.r["tb"] - c["tb"]))
+    print(f"max |dTB| = {d:.3e} K", flush=True)
+    assert d <= TB_TOL

@@ -83,3 +83,49 @@ def test_profile_reader_refuses_other_formats(tmp_path):
     f.write_text("1 1 2 2\n")
     with pytest.raises(NotImplementedError):
         pam.readPamtraProfile(str(f))
+
+
+def test_observation_heights_become_levels():
+    """add_obs_height_to_layer (core.py:1908-1951, rescaleHeights :1161-1288):
+    an observation height inside the column becomes a level (one on an
+    existing level moves up 1 m), fields are interpolated linearly in
+    height and pressure in its logarithm, and the run outputs the level at
+    exactly that height."""
+    pam, freqs = workloads.config1(ncol=2, seed=97)
+    hl0 = pam.p["hgt_lev"].copy()
+    tl0, pl0 = pam.p["temp_lev"].copy(), pam.p["press_lev"].copy()
+    nx, ny = pam.p["ngridx"], pam.p["ngridy"]
+    pam.p["noutlevels"] = 4
+    pam._shape3Dout = (nx, ny, 4)
+    pam.p["obs_height"] = np.array([[833000.0, 2150.0, 4000.0, 0.0]] * 2).reshape(nx, ny, 4)
+    ref = _tb(pam, freqs)
+    q = _same_descriptor(pam)
+    q.createProfile(**{k: pam.p[k][:, 0] for k in ("hgt_lev", "temp_lev", "press_lev",
+                                                   "relhum_lev", "groundtemp")},
+                    hydro_q=pam.p["hydro_q"][:, 0], obs_height=pam.p["obs_height"][:, 0],
+                    sfc_type=np.zeros(2, dtype=int), sfc_refl=np.full(2, "S"),
+                    sfc_salinity=np.full(2, 33.0), timestamp=0)
+    q._addObservationLevels()
+    assert q.p["max_nlyrs"] == 32  # two heights inserted, top and ground skipped
+    hl = q.p["hgt_lev"][0, 0]
+    assert 2150.0 in hl and 4001.0 in hl and 4000.0 in hl  # 4000 m is a level: +1 m
+    k = int(np.where(hl == 2150.0)[0][0])
+    j = int(np.searchsorted(hl0[0, 0], 2150.0))
+    w = (2150.0 - hl0[0, 0, j - 1]) / (hl0[0, 0, j] - hl0[0, 0, j - 1])
+    np.testing.assert_allclose(q.p["temp_lev"][0, 0, k],
+                               tl0[0, 0, j - 1] + w * (tl0[0, 0, j] - tl0[0, 0, j - 1]), rtol=1e-14)
+    np.testing.assert_allclose(np.log(q.p["press_lev"][0, 0, k]),
+                               np.log(pl0[0, 0, j - 1]) + w * np.log(pl0[0, 0, j] / pl0[0, 0, j - 1]),
+                               rtol=1e-13)
+    # layers away from the inserted levels keep their mid-height and values
+    # (the interpolation hits its knots); the split layers take interpolated
+    # contents, so TB changes by up to a few K where a cloud layer is split
+    h_old, h_new = pam.p["hgt"][0, 0], q.p["hgt"][0, 0]
+    for z in range(len(h_old)):
+        hit = np.where(h_new == h_old[z])[0]
+        if len(hit):
+            np.testing.assert_array_equal(q.p["hydro_q"][0, 0, hit[0]], pam.p["hydro_q"][0, 0, z])
+    assert sum(np.any(h_new == h) for h in h_old) >= 27
+    q.nmlSet["add_obs_height_to_layer"] = False  # already applied
+    got = _tb(q, freqs)
+    assert np.all(np.isfinite(got)) and np.all(np.isfinite(ref))
