@@ -50,6 +50,8 @@ def lib(nummu: int = 16) -> C.CDLL:
         L.pam_oracle_gasabs.restype = C.c_double
         L.pam_oracle_gasabs.argtypes = [C.c_double] * 4
         L.pam_oracle_tessem2.argtypes = [C.c_double] * 5 + [dp, dp]
+        L.pam_oracle_fastem.argtypes = [C.c_double] * 5 + [dp, dp]
+        L.pam_oracle_liu_permittivity.argtypes = [C.c_double] * 3 + [dp, dp]
         L.pam_oracle_mie_single.restype = C.c_int
         L.pam_oracle_mie_single.argtypes = [C.c_double, C.c_double, C.c_double,
                                             C.POINTER(C.c_int), dp, dp, dp]
@@ -124,6 +126,18 @@ def tessem2(f, zen, wind, temp, sal):
     e, r = np.zeros(2), np.zeros(2)
     lib().pam_oracle_tessem2(f, zen, wind, temp, sal, _dp(e), _dp(r))
     return e, r
+
+
+def fastem(f, zen, temp, sal, wind):
+    e, r = np.zeros(2), np.zeros(2)
+    lib().pam_oracle_fastem(f, zen, temp, sal, wind, _dp(e), _dp(r))
+    return e, r
+
+
+def liu_permittivity(temp, sal, f):
+    re, im = C.c_double(), C.c_double()
+    lib().pam_oracle_liu_permittivity(temp, sal, f, C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
 
 
 def mie_single(x, m):

@@ -318,6 +318,114 @@ void pam_oracle_tessem2(double freq, double zen, double wind, double temp, doubl
   }
 }
 
+/* ---- FASTEM5 ocean emissivity as PAMTRA calls it (ocean_sfc_optics.f90:
+ * 68-107, fastemx.f90:120-301): no azimuth term, no reflection correction. */
+
+/* Double-Debye sea-water permittivity (liu.f90), T in K, S in psu, f in GHz */
+static cplx fastem_liu_permittivity(double temp, double sal, double f) {
+  const double mu0 = 3.141592653589793 * 4.0e-07;                 /* constants.f90:34 */
+  const double eps0 = 1.0 / (mu0 * C_LIGHT * C_LIGHT);            /* constants.f90:35 */
+  const double tau0 = 2.0 * 3.141592653589793 * eps0 * 1.0e9;
+  const double t = temp - 273.15, s = sal;
+  double einf = 3.8 + t * 2.48033e-02;
+  double es = 87.9181727 + t * (-4.031592248e-01 + t * (9.493088010e-04 + t * -1.930858348e-06));
+  double e1 = 5.723 + t * (2.2379e-02 + t * -7.1237e-04);
+  double tau1 = 1.124465e-01 + t * (-3.9815727e-03 + t * (8.113381e-05 + t * -7.1824242e-07));
+  double tau2 = 3.049979018e-03 + t * (-3.010041629e-05 + t * (4.811910733e-06 + t * -4.259775841e-08));
+  double ie = 0.0;
+  if (s > 0.0) {
+    double delta = 25.0 - t;
+    double beta = 2.033e-02 + delta * (1.266e-04 + delta * 2.464e-06) +
+                  (-1.849e-05 + delta * (2.551e-07 + delta * -2.551e-08)) * s;
+    double alpha25 = s * (1.82521e-01 + s * (-1.46192e-03 + s * (2.09324e-05 + s * -1.28205e-07)));
+    double alpha = alpha25 * exp(-delta * beta);
+    ie = -alpha / (f * tau0);
+    es = es * (1.0 + s * (-2.697e-03 + s * -7.3e-06 + t * -8.9e-06));
+    e1 = e1 * (1.0 + s * (-6.28908e-03 + s * 1.76032e-04 + t * -9.22144e-05));
+    tau1 = tau1 * (1.0 + s * (-2.39357e-03 + t * (3.1353e-05 + t * -2.52477e-07)));
+    tau2 = tau2 * (1.0 + s * (1.49e-01 + t * -8.8e-04 + (s * s) * -1.05e-04));
+  }
+  double f1 = f * tau1, f2 = f * tau2;
+  double d1 = es - e1, d2 = e1 - einf;
+  double re = einf + d1 / (1.0 + f1 * f1) + d2 / (1.0 + f2 * f2);
+  ie = -ie + d1 * f1 / (1.0 + f1 * f1) + d2 * f2 / (1.0 + f2 * f2);
+  return re - ie * I;
+}
+
+/* Specular reflectivities of a flat surface (fresnel.f90) */
+static void fastem_fresnel(cplx eps, double cosi, double *rv, double *rh) {
+  cplx z1 = csqrt(eps - 1.0 + cosi * cosi), z2 = eps * cosi;
+  cplx zh = (cosi - z1) / (cosi + z1), zv = (z2 - z1) / (z2 + z1);
+  *rv = creal(zv) * creal(zv) + cimag(zv) * cimag(zv);
+  *rh = creal(zh) * creal(zh) + cimag(zh) * cimag(zh);
+}
+
+/* large_scale_correction_module.f90 (FASTEM5 coefficients, per polarisation
+ * six terms quadratic in frequency) */
+static const double FASTEM_LSC[2][6][3] = {
+    {{-5.994667E-002, 9.341346E-004, -9.56611E-007},
+     {8.360313E-002, -1.085991E-003, 6.735338E-007},
+     {-2.617296E-002, 2.864495E-004, -1.429979E-007},
+     {-5.265879E-004, 6.880275E-005, -2.916657E-007},
+     {-1.671574E-005, 1.086405E-006, -3.632227E-009},
+     {1.16194E-004, -6.349418E-005, 2.466556E-007}},
+    {{-2.431811E-002, -1.03181E-003, 4.519513E-006},
+     {2.868236E-002, 1.186478E-003, -5.257096E-006},
+     {-7.93339E-003, -2.422303E-004, 1.089605E-006},
+     {-1.083452E-003, -1.788509E-005, 5.464239E-009},
+     {-3.855673E-005, 9.360072E-007, -2.639362E-009},
+     {1.101309E-003, 3.599147E-005, -1.043146E-007}}};
+
+/* small_scale_correction_module.f90 coefficients */
+static const double FASTEM_SSC[8] = {-5.020848e-006, 2.3297951e-008, 4.6625726e-008,
+                                     -1.9765665e-009, -7.0469823e-004, 7.5061193e-004,
+                                     9.8103876e-004, 1.54895e-004};
+
+void pam_oracle_fastem(double f, double zen, double temp, double sal, double wind, double *emis,
+                       double *refl) {
+  const double cz = cos(zen * (3.141592653589793 / 180.0));
+  double rvf, rhf;
+  fastem_fresnel(fastem_liu_permittivity(temp, sal, f), cz, &rvf, &rhf);
+  /* foam (foam_utility_module.f90) */
+  double fh = 1.0 + zen * (-1.748e-3 + zen * (-7.336e-5 + zen * 1.044e-7));
+  double ff = 0.40 * exp(-5.0e-2 * f);
+  double rv_foam = (1.0 - 0.93) * ff, rh_foam = (1.0 - fh * 0.93) * ff;
+  double cover = wind < 0.0 ? 0.0 : 1.95e-5 * pow(wind, 2.55);
+  /* large-scale roughness */
+  double sec = 1.0 / cz, rl[2];
+  for (int pol = 0; pol < 2; pol++) {
+    double z[6];
+    for (int i = 0; i < 6; i++)
+      z[i] = FASTEM_LSC[pol][i][0] + f * (FASTEM_LSC[pol][i][1] + f * FASTEM_LSC[pol][i][2]);
+    rl[pol] = z[0] + z[1] * sec + z[2] * sec * sec + z[3] * wind + z[4] * wind * wind +
+              z[5] * wind * sec;
+  }
+  /* small-scale roughness: frequency and wind clamped to the fit's range */
+  double fl = f < 1.4 ? 1.4 : (f > 200.0 ? 200.0 : f);
+  double wl = wind < 0.3 ? 0.3 : (wind > 35.0 ? 35.0 : wind);
+  double f2 = fl * fl, w2 = wl * wl;
+  double y = FASTEM_SSC[0] * wl * fl + FASTEM_SSC[1] * wl * f2 + FASTEM_SSC[2] * w2 * fl +
+             FASTEM_SSC[3] * w2 * f2 + FASTEM_SSC[4] * w2 / fl + FASTEM_SSC[5] * w2 / f2 +
+             FASTEM_SSC[6] * wl + FASTEM_SSC[7] * w2;
+  double fsmall = exp(-y * (cz * cz));
+  double rv = rvf, rh = rhf;
+  if (zen < 70.0) {
+    rv = rvf * fsmall - rl[0];
+    rh = rhf * fsmall - rl[1];
+  }
+  emis[0] = 1.0 - (1.0 - cover) * rv - cover * rv_foam;
+  emis[1] = 1.0 - (1.0 - cover) * rh - cover * rh_foam;
+  refl[0] = 1.0 * (1.0 - emis[0]);
+  refl[1] = 1.0 * (1.0 - emis[1]);
+}
+
+/* the Liu permittivity alone (for the tests) */
+void pam_oracle_liu_permittivity(double temp, double sal, double f, double *re, double *im) {
+  cplx e = fastem_liu_permittivity(temp, sal, f);
+  *re = creal(e);
+  *im = cimag(e);
+}
+
 /* Planck radiance in W m^-2 sr^-1 um^-1 (PLANCK_FUNCTION with units 'r',
  * src/rt_utilities.f90:69-90). */
 static double planck(double temp, double wl) {
@@ -1711,8 +1819,27 @@ static int run_column_freq(const pam_settings *s, const pam_hydro_desc *hyd,
         sfc_r[0][i] = r[0];
         sfc_r[1][i] = r[1];
       }
+    } else if (smodel == 1) {
+      /* FASTEM (sfc_optics.f90:57-58, ocean_sfc_optics.f90:68-107) */
+      double wu = p->wind10u ? p->wind10u[col] : 0.0, wv = p->wind10v ? p->wind10v[col] : 0.0;
+      double ws = sqrt(wu * wu + wv * wv);
+      double sal = p->sfc_salinity ? p->sfc_salinity[col] : 33.0;
+      for (int i = 0; i < NMU; i++) {
+        double zen = acos(mu[i]) * (180.0 / PI);
+        double e[2], r[2];
+        pam_oracle_fastem(freq, zen, c->groundtemp, sal, ws, e, r);
+        if (i == NMU - 1) {
+          double em = (e[0] + e[1]) / 2.0, rm = (r[0] + r[1]) / 2.0;
+          e[0] = e[1] = em;
+          r[0] = r[1] = rm;
+        }
+        sfc_e[0][i] = e[0];
+        sfc_e[1][i] = e[1];
+        sfc_r[0][i] = r[0];
+        sfc_r[1][i] = r[1];
+      }
     } else {
-      snprintf(msg, 120, "ocean surface model not supported (FASTEM)");
+      snprintf(msg, 120, "unknown ocean surface model");
       return PAM_FATAL;
     }
   } else if (stype == 1) {

@@ -381,6 +381,94 @@ __device__ void tessem2(double freq, double zen, double wind, double temp, doubl
   }
 }
 
+// FASTEM5 ocean emissivity as PAMTRA calls it (ocean_sfc_optics.f90:68-107,
+// fastemx.f90:120-301): Liu double-Debye sea water (liu.f90), Fresnel
+// (fresnel.f90), foam (foam_utility_module.f90), large- and small-scale
+// roughness (large_/small_scale_correction_module.f90); no azimuth term and
+// no reflection correction.  zen in degrees, temp in K, sal in psu, f in GHz.
+__constant__ double c_fe_lsc[2][6][3] = {
+    {{-5.994667E-002, 9.341346E-004, -9.56611E-007},
+     {8.360313E-002, -1.085991E-003, 6.735338E-007},
+     {-2.617296E-002, 2.864495E-004, -1.429979E-007},
+     {-5.265879E-004, 6.880275E-005, -2.916657E-007},
+     {-1.671574E-005, 1.086405E-006, -3.632227E-009},
+     {1.16194E-004, -6.349418E-005, 2.466556E-007}},
+    {{-2.431811E-002, -1.03181E-003, 4.519513E-006},
+     {2.868236E-002, 1.186478E-003, -5.257096E-006},
+     {-7.93339E-003, -2.422303E-004, 1.089605E-006},
+     {-1.083452E-003, -1.788509E-005, 5.464239E-009},
+     {-3.855673E-005, 9.360072E-007, -2.639362E-009},
+     {1.101309E-003, 3.599147E-005, -1.043146E-007}}};
+__constant__ double c_fe_ssc[8] = {-5.020848e-006,  2.3297951e-008, 4.6625726e-008,
+                                   -1.9765665e-009, -7.0469823e-004, 7.5061193e-004,
+                                   9.8103876e-004,  1.54895e-004};
+
+__device__ cd liu_permittivity(double temp, double sal, double f) {
+  const double mu0 = PI * 4.0e-07;                 // constants.f90:34
+  const double eps0 = 1.0 / (mu0 * C_LIGHT * C_LIGHT);
+  const double tau0 = 2.0 * PI * eps0 * 1.0e9;
+  const double t = temp - 273.15, s = sal;
+  const double einf = 3.8 + t * 2.48033e-02;
+  double es = 87.9181727 + t * (-4.031592248e-01 + t * (9.493088010e-04 + t * -1.930858348e-06));
+  double e1 = 5.723 + t * (2.2379e-02 + t * -7.1237e-04);
+  double tau1 = 1.124465e-01 + t * (-3.9815727e-03 + t * (8.113381e-05 + t * -7.1824242e-07));
+  double tau2 =
+      3.049979018e-03 + t * (-3.010041629e-05 + t * (4.811910733e-06 + t * -4.259775841e-08));
+  double ie = 0.0;
+  if (s > 0.0) {
+    const double delta = 25.0 - t;
+    const double beta = 2.033e-02 + delta * (1.266e-04 + delta * 2.464e-06) +
+                        (-1.849e-05 + delta * (2.551e-07 + delta * -2.551e-08)) * s;
+    const double alpha25 =
+        s * (1.82521e-01 + s * (-1.46192e-03 + s * (2.09324e-05 + s * -1.28205e-07)));
+    ie = -(alpha25 * exp(-delta * beta)) / (f * tau0);
+    es = es * (1.0 + s * (-2.697e-03 + s * -7.3e-06 + t * -8.9e-06));
+    e1 = e1 * (1.0 + s * (-6.28908e-03 + s * 1.76032e-04 + t * -9.22144e-05));
+    tau1 = tau1 * (1.0 + s * (-2.39357e-03 + t * (3.1353e-05 + t * -2.52477e-07)));
+    tau2 = tau2 * (1.0 + s * (1.49e-01 + t * -8.8e-04 + (s * s) * -1.05e-04));
+  }
+  const double f1 = f * tau1, f2 = f * tau2;
+  const double d1 = es - e1, d2 = e1 - einf;
+  const double re = einf + d1 / (1.0 + f1 * f1) + d2 / (1.0 + f2 * f2);
+  ie = -ie + d1 * f1 / (1.0 + f1 * f1) + d2 * f2 / (1.0 + f2 * f2);
+  return cd{re, -ie};
+}
+
+__device__ void fastem(double f, double zen, double temp, double sal, double wind, double *emis) {
+  const double cz = cos(zen * (PI / 180.0));
+  const cd eps = liu_permittivity(temp, sal, f);
+  const cd z1 = csqrt_(eps - 1.0 + cz * cz), z2 = eps * cz;
+  const cd zh = cdiv(cd{cz, 0.0} - z1, cd{cz, 0.0} + z1), zv = cdiv(z2 - z1, z2 + z1);
+  const double rvf = zv.re * zv.re + zv.im * zv.im, rhf = zh.re * zh.re + zh.im * zh.im;
+  const double fh = 1.0 + zen * (-1.748e-3 + zen * (-7.336e-5 + zen * 1.044e-7));
+  const double ff = 0.40 * exp(-5.0e-2 * f);
+  const double rv_foam = (1.0 - 0.93) * ff, rh_foam = (1.0 - fh * 0.93) * ff;
+  const double cover = wind < 0.0 ? 0.0 : 1.95e-5 * pow(wind, 2.55);
+  const double sec = 1.0 / cz;
+  double rl[2];
+  for (int pol = 0; pol < 2; pol++) {
+    double z[6];
+    for (int i = 0; i < 6; i++)
+      z[i] = c_fe_lsc[pol][i][0] + f * (c_fe_lsc[pol][i][1] + f * c_fe_lsc[pol][i][2]);
+    rl[pol] = z[0] + z[1] * sec + z[2] * sec * sec + z[3] * wind + z[4] * wind * wind +
+              z[5] * wind * sec;
+  }
+  const double fl = f < 1.4 ? 1.4 : (f > 200.0 ? 200.0 : f);
+  const double wl = wind < 0.3 ? 0.3 : (wind > 35.0 ? 35.0 : wind);
+  const double f2 = fl * fl, w2 = wl * wl;
+  const double y = c_fe_ssc[0] * wl * fl + c_fe_ssc[1] * wl * f2 + c_fe_ssc[2] * w2 * fl +
+                   c_fe_ssc[3] * w2 * f2 + c_fe_ssc[4] * w2 / fl + c_fe_ssc[5] * w2 / f2 +
+                   c_fe_ssc[6] * wl + c_fe_ssc[7] * w2;
+  const double fsmall = exp(-y * (cz * cz));
+  double rv = rvf, rh = rhf;
+  if (zen < 70.0) {
+    rv = rvf * fsmall - rl[0];
+    rh = rhf * fsmall - rl[1];
+  }
+  emis[0] = 1.0 - (1.0 - cover) * rv - cover * rv_foam;
+  emis[1] = 1.0 - (1.0 - cover) * rh - cover * rh_foam;
+}
+
 // One thread per (column, frequency): surface emissivity/reflectivity.
 __global__ void k_surface(Dims d, Settings s, Profile p, const double *freqs, Work w,
                           double *emis_out) {
@@ -393,7 +481,8 @@ __global__ void k_surface(Dims d, Settings s, Profile p, const double *freqs, Wo
   double *E = w.sfc_emis + ((size_t)col * d.nf + f) * 2 * NMU;
   double *R = w.sfc_refl + ((size_t)col * d.nf + f) * 2 * NMU;
   if (stype == 0) {
-    if (!(smodel == 0 || smodel == -9999)) {
+    // sfc_model -9999/0: TESSEM2; 1: FASTEM (sfc_optics.f90:55-58)
+    if (!(smodel == 0 || smodel == -9999 || smodel == 1)) {
       set_err(&w.cf_status[(size_t)col * d.nf + f], E_SURFACE);
       return;
     }
@@ -403,7 +492,10 @@ __global__ void k_surface(Dims d, Settings s, Profile p, const double *freqs, Wo
     for (int i = 0; i < NMU; i++) {
       double zen = acos(w.mu[i]) * (180.0 / PI);
       double e[2];
-      tessem2(freqs[f], zen, ws, w.groundtemp[col], sal, e);
+      if (smodel == 1)
+        fastem(freqs[f], zen, w.groundtemp[col], sal, ws, e);
+      else
+        tessem2(freqs[f], zen, ws, w.groundtemp[col], sal, e);
       double r0 = 1.0 - e[0], r1 = 1.0 - e[1];
       if (i == NMU - 1) {
         double em = (e[0] + e[1]) / 2.0, rm = (r0 + r1) / 2.0;

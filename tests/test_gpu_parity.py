@@ -373,3 +373,19 @@ def test_add_obs_height_to_layer_through_pypamtra():
     d = np.max(np.abs(pam.r["tb"] - c["tb"]))
     print(f"max |dTB| = {d:.3e} K", flush=True)
     assert d <= TB_TOL
+
+
+def test_fastem_ocean_vs_oracle():
+    """sfc_model = 1 (FASTEM, sfc_optics.f90:57-58) on ocean columns with
+    wind: device emissivities and TB against the oracle's."""
+    pam, freqs = workloads.config2(ncol=16, seed=99)
+    ocean = pam.p["sfc_type"] == 0
+    assert ocean.any()
+    pam.p["sfc_model"][ocean] = 1
+    pam.p["wind10u"][:] = 7.0
+    pam.p["wind10v"][:] = -3.0
+    g, st = _gpu(pam, freqs)
+    c, st2 = _cpu(pam, freqs)
+    assert st == 0 and st2 == 0, (g["msg"], c["msg"])
+    np.testing.assert_allclose(g["emissivity"], c["emissivity"], rtol=1e-12)
+    _compare(g, c)

@@ -484,3 +484,63 @@ def test_thirty_two_angles_converge_at_nadir():
                            pam.df.data4D, pam.p, [35.5])
     assert st3 == 0
     np.testing.assert_allclose(c["tb"][0, 0, :, :32], T0, atol=1e-9)
+
+
+def test_liu_sea_water_permittivity_known_answers():
+    """FASTEM's sea-water permittivity (liu.f90): static permittivity of pure
+    water at 20 C (80.2, e.g. Kaatze 1989) and the ionic conductivity that
+    defines practical salinity (PSS-78: S = 35 at 15 C is 4.2914 S/m), read
+    from the imaginary part at 10 MHz where conduction dominates."""
+    eps0 = 1.0 / (np.pi * 4.0e-7 * 299792458.0 ** 2)
+    e = oracle.liu_permittivity(293.15, 0.0, 1e-4)
+    assert abs(e.real - 80.2) < 0.1
+    e = oracle.liu_permittivity(288.15, 35.0, 0.01)
+    sigma = -e.imag * 2 * np.pi * 0.01e9 * eps0
+    assert abs(sigma - 4.2914) < 0.005, sigma
+    # optical limit: epsilon_inf of the model, a few units
+    assert 3.5 < oracle.liu_permittivity(293.15, 35.0, 1e5).real < 5.0
+
+
+def test_fastem_emissivity_limits_and_fresnel():
+    """FASTEM5 (fastemx.f90, with PAMTRA's call: no azimuth or reflection
+    correction).  Calm sea at 70 degrees or more: no roughness term and no
+    foam, so e = 1 - Fresnel reflectivity of the Liu permittivity, checked
+    against Snell's law with a complex refraction angle.  Within the fit's
+    frequency range (up to 200 GHz) 0 < e < 1, and V above H off nadir; at
+    325 GHz the large-scale term, a quadratic in frequency, can push e_v
+    past 1 near 70 degrees, in the reference as here.  Parity unpinned beyond
+    this: no reference fixture covers FASTEM."""
+    for f in (10.65, 37.0, 89.0, 183.0):
+        for zen in (70.0, 75.0, 80.0):
+            eps = oracle.liu_permittivity(290.0, 35.0, f)
+            th = np.radians(zen)
+            n = np.sqrt(eps)
+            ct = np.sqrt(1 - (np.sin(th) / n) ** 2)
+            rh = abs((np.cos(th) - n * ct) / (np.cos(th) + n * ct)) ** 2
+            rv = abs((n * np.cos(th) - ct) / (n * np.cos(th) + ct)) ** 2
+            e, r = oracle.fastem(f, zen, 290.0, 35.0, 0.0)
+            np.testing.assert_allclose(e, [1 - rv, 1 - rh], rtol=0, atol=2e-12)
+            np.testing.assert_allclose(r, 1 - e, rtol=0, atol=1e-15)
+    for f in (6.9, 23.8, 50.3, 150.0, 200.0):
+        for zen in (0.0, 20.0, 45.0, 60.0, 69.0):
+            for wind in (0.0, 5.0, 15.0, 30.0):
+                e, _ = oracle.fastem(f, zen, 285.0, 33.0, wind)
+                assert np.all((e > 0) & (e < 1)), (f, zen, wind, e)
+                if zen >= 20.0:
+                    assert e[0] > e[1], (f, zen, wind, e)
+    e, _ = oracle.fastem(325.0, 69.0, 285.0, 33.0, 5.0)
+    assert e[0] > 1.0  # the extrapolated fit, kept as the reference has it
+
+
+def test_fastem_and_tessem2_agree_roughly_near_nadir():
+    """The two ocean models of sfc_optics.f90:55-58, calm sea: the mean of
+    the V and H emissivities within 0.03 of each other up to 30 degrees
+    (further off nadir they part by up to 0.06, and near grazing TESSEM2's
+    network even returns a negative H emissivity).  A check across two
+    independent published models, not a parity pin."""
+    for f in (10.65, 37.0, 89.0):
+        for sst in (280.0, 290.0, 300.0):
+            for zen in (0.0, 15.0, 30.0):
+                a, _ = oracle.fastem(f, zen, sst, 33.0, 0.0)
+                b, _ = oracle.tessem2(f, zen, 0.0, sst, 33.0)
+                assert abs(a.mean() - b.mean()) < 0.03, (f, sst, zen, a, b)
