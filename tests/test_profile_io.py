@@ -129,3 +129,37 @@ def test_observation_heights_become_levels():
     q.nmlSet["add_obs_height_to_layer"] = False  # already applied
     got = _tb(q, freqs)
     assert np.all(np.isfinite(got)) and np.all(np.isfinite(ref))
+
+
+def _one_category(moment_in):
+    pam = pamtra_amd.pyPamtra()
+    pam.df.addHydrometeor((
+        "test_hyd", -99.0, -1, -99.0, -99.0, -99.0, -99.0, -99.0,
+        moment_in, 30, "exp", -99.0, -99.0, 8.0e6, -99.0, 1.0e-5, 6.0e-3,
+        "mie-sphere", "khvorostyanov01_drops", -99.0,
+    ))
+    pam = pamtra_amd.importer.createUsStandardProfile(pam, hgt_lev=[0.0, 1000.0, 2000.0])
+    for k in ("press", "temp", "relhum"):
+        pam.p[k] = 0.5 * (pam.p[k + "_lev"][..., 1:] + pam.p[k + "_lev"][..., :-1])
+    return pam
+
+
+# the moment_in cases of the file format, as the reference's own round-trip
+# tests cover them (tests/test_profile_io.py:17-76 of the reference)
+@pytest.mark.parametrize("kind", ["lev", "lay"])
+@pytest.mark.parametrize("moment_in", [0, 1, 2, 3, 12, 13, 23])
+@pytest.mark.parametrize("layer", [0, 1])
+def test_every_moment_combination_round_trips(tmp_path, kind, moment_in, layer):
+    values = {"hydro_n": 1.23e6, "hydro_reff": 4.56e-4, "hydro_q": 7.89e-4}
+    keys = pamtra_amd.pyPamtra._MOMENT_KEYS[moment_in]
+    pam = _one_category(moment_in)
+    for k in keys:
+        pam.p[k][0, 0, layer, 0] = values[k]
+    f = str(tmp_path / f"profile.{kind}")
+    pam.writePamtraProfile(f)
+    back = _one_category(moment_in)
+    back.readPamtraProfile(f)
+    for k in keys:
+        np.testing.assert_allclose(back.p[k][0, 0, layer, 0], values[k], rtol=1e-6)
+    for k in set(values) - set(keys):
+        assert np.all(np.isnan(back.p[k]))  # not in the file
