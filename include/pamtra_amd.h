@@ -49,8 +49,11 @@ enum pam_dist_name {
   PAM_DIST_LOGN = 6
 };
 
-/* descriptor file column "scat_name" (src/scatProperties.f90:722-769). */
-enum pam_scat_name { PAM_SCAT_DISABLED = 0, PAM_SCAT_MIE_SPHERE = 1 };
+/* descriptor file column "scat_name": "mie-sphere" (src/scatProperties.f90:722-769)
+ * and "ssrg-rt3[_kappa_beta_gamma[_zeta]]", the self-similar Rayleigh-Gans
+ * aggregate model in its Legendre-series form (src/scatProperties.f90:771-863,
+ * src/rayleigh_gans.f90:467-872). */
+enum pam_scat_name { PAM_SCAT_DISABLED = 0, PAM_SCAT_MIE_SPHERE = 1, PAM_SCAT_SSRG_RT3 = 2 };
 
 /* arithmetic of the device path */
 enum pam_precision { PAM_PREC_FP64 = 0, PAM_PREC_MIXED = 1 };
@@ -108,6 +111,10 @@ typedef struct pam_hydro_desc {
   int32_t scat_name;
   int32_t pad_;
   double canting;
+  /* PAM_SCAT_SSRG_RT3 only: kappa, beta, gamma, zeta of the scattering
+   * name (defaults 0.25, 1.0, 1.66 and 0.04 rounded to single precision,
+   * src/scatProperties.f90:793-797) */
+  double rg_kappa, rg_beta, rg_gamma, rg_zeta;
 } pam_hydro_desc;
 
 /* Optional per-(x,y,z,hydro) overrides of descriptor columns, the

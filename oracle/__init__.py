@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpam_oracle.so")
 LIB_PATHS = {16: LIB_PATH, 32: os.path.join(HERE, "libpam_oracle_n64.so")}
 _libs = {}
+MAXNLEG = 200  # PAM_MAXNLEG of include/pamtra_amd.h
 # the 32-angle oracle keeps 64x64 operators on the stack of its OpenMP
 # threads; libgomp reads this when it starts
 os.environ.setdefault("OMP_STACKSIZE", "64M")
@@ -59,6 +60,11 @@ def lib(nummu: int = 16) -> C.CDLL:
         L.pam_oracle_layer_doubling.restype = C.c_int
         L.pam_oracle_layer_doubling.argtypes = [dp] + [C.c_double] * 7 + [dp, dp, dp, dp,
                                                                           C.POINTER(C.c_int)]
+        L.pam_oracle_ssrg_shape.restype = C.c_double
+        L.pam_oracle_ssrg_shape.argtypes = [C.c_double] * 5
+        L.pam_oracle_ssrg.restype = C.c_int
+        L.pam_oracle_ssrg.argtypes = [C.c_double, C.c_int, dp, dp, dp, dp] + [C.c_double] * 8 + \
+            [dp, dp, C.POINTER(C.c_int)]
         L.pam_oracle_scatcnv.argtypes = [C.c_int, dp, C.c_double, C.c_double, dp]
         L.pam_oracle_esat.restype = C.c_double
         L.pam_oracle_esat.argtypes = [C.c_double]
@@ -173,6 +179,25 @@ def layer_doubling(smat, kext, salb, gas, zdiff, t_top, t_bot, wavelength_um):
                                          wavelength_um, _dp(R), _dp(T), _dp(sp), _dp(sm),
                                          C.byref(nd))
     return st, R, T, sp, sm, nd.value
+
+
+def ssrg_shape(x, kappa, beta, gamma, zeta):
+    """Self-similar Rayleigh-Gans shape factor at electrical size x."""
+    return lib().pam_oracle_ssrg_shape(x, kappa, beta, gamma, zeta)
+
+
+def ssrg(freq_hz, diameter, del_d, ndens, mass, as_ratio, canting, m, kappa, beta, gamma, zeta):
+    """One SSRGA hydrometeor (Legendre-series form).  m: refractive index of
+    solid ice, n + i k with k >= 0.  Returns (status, extinction, albedo,
+    back_scatt, legen[3][MAXNLEG], nlegen)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (diameter, del_d, ndens, mass)]
+    out = np.zeros(3)
+    legen = np.zeros((3, MAXNLEG))
+    nl = C.c_int()
+    st = lib().pam_oracle_ssrg(freq_hz, len(arrs[0]), *[_dp(a) for a in arrs], as_ratio, canting,
+                               m.real, m.imag, kappa, beta, gamma, zeta, _dp(out), _dp(legen),
+                               C.byref(nl))
+    return st, out[0], out[1], out[2], legen, nl.value
 
 
 def esat(t_k):

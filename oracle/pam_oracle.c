@@ -620,6 +620,34 @@ typedef struct {
   int64_t sum_series; /* sum over bins of the series length, work counter */
 } mie_result;
 
+/* Normalisation of the weighted phase sums and their Legendre series with
+ * the trimming to the last coefficient above 1e-7 (single precision)
+ * (mie_spheres.f90:272-314, rayleigh_gans.f90:771-812). */
+static void legendre_series(int nquad, int nlegen, const double *mu, const double *wts,
+                            double *sump, double tmp, mie_result *res) {
+  for (int i = 0; i < nquad; i++)
+    for (int s = 0; s < 3; s++) sump[s * nquad + i] = tmp * sump[s * nquad + i] * wts[i];
+  double coef[3][MAXNLEG + 1];
+  memset(coef, 0, sizeof(coef));
+  for (int i = 0; i < nquad; i++) {
+    double pl1 = 1.0, pl = 1.0, pl2 = 0.0;
+    for (int l = 0; l <= nlegen; l++) {
+      if (l > 0)
+        pl = (double)(2 * l - 1) * mu[i] * pl1 / (double)l - (double)(l - 1) * pl2 / (double)l;
+      for (int s = 0; s < 3; s++) coef[s][l] = coef[s][l] + sump[s * nquad + i] * pl;
+      pl2 = pl1;
+      pl1 = pl;
+    }
+  }
+  int nl_out = 0;
+  for (int l = 0; l <= nlegen; l++) {
+    double h = (double)((float)(2 * l + 1) / 2.0f);
+    for (int s = 0; s < 3; s++) res->legen[s][l] = h * coef[s][l];
+    if (res->legen[0][l] > SP(1.0e-7)) nl_out = l;
+  }
+  res->nlegen = nl_out;
+}
+
 /* calc_mie_spheres (src/mie_spheres.f90:14-345) for one (layer, hydrometeor)
  * bin set.  Series 4 (P34) is computed by the reference but never reaches the
  * NSTOKES=2 phase matrix (NUMBER_SUMS case 2, rt_utilities.f90:205-240), so
@@ -699,29 +727,7 @@ static int calc_mie_spheres(double freq_hz, int liq_ice, int nbins, const double
   res->back_scatt = sumqback;
   res->extinction = extinction;
   res->albedo = scatter / extinction;
-  double tmp = (wavelength * wavelength / (PI * scatter));
-  for (int i = 0; i < nquad; i++)
-    for (int s = 0; s < 3; s++) sump[s * nquad + i] = tmp * sump[s * nquad + i] * wts[i];
-  double coef[3][MAXNLEG + 1];
-  memset(coef, 0, sizeof(coef));
-  for (int i = 0; i < nquad; i++) {
-    double pl1 = 1.0, pl = 1.0, pl2 = 0.0;
-    for (int l = 0; l <= nlegen; l++) {
-      if (l > 0)
-        pl = (double)(2 * l - 1) * mu[i] * pl1 / (double)l - (double)(l - 1) * pl2 / (double)l;
-      for (int s = 0; s < 3; s++) coef[s][l] = coef[s][l] + sump[s * nquad + i] * pl;
-      pl2 = pl1;
-      pl1 = pl;
-    }
-  }
-  int nleg = nlegen;
-  int nl_out = 0;
-  for (int l = 0; l <= nleg; l++) {
-    double h = (double)((float)(2 * l + 1) / 2.0f);
-    for (int s = 0; s < 3; s++) res->legen[s][l] = h * coef[s][l];
-    if (res->legen[0][l] > SP(1.0e-7)) nl_out = l;
-  }
-  res->nlegen = nl_out;
+  legendre_series(nquad, nlegen, mu, wts, sump, wavelength * wavelength / (PI * scatter), res);
   res->nterms = nterms;
   res->nquad = nquad;
   res->sum_series = sum_series;
@@ -738,6 +744,142 @@ static int calc_mie_spheres(double freq_hz, int liq_ice, int nbins, const double
     return PAM_FATAL;
   }
   return PAM_SUCCESS;
+}
+
+/* ------------------------------------------------------------------ */
+/* self-similar Rayleigh-Gans, Legendre-series form                     */
+/* ------------------------------------------------------------------ */
+
+static double sq(double x) { return x * x; }
+
+/* calc_shape_factor (src/rayleigh_gans.f90:848-872): the braces of Hogan et
+ * al. (2017) eq. 4 at electrical size x, the first fluctuation term scaled
+ * by zeta. */
+double pam_oracle_ssrg_shape(double x, double kappa, double beta, double gamma, double zeta) {
+  double scale = sq(cos(x) * ((1.0 + kappa / 3.0) * (1.0 / (2.0 * x + PI) - 1.0 / (2.0 * x - PI)) -
+                              kappa * (1.0 / (2.0 * x + 3.0 * PI) - 1.0 / (2.0 * x - 3.0 * PI))));
+  int jmax = (int)floor(5.0 * x / PI + 1.0);
+  double summ = zeta * pow(2.0, -gamma) * (1.0 / sq(2.0 * (x + PI)) + 1.0 / sq(2.0 * (x - PI)));
+  for (int jj = 2; jj <= jmax; jj++)
+    summ = summ + pow(2.0 * jj, -gamma) *
+                      (1.0 / sq(2.0 * (x + PI * jj)) + 1.0 / sq(2.0 * (x - PI * jj)));
+  summ = summ * beta * sq(sin(x));
+  return PI * PI * 0.25 * (scale + summ);
+}
+
+/* calc_self_similar_rayleigh_gans_rt3 (src/rayleigh_gans.f90:467-846) for
+ * one (layer, hydrometeor) bin set: extinction and scattering from a
+ * 359-step sum over the scattering angle, backscattering at pi, and the
+ * Legendre series of the amplitude-matrix sums at the Gauss nodes of the
+ * Mie series length of the largest particle.  The per-bin parameter lists
+ * hold the parsed kappa, beta, gamma, zeta. */
+static int calc_ssrg_rt3(double freq_hz, int liq_ice, int nbins, const double *diameter,
+                         const double *del_d, const double *ndens, const double *mass,
+                         double as_ratio, double canting, double refre, double refim,
+                         double kappa, double beta, double gamma, double zeta, int lhyd_absorption,
+                         mie_result *res, cplx *wa, cplx *wb, cplx *wd, char *msg) {
+  memset(res, 0, sizeof(*res));
+  int bad = liq_ice != -1 || !(canting == 0 || canting == 90) || !(as_ratio > 0.0) || nbins <= 0 ||
+            !(freq_hz > 0);
+  for (int i = 0; i < nbins; i++)
+    if (!(mass[i] >= 0) || !(ndens[i] >= 0) || !(diameter[i] > 0) || !(del_d[i] > 0)) bad = 1;
+  if (bad) {
+    snprintf(msg, 120, "calc_self_similar_rayleigh_gans_rt3: assertation error");
+    return PAM_FATAL;
+  }
+  double wavelength = C_LIGHT / freq_hz;
+  double wave_num = 2.0 * PI / wavelength;
+  cplx dielectric = (refre + I * refim) * (refre + I * refim);
+  cplx K = (dielectric - 1.0) / (dielectric + 2.0);
+  double K2 = sq(cabs(K));
+  double x = PI * diameter[nbins - 1] / wavelength;
+  int nterms = 0;
+  if (miecalc(&nterms, x, refre + I * refim, wa, wb, wd) != PAM_SUCCESS) {
+    snprintf(msg, 120, "error in miecalc 4 ssrg!");
+    return PAM_FATAL;
+  }
+  int nlegen = 2 * nterms;
+  if (nlegen > MAXNLEG - 1) nlegen = MAXNLEG - 1;
+  int nquad = (nlegen + 2 * nterms + 2) / 2;
+  double *mu = (double *)malloc(sizeof(double) * 2 * nquad);
+  double *wts = mu + nquad;
+  pam_oracle_gausquad(nquad, mu, wts);
+  double *sump = (double *)calloc((size_t)3 * nquad, sizeof(double));
+  double sumqe = 0.0, sumqs = 0.0, sumqback = 0.0;
+  const int jmax = 360;
+  for (int ii = 0; ii < nbins; ii++) {
+    if (ndens[ii] <= 0) continue;
+    double nd = ndens[ii], dd = del_d[ii];
+    double d_wave = (canting == 0) ? diameter[ii] * as_ratio : diameter[ii];
+    double volume = mass[ii] / RHO_ICE;
+    double prefactor = 9.0 * pow(wave_num, 4) * K2 * sq(volume) / (4.0 * PI);
+    double cabs_ = 3.0 * wave_num * volume * cimag(K);
+    double qscat = 0.0, phas = 0.0;
+    for (int ia = 2; ia <= jmax; ia++) {
+      double th = (double)(ia - 1) * PI / (double)(jmax - 1);
+      double xs = wave_num * d_wave * sin(th * 0.5);
+      phas = prefactor * pam_oracle_ssrg_shape(xs, kappa, beta, gamma, zeta) * (1.0 + sq(cos(th))) * 0.5;
+      qscat = qscat + phas * sin(th);
+    }
+    double qback = phas;
+    qscat = 0.5 * PI * qscat / (double)jmax;
+    double qext = qscat + cabs_;
+    sumqe = sumqe + (qext * nd * dd);
+    sumqs = sumqs + (qscat * nd * dd);
+    sumqback = sumqback + (qback * nd * dd);
+    for (int ia = 0; ia < nquad; ia++) {
+      double th = acos(mu[ia]);
+      double xs = wave_num * d_wave * sin(th * 0.5);
+      double shape = pam_oracle_ssrg_shape(xs, kappa, beta, gamma, zeta);
+      cplx s22 = -I * 3.0 * pow(wave_num, 3) * K * volume * sqrt(shape) / (4.0 * PI);
+      cplx s11 = s22 * cos(th);
+      sump[ia] = sump[ia] + 0.5 * (sq(cabs(s11)) + sq(cabs(s22))) * nd * dd;
+      sump[nquad + ia] = sump[nquad + ia] + 0.5 * (sq(cabs(s11)) - sq(cabs(s22))) * nd * dd;
+      sump[2 * nquad + ia] = sump[2 * nquad + ia] + creal(conj(s11) * s22) * nd * dd;
+    }
+  }
+  double extinction = lhyd_absorption ? sumqe : sumqe - sumqs;
+  double scatter = sumqs;
+  res->back_scatt = sumqback;
+  res->extinction = extinction;
+  res->albedo = scatter / extinction;
+  legendre_series(nquad, nlegen, mu, wts, sump, wavelength * wavelength / (PI * scatter), res);
+  res->nterms = nterms;
+  res->nquad = nquad;
+  free(mu);
+  free(sump);
+  for (int s = 0; s < 3; s++)
+    for (int l = 0; l < MAXNLEG; l++)
+      if (isnan(res->legen[s][l])) {
+        snprintf(msg, 120, "calc_self_similar_rayleigh_gans_rt3: nan in legen");
+        return PAM_FATAL;
+      }
+  if (!(res->extinction > 0) || !(scatter > 0) || !(res->back_scatt > 0)) {
+    snprintf(msg, 120, "calc_self_similar_rayleigh_gans_rt3: extinction/scatter/back_scatt must be positive");
+    return PAM_FATAL;
+  }
+  return PAM_SUCCESS;
+}
+
+/* One SSRGA hydrometeor of nbins bins, for the parity tests: out = extinction,
+ * albedo, back_scatt; legen[3][MAXNLEG]; returns the status, nlegen in *nl. */
+int pam_oracle_ssrg(double freq_hz, int nbins, const double *diameter, const double *del_d,
+                    const double *ndens, const double *mass, double as_ratio, double canting,
+                    double refre, double refim, double kappa, double beta, double gamma,
+                    double zeta, double *out, double *legen, int *nl) {
+  cplx *buf = (cplx *)malloc(sizeof(cplx) * 3 * (MIE_MAXTERMS + 20));
+  mie_result r;
+  char msg[128];
+  int st = calc_ssrg_rt3(freq_hz, -1, nbins, diameter, del_d, ndens, mass, as_ratio, canting,
+                         refre, refim, kappa, beta, gamma, zeta, 1, &r, buf,
+                         buf + MIE_MAXTERMS + 20, buf + 2 * (MIE_MAXTERMS + 20), msg);
+  free(buf);
+  out[0] = r.extinction;
+  out[1] = r.albedo;
+  out[2] = r.back_scatt;
+  memcpy(legen, r.legen, sizeof(r.legen));
+  *nl = r.nlegen;
+  return st;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1739,8 +1881,22 @@ static int layer_hydrometeors(const pam_settings *s, const pam_hydro_desc *hyd,
     else
       ref_ice(T, freq, &nr, &ni);
     mie_result mr;
-    st = calc_mie_spheres(freq * 1e9, hp.liq_ice, nb, bins.diameter2scat, bins.delta_d_ds, ndens,
-                          bins.density2scat, nr, ni, s->lhyd_absorption, &mr, wa, wb, wd, msg);
+    if (d->scat_name == PAM_SCAT_SSRG_RT3) {
+      /* per-bin lists from the descriptor, negative or NaN read as 0
+       * (scatProperties.f90:276-287); mass_ds (drop_size_dist.f90:162-165) */
+      double ar = (hp.as_ratio < 0 || isnan(hp.as_ratio)) ? 0.0 : hp.as_ratio;
+      double cant = (d->canting < 0 || isnan(d->canting)) ? 0.0 : d->canting;
+      double *mass = (double *)malloc(sizeof(double) * nb);
+      for (int i = 0; i < nb; i++) mass[i] = hp.a_ms * pow(bins.d_ds[i], hp.b_ms);
+      st = calc_ssrg_rt3(freq * 1e9, hp.liq_ice, nb, bins.diameter2scat, bins.delta_d_ds, ndens,
+                         mass, ar, cant, nr, ni, d->rg_kappa, d->rg_beta, d->rg_gamma, d->rg_zeta,
+                         s->lhyd_absorption, &mr, wa, wb, wd, msg);
+      free(mass);
+    } else {
+      st = calc_mie_spheres(freq * 1e9, hp.liq_ice, nb, bins.diameter2scat, bins.delta_d_ds,
+                            ndens, bins.density2scat, nr, ni, s->lhyd_absorption, &mr, wa, wb, wd,
+                            msg);
+    }
     free(ndens);
     free(bb);
     if (st == PAM_FATAL) return st;

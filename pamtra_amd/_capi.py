@@ -32,7 +32,7 @@ DIST_NAMES = {
     "mgamma": 5,
     "logn": 6,
 }
-SCAT_NAMES = {"disabled": 0, "mie-sphere": 1}
+SCAT_NAMES = {"disabled": 0, "mie-sphere": 1, "ssrg-rt3": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
 _dp = C.POINTER(C.c_double)
@@ -91,6 +91,10 @@ class PamHydroDesc(C.Structure):
         ("scat_name", C.c_int32),
         ("pad_", C.c_int32),
         ("canting", C.c_double),
+        ("rg_kappa", C.c_double),
+        ("rg_beta", C.c_double),
+        ("rg_gamma", C.c_double),
+        ("rg_zeta", C.c_double),
     ]
 
 
@@ -256,8 +260,7 @@ def make_hydros(descriptorFile) -> C.Array:
         if dist not in DIST_NAMES:
             raise NotImplementedError(f"dist_name {dist!r} is not implemented")
         scat = str(row["scat_name"]).strip()
-        if scat not in SCAT_NAMES:
-            raise NotImplementedError(f"scat_name {scat!r}: only 'mie-sphere' is implemented")
+        scat, rg = scattering_model(scat)
         h.as_ratio = num("as_ratio")
         h.liq_ice = int(row["liq_ice"])
         h.moment_in = int(row["moment_in"])
@@ -272,7 +275,40 @@ def make_hydros(descriptorFile) -> C.Array:
         h.d_1, h.d_2 = num("d_1"), num("d_2")
         h.scat_name = SCAT_NAMES[scat]
         h.canting = num("canting")
+        h.rg_kappa, h.rg_beta, h.rg_gamma, h.rg_zeta = rg
     return arr
+
+
+# self-similar Rayleigh-Gans defaults for aggregates (Hogan and Westbrook
+# 2014), zeta as the single-precision literal 0.04 (scatProperties.f90:793-797)
+SSRG_DEFAULTS = (0.25, 1.0, 1.66, float(np.float32(0.04)))
+
+
+def scattering_model(name: str):
+    """scat_name -> (model key of SCAT_NAMES, (kappa, beta, gamma, zeta)).
+
+    "ssrg-rt3" takes its parameters from the name as
+    ssrg-rt3_<kappa>_<beta>_<gamma>[_<zeta>], zeta 1.0 when left out
+    (scatProperties.f90:771-798).  The reference reads only the first eight
+    characters for the model and the rest from the tenth on; a name with
+    fewer than three or more than four parameters leaves it reading an unset
+    token, which is refused here."""
+    if name in ("disabled", "mie-sphere"):
+        return name, (0.0, 0.0, 0.0, 0.0)
+    if name[:8] == "ssrg-rt3":
+        if len(name) == 8:
+            return "ssrg-rt3", SSRG_DEFAULTS
+        tok = name[9:].split("_")
+        if len(tok) not in (3, 4):
+            raise ValueError(f"scat_name {name!r}: ssrg-rt3 takes kappa_beta_gamma[_zeta]")
+        try:
+            vals = [float(t) for t in tok]
+        except ValueError:
+            raise ValueError(f"scat_name {name!r}: parameters are not numbers") from None
+        if len(vals) == 3:
+            vals.append(1.0)
+        return "ssrg-rt3", tuple(vals)
+    raise NotImplementedError(f"scat_name {name!r}: 'mie-sphere' and 'ssrg-rt3' are implemented")
 
 
 @dataclass

@@ -215,7 +215,7 @@ static const char *err_text(int32_t code) {
     case E_NAN_LAYER: return "found nan in layer variables";
     case E_HGT_ORDER: return "atmo_hgt_lev must be defined bottom-up";
     case E_GAS: return "error in rosen98_gasabs (frequency, temperature or pressure out of range)";
-    case E_SURFACE: return "surface model not supported (FASTEM, land atlases)";
+    case E_SURFACE: return "surface model not supported (land atlases)";
     case E_MASS_SIZE: return "error in make_mass_size";
     case E_DIST_PARAMS: return "error in make_dist_params";
     case E_ADAPTIVE: return "make_dist: adaptive grid failed";
@@ -230,6 +230,9 @@ static const char *err_text(int32_t code) {
     case E_SINGULAR: return "Matrix inversion failed";
     case E_REFL: return "Wrong or no ground_type selected";
     case E_NAN_COEF: return "nan in Legendre coefficients";
+    case E_SSRG_INPUT:
+      return "calc_self_similar_rayleigh_gans_rt3: assertion error (liq_ice must be -1, canting 0 or 90, "
+             "as_ratio > 0)";
     default: return "error";
   }
 }
@@ -447,7 +450,7 @@ static size_t col_bytes(const Dims &d) {
 }
 
 static size_t item_bytes_per(const Dims &d) {
-  size_t per = 2 * 4 + (size_t)d.nh * d.nbmax * 4 * 8 + (size_t)d.nh * 4;
+  size_t per = 2 * 4 + (size_t)d.nh * d.nbmax * 4 * 8 + (size_t)d.nh * (4 + 8);
   per += (size_t)d.nf * (3 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8 + 4 + 8);
   return per;
 }
@@ -463,6 +466,7 @@ static void carve_items(Work &w, void *base, const Dims &d, size_t nitems) {
   w.dsd_nd = cv.take<double>(nb);
   w.dsd_rho = cv.take<double>(nb);
   w.dsd_flag = cv.take<int32_t>(nitems * d.nh);
+  w.dsd_ar = cv.take<double>(nitems * d.nh);
   w.kext = cv.take<double>(nif);
   w.salb = cv.take<double>(nif);
   w.back = cv.take<double>(nif);
@@ -600,6 +604,8 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   s.pers_dbl = (int)c->tune[1];
   s.pers_add = (int)c->tune[2];
   s.ncu = c->ncu;
+  for (int i = 0; i < d.nh; i++)
+    if (hydros[i].scat_name == PAM_SCAT_SSRG_RT3) s.ssrg = 1;
 
   Profile p;
   p.nlyrs = pin->nlyrs;

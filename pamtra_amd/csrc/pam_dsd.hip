@@ -297,6 +297,10 @@ __global__ void __launch_bounds__(128) k_dsd(Dims d, Settings s, Profile p,
   int32_t *cst = w.col_status + col;
   const pam_hydro_desc &hd = hyd[hh];
   size_t i4 = oy * d.nh + hh;
+  // as_ratio of the layer for the self-similar Rayleigh-Gans size along the
+  // beam (scatProperties.f90:276, 286-287: negative or NaN reads as 0)
+  const double ar4 = h4val(h4.as_ratio, hd.as_ratio, i4);
+  w.dsd_ar[(size_t)li * d.nh + hh] = (ar4 < 0.0 || isnan(ar4)) ? 0.0 : ar4;
   HP h;
   h.as_ratio = h4val(h4.as_ratio, hd.as_ratio, i4);
   h.rho_ms = h4val(h4.rho_ms, hd.rho_ms, i4);
@@ -489,7 +493,9 @@ __global__ void __launch_bounds__(128) k_dsd(Dims d, Settings s, Profile p,
       }
     }
     od[i] = deff;
-    orho[i] = rho;
+    // the self-similar Rayleigh-Gans model takes the particle mass mass_ds
+    // (drop_size_dist.f90:162-165) instead of a density
+    orho[i] = (hd.scat_name == PAM_SCAT_SSRG_RT3 && h.liq_ice != 1) ? h.a_ms * pow(dd, h.b_ms) : rho;
     ond[i] = ond[i] / odd[i];
   }
   for (int i = nbin; i < d.nbmax; i++) {

@@ -56,6 +56,7 @@ enum ErrCode : int32_t {
   E_SINGULAR = 18 << 8,
   E_REFL = 19 << 8,
   E_NAN_COEF = 20 << 8,
+  E_SSRG_INPUT = 21 << 8,
 };
 
 // one call's shapes and settings, passed by value to every kernel
@@ -78,6 +79,7 @@ struct Settings {
   // takes units from a work queue (0: one workgroup per unit)
   int pers_mie, pers_dbl, pers_add;
   int ncu;  // compute units of the device
+  int ssrg;  // some hydrometeor uses the self-similar Rayleigh-Gans model
   // radar simulator, "simple" mode (active): attenuation 0 disabled,
   // 1 top-down, 2 bottom-up; polarisations 0 NN, 1 HH, 2 VV, 3 HV, 4 VH
   int active, radar_att, radar_npol, radar_pol[5];
@@ -122,8 +124,11 @@ struct Work {
   int32_t *cf_status;                  // [ncol][nf]
   // per item of the chunk
   int32_t *item_col, *item_layer;      // [nitems]
+  // dsd_rho: particle density for Mie, particle mass for self-similar
+  // Rayleigh-Gans hydrometeors
   double *dsd_d, *dsd_dd, *dsd_nd, *dsd_rho;  // [nitems][nh][nbmax]
   int32_t *dsd_flag;                   // [nitems][nh]
+  double *dsd_ar;                      // [nitems][nh] as_ratio of the layer
   // per (item, frequency) of the chunk
   double *kext, *salb;                 // [nitems*nf]
   double *back;                        // [nitems*nf] co-polar backscattering [m2/m3]

@@ -239,8 +239,139 @@ struct MieLds {
   int slot;                    // unit index of a persistent workgroup
 };
 
+// ---------------------------------------------------------------------
+// Self-similar Rayleigh-Gans aggregates in Legendre-series form
+// (calc_self_similar_rayleigh_gans_rt3, rayleigh_gans.f90:467-846): the
+// same outputs as the Mie spheres (extinction, albedo, backscattering and
+// the normalised phase-function sums at the Gauss nodes), so the layer
+// accumulation, the Legendre projection and the RT are shared.
+// ---------------------------------------------------------------------
+constexpr int SSRG_QANG = 360;    // angles of the scattering integral, 0..pi
+constexpr int SSRG_TG = U_CAP - 3 * MIE_MAXBIN;  // (2j)^-gamma table in the U stage
+static_assert(SSRG_TG >= 1024, "power table");
+
+// shape factor of the SSRGA phase function at electrical size x
+// (calc_shape_factor, rayleigh_gans.f90:848-872); tg[j-1] = (2j)^-gamma
+__device__ double ssrg_shape(double x, double kappa, double beta, double gamma, double zeta,
+                             const double *tg) {
+  const double c = cos(x);
+  const double a = (1.0 + kappa / 3.0) * (1.0 / (2.0 * x + PI) - 1.0 / (2.0 * x - PI)) -
+                   kappa * (1.0 / (2.0 * x + 3.0 * PI) - 1.0 / (2.0 * x - 3.0 * PI));
+  const double scale = (c * a) * (c * a);
+  const int jmax = (int)floor(5.0 * x / PI + 1.0);
+  double tp = 2.0 * (x + PI), tm = 2.0 * (x - PI);
+  double summ = zeta * tg[0] * (1.0 / (tp * tp) + 1.0 / (tm * tm));
+  for (int j = 2; j <= jmax; j++) {
+    const double g = (j <= SSRG_TG) ? tg[j - 1] : pow(2.0 * j, -gamma);
+    tp = 2.0 * (x + PI * j);
+    tm = 2.0 * (x - PI * j);
+    summ = summ + g * (1.0 / (tp * tp) + 1.0 / (tm * tm));
+  }
+  const double sx = sin(x);
+  summ = summ * beta * (sx * sx);
+  return PI * PI * 0.25 * (scale + summ);
+}
+
+// Efficiency sums and phase-function sums of one SSRGA hydrometeor.  Waves
+// take bins for the 359-angle scattering integral (the per-bin terms summed
+// afterwards in bin order, as the reference does); threads take the Gauss
+// nodes for the phase-function sums, bins in order.  Returns false after
+// setting the status on an input the reference refuses.
+__device__ bool ssrg_sums(const pam_hydro_desc &hd, double ar, int nbin, const double *dia,
+                          const double *dd, const double *nd, const double *mass, cd refr,
+                          double wavelength, int nquad, const double *tmu, double *s_u,
+                          double (*s_ps)[MAX_NQUAD], double *sumq, int32_t *st) {
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+  double cant = hd.canting;
+  if (cant < 0.0 || isnan(cant)) cant = 0.0;  // scatProperties.f90:284-285
+  // assertions of rayleigh_gans.f90:590-616
+  if (hd.liq_ice != -1 || !(cant == 0.0 || cant == 90.0) || !(ar > 0.0)) {
+    if (t == 0) set_err(st, E_SSRG_INPUT);
+    return false;
+  }
+  const double kappa = hd.rg_kappa, beta = hd.rg_beta, gamma = hd.rg_gamma, zeta = hd.rg_zeta;
+  double *bq = s_u, *tg = s_u + 3 * MIE_MAXBIN;
+  for (int j = t; j < SSRG_TG; j += MIE_WG) tg[j] = pow(2.0 * (j + 1), -gamma);
+  // dielectric factor of solid ice, (n + i k)^2 (rayleigh_gans.f90:642-645)
+  const cd eps = refr * refr;
+  const cd K = cdiv(eps - 1.0, eps + 2.0);
+  const double K2 = K.re * K.re + K.im * K.im;
+  const double k = 2.0 * PI / wavelength;
+  const double k2 = k * k;
+  __syncthreads();
+  // scattering integral every 1/359 of pi (rayleigh_gans.f90:713-725)
+  constexpr int LAST_LANE = (SSRG_QANG - 2) % 64;
+  for (int ii = wv; ii < nbin; ii += MIE_WG / 64) {
+    if (!(nd[ii] > 0.0)) {
+      if (lane == 0) bq[ii] = bq[MIE_MAXBIN + ii] = bq[2 * MIE_MAXBIN + ii] = 0.0;
+      continue;
+    }
+    const double dw = (cant == 0.0) ? dia[ii] * ar : dia[ii];
+    const double vol = mass[ii] / 917.0;
+    const double pref = 9.0 * (k2 * k2) * K2 * (vol * vol) / (4.0 * PI);
+    double q = 0.0, last = 0.0;
+    for (int ia = 2 + lane; ia <= SSRG_QANG; ia += 64) {
+      const double th = (double)(ia - 1) * PI / (double)(SSRG_QANG - 1);
+      const double x = k * dw * sin(th * 0.5);
+      const double ct = cos(th);
+      last = pref * ssrg_shape(x, kappa, beta, gamma, zeta, tg) * (1.0 + ct * ct) * 0.5;
+      q += last * sin(th);
+    }
+    q = wave_sum(q);
+    const double qback = __shfl(last, LAST_LANE, 64);
+    if (lane == 0) {
+      const double qscat = 0.5 * PI * q / (double)SSRG_QANG;
+      const double qext = qscat + 3.0 * k * vol * K.im;
+      bq[ii] = qext * nd[ii] * dd[ii];
+      bq[MIE_MAXBIN + ii] = qscat * nd[ii] * dd[ii];
+      bq[2 * MIE_MAXBIN + ii] = qback * nd[ii] * dd[ii];
+    }
+  }
+  // amplitude-matrix sums at the Gauss nodes (rayleigh_gans.f90:733-744)
+  if (t < nquad) {
+    const double th = acos(tmu[t]);
+    const double ct = cos(th), sh = sin(th * 0.5);
+    const double k3 = k2 * k;
+    double p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    for (int ii = 0; ii < nbin; ii++) {
+      const double n = nd[ii];
+      if (!(n > 0.0)) continue;
+      const double dw = (cant == 0.0) ? dia[ii] * ar : dia[ii];
+      const double vol = mass[ii] / 917.0;
+      const double amp = 3.0 * k3 * vol * sqrt(ssrg_shape(k * dw * sh, kappa, beta, gamma, zeta, tg)) /
+                         (4.0 * PI);
+      // s22 = -i amp K, s11 = s22 cos(theta)
+      const double s22r = amp * K.im, s22i = -amp * K.re;
+      const double s11r = s22r * ct, s11i = s22i * ct;
+      const double a11 = s11r * s11r + s11i * s11i, a22 = s22r * s22r + s22i * s22i;
+      p1 = p1 + 0.5 * (a11 + a22) * n * dd[ii];
+      p2 = p2 + 0.5 * (a11 - a22) * n * dd[ii];
+      p3 = p3 + (s11r * s22r + s11i * s22i) * n * dd[ii];
+    }
+    s_ps[0][t] = p1;
+    s_ps[1][t] = p2;
+    s_ps[2][t] = p3;
+  }
+  __syncthreads();
+  if (t == 0) {  // bin order, as the reference sums
+    double e = 0.0, sc = 0.0, b = 0.0;
+    for (int ii = 0; ii < nbin; ii++) {
+      e = e + bq[ii];
+      sc = sc + bq[MIE_MAXBIN + ii];
+      b = b + bq[2 * MIE_MAXBIN + ii];
+    }
+    sumq[0] = e;
+    sumq[1] = sc;
+    sumq[2] = b;
+  }
+  return true;
+}
+
 // One (scattering layer, channel) unit: Mie of every hydrometeor of the
 // layer and their accumulation into the layer's Legendre series.
+// SSRG: the instantiation that also carries the self-similar Rayleigh-Gans
+// branch (its registers would otherwise spill in the Mie-only kernel).
+template <bool SSRG>
 __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
                                          const pam_hydro_desc *hyd, const double *freqs,
                                          const Work &w, int itf, MieLds &L) {
@@ -314,13 +445,25 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
     const int nquad = mie_nquad(nterms);
     const double *tmu = w.atab + w.atab_off[nterms], *twt = tmu + nquad, *tpi = twt + nquad,
                  *ttau = tpi + (size_t)nterms * nquad, *tpl = ttau + (size_t)nterms * nquad;
-    (void)tmu;
     // the Gauss nodes come in pairs -mu, +mu (the middle node is mu = 0 when
     // nquad is odd): phase B works on the nh2 angles with mu >= 0
     const int nh2 = (nquad + 1) >> 1, j0 = nquad - nh2;
     const int nct = (nh2 + 15) >> 4;
     for (int k = t; k < 3 * MAX_NQUAD; k += MIE_WG) (&s_ps[0][0])[k] = 0.0;
     double sumqe = 0.0, sumqs = 0.0, sumqb = 0.0;
+    if (SSRG && hyd[hh].scat_name == PAM_SCAT_SSRG_RT3) {
+      __syncthreads();
+      double q3[3];
+      if (!ssrg_sums(hyd[hh], w.dsd_ar[(size_t)li_item * d.nh + hh], nbin, dia, dd, nd, rho,
+                     cd{s_hm[hh][0], s_hm[hh][1]}, wavelength, nquad, tmu, s_u, s_ps, q3, st))
+        return;
+      if (t == 0) {
+        sumqe = q3[0];
+        sumqs = q3[1];
+        sumqb = q3[2];
+        work_bins += nbin;
+      }
+    } else {
     // series length of every bin (mie_scat_utilities.f90:34)
     for (int e = t; e < nbin; e += MIE_WG) {
       int v = 0;
@@ -663,6 +806,7 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
         }
       }
     }
+    }  // Mie spheres
     if (t == 0) {
       s_sum[0] = sumqe;
       s_sum[1] = sumqs;
@@ -766,14 +910,14 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
 
 // One unit per workgroup, or (PERS) a persistent grid taking units from a
 // work queue (separate instantiations).
-template <bool PERS>
+template <bool PERS, bool SSRG>
 __global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(2))) k_mie(
     Dims d, Settings s, const pam_hydro_desc *hyd, const double *freqs, Work w) {
   __shared__ MieLds L;
   const int n = d.nitems * d.nf;
   int itf = PERS ? next_unit(&w.queue[0], &L.slot) : (int)blockIdx.x;
   while (itf < n) {
-    mie_unit(d, s, hyd, freqs, w, itf, L);
+    mie_unit<SSRG>(d, s, hyd, freqs, w, itf, L);
     if (!PERS) break;
     itf = next_unit(&w.queue[0], &L.slot);
   }
@@ -786,10 +930,16 @@ void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, con
   size_t n = (size_t)d.nitems * d.nf;
   if (n == 0) return;
   const dim3 grid((unsigned)persistent_grid(n, s.pers_mie, s.ncu));
-  if (grid.x < n)
-    hipLaunchKernelGGL(k_mie<true>, grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
-  else
-    hipLaunchKernelGGL(k_mie<false>, grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+  if (s.ssrg) {
+    if (grid.x < n)
+      hipLaunchKernelGGL((k_mie<true, true>), grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+    else
+      hipLaunchKernelGGL((k_mie<false, true>), grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+  } else if (grid.x < n) {
+    hipLaunchKernelGGL((k_mie<true, false>), grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+  } else {
+    hipLaunchKernelGGL((k_mie<false, false>), grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+  }
 }
 
 }  // namespace pam
