@@ -195,6 +195,10 @@ class pyPamtra(object):
             hl[..., -1] = h[..., -1] + (h[..., -1] - h[..., -2]) * 0.5
             self.p["hgt_lev"] = hl
 
+        # 1-based grid indices of the columns (core.py:885-886)
+        gi, gj = np.indices(self._shape2D)
+        self.p["model_i"], self.p["model_j"] = gi + 1, gj + 1
+
         if "timestamp" not in kwargs:
             self.p["unixtime"] = np.ones(self._shape2D) * int(time.time())
         else:
@@ -535,3 +539,110 @@ class pyPamtra(object):
         self.r["nmlSettings"] = self.nmlSet
         if self.set["pyVerbose"] > 0:
             print("pyPamtra runtime:", time.time() - tttt)
+
+    # ------------------------------------------------------------------
+    def writeResultsToNetCDF(self, fname, profileVars="all", wpNames=[], ncForm="NETCDF3_CLASSIC",
+                             xarrayCompatibleOutput=False, ncCompression=False,
+                             lfracCompatibility=False):
+        """Writes the results to a netCDF file with the reference's dimensions,
+        variables, types, units, fill values and global attributes
+        (core.py:2163-2556).
+
+        The netCDF-3 forms are written by scipy's netCDF-3 writer:
+        NETCDF3_CLASSIC and NETCDF3 as the classic format, NETCDF3_64BIT with
+        64-bit offsets.  The HDF5-based NETCDF4 and NETCDF4_CLASSIC forms need
+        the netCDF4 package.  The single-character passive_polarisation
+        variable carries no _FillValue.  Radar spectra and moments are not
+        produced by this library, so their variables are never written."""
+        import datetime
+
+        from scipy.io import netcdf_file
+
+        forms = {"NETCDF3_CLASSIC": 1, "NETCDF3": 1, "NETCDF3_64BIT": 2}
+        if ncForm in ("NETCDF4_CLASSIC", "NETCDF4"):
+            raise NotImplementedError(f"ncForm {ncForm!r} needs the netCDF4 package; "
+                                      "use NETCDF3_CLASSIC or NETCDF3_64BIT")
+        if ncForm not in forms:
+            raise ValueError("Unknown nc form " + ncForm)
+        if not hasattr(self, "r") or "pamtraVersion" not in self.r:
+            raise IOError("run runPamtra first!")
+        nml = self.r["nmlSettings"]
+        f = netcdf_file(fname, "w", version=forms[ncForm])
+        try:
+            f.history = ("Created with pyPamtra (Version: " + self.r["pamtraVersion"] +
+                         ", Git Hash: " + self.r["pamtraHash"] + ") by " + self.nmlSet["creator"] +
+                         " (University of Cologne, IGMK) at " +
+                         datetime.datetime.now().strftime("%Y/%m/%d %H:%M:%S"))
+            f.properties = str(nml)
+            f.createDimension("grid_x", int(self.p["ngridx"]))
+            f.createDimension("grid_y", int(self.p["ngridy"]))
+            f.createDimension("frequency", int(self.set["nfreqs"]))
+            outdim = "outlevel" if xarrayCompatibleOutput else "outlevels"
+            if nml["passive"]:
+                f.createDimension("angles", len(self.r["angles_deg"]))
+                f.createDimension(outdim, int(self.p["noutlevels"]))
+                f.createDimension("passive_polarisation", int(self._nstokes))
+            if nml["active"]:
+                f.createDimension("heightbins", int(self.p["max_nlyrs"]))
+                f.createDimension("radar_polarisation", int(self.set["radar_npol"]))
+                f.createDimension("attenuation_polarisation", int(self.set["att_npol"]))
+                f.createDimension("radar_peak_number", int(self.nmlSet["radar_npeaks"]))
+            dim2d = ("grid_x", "grid_y")
+
+            def var(name, typ, dims, units, data):
+                v = f.createVariable(name, typ, dims)
+                v.units = units
+                dt = np.float32 if typ == "f" else np.int32
+                v._FillValue = dt(missingNumber)
+                v[:] = np.asarray(data, dtype=dt).reshape(v.shape)
+                return v
+
+            var("frequency", "f", ("frequency",), "GHz", self.set["freqs"])
+            var("grid_x", "f", ("grid_x",), "-", np.arange(self.p["ngridx"]))
+            var("grid_y", "f", ("grid_y",), "-", np.arange(self.p["ngridy"]))
+            if nml["active"]:
+                var("heightbins", "i", ("heightbins",), "-", np.arange(self.p["max_nlyrs"]))
+                var("height", "f", dim2d + ("heightbins",), "m", self.r["radar_hgt"])
+            if nml["passive"]:
+                var("angles", "f", ("angles",), "deg", self.r["angles_deg"])
+                v = f.createVariable("passive_polarisation", "c", ("passive_polarisation",))
+                v.units = "-"
+                v[:] = np.array(["V", "H"][:self._nstokes], dtype="S1")
+                if xarrayCompatibleOutput:
+                    var("outlevel", "f", (outdim,), "-", np.arange(self.p["noutlevels"]))
+                var("outlevels", "f", dim2d + (outdim,), "m over sea level",
+                    self.p["obs_height"])
+            var("model_i", "i", dim2d, "-", self.p["model_i"])
+            var("model_j", "i", dim2d, "-", self.p["model_j"])
+            var("nlyr", "i", dim2d, "-", self.p["nlyrs"])
+            var("datatime", "i", dim2d, "seconds since 1970-01-01 00:00:00", self.p["unixtime"])
+            var("longitude", "f", dim2d, "deg.dec", self.p["lon"])
+            var("latitude", "f", dim2d, "deg.dec", self.p["lat"])
+            if lfracCompatibility:
+                var("lfrac", "f", dim2d, "-", self.p["sfc_type"])
+            var("sfc_type", "i", dim2d, "0: water, 1: land", self.p["sfc_type"])
+            if nml["active"]:
+                d3 = dim2d + ("heightbins",)
+                var("Ze", "f", d3 + ("frequency", "radar_polarisation", "radar_peak_number"),
+                    "dBz", self.r["Ze"])
+                var("Attenuation_Hydrometeors", "f", d3 + ("frequency", "attenuation_polarisation"),
+                    "dB", self.r["Att_hydro"])
+                var("Attenuation_Atmosphere", "f", d3 + ("frequency",), "dB", self.r["Att_atmo"])
+                if "PIA" in self.r:
+                    var("Path_Integrated_Attenuation", "f", d3 + ("frequency",), "dB",
+                        self.r["PIA"])
+            if nml["passive"]:
+                var("tb", "f", dim2d + (outdim, "angles", "frequency", "passive_polarisation"), "K",
+                    self.r["tb"])
+            if (profileVars == "all" or "iwv" in profileVars) and "iwv" in self.p:
+                var("iwv", "f", dim2d, "kg/m^2", self.p["iwv"])
+            if (profileVars == "all" or "hydro_wp" in profileVars) and "hydro_wp" in self.p:
+                for i, wp in enumerate(wpNames):
+                    var(wp, "f", dim2d, "kg/m^2", self.p["hydro_wp"][..., i])
+            for key in ("cloudBase", "cloudTop"):
+                if (profileVars == "all" or key in profileVars) and key in self.p:
+                    var(key, "f", dim2d, "m", self.p[key])
+        finally:
+            f.close()
+        if self.set["pyVerbose"] > 0:
+            print(fname, "written")

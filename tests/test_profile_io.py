@@ -163,3 +163,80 @@ def test_every_moment_combination_round_trips(tmp_path, kind, moment_in, layer):
         np.testing.assert_allclose(back.p[k][0, 0, layer, 0], values[k], rtol=1e-6)
     for k in set(values) - set(keys):
         assert np.all(np.isnan(back.p[k]))  # not in the file
+
+
+def _results_from_oracle(pam, freqs, nml):
+    """self.r as runPamtra fills it, from the oracle (the writer is host code)."""
+    out, st, _ = oracle.run(nml, pam.set, pam.df.data, pam.df.data4D, pam.p, freqs, nthreads=4)
+    assert st == 0
+    pam.set["freqs"], pam.set["nfreqs"] = list(freqs), len(freqs)
+    pam.set["radar_pol"] = nml["radar_polarisation"].split(",")
+    pam.set["radar_npol"], pam.set["att_npol"] = len(pam.set["radar_pol"]), 1
+    pam.r = {"tb": out["tb"], "angles_deg": out["angles_deg"], "pamtraVersion": "test",
+             "pamtraHash": "test", "nmlSettings": nml}
+    if nml["active"]:
+        pam.r.update(Ze=out["ze"], Att_hydro=out["att_hydro"], Att_atmo=out["att_atmo"],
+                     radar_hgt=out["radar_hgt"])
+    return pam
+
+
+@pytest.mark.parametrize("form,magic", [("NETCDF3_CLASSIC", b"CDF\x01"),
+                                        ("NETCDF3_64BIT", b"CDF\x02")])
+@pytest.mark.parametrize("active", [False, True])
+def test_results_netcdf_layout(tmp_path, form, magic, active):
+    """writeResultsToNetCDF (reference core.py:2163-2556): the dimensions,
+    variables, types, units and fill values of the reference's file, read
+    back with an independent netCDF-3 reader."""
+    from scipy.io import netcdf_file
+    from _scenario import FREQUENCIES, build_pamtra
+    pam = build_pamtra()
+    nml = dict(pam.nmlSet, active=active, radar_polarisation="NN,HV")
+    pam = _results_from_oracle(pam, FREQUENCIES, nml)
+    pam.p["iwv"] = np.full(pam._shape2D, 12.5)
+    fn = str(tmp_path / "out.nc")
+    pam.writeResultsToNetCDF(fn, ncForm=form)
+    with open(fn, "rb") as fh:
+        assert fh.read(4) == magic
+    with netcdf_file(fn, "r", mmap=False) as nc:
+        dims = {"grid_x": 1, "grid_y": 1, "frequency": 2, "angles": 32, "outlevels": 2,
+                "passive_polarisation": 2}
+        names = {"frequency", "grid_x", "grid_y", "angles", "passive_polarisation", "outlevels",
+                 "model_i", "model_j", "nlyr", "datatime", "longitude", "latitude", "sfc_type",
+                 "tb", "iwv"}
+        if active:
+            dims.update(heightbins=3, radar_polarisation=2, attenuation_polarisation=1,
+                        radar_peak_number=1)
+            names |= {"heightbins", "height", "Ze", "Attenuation_Hydrometeors",
+                      "Attenuation_Atmosphere"}
+        assert dict(nc.dimensions) == dims
+        assert set(nc.variables) == names
+        tb = nc.variables["tb"]
+        assert tb.dimensions == ("grid_x", "grid_y", "outlevels", "angles", "frequency",
+                                 "passive_polarisation")
+        assert tb.units == b"K" and tb.data.dtype == np.dtype(">f4")
+        assert tb._FillValue == np.float32(-9999.0)
+        np.testing.assert_array_equal(tb.data, pam.r["tb"].astype(np.float32))
+        assert nc.variables["nlyr"].data.dtype == np.dtype(">i4")
+        assert nc.variables["nlyr"]._FillValue == -9999
+        assert b"".join(nc.variables["passive_polarisation"].data.tolist()) == b"VH"
+        np.testing.assert_array_equal(nc.variables["outlevels"].data,
+                                      pam.p["obs_height"].astype(np.float32))
+        assert nc.variables["datatime"].units == b"seconds since 1970-01-01 00:00:00"
+        assert nc.history.startswith(b"Created with pyPamtra (Version: test")
+        if active:
+            np.testing.assert_array_equal(nc.variables["Ze"].data, pam.r["Ze"].astype(np.float32))
+            assert nc.variables["Ze"].units == b"dBz"
+            np.testing.assert_array_equal(nc.variables["Attenuation_Hydrometeors"].data,
+                                          pam.r["Att_hydro"].astype(np.float32))
+
+
+def test_results_netcdf_refusals(tmp_path):
+    from _scenario import build_pamtra
+    pam = build_pamtra()
+    with pytest.raises(IOError):
+        pam.writeResultsToNetCDF(str(tmp_path / "a.nc"))
+    pam.r = {"pamtraVersion": "x"}
+    with pytest.raises(NotImplementedError):
+        pam.writeResultsToNetCDF(str(tmp_path / "a.nc"), ncForm="NETCDF4")
+    with pytest.raises(ValueError):
+        pam.writeResultsToNetCDF(str(tmp_path / "a.nc"), ncForm="HDF9")

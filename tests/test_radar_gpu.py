@@ -75,3 +75,19 @@ def test_radar_mixed_precision_is_the_fp64_radar():
     assert st == st2 == 0
     for k in ("ze", "att_hydro", "att_atmo", "radar_hgt"):
         np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_results_netcdf_after_runpamtra(tmp_path):
+    """runPamtra's default active+passive results through writeResultsToNetCDF
+    (reference core.py:2163-2556), read back with scipy's netCDF-3 reader."""
+    from scipy.io import netcdf_file
+    pam = build_pamtra()
+    pam.runPamtra(FREQUENCIES)
+    fn = str(tmp_path / "golden.nc")
+    pam.writeResultsToNetCDF(fn)
+    with netcdf_file(fn, "r", mmap=False) as nc:
+        np.testing.assert_array_equal(nc.variables["tb"].data, pam.r["tb"].astype(np.float32))
+        np.testing.assert_array_equal(nc.variables["Ze"].data, pam.r["Ze"].astype(np.float32))
+        np.testing.assert_array_equal(nc.variables["height"].data,
+                                      pam.r["radar_hgt"].astype(np.float32))
+        np.testing.assert_allclose(nc.variables["tb"].data, GOLDEN["tb"], rtol=1e-5)
