@@ -265,13 +265,15 @@ class pyPamtra(object):
 
     def readPamtraProfile(self, inputFile):
         """Reads a .lay or .lev profile (core.py:339-504); the descriptor
-        file must be defined first.  The classic format is not supported."""
+        file must be defined first.  Any other extension is read as the
+        classic format (readClassicPamtraProfile)."""
         import calendar
         import datetime
         assert self.df.nhydro > 0
         kind = inputFile.split(".")[-1]
         if kind not in ("lay", "lev"):
-            raise NotImplementedError("only the .lay and .lev profile formats are supported")
+            self.readClassicPamtraProfile(inputFile)
+            return
         with open(inputFile) as fh:
             rows = (line.split() for line in fh if line.strip())
             nx, ny, L, nout = (int(v) for v in next(rows)[:4])
@@ -340,6 +342,69 @@ class pyPamtra(object):
         if kind == "lay":
             p["hgt_lev"][..., 1:-1] = 0.5 * (p["hgt"][..., :-1] + p["hgt"][..., 1:])
             p["hgt_lev"][..., -1] = p["hgt"][..., -1] + (p["hgt"][..., -1] - p["hgt"][..., -2]) * 0.5
+
+    def readClassicPamtraProfile(self, inputFile, n_moments=1):
+        """Reads the classic profile format (core.py:505-649): a header
+        "year month day hhmm ngridx ngridy nlyrs deltax deltay", then per
+        column the model indices, "lat lon lfrac wind10u wind10v", the paths
+        "iwv cwp iwp rwp swp gwp [hwp]", the ground level "hgt press temp
+        relhum[%]" and one line per layer with the upper level and the
+        contents of cloud, ice, rain, snow, graupel (and, with n_moments = 2,
+        hail and the six number concentrations).  The categories become
+        hydro_q / hydro_n in that order (4 + n_moments of them)."""
+        import calendar
+        import datetime
+        if n_moments not in (1, 2):
+            raise IOError("n_moments mus be 1 or 2")
+        with open(inputFile) as fh:
+            rows = (line.split() for line in fh if line.strip())
+            year, mon, day, hhmm, nx, ny, L = next(rows)[:7]
+            nx, ny, L = int(nx), int(ny), int(L)
+            p = self.p
+            p["ngridx"], p["ngridy"], p["max_nlyrs"], p["noutlevels"] = nx, ny, L, 2
+            nh = 4 + n_moments
+            self._shape2D = (nx, ny)
+            self._shape3D = (nx, ny, L)
+            self._shape3Dplus = (nx, ny, L + 1)
+            self._shape3Dout = (nx, ny, 2)
+            self._shape4D = (nx, ny, L, nh)
+            for k in ("model_i", "model_j", "lat", "lon", "wind10u", "wind10v", "iwv", "cwp",
+                      "iwp", "rwp", "swp", "gwp", "hwp"):
+                p[k] = np.zeros(self._shape2D)
+            p["sfc_type"] = np.full(self._shape2D, missingIntNumber, dtype=int)
+            p["radar_prop"] = np.full(self._shape2D + (2,), np.nan)
+            for k in ("hgt_lev", "temp_lev", "press_lev", "relhum_lev"):
+                p[k] = np.zeros(self._shape3Dplus)
+            q = np.zeros(self._shape4D)
+            n = np.zeros(self._shape4D)
+            paths = ("iwv", "cwp", "iwp", "rwp", "swp", "gwp", "hwp")[:6 + n_moments - 1]
+            for x in range(nx):
+                for y in range(ny):
+                    p["model_i"][x, y], p["model_j"][x, y] = (int(v) for v in next(rows))
+                    lat, lon, lfrac, u, v = (float(t) for t in next(rows))
+                    p["lat"][x, y], p["lon"][x, y] = lat, lon
+                    p["wind10u"][x, y], p["wind10v"][x, y] = u, v
+                    for key, val in zip(paths, next(rows), strict=True):
+                        p[key][x, y] = float(val)
+                    (p["hgt_lev"][x, y, 0], p["press_lev"][x, y, 0], p["temp_lev"][x, y, 0],
+                     p["relhum_lev"][x, y, 0]) = (float(t) for t in next(rows))
+                    p["sfc_type"][x, y] = np.around(lfrac)  # lfrac is deprecated
+                    for z in range(L):
+                        vals = [float(t) for t in next(rows)]
+                        if len(vals) != 4 + nh + (nh if n_moments == 2 else 0):
+                            raise IOError(f"{inputFile}: {len(vals)} values on a layer line")
+                        (p["hgt_lev"][x, y, z + 1], p["press_lev"][x, y, z + 1],
+                         p["temp_lev"][x, y, z + 1], p["relhum_lev"][x, y, z + 1]) = vals[:4]
+                        q[x, y, z, :] = vals[4:4 + nh] if n_moments == 2 else vals[4:]
+                        if n_moments == 2:
+                            n[x, y, z, :] = vals[4 + nh:]
+        p["relhum_lev"] = p["relhum_lev"] / 100.
+        p["nlyrs"] = np.full(self._shape2D, L, dtype=int)
+        p["unixtime"] = np.full(self._shape2D, calendar.timegm(datetime.datetime(
+            int(year), int(mon), int(day), int(hhmm[0:2]), int(hhmm[2:4])).timetuple()), dtype=int)
+        p["hydro_q"], p["hydro_n"], p["hydro_reff"] = q, n, np.zeros(self._shape4D)
+        for k in ("airturb", "wind_w", "wind_uv", "turb_edr"):
+            p[k] = np.full(self._shape3D, np.nan)
 
     def writePamtraProfile(self, profileFile):
         """Writes the profile as .lay or .lev, by the file extension

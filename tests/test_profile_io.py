@@ -77,12 +77,61 @@ def test_lay_profile_round_trip(tmp_path):
     assert d < 0.02, d
 
 
-def test_profile_reader_refuses_other_formats(tmp_path):
-    pam, _ = workloads.config1(ncol=1)
-    f = tmp_path / "profile.dat"
-    f.write_text("1 1 2 2\n")
-    with pytest.raises(NotImplementedError):
-        pam.readPamtraProfile(str(f))
+def _classic_file(path, n_moments):
+    """A 2x1-column, 2-layer classic profile; layer z of column c holds
+    contents 1e-5 * (10 c + z + 1) * (category + 1)."""
+    nh = 4 + n_moments
+    lines = ["2013 06 11 1430 2 1 2 1000.0 1000.0"]
+    for c in range(2):
+        lines += [f"{c + 1} 1", f"{50.0 + c} {7.0 + c} {float(c)} 1.5 -2.5",
+                  " ".join(str(0.1 * (k + 1) + c) for k in range(5 + n_moments)),
+                  f"{10.0 * c} 101325.0 {288.0 + c} 80.0"]
+        for z in range(2):
+            qn = [1e-5 * (10 * c + z + 1) * (h + 1) for h in range(nh)]
+            if n_moments == 2:
+                qn += [1e3 * (10 * c + z + 1) * (h + 1) for h in range(nh)]
+            lines.append(" ".join(str(v) for v in [1000.0 * (z + 1) + c, 90000.0 - 10000 * z,
+                                                    280.0 - 6 * z, 70.0 - z] + qn))
+    path.write_text("\n".join(lines) + "\n")
+
+
+@pytest.mark.parametrize("n_moments", [1, 2])
+def test_classic_profile_format(tmp_path, n_moments):
+    """readClassicPamtraProfile (reference core.py:505-649): header, per
+    column indices, position and surface, the paths, the ground level and
+    one line per layer; relative humidity given in percent; the categories
+    cloud, ice, rain, snow, graupel (and hail) as hydro_q / hydro_n."""
+    import calendar
+    f = tmp_path / "profile.cla"
+    _classic_file(f, n_moments)
+    pam = _one_category(3)
+    if n_moments == 1:
+        pam.readPamtraProfile(str(f))  # any extension but .lev/.lay
+    else:
+        pam.readClassicPamtraProfile(str(f), n_moments=2)
+    p = pam.p
+    nh = 4 + n_moments
+    assert (p["ngridx"], p["ngridy"], p["max_nlyrs"], p["noutlevels"]) == (2, 1, 2, 2)
+    assert p["hydro_q"].shape == (2, 1, 2, nh)
+    np.testing.assert_array_equal(p["nlyrs"], 2)
+    assert p["unixtime"][0, 0] == calendar.timegm((2013, 6, 11, 14, 30, 0))
+    np.testing.assert_array_equal(p["model_i"][:, 0], [1, 2])
+    np.testing.assert_array_equal(p["lat"][:, 0], [50.0, 51.0])
+    np.testing.assert_array_equal(p["sfc_type"][:, 0], [0, 1])
+    np.testing.assert_array_equal(p["wind10v"], -2.5)
+    np.testing.assert_allclose(p["iwv"][:, 0], [0.1, 1.1])
+    np.testing.assert_allclose(p["gwp"][:, 0], [0.6, 1.6])
+    np.testing.assert_allclose(p["hwp"][:, 0], [0.0, 0.0] if n_moments == 1 else [0.7, 1.7])
+    np.testing.assert_array_equal(p["hgt_lev"][1, 0], [10.0, 1001.0, 2001.0])
+    np.testing.assert_allclose(p["relhum_lev"][0, 0], [0.8, 0.7, 0.69])
+    np.testing.assert_array_equal(p["temp_lev"][1, 0], [289.0, 280.0, 274.0])
+    for c in range(2):
+        for z in range(2):
+            np.testing.assert_allclose(p["hydro_q"][c, 0, z],
+                                       1e-5 * (10 * c + z + 1) * (np.arange(nh) + 1))
+            want_n = 1e3 * (10 * c + z + 1) * (np.arange(nh) + 1) if n_moments == 2 else 0.0
+            np.testing.assert_allclose(p["hydro_n"][c, 0, z], want_n)
+    assert np.all(np.isnan(p["airturb"]))
 
 
 def test_observation_heights_become_levels():
