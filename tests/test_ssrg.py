@@ -276,3 +276,26 @@ def test_refused_inputs_on_the_device(case):
     c, st2 = _cpu(pam, FREQS)
     assert st != 0 and st2 != 0
     assert "rayleigh_gans" in g["msg"], g["msg"]
+
+
+@pytest.mark.gpu
+def test_aggregates_at_32_angles_vs_oracle():
+    """The 32-angle library (64x64 operators) with the aggregate producer,
+    against the 32-angle oracle."""
+    from pamtra_amd import _lib
+    pam = _rain_and_aggregates("ssrg-rt3_0.19_0.23_1.67")
+    sets = dict(pam.set, _nangles=32)
+    m = _capi.marshal(passive_nml(pam), sets, pam.df.data, pam.df.data4D, pam.p)
+    assert m.nummu == 32
+    lib, ctx = _lib.library(32), _lib.get_context(0, 32)
+    f, fp = _capi.freqs_array(FREQS)
+    hr = _capi.alloc_results(m, len(f), diagnostics=True)
+    print("pam_run ssrg 32 angles", flush=True)
+    st = lib.pam_run(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d), C.byref(m.profile),
+                     fp, len(f), C.byref(hr.struct))
+    g = dict(hr.arrays)
+    c, st2, _ = oracle.run(passive_nml(pam), sets, pam.df.data, pam.df.data4D, pam.p, FREQS,
+                           nthreads=8, diagnostics=True)
+    assert st == 0 and st2 == 0
+    assert g["tb"].shape[3] == 64
+    _compare(g, c)
