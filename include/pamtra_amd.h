@@ -130,6 +130,18 @@ typedef struct pam_hydro_4d {
   const double *p_1, *p_2, *p_3, *p_4;
   const double *d_1, *d_2;
   const int32_t *nbin;
+  /* Full-spectrum input (nmlSet hydro_fullspec with pyPamtra
+   * df.addFullSpectra; src/hydrometeor_extinction.f90:120-150,
+   * src/scatProperties.f90:258-270).  fs_nbin > 0 replaces the size
+   * distribution of every hydrometeor by fs_nbin given bins: arrays of shape
+   * (ngridx, ngridy, max_nlyrs, nhydro, fs_nbin), fs_d_bound with fs_nbin + 1
+   * edges.  fs_n is the number per bin [m^-3] (not normalised); fs_rho and
+   * fs_d the density and diameter the Mie spheres take; fs_mass, fs_as_ratio,
+   * fs_canting and fs_rg_* the self-similar Rayleigh-Gans inputs. */
+  int32_t fs_nbin;
+  int32_t fs_pad_;
+  const double *fs_rho, *fs_d, *fs_d_bound, *fs_n, *fs_mass, *fs_as_ratio, *fs_canting;
+  const double *fs_rg_kappa, *fs_rg_beta, *fs_rg_gamma, *fs_rg_zeta;
 } pam_hydro_4d;
 
 /* The profile dictionary `pam.p` (python/pyPamtra/core.py:756-976).  NaN

@@ -85,10 +85,10 @@ def _dp(a):
 
 
 def run(nmlSets, sets, descriptorFile, descriptorFile4D, profile, freqs, nthreads=1,
-        diagnostics=False):
+        diagnostics=False, descriptorFileFS=None):
     """CPU forward operator on the same inputs as the product's
     PamtraFortranWrapper.  Returns (results dict, status, work counters)."""
-    m = _capi.marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile)
+    m = _capi.marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile, descriptorFileFS)
     f, fp = _capi.freqs_array(freqs)
     hr = _capi.alloc_results(m, len(f), diagnostics=diagnostics)
     work = (C.c_int64 * 4)()

@@ -775,14 +775,16 @@ double pam_oracle_ssrg_shape(double x, double kappa, double beta, double gamma, 
  * hold the parsed kappa, beta, gamma, zeta. */
 static int calc_ssrg_rt3(double freq_hz, int liq_ice, int nbins, const double *diameter,
                          const double *del_d, const double *ndens, const double *mass,
-                         double as_ratio, double canting, double refre, double refim,
-                         double kappa, double beta, double gamma, double zeta, int lhyd_absorption,
-                         mie_result *res, cplx *wa, cplx *wb, cplx *wd, char *msg) {
+                         const double *as_ratio, const double *canting, double refre, double refim,
+                         const double *kappa, const double *beta, const double *gamma,
+                         const double *zeta, int lhyd_absorption, mie_result *res, cplx *wa,
+                         cplx *wb, cplx *wd, char *msg) {
   memset(res, 0, sizeof(*res));
-  int bad = liq_ice != -1 || !(canting == 0 || canting == 90) || !(as_ratio > 0.0) || nbins <= 0 ||
-            !(freq_hz > 0);
+  int bad = liq_ice != -1 || nbins <= 0 || !(freq_hz > 0);
   for (int i = 0; i < nbins; i++)
-    if (!(mass[i] >= 0) || !(ndens[i] >= 0) || !(diameter[i] > 0) || !(del_d[i] > 0)) bad = 1;
+    if (!(mass[i] >= 0) || !(ndens[i] >= 0) || !(diameter[i] > 0) || !(del_d[i] > 0) ||
+        !(canting[i] == 0 || canting[i] == 90) || !(as_ratio[i] > 0.0))
+      bad = 1;
   if (bad) {
     snprintf(msg, 120, "calc_self_similar_rayleigh_gans_rt3: assertation error");
     return PAM_FATAL;
@@ -810,7 +812,7 @@ static int calc_ssrg_rt3(double freq_hz, int liq_ice, int nbins, const double *d
   for (int ii = 0; ii < nbins; ii++) {
     if (ndens[ii] <= 0) continue;
     double nd = ndens[ii], dd = del_d[ii];
-    double d_wave = (canting == 0) ? diameter[ii] * as_ratio : diameter[ii];
+    double d_wave = (canting[ii] == 0) ? diameter[ii] * as_ratio[ii] : diameter[ii];
     double volume = mass[ii] / RHO_ICE;
     double prefactor = 9.0 * pow(wave_num, 4) * K2 * sq(volume) / (4.0 * PI);
     double cabs_ = 3.0 * wave_num * volume * cimag(K);
@@ -818,7 +820,8 @@ static int calc_ssrg_rt3(double freq_hz, int liq_ice, int nbins, const double *d
     for (int ia = 2; ia <= jmax; ia++) {
       double th = (double)(ia - 1) * PI / (double)(jmax - 1);
       double xs = wave_num * d_wave * sin(th * 0.5);
-      phas = prefactor * pam_oracle_ssrg_shape(xs, kappa, beta, gamma, zeta) * (1.0 + sq(cos(th))) * 0.5;
+      phas = prefactor * pam_oracle_ssrg_shape(xs, kappa[ii], beta[ii], gamma[ii], zeta[ii]) *
+             (1.0 + sq(cos(th))) * 0.5;
       qscat = qscat + phas * sin(th);
     }
     double qback = phas;
@@ -830,7 +833,7 @@ static int calc_ssrg_rt3(double freq_hz, int liq_ice, int nbins, const double *d
     for (int ia = 0; ia < nquad; ia++) {
       double th = acos(mu[ia]);
       double xs = wave_num * d_wave * sin(th * 0.5);
-      double shape = pam_oracle_ssrg_shape(xs, kappa, beta, gamma, zeta);
+      double shape = pam_oracle_ssrg_shape(xs, kappa[ii], beta[ii], gamma[ii], zeta[ii]);
       cplx s22 = -I * 3.0 * pow(wave_num, 3) * K * volume * sqrt(shape) / (4.0 * PI);
       cplx s11 = s22 * cos(th);
       sump[ia] = sump[ia] + 0.5 * (sq(cabs(s11)) + sq(cabs(s22))) * nd * dd;
@@ -868,11 +871,22 @@ int pam_oracle_ssrg(double freq_hz, int nbins, const double *diameter, const dou
                     double refre, double refim, double kappa, double beta, double gamma,
                     double zeta, double *out, double *legen, int *nl) {
   cplx *buf = (cplx *)malloc(sizeof(cplx) * 3 * (MIE_MAXTERMS + 20));
+  double *lists = (double *)malloc(sizeof(double) * 6 * nbins);
+  for (int i = 0; i < nbins; i++) {
+    lists[i] = as_ratio;
+    lists[nbins + i] = canting;
+    lists[2 * nbins + i] = kappa;
+    lists[3 * nbins + i] = beta;
+    lists[4 * nbins + i] = gamma;
+    lists[5 * nbins + i] = zeta;
+  }
   mie_result r;
   char msg[128];
-  int st = calc_ssrg_rt3(freq_hz, -1, nbins, diameter, del_d, ndens, mass, as_ratio, canting,
-                         refre, refim, kappa, beta, gamma, zeta, 1, &r, buf,
-                         buf + MIE_MAXTERMS + 20, buf + 2 * (MIE_MAXTERMS + 20), msg);
+  int st = calc_ssrg_rt3(freq_hz, -1, nbins, diameter, del_d, ndens, mass, lists, lists + nbins,
+                         refre, refim, lists + 2 * nbins, lists + 3 * nbins, lists + 4 * nbins,
+                         lists + 5 * nbins, 1, &r, buf, buf + MIE_MAXTERMS + 20,
+                         buf + 2 * (MIE_MAXTERMS + 20), msg);
+  free(lists);
   free(buf);
   out[0] = r.extinction;
   out[1] = r.albedo;
@@ -1806,6 +1820,30 @@ typedef struct {
   double coef[3][MAXNLEG + 1];
 } layer_optics;
 
+/* One hydrometeor's optics into the layer sums (scatProperties.f90:920-946). */
+static int accumulate_hydro(layer_optics *lo, const mie_result *mr, double *kexttot,
+                            double *salbedo, int *nleg_coef, char *msg) {
+  if (mr->nlegen > *nleg_coef) *nleg_coef = mr->nlegen;
+  lo->back = lo->back + mr->back_scatt;
+  *kexttot = *kexttot + mr->extinction;
+  if (*kexttot == 0.0)
+    *salbedo = 0.0;
+  else
+    *salbedo = *salbedo + (mr->albedo * mr->extinction);
+  for (int jj = 0; jj < *nleg_coef; jj++)
+    for (int k = 0; k < 3; k++)
+      lo->coef[k][jj] = lo->coef[k][jj] + (mr->legen[k][jj] * mr->albedo * mr->extinction);
+  if (*kexttot < 0.0 || isnan(*kexttot)) {
+    snprintf(msg, 120, "rt_kexttot smaller than zero or nan");
+    return PAM_FATAL;
+  }
+  if (*nleg_coef > MAXNLEG - 1) {
+    snprintf(msg, 120, "nlegen_coef greater maxnleg");
+    return PAM_FATAL;
+  }
+  return PAM_SUCCESS;
+}
+
 /* hydrometeor_extinction (src/hydrometeor_extinction.f90:106-364) for one
  * layer (bottom-up index z) and frequency, scatProperties accumulation
  * (scatProperties.f90:920-946) and finalize (:993-1002). */
@@ -1845,8 +1883,60 @@ static int layer_hydrometeors(const pam_settings *s, const pam_hydro_desc *hyd,
     hp.moment_in = d->moment_in;
     hp.dist_name = d->dist_name;
     if (hp.dist_name == PAM_DIST_MONO) hp.nbin = 2;
-    if (hp.dist_name == PAM_DIST_DISABLED) continue;
     double T = c->temp[z], P = c->press[z], qv = c->q_hum[z];
+    /* full spectrum given (hydrometeor_extinction.f90:120-150): the bins
+     * replace the size distribution; present when they hold particles or
+     * the threshold is not positive */
+    const int fs = h4 && h4->fs_nbin > 0;
+    const size_t b5 = (base4 + h) * (size_t)(fs ? h4->fs_nbin : 0);
+    if (fs) {
+      int nb = h4->fs_nbin;
+      double nsum = 0.0;
+      for (int i = 0; i < nb; i++) nsum += h4->fs_n[b5 + i];
+      if (!(nsum > 0.0) && s->hydro_threshold > 0) continue;
+      lo->present = 1;
+      if (d->scat_name == PAM_SCAT_DISABLED) continue;
+      double *fb = (double *)malloc(sizeof(double) * 2 * nb);
+      double *fdd = fb, *fnd = fb + nb;
+      const double *bound = h4->fs_d_bound + (base4 + h) * (size_t)(nb + 1);
+      for (int i = 0; i < nb; i++) {
+        fdd[i] = bound[i + 1] - bound[i];
+        fnd[i] = h4->fs_n[b5 + i] / fdd[i];
+      }
+      double nr, ni;
+      if (hp.liq_ice == 1)
+        ref_water(T - SP(273.15), freq, &nr, &ni);
+      else
+        ref_ice(T, freq, &nr, &ni);
+      mie_result mr;
+      int st;
+      if (d->scat_name == PAM_SCAT_SSRG_RT3) {
+        double *ll = (double *)malloc(sizeof(double) * 2 * nb);
+        for (int i = 0; i < nb; i++) {
+          double a = h4->fs_as_ratio[b5 + i], cn = h4->fs_canting[b5 + i];
+          ll[i] = (a < 0 || isnan(a)) ? 0.0 : a;
+          ll[nb + i] = (cn < 0 || isnan(cn)) ? 0.0 : cn;
+        }
+        st = calc_ssrg_rt3(freq * 1e9, hp.liq_ice, nb, h4->fs_d + b5, fdd, fnd, h4->fs_mass + b5,
+                           ll, ll + nb, nr, ni, h4->fs_rg_kappa + b5, h4->fs_rg_beta + b5,
+                           h4->fs_rg_gamma + b5, h4->fs_rg_zeta + b5, s->lhyd_absorption, &mr, wa,
+                           wb, wd, msg);
+        free(ll);
+      } else {
+        st = calc_mie_spheres(freq * 1e9, hp.liq_ice, nb, h4->fs_d + b5, fdd, fnd,
+                              h4->fs_rho + b5, nr, ni, s->lhyd_absorption, &mr, wa, wb, wd, msg);
+      }
+      free(fb);
+      if (st == PAM_FATAL) return st;
+      if (work) {
+        work[0] += nb;
+        work[1] += mr.sum_series;
+      }
+      st = accumulate_hydro(lo, &mr, &kexttot, &salbedo, &nleg_coef, msg);
+      if (st == PAM_FATAL) return st;
+      continue;
+    }
+    if (hp.dist_name == PAM_DIST_DISABLED) continue;
     double qall = (s->hydro_includehydroinrhoair && !qnan) ? qsum : 0.0;
     double qin = p->hydro_q ? p->hydro_q[base4 + h] : NAN;
     double nin = p->hydro_n ? p->hydro_n[base4 + h] : NAN;
@@ -1886,12 +1976,20 @@ static int layer_hydrometeors(const pam_settings *s, const pam_hydro_desc *hyd,
        * (scatProperties.f90:276-287); mass_ds (drop_size_dist.f90:162-165) */
       double ar = (hp.as_ratio < 0 || isnan(hp.as_ratio)) ? 0.0 : hp.as_ratio;
       double cant = (d->canting < 0 || isnan(d->canting)) ? 0.0 : d->canting;
-      double *mass = (double *)malloc(sizeof(double) * nb);
-      for (int i = 0; i < nb; i++) mass[i] = hp.a_ms * pow(bins.d_ds[i], hp.b_ms);
+      double *ll = (double *)malloc(sizeof(double) * 7 * nb);
+      for (int i = 0; i < nb; i++) {
+        ll[i] = hp.a_ms * pow(bins.d_ds[i], hp.b_ms);
+        ll[nb + i] = ar;
+        ll[2 * nb + i] = cant;
+        ll[3 * nb + i] = d->rg_kappa;
+        ll[4 * nb + i] = d->rg_beta;
+        ll[5 * nb + i] = d->rg_gamma;
+        ll[6 * nb + i] = d->rg_zeta;
+      }
       st = calc_ssrg_rt3(freq * 1e9, hp.liq_ice, nb, bins.diameter2scat, bins.delta_d_ds, ndens,
-                         mass, ar, cant, nr, ni, d->rg_kappa, d->rg_beta, d->rg_gamma, d->rg_zeta,
-                         s->lhyd_absorption, &mr, wa, wb, wd, msg);
-      free(mass);
+                         ll, ll + nb, ll + 2 * nb, nr, ni, ll + 3 * nb, ll + 4 * nb, ll + 5 * nb,
+                         ll + 6 * nb, s->lhyd_absorption, &mr, wa, wb, wd, msg);
+      free(ll);
     } else {
       st = calc_mie_spheres(freq * 1e9, hp.liq_ice, nb, bins.diameter2scat, bins.delta_d_ds,
                             ndens, bins.density2scat, nr, ni, s->lhyd_absorption, &mr, wa, wb, wd,
@@ -1904,24 +2002,8 @@ static int layer_hydrometeors(const pam_settings *s, const pam_hydro_desc *hyd,
       work[0] += nb;
       work[1] += mr.sum_series;
     }
-    if (mr.nlegen > nleg_coef) nleg_coef = mr.nlegen;
-    lo->back = lo->back + mr.back_scatt;
-    kexttot = kexttot + mr.extinction;
-    if (kexttot == 0.0)
-      salbedo = 0.0;
-    else
-      salbedo = salbedo + (mr.albedo * mr.extinction);
-    for (int jj = 0; jj < nleg_coef; jj++)
-      for (int k = 0; k < 3; k++)
-        lo->coef[k][jj] = lo->coef[k][jj] + (mr.legen[k][jj] * mr.albedo * mr.extinction);
-    if (kexttot < 0.0 || isnan(kexttot)) {
-      snprintf(msg, 120, "rt_kexttot smaller than zero or nan");
-      return PAM_FATAL;
-    }
-    if (nleg_coef > MAXNLEG - 1) {
-      snprintf(msg, 120, "nlegen_coef greater maxnleg");
-      return PAM_FATAL;
-    }
+    st = accumulate_hydro(lo, &mr, &kexttot, &salbedo, &nleg_coef, msg);
+    if (st == PAM_FATAL) return st;
   }
   /* finalize_rt3_scatProperties */
   salbedo = salbedo / kexttot;

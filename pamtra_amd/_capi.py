@@ -102,8 +102,18 @@ HYDRO4D_KEYS = ["as_ratio", "rho_ms", "a_ms", "b_ms", "alpha_as", "beta_as",
                 "p_1", "p_2", "p_3", "p_4", "d_1", "d_2"]
 
 
+# full-spectrum arrays (descriptorFile.addFullSpectra) -> pam_hydro_4d fields;
+# area_ds and fallvelocity enter only the radar spectra, which are not built
+FULLSPEC_KEYS = {"rho_ds": "fs_rho", "d_ds": "fs_d", "d_bound_ds": "fs_d_bound", "n_ds": "fs_n",
+                 "mass_ds": "fs_mass", "as_ratio": "fs_as_ratio", "canting": "fs_canting",
+                 "rg_kappa_ds": "fs_rg_kappa", "rg_beta_ds": "fs_rg_beta",
+                 "rg_gamma_ds": "fs_rg_gamma", "rg_zeta_ds": "fs_rg_zeta"}
+
+
 class PamHydro4D(C.Structure):
-    _fields_ = [(k, _dp) for k in HYDRO4D_KEYS] + [("nbin", _ip)]
+    _fields_ = [(k, _dp) for k in HYDRO4D_KEYS] + [("nbin", _ip), ("fs_nbin", C.c_int32),
+                                                   ("fs_pad_", C.c_int32)] + \
+        [(v, _dp) for v in FULLSPEC_KEYS.values()]
 
 
 PROFILE_DOUBLE_KEYS = ["hgt_lev", "temp_lev", "press_lev", "relhum_lev", "hgt", "temp",
@@ -327,9 +337,12 @@ class Marshalled:
     nummu: int = NUMMU       # quadrature angles per hemisphere (sets["_nangles"])
 
 
-def marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile) -> Marshalled:
+def marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile,
+            descriptorFileFS=None) -> Marshalled:
     """Build the C structs from the pyPamtra dictionaries.  The returned
-    object keeps every array alive for the duration of the call."""
+    object keeps every array alive for the duration of the call.  With
+    nmlSet hydro_fullspec the full-spectrum arrays of descriptorFileFS
+    (descriptorFile.addFullSpectra) replace the size distributions."""
     keep = []
     nx, ny = int(profile["ngridx"]), int(profile["ngridy"])
     L = int(profile["max_nlyrs"])
@@ -350,6 +363,19 @@ def marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile) -> Marshal
             setattr(h4, key, _ptr(a, C.c_double))
         else:
             raise NotImplementedError(f"4D descriptor field {key!r} is not supported")
+    if nmlSets.get("hydro_fullspec", False):
+        fs = descriptorFileFS or {}
+        missing = [k for k in FULLSPEC_KEYS if k not in fs]
+        if missing:
+            raise ValueError(f"hydro_fullspec needs df.addFullSpectra(); missing {missing}")
+        nb = int(np.shape(fs["d_ds"])[-1])
+        if nb < 1:
+            raise ValueError("hydro_fullspec: no bins")
+        h4.fs_nbin = nb
+        for key, fld in FULLSPEC_KEYS.items():
+            a = _as_f64(fs[key], (nx, ny, L, nh, nb + 1 if key == "d_bound_ds" else nb))
+            keep.append(a)
+            setattr(h4, fld, _ptr(a, C.c_double))
     p = PamProfile()
     p.ngridx, p.ngridy, p.max_nlyrs, p.noutlevels, p.nhydro = nx, ny, L, nout, nh
     shapes = {

@@ -572,6 +572,18 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     HIP_OK(hipMemcpy(hb.data(), h4in->nbin, hb.size() * 4, hipMemcpyDeviceToHost));
     for (int32_t v : hb) d.nbmax = std::max(d.nbmax, (int)v);
   }
+  if (h4in && h4in->fs_nbin > 0) {
+    const double *const fs[] = {h4in->fs_rho,     h4in->fs_d,         h4in->fs_d_bound,
+                                h4in->fs_n,       h4in->fs_mass,      h4in->fs_as_ratio,
+                                h4in->fs_canting, h4in->fs_rg_kappa,  h4in->fs_rg_beta,
+                                h4in->fs_rg_gamma, h4in->fs_rg_zeta};
+    for (const double *a : fs)
+      if (!a) {
+        set_msg(res, "hydro_fullspec: every full-spectrum array is required");
+        return PAM_FATAL;
+      }
+    d.nbmax = std::max(d.nbmax, (int)h4in->fs_nbin);
+  }
   d.col0 = 0;
   d.ncol_chunk = d.ncol;
   d.nitems = 0;
@@ -645,6 +657,18 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     h4.d_1 = h4in->d_1;
     h4.d_2 = h4in->d_2;
     h4.nbin = h4in->nbin;
+    h4.fs_nbin = h4in->fs_nbin > 0 ? h4in->fs_nbin : 0;
+    h4.fs_rho = h4in->fs_rho;
+    h4.fs_d = h4in->fs_d;
+    h4.fs_d_bound = h4in->fs_d_bound;
+    h4.fs_n = h4in->fs_n;
+    h4.fs_mass = h4in->fs_mass;
+    h4.fs_as_ratio = h4in->fs_as_ratio;
+    h4.fs_canting = h4in->fs_canting;
+    h4.fs_rg_kappa = h4in->fs_rg_kappa;
+    h4.fs_rg_beta = h4in->fs_rg_beta;
+    h4.fs_rg_gamma = h4in->fs_rg_gamma;
+    h4.fs_rg_zeta = h4in->fs_rg_zeta;
   }
 
   // small buffers: hydro table, frequencies, counters
@@ -970,6 +994,17 @@ static const char *first_host_array(const pam_hydro_4d *h4, const pam_profile *p
       {h4 ? h4->p_3 : nullptr, "hydro4d.p_3"},     {h4 ? h4->p_4 : nullptr, "hydro4d.p_4"},
       {h4 ? h4->d_1 : nullptr, "hydro4d.d_1"},     {h4 ? h4->d_2 : nullptr, "hydro4d.d_2"},
       {h4 ? h4->nbin : nullptr, "hydro4d.nbin"},
+      {h4 ? h4->fs_rho : nullptr, "hydro4d.fs_rho"},
+      {h4 ? h4->fs_d : nullptr, "hydro4d.fs_d"},
+      {h4 ? h4->fs_d_bound : nullptr, "hydro4d.fs_d_bound"},
+      {h4 ? h4->fs_n : nullptr, "hydro4d.fs_n"},
+      {h4 ? h4->fs_mass : nullptr, "hydro4d.fs_mass"},
+      {h4 ? h4->fs_as_ratio : nullptr, "hydro4d.fs_as_ratio"},
+      {h4 ? h4->fs_canting : nullptr, "hydro4d.fs_canting"},
+      {h4 ? h4->fs_rg_kappa : nullptr, "hydro4d.fs_rg_kappa"},
+      {h4 ? h4->fs_rg_beta : nullptr, "hydro4d.fs_rg_beta"},
+      {h4 ? h4->fs_rg_gamma : nullptr, "hydro4d.fs_rg_gamma"},
+      {h4 ? h4->fs_rg_zeta : nullptr, "hydro4d.fs_rg_zeta"},
       {r->tb, "results.tb"},                 {r->emissivity, "results.emissivity"},
       {r->angles_deg, "results.angles_deg"}, {r->status, "results.status"},
       {r->kextatmo, "results.kextatmo"},     {r->kexttot, "results.kexttot"},
@@ -1054,6 +1089,20 @@ extern "C" int pam_run(pam_context *c, const pam_settings *st, const pam_hydro_d
   reg((const void *const *)&d4.d_1, n4 * 8);
   reg((const void *const *)&d4.d_2, n4 * 8);
   reg((const void *const *)&d4.nbin, n4 * 4);
+  if (d4.fs_nbin > 0) {
+    const size_t n5 = n4 * (size_t)d4.fs_nbin;
+    reg((const void *const *)&d4.fs_rho, n5 * 8);
+    reg((const void *const *)&d4.fs_d, n5 * 8);
+    reg((const void *const *)&d4.fs_d_bound, (n5 + n4) * 8);
+    reg((const void *const *)&d4.fs_n, n5 * 8);
+    reg((const void *const *)&d4.fs_mass, n5 * 8);
+    reg((const void *const *)&d4.fs_as_ratio, n5 * 8);
+    reg((const void *const *)&d4.fs_canting, n5 * 8);
+    reg((const void *const *)&d4.fs_rg_kappa, n5 * 8);
+    reg((const void *const *)&d4.fs_rg_beta, n5 * 8);
+    reg((const void *const *)&d4.fs_rg_gamma, n5 * 8);
+    reg((const void *const *)&d4.fs_rg_zeta, n5 * 8);
+  }
   pam_results dr = *res;
   struct Out {
     void *h;

@@ -235,7 +235,7 @@ __device__ __forceinline__ double q2abs(double spec, double t, double p, double 
 // One thread per column: which hydrometeors are present in each layer
 // (hydrometeor_extinction.f90:236-285) and the number of scattering layers.
 __global__ void k_layer_flags(Dims d, Settings s, Profile p, const pam_hydro_desc *hyd,
-                              Work w) {
+                              Hydro4D h4, Work w) {
   int cl = blockIdx.x * blockDim.x + threadIdx.x;
   if (cl >= d.ncol_chunk) return;
   int col = d.col0 + cl;
@@ -260,7 +260,15 @@ __global__ void k_layer_flags(Dims d, Settings s, Profile p, const pam_hydro_des
       size_t oy = (size_t)col * d.ML + z;
       double T = w.temp[oy], P = w.press[oy], qv = w.q_hum[oy];
       double qall = (s.incl_hydro_rho && !qnan) ? qsum : 0.0;
-      for (int hh = 0; hh < d.nh; hh++) {
+      for (int hh = 0; hh < d.nh && h4.fs_nbin > 0; hh++) {
+        // full spectrum: present when its bins hold particles or the
+        // threshold is not positive (hydrometeor_extinction.f90:141-148)
+        const double *n = h4.fs_n + (b4 + hh) * (size_t)h4.fs_nbin;
+        double nsum = 0.0;
+        for (int i = 0; i < h4.fs_nbin; i++) nsum += n[i];
+        if (nsum > 0.0 || s.threshold <= 0.0) mask |= (1u << hh);
+      }
+      for (int hh = 0; hh < d.nh && h4.fs_nbin == 0; hh++) {
         const pam_hydro_desc &hd = hyd[hh];
         if (hd.dist_name == PAM_DIST_DISABLED) continue;
         double qin = p.hydro_q ? p.hydro_q[b4 + hh] : __builtin_nan("");
@@ -547,10 +555,9 @@ static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1
 
 void launch_columns(const Dims &d, const Settings &s, const Profile &p, const pam_hydro_desc *hyd,
                     const Hydro4D &h4, const double *freqs, Work &w, hipStream_t st) {
-  (void)h4;
   hipLaunchKernelGGL(k_fill_columns, dim3(nblk(d.ncol_chunk, 128)), dim3(128), 0, st, d, p, w);
   hipLaunchKernelGGL(k_layer_flags, dim3(nblk(d.ncol_chunk, 128)), dim3(128), 0, st, d, s, p, hyd,
-                     w);
+                     h4, w);
   size_t ng = (size_t)d.ncol_chunk * d.nf * d.ML;
   hipLaunchKernelGGL(k_gas, dim3(nblk(ng, 256)), dim3(256), 0, st, d, s, freqs, w);
 }

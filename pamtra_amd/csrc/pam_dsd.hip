@@ -301,6 +301,30 @@ __global__ void __launch_bounds__(128) k_dsd(Dims d, Settings s, Profile p,
   // beam (scatProperties.f90:276, 286-287: negative or NaN reads as 0)
   const double ar4 = h4val(h4.as_ratio, hd.as_ratio, i4);
   w.dsd_ar[(size_t)li * d.nh + hh] = (ar4 < 0.0 || isnan(ar4)) ? 0.0 : ar4;
+  if (h4.fs_nbin > 0) {
+    // full spectrum given (hydrometeor_extinction.f90:120-150): the bins
+    // replace the size distribution; the scattering diameter and density
+    // are d_ds and rho_ds, delta_d the edge differences, ndens n / delta_d
+    const int nb = h4.fs_nbin;
+    const size_t b5 = i4 * (size_t)nb, b6 = i4 * (size_t)(nb + 1);
+    const size_t o5 = ((size_t)li * d.nh + hh) * d.nbmax;
+    double *fd = w.dsd_d + o5, *fdd = w.dsd_dd + o5, *fnd = w.dsd_nd + o5, *frho = w.dsd_rho + o5;
+    const bool ssrg = hd.scat_name == PAM_SCAT_SSRG_RT3;
+    for (int i = 0; i < nb; i++) {
+      fd[i] = h4.fs_d[b5 + i];
+      fdd[i] = h4.fs_d_bound[b6 + i + 1] - h4.fs_d_bound[b6 + i];
+      fnd[i] = h4.fs_n[b5 + i] / fdd[i];
+      frho[i] = ssrg ? h4.fs_mass[b5 + i] : h4.fs_rho[b5 + i];
+    }
+    for (int i = nb; i < d.nbmax; i++) {
+      fd[i] = 1.0;
+      fdd[i] = 1.0;
+      fnd[i] = 0.0;
+      frho[i] = 1000.0;
+    }
+    *flag = (hd.scat_name != PAM_SCAT_DISABLED) ? nb : 0;
+    return;
+  }
   HP h;
   h.as_ratio = h4val(h4.as_ratio, hd.as_ratio, i4);
   h.rho_ms = h4val(h4.rho_ms, hd.rho_ms, i4);

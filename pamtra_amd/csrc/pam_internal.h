@@ -102,6 +102,10 @@ struct Hydro4D {
   const double *as_ratio, *rho_ms, *a_ms, *b_ms, *alpha_as, *beta_as, *p_1, *p_2, *p_3, *p_4,
       *d_1, *d_2;
   const int32_t *nbin;
+  // full spectrum given per (column, layer, hydrometeor, bin); fs_nbin 0: none
+  int fs_nbin;
+  const double *fs_rho, *fs_d, *fs_d_bound, *fs_n, *fs_mass, *fs_as_ratio, *fs_canting,
+      *fs_rg_kappa, *fs_rg_beta, *fs_rg_gamma, *fs_rg_zeta;
 };
 
 // device workspace (all device pointers)

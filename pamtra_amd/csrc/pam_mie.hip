@@ -254,15 +254,16 @@ static_assert(SSRG_TG >= 1024, "power table");
 // (calc_shape_factor, rayleigh_gans.f90:848-872); tg[j-1] = (2j)^-gamma
 __device__ double ssrg_shape(double x, double kappa, double beta, double gamma, double zeta,
                              const double *tg) {
+  // tg: (2j)^-gamma for j <= SSRG_TG, or null (per-bin gamma): computed here
   const double c = cos(x);
   const double a = (1.0 + kappa / 3.0) * (1.0 / (2.0 * x + PI) - 1.0 / (2.0 * x - PI)) -
                    kappa * (1.0 / (2.0 * x + 3.0 * PI) - 1.0 / (2.0 * x - 3.0 * PI));
   const double scale = (c * a) * (c * a);
   const int jmax = (int)floor(5.0 * x / PI + 1.0);
   double tp = 2.0 * (x + PI), tm = 2.0 * (x - PI);
-  double summ = zeta * tg[0] * (1.0 / (tp * tp) + 1.0 / (tm * tm));
+  double summ = zeta * (tg ? tg[0] : pow(2.0, -gamma)) * (1.0 / (tp * tp) + 1.0 / (tm * tm));
   for (int j = 2; j <= jmax; j++) {
-    const double g = (j <= SSRG_TG) ? tg[j - 1] : pow(2.0 * j, -gamma);
+    const double g = (tg && j <= SSRG_TG) ? tg[j - 1] : pow(2.0 * j, -gamma);
     tp = 2.0 * (x + PI * j);
     tm = 2.0 * (x - PI * j);
     summ = summ + g * (1.0 / (tp * tp) + 1.0 / (tm * tm));
@@ -277,21 +278,41 @@ __device__ double ssrg_shape(double x, double kappa, double beta, double gamma, 
 // afterwards in bin order, as the reference does); threads take the Gauss
 // nodes for the phase-function sums, bins in order.  Returns false after
 // setting the status on an input the reference refuses.
-__device__ bool ssrg_sums(const pam_hydro_desc &hd, double ar, int nbin, const double *dia,
-                          const double *dd, const double *nd, const double *mass, cd refr,
-                          double wavelength, int nquad, const double *tmu, double *s_u,
-                          double (*s_ps)[MAX_NQUAD], double *sumq, int32_t *st) {
+// Per-bin inputs: the descriptor's values, or the full spectrum's lists
+// (scatProperties.f90:258-287: negative or NaN canting and as_ratio read as 0).
+struct SsrgBins {
+  const double *ar, *cant, *kap, *bet, *gam, *zet;  // per bin, or null
+  double ar0, cant0, kap0, bet0, gam0, zet0;
+  __device__ static double clean(double v) { return (v < 0.0 || isnan(v)) ? 0.0 : v; }
+  __device__ double as_ratio(int i) const { return clean(ar ? ar[i] : ar0); }
+  __device__ double canting(int i) const { return clean(cant ? cant[i] : cant0); }
+  __device__ double kappa(int i) const { return kap ? kap[i] : kap0; }
+  __device__ double beta(int i) const { return bet ? bet[i] : bet0; }
+  __device__ double gamma(int i) const { return gam ? gam[i] : gam0; }
+  __device__ double zeta(int i) const { return zet ? zet[i] : zet0; }
+};
+
+__device__ bool ssrg_sums(const pam_hydro_desc &hd, const SsrgBins &pb, int nbin,
+                          const double *dia, const double *dd, const double *nd,
+                          const double *mass, cd refr, double wavelength, int nquad,
+                          const double *tmu, double *s_u, double (*s_ps)[MAX_NQUAD], double *sumq,
+                          int *s_flag, int32_t *st) {
   const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
-  double cant = hd.canting;
-  if (cant < 0.0 || isnan(cant)) cant = 0.0;  // scatProperties.f90:284-285
-  // assertions of rayleigh_gans.f90:590-616
-  if (hd.liq_ice != -1 || !(cant == 0.0 || cant == 90.0) || !(ar > 0.0)) {
+  // assertions of rayleigh_gans.f90:590-616, over the bins
+  if (t == 0) *s_flag = (hd.liq_ice != -1);
+  __syncthreads();
+  for (int i = t; i < nbin; i += MIE_WG) {
+    const double c = pb.canting(i);
+    if (!(c == 0.0 || c == 90.0) || !(pb.as_ratio(i) > 0.0)) *s_flag = 1;
+  }
+  __syncthreads();
+  if (*s_flag) {
     if (t == 0) set_err(st, E_SSRG_INPUT);
     return false;
   }
-  const double kappa = hd.rg_kappa, beta = hd.rg_beta, gamma = hd.rg_gamma, zeta = hd.rg_zeta;
-  double *bq = s_u, *tg = s_u + 3 * MIE_MAXBIN;
-  for (int j = t; j < SSRG_TG; j += MIE_WG) tg[j] = pow(2.0 * (j + 1), -gamma);
+  double *bq = s_u, *tg = pb.gam ? nullptr : s_u + 3 * MIE_MAXBIN;
+  if (tg)
+    for (int j = t; j < SSRG_TG; j += MIE_WG) tg[j] = pow(2.0 * (j + 1), -pb.gam0);
   // dielectric factor of solid ice, (n + i k)^2 (rayleigh_gans.f90:642-645)
   const cd eps = refr * refr;
   const cd K = cdiv(eps - 1.0, eps + 2.0);
@@ -306,9 +327,10 @@ __device__ bool ssrg_sums(const pam_hydro_desc &hd, double ar, int nbin, const d
       if (lane == 0) bq[ii] = bq[MIE_MAXBIN + ii] = bq[2 * MIE_MAXBIN + ii] = 0.0;
       continue;
     }
-    const double dw = (cant == 0.0) ? dia[ii] * ar : dia[ii];
+    const double dw = (pb.canting(ii) == 0.0) ? dia[ii] * pb.as_ratio(ii) : dia[ii];
     const double vol = mass[ii] / 917.0;
     const double pref = 9.0 * (k2 * k2) * K2 * (vol * vol) / (4.0 * PI);
+    const double kappa = pb.kappa(ii), beta = pb.beta(ii), gamma = pb.gamma(ii), zeta = pb.zeta(ii);
     double q = 0.0, last = 0.0;
     for (int ia = 2 + lane; ia <= SSRG_QANG; ia += 64) {
       const double th = (double)(ia - 1) * PI / (double)(SSRG_QANG - 1);
@@ -336,10 +358,11 @@ __device__ bool ssrg_sums(const pam_hydro_desc &hd, double ar, int nbin, const d
     for (int ii = 0; ii < nbin; ii++) {
       const double n = nd[ii];
       if (!(n > 0.0)) continue;
-      const double dw = (cant == 0.0) ? dia[ii] * ar : dia[ii];
+      const double dw = (pb.canting(ii) == 0.0) ? dia[ii] * pb.as_ratio(ii) : dia[ii];
       const double vol = mass[ii] / 917.0;
-      const double amp = 3.0 * k3 * vol * sqrt(ssrg_shape(k * dw * sh, kappa, beta, gamma, zeta, tg)) /
-                         (4.0 * PI);
+      const double shape =
+          ssrg_shape(k * dw * sh, pb.kappa(ii), pb.beta(ii), pb.gamma(ii), pb.zeta(ii), tg);
+      const double amp = 3.0 * k3 * vol * sqrt(shape) / (4.0 * PI);
       // s22 = -i amp K, s11 = s22 cos(theta)
       const double s22r = amp * K.im, s22i = -amp * K.re;
       const double s11r = s22r * ct, s11i = s22i * ct;
@@ -373,8 +396,9 @@ __device__ bool ssrg_sums(const pam_hydro_desc &hd, double ar, int nbin, const d
 // branch (its registers would otherwise spill in the Mie-only kernel).
 template <bool SSRG>
 __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
-                                         const pam_hydro_desc *hyd, const double *freqs,
-                                         const Work &w, int itf, MieLds &L) {
+                                         const pam_hydro_desc *hyd, const Hydro4D &h4,
+                                         const double *freqs, const Work &w, int itf,
+                                         MieLds &L) {
   double *s_u = L.u;
   double(*s_bin)[MIE_WG] = L.bin;
   double *s_wb = L.wb;
@@ -454,8 +478,21 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
     if (SSRG && hyd[hh].scat_name == PAM_SCAT_SSRG_RT3) {
       __syncthreads();
       double q3[3];
-      if (!ssrg_sums(hyd[hh], w.dsd_ar[(size_t)li_item * d.nh + hh], nbin, dia, dd, nd, rho,
-                     cd{s_hm[hh][0], s_hm[hh][1]}, wavelength, nquad, tmu, s_u, s_ps, q3, st))
+      const pam_hydro_desc &hd = hyd[hh];
+      SsrgBins pb{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                  w.dsd_ar[(size_t)li_item * d.nh + hh], hd.canting, hd.rg_kappa, hd.rg_beta,
+                  hd.rg_gamma, hd.rg_zeta};
+      if (h4.fs_nbin > 0) {  // the full spectrum's per-bin lists
+        const size_t b5 = (oy * d.nh + hh) * (size_t)h4.fs_nbin;
+        pb.ar = h4.fs_as_ratio + b5;
+        pb.cant = h4.fs_canting + b5;
+        pb.kap = h4.fs_rg_kappa + b5;
+        pb.bet = h4.fs_rg_beta + b5;
+        pb.gam = h4.fs_rg_gamma + b5;
+        pb.zet = h4.fs_rg_zeta + b5;
+      }
+      if (!ssrg_sums(hd, pb, nbin, dia, dd, nd, rho, cd{s_hm[hh][0], s_hm[hh][1]}, wavelength,
+                     nquad, tmu, s_u, s_ps, q3, &s_nan, st))
         return;
       if (t == 0) {
         sumqe = q3[0];
@@ -912,12 +949,12 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
 // work queue (separate instantiations).
 template <bool PERS, bool SSRG>
 __global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(2))) k_mie(
-    Dims d, Settings s, const pam_hydro_desc *hyd, const double *freqs, Work w) {
+    Dims d, Settings s, const pam_hydro_desc *hyd, Hydro4D h4, const double *freqs, Work w) {
   __shared__ MieLds L;
   const int n = d.nitems * d.nf;
   int itf = PERS ? next_unit(&w.queue[0], &L.slot) : (int)blockIdx.x;
   while (itf < n) {
-    mie_unit<SSRG>(d, s, hyd, freqs, w, itf, L);
+    mie_unit<SSRG>(d, s, hyd, h4, freqs, w, itf, L);
     if (!PERS) break;
     itf = next_unit(&w.queue[0], &L.slot);
   }
@@ -925,20 +962,23 @@ __global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(2))
 
 void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, const Hydro4D &h4,
                 const Profile &p, const double *freqs, Work &w, hipStream_t st) {
-  (void)h4;
   (void)p;
   size_t n = (size_t)d.nitems * d.nf;
   if (n == 0) return;
   const dim3 grid((unsigned)persistent_grid(n, s.pers_mie, s.ncu));
   if (s.ssrg) {
     if (grid.x < n)
-      hipLaunchKernelGGL((k_mie<true, true>), grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+      hipLaunchKernelGGL((k_mie<true, true>), grid, dim3(MIE_WG), 0, st, d, s, hyd, h4, freqs,
+                         w);
     else
-      hipLaunchKernelGGL((k_mie<false, true>), grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+      hipLaunchKernelGGL((k_mie<false, true>), grid, dim3(MIE_WG), 0, st, d, s, hyd, h4, freqs,
+                         w);
   } else if (grid.x < n) {
-    hipLaunchKernelGGL((k_mie<true, false>), grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+    hipLaunchKernelGGL((k_mie<true, false>), grid, dim3(MIE_WG), 0, st, d, s, hyd, h4, freqs,
+                         w);
   } else {
-    hipLaunchKernelGGL((k_mie<false, false>), grid, dim3(MIE_WG), 0, st, d, s, hyd, freqs, w);
+    hipLaunchKernelGGL((k_mie<false, false>), grid, dim3(MIE_WG), 0, st, d, s, hyd, h4, freqs,
+                         w);
   }
 }
 

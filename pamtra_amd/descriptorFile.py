@@ -107,3 +107,25 @@ class pamDescriptorFile(object):
         self.data = rfn.append_fields(self.data, key, val, dtypes=dtype, usemask=False,
                                       asrecarray=True)
         del self.data4D[key]
+
+    def addFullSpectra(self):
+        """Creates the full-spectrum arrays (x, y, z, hydrometeor, bin) of
+        nbin bins, nbin = the descriptor's (equal) bin count; d_bound_ds has
+        nbin + 1 edges (descriptorFile.py:173-236).  Fill them, then set
+        nmlSet["hydro_fullspec"] = True: the given bins replace the size
+        distributions.  The self-similar Rayleigh-Gans parameters default
+        to kappa 0.19, beta 0.23, gamma 5/3, zeta 1."""
+        assert len(self.data) > 0
+        assert np.min(self.data["nbin"]) == np.max(self.data["nbin"])
+        self.fs_nbin = int(np.max(self.data["nbin"]))
+        par = self.parent
+        s4 = (par.p["ngridx"], par.p["ngridy"], par.p["max_nlyrs"], self.nhydro)
+        par._shape5Dplus = s4 + (self.fs_nbin + 1,)
+        par._shape5D = s4 + (self.fs_nbin,)
+        for key in ("rho_ds", "d_ds", "n_ds", "mass_ds", "area_ds", "as_ratio", "canting",
+                    "fallvelocity"):
+            self.dataFullSpec[key] = np.zeros(par._shape5D)
+        self.dataFullSpec["d_bound_ds"] = np.zeros(par._shape5Dplus)
+        for key, v in (("rg_kappa_ds", 0.19), ("rg_beta_ds", 0.23), ("rg_gamma_ds", 5. / 3.),
+                       ("rg_zeta_ds", 1.)):
+            self.dataFullSpec[key] = np.full(par._shape5D, v)

@@ -44,12 +44,10 @@ def PamtraFortranWrapper(sets, nmlSets, descriptorFile, descriptorFile4D, descri
         if isinstance(v, np.ndarray) and v.ndim >= 2 and "lat" in profile:
             assert v.shape[0] == profile["lat"].shape[0]
             assert v.shape[1] == profile["lat"].shape[1]
-    if nmlSets.get("hydro_fullspec", False) or descriptorFileFS:
-        raise NotImplementedError("hydro_fullspec is not supported")
     nml = dict(nmlSets)
     if not nml.get("passive", True):
         raise NotImplementedError("passive=False leaves nothing for this library to compute")
-    m = _capi.marshal(nml, sets, descriptorFile, descriptorFile4D, profile)
+    m = _capi.marshal(nml, sets, descriptorFile, descriptorFile4D, profile, descriptorFileFS)
     lib = library(m.nummu)
     ctx = get_context(0, m.nummu)
     freqs, fp = _capi.freqs_array(sets["freqs"])
