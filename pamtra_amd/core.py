@@ -711,3 +711,216 @@ class pyPamtra(object):
             f.close()
         if self.set["pyVerbose"] > 0:
             print(fname, "written")
+
+    # ------------------------------------------------------------------
+    # profile handling and session files around the run (core.py:277-338,
+    # 1001-1160, 1337-1358, 1570-1675, 2072-2162, 2555-2594)
+
+    def _set_shapes(self):
+        p, nh = self.p, self.df.nhydro
+        nx, ny, L = p["ngridx"], p["ngridy"], p["max_nlyrs"]
+        self.df.fs_nbin = self.df.dataFullSpec["d_ds"].shape[-1] if "d_ds" in self.df.dataFullSpec \
+            else 0
+        self._shape2D = (nx, ny)
+        self._shape3D = (nx, ny, L)
+        self._shape3Dplus = (nx, ny, L + 1)
+        self._shape3Dout = (nx, ny, p["noutlevels"])
+        self._shape3Dhyd = (nx, ny, nh)
+        self._shape4D = (nx, ny, L, nh)
+        self._shape5Dplus = (nx, ny, L, nh, self.df.fs_nbin + 1)
+        self._shape5D = (nx, ny, L, nh, self.df.fs_nbin)
+
+    def _reshape_all(self, take):
+        """Applies take(array, trailing_ndim) to every per-column array of
+        the profile, the 4D descriptor and the full spectrum."""
+        p = self.p
+        for k in list(p):
+            v = p[k]
+            if isinstance(v, np.ndarray) and v.ndim >= 2:
+                p[k] = take(v, v.ndim - 2)
+        for k in list(self.df.data4D):
+            self.df.data4D[k] = take(self.df.data4D[k], 2)
+        for k in list(self.df.dataFullSpec):
+            self.df.dataFullSpec[k] = take(self.df.dataFullSpec[k], 3)
+
+    def filterProfiles(self, condition):
+        """Keeps the columns where the 2D boolean `condition` holds; the grid
+        becomes (n, 1) (core.py:1001-1078)."""
+        condition = np.asarray(condition)
+        if condition.shape != self._shape2D and condition.shape != (int(np.prod(self._shape2D)),):
+            raise ValueError("shape mismatch, shape of condition must be 2D field!")
+        condition = condition.reshape(self._shape2D)
+        assert np.sum(condition) >= 1
+        n = int(np.sum(condition))
+        self._reshape_all(lambda a, t: a[condition].reshape((n, 1) + a.shape[2:]))
+        self.p["ngridx"], self.p["ngridy"] = n, 1
+        self._set_shapes()
+
+    def tileProfiles(self, rep2D):
+        """Repeats the columns rep2D = (rx, ry) times (core.py:1080-1133)."""
+        assert len(rep2D) == 2
+        self._reshape_all(lambda a, t: np.tile(a, tuple(rep2D) + (1,) * t))
+        self.p["ngridx"] *= rep2D[0]
+        self.p["ngridy"] *= rep2D[1]
+        self._set_shapes()
+
+    def addProfile(self, profile, axis=0):
+        """Appends the columns of another profile dict along `axis`; the
+        layer counts must agree (core.py:1135-1159)."""
+        for k in list(self.p):
+            if np.ndim(self.p[k]) >= 2:
+                self.p[k] = np.concatenate((self.p[k], profile[k]), axis=axis)
+        self.p["ngridx"], self.p["ngridy"] = np.shape(self.p["hgt_lev"])[:2]
+        self._set_shapes()
+
+    def writeNmlFile(self, nmlFile):
+        """Writes nmlSet as a Fortran namelist "&settings ... /"
+        (core.py:277-305)."""
+        with open(nmlFile, "w") as f:
+            f.write("&settings\n\r")
+            for key, v in self.nmlSet.items():
+                if isinstance(v, (bool, np.bool_)):
+                    f.write("%s=.%s.\n\r" % (key, str(bool(v)).lower()))
+                elif isinstance(v, (int, np.integer)):
+                    f.write("%s=%i\n\r" % (key, int(v)))
+                elif isinstance(v, (float, np.floating)):
+                    f.write("%s=%f\n\r" % (key, np.float64(v)))
+                elif isinstance(v, str):
+                    f.write("%s='%s'\n\r" % (key, v))
+                else:
+                    raise ValueError("cannot determine type of nml key " + key)
+            f.write("/\n\r")
+
+    def readNmlFile(self, inputFile):
+        """Reads a Fortran namelist file into nmlSet: numbers (d exponents
+        too), .true./.false., strings; keys not among the defaults are
+        skipped (core.py:307-335)."""
+        import os
+        import re
+        if not os.path.exists(inputFile):
+            raise IOError("file not found: " + inputFile)
+        text = open(inputFile).read()
+        defaults = pyPamtra().nmlSet
+        # a group ends at a "/" that starts a line (values may hold paths)
+        for group in re.findall(r"&\w+(.*?)[\n\r]\s*/", text, flags=re.S):
+            for key, raw in re.findall(r"(\w+)\s*=\s*('[^']*'|\"[^\"]*\"|[^,\s]+)", group):
+                key = key.lower()
+                if key not in defaults:
+                    if self.set["pyVerbose"] > 0:
+                        print("Setting '" + key + "' from '" + inputFile + "' skipped.")
+                    continue
+                raw = raw.strip()
+                if raw.lower() in (".true.", "t", ".t."):
+                    value = True
+                elif raw.lower() in (".false.", "f", ".f."):
+                    value = False
+                elif raw[0] in "'\"":
+                    value = raw[1:-1]
+                else:
+                    try:
+                        value = float(raw.replace("d", "e").replace("D", "e"))
+                    except ValueError:
+                        value = raw
+                self.nmlSet[key] = value
+
+    def _calcMoistRho(self):
+        from . import meteoSI
+        h = dict()
+        h["dz"] = self.p["hgt_lev"][..., 1:] - self.p["hgt_lev"][..., :-1]
+        rh = self.p["relhum"] / 100.
+        h["vapor"] = meteoSI.rh2q(rh, self.p["temp"], self.p["press"])
+        h["sum_hydro_q"] = np.nansum(self.p["hydro_q"], axis=-1)
+        h["rho_moist"] = meteoSI.moist_rho_rh(self.p["press"], self.p["temp"], rh,
+                                              h["sum_hydro_q"])
+        self._helperP = h
+
+    def addIntegratedValues(self):
+        """iwv and the hydrometeor paths hydro_wp [kg/m2] (core.py:1337-1349);
+        needs the layer fields temp, press, relhum [%]."""
+        self._shape3Dhyd = (self.p["ngridx"], self.p["ngridy"], self.df.nhydro)
+        self._calcMoistRho()
+        h = self._helperP
+        self.p["iwv"] = np.nansum(h["vapor"] * h["rho_moist"] * h["dz"], axis=-1)
+        self.p["hydro_wp"] = np.zeros(self._shape3Dhyd)
+        if not np.all(self.p["hydro_q"] == 0):
+            for i in range(self.df.nhydro):
+                self.p["hydro_wp"][..., i] = np.nansum(
+                    self.p["hydro_q"][..., i] * h["rho_moist"] * h["dz"], axis=-1)
+
+    def runParallelPamtra(self, freqs, pp_local_workers="auto", pp_deltaF=1, pp_deltaX=0,
+                          pp_deltaY=0, checkData=True, timeout=None):
+        """The reference splits the run over local worker processes
+        (core.py:1570-1675); here the device already runs every column and
+        channel of one call in parallel, so this is runPamtra with the same
+        results."""
+        return self.runPamtra(freqs, checkData=checkData)
+
+    def averageResultTbs(self, translatorDict):
+        """Averages the TB of channels given as several frequencies, e.g.
+        {120.15: [117.35, 120.15]}; keeps the originals in
+        r["not_averaged_tb"] and set["not_averaged_freqs"] (core.py:2555-2593)."""
+        assert not self.nmlSet["active"]
+        assert self.nmlSet["passive"]
+        self.r["not_averaged_tb"] = deepcopy(self.r["tb"])
+        self.set["not_averaged_freqs"] = deepcopy(self.set["freqs"])
+        self.set["freqs"] = sorted(translatorDict.keys())
+        self.set["nfreqs"] = len(self.set["freqs"])
+        tb0 = self.r["not_averaged_tb"]
+        self.r["tb"] = np.ones(tb0.shape[:4] + (self.set["nfreqs"], tb0.shape[5])) * missingNumber
+        old = np.array(self.set["not_averaged_freqs"])
+        for ff, (freqNew, freqList) in enumerate(sorted(translatorDict.items())):
+            idx = []
+            for fq in freqList:
+                found = np.where(fq == old)[0]
+                assert len(found) == 1
+                idx.append(found[0])
+            self.r["tb"][:, :, :, :, ff, :] = np.mean(tb0[:, :, :, :, idx, :], axis=4)
+
+    def writeResultsToNumpy(self, fname, seperateFiles=False):
+        """Saves results, profile, settings and descriptors: one pickle
+        file, or one .npy file per entry in the directory `fname`
+        (core.py:2072-2114)."""
+        import os
+        import pickle
+        if not seperateFiles:
+            with open(fname, "wb") as f:
+                pickle.dump([self.r, self.p, self.nmlSet, self.set, self.df.data, self.df.data4D,
+                             self.df.dataFullSpec], f)
+            return
+        os.makedirs(fname)
+        for name, dic in (("r", self.r), ("p", self.p), ("nmlSet", self.nmlSet), ("set", self.set),
+                          ("df.data4D", self.df.data4D), ("df.dataFullSpec", self.df.dataFullSpec)):
+            for key, data in dic.items():
+                path = fname + "/" + name + "%" + key + "%" + ".npy"
+                if isinstance(data, np.ma.MaskedArray):
+                    data = data.filled(-9999)
+                if isinstance(data, np.ndarray) and data.dtype == np.float64:
+                    data = data.astype("f4")
+                elif isinstance(data, np.ndarray) and data.dtype == np.int64:
+                    data = data.astype("i4")
+                np.save(path, np.asarray(data) if not isinstance(data, (dict, list)) else
+                        np.array(data, dtype=object), allow_pickle=True)
+
+    def loadResultsFromNumpy(self, fname):
+        """Loads what writeResultsToNumpy wrote (core.py:2116-2161).  Only
+        for files this library wrote: they are pickles."""
+        import os
+        import pickle
+        if os.path.isdir(fname):
+            self.r, self.p, self.set = dict(), dict(), dict()
+            self.df.data4D, self.df.dataFullSpec = dict(), dict()
+            for fn in os.listdir(fname):
+                key, sub, _ = fn.split("%")
+                v = np.load(fname + "/" + fn, allow_pickle=True)
+                v = v.item() if v.dtype == object and v.ndim == 0 else v
+                target = {"r": self.r, "p": self.p, "nmlSet": self.nmlSet, "set": self.set,
+                          "df.data4D": self.df.data4D,
+                          "df.dataFullSpec": self.df.dataFullSpec}[key]
+                target[sub] = v[()] if isinstance(v, np.ndarray) and v.ndim == 0 else v
+            return
+        with open(fname, "rb") as f:
+            (self.r, self.p, self.nmlSet, self.set, self.df.data, self.df.data4D,
+             self.df.dataFullSpec) = pickle.load(f)
+        self.df.nhydro = len(self.df.data)
+        self.p.setdefault("noutlevels", 2)
+        self._set_shapes()

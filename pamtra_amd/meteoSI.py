@@ -14,6 +14,8 @@ Grav = 9.80665          # standard gravity [m/s2]
 Rstar = 8.31432         # universal gas constant of the 1976 standard [J/(mol K)]
 M0 = 0.0289644          # molar mass of dry air [kg/mol]
 Rair = Rstar / M0       # [J/(kg K)]
+R_D = 287.0596736665907  # dry air [J/(kg K)] (constants.f90:29)
+R_V = 461.5249933083879  # water vapour [J/(kg K)] (constants.f90:30)
 R_EARTH = 6356766.0     # effective earth radius of the 1976 standard [m]
 missingNumber = -9999.
 
@@ -72,7 +74,21 @@ def rh2q(rh, T, p):
     return 0.622 * e / (p - 0.378 * e)
 
 
-def moist_rho_rh(p, T, rh):
-    """Density of moist air [kg/m3] from p [Pa], T [K], rh [1]."""
-    q = rh2q(rh, T, p)
-    return p / (Rair * (1 + 0.61 * q) * T)
+def moist_rho_rh(p, T, rh, *qm):
+    """Density of moist air [kg/m3] from p [Pa], T [K], rh [1]; qm: contents
+    [kg/kg] of other species in the air mass (meteoSI.py:51-73)."""
+    if np.any(np.asarray(rh) > 5):
+        raise TypeError("rh must not be in %")
+    return moist_rho_q(p, T, rh2q(rh, T, p), *qm)
+
+
+def moist_rho_q(p, T, q, *qm):
+    """Density of moist air [kg/m3] from p [Pa], T [K], specific humidity q
+    and the contents qm of other species: p / (R_d (1 + (R_v/R_d - 1) q - qm) T),
+    the virtual-temperature form of conversions.f90:5-24."""
+    qs = 0.0
+    if len(qm) > 0:
+        qs = np.ma.array(qm).filled(0)
+        qs[qs < 0] = 0
+        qs = np.sum(qs, axis=0)
+    return p / (R_D * (1 + (R_V / R_D - 1) * q - qs) * T)

@@ -1,0 +1,105 @@
+"""The pyPamtra conveniences around the run (reference core.py:277-338,
+1001-1160, 1337-1358, 2072-2162, 2555-2594): column selection, tiling and
+appending, namelist files, integrated values, channel averaging and the
+session files.  Column operations are checked through the oracle: a column
+keeps its brightness temperatures wherever it moves."""
+
+import numpy as np
+import pytest
+
+import oracle
+import pamtra_amd
+from pamtra_amd import meteoSI, workloads
+from _scenario import FREQUENCIES, build_pamtra, passive_nml
+
+
+def _tb(pam, freqs):
+    out, st, _ = oracle.run(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p, freqs,
+                            nthreads=4)
+    assert st == 0, out["msg"]
+    return out["tb"]
+
+
+def test_filter_tile_and_add_profiles():
+    pam, freqs = workloads.config1(ncol=4, seed=301)
+    freqs = freqs[:2]
+    tb = _tb(pam, freqs)                       # (4, 1, ...)
+    keep = np.array([[True], [False], [True], [False]])
+    pam.filterProfiles(keep)
+    assert pam._shape2D == (2, 1) and pam.p["hydro_q"].shape[:2] == (2, 1)
+    np.testing.assert_array_equal(_tb(pam, freqs), tb[[0, 2]])
+    pam.tileProfiles((1, 3))
+    assert pam._shape2D == (2, 3)
+    np.testing.assert_array_equal(_tb(pam, freqs), np.tile(tb[[0, 2]], (1, 3, 1, 1, 1, 1)))
+    other, _ = workloads.config1(ncol=4, seed=301)
+    other.filterProfiles(np.array([True, True, False, False]))
+    other.tileProfiles((1, 3))
+    pam.addProfile(other.p, axis=0)
+    assert pam._shape2D == (4, 3)
+    np.testing.assert_array_equal(_tb(pam, freqs)[2:], np.tile(tb[[0, 1]], (1, 3, 1, 1, 1, 1)))
+
+
+def test_namelist_file_round_trip(tmp_path):
+    pam = pamtra_amd.pyPamtra()
+    pam.nmlSet.update(active=False, hydro_threshold=2.5, radar_polarisation="NN,HV",
+                      radar_nfft=512, outpol="IQ")
+    f = str(tmp_path / "run.nml")
+    pam.writeNmlFile(f)
+    text = open(f).read()
+    assert text.startswith("&settings") and "active=.false." in text
+    back = pamtra_amd.pyPamtra()
+    back.readNmlFile(f)
+    for k in ("active", "radar_polarisation", "outpol"):
+        assert back.nmlSet[k] == pam.nmlSet[k]
+    assert back.nmlSet["radar_nfft"] == 512
+    assert back.nmlSet["hydro_threshold"] == 2.5  # written with %f, as the reference does
+    with pytest.raises(IOError):
+        back.readNmlFile(str(tmp_path / "missing.nml"))
+
+
+def test_integrated_values():
+    """iwv = sum q_v rho dz and hydro_wp = sum q_h rho dz with the moist air
+    density p / (R_d (1 + (R_v/R_d - 1) q_v - sum q_h) T)."""
+    pam = build_pamtra()
+    pam.p["relhum"] = np.full(pam._shape3D, 60.0)
+    pam.p["temp"] = 0.5 * (pam.p["temp_lev"][..., 1:] + pam.p["temp_lev"][..., :-1])
+    pam.p["press"] = np.sqrt(pam.p["press_lev"][..., 1:] * pam.p["press_lev"][..., :-1])
+    pam.addIntegratedValues()
+    p, T = pam.p["press"], pam.p["temp"]
+    e = 0.6 * meteoSI.e_sat_gg_water(T)
+    qv = 0.622 * e / (p - 0.378 * e)
+    qh = pam.p["hydro_q"].sum(axis=-1)
+    rho = p / (287.0596736665907 * (1 + (461.5249933083879 / 287.0596736665907 - 1) * qv - qh) * T)
+    dz = np.diff(pam.p["hgt_lev"], axis=-1)
+    np.testing.assert_allclose(pam.p["iwv"], np.sum(qv * rho * dz, axis=-1), rtol=1e-12)
+    np.testing.assert_allclose(pam.p["hydro_wp"][0, 0, 0], 1e-3 * rho[0, 0, 1] * dz[0, 0, 1],
+                               rtol=1e-12)
+    assert pam.p["iwv"][0, 0] > 0
+
+
+def test_average_result_tbs():
+    pam = build_pamtra()
+    pam.nmlSet["active"] = False
+    pam.set["freqs"] = [117.35, 120.15, 183.31]
+    tb = np.random.default_rng(5).uniform(200, 280, size=(1, 1, 2, 32, 3, 2))
+    pam.r = {"tb": tb.copy()}
+    pam.averageResultTbs({120.15: [117.35, 120.15], 183.31: [183.31]})
+    assert pam.set["freqs"] == [120.15, 183.31]
+    np.testing.assert_allclose(pam.r["tb"][..., 0, :], tb[..., :2, :].mean(axis=4))
+    np.testing.assert_array_equal(pam.r["tb"][..., 1, :], tb[..., 2, :])
+    np.testing.assert_array_equal(pam.r["not_averaged_tb"], tb)
+
+
+@pytest.mark.parametrize("separate", [False, True])
+def test_session_files_round_trip(tmp_path, separate):
+    pam = build_pamtra()
+    pam.set["freqs"] = FREQUENCIES
+    pam.r = {"tb": np.arange(12.0).reshape(1, 1, 2, 3, 2, 1), "pamtraVersion": "x"}
+    f = str(tmp_path / ("session" if separate else "session.pkl"))
+    pam.writeResultsToNumpy(f, seperateFiles=separate)
+    back = pamtra_amd.pyPamtra()
+    back.df.addHydrometeor(tuple(pam.df.data[0].tolist()))
+    back.loadResultsFromNumpy(f)
+    np.testing.assert_allclose(back.r["tb"], pam.r["tb"])
+    np.testing.assert_allclose(back.p["hydro_q"], pam.p["hydro_q"], rtol=1e-6)
+    assert back.nmlSet["active"] == pam.nmlSet["active"]

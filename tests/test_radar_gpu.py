@@ -91,3 +91,11 @@ def test_results_netcdf_after_runpamtra(tmp_path):
         np.testing.assert_array_equal(nc.variables["height"].data,
                                       pam.r["radar_hgt"].astype(np.float32))
         np.testing.assert_allclose(nc.variables["tb"].data, GOLDEN["tb"], rtol=1e-5)
+
+
+def test_run_parallel_pamtra_is_runpamtra():
+    """runParallelPamtra (reference core.py:1570-1675) gives runPamtra's results."""
+    pam = build_pamtra()
+    pam.runParallelPamtra(FREQUENCIES)
+    np.testing.assert_allclose(pam.r["tb"], GOLDEN["tb"], rtol=1e-5)
+    np.testing.assert_allclose(pam.r["Ze"], GOLDEN["Ze"], rtol=1e-5)
