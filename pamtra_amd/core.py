@@ -924,3 +924,56 @@ class pyPamtra(object):
         self.df.nhydro = len(self.df.data)
         self.p.setdefault("noutlevels", 2)
         self._set_shapes()
+
+    def addCloudShape(self):
+        """cloudBase and cloudTop [m] of the liquid clouds that
+        meteoSI.detect_liq_cloud finds on the first nlyrs levels; with several
+        layers the lowest base and the uppermost top (core.py:1360-1375)."""
+        from .meteoSI import detect_liq_cloud
+        self.p["cloudTop"] = np.ones(self._shape2D) * missingNumber
+        self.p["cloudBase"] = np.ones(self._shape2D) * missingNumber
+        for x in range(self._shape2D[0]):
+            for y in range(self._shape2D[1]):
+                n = int(self.p["nlyrs"][x, y])
+                i_top, i_base, _ = detect_liq_cloud(self.p["hgt_lev"][x, y, :n],
+                                                    self.p["temp_lev"][x, y, :n],
+                                                    self.p["relhum_lev"][x, y, :n])
+                if len(i_top) > 0:
+                    self.p["cloudTop"][x, y] = self.p["hgt_lev"][x, y, i_top[-1] + 1]
+                    self.p["cloudBase"][x, y] = self.p["hgt_lev"][x, y, i_base[0]]
+
+    def createFullProfile(self, timestamp, lat, lon, wind10u, wind10v, obs_height, hgt_lev,
+                          press_lev, temp_lev, relhum_lev, hydro_q, hydro_n, hydro_reff, radar_prop,
+                          sfc_type, sfc_model, sfc_refl, sfc_salinity, sfc_slf, sfc_sif):
+        """A complete profile from arrays of one, two or three grid
+        dimensions, only reshaped: nothing is derived or filled
+        (core.py:1449-1516).  model_i/model_j are the 1-based grid indices."""
+        nd = np.ndim(hgt_lev)
+        if nd == 1:
+            nx, ny = 1, 1
+        elif nd == 2:
+            nx, ny = np.shape(hgt_lev)[0], 1
+        elif nd == 3:
+            nx, ny = np.shape(hgt_lev)[:2]
+        else:
+            raise IOError("Too many dimensions!")
+        p = self.p
+        p["ngridx"], p["ngridy"] = nx, ny
+        p["nlyrs"] = (np.sum(np.asarray(hgt_lev) != missingNumber, axis=-1) - 1).reshape(nx, ny)
+        p["max_nlyrs"] = np.shape(hgt_lev)[-1] - 1
+        p["noutlevels"] = np.shape(obs_height)[-1]
+        self._set_shapes()
+        s2, s3o, s3p, s4 = self._shape2D, self._shape3Dout, self._shape3Dplus, self._shape4D
+        for k, v, shp in (("unixtime", timestamp, s2), ("lat", lat, s2), ("lon", lon, s2),
+                          ("wind10u", wind10u, s2), ("wind10v", wind10v, s2),
+                          ("obs_height", obs_height, s3o), ("hydro_q", hydro_q, s4),
+                          ("hydro_n", hydro_n, s4), ("hydro_reff", hydro_reff, s4),
+                          ("hgt_lev", hgt_lev, s3p), ("temp_lev", temp_lev, s3p),
+                          ("press_lev", press_lev, s3p), ("relhum_lev", relhum_lev, s3p),
+                          ("radar_prop", radar_prop, s2), ("sfc_type", sfc_type, s2),
+                          ("sfc_model", sfc_model, s2), ("sfc_refl", sfc_refl, s2),
+                          ("sfc_salinity", sfc_salinity, s2), ("sfc_slf", sfc_slf, s2),
+                          ("sfc_sif", sfc_sif, s2)):
+            p[k] = np.asarray(v).reshape(shp)
+        gi, gj = np.indices(s2)
+        p["model_i"], p["model_j"] = gi + 1, gj + 1

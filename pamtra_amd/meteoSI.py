@@ -92,3 +92,26 @@ def moist_rho_q(p, T, q, *qm):
         qs[qs < 0] = 0
         qs = np.sum(qs, axis=0)
     return p / (R_D * (1 + (R_V / R_D - 1) * q - qs) * T)
+
+
+def detect_liq_cloud(z, t, rh):
+    """Liquid cloud layers on a level grid (meteoSI.py:253-314): a layer is
+    cloudy when the mean of its two levels' rh exceeds 0.95 and their mean
+    temperature 253.15 K.  Returns (i_top, i_base, i_cloud): levels that only
+    close a cloudy layer from above, levels that only open one, and all
+    levels touching a cloudy layer; empty lists when tops and bases do not
+    pair up."""
+    n = len(z)
+    flag = np.zeros(n, dtype=int)
+    for i in range(n - 1):
+        if (rh[i + 1] + rh[i]) / 2. > 0.95 and (t[i + 1] + t[i]) / 2. > 253.15:
+            flag[i] |= 1
+            flag[i + 1] |= 2
+    i_cloud = np.where(flag != 0)[0]
+    i_top, i_base = [], []
+    if len(i_cloud):
+        i_base = np.where(flag == 1)[0]
+        i_top = np.where(flag == 2)[0]
+        if len(i_top) != len(i_base):
+            return [], [], []
+    return i_top, i_base, i_cloud

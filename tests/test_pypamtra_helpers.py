@@ -103,3 +103,44 @@ def test_session_files_round_trip(tmp_path, separate):
     np.testing.assert_allclose(back.r["tb"], pam.r["tb"])
     np.testing.assert_allclose(back.p["hydro_q"], pam.p["hydro_q"], rtol=1e-6)
     assert back.nmlSet["active"] == pam.nmlSet["active"]
+
+
+def test_cloud_shape():
+    """Levels 1-3 saturated and warm: one cloudy layer pair from level 1 to 3
+    (base at level 1, the top read one level above the last closing level,
+    as the reference reads it); a cold saturated layer is no liquid cloud."""
+    pam = build_pamtra()
+    hl = np.array([0.0, 500.0, 1000.0, 1500.0])
+    t = np.array([285.0, 282.0, 279.0, 276.0])
+    rh = np.array([0.5, 0.99, 0.99, 0.5])
+    i_top, i_base, i_cloud = meteoSI.detect_liq_cloud(hl, t, rh)
+    flags = [(rh[i] + rh[i + 1]) / 2 > 0.95 for i in range(3)]
+    assert flags == [False, True, False]
+    assert list(i_base) == [1] and list(i_top) == [2] and list(i_cloud) == [1, 2]
+    assert all(len(v) == 0 for v in meteoSI.detect_liq_cloud(hl, t - 40.0, rh))
+    pam.p["hgt_lev"][0, 0] = hl
+    pam.p["temp_lev"][0, 0] = t
+    pam.p["relhum_lev"][0, 0] = rh
+    pam.p["nlyrs"][:] = 3
+    pam.addCloudShape()
+    assert pam.p["cloudBase"][0, 0] == 500.0 and pam.p["cloudTop"][0, 0] == 1500.0
+
+
+def test_create_full_profile_only_reshapes():
+    pam = pamtra_amd.pyPamtra()
+    pam.df.addHydrometeor(tuple(build_pamtra().df.data[0].tolist()))
+    nx, L = 3, 4
+    rng = np.random.default_rng(2)
+    f2 = lambda: rng.uniform(size=nx)
+    hl = np.tile(np.arange(L + 1) * 1000.0, (nx, 1))
+    hl[2, -1] = -9999.0
+    pam.createFullProfile(np.arange(nx), f2(), f2(), f2(), f2(), np.full((nx, 2), 833000.0), hl,
+                          rng.uniform(size=(nx, L + 1)), rng.uniform(size=(nx, L + 1)),
+                          rng.uniform(size=(nx, L + 1)), rng.uniform(size=(nx, L, 1)),
+                          rng.uniform(size=(nx, L, 1)), rng.uniform(size=(nx, L, 1)),
+                          f2(), np.zeros(nx, dtype=int), np.zeros(nx, dtype=int),
+                          np.full(nx, "S"), f2(), f2(), f2())
+    assert pam._shape2D == (3, 1) and pam.p["hgt_lev"].shape == (3, 1, 5)
+    np.testing.assert_array_equal(pam.p["nlyrs"][:, 0], [4, 4, 3])
+    np.testing.assert_array_equal(pam.p["model_i"][:, 0], [1, 2, 3])
+    assert pam.p["obs_height"].shape == (3, 1, 2) and pam.p["hydro_q"].shape == (3, 1, 4, 1)
