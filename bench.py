@@ -204,6 +204,41 @@ def large_domain(args, torch):
     return res
 
 
+def aggregates(args):
+    """Cost of the self-similar Rayleigh-Gans producer: the four-category
+    columns (config 3 shapes) with snow and graupel as Mie soft spheres and
+    as SSRGA aggregates, FP64, stage times of each."""
+    from pamtra_amd import _capi, _lib, workloads
+    lib, ctx = _lib.library(), _lib.get_context(0)
+    res = {}
+    for kind in ("mie", "ssrg"):
+        pam, freqs = workloads.config3(ncol=args.aggregates_cols, seed=1003)
+        if kind == "ssrg":
+            pam.df.data["scat_name"][2] = "ssrg-rt3_0.19_0.23_1.67"
+            pam.df.data["as_ratio"][2] = 0.6
+            pam.df.data["scat_name"][3] = "ssrg-rt3"
+            pam.df.data["as_ratio"][3] = 0.8
+        m = _capi.marshal(dict(pam.nmlSet), pam.set, pam.df.data, pam.df.data4D, pam.p)
+        f, fp = _capi.freqs_array(freqs)
+        hr = _capi.alloc_results(m, len(f))
+        names = lib.pam_stage_names().decode().split(",")
+        acc = np.zeros(len(names))
+        for i in range(3):
+            st = lib.pam_run(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),
+                             C.byref(m.profile), fp, len(f), C.byref(hr.struct))
+            if st == _capi.PAM_FATAL:
+                raise RuntimeError("aggregates: " + _capi.results_message(hr))
+            ms = (C.c_double * len(names))()
+            lib.pam_last_stage_times(ctx, ms, len(names))
+            if i:
+                acc += np.array(ms[:])
+            log(f"aggregates {kind} call {i} done")
+        res[kind] = {n: round(v / 2, 3) for n, v in zip(names, acc)}
+    res["workload"] = (f"{args.aggregates_cols} columns of the four-category config-3 shapes, "
+                       "7 channels; snow and graupel as Mie soft spheres or SSRGA aggregates")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,6 +270,10 @@ def main():
     ap.add_argument("--large-cols", type=int, default=2048,
                     help="columns of the large-domain (32-angle) sample (0: skip it)")
     ap.add_argument("--large-steps", type=int, default=2)
+    ap.add_argument("--aggregates-cols", type=int, default=0,
+                    help="also time the SSRGA producer on this many four-category columns")
+    ap.add_argument("--aggregates-only", action="store_true",
+                    help="only the SSRGA producer timing (needs --aggregates-cols)")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the large-domain (32-angle) sample and print its summary")
     args = ap.parse_args()
@@ -249,6 +288,9 @@ def main():
 
     if args.large_only:
         print(json.dumps({"large_domain": large_domain(args, torch)}))
+        return
+    if args.aggregates_only:
+        print(json.dumps({"aggregates": aggregates(args)}))
         return
     rates = [float(x) for x in str(args.serve_rate).split(",") if x.strip()]
     if args.serve_only:
@@ -470,6 +512,8 @@ def main():
 
     if args.large_cols > 0:
         out["large_domain"] = large_domain(args, torch)
+    if args.aggregates_cols > 0:
+        out["aggregates"] = aggregates(args)
 
     if not args.no_cpu:
         import oracle
