@@ -266,7 +266,9 @@ __device__ double ssrg_shape(double x, double kappa, double beta, double gamma, 
     const double g = (tg && j <= SSRG_TG) ? tg[j - 1] : pow(2.0 * j, -gamma);
     tp = 2.0 * (x + PI * j);
     tm = 2.0 * (x - PI * j);
-    summ = summ + g * (1.0 / (tp * tp) + 1.0 / (tm * tm));
+    // 1/tp^2 + 1/tm^2 with one refined reciprocal instead of two divisions
+    const double a2 = tp * tp, b2 = tm * tm;
+    summ = summ + g * ((a2 + b2) * rcp_nr(a2 * b2));
   }
   const double sx = sin(x);
   summ = summ * beta * (sx * sx);
