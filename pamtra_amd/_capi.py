@@ -318,7 +318,16 @@ def scattering_model(name: str):
         if len(vals) == 3:
             vals.append(1.0)
         return "ssrg-rt3", tuple(vals)
-    raise NotImplementedError(f"scat_name {name!r}: 'mie-sphere' and 'ssrg-rt3' are implemented")
+    if name[:5] == "ssrga":
+        why = "the reference's calc_single_ssrga leaves its amplitude matrix unset"
+    elif name[:16] == "ss-rayleigh-gans" or name in ("rayleigh", "rayleigh-gans"):
+        why = "an active-only backscatter model in the reference (passive runs stop on it)"
+    elif name == "tmatrix":
+        why = "the T-matrix producer is not built"
+    else:
+        why = "unknown scattering model"
+    raise NotImplementedError(f"scat_name {name!r}: {why}; 'mie-sphere' and 'ssrg-rt3' "
+                              "are implemented")
 
 
 @dataclass

@@ -299,3 +299,11 @@ def test_aggregates_at_32_angles_vs_oracle():
     assert st == 0 and st2 == 0
     assert g["tb"].shape[3] == 64
     _compare(g, c)
+
+
+def test_other_scattering_names_are_refused_with_the_reason():
+    for name, word in (("ssrga_0.2", "unset"), ("ss-rayleigh-gans", "active-only"),
+                       ("rayleigh", "active-only"), ("tmatrix", "T-matrix"),
+                       ("liudb_8", "unknown")):
+        with pytest.raises(NotImplementedError, match=word):
+            _capi.scattering_model(name)
