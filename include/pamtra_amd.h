@@ -46,7 +46,14 @@ enum pam_dist_name {
   PAM_DIST_EXP_FIELD_T = 3,
   PAM_DIST_EXP_RYAN = 4,
   PAM_DIST_MGAMMA = 5,
-  PAM_DIST_LOGN = 6
+  PAM_DIST_LOGN = 6,
+  PAM_DIST_EXP_COSMO_SNOW = 7,  /* COSMO two-moment snow, n_0(T, q) (make_dist_param.f90:343-374) */
+  PAM_DIST_MONO_COSMO_ICE = 8,  /* COSMO ice, N(T), D <= 200 um (make_dist_param.f90:163-178) */
+  PAM_DIST_MONO_ECHAM_CL = 9,   /* ECHAM6 liquid cloud, land/sea N (make_dist_param.f90:180-200) */
+  PAM_DIST_MONO_ECHAM_ICE = 10, /* ECHAM6 ice volume diameter (make_dist_param.f90:202-209) */
+  PAM_DIST_MGAMMA_MNH = 11,     /* Meso-NH, N_tot = C lambda^x (make_dist_param.f90:486-497) */
+  PAM_DIST_NORM_GAMMA = 12,     /* normalised gamma, D_m and N_0* (make_dist_param.f90:537-600) */
+  PAM_DIST_NORM_MGAMMA = 13     /* normalised modified gamma (make_dist_param.f90:602-624) */
 };
 
 /* descriptor file column "scat_name": "mie-sphere" (src/scatProperties.f90:722-769)

@@ -29,6 +29,8 @@ def build(force: bool = False) -> str:
     """Compiles the oracle (both angle counts) with the committed Makefile."""
     if force or not all(os.path.exists(p) for p in LIB_PATHS.values()):
         subprocess.run(["make", "-s", "-B" if force else "-s", "-C", HERE], check=True)
+    from . import refbind
+    refbind.build(force=force)
     return LIB_PATH
 
 
@@ -50,6 +52,15 @@ def lib(nummu: int = 16) -> C.CDLL:
         L.pam_oracle_eps_ice.argtypes = [C.c_double, C.c_double, dp, dp]
         L.pam_oracle_gasabs.restype = C.c_double
         L.pam_oracle_gasabs.argtypes = [C.c_double] * 4
+        L.pam_oracle_gasabs_parts.argtypes = [C.c_double] * 4 + [dp, dp]
+        L.pam_oracle_fastem_parts.argtypes = [C.c_double] * 5 + [dp]
+        L.pam_oracle_eps_mix.argtypes = [C.c_double] * 3 + [dp, dp]
+        L.pam_oracle_mie_coefs.restype = C.c_int
+        L.pam_oracle_mie_coefs.argtypes = [C.c_double] * 3 + [C.c_int, C.POINTER(C.c_int), dp, dp,
+                                                              dp]
+        L.pam_oracle_mie_spheres.restype = C.c_int
+        L.pam_oracle_mie_spheres.argtypes = [C.c_double, C.c_int, C.c_int, dp, dp, dp, dp,
+                                             C.c_double, C.c_double, dp, dp, C.POINTER(C.c_int)]
         L.pam_oracle_tessem2.argtypes = [C.c_double] * 5 + [dp, dp]
         L.pam_oracle_fastem.argtypes = [C.c_double] * 5 + [dp, dp]
         L.pam_oracle_liu_permittivity.argtypes = [C.c_double] * 3 + [dp, dp]
@@ -74,8 +85,8 @@ def lib(nummu: int = 16) -> C.CDLL:
         L.pam_oracle_rad2tb.argtypes = [C.c_double, C.c_double]
         L.pam_oracle_dsd.restype = C.c_int
         L.pam_oracle_dsd.argtypes = [C.POINTER(_capi.PamHydroDesc), C.POINTER(_capi.PamSettings),
-                                     C.c_double, C.c_double, C.c_double, C.c_double,
-                                     dp, dp, dp, dp, dp]
+                                     C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, dp, dp, dp, dp, dp]
         _libs[nummu] = L
     return _libs[nummu]
 
@@ -126,6 +137,47 @@ def eps_ice(t_k, f_ghz):
 
 def gasabs(f_ghz, t_k, rho_v, p_pa):
     return lib().pam_oracle_gasabs(f_ghz, t_k, rho_v, p_pa)
+
+
+def gasabs_parts(f_ghz, t_k, rho_v, p_pa):
+    """(dry air, water vapour) absorption in Np/km."""
+    a, w = C.c_double(), C.c_double()
+    lib().pam_oracle_gasabs_parts(f_ghz, t_k, rho_v, p_pa, C.byref(a), C.byref(w))
+    return a.value, w.value
+
+
+def fastem_parts(f, zen, temp, sal, wind):
+    """(Re eps, Im eps, Rv, Rh Fresnel, foam cover, Rv, Rh foam, Rv, Rh
+    large-scale, small-scale factor, cos zenith)."""
+    out = np.zeros(11)
+    lib().pam_oracle_fastem_parts(f, zen, temp, sal, wind, _dp(out))
+    return out
+
+
+def eps_mix(eps1, mix_de):
+    re, im = C.c_double(), C.c_double()
+    lib().pam_oracle_eps_mix(eps1.real, eps1.imag, mix_de, C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def mie_coefs(x, m, maxn=10000):
+    """(status, nterms, a[n], b[n], (Qext, Qsca, Qback)); m = n + i k, k >= 0."""
+    a, b, q = np.zeros((maxn, 2)), np.zeros((maxn, 2)), np.zeros(3)
+    nt = C.c_int()
+    st = lib().pam_oracle_mie_coefs(x, m.real, abs(m.imag), maxn, C.byref(nt), _dp(a), _dp(b),
+                                    _dp(q))
+    n = min(nt.value, maxn)
+    return st, nt.value, a[:n, 0] + 1j * a[:n, 1], b[:n, 0] + 1j * b[:n, 1], q
+
+
+def mie_spheres(freq_hz, liq_ice, diameter, del_d, ndens, density, m):
+    """(status, (ext, albedo, back), legen[3][200], nlegen)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (diameter, del_d, ndens, density)]
+    out, legen = np.zeros(3), np.zeros((3, MAXNLEG))
+    nl = C.c_int()
+    st = lib().pam_oracle_mie_spheres(freq_hz, liq_ice, len(arrs[0]), *[_dp(a) for a in arrs],
+                                      m.real, abs(m.imag), _dp(out), _dp(legen), C.byref(nl))
+    return st, out, legen, nl.value
 
 
 def tessem2(f, zen, wind, temp, sal):
@@ -213,13 +265,17 @@ def rad2tb(r, wl):
     return lib().pam_oracle_rad2tb(r, wl)
 
 
-def dsd(desc_row: dict, nmlSets: dict, q_h, n_tot, r_eff, layer_t):
+def dsd(desc_row: dict, nmlSets: dict, q_h, n_tot, r_eff, layer_t, press=8.5e4, sfc_type=0):
+    """run_drop_size_dist of one (layer, hydrometeor): (status, [d_ds,
+    delta_d_ds, n_ds, density2scat, diameter2scat]).  press and sfc_type
+    enter the ECHAM monodisperse forms only."""
     rec = np.rec.fromrecords([tuple(desc_row[k] for k in DESC_NAMES)], names=DESC_NAMES)
     h = _capi.make_hydros(rec)
     s = _capi.make_settings(nmlSets)
-    nb = int(desc_row["nbin"]) if desc_row["dist_name"] != "mono" else 2
+    nb = int(desc_row["nbin"]) if desc_row["dist_name"] not in ("mono", "mono_cosmo_ice") else 2
     arrs = [np.zeros(nb) for _ in range(5)]
-    st = lib().pam_oracle_dsd(h, C.byref(s), q_h, n_tot, r_eff, layer_t, *[_dp(a) for a in arrs])
+    st = lib().pam_oracle_dsd(h, C.byref(s), q_h, n_tot, r_eff, layer_t, press, int(sfc_type),
+                              *[_dp(a) for a in arrs])
     return st, arrs
 
 

@@ -252,6 +252,16 @@ double pam_oracle_gasabs(double f, double tk, double rhowv, double pres) {
   return (absair + abswv) / 1.0e3;
 }
 
+/* The two parts of rosen98_gasabs, Np/km, for the parity tests: dry air
+ * (N2 + O2) and water vapour (lines + continuum). */
+void pam_oracle_gasabs_parts(double f, double tk, double rhowv, double pres, double *air,
+                             double *wv) {
+  double pmb = pres / 100.0;
+  double vapden = rhowv * 1000.0;
+  *air = absn2(tk, pmb, f) + o2abs(tk, pmb, vapden, f);
+  *wv = abh2o(tk, pmb, vapden, f);
+}
+
 /* ------------------------------------------------------------------ */
 /* surface                                                             */
 /* ------------------------------------------------------------------ */
@@ -397,7 +407,7 @@ void pam_oracle_fastem(double f, double zen, double temp, double sal, double win
     double z[6];
     for (int i = 0; i < 6; i++)
       z[i] = FASTEM_LSC[pol][i][0] + f * (FASTEM_LSC[pol][i][1] + f * FASTEM_LSC[pol][i][2]);
-    rl[pol] = z[0] + z[1] * sec + z[2] * sec * sec + z[3] * wind + z[4] * wind * wind +
+    rl[pol] = z[0] + z[1] * sec + z[2] * (sec * sec) + z[3] * wind + z[4] * (wind * wind) +
               z[5] * wind * sec;
   }
   /* small-scale roughness: frequency and wind clamped to the fit's range */
@@ -417,6 +427,44 @@ void pam_oracle_fastem(double f, double zen, double temp, double sal, double win
   emis[1] = 1.0 - (1.0 - cover) * rh - cover * rh_foam;
   refl[0] = 1.0 * (1.0 - emis[0]);
   refl[1] = 1.0 * (1.0 - emis[1]);
+}
+
+/* The FASTEM pieces as compute_fastemx combines them, for the parity tests:
+ * out = (Re eps, Im eps, Rv, Rh Fresnel, foam cover, Rv, Rh foam, Rv, Rh
+ * large-scale correction, small-scale factor exp(-y cos^2 z), cos zenith). */
+void pam_oracle_fastem_parts(double f, double zen, double temp, double sal, double wind,
+                             double *out) {
+  const double cz = cos(zen * (3.141592653589793 / 180.0));
+  cplx eps = fastem_liu_permittivity(temp, sal, f);
+  double rvf, rhf;
+  fastem_fresnel(eps, cz, &rvf, &rhf);
+  double fh = 1.0 + zen * (-1.748e-3 + zen * (-7.336e-5 + zen * 1.044e-7));
+  double ff = 0.40 * exp(-5.0e-2 * f);
+  double sec = 1.0 / cz, rl[2];
+  for (int pol = 0; pol < 2; pol++) {
+    double z[6];
+    for (int i = 0; i < 6; i++)
+      z[i] = FASTEM_LSC[pol][i][0] + f * (FASTEM_LSC[pol][i][1] + f * FASTEM_LSC[pol][i][2]);
+    rl[pol] = z[0] + z[1] * sec + z[2] * (sec * sec) + z[3] * wind + z[4] * (wind * wind) +
+              z[5] * wind * sec;
+  }
+  double fl = f < 1.4 ? 1.4 : (f > 200.0 ? 200.0 : f);
+  double wl = wind < 0.3 ? 0.3 : (wind > 35.0 ? 35.0 : wind);
+  double f2 = fl * fl, w2 = wl * wl;
+  double y = FASTEM_SSC[0] * wl * fl + FASTEM_SSC[1] * wl * f2 + FASTEM_SSC[2] * w2 * fl +
+             FASTEM_SSC[3] * w2 * f2 + FASTEM_SSC[4] * w2 / fl + FASTEM_SSC[5] * w2 / f2 +
+             FASTEM_SSC[6] * wl + FASTEM_SSC[7] * w2;
+  out[0] = creal(eps);
+  out[1] = cimag(eps);
+  out[2] = rvf;
+  out[3] = rhf;
+  out[4] = wind < 0.0 ? 0.0 : 1.95e-5 * pow(wind, 2.55);
+  out[5] = (1.0 - 0.93) * ff;
+  out[6] = (1.0 - fh * 0.93) * ff;
+  out[7] = rl[0];
+  out[8] = rl[1];
+  out[9] = exp(-y * (cz * cz));
+  out[10] = cz;
 }
 
 /* the Liu permittivity alone (for the tests) */
@@ -612,6 +660,35 @@ void pam_oracle_mie_angle(double x, double mre, double mim, double mu, double *p
   free(buf);
 }
 
+/* eps_mix of air and eps1 (the mixing calc_mie_spheres applies), for the tests */
+void pam_oracle_eps_mix(double re1, double im1, double mix_de, double *re, double *im) {
+  cplx e = eps_mix_air(re1 + im1 * I, mix_de);
+  *re = creal(e);
+  *im = cimag(e);
+}
+
+/* miecalc + miecross for one sphere: a_n, b_n (n = 1..nterms) as (re, im)
+ * pairs, q = (Q_ext, Q_sca, Q_back).  For the parity tests. */
+int pam_oracle_mie_coefs(double x, double mre, double mim, int maxn, int *nterms, double *a,
+                         double *b, double *q) {
+  cplx *buf = (cplx *)malloc(sizeof(cplx) * 3 * (MIE_MAXTERMS + 20));
+  cplx *ca = buf, *cb = buf + MIE_MAXTERMS + 20, *d = buf + 2 * (MIE_MAXTERMS + 20);
+  int nt = 0;
+  int st = miecalc(&nt, x, mre - mim * I, ca, cb, d);
+  *nterms = nt;
+  if (st == PAM_SUCCESS) {
+    miecross(nt, x, ca, cb, &q[0], &q[1], &q[2]);
+    for (int n = 1; n <= nt && n <= maxn; n++) {
+      a[2 * (n - 1)] = creal(ca[n]);
+      a[2 * (n - 1) + 1] = cimag(ca[n]);
+      b[2 * (n - 1)] = creal(cb[n]);
+      b[2 * (n - 1) + 1] = cimag(cb[n]);
+    }
+  }
+  free(buf);
+  return st;
+}
+
 typedef struct {
   double extinction, albedo, back_scatt;
   int nlegen;
@@ -744,6 +821,25 @@ static int calc_mie_spheres(double freq_hz, int liq_ice, int nbins, const double
     return PAM_FATAL;
   }
   return PAM_SUCCESS;
+}
+
+/* calc_mie_spheres for one bin set with lhyd_absorption on, for the tests:
+ * out = (extinction, albedo, back_scatt), legen[3][MAXNLEG]. */
+int pam_oracle_mie_spheres(double freq_hz, int liq_ice, int nbins, const double *diameter,
+                           const double *del_d, const double *ndens, const double *density,
+                           double refre, double refim, double *out, double *legen, int *nl) {
+  cplx *buf = (cplx *)malloc(sizeof(cplx) * 3 * (MIE_MAXTERMS + 20));
+  mie_result r;
+  char msg[128];
+  int st = calc_mie_spheres(freq_hz, liq_ice, nbins, diameter, del_d, ndens, density, refre, refim,
+                            1, &r, buf, buf + MIE_MAXTERMS + 20, buf + 2 * (MIE_MAXTERMS + 20), msg);
+  free(buf);
+  out[0] = r.extinction;
+  out[1] = r.albedo;
+  out[2] = r.back_scatt;
+  memcpy(legen, r.legen, sizeof(r.legen));
+  *nl = r.nlegen;
+  return st;
 }
 
 /* ------------------------------------------------------------------ */
@@ -950,13 +1046,26 @@ static int make_mass_size(hydro_params *h, char *msg) {
 }
 
 typedef struct {
-  double n_0, lambda, mu, gam, n_t, sig, d_ln, d_mono;
+  double n_0, lambda, mu, gam, n_t, sig, d_ln, d_mono, d_m, n_0_star, b;
 } dist_params;
 
-/* make_dist_params (src/make_dist_param.f90:2-631) for the distributions
- * this library supports: mono, exp, exp_field_t, exp_ryan, mgamma, logn. */
+/* the three monodisperse families of make_dist_param.f90:133-224; they take
+ * the three-edge grid of make_dist.f90:88-99 */
+static int is_mono(int dist) {
+  return dist == PAM_DIST_MONO || dist == PAM_DIST_MONO_COSMO_ICE ||
+         dist == PAM_DIST_MONO_ECHAM_CL || dist == PAM_DIST_MONO_ECHAM_ICE;
+}
+
+/* rho_air (rho_air.f90): dry-air density, r = 287.04 J/(kg K) */
+static double rho_air(double t, double p) { return p / (t * 28704.e-2); }
+
+/* make_dist_params (src/make_dist_param.f90:2-631).  layer_p and sfc_type
+ * (land > 0.5) enter the ECHAM monodisperse forms only.  'const' and
+ * 'const_cosmo_ice' are refused before this point: make_dist never sets their
+ * f_ds (make_dist.f90:164-215), so the reference's result is undefined. */
 static int make_dist_params(const hydro_params *h, double q_h, double n_tot, double r_eff,
-                            double layer_t, dist_params *o, char *msg) {
+                            double layer_t, double layer_p, int sfc_type, dist_params *o,
+                            char *msg) {
   o->n_0 = -99.;
   o->lambda = -99.;
   o->gam = NAN;
@@ -965,6 +1074,9 @@ static int make_dist_params(const hydro_params *h, double q_h, double n_tot, dou
   o->sig = -99.;
   o->d_ln = NAN;
   o->d_mono = -99.;
+  o->d_m = -99.;
+  o->n_0_star = -99.;
+  o->b = h->b_ms;
   int mi = h->moment_in;
   if ((mi == 3 && q_h <= 0.) || (mi == 1 && n_tot <= 0.) || (mi == 2 && r_eff <= 0.) ||
       (mi == 13 && (q_h <= 0. || n_tot <= 0.)) || (mi == 23 && (q_h <= 0. || r_eff <= 0.)) ||
@@ -974,21 +1086,62 @@ static int make_dist_params(const hydro_params *h, double q_h, double n_tot, dou
   }
   double a = h->a_ms, b = h->b_ms, p1 = h->p_1, p2 = h->p_2, p3 = h->p_3, p4 = h->p_4;
   switch (h->dist_name) {
+    case PAM_DIST_MONO_COSMO_ICE:
+    case PAM_DIST_MONO_ECHAM_CL:
+    case PAM_DIST_MONO_ECHAM_ICE:
     case PAM_DIST_MONO: {
       o->lambda = 0.0;
       o->mu = 0.0;
       o->gam = 0.0;
-      if (h->d_1 != -99. && p1 == -99.) {
+      if (h->dist_name == PAM_DIST_MONO_COSMO_ICE) {
+        if (mi == 3 || mi == 13) {
+          /* N_tot(T) of COSMO's ice (make_dist_param.f90:163-178) */
+          double w1 = 1.e2 * exp(0.2 * (273.15 - layer_t));
+          o->n_0 = w1 / DELTA_D_MONO;
+          o->d_mono = pow(q_h / (w1 * a), 1.0 / b);
+          if (o->d_mono > 2.e-4) {
+            o->d_mono = 2.e-4;
+            o->n_0 = q_h / (a * pow(o->d_mono, b)) / DELTA_D_MONO;
+          }
+        }
+      } else if (h->dist_name == PAM_DIST_MONO_ECHAM_CL) {
+        /* make_dist_param.f90:180-200; kappa is a default-real literal */
+        double kappa;
+        if (sfc_type > 0.5) {
+          o->n_0 = 220.e6;
+          kappa = SP(1.143);
+        } else {
+          o->n_0 = 80.e6;
+          kappa = SP(1.077);
+        }
+        if (layer_p < 8.e4) {
+          double r = 8.e4 / fmax(1.e4, layer_p);
+          o->n_0 = (50.0 + (o->n_0 / 1.e6 - 50.0) * exp(1 - r * r)) * 1.e6;
+        }
+        double rho_dry = rho_air(layer_t, layer_p);
+        o->d_mono = kappa * 2.0 *
+                    pow(q_h * rho_dry * 3.0 / (4.0 * PI * o->n_0 * RHO_WATER), 1.0 / 3.0);
+        if (o->d_mono > 48.e-6) {
+          o->d_mono = 48.e-6;
+          o->n_0 = 6. * q_h * rho_dry / (PI * (o->d_mono * o->d_mono * o->d_mono));
+        }
+      } else if (h->dist_name == PAM_DIST_MONO_ECHAM_ICE) {
+        /* mean ice-crystal volume diameter (make_dist_param.f90:202-209) */
+        double rho_dry = rho_air(layer_t, layer_p);
+        double w = 83.8 * pow(1.e3 * q_h * rho_dry, 0.216);
+        o->d_mono = 2.e-6 * pow(sqrt(2809.0 * pow(w, 3.0) + 5113188.0) - 2261.0, 1.0 / 3.0);
+        o->n_0 = q_h * rho_dry / (DELTA_D_MONO * h->rho_ms * (PI / 6.0) * pow(o->d_mono, 3.0));
+      } else if (h->d_1 != -99. && p1 == -99.) {
         o->d_mono = h->d_1;
         if (mi == 3) o->n_0 = q_h / (DELTA_D_MONO * a * pow(h->d_1, b));
         if (mi == 1) o->n_0 = n_tot / DELTA_D_MONO;
       }
-      if (p1 != -99. && h->d_1 == -99.) {
+      if (h->dist_name == PAM_DIST_MONO && p1 != -99. && h->d_1 == -99.) {
         o->n_0 = p1 / DELTA_D_MONO;
         if (mi == 3) o->d_mono = pow(q_h / (p1 * a), 1.0 / b);
         if (mi == 2) o->d_mono = r_eff * 2.0;
       }
-      if (mi == 23) {
+      if (h->dist_name == PAM_DIST_MONO && mi == 23) {
         o->d_mono = r_eff * 2.0;
         o->n_0 = q_h / (DELTA_D_MONO * a * pow(o->d_mono, b));
       }
@@ -1000,6 +1153,7 @@ static int make_dist_params(const hydro_params *h, double q_h, double n_tot, dou
     }
     case PAM_DIST_EXP:
     case PAM_DIST_EXP_FIELD_T:
+    case PAM_DIST_EXP_COSMO_SNOW:
     case PAM_DIST_EXP_RYAN: {
       o->gam = 1.0;
       o->mu = 0.0;
@@ -1054,6 +1208,38 @@ static int make_dist_params(const hydro_params *h, double q_h, double n_tot, dou
         if (mi == 3) o->lambda = pow(a * o->n_0 * g / q_h, 1.0 / (b + 1.0));
         if (mi == 2) o->lambda = 3.0 / r_eff;
       }
+      if (h->dist_name == PAM_DIST_EXP_COSMO_SNOW && mi == 3) {
+        /* COSMO two-moment snow (make_dist_param.f90:343-374): the moment
+         * ratio fit of Field et al. 2005 at n = 3 gives n_0(T, q); the fit's
+         * coefficients are default-real literals */
+        static const float mma[10] = {5.065339f, -0.062659f, -3.032362f, 0.029469f, -0.000285f,
+                                      0.312550f, 0.000204f,  0.003199f,  0.000000f, -0.015952f};
+        static const float mmb[10] = {0.476221f, -0.015896f, 0.165977f, 0.007468f, -0.000141f,
+                                      0.060366f, 0.000079f,  0.000594f, 0.000000f, -0.003577f};
+        double ztc = layer_t - SP(273.15);
+        ztc = fmax(fmin(ztc, 0.0), -40.0);
+        const double nn = 3.0, z2 = ztc * ztc;
+        double hlp = (double)mma[0] + (double)mma[1] * ztc + (double)mma[2] * nn +
+                     (double)mma[3] * ztc * nn + (double)mma[4] * z2 + (double)mma[5] * 9.0 +
+                     (double)mma[6] * z2 * nn + (double)mma[7] * ztc * 9.0 +
+                     (double)mma[8] * (z2 * ztc) + (double)mma[9] * 27.0;
+        double alf = pow(10.0, hlp);
+        double bet = (double)mmb[0] + (double)mmb[1] * ztc + (double)mmb[2] * nn +
+                     (double)mmb[3] * ztc * nn + (double)mmb[4] * z2 + (double)mmb[5] * 9.0 +
+                     (double)mmb[6] * z2 * nn + (double)mmb[7] * ztc * 9.0 +
+                     (double)mmb[8] * (z2 * ztc) + (double)mmb[9] * 27.0;
+        double m2s = q_h / a;
+        double m3s = alf * exp(bet * log(m2s));
+        hlp = 7.628e6 * exp(-0.107 * ztc);
+        double m2s2 = m2s * m2s;
+        double n0 = 13.5 * (m2s2 * m2s2) / (m3s * m3s * m3s);
+        n0 = fmax(n0, 0.5 * hlp);
+        n0 = fmin(n0, 1e2 * hlp);
+        n0 = fmin(n0, 1e9);
+        n0 = fmax(n0, 1e6);
+        o->n_0 = n0;
+        o->lambda = pow(a * o->n_0 * g / q_h, 1.0 / (b + 1.0));
+      }
       if (o->n_0 <= 0.0 || o->lambda <= 0.0) {
         snprintf(msg, 120, "make_dist_params: exponential case incomplete");
         return PAM_FATAL;
@@ -1103,6 +1289,7 @@ static int make_dist_params(const hydro_params *h, double q_h, double n_tot, dou
       }
       return PAM_SUCCESS;
     }
+    case PAM_DIST_MGAMMA_MNH:
     case PAM_DIST_MGAMMA: {
       if (p3 == -99. || p4 == -99.) {
         snprintf(msg, 120, "make_dist_params: mgamma needs p_3 and p_4");
@@ -1137,6 +1324,16 @@ static int make_dist_params(const hydro_params *h, double q_h, double n_tot, dou
           o->n_0 = n_tot * gam * pow(o->lambda, w1b) / tgamma(w1b);
         }
       }
+      if (h->dist_name == PAM_DIST_MGAMMA_MNH) {
+        /* Meso-NH: N_tot = C lambda^x, C = p_1, x = p_2 (make_dist_param.f90:486-497) */
+        if (p1 < 0. || p2 == -99.) {
+          snprintf(msg, 120, "make_dist_params: mgamma_MNH needs p_1 >= 0 and p_2");
+          return PAM_FATAL;
+        }
+        double w2 = tgamma((mu + b + 1.0) / gam), w3 = tgamma((mu + 1.0) / gam);
+        o->lambda = pow(a * p1 / q_h * w2 / w3, gam / (b - p2));
+        o->n_0 = gam * p1 / w3 * pow(o->lambda, (mu + 1.0 + p2) / gam);
+      }
       if (p1 == -99. && p2 == -99.) {
         if (mi == 13) {
           double w2 = tgamma((mu + b + 1.0) / gam), w3 = tgamma((mu + 1.0) / gam);
@@ -1162,6 +1359,51 @@ static int make_dist_params(const hydro_params *h, double q_h, double n_tot, dou
       }
       return PAM_SUCCESS;
     }
+    case PAM_DIST_NORM_GAMMA: {
+      /* normalised gamma (make_dist_param.f90:537-600): D_m = M_(b+1)/M_b,
+       * N_0* of Testud et al. 2001 with the mass-size exponent b */
+      if (mi != 0 && mi != 23) {
+        snprintf(msg, 120, "make_dist_params: norm_gamma needs moment_in 0 or 23");
+        return PAM_FATAL;
+      }
+      if (mi == 0) {
+        if (p1 == -99. || p2 == -99. || !((p3 != -99.) ^ (p4 != -99.))) {
+          snprintf(msg, 120, "make_dist_params: norm_gamma needs p_1, p_2 and p_3 xor p_4");
+          return PAM_FATAL;
+        }
+        o->d_m = p1;
+        o->n_0_star = p2;
+        if (p3 != -99.) o->mu = p3;
+        if (p4 != -99.) o->mu = p4 - (b + 1);
+      } else {
+        o->d_m = r_eff * 2;
+        double w1 = tgamma(b + 1);
+        double w2 = w1 / pow(b + 1, b + 1);
+        o->n_0_star = q_h / (a * pow(o->d_m, b + 1) * w2);
+        if (p3 != -99.) {
+          o->mu = p3;
+        } else {
+          if (h->liq_ice != 1) {
+            snprintf(msg, 120, "make_dist_params: norm_gamma estimates mu from D_m for rain only");
+            return PAM_FATAL;
+          }
+          /* Mroz et al. 2023 eq. 21; the exponent is a default-real literal */
+          o->mu = 10. * pow(o->d_m * 1000., SP(-0.8)) - 4.;
+        }
+      }
+      return PAM_SUCCESS;
+    }
+    case PAM_DIST_NORM_MGAMMA:
+      /* normalised modified gamma (make_dist_param.f90:602-624) */
+      if (p1 == -99. || p2 == -99. || p3 == -99. || p4 == -99. || mi != 0) {
+        snprintf(msg, 120, "make_dist_params: norm_mgamma needs p_1..p_4 and moment_in 0");
+        return PAM_FATAL;
+      }
+      o->d_m = p1;
+      o->n_0_star = p2;
+      o->mu = p3;
+      o->gam = p4;
+      return PAM_SUCCESS;
     default:
       snprintf(msg, 120, "make_dist_params: distribution not implemented");
       return PAM_FATAL;
@@ -1169,6 +1411,23 @@ static int make_dist_params(const hydro_params *h, double q_h, double n_tot, dou
 }
 
 static double dist_value(int dist, const dist_params *p, double d) {
+  if (dist == PAM_DIST_NORM_GAMMA) {
+    /* make_dist.f90:171-178 */
+    const double b = p->b, mu = p->mu;
+    double x = d / p->d_m;
+    double t1 = tgamma(b + 1) / pow(b + 1, b + 1) * pow(b + mu + 1, b + mu + 1) / tgamma(b + mu + 1);
+    double t2 = exp(-(b + mu + 1) * x);
+    return p->n_0_star * t1 * pow(x, mu) * t2;
+  }
+  if (dist == PAM_DIST_NORM_MGAMMA) {
+    /* make_dist.f90:186-196 */
+    const double b = p->b, mu = p->mu, gam = p->gam;
+    double g1 = tgamma((mu + b + 1.) / gam), g2 = tgamma((mu + b + 2.) / gam);
+    double t1 = gam * tgamma(b + 1) / pow(b + 1, b + 1) * pow(g2, b + 1 + mu) / pow(g1, b + 2 + mu);
+    double x = d / p->d_m;
+    double t2 = exp(-pow(x * g2 / g1, gam));
+    return p->n_0_star * t1 * pow(x, mu) * t2;
+  }
   if (dist == PAM_DIST_LOGN)
     return p->n_t / (d * p->sig * sqrt(2.0 * PI)) *
            exp(-(pow(log(d) - p->d_ln, 2.0)) / (2.0 * (p->sig * p->sig)));
@@ -1183,7 +1442,7 @@ static int make_dist(const hydro_params *h, const dist_params *p, double q_h, in
   const double thres_n = 0.1;
   const double min_d = SP(1.0e-7), max_d = SP(2.0e-2);
   int skip = 0;
-  if (h->dist_name == PAM_DIST_MONO) {
+  if (is_mono(h->dist_name)) {
     d_bound[0] = p->d_mono - DELTA_D_MONO * 0.5;
     d_bound[1] = p->d_mono;
     d_bound[2] = p->d_mono + DELTA_D_MONO * 0.5;
@@ -1287,11 +1546,12 @@ static int make_dist(const hydro_params *h, const dist_params *p, double q_h, in
  * in `bins` hold nbin entries each.  Frequency-dependent densities
  * (as_ratio -20 / -30) are refused. */
 static int run_dsd(hydro_params *h, double q_h, double n_tot, double r_eff, double layer_t,
-                   const pam_settings *s, dsd_bins *bins, char *msg) {
+                   double layer_p, int sfc_type, const pam_settings *s, dsd_bins *bins,
+                   char *msg) {
   int st = make_mass_size(h, msg);
   if (st == PAM_FATAL) return st;
   dist_params dp;
-  st = make_dist_params(h, q_h, n_tot, r_eff, layer_t, &dp, msg);
+  st = make_dist_params(h, q_h, n_tot, r_eff, layer_t, layer_p, sfc_type, &dp, msg);
   if (st == PAM_FATAL) return st;
   int nbin = h->nbin;
   double *d_bound = (double *)malloc(sizeof(double) * 2 * (nbin + 1));
@@ -1882,7 +2142,8 @@ static int layer_hydrometeors(const pam_settings *s, const pam_hydro_desc *hyd,
     hp.liq_ice = d->liq_ice;
     hp.moment_in = d->moment_in;
     hp.dist_name = d->dist_name;
-    if (hp.dist_name == PAM_DIST_MONO) hp.nbin = 2;
+    /* hydrometeor_extinction.f90:200-202 */
+    if (hp.dist_name == PAM_DIST_MONO || hp.dist_name == PAM_DIST_MONO_COSMO_ICE) hp.nbin = 2;
     double T = c->temp[z], P = c->press[z], qv = c->q_hum[z];
     /* full spectrum given (hydrometeor_extinction.f90:120-150): the bins
      * replace the size distribution; present when they hold particles or
@@ -1951,7 +2212,8 @@ static int layer_hydrometeors(const pam_settings *s, const pam_hydro_desc *hyd,
     int nb = hp.nbin;
     double *bb = (double *)malloc(sizeof(double) * 5 * nb);
     dsd_bins bins = {nb, bb, bb + nb, bb + 2 * nb, bb + 3 * nb, bb + 4 * nb};
-    int st = run_dsd(&hp, q_h, n_tot, r_eff, T, s, &bins, msg);
+    const int stype = p->sfc_type ? p->sfc_type[(size_t)ix * p->ngridy + iy] : -9999;
+    int st = run_dsd(&hp, q_h, n_tot, r_eff, T, P, stype, s, &bins, msg);
     if (st == PAM_FATAL) {
       free(bb);
       return st;
@@ -2565,13 +2827,13 @@ double pam_oracle_rad2tb(double rad, double wl) { return rad2tb(rad, wl); }
 
 /* DSD of one (layer, hydrometeor): returns bins; for tests of the adaptive grid */
 int pam_oracle_dsd(const pam_hydro_desc *d, const pam_settings *s, double q_h, double n_tot,
-                   double r_eff, double layer_t, double *d_ds, double *delta, double *n_ds,
-                   double *rho, double *dscat) {
+                   double r_eff, double layer_t, double layer_p, int sfc_type, double *d_ds,
+                   double *delta, double *n_ds, double *rho, double *dscat) {
   hydro_params hp = {d->as_ratio, d->rho_ms, d->a_ms, d->b_ms, d->alpha_as, d->beta_as,
                      d->p_1,      d->p_2,    d->p_3,  d->p_4,  d->d_1,      d->d_2,
                      d->liq_ice,  d->moment_in, d->nbin, d->dist_name};
-  if (hp.dist_name == PAM_DIST_MONO) hp.nbin = 2;
+  if (hp.dist_name == PAM_DIST_MONO || hp.dist_name == PAM_DIST_MONO_COSMO_ICE) hp.nbin = 2;
   dsd_bins b = {hp.nbin, d_ds, delta, n_ds, rho, dscat};
   char msg[128];
-  return run_dsd(&hp, q_h, n_tot, r_eff, layer_t, s, &b, msg);
+  return run_dsd(&hp, q_h, n_tot, r_eff, layer_t, layer_p, sfc_type, s, &b, msg);
 }

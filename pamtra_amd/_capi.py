@@ -31,7 +31,17 @@ DIST_NAMES = {
     "exp_ryan": 4,
     "mgamma": 5,
     "logn": 6,
+    "exp_cosmo_snow": 7,
+    "mono_cosmo_ice": 8,
+    "mono_echam_cl": 9,
+    "mono_echam_ice": 10,
+    "mgamma_MNH": 11,
+    "norm_gamma": 12,
+    "norm_mgamma": 13,
 }
+# distributions the reference names but whose bins it never fills:
+# make_dist.f90:164-215 sets f_ds for neither, so its result is undefined
+UNDEFINED_DISTS = ("const", "const_cosmo_ice")
 SCAT_NAMES = {"disabled": 0, "mie-sphere": 1, "ssrg-rt3": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
@@ -266,9 +276,18 @@ def make_hydros(descriptorFile) -> C.Array:
         def num(key, default=-99.0):
             return float(row[key]) if key in names else default
 
-        dist = str(row["dist_name"]).strip()
+        dist = str(row["dist_name"]).strip().strip("'\"")
+        if dist in UNDEFINED_DISTS:
+            raise NotImplementedError(
+                f"dist_name {dist!r}: the reference's make_dist leaves its bins unset "
+                "(make_dist.f90:164-215), so it has no defined result to reproduce")
         if dist not in DIST_NAMES:
             raise NotImplementedError(f"dist_name {dist!r} is not implemented")
+        if dist in ("mono_echam_cl", "mono_echam_ice") and "nbin" in names and int(row["nbin"]) != 2:
+            # hydrometeor_extinction.f90:200-202 forces nbin = 2 only for mono and
+            # mono_cosmo_ice; make_dist.f90:89-97 fills three edges either way
+            raise ValueError(f"dist_name {dist!r} needs nbin = 2 (the reference reads three "
+                             "bin edges only)")
         scat = str(row["scat_name"]).strip()
         scat, rg = scattering_model(scat)
         h.as_ratio = num("as_ratio")

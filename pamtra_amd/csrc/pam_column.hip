@@ -458,7 +458,7 @@ __device__ void fastem(double f, double zen, double temp, double sal, double win
     double z[6];
     for (int i = 0; i < 6; i++)
       z[i] = c_fe_lsc[pol][i][0] + f * (c_fe_lsc[pol][i][1] + f * c_fe_lsc[pol][i][2]);
-    rl[pol] = z[0] + z[1] * sec + z[2] * sec * sec + z[3] * wind + z[4] * wind * wind +
+    rl[pol] = z[0] + z[1] * sec + z[2] * (sec * sec) + z[3] * wind + z[4] * (wind * wind) +
               z[5] * wind * sec;
   }
   const double fl = f < 1.4 ? 1.4 : (f > 200.0 ? 200.0 : f);
