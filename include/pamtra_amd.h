@@ -34,6 +34,9 @@ extern "C" {
 #define PAM_NSTOKES 2
 #define PAM_NANGLES (2 * PAM_NUMMU)
 #define PAM_MAXNLEG 200
+/* size bins per hydrometeor accepted by the device (the reference allocates
+ * any nbin; its Mie path is bounded by the memory of one column) */
+#define PAM_MAX_NBIN 4000
 
 /* Error status codes (src/report_module.f90:9-14). */
 enum pam_status { PAM_SUCCESS = 0, PAM_WARNING = 1, PAM_FATAL = 2, PAM_INFO = 3 };

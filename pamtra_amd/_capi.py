@@ -182,11 +182,26 @@ def _ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
+def _check_tensor(a, dtype_name, shape):
+    """A device tensor handed to pam_run_device must already be what the
+    kernels read: the exact dtype, contiguous, on the GPU and of the expected
+    element count.  Anything else would be read as raw memory of that type."""
+    if str(a.dtype) != "torch." + dtype_name:
+        raise ValueError(f"tensor of dtype {a.dtype}: {dtype_name} is required")
+    if not a.is_contiguous():
+        raise ValueError("tensor is not contiguous")
+    if shape is not None and a.numel() != int(np.prod(shape)):
+        raise ValueError(f"tensor of {a.numel()} elements: shape {tuple(shape)} is expected")
+    if a.device.type != "cuda":
+        raise ValueError(f"tensor on {a.device}: a device tensor is required")
+    return a
+
+
 def _as_f64(a, shape=None):
     if a is None:
         return None
     if _is_torch(a):
-        return a
+        return _check_tensor(a, "float64", shape)
     arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
     if shape is not None:
         arr = arr.reshape(shape)
@@ -197,7 +212,7 @@ def _as_i32(a, shape=None):
     if a is None:
         return None
     if _is_torch(a):
-        return a
+        return _check_tensor(a, "int32", shape)
     arr = np.ascontiguousarray(np.asarray(a).astype(np.int32))
     if shape is not None:
         arr = arr.reshape(shape)
@@ -339,8 +354,13 @@ def scattering_model(name: str):
         return "ssrg-rt3", tuple(vals)
     if name[:5] == "ssrga":
         why = "the reference's calc_single_ssrga leaves its amplitude matrix unset"
-    elif name[:16] == "ss-rayleigh-gans" or name in ("rayleigh", "rayleigh-gans"):
-        why = "an active-only backscatter model in the reference (passive runs stop on it)"
+    elif name[:16] == "ss-rayleigh-gans":
+        why = "the reference's passive routine asserts it is unimplemented (rayleigh_gans.f90:1019-1020)"
+    elif name in ("rayleigh", "rayleigh-gans"):
+        why = ("a backscatter-only model with zero extinction and albedo in the reference "
+               "(scatProperties.f90:625-720); not built")
+    elif name[:5] in ("liudb", "hongd"):
+        why = "a DDA-database model that needs the reference's external data files; not built"
     elif name == "tmatrix":
         why = "the T-matrix producer is not built"
     else:

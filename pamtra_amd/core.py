@@ -457,6 +457,9 @@ class pyPamtra(object):
         negative hydrometeor moments become 0, other negative values
         missing.  wind_w, which the reference leaves out, is reset to
         missing on the new grid."""
+        assert len(self.df.dataFullSpec) == 0, \
+            "rescaleHeights does not regrid the full-spectrum input (hydro_fullspec)"
+
         def extrap(x, xp, yp):
             y = np.interp(x, xp, yp)
             lo, hi = x < xp[0], x > xp[-1]

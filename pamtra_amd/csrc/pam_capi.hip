@@ -564,7 +564,8 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   d.nf = nfreq;
   d.nbmax = 1;
   for (int h = 0; h < d.nh; h++) {
-    int nb = hydros[h].dist_name == PAM_DIST_MONO ? 2 : hydros[h].nbin;
+    const int dn = hydros[h].dist_name;
+    int nb = (dn == PAM_DIST_MONO || dn == PAM_DIST_MONO_COSMO_ICE) ? 2 : hydros[h].nbin;
     d.nbmax = std::max(d.nbmax, nb);
   }
   if (h4in && h4in->nbin) {
@@ -583,6 +584,10 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
         return PAM_FATAL;
       }
     d.nbmax = std::max(d.nbmax, (int)h4in->fs_nbin);
+  }
+  if (d.nbmax > PAM_MAX_NBIN) {
+    set_msg(res, "nbin: at most 4000 size bins per hydrometeor are supported");
+    return PAM_FATAL;
   }
   d.col0 = 0;
   d.ncol_chunk = d.ncol;

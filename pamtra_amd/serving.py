@@ -213,6 +213,12 @@ class ForwardServer:
         """Serves `requests` (arrival times in seconds from now when
         `realtime`, else all at once in order) and returns their results in
         request order."""
+        for r in requests:
+            if len(r.pam.df.dataFullSpec):
+                # the workspace bound counts layers from hydro_q, which the
+                # full-spectrum input does not use
+                raise ValueError(f"request {r.rid}: full-spectrum input (hydro_fullspec) is "
+                                 "not served; call runPamtra for it")
         order = sorted(range(len(requests)), key=lambda i: (requests[i].arrival, i))
         needs = [self.need_bytes(r) for r in requests]
         for i, n in enumerate(needs):

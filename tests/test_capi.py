@@ -97,6 +97,27 @@ def test_marshal_rejects_unsupported_options():
         _capi.marshal(pam2.nmlSet, pam2.set, pam2.df.data, pam2.df.data4D, pam2.p)
 
 
+def test_marshal_refuses_tensors_the_kernels_cannot_read():
+    """A torch input must be float64 (int32 for counts), contiguous, of the
+    expected element count and on the device; each fault is named."""
+    import numpy as np
+    import torch
+    from _scenario import build_pamtra
+    cases = [
+        (lambda q: torch.tensor(q, dtype=torch.float32), "float64 is required"),
+        (lambda q: torch.tensor(np.repeat(q, 2, axis=-1))[..., ::2], "not contiguous"),
+        (lambda q: torch.tensor(q.ravel()[:-1].copy()), "elements"),
+        (lambda q: torch.tensor(q), "device tensor"),
+    ]
+    for make, word in cases:
+        pam = build_pamtra()
+        q = np.ascontiguousarray(pam.p["hydro_q"])
+        p = dict(pam.p)
+        p["hydro_q"] = make(q)
+        with pytest.raises(ValueError, match=word):
+            _capi.marshal(pam.nmlSet, pam.set, pam.df.data, pam.df.data4D, p)
+
+
 def test_product_path_fails_loudly_without_a_gpu(monkeypatch):
     """No CPU fallback: without a visible device runPamtra raises."""
     from pamtra_amd import _lib

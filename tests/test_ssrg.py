@@ -302,8 +302,9 @@ def test_aggregates_at_32_angles_vs_oracle():
 
 
 def test_other_scattering_names_are_refused_with_the_reason():
-    for name, word in (("ssrga_0.2", "unset"), ("ss-rayleigh-gans", "active-only"),
-                       ("rayleigh", "active-only"), ("tmatrix", "T-matrix"),
-                       ("liudb_8", "unknown")):
+    for name, word in (("ssrga_0.2", "unset"), ("ss-rayleigh-gans", "asserts"),
+                       ("rayleigh", "backscatter-only"), ("rayleigh-gans", "backscatter-only"),
+                       ("tmatrix", "T-matrix"), ("liudb_8", "DDA-database"),
+                       ("hongdb_3", "DDA-database"), ("nonsense", "unknown")):
         with pytest.raises(NotImplementedError, match=word):
             _capi.scattering_model(name)
