@@ -453,6 +453,8 @@ def marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile,
     refl = profile.get("sfc_refl")
     if refl is not None:
         if _is_torch(refl):
+            _check_tensor(refl, "uint8", (nx, ny))
+            keep.append(refl)
             p.sfc_refl = C.cast(C.c_void_p(refl.data_ptr()), C.c_char_p)
         else:
             r = np.asarray(refl)

@@ -213,6 +213,7 @@ def test_device_entry_point_matches_host_entry_point():
     st = lib.pam_run_device(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),
                             C.byref(m.profile), fp, nf, C.byref(r), None)
     assert st == 0, msg.value
+    assert np.all(np.isfinite(host["tb"]))
     np.testing.assert_array_equal(tb.cpu().numpy(), host["tb"])
 
 
@@ -339,6 +340,7 @@ def test_device_entry_with_4d_descriptors_and_host_array_refusal():
     print("pam_run_device with HBM 4D descriptors", flush=True)
     st, tb, msg = _run_device(lib, ctx, m, freqs, torch)
     assert st == 0, msg
+    assert np.all(np.isfinite(host["tb"]))
     np.testing.assert_array_equal(tb.cpu().numpy(), host["tb"])
     h4 = np.ascontiguousarray(pam.df.data4D["rho_ms"], dtype=np.float64)
     hip = C.CDLL("libamdhip64.so.7")
