@@ -17,6 +17,7 @@
 #ifndef PAMTRA_AMD_H
 #define PAMTRA_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -253,6 +254,24 @@ int pam_run_device(pam_context *ctx, const pam_settings *settings,
                    const pam_hydro_desc *hydros, const pam_hydro_4d *hydro4d,
                    const pam_profile *profile, const double *freqs,
                    int32_t nfreq, pam_results *results, void *stream);
+
+/* As pam_run_device, but allocating nothing: no device memory, no events,
+ * no graphs.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Its workspace is a fixed array of
+ * PAM_NOALLOC_SCRATCH_BYTES per device, loaded with the library and held
+ * until the process ends; columns are processed in chunks that fit it, and
+ * a call whose smallest chunk (one column's optics) does not fit is refused
+ * with PAM_FATAL and a message.  Every call on one device shares that array
+ * and leaves its contents behind: calls that overlap on one device (other
+ * host threads or contexts, or calls on different streams) race.  The call
+ * synchronises `stream` before it returns; the stage times of
+ * pam_last_stage_times are not recorded. */
+#define PAM_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int pam_run_device_noalloc(pam_context *ctx, const pam_settings *settings,
+                           const pam_hydro_desc *hydros, const pam_hydro_4d *hydro4d,
+                           const pam_profile *profile, const double *freqs,
+                           int32_t nfreq, pam_results *results, void *stream);
 
 /* Kernel-level timings of the last call (milliseconds, measured with HIP
  * events on the call's stream), in the order of pam_stage_names(). */

@@ -22,6 +22,7 @@ NANGLES = 2 * NUMMU
 LIBRARIES = {16: "libpamtra_amd.so", 32: "libpamtra_amd_n64.so"}
 
 PAM_SUCCESS, PAM_WARNING, PAM_FATAL, PAM_INFO = 0, 1, 2, 3
+NOALLOC_SCRATCH_BYTES = 256 << 20  # PAM_NOALLOC_SCRATCH_BYTES
 
 DIST_NAMES = {
     "disabled": 0,
@@ -549,6 +550,8 @@ def declare_library(lib: C.CDLL) -> C.CDLL:
     lib.pam_run.argtypes = run_args
     lib.pam_run_device.restype = C.c_int
     lib.pam_run_device.argtypes = run_args + [C.c_void_p]
+    lib.pam_run_device_noalloc.restype = C.c_int
+    lib.pam_run_device_noalloc.argtypes = run_args + [C.c_void_p]
     lib.pam_last_stage_times.restype = C.c_int
     lib.pam_last_stage_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int32]
     lib.pam_stage_names.restype = C.c_char_p
@@ -574,7 +577,7 @@ def declare_library(lib: C.CDLL) -> C.CDLL:
 
 
 EXPORTED_SYMBOLS = ["pam_version", "pam_device_count", "pam_context_create",
-                    "pam_context_destroy", "pam_run", "pam_run_device",
+                    "pam_context_destroy", "pam_run", "pam_run_device", "pam_run_device_noalloc",
                     "pam_last_stage_times", "pam_stage_names", "pam_last_work",
                     "pam_workspace_bytes", "pam_context_set_workspace_limit", "pam_context_bytes",
                     "pam_context_trim", "pam_tuning_keys", "pam_context_set_tuning",

@@ -33,6 +33,48 @@ using namespace pam;
 
 constexpr int PAM_NWORK = 12;
 static const char *k_stage_names = "columns,dsd,mie,phase,doubling,adding";
+
+// The workspace of pam_run_device_noalloc: one array per device, loaded with
+// the code object and held until the process ends.  Host code never passes
+// `scratch` itself to a kernel (on the host it only names the symbol): its
+// device address comes from hipGetSymbolAddress at pam_context_create, once
+// per device, and is kept in g_scratch_ptr.
+static __device__ __align__(256) unsigned char scratch[PAM_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mutex;
+static std::map<int, void *> g_scratch_ptr;
+
+// Device address of this device's `scratch` (the device is current); null on
+// failure.
+static void *scratch_address(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void *ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  g_scratch_ptr[device] = ptr;
+  return ptr;
+}
+
+// Workspace regions of one call: grown DevBufs of the context, or (arena
+// set) consecutive pieces of a fixed region, with nothing allocated.
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+  // n bytes at a 256-byte boundary, or null when they do not fit
+  void *take(size_t n) {
+    const size_t a = (used + 255) & ~(size_t)255;
+    if (a > size || n > size - a) return nullptr;
+    used = a + n;
+    return base + a;
+  }
+  size_t left() const {
+    const size_t a = (used + 255) & ~(size_t)255;
+    return a >= size ? 0 : size - a;
+  }
+};
 constexpr int NSTAGE = 6;
 constexpr int PAM_NTUNE = 4;  // entries of k_tuning_keys
 
@@ -146,6 +188,7 @@ struct pam_context {
   // captured with (replayed while they are unchanged)
   hipGraphExec_t gexec[5] = {};   // one graph per timed stage of the chunk
   std::vector<unsigned char> gkey;
+  void *scratch = nullptr;        // device address of this device's `scratch`
 };
 
 static const char *k_tuning_keys = "mie_persistent,doubling_persistent,adding_persistent,graph";
@@ -337,6 +380,13 @@ extern "C" int pam_context_create(pam_context **out, int device) {
   c->budget_bytes = (size_t)(gb * (1ull << 30));
   if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     c->ncu = 0;
+  c->scratch = scratch_address(device);
+  if (!c->scratch) {
+    (void)hipStreamDestroy(c->stream);
+    (void)hipFree(c->d_mu);
+    delete c;
+    return PAM_FATAL;
+  }
   *out = c;
   return PAM_SUCCESS;
 }
@@ -508,9 +558,15 @@ extern "C" int64_t pam_workspace_bytes(int64_t ncol, int32_t max_nlyrs, int32_t 
                    64 * 256 + io + 64 * 256 + small);
 }
 
+// One call.  arena null: the workspace grows in the context's buffers, stage
+// times are recorded with events, and a single-chunk call may replay graphs.
+// arena set: every workspace region is carved from it, the items are chunked
+// to fit what is left, and nothing is allocated or created (no events, no
+// graphs); a call that cannot fit is refused.
 static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_desc *hydros,
                       const pam_hydro_4d *h4in, const pam_profile *pin, const double *freqs,
-                      int32_t nfreq, pam_results *res, hipStream_t stream) {
+                      int32_t nfreq, pam_results *res, hipStream_t stream,
+                      Arena *arena = nullptr) {
   if (!c || !st || !pin || !freqs || !res || !res->tb) {
     set_msg(res, "pam_run: NULL argument");
     return PAM_FATAL;
@@ -539,7 +595,23 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
       return PAM_FATAL;
     }
   HIP_OK(hipSetDevice(c->device));
-  if (!stream && c->tune[3]) {
+  // a workspace region: from the arena, or the context buffer grown to n
+  auto region = [&](DevBuf &b, size_t n) -> void * {
+    if (arena) return arena->take(n);
+    return b.ensure(n) == hipSuccess ? b.p : nullptr;
+  };
+  auto no_room = [&](const char *what) -> int {
+    char m[200];
+    if (arena)
+      snprintf(m, sizeof(m),
+               "pam_run_device_noalloc: %s does not fit the %zu MiB scratch; split the call",
+               what, (size_t)(arena->size >> 20));
+    else
+      snprintf(m, sizeof(m), "HIP error: device allocation failed (%s)", what);
+    set_msg(res, m);
+    return PAM_FATAL;
+  };
+  if (!stream && c->tune[3] && !arena) {
     // the legacy stream cannot be captured: the call runs on the context's
     // stream, after everything queued before it on the legacy stream (the
     // call synchronises before it returns)
@@ -624,6 +696,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   for (int i = 0; i < d.nh; i++)
     if (hydros[i].scat_name == PAM_SCAT_SSRG_RT3) s.ssrg = 1;
 
+
   Profile p;
   p.nlyrs = pin->nlyrs;
   p.hgt_lev = pin->hgt_lev;
@@ -679,8 +752,9 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   // small buffers: hydro table, frequencies, counters
   size_t small = 4096 + sizeof(pam_hydro_desc) * std::max(1, d.nh) + 16 * (size_t)nfreq + 256 + 64 * 8 +
                  3 * 4 * (PH_NBUCKET + 1) + 256;
-  HIP_OK(c->smallbuf.ensure(small));
-  Carver sc(c->smallbuf.p);
+  void *small_p = region(c->smallbuf, small);
+  if (!small_p) return no_room("the call's tables");
+  Carver sc(small_p);
   pam_hydro_desc *d_hyd = sc.take<pam_hydro_desc>(std::max(1, d.nh));
   double *d_freqs = sc.take<double>(nfreq);
   unsigned long long *d_cnt = sc.take<unsigned long long>(PAM_NWORK);
@@ -694,14 +768,16 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   HIP_OK(hipMemcpyAsync(d_freqs, freqs, 8 * (size_t)nfreq, hipMemcpyHostToDevice, stream));
   if (st->active && st->radar_k2)
     HIP_OK(hipMemcpyAsync(d_k2, st->radar_k2, 8 * (size_t)nfreq, hipMemcpyHostToDevice, stream));
-  HIP_OK(hipMemsetAsync(d_cnt, 0, 8 * PAM_NWORK, stream));
+  // counters and status words are cleared by a fill kernel, not a memset
+  launch_fill_u32((uint32_t *)d_cnt, 2 * PAM_NWORK, 0, stream);
 
   // per-column workspace
-  HIP_OK(c->colbuf.ensure(col_bytes(d)));
+  void *col_p = region(c->colbuf, col_bytes(d));
+  if (!col_p) return no_room("the per-column workspace");
   Work w;
   memset(&w, 0, sizeof(w));
   {
-    Carver cv(c->colbuf.p);
+    Carver cv(col_p);
     size_t ncol = d.ncol, ML = d.ML, nf = d.nf;
     w.hgt_lev = cv.take<double>(ncol * (ML + 1));
     w.temp_lev = cv.take<double>(ncol * (ML + 1));
@@ -725,8 +801,8 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     w.cf_status = cv.take<int32_t>(ncol * nf);
     w.redo_add = cv.take<int32_t>(ncol * nf);
     w.redo_lev = cv.take<int32_t>(ncol * nf);
-    HIP_OK(hipMemsetAsync(w.col_status, 0, ncol * 4, stream));
-    HIP_OK(hipMemsetAsync(w.cf_status, 0, ncol * nf * 4, stream));
+    launch_fill_u32((uint32_t *)w.col_status, ncol, 0, stream);
+    launch_fill_u32((uint32_t *)w.cf_status, ncol * nf, 0, stream);
   }
   w.mu = c->d_mu;
   w.qw = c->d_qw;
@@ -741,8 +817,8 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   w.counters = d_cnt;
   w.redo_n = d_redo_n;
   if (RT_SLOT_DOUBLES > 0) {
-    HIP_OK(c->rtbuf.ensure(RT_SLOT_DOUBLES * 8 * (size_t)std::max(1, c->ncu)));
-    w.rt_scratch = (double *)c->rtbuf.p;
+    w.rt_scratch = (double *)region(c->rtbuf, RT_SLOT_DOUBLES * 8 * (size_t)std::max(1, c->ncu));
+    if (!w.rt_scratch) return no_room("the adding kernels' scratch");
   }
   w.queue = d_queue;
   w.radar_k2 = (st->active && st->radar_k2) ? d_k2 : nullptr;
@@ -750,6 +826,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   std::vector<hipEvent_t> evs;
   std::vector<int> ev_stage;
   auto mark = [&](int stage) -> hipError_t {
+    if (arena) return hipSuccess;  // no stage times without events
     hipEvent_t e;
     hipError_t r = hipEventCreate(&e);
     if (r != hipSuccess) return r;
@@ -789,6 +866,12 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   size_t budget = c->budget_bytes;
   if (const char *e = getenv("PAM_WORKSPACE_GB")) budget = (size_t)(atof(e) * (1ull << 30));
   size_t max_items = std::max<size_t>(1, budget / per_item);
+  if (arena) {
+    const size_t room = arena->left();
+    max_items = room > 64 * 256 ? (room - 64 * 256) / per_item : 0;
+    if (max_items == 0 && total > 0) return no_room("one scattering layer's optics");
+    max_items = std::max<size_t>(1, max_items);
+  }
   // chunks of whole columns with about equal item counts (no small tail chunk)
   std::vector<std::pair<int, int>> chunks;
   {
@@ -812,7 +895,9 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     for (int i = ch.first; i < ch.second; i++) n += cnt[i];
     chunk_items_max = std::max<size_t>(chunk_items_max, (size_t)n);
   }
-  HIP_OK(c->itembuf.ensure(std::max<size_t>(1, chunk_items_max) * per_item + 64 * 256));
+  if (arena && chunk_items_max > max_items) return no_room("the optics of one column");
+  void *item_p = region(c->itembuf, std::max<size_t>(1, chunk_items_max) * per_item + 64 * 256);
+  if (!item_p) return no_room("the per-layer optics");
   c->work[0] = d.ncol;
   c->work[1] = d.nf;
   c->work[2] = total * d.nf;
@@ -855,7 +940,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     }
     return hipGetLastError();
   };
-  const bool use_graph = c->tune[3] && chunks.size() == 1;
+  const bool use_graph = c->tune[3] && chunks.size() == 1 && !arena;
   for (auto &ch : chunks) {
     Dims dc;
     memcpy(&dc, &d, sizeof(d));
@@ -865,7 +950,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     int64_t n = 0;
     for (int i = ch.first; i < ch.second; i++) n += cnt[i];
     dc.nitems = (int)n;
-    carve_items(w, c->itembuf.p, dc, std::max<size_t>(1, (size_t)n));
+    carve_items(w, item_p, dc, std::max<size_t>(1, (size_t)n));
     if (use_graph) {
       // the launch arguments the graphs bake in: shapes, settings,
       // workspace and the caller's device pointers
@@ -1036,6 +1121,29 @@ extern "C" int pam_run_device(pam_context *c, const pam_settings *st, const pam_
     }
   }
   return run_device(c, st, hyd, h4, p, freqs, nfreq, res, (hipStream_t)stream);
+}
+
+extern "C" int pam_run_device_noalloc(pam_context *c, const pam_settings *st,
+                                      const pam_hydro_desc *hyd, const pam_hydro_4d *h4,
+                                      const pam_profile *p, const double *freqs, int32_t nfreq,
+                                      pam_results *res, void *stream) {
+  if (c && p && res) {
+    if (hipSetDevice(c->device) != hipSuccess) return PAM_FATAL;
+    if (const char *bad = first_host_array(h4, p, res)) {
+      char m[160];
+      snprintf(m, sizeof(m), "pam_run_device_noalloc: %s is not device memory", bad);
+      set_msg(res, m);
+      return PAM_FATAL;
+    }
+  }
+  if (!c || !c->scratch) {
+    set_msg(res, "pam_run_device_noalloc: no context or no scratch on its device");
+    return PAM_FATAL;
+  }
+  Arena a;
+  a.base = (char *)c->scratch;
+  a.size = PAM_NOALLOC_SCRATCH_BYTES;
+  return run_device(c, st, hyd, h4, p, freqs, nfreq, res, (hipStream_t)stream, &a);
 }
 
 // Host arrays in, host arrays out: stage through one device allocation.
