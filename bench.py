@@ -177,8 +177,8 @@ def large_domain(args, torch):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.large_steps
         tbs[precision] = tb.cpu().numpy()
-        work = (C.c_int64 * 12)()
-        lib.pam_last_work(ctx, work, 12)
+        work = (C.c_int64 * 14)()
+        lib.pam_last_work(ctx, work, 14)
         stage = {n: v / args.large_steps for n, v in zip(names, acc)}
         res[precision] = {"value": round(nx * ny * nf / dt, 2), "ms_per_step": round(dt * 1e3, 3),
                           "stage_ms": {n: round(v, 3) for n, v in stage.items()},
@@ -384,8 +384,8 @@ def main():
         for i in range(args.warmup):
             step()
     dt, stage_ms = timed(step, args.steps)
-    work = (C.c_int64 * 12)()
-    lib.pam_last_work(ctx, work, 12)
+    work = (C.c_int64 * 14)()
+    lib.pam_last_work(ctx, work, 14)
     work = list(work)
     value = nx * ny * nf / dt
 

@@ -284,7 +284,9 @@ const char *pam_stage_names(void);
  * [7] sum over Mie bins of the series length, [8] sum over scattering items
  * of (NUMPTS/2+1)*(nlegen+1), the phase-matrix work, [9] 32x32 matrix
  * products executed by the doubling, [10] the same for the adding,
- * [11] units recomputed by the Gauss-Jordan pass. */
+ * [11] units recomputed by the Gauss-Jordan pass, [12] sum over Mie bins of
+ * nquad * series length (the angular sums), [13] sum over (item,
+ * hydrometeor) of nquad * (nlegen + 1) (the Legendre projection). */
 int pam_last_work(const pam_context *ctx, int64_t *counts, int32_t n);
 
 /* Device bytes one call needs for these dimensions, with `nitems` layers

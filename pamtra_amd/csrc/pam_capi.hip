@@ -31,7 +31,7 @@ using namespace pam;
     }                                                                          \
   } while (0)
 
-constexpr int PAM_NWORK = 12;
+constexpr int PAM_NWORK = 14;
 static const char *k_stage_names = "columns,dsd,mie,phase,doubling,adding";
 
 // The workspace of pam_run_device_noalloc: one array per device, loaded with
@@ -172,6 +172,7 @@ struct pam_context {
   hipStream_t stream = nullptr;  // the stream of pam_run (non-blocking)
   DevBuf colbuf, itembuf, iobuf, smallbuf;
   DevBuf rtbuf;  // global scratch of the adding kernels at N = 64
+  DevBuf miebuf; // U slots of the coefficient-first Mie kernel
   const double *d_atab = nullptr;
   const int64_t *d_atab_off = nullptr;
   int atab_nmax = 0;
@@ -399,6 +400,7 @@ extern "C" int pam_context_destroy(pam_context *c) {
   c->iobuf.release();
   c->smallbuf.release();
   c->rtbuf.release();
+  c->miebuf.release();
   drop_graph(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->d_mu) (void)hipFree(c->d_mu);
@@ -445,7 +447,8 @@ extern "C" int pam_context_set_workspace_limit(pam_context *c, int64_t bytes) {
 
 extern "C" int64_t pam_context_bytes(const pam_context *c) {
   if (!c) return -1;
-  return (int64_t)(c->colbuf.cap + c->itembuf.cap + c->iobuf.cap + c->smallbuf.cap + c->rtbuf.cap);
+  return (int64_t)(c->colbuf.cap + c->itembuf.cap + c->iobuf.cap + c->smallbuf.cap + c->rtbuf.cap +
+                   c->miebuf.cap);
 }
 
 extern "C" int pam_context_trim(pam_context *c) {
@@ -458,6 +461,7 @@ extern "C" int pam_context_trim(pam_context *c) {
   c->iobuf.release();
   c->smallbuf.release();
   c->rtbuf.release();
+  c->miebuf.release();
   return PAM_SUCCESS;
 }
 
@@ -695,7 +699,10 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   s.ncu = c->ncu;
   for (int i = 0; i < d.nh; i++)
     if (hydros[i].scat_name == PAM_SCAT_SSRG_RT3) s.ssrg = 1;
-
+  {
+    const char *mk = getenv("PAM_MIE");
+    s.mie_chunked = (mk && strcmp(mk, "chunked") == 0) ? 1 : 0;
+  }
 
   Profile p;
   p.nlyrs = pin->nlyrs;
@@ -819,6 +826,19 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   if (RT_SLOT_DOUBLES > 0) {
     w.rt_scratch = (double *)region(c->rtbuf, RT_SLOT_DOUBLES * 8 * (size_t)std::max(1, c->ncu));
     if (!w.rt_scratch) return no_room("the adding kernels' scratch");
+  }
+  if (!s.ssrg && !s.mie_chunked) {
+    // one U slot per workgroup of the Mie kernel's persistent grid; within
+    // the fixed scratch of the allocation-free entry, at most a quarter of it
+    const int per_cu = s.pers_mie > 0 ? s.pers_mie : MIE2_PER_CU;
+    w.mie_ld = mie_slot_rows(d.nbmax);
+    w.mie_slot_doubles = mie_slot_doubles(d.nbmax);
+    size_t slots = (size_t)std::max(1, c->ncu) * per_cu;
+    if (arena) slots = std::min(slots, arena->size / 4 / (w.mie_slot_doubles * 8));
+    if (slots == 0) return no_room("one Mie workgroup's slot");
+    w.mie_slots = (int)slots;
+    w.mie_u = (double *)region(c->miebuf, slots * w.mie_slot_doubles * 8);
+    if (!w.mie_u) return no_room("the Mie kernel's slots");
   }
   w.queue = d_queue;
   w.radar_k2 = (st->active && st->radar_k2) ? d_k2 : nullptr;

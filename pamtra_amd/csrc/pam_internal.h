@@ -80,6 +80,10 @@ struct Settings {
   int pers_mie, pers_dbl, pers_add;
   int ncu;  // compute units of the device
   int ssrg;  // some hydrometeor uses the self-similar Rayleigh-Gans model
+  // 1: the chunked Mie kernel for every call (PAM_MIE=chunked, a test mode);
+  // otherwise calls without self-similar Rayleigh-Gans take the
+  // coefficient-first kernel
+  int mie_chunked;
   // radar simulator, "simple" mode (active): attenuation 0 disabled,
   // 1 top-down, 2 bottom-up; polarisations 0 NN, 1 HH, 2 VV, 3 HV, 4 VH
   int active, radar_att, radar_npol, radar_pol[5];
@@ -160,8 +164,13 @@ struct Work {
   int32_t *redo_add;                   // [ncol*nf]
   int32_t *redo_lev;                   // [ncol*nf]: units with an interior output level
   double *rt_scratch;                  // [ncu][RT_SLOT_DOUBLES] at N = 64, else null
+  // U slots of the coefficient-first Mie kernel: one per workgroup of its
+  // persistent grid, mie_ld rows (16 per 8 bins) x 2 MAX_NTERMS columns
+  double *mie_u;
+  size_t mie_slot_doubles;
+  int mie_ld, mie_slots;
   // work counters (atomics)
-  unsigned long long *counters;        // [12], see pam_last_work
+  unsigned long long *counters;        // [PAM_NWORK], see pam_last_work
   // work-queue counters of the persistent grids: mie, doubling, adding
   // (cleared per chunk)
   uint32_t *queue;                     // [4]
@@ -272,6 +281,10 @@ void build_phase_tables(double *tab, const int64_t *d_off, const double *d_mu, h
 void launch_phase(const Dims &d, const Settings &s, Work &w, hipStream_t st);
 void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, const Hydro4D &h4,
                 const Profile &p, const double *freqs, Work &w, hipStream_t st);
+// rows and doubles of one U slot of the coefficient-first Mie kernel
+int mie_slot_rows(int nbmax);
+size_t mie_slot_doubles(int nbmax);
+constexpr int MIE2_PER_CU = 8;  // default workgroups per CU of its persistent grid
 void launch_surface(const Dims &d, const Settings &s, const Profile &p, const double *freqs,
                     Work &w, double *emis_out, hipStream_t st);
 void launch_doubling(const Dims &d, const Settings &s, const double *freqs, Work &w,

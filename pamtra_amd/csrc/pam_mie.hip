@@ -17,6 +17,8 @@
 // only on the series length N (nquad and nlegen are functions of N), so they
 // are tabulated once per context (build_angle_tables) with the recurrences
 // the reference evaluates per bin.
+#include <algorithm>
+
 #include "pam_internal.h"
 
 namespace pam {
@@ -451,7 +453,7 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
   __syncthreads();
   double kexttot = 0.0, salbedo = 0.0, backtot = 0.0;
   int nleg_coef = 0;
-  long long work_bins = 0, work_series = 0;
+  long long work_bins = 0, work_series = 0, work_ang = 0, work_proj = 0;
 
   for (int hh = 0; hh < d.nh; hh++) {
     const int nbin = s_hnb[hh];
@@ -842,6 +844,7 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
           sumqb = sumqb + s_red[q][2];
           work_bins += s_redi[q][0];
           work_series += s_redi[q][1];
+          work_ang += (long long)nquad * s_redi[q][1];
         }
       }
     }
@@ -852,6 +855,7 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
       s_sum[2] = sumqb;
       s_nlh = 0;
       s_nan = 0;
+      work_proj += (long long)nquad * (nlegen + 1);
     }
     __syncthreads();
     sumqe = s_sum[0];
@@ -944,7 +948,484 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
     w.nleg[itf] = nleg_coef;
     atomicAdd(&w.counters[6], (unsigned long long)work_bins);
     atomicAdd(&w.counters[7], (unsigned long long)work_series);
+    atomicAdd(&w.counters[12], (unsigned long long)work_ang);
+    atomicAdd(&w.counters[13], (unsigned long long)work_proj);
   }
+}
+
+// ---------------------------------------------------------------------
+// Mie spheres, coefficient-first form (the kernel for calls without
+// self-similar Rayleigh-Gans hydrometeors).  One 128-thread workgroup per
+// (scattering layer, channel) unit, persistent, with a slot of global
+// scratch per workgroup.  Per hydrometeor:
+//   A. threads over all size bins at once: the D_n recursion, the upward
+//      psi_n/chi_n recursion with a_n, b_n (miecalc) and the efficiencies
+//      (miecross) in one serial pass per bin; a_n, b_n go to the slot as
+//      the U matrix of the S1/S2 product (column-major, row pairs re/im per
+//      bin as in u_row_re), D_n passes through the a-columns;
+//   B. waves over (angle tile, row tile): S1/S2 on the FP64 matrix cores
+//      with U read from the slot (L2-resident, just written by this
+//      workgroup) and series terms beyond a bin's own length read as zero;
+//      the DSD-weighted phase sums into LDS;
+//   C. threads over Legendre orders, kept in registers across hydrometeors.
+// No stage is chunked over bins, so a hydrometeor costs one pass of each
+// phase and four barriers, and the small LDS footprint keeps several units
+// resident per compute unit.
+// ---------------------------------------------------------------------
+constexpr int MIE2_WG = 128;
+constexpr int MIE2_NW = MIE2_WG / 64;
+constexpr int MIE2_TAB = 640;          // LDS stage of c_n pi_n, c_n tau_n (n x angles)
+constexpr int MIE2_NL = (MAXNLEG + MIE2_WG - 1) / MIE2_WG;  // Legendre orders per thread
+
+struct Mie2Lds {
+  double ps[3][MAX_NQUAD];     // weighted phase sums per angle
+  double tab[2 * MIE2_TAB];    // angle table stage
+  double wb[MIE_MAXBIN];       // per bin: n(D) dD if the bin scatters, else 0
+  short nmb[MIE_MAXBIN];       // per bin: series length (0: skipped)
+  double pw[MIE2_NW][6 * 16];  // per-wave phase sums of shared angle tiles
+  double red[MIE2_NW][3];
+  int redi[MIE2_NW][2];
+  double hm[MAX_HYDRO][2];
+  int hnb[MAX_HYDRO], hnt[MAX_HYDRO];
+  int nlh, nan_;
+  int slot;
+};
+
+// Rows of the U slot for bin b (re; im = re + 4): 8 bins per 16-row tile.
+__device__ __forceinline__ int mie2_row(int b) { return 16 * (b >> 3) + u_row_re(b & 7); }
+// bin of row r of a 16-row tile
+__device__ __forceinline__ int mie2_tile_bin(int r) { return (r & 3) + 4 * (r >> 3); }
+
+__device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
+                                          const pam_hydro_desc *hyd, const double *freqs,
+                                          const Work &w, int itf, double *U, Mie2Lds &L) {
+  const int t = threadIdx.x;
+  const int wv = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
+  const int li_item = itf / d.nf, f = itf % d.nf;
+  const int col = w.item_col[li_item], z = w.item_layer[li_item];
+  int32_t *st = w.itf_status + itf;
+  const size_t oy = (size_t)col * d.ML + z;
+  const double freq = freqs[f];
+  const double wavelength = C_LIGHT / (freq * 1e9);
+  const int ld = w.mie_ld;  // rows of the U slot (16 per 8 bins)
+
+  // per hydrometeor, one thread each: bins, refractive index
+  // (scatProperties.f90:227-243), series length of the largest bin
+  // (mie_spheres.f90:139-158)
+  if (t < d.nh) {
+    const int nbin = w.dsd_flag[(size_t)li_item * d.nh + t];
+    int nterms = 0;
+    cd m = cd{0.0, 0.0};
+    if (nbin > 0) {
+      const double T = w.temp[oy];
+      if (hyd[t].liq_ice == 1)
+        m = csqrt_(eps_water_tkc(T - sp(273.15), freq));
+      else
+        m = csqrt_(eps_ice(T, freq));
+      const double *dia = w.dsd_d + ((size_t)li_item * d.nh + t) * d.nbmax;
+      nterms = mie_nstop(PI * dia[nbin - 1] / wavelength);
+    }
+    L.hnb[t] = nbin;
+    L.hnt[t] = nterms;
+    L.hm[t][0] = m.re;
+    L.hm[t][1] = m.im;
+  }
+  double lc[NCOEF][MIE2_NL];
+#pragma unroll
+  for (int k = 0; k < NCOEF; k++)
+#pragma unroll
+    for (int j = 0; j < MIE2_NL; j++) lc[k][j] = 0.0;
+  __syncthreads();
+  double kexttot = 0.0, salbedo = 0.0, backtot = 0.0;
+  int nleg_coef = 0;
+  long long work_bins = 0, work_series = 0, work_ang = 0, work_proj = 0;
+
+  for (int hh = 0; hh < d.nh; hh++) {
+    const int nbin = L.hnb[hh];
+    if (nbin <= 0) continue;
+    const size_t ob = ((size_t)li_item * d.nh + hh) * d.nbmax;
+    const double *dia = w.dsd_d + ob, *dd = w.dsd_dd + ob, *nd = w.dsd_nd + ob,
+                 *rho = w.dsd_rho + ob;
+    const cd m_ice = cd{L.hm[hh][0], -L.hm[hh][1]};
+    const bool is_ice = (hyd[hh].liq_ice == -1);
+    const int nterms = L.hnt[hh];
+    if (nterms < 1 || nterms > w.atab_nmax || nterms > MAX_NTERMS || nbin > MIE_MAXBIN ||
+        16 * ((nbin + 7) >> 3) > ld) {
+      if (t == 0) set_err(st, E_MIE_TERMS);
+      return;
+    }
+    const int nph = max(4, (nterms + 3) & ~3);  // b-columns start here
+    const int nlegen = mie_nlegen(nterms);
+    const int nquad = mie_nquad(nterms);
+    const double *tmu = w.atab + w.atab_off[nterms], *twt = tmu + nquad, *tpi = twt + nquad,
+                 *ttau = tpi + (size_t)nterms * nquad, *tpl = ttau + (size_t)nterms * nquad;
+    const int nh2 = (nquad + 1) >> 1, j0 = nquad - nh2;
+    const int nct = (nh2 + 15) >> 4;
+    const int nrt = (nbin + 7) >> 3;
+    for (int k = t; k < 3 * MAX_NQUAD; k += MIE2_WG) (&L.ps[0][0])[k] = 0.0;
+    // c_n pi_n, c_n tau_n at the angles mu >= 0 into LDS when they fit
+    const bool tab_lds = nterms * nh2 <= MIE2_TAB;
+    if (tab_lds)
+      for (int e = t; e < nterms * nh2; e += MIE2_WG) {
+        const int n = e / nh2, h = e - n * nh2;
+        L.tab[e] = tpi[(size_t)n * nquad + j0 + h];
+        L.tab[MIE2_TAB + e] = ttau[(size_t)n * nquad + j0 + h];
+      }
+
+    // --- A: one thread per bin, the whole series (mie_scat_utilities.f90:8-116)
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+    int nbw = 0, nsw = 0;
+    for (int ir = t; ir < nbin; ir += MIE2_WG) {
+      int nmie = 0;
+      if (nd[ir] > 0.0) {
+        const int ns = mie_nstop(PI * dia[ir] / wavelength);
+        if (ns > nterms) set_err(st, E_MIE_TERMS);
+        nmie = min(ns, nterms);
+      }
+      L.nmb[ir] = (short)nmie;
+      L.wb[ir] = nmie > 0 ? nd[ir] * dd[ir] : 0.0;
+      if (nmie <= 0) continue;
+      // soft sphere: index of the ice-air mixture at the bin's density
+      // (mie_spheres.f90:187-192), decided per bin
+      const cd msp = (is_ice && rho[ir] != 917.0) ? eps_mix_air(m_ice, rho[ir]) : m_ice;
+      const double x = PI * dia[ir] / wavelength;
+      const cd mc = conj(msp);
+      const cd y = mc * x;
+      const cd yinv = cdiv_m(cd{1.0, 0.0}, y);
+      const double xinv = rcp_nr(x);
+      const cd mcinv = cdiv_m(cd{1.0, 0.0}, mc);
+      double *ure = U + mie2_row(ir), *uim = ure + 4;
+      // downward log-derivative recursion from n = nmie + 15; D_n (n <= nmie)
+      // parked in the a-column n-1
+      cd dn = cd{0.0, 0.0};
+      for (int n = nmie + 15; n >= 2; n--) {
+        const cd ny = (double)n * yinv;
+        dn = ny - cdiv_m(cd{1.0, 0.0}, dn + ny);
+        if (n - 1 <= nmie) {
+          ure[(size_t)(n - 2) * ld] = dn.re;
+          uim[(size_t)(n - 2) * ld] = dn.im;
+        }
+      }
+      // upward Riccati-Bessel recursion, a_n, b_n and the efficiency sums in
+      // ascending n
+      double psim = cos(x), psin = sin(x), chim = -sin(x), chin = cos(x);
+      double s1 = 0.0, s2 = 0.0;
+      cd s3 = cd{0.0, 0.0};
+      for (int n = 1; n <= nmie; n++) {
+        const double fn = (double)(2 * n - 1) * xinv;
+        double tmp = psin;
+        psin = fn * psin - psim;
+        psim = tmp;
+        tmp = chin;
+        chin = fn * chin - chim;
+        chim = tmp;
+        const cd xin = cd{sp(psin), sp(-chin)};
+        const cd xim = cd{sp(psim), sp(-chim)};
+        const cd Dn = cd{ure[(size_t)(n - 1) * ld], uim[(size_t)(n - 1) * ld]};
+        const double nx = (double)n * xinv;
+        cd ct = Dn * mcinv + nx;
+        const cd an = cdiv_m(ct * psin - psim, ct * xin - xim);
+        ct = mc * Dn + nx;
+        const cd bn = cdiv_m(ct * psin - psim, ct * xin - xim);
+        ure[(size_t)(n - 1) * ld] = an.re;
+        uim[(size_t)(n - 1) * ld] = an.im;
+        ure[(size_t)(nph + n - 1) * ld] = bn.re;
+        uim[(size_t)(nph + n - 1) * ld] = bn.im;
+        const double wn = (double)(2 * n + 1);
+        s1 = s1 + wn * (an.re + bn.re);
+        s2 = s2 + wn * ((an.re * an.re + an.im * an.im) + (bn.re * bn.re + bn.im * bn.im));
+        const double ws = (n & 1) ? -wn : wn;
+        s3 = s3 + ws * cd{an.re - bn.re, an.im - bn.im};
+      }
+      const double rr = dia[ir] / 2.0;
+      const double ar = (rr * rr) * PI;
+      const double qe = 2.0 / (x * x) * s1, qs = 2.0 / (x * x) * s2;
+      const double qb = ((1.0 / (x * x) * s3) * conj(s3)).re;
+      q0 += qe * ar * nd[ir] * dd[ir];
+      q1 += qs * ar * nd[ir] * dd[ir];
+      q2 += qb * ar * nd[ir] * dd[ir];
+      nbw += 1;
+      nsw += nmie;
+    }
+    q0 = wave_sum(q0);
+    q1 = wave_sum(q1);
+    q2 = wave_sum(q2);
+    nbw = wave_sum(nbw);
+    nsw = wave_sum(nsw);
+    if (lane == 0) {
+      L.red[wv][0] = q0;
+      L.red[wv][1] = q1;
+      L.red[wv][2] = q2;
+      L.redi[wv][0] = nbw;
+      L.redi[wv][1] = nsw;
+    }
+    __syncthreads();  // U, the series lengths, weights and table stage are ready
+
+    // --- B: S1, S2 of every (bin, angle) on the matrix cores and the
+    // DSD-weighted phase-function sums (mieangle, mie_spheres.f90:238-247),
+    // parity-split over mu >= 0 as in the chunked kernel.  Waves own angle
+    // tiles; with fewer tiles than waves, wpc waves share a tile, each over
+    // every wpc-th row tile.  A row tile's own longest series sets its
+    // number of k-steps.
+    const int wpc = nct >= MIE2_NW ? 1 : MIE2_NW / nct;
+    for (int ctw = (wpc > 1 ? wv / wpc : wv); ctw < nct; ctw += (wpc > 1 ? nct : MIE2_NW)) {
+      const int sub = wpc > 1 ? wv % wpc : 0;
+      const int hix = 16 * ctw + li;
+      const bool ang_ok = hix < nh2;
+      const int ang = j0 + hix, angm = nquad - 1 - ang;
+      double p1 = 0.0, p2 = 0.0, p3 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+      for (int rt = sub; rt < nrt; rt += wpc) {
+        // longest series of the tile's 8 bins, and of this lane's row
+        int nmt = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int b = 8 * rt + q;
+          nmt = max(nmt, b < nbin ? (int)L.nmb[b] : 0);
+        }
+        const int brow = 8 * rt + mie2_tile_bin(li);
+        const int nrow = brow < nbin ? (int)L.nmb[brow] : 0;
+        const int np = max(4, (nmt + 3) & ~3);
+        const int nsk = np >> 2;
+        d4m_t x0 = {0.0, 0.0, 0.0, 0.0}, y0 = x0, x1 = x0, y1 = x0;
+        const double *urow = U + 16 * rt + li;
+        for (int sk = 0; sk < nsk; sk++) {
+#pragma unroll
+          for (int g = 0; g < 2; g++) {
+            int cu, n;
+            bool ap;
+            mie_kmap(g, 4 * sk + lk, np, cu, n, ap);
+            double pv = 0.0, tv = 0.0;
+            if (ang_ok && n <= nterms) {
+              if (tab_lds) {
+                pv = L.tab[(n - 1) * nh2 + hix];
+                tv = L.tab[MIE2_TAB + (n - 1) * nh2 + hix];
+              } else {
+                pv = tpi[(size_t)(n - 1) * nquad + ang];
+                tv = ttau[(size_t)(n - 1) * nquad + ang];
+              }
+            }
+            const int cg = ap ? n - 1 : nph + n - 1;
+            const double a = (n <= nrow) ? urow[(size_t)cg * ld] : 0.0;
+            if (g == 0) {
+              x0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? pv : tv, x0, 0, 0, 0);
+              y0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? tv : pv, y0, 0, 0, 0);
+            } else {
+              x1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? pv : tv, x1, 0, 0, 0);
+              y1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? tv : pv, y1, 0, 0, 0);
+            }
+          }
+        }
+#pragma unroll
+        for (int h2 = 0; h2 < 2; h2++) {
+          const int bw = 8 * rt + 4 * h2 + lk;
+          const double wgt = bw < nbin ? L.wb[bw] : 0.0;
+          const int re = 2 * h2, im = 2 * h2 + 1;
+          // +mu
+          double s1r = x0[re] + x1[re], s1i = x0[im] + x1[im];
+          double s2r = y1[re] + y0[re], s2i = y1[im] + y0[im];
+          double a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
+          p1 += (0.5 * (a2 + a1)) * wgt;
+          p2 += (0.5 * (a2 - a1)) * wgt;
+          p3 += (s2r * s1r + s2i * s1i) * wgt;
+          // -mu
+          s1r = x0[re] - x1[re];
+          s1i = x0[im] - x1[im];
+          s2r = y1[re] - y0[re];
+          s2i = y1[im] - y0[im];
+          a1 = s1r * s1r + s1i * s1i;
+          a2 = s2r * s2r + s2i * s2i;
+          m1 += (0.5 * (a2 + a1)) * wgt;
+          m2 += (0.5 * (a2 - a1)) * wgt;
+          m3 += (s2r * s1r + s2i * s1i) * wgt;
+        }
+      }
+      // the four lane groups hold different bins of the same angle
+      p1 += __shfl_xor(p1, 16, 64);
+      p1 += __shfl_xor(p1, 32, 64);
+      p2 += __shfl_xor(p2, 16, 64);
+      p2 += __shfl_xor(p2, 32, 64);
+      p3 += __shfl_xor(p3, 16, 64);
+      p3 += __shfl_xor(p3, 32, 64);
+      m1 += __shfl_xor(m1, 16, 64);
+      m1 += __shfl_xor(m1, 32, 64);
+      m2 += __shfl_xor(m2, 16, 64);
+      m2 += __shfl_xor(m2, 32, 64);
+      m3 += __shfl_xor(m3, 16, 64);
+      m3 += __shfl_xor(m3, 32, 64);
+      if (lk == 0 && ang_ok) {
+        if (wpc == 1) {  // this wave alone owns the pair
+          L.ps[0][ang] = p1;
+          L.ps[1][ang] = p2;
+          L.ps[2][ang] = p3;
+          if (angm != ang) {
+            L.ps[0][angm] = m1;
+            L.ps[1][angm] = m2;
+            L.ps[2][angm] = m3;
+          }
+        } else {
+          double *pw = L.pw[wv];
+          pw[li] = p1;
+          pw[16 + li] = p2;
+          pw[32 + li] = p3;
+          pw[48 + li] = m1;
+          pw[64 + li] = m2;
+          pw[80 + li] = m3;
+        }
+      }
+    }
+    __syncthreads();
+    if (wpc > 1 && t < 16 * nct) {  // the shared tiles, waves in order
+      const int ctw = t >> 4, l16 = t & 15, hix = t;
+      if (hix < nh2) {
+        const int ang = j0 + hix, angm = nquad - 1 - ang;
+        double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < wpc; q++) {
+          const double *pw = L.pw[ctw * wpc + q];
+#pragma unroll
+          for (int k = 0; k < 6; k++) v[k] += pw[16 * k + l16];
+        }
+        L.ps[0][ang] = v[0];
+        L.ps[1][ang] = v[1];
+        L.ps[2][ang] = v[2];
+        if (angm != ang) {
+          L.ps[0][angm] = v[3];
+          L.ps[1][angm] = v[4];
+          L.ps[2][angm] = v[5];
+        }
+      }
+    }
+    // efficiency sums, waves in order
+    double sumqe = 0.0, sumqs = 0.0, sumqb = 0.0;
+    for (int q = 0; q < MIE2_NW; q++) {
+      sumqe = sumqe + L.red[q][0];
+      sumqs = sumqs + L.red[q][1];
+      sumqb = sumqb + L.red[q][2];
+    }
+    if (t == 0) {
+      long long ns_h = 0;
+      for (int q = 0; q < MIE2_NW; q++) {
+        work_bins += L.redi[q][0];
+        ns_h += L.redi[q][1];
+      }
+      work_series += ns_h;
+      work_ang += (long long)nquad * ns_h;
+      work_proj += (long long)nquad * (nlegen + 1);
+      L.nlh = 0;
+      L.nan_ = 0;
+    }
+    const double extinction = s.lhyd_abs ? sumqe : sumqe - sumqs;
+    const double scatter = sumqs;
+    const double albedo = scatter / extinction;
+    if (!(extinction > 0.0) || !(scatter > 0.0) || !(sumqb > 0.0)) {
+      if (t == 0) set_err(st, E_MIE_RESULT);
+      return;
+    }
+    __syncthreads();  // phase sums complete
+    // weighted phase sums per angle, normalised (mie_spheres.f90:272-280), in place
+    const double tmp = wavelength * wavelength / (PI * scatter);
+    for (int k = t; k < nquad; k += MIE2_WG) {
+      const double wt = twt[k];
+      L.ps[0][k] = tmp * L.ps[0][k] * wt;
+      L.ps[1][k] = tmp * L.ps[1][k] * wt;
+      L.ps[2][k] = tmp * L.ps[2][k] * wt;
+    }
+    __syncthreads();
+    // --- C: Legendre projection (mie_spheres.f90:272-314), thread per
+    // order, over the pairs of angles: P_l(-mu) = (-1)^l P_l(mu)
+    double g[3][MIE2_NL];
+#pragma unroll
+    for (int j = 0; j < MIE2_NL; j++) {
+      const int l = t + MIE2_WG * j;
+      g[0][j] = g[1][j] = g[2][j] = 0.0;
+      if (l <= nlegen) {
+        double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+        const double sg = (l & 1) ? -1.0 : 1.0;
+#pragma unroll 4
+        for (int h = 0; h < nh2; h++) {
+          const int k = j0 + h, km = nquad - 1 - k;
+          const double pl = tpl[(size_t)k * (nlegen + 1) + l];
+          double v1 = L.ps[0][k], v2 = L.ps[1][k], v3 = L.ps[2][k];
+          if (km != k) {
+            v1 += sg * L.ps[0][km];
+            v2 += sg * L.ps[1][km];
+            v3 += sg * L.ps[2][km];
+          }
+          c1 += v1 * pl;
+          c2 += v2 * pl;
+          c3 += v3 * pl;
+        }
+        const double hl = (double)((float)(2 * l + 1) / 2.0f);
+        g[0][j] = hl * c1;
+        g[1][j] = hl * c2;
+        g[2][j] = hl * c3;
+        if (g[0][j] > sp(1.0e-7)) atomicMax(&L.nlh, l);
+        if (isnan(g[0][j]) || isnan(c2) || isnan(c3)) L.nan_ = 1;
+      }
+    }
+    __syncthreads();
+    if (L.nan_) {
+      if (t == 0) set_err(st, E_NAN_COEF);
+      return;
+    }
+    // accumulate over hydrometeors (scatProperties.f90:920-946)
+    const int nl_h = L.nlh;
+    if (nl_h > nleg_coef) nleg_coef = nl_h;
+    kexttot = kexttot + extinction;
+    backtot = backtot + sumqb;  // rt_back (scatProperties.f90:743-757, 922)
+    salbedo = (kexttot == 0.0) ? 0.0 : salbedo + (albedo * extinction);
+#pragma unroll
+    for (int j = 0; j < MIE2_NL; j++) {
+      const int l = t + MIE2_WG * j;
+      if (l < nleg_coef && l <= nlegen) {
+        lc[0][j] = lc[0][j] + (g[0][j] * albedo * extinction);
+        lc[1][j] = lc[1][j] + (g[1][j] * albedo * extinction);
+        lc[2][j] = lc[2][j] + (g[2][j] * albedo * extinction);
+      }
+    }
+    if (kexttot < 0.0 || isnan(kexttot)) {
+      if (t == 0) set_err(st, E_KEXT);
+      return;
+    }
+    if (nleg_coef > MAXNLEG - 1) {
+      if (t == 0) set_err(st, E_NLEG);
+      return;
+    }
+    __syncthreads();  // the next hydrometeor rewrites LDS and the slot
+  }
+  // finalize_rt3_scatProperties and output
+  salbedo = salbedo / kexttot;
+  const double norm = salbedo * kexttot;
+  double *out = w.coef + (size_t)itf * NCOEF * MAXNLEG;
+#pragma unroll
+  for (int j = 0; j < MIE2_NL; j++) {
+    const int l = t + MIE2_WG * j;
+    if (l <= nleg_coef && l < MAXNLEG) {
+      out[l] = lc[0][j] / norm;
+      out[MAXNLEG + l] = lc[1][j] / norm;
+      out[2 * MAXNLEG + l] = lc[2][j] / norm;
+    }
+  }
+  if (t == 0) {
+    w.kext[itf] = kexttot;
+    w.back[itf] = backtot;
+    w.salb[itf] = salbedo;
+    w.nleg[itf] = nleg_coef;
+    atomicAdd(&w.counters[6], (unsigned long long)work_bins);
+    atomicAdd(&w.counters[7], (unsigned long long)work_series);
+    atomicAdd(&w.counters[12], (unsigned long long)work_ang);
+    atomicAdd(&w.counters[13], (unsigned long long)work_proj);
+  }
+}
+
+// Persistent grid of the coefficient-first Mie: workgroup b owns slot b of
+// the U scratch and takes units from the work queue until it is drained.
+__global__ void __launch_bounds__(MIE2_WG) __attribute__((amdgpu_waves_per_eu(2))) k_mie2(Dims d, Settings s, const pam_hydro_desc *hyd,
+                                                   const double *freqs, Work w) {
+  __shared__ Mie2Lds L;
+  double *U = w.mie_u + (size_t)blockIdx.x * w.mie_slot_doubles;
+  const int n = d.nitems * d.nf;
+  for (int itf = next_unit(&w.queue[0], &L.slot); itf < n; itf = next_unit(&w.queue[0], &L.slot))
+    mie2_unit(d, s, hyd, freqs, w, itf, U, L);
 }
 
 // One unit per workgroup, or (PERS) a persistent grid taking units from a
@@ -962,11 +1443,19 @@ __global__ void __launch_bounds__(MIE_WG) __attribute__((amdgpu_waves_per_eu(2))
   }
 }
 
+int mie_slot_rows(int nbmax) { return 16 * ((nbmax + 7) / 8); }
+size_t mie_slot_doubles(int nbmax) { return (size_t)mie_slot_rows(nbmax) * 2 * MAX_NTERMS; }
+
 void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, const Hydro4D &h4,
                 const Profile &p, const double *freqs, Work &w, hipStream_t st) {
   (void)p;
   size_t n = (size_t)d.nitems * d.nf;
   if (n == 0) return;
+  if (!s.ssrg && !s.mie_chunked && w.mie_u && w.mie_slots > 0) {
+    const unsigned grid = (unsigned)std::min<size_t>(n, (size_t)w.mie_slots);
+    hipLaunchKernelGGL(k_mie2, dim3(grid), dim3(MIE2_WG), 0, st, d, s, hyd, freqs, w);
+    return;
+  }
   const dim3 grid((unsigned)persistent_grid(n, s.pers_mie, s.ncu));
   if (s.ssrg) {
     if (grid.x < n)
