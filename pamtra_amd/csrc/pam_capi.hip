@@ -702,6 +702,17 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   {
     const char *mk = getenv("PAM_MIE");
     s.mie_chunked = (mk && strcmp(mk, "chunked") == 0) ? 1 : 0;
+    s.fault = FAULT_NONE;
+    if (const char *ft = getenv("PAM_TEST_FAULT")) {
+      char kind[16] = {0};
+      int fc = -1, ff = -1;
+      if (sscanf(ft, "%15[a-z_]:%d:%d", kind, &fc, &ff) == 3) {
+        s.fault = strcmp(kind, "nan_coef") == 0 ? FAULT_NAN_COEF
+                  : strcmp(kind, "singular") == 0 ? FAULT_SINGULAR : FAULT_NONE;
+        s.fault_col = fc;
+        s.fault_ch = ff;
+      }
+    }
   }
 
   Profile p;

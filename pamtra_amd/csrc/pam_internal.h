@@ -59,6 +59,8 @@ enum ErrCode : int32_t {
   E_SSRG_INPUT = 21 << 8,
 };
 
+enum FaultKind : int { FAULT_NONE = 0, FAULT_NAN_COEF = 1, FAULT_SINGULAR = 2 };
+
 // one call's shapes and settings, passed by value to every kernel
 struct Dims {
   int nx, ny, ncol, ML, nout, nh, nf, nbmax;
@@ -84,6 +86,12 @@ struct Settings {
   // otherwise calls without self-similar Rayleigh-Gans take the
   // coefficient-first kernel
   int mie_chunked;
+  // test mode (PAM_TEST_FAULT=<kind>:<column>:<channel>): the named (column,
+  // channel) meets the failure the reference stops on, so that its isolation
+  // can be tested; kind FAULT_NAN_COEF makes its Legendre coefficients NaN
+  // (scatcnv.f90:38-43), FAULT_SINGULAR its surface inverse singular
+  // (radmat.f90:180-189)
+  int fault, fault_col, fault_ch;
   // radar simulator, "simple" mode (active): attenuation 0 disabled,
   // 1 top-down, 2 bottom-up; polarisations 0 NN, 1 HH, 2 VV, 3 HV, 4 VH
   int active, radar_att, radar_npol, radar_pol[5];

@@ -906,6 +906,7 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
       if (g1 > sp(1.0e-7)) atomicMax(&s_nlh, t);
       if (isnan(g1) || isnan(c2) || isnan(c3)) s_nan = 1;
     }
+    if (t == 0 && s.fault == FAULT_NAN_COEF && col == s.fault_col && f == s.fault_ch) s_nan = 1;
     __syncthreads();
     if (s_nan) {
       if (t == 0) set_err(st, E_NAN_COEF);
@@ -974,19 +975,19 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
 // ---------------------------------------------------------------------
 constexpr int MIE2_WG = 128;
 constexpr int MIE2_NW = MIE2_WG / 64;
-constexpr int MIE2_TAB = 640;          // LDS stage of c_n pi_n, c_n tau_n (n x angles)
+constexpr int MIE2_TAB = 512;          // LDS stage of c_n pi_n, c_n tau_n (n x angles)
 constexpr int MIE2_NL = (MAXNLEG + MIE2_WG - 1) / MIE2_WG;  // Legendre orders per thread
 
 struct Mie2Lds {
   double ps[3][MAX_NQUAD];     // weighted phase sums per angle
   double tab[2 * MIE2_TAB];    // angle table stage
-  double wb[MIE_MAXBIN];       // per bin: n(D) dD if the bin scatters, else 0
   short nmb[MIE_MAXBIN];       // per bin: series length (0: skipped)
   double pw[MIE2_NW][6 * 16];  // per-wave phase sums of shared angle tiles
   double red[MIE2_NW][3];
   int redi[MIE2_NW][2];
   double hm[MAX_HYDRO][2];
   int hnb[MAX_HYDRO], hnt[MAX_HYDRO];
+  double lc[NCOEF][MAXNLEG + 1];  // Legendre sums over hydrometeors (thread l owns order l)
   int nlh, nan_;
   int slot;
 };
@@ -1030,11 +1031,7 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
     L.hm[t][0] = m.re;
     L.hm[t][1] = m.im;
   }
-  double lc[NCOEF][MIE2_NL];
-#pragma unroll
-  for (int k = 0; k < NCOEF; k++)
-#pragma unroll
-    for (int j = 0; j < MIE2_NL; j++) lc[k][j] = 0.0;
+  for (int k = t; k < NCOEF * (MAXNLEG + 1); k += MIE2_WG) (&L.lc[0][0])[k] = 0.0;
   __syncthreads();
   double kexttot = 0.0, salbedo = 0.0, backtot = 0.0;
   int nleg_coef = 0;
@@ -1083,7 +1080,6 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
         nmie = min(ns, nterms);
       }
       L.nmb[ir] = (short)nmie;
-      L.wb[ir] = nmie > 0 ? nd[ir] * dd[ir] : 0.0;
       if (nmie <= 0) continue;
       // soft sphere: index of the ice-air mixture at the bin's density
       // (mie_spheres.f90:187-192), decided per bin
@@ -1111,7 +1107,19 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
       double psim = cos(x), psin = sin(x), chim = -sin(x), chin = cos(x);
       double s1 = 0.0, s2 = 0.0;
       cd s3 = cd{0.0, 0.0};
-      for (int n = 1; n <= nmie; n++) {
+      // D_n is read back four terms ahead of its use (dq[] rotates), the
+      // loads issued before the stores of the terms between
+      cd dq[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        dq[q] = (q < nmie) ? cd{ure[(size_t)q * ld], uim[(size_t)q * ld]} : cd{0.0, 0.0};
+      for (int n0 = 1; n0 <= nmie; n0 += 4)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int n = n0 + q;
+        if (n > nmie) break;
+        const cd Dn = dq[q];
+        if (n + 4 <= nmie) dq[q] = cd{ure[(size_t)(n + 3) * ld], uim[(size_t)(n + 3) * ld]};
         const double fn = (double)(2 * n - 1) * xinv;
         double tmp = psin;
         psin = fn * psin - psim;
@@ -1121,7 +1129,6 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
         chim = tmp;
         const cd xin = cd{sp(psin), sp(-chin)};
         const cd xim = cd{sp(psim), sp(-chim)};
-        const cd Dn = cd{ure[(size_t)(n - 1) * ld], uim[(size_t)(n - 1) * ld]};
         const double nx = (double)n * xinv;
         cd ct = Dn * mcinv + nx;
         const cd an = cdiv_m(ct * psin - psim, ct * xin - xim);
@@ -1188,37 +1195,48 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
         const int nsk = np >> 2;
         d4m_t x0 = {0.0, 0.0, 0.0, 0.0}, y0 = x0, x1 = x0, y1 = x0;
         const double *urow = U + 16 * rt + li;
-        for (int sk = 0; sk < nsk; sk++) {
-#pragma unroll
-          for (int g = 0; g < 2; g++) {
-            int cu, n;
-            bool ap;
-            mie_kmap(g, 4 * sk + lk, np, cu, n, ap);
-            double pv = 0.0, tv = 0.0;
-            if (ang_ok && n <= nterms) {
-              if (tab_lds) {
-                pv = L.tab[(n - 1) * nh2 + hix];
-                tv = L.tab[MIE2_TAB + (n - 1) * nh2 + hix];
-              } else {
-                pv = tpi[(size_t)(n - 1) * nquad + ang];
-                tv = ttau[(size_t)(n - 1) * nquad + ang];
-              }
-            }
-            const int cg = ap ? n - 1 : nph + n - 1;
-            const double a = (n <= nrow) ? urow[(size_t)cg * ld] : 0.0;
-            if (g == 0) {
-              x0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? pv : tv, x0, 0, 0, 0);
-              y0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? tv : pv, y0, 0, 0, 0);
+        // one k-step: both parity groups, operand a from the slot
+        auto kstep = [&](int sk, int g, double a) {
+          int cu, n;
+          bool ap;
+          mie_kmap(g, 4 * sk + lk, np, cu, n, ap);
+          double pv = 0.0, tv = 0.0;
+          if (ang_ok && n <= nterms) {
+            if (tab_lds) {
+              pv = L.tab[(n - 1) * nh2 + hix];
+              tv = L.tab[MIE2_TAB + (n - 1) * nh2 + hix];
             } else {
-              x1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? pv : tv, x1, 0, 0, 0);
-              y1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? tv : pv, y1, 0, 0, 0);
+              pv = tpi[(size_t)(n - 1) * nquad + ang];
+              tv = ttau[(size_t)(n - 1) * nquad + ang];
             }
           }
+          if (g == 0) {
+            x0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? pv : tv, x0, 0, 0, 0);
+            y0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? tv : pv, y0, 0, 0, 0);
+          } else {
+            x1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? pv : tv, x1, 0, 0, 0);
+            y1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? tv : pv, y1, 0, 0, 0);
+          }
+        };
+        auto uload = [&](int sk, int g) -> double {
+          int cu, n;
+          bool ap;
+          mie_kmap(g, 4 * sk + lk, np, cu, n, ap);
+          const int cg = ap ? n - 1 : nph + n - 1;
+          return (n <= nrow) ? urow[(size_t)cg * ld] : 0.0;
+        };
+        // unrolled so that the operand loads of several k-steps are in
+        // flight together (the tables come from memory for long series)
+#pragma unroll 4
+        for (int sk = 0; sk < nsk; sk++) {
+          const double a0 = uload(sk, 0), a1 = uload(sk, 1);
+          kstep(sk, 0, a0);
+          kstep(sk, 1, a1);
         }
 #pragma unroll
         for (int h2 = 0; h2 < 2; h2++) {
           const int bw = 8 * rt + 4 * h2 + lk;
-          const double wgt = bw < nbin ? L.wb[bw] : 0.0;
+          const double wgt = (bw < nbin && L.nmb[bw] > 0) ? nd[bw] * dd[bw] : 0.0;
           const int re = 2 * h2, im = 2 * h2 + 1;
           // +mu
           double s1r = x0[re] + x1[re], s1i = x0[im] + x1[im];
@@ -1362,6 +1380,7 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
         if (isnan(g[0][j]) || isnan(c2) || isnan(c3)) L.nan_ = 1;
       }
     }
+    if (t == 0 && s.fault == FAULT_NAN_COEF && col == s.fault_col && f == s.fault_ch) L.nan_ = 1;
     __syncthreads();
     if (L.nan_) {
       if (t == 0) set_err(st, E_NAN_COEF);
@@ -1377,9 +1396,9 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
     for (int j = 0; j < MIE2_NL; j++) {
       const int l = t + MIE2_WG * j;
       if (l < nleg_coef && l <= nlegen) {
-        lc[0][j] = lc[0][j] + (g[0][j] * albedo * extinction);
-        lc[1][j] = lc[1][j] + (g[1][j] * albedo * extinction);
-        lc[2][j] = lc[2][j] + (g[2][j] * albedo * extinction);
+        L.lc[0][l] = L.lc[0][l] + (g[0][j] * albedo * extinction);
+        L.lc[1][l] = L.lc[1][l] + (g[1][j] * albedo * extinction);
+        L.lc[2][l] = L.lc[2][l] + (g[2][j] * albedo * extinction);
       }
     }
     if (kexttot < 0.0 || isnan(kexttot)) {
@@ -1400,9 +1419,9 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
   for (int j = 0; j < MIE2_NL; j++) {
     const int l = t + MIE2_WG * j;
     if (l <= nleg_coef && l < MAXNLEG) {
-      out[l] = lc[0][j] / norm;
-      out[MAXNLEG + l] = lc[1][j] / norm;
-      out[2 * MAXNLEG + l] = lc[2][j] / norm;
+      out[l] = L.lc[0][l] / norm;
+      out[MAXNLEG + l] = L.lc[1][l] / norm;
+      out[2 * MAXNLEG + l] = L.lc[2][l] / norm;
     }
   }
   if (t == 0) {
@@ -1419,7 +1438,7 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
 
 // Persistent grid of the coefficient-first Mie: workgroup b owns slot b of
 // the U scratch and takes units from the work queue until it is drained.
-__global__ void __launch_bounds__(MIE2_WG) __attribute__((amdgpu_waves_per_eu(2))) k_mie2(Dims d, Settings s, const pam_hydro_desc *hyd,
+__global__ void __launch_bounds__(MIE2_WG) __attribute__((amdgpu_waves_per_eu(3))) k_mie2(Dims d, Settings s, const pam_hydro_desc *hyd,
                                                    const double *freqs, Work w) {
   __shared__ Mie2Lds L;
   double *U = w.mie_u + (size_t)blockIdx.x * w.mie_slot_doubles;

@@ -1700,6 +1700,10 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
     return true;
   }
   surface_matrix(refl, Rf, smu, sqw, R2);
+  if (s.fault == FAULT_SINGULAR && col == s.fault_col && f == s.fault_ch) {
+    fail(E_SINGULAR | PAM_FATAL);  // test mode: the surface inverse failed
+    return true;
+  }
   if (lev_sfc >= 0) {
     if (!radiance_at_surface<GJ>(stk, R2, sky, gnd, G, X, vs, ur, dr, sinv, nprod)) {
       if (!GJ) return false;
