@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "columns·channels/s at ≤0.05 K TB error vs FP64; p50/p99 request latency"
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector)
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_HBM_GBS = 8000.0
 NN = 32
 
@@ -110,17 +111,22 @@ def serve(args, precision="fp64", tuning=None, rate=50.0):
         # persistent grids only: a graph is not replayed across requests of
         # different shapes
         shapes = {k: v for k, v in tuning.items() if not k.startswith("_") and k != "graph"}
-    with serving.ForwardServer(slots=args.serve_slots, hbm_budget_gb=64.0,
-                               tuning=shapes) as srv:
-        srv.run(reqs[:2], realtime=False)  # warm the slots' workspace and code
+    if args.serve_mode == "batching":
+        srv = serving.BatchingServer(workers=2, hbm_budget_gb=64.0, tuning=shapes)
+    else:
+        srv = serving.ForwardServer(slots=args.serve_slots, hbm_budget_gb=64.0, tuning=shapes)
+    with srv:
+        srv.run(reqs[:2], realtime=False)  # warm the workspace and code
         t0 = time.perf_counter()
         res = srv.run(reqs, realtime=True)
         wall = time.perf_counter() - t0
         peak = srv.peak_reserved
+        calls = getattr(srv, "calls", None)
     summ = serving.summarize(res, reqs, wall)
-    summ.update({"offered_req_s": rate, "slots": args.serve_slots,
+    summ.update({"offered_req_s": rate, "mode": args.serve_mode,
+                 "slots": 2 if args.serve_mode == "batching" else args.serve_slots,
                  "hbm_budget_gib": 64.0, "peak_workspace_gib": round(peak / 2 ** 30, 3),
-                 "precision": precision, "launch_tuning": shapes,
+                 "precision": precision, "launch_tuning": shapes, "device_calls": calls,
                  "mix": "1-4096 columns (log-uniform), 20-150 layers, 1-12 channels of "
                         "10.65-325 GHz, 0-4 of cloud/ice/snow/graupel with 100 bins"})
     log(f"serving: {summ}")
@@ -204,6 +210,72 @@ def large_domain(args, torch):
     return res
 
 
+def config3_leg(args, torch):
+    """The mixed-precision configuration at full size: 16384 columns, four
+    categories (cloud, ice, snow, graupel) of 100 bins, config 2's grid and 7
+    channels; FP64 and mixed, each timed, with the per-channel TB difference."""
+    from pamtra_amd import _capi, _lib, workloads
+    t0 = time.time()
+    pam, freqs = workloads.config3(ncol=args.config3_cols, seed=1003)
+    log(f"config-3 workload built in {time.time() - t0:.1f} s")
+    lib, ctx = _lib.library(), _lib.get_context(0)
+    prof = to_device(pam, torch)
+    d4 = to_device_4d(pam, torch)
+    nml = dict(pam.nmlSet)
+    nf = len(freqs)
+    f, fp = _capi.freqs_array(freqs)
+    names = lib.pam_stage_names().decode().split(",")
+    res, tbs = {}, {}
+    for precision in ("fp64", "mixed"):
+        m = _capi.marshal(nml, dict(pam.set, precision=precision), pam.df.data, d4, prof)
+        nx, ny, nout = m.nx, m.ny, m.noutlevels
+        tb = torch.zeros((nx, ny, nout, 32, nf, 2), dtype=torch.float64, device="cuda")
+        status = torch.zeros((nx, ny, nf), dtype=torch.int32, device="cuda")
+        r = _capi.PamResults()
+        r.tb = C.cast(C.c_void_p(tb.data_ptr()), C.POINTER(C.c_double))
+        r.status = C.cast(C.c_void_p(status.data_ptr()), C.POINTER(C.c_int32))
+
+        def call():
+            if lib.pam_run_device(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),
+                                  C.byref(m.profile), fp, nf, C.byref(r), None) == _capi.PAM_FATAL:
+                raise RuntimeError("pam_run_device (config 3) failed")
+
+        call()
+        torch.cuda.synchronize()
+        acc = np.zeros(len(names))
+        t0 = time.perf_counter()
+        for i in range(args.config3_steps):
+            call()
+            ms = (C.c_double * len(names))()
+            lib.pam_last_stage_times(ctx, ms, len(names))
+            acc += np.array(ms[:])
+            log(f"config-3 {precision} step {i} done")
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.config3_steps
+        tbs[precision] = tb.cpu().numpy()
+        work = (C.c_int64 * 14)()
+        lib.pam_last_work(ctx, work, 14)
+        res[precision] = {"value": round(nx * ny * nf / dt, 2), "ms_per_step": round(dt * 1e3, 3),
+                          "stage_ms": {n: round(v / args.config3_steps, 3)
+                                       for n, v in zip(names, acc)},
+                          "status_nonzero": int((status.cpu().numpy() != 0).sum())}
+        if precision == "mixed":
+            # FP32 matrix-core rate of the mixed doubling: 14 N^3 flops a step
+            dbl = res[precision]["stage_ms"]["doubling"]
+            ach = work[3] * 14 * NN ** 3 / (dbl * 1e-3) / 1e12
+            res[precision]["doubling_fp32_roofline"] = {
+                "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP32_TFLOPS, 4)}
+    dch = np.max(np.abs(tbs["mixed"] - tbs["fp64"]), axis=(0, 1, 2, 3, 5))
+    res["mixed"]["max_abs_dtb_vs_fp64_K"] = {f"{fr:g}": float(v) for fr, v in zip(freqs, dch)}
+    res["mixed"]["speedup_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
+    res["workload"] = (f"{args.config3_cols} columns, 20-120 layers, 7 channels 22.235-183.31 "
+                       "GHz, cloud/ice/snow/graupel Mie 100 bins each (snow soft spheres by "
+                       "column), NSTOKES=2, 16 angles")
+    log(f"config 3: {res}")
+    return res
+
+
 def aggregates(args):
     """Cost of the self-similar Rayleigh-Gans producer: the four-category
     columns (config 3 shapes) with snow and graupel as Mie soft spheres and
@@ -253,8 +325,11 @@ def main():
                     help="skip the pam_run (host arrays, PCIe-inclusive) measurement")
     ap.add_argument("--serve-requests", type=int, default=100,
                     help="requests of the serving mix (0: no serving run)")
-    ap.add_argument("--serve-rate", type=str, default="10,50",
+    ap.add_argument("--serve-rate", type=str, default="50,100,200,500,1000,2000",
                     help="offered requests/s (Poisson), comma-separated: one serving run each")
+    ap.add_argument("--serve-mode", choices=("batching", "slots"), default="batching",
+                    help="batching: waiting requests share calls per channel; slots: one "
+                         "call per request on concurrent contexts")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving mix and print its summary")
@@ -270,6 +345,10 @@ def main():
     ap.add_argument("--large-cols", type=int, default=2048,
                     help="columns of the large-domain (32-angle) sample (0: skip it)")
     ap.add_argument("--large-steps", type=int, default=2)
+    ap.add_argument("--config3-cols", type=int, default=16384,
+                    help="columns of the four-category mixed-precision configuration "
+                         "(0: skip it)")
+    ap.add_argument("--config3-steps", type=int, default=2)
     ap.add_argument("--aggregates-cols", type=int, default=0,
                     help="also time the SSRGA producer on this many four-category columns")
     ap.add_argument("--aggregates-only", action="store_true",
@@ -416,6 +495,21 @@ def main():
                 "mfma_executed_tflops": (round(executed[dom_name] / (stage_ms[dom] * 1e-3) / 1e12, 3)
                                          if dom_name in executed else None)}
 
+    # The Mie stage against the FP64 peak, counted on the reference's
+    # algorithm (mie_scat_utilities.f90:8-160, mie_spheres.f90:178-314): per
+    # bin the D_n recursion from nt+15 (20 flops a step), 76 flops per term
+    # for a_n, b_n and the efficiencies, 28 per term and angle for S1/S2 and
+    # the phase sums; per (layer, hydrometeor) 13 per angle and order for the
+    # Legendre projection.  Counters: [6] bins, [7] sum of series lengths,
+    # [12] sum of nquad * series length, [13] sum of nquad * (nlegen + 1).
+    f_mie = 20 * (work[7] + 15 * work[6]) + 76 * work[7] + 28 * work[12] + 13 * work[13]
+    mie_ms = stage_ms[names.index("mie")]
+    ach_mie = f_mie / (mie_ms * 1e-3) / 1e12
+    mie_roof = {"bound": "fp64", "kernel": "mie", "achieved": round(ach_mie, 3),
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach_mie / PEAK_FP64_TFLOPS, 4), "flops": f_mie,
+                "bins": work[6], "series_terms": work[7]}
+
     tb_h = tb.cpu().numpy()
     st_h = status.cpu().numpy()
 
@@ -501,6 +595,7 @@ def main():
         "mixed_precision": mixed,
         "launch_tuning": tuning,
         "roofline": roof,
+        "mie_roofline": mie_roof,
     }
 
     if args.serve_requests > 0:
@@ -510,6 +605,8 @@ def main():
             out["serving"]["mixed"] = [serve(args, "mixed", tuning=tuning and tuning.get("mixed"),
                                              rate=r) for r in rates]
 
+    if args.config3_cols > 0:
+        out["config3"] = config3_leg(args, torch)
     if args.large_cols > 0:
         out["large_domain"] = large_domain(args, torch)
     if args.aggregates_cols > 0:
@@ -518,7 +615,11 @@ def main():
     if not args.no_cpu:
         import oracle
         n = min(args.cpu_cols, nx)
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # every host core this process may use, at most the 16 a one-GPU box
+        # grants (its nproc can show more)
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (
+            os.cpu_count() or 1)
+        threads = args.cpu_threads or min(16, avail)
         sub = slice_columns(pam, n)
         t0 = time.perf_counter()
         ref, st, _ = oracle.run(nml, pam.set, pam.df.data, pam.df.data4D, sub, freqs,
@@ -527,7 +628,8 @@ def main():
         ok = np.isfinite(ref["tb"])
         dmax = float(np.max(np.abs(tb_h[:n][ok] - ref["tb"][ok])))
         out["cpu_baseline"] = {"value": round(n * nf / tc, 3), "unit": "columns·channels/s",
-                               "cores": threads, "kind": "port",
+                               "cores": threads, "nproc": os.cpu_count(),
+                               "cores_available": avail, "kind": "port",
                                "sample": f"first {n} of the {nx} columns x {nf} channels, "
                                          f"{tc:.1f} s"}
         out["max_abs_dtb_vs_oracle_K"] = dmax

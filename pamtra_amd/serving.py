@@ -214,7 +214,7 @@ class ForwardServer:
         `realtime`, else all at once in order) and returns their results in
         request order."""
         for r in requests:
-            if len(r.pam.df.dataFullSpec):
+            if len(getattr(getattr(r.pam, "df", None), "dataFullSpec", ())):
                 # the workspace bound counts layers from hydro_q, which the
                 # full-spectrum input does not use
                 raise ValueError(f"request {r.rid}: full-spectrum input (hydro_fullspec) is "
@@ -301,3 +301,257 @@ def summarize(results: Sequence[Result], requests: Sequence[Request], wall: floa
             "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "failed": int(sum(1 for r in results if r.status >= _capi.PAM_FATAL))}
+
+
+# ---------------------------------------------------------------------------
+# Continuous batching
+# ---------------------------------------------------------------------------
+
+LEVEL_KEYS = ("hgt_lev", "temp_lev", "press_lev", "relhum_lev")
+LAYER_KEYS = ("hgt", "temp", "press", "relhum", "hydro_q", "hydro_n", "hydro_reff")
+COLUMN_KEYS = ("nlyrs", "groundtemp", "wind10u", "wind10v", "sfc_salinity", "sfc_emissivity",
+               "sfc_type", "sfc_model", "sfc_refl")
+MISSING = -9999.0
+
+
+def _group_key(pam) -> tuple:
+    """Requests whose columns may share one call: the same descriptor table,
+    4D descriptor fields, settings and output levels."""
+    sets = {k: v for k, v in pam.set.items() if not k.startswith("_") or k == "_nangles"}
+    nml = dict(pam.nmlSet)
+    return (pam.df.data.tobytes(), tuple(sorted(pam.df.data4D)), repr(sorted(sets.items())),
+            repr(sorted((k, str(v)) for k, v in nml.items())), int(pam.p["noutlevels"]),
+            int(getattr(pam, "_nangles", 16)))
+
+
+def merge_columns(pams: Sequence[object]) -> dict:
+    """One profile dict (and 4D descriptor dict) holding the columns of
+    several requests, in order, along x.  Layers are padded to the largest
+    max_nlyrs with the missing value; each column keeps its own nlyrs, so the
+    padding is never read as atmosphere."""
+    ML = max(int(p.p["max_nlyrs"]) for p in pams)
+    nout = int(pams[0].p["noutlevels"])
+    nh = len(pams[0].df.data)
+    parts: dict = {}
+    parts4: dict = {}
+
+    def flat(a, ncol):
+        a = np.asarray(a)
+        return a.reshape((ncol,) + a.shape[2:])
+
+    for pam in pams:
+        p = pam.p
+        ncol = int(p["ngridx"]) * int(p["ngridy"])
+        L = int(p["max_nlyrs"])
+        for key in LEVEL_KEYS + LAYER_KEYS:
+            if key not in p:
+                continue
+            a = flat(p[key], ncol).astype(np.float64)
+            want = ML + 1 if key in LEVEL_KEYS else ML
+            if a.shape[1] < want:
+                pad = np.full((ncol, want - a.shape[1]) + a.shape[2:], MISSING)
+                a = np.concatenate([a, pad], axis=1)
+            parts.setdefault(key, []).append(a)
+        for key in COLUMN_KEYS:
+            if key in p:
+                parts.setdefault(key, []).append(flat(p[key], ncol))
+        if "obs_height" in p:
+            parts.setdefault("obs_height", []).append(flat(p["obs_height"], ncol).astype(np.float64))
+        for key, a in pam.df.data4D.items():
+            a = flat(a, ncol)
+            if a.shape[1] < ML:
+                fill = 0 if key == "nbin" else MISSING
+                a = np.concatenate([a, np.full((ncol, ML - L) + a.shape[2:], fill,
+                                               dtype=a.dtype)], axis=1)
+            parts4.setdefault(key, []).append(a)
+    ntot = sum(x.shape[0] for x in parts["nlyrs"])
+    out = {k: np.concatenate(v, axis=0).reshape((ntot, 1) + v[0].shape[1:])
+           for k, v in parts.items()}
+    out.update({"ngridx": ntot, "ngridy": 1, "max_nlyrs": ML, "noutlevels": nout, "nhydro": nh})
+    d4 = {k: np.concatenate(v, axis=0).reshape((ntot, 1) + v[0].shape[1:])
+          for k, v in parts4.items()}
+    return out, d4
+
+
+class BatchingServer:
+    """Continuous batching of forward-operator requests on one device.
+
+    Requests that are waiting when the device frees up are packed into
+    shared calls instead of running one call each: requests with the same
+    descriptor table, settings and output levels (``_group_key``) form a
+    group; for each channel of the group one call runs over the columns of
+    every request that asked for it (columns concatenated along x, layers
+    padded), so small requests fill the launches of the others.  A request
+    is done when the calls of all its channels are.  ``workers`` contexts
+    (each a host thread and stream) alternate, so one batch is assembled on
+    the host while the previous one runs.
+
+    Each (column, channel) is still computed on its own by the same kernels,
+    so a request's result is bitwise that of running it alone.
+    """
+
+    def __init__(self, device: int = 0, workers: int = 2, hbm_budget_gb: float = 64.0,
+                 max_batch_columns: int = 65536, tuning: Optional[dict] = None):
+        if workers < 1:
+            raise ValueError("workers must be >= 1")
+        self.workers = workers
+        self.budget = int(hbm_budget_gb * (1 << 30))
+        self.max_batch_columns = max_batch_columns
+        self._lib = _lib.library()
+        if self._lib.pam_device_count() <= device:
+            raise _lib.PamLibraryError(f"no HIP device {device}")
+        self._ctx: List[C.c_void_p] = []
+        for _ in range(workers):
+            h = C.c_void_p()
+            if self._lib.pam_context_create(C.byref(h), device) != 0 or not h.value:
+                raise _lib.PamLibraryError("pam_context_create failed")
+            # each worker's optics in chunks of at most its share of the budget
+            self._lib.pam_context_set_workspace_limit(h, int(0.6 * self.budget / workers))
+            if tuning:
+                from . import autotune
+                autotune.apply(self._lib, h, tuning)
+            self._ctx.append(h)
+        self.calls = 0
+        self.peak_reserved = 0
+
+    def close(self):
+        for h in self._ctx:
+            self._lib.pam_context_destroy(h)
+        self._ctx = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _run_batch(self, slot: int, reqs: Sequence[Request]):
+        """All calls of one batch on worker `slot`; returns per request
+        (status, message, tb)."""
+        out = {}
+        tbs = {}
+        status = {}
+        msgs = {}
+        for r in reqs:
+            p = r.pam.p
+            nx, ny, nout = int(p["ngridx"]), int(p["ngridy"]), int(p["noutlevels"])
+            nang = 2 * int(getattr(r.pam, "_nangles", 16))
+            tbs[r.rid] = np.full((nx, ny, nout, nang, len(r.freqs), 2), np.nan)
+            status[r.rid] = _capi.PAM_SUCCESS
+            msgs[r.rid] = ""
+        groups: dict = {}
+        for r in reqs:
+            groups.setdefault(_group_key(r.pam), []).append(r)
+        lib, ctx = self._lib, self._ctx[slot]
+        for members in groups.values():
+            chans = sorted({float(f) for r in members for f in r.freqs})
+            for ch in chans:
+                use = [r for r in members if ch in [float(f) for f in r.freqs]]
+                prof, d4 = merge_columns([r.pam for r in use])
+                first = use[0].pam
+                nml = dict(first.nmlSet)
+                nml["active"] = False
+                m = _capi.marshal(nml, first.set, first.df.data, d4, prof)
+                f, fp = _capi.freqs_array([ch])
+                hr = _capi.alloc_results(m, 1)
+                st = lib.pam_run(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),
+                                 C.byref(m.profile), fp, 1, C.byref(hr.struct))
+                self.calls += 1
+                self.peak_reserved = max(self.peak_reserved,
+                                         sum(int(lib.pam_context_bytes(h)) for h in self._ctx))
+                tb = hr.arrays["tb"]
+                sts = hr.arrays.get("status")
+                msg = _capi.results_message(hr)
+                o = 0
+                for r in use:
+                    p = r.pam.p
+                    nx, ny = int(p["ngridx"]), int(p["ngridy"])
+                    n = nx * ny
+                    k = [float(v) for v in r.freqs].index(ch)
+                    tbs[r.rid][..., k, :] = tb[o:o + n, 0, ..., 0, :].reshape(
+                        tbs[r.rid][..., k, :].shape)
+                    rs = int(st) if sts is None else int(np.max(sts[o:o + n] & 0xff))
+                    if rs > status[r.rid]:
+                        status[r.rid], msgs[r.rid] = rs, msg
+                    o += n
+        for r in reqs:
+            out[r.rid] = (status[r.rid], msgs[r.rid], tbs[r.rid])
+        return out
+
+    def run(self, requests: Sequence[Request], realtime: bool = True) -> List[Result]:
+        """Serves `requests` (arrival times in seconds from now when
+        `realtime`, else all at once) and returns their results in request
+        order."""
+        for r in requests:
+            if len(getattr(getattr(r.pam, "df", None), "dataFullSpec", ())):
+                raise ValueError(f"request {r.rid}: full-spectrum input (hydro_fullspec) is "
+                                 "not served; call runPamtra for it")
+        order = sorted(range(len(requests)), key=lambda i: (requests[i].arrival, i))
+        by_rid = {requests[i].rid: i for i in range(len(requests))}
+        if len(by_rid) != len(requests):
+            raise ValueError("request ids must be unique")
+        done: "queue.Queue" = queue.Queue()
+        jobs = [queue.Queue() for _ in range(self.workers)]
+        t0 = time.perf_counter()
+        now = (lambda: time.perf_counter() - t0) if realtime else (lambda: 0.0)
+
+        def worker(slot):
+            while True:
+                item = jobs[slot].get()
+                if item is None:
+                    return
+                batch, t_start = item
+                try:
+                    res = self._run_batch(slot, [requests[i] for i in batch])
+                except Exception as e:  # surfaced per request, the loop goes on
+                    res = {requests[i].rid: (_capi.PAM_FATAL, f"{type(e).__name__}: {e}", None)
+                           for i in batch}
+                done.put((batch, slot, res, t_start, now()))
+
+        threads = [threading.Thread(target=worker, args=(s,), daemon=True)
+                   for s in range(self.workers)]
+        for th in threads:
+            th.start()
+        free = list(range(self.workers))
+        results: List[Optional[Result]] = [None] * len(requests)
+        nxt = 0
+        running = 0
+        try:
+            while nxt < len(order) or running:
+                # a free worker takes every request that has arrived, up to
+                # the batch's column limit (at least one)
+                while free and nxt < len(order) and (not realtime or
+                                                     requests[order[nxt]].arrival <= now()):
+                    batch, cols = [], 0
+                    while nxt < len(order) and (not realtime or
+                                                requests[order[nxt]].arrival <= now()):
+                        i = order[nxt]
+                        c = requests[i].columns
+                        if batch and cols + c > self.max_batch_columns:
+                            break
+                        batch.append(i)
+                        cols += c
+                        nxt += 1
+                    slot = free.pop(0)
+                    jobs[slot].put((batch, now()))
+                    running += 1
+                timeout = None
+                if nxt < len(order) and free and realtime:
+                    timeout = max(0.0, requests[order[nxt]].arrival - now())
+                try:
+                    batch, slot, res, t_start, t_end = done.get(timeout=timeout)
+                except queue.Empty:
+                    continue
+                for i in batch:
+                    r = requests[i]
+                    st, msg, tb = res[r.rid]
+                    results[i] = Result(r.rid, st, msg, tb, r.arrival if realtime else 0.0,
+                                        t_start, t_end, slot, 0)
+                running -= 1
+                free.append(slot)
+        finally:
+            for q in jobs:
+                q.put(None)
+            for th in threads:
+                th.join()
+        return results  # type: ignore[return-value]

@@ -101,3 +101,35 @@ def test_a_failing_call_is_reported_and_the_loop_goes_on():
     res = srv.run(_requests(4), realtime=False)
     assert res[1].status == 2 and "boom" in res[1].message
     assert all(res[i].status == 0 for i in (0, 2, 3))
+
+
+def test_merged_columns_give_each_request_its_own_result():
+    """merge_columns: requests of different layer counts concatenated along x
+    and padded give, through the oracle, exactly each request's own TB."""
+    import oracle
+    from pamtra_amd import workloads
+    mix = workloads.serving_requests(3, seed=31, max_cols=3)
+    pams = [p for p, _ in mix]
+    assert len({int(p.p["max_nlyrs"]) for p in pams}) > 1
+    ch = [36.5]
+    prof, d4 = serving.merge_columns(pams)
+    assert prof["ngridx"] == sum(int(p.p["ngridx"]) for p in pams)
+    nml = dict(pams[0].nmlSet)
+    nml["active"] = False
+    merged, st, _ = oracle.run(nml, pams[0].set, pams[0].df.data, d4, prof, ch, nthreads=4)
+    assert st == 0
+    o = 0
+    for p in pams:
+        alone, st2, _ = oracle.run(nml, p.set, p.df.data, p.df.data4D, p.p, ch, nthreads=4)
+        assert st2 == 0
+        n = int(p.p["ngridx"])
+        np.testing.assert_array_equal(merged["tb"][o:o + n], alone["tb"])
+        o += n
+
+
+def test_group_key_separates_settings_and_descriptors():
+    from pamtra_amd import workloads
+    (a, _), (b, _) = workloads.serving_requests(2, seed=32, max_cols=2)
+    assert serving._group_key(a) == serving._group_key(b)
+    b.set["precision"] = "mixed"
+    assert serving._group_key(a) != serving._group_key(b)

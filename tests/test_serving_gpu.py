@@ -58,3 +58,24 @@ def test_tight_budget_serialises_but_completes():
         res = srv.run(reqs, realtime=False)
         assert srv.peak_reserved <= srv.budget
     assert all(r.status == 0 for r in res)
+
+
+def test_batching_server_packs_requests_and_keeps_each_result():
+    """Continuous batching: the requests waiting together share one call per
+    channel (fewer calls than request-channels), and every request's TB is
+    bitwise its result alone."""
+    reqs = _reqs(10, 40, seed=2025)
+    alone = []
+    with serving.ForwardServer(slots=1, hbm_budget_gb=8.0) as srv:
+        for r in reqs:
+            (res,) = srv.run([r], realtime=False)
+            assert res.status == 0, res.message
+            alone.append(res.tb)
+    with serving.BatchingServer(workers=2, hbm_budget_gb=8.0) as bs:
+        batched = bs.run(reqs, realtime=False)
+        calls = bs.calls
+    print(f"{calls} calls for {sum(len(r.freqs) for r in reqs)} request-channels", flush=True)
+    assert calls < sum(len(r.freqs) for r in reqs)
+    for a, m in zip(alone, batched):
+        assert m.status == 0, m.message
+        np.testing.assert_array_equal(a, m.tb)
