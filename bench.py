@@ -92,6 +92,9 @@ def measured_traffic(kernel):
     return None
 
 
+_BATCHED_RATE = {}  # precision -> col*ch/s of the serving mix served all at once
+
+
 def serve(args, precision="fp64", tuning=None, rate=50.0):
     """The serving mix at an offered Poisson rate: throughput and p50/p99
     latency (arrival to result on the host, input staging included)."""
@@ -122,7 +125,15 @@ def serve(args, precision="fp64", tuning=None, rate=50.0):
         wall = time.perf_counter() - t0
         peak = srv.peak_reserved
         calls = getattr(srv, "calls", None)
+        work = sum(r.columns * len(r.freqs) for r in reqs)
+        if precision not in _BATCHED_RATE:
+            # the same requests all waiting at once: the batched rate of this mix
+            t0 = time.perf_counter()
+            srv.run(reqs, realtime=False)
+            _BATCHED_RATE[precision] = work / (time.perf_counter() - t0)
     summ = serving.summarize(res, reqs, wall)
+    summ["batched_rate_same_mix"] = round(_BATCHED_RATE[precision], 2)
+    summ["fraction_of_batched"] = round(summ["throughput"] / _BATCHED_RATE[precision], 3)
     summ.update({"offered_req_s": rate, "mode": args.serve_mode,
                  "slots": 2 if args.serve_mode == "batching" else args.serve_slots,
                  "hbm_budget_gib": 64.0, "peak_workspace_gib": round(peak / 2 ** 30, 3),

@@ -382,7 +382,7 @@ class BatchingServer:
     group; for each channel of the group one call runs over the columns of
     every request that asked for it (columns concatenated along x, layers
     padded), so small requests fill the launches of the others.  A request
-    is done when the calls of all its channels are.  ``workers`` contexts
+    is returned as soon as the calls of all its channels are done.  ``workers`` contexts
     (each a host thread and stream) alternate, so one batch is assembled on
     the host while the previous one runs.
 
@@ -391,7 +391,7 @@ class BatchingServer:
     """
 
     def __init__(self, device: int = 0, workers: int = 2, hbm_budget_gb: float = 64.0,
-                 max_batch_columns: int = 65536, tuning: Optional[dict] = None):
+                 max_batch_columns: int = 16384, tuning: Optional[dict] = None):
         if workers < 1:
             raise ValueError("workers must be >= 1")
         self.workers = workers
@@ -425,10 +425,12 @@ class BatchingServer:
     def __exit__(self, *exc):
         self.close()
 
-    def _run_batch(self, slot: int, reqs: Sequence[Request]):
+    def _run_batch(self, slot: int, reqs: Sequence[Request], finished=None):
         """All calls of one batch on worker `slot`; returns per request
-        (status, message, tb)."""
+        (status, message, tb).  `finished(rid, status, message, tb)` is
+        called as soon as the last channel of a request is done."""
         out = {}
+        left = {r.rid: len(r.freqs) for r in reqs}
         tbs = {}
         status = {}
         msgs = {}
@@ -474,6 +476,9 @@ class BatchingServer:
                     if rs > status[r.rid]:
                         status[r.rid], msgs[r.rid] = rs, msg
                     o += n
+                    left[r.rid] -= 1
+                    if left[r.rid] == 0 and finished is not None:
+                        finished(r.rid, status[r.rid], msgs[r.rid], tbs[r.rid])
         for r in reqs:
             out[r.rid] = (status[r.rid], msgs[r.rid], tbs[r.rid])
         return out
@@ -501,12 +506,21 @@ class BatchingServer:
                 if item is None:
                     return
                 batch, t_start = item
+                reported = set()
+
+                def finished(rid, st, msg, tb):
+                    reported.add(rid)
+                    done.put(("req", by_rid[rid], slot, (st, msg, tb), t_start, now()))
+
                 try:
-                    res = self._run_batch(slot, [requests[i] for i in batch])
+                    self._run_batch(slot, [requests[i] for i in batch], finished)
                 except Exception as e:  # surfaced per request, the loop goes on
-                    res = {requests[i].rid: (_capi.PAM_FATAL, f"{type(e).__name__}: {e}", None)
-                           for i in batch}
-                done.put((batch, slot, res, t_start, now()))
+                    for i in batch:
+                        if requests[i].rid not in reported:
+                            done.put(("req", i, slot, (_capi.PAM_FATAL,
+                                                       f"{type(e).__name__}: {e}", None),
+                                      t_start, now()))
+                done.put(("batch", None, slot, None, t_start, now()))
 
         threads = [threading.Thread(target=worker, args=(s,), daemon=True)
                    for s in range(self.workers)]
@@ -539,16 +553,17 @@ class BatchingServer:
                 if nxt < len(order) and free and realtime:
                     timeout = max(0.0, requests[order[nxt]].arrival - now())
                 try:
-                    batch, slot, res, t_start, t_end = done.get(timeout=timeout)
+                    kind, i, slot, res, t_start, t_end = done.get(timeout=timeout)
                 except queue.Empty:
                     continue
-                for i in batch:
+                if kind == "req":
                     r = requests[i]
-                    st, msg, tb = res[r.rid]
+                    st, msg, tb = res
                     results[i] = Result(r.rid, st, msg, tb, r.arrival if realtime else 0.0,
                                         t_start, t_end, slot, 0)
-                running -= 1
-                free.append(slot)
+                else:
+                    running -= 1
+                    free.append(slot)
         finally:
             for q in jobs:
                 q.put(None)

@@ -74,7 +74,8 @@ def test_golden_vs_oracle_with_diagnostics():
 
 
 def test_config1_columns_vs_oracle():
-    pam, freqs = workloads.config1(ncol=24, seed=1001)
+    """The whole small configuration: 64 columns."""
+    pam, freqs = workloads.config1(ncol=64, seed=1001)
     g, st = _gpu(pam, freqs)
     c, st2 = _cpu(pam, freqs)
     assert st == 0 and st2 == 0, (g["msg"], c["msg"])
@@ -82,12 +83,13 @@ def test_config1_columns_vs_oracle():
 
 
 def test_config2_columns_vs_oracle():
-    """Uneven layer counts, 7 channels, ocean (TESSEM2) and Lambertian land."""
-    pam, freqs = workloads.config2(ncol=24, seed=1002)
+    """Uneven layer counts, 7 channels, ocean (TESSEM2) and Lambertian land:
+    the first 256 columns of the batched configuration."""
+    pam, freqs = workloads.config2(ncol=256, seed=1002)
     g, st = _gpu(pam, freqs)
     c, st2 = _cpu(pam, freqs)
     assert st == 0 and st2 == 0, (g["msg"], c["msg"])
-    np.testing.assert_allclose(g["emissivity"], c["emissivity"], rtol=1e-12)
+    np.testing.assert_allclose(g["emissivity"], c["emissivity"], rtol=1e-12, atol=1e-15)
     _compare(g, c)
 
 
@@ -389,5 +391,30 @@ def test_fastem_ocean_vs_oracle():
     g, st = _gpu(pam, freqs)
     c, st2 = _cpu(pam, freqs)
     assert st == 0 and st2 == 0, (g["msg"], c["msg"])
-    np.testing.assert_allclose(g["emissivity"], c["emissivity"], rtol=1e-12)
+    np.testing.assert_allclose(g["emissivity"], c["emissivity"], rtol=1e-12, atol=1e-15)
     _compare(g, c)
+
+
+def test_iq_output_vs_oracle():
+    """outpol = 'IQ' (rt_utilities.f90:687-737): I and Q instead of V and H."""
+    pam, freqs = workloads.config2(ncol=16, seed=1010)
+    pam.nmlSet["outpol"] = "IQ"
+    g, st = _gpu(pam, freqs)
+    c, st2 = _cpu(pam, freqs)
+    assert st == st2 and st in (0, 1), (g["msg"], c["msg"])
+    _compare(g, c)
+    pam.nmlSet["outpol"] = "VH"
+    v, _ = _gpu(pam, freqs)
+    # the same radiances: B(V) + B(H) = 2 B(I) and B(V) - B(H) = 2 B(Q), with
+    # B the Planck radiance of CONVERT_OUTPUT (sign kept for negative Q)
+    wl = 299792458.0 / (np.asarray(freqs) * 1e3)
+    wl = wl[None, None, None, None, :]
+
+    def planck(t):
+        a = np.abs(t)
+        return np.sign(t) * 1.1911e8 / wl ** 5 / (np.exp(1.4388e4 / (wl * a)) - 1.0)
+
+    bi, bq = planck(g["tb"][..., 0]), planck(g["tb"][..., 1])
+    bv, bh = planck(v["tb"][..., 0]), planck(v["tb"][..., 1])
+    np.testing.assert_allclose(bv + bh, 2.0 * bi, rtol=1e-9)
+    np.testing.assert_allclose(bv - bh, 2.0 * bq, rtol=1e-6, atol=1e-9 * np.max(np.abs(bi)))
