@@ -975,7 +975,7 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
 // ---------------------------------------------------------------------
 constexpr int MIE2_WG = 128;
 constexpr int MIE2_NW = MIE2_WG / 64;
-constexpr int MIE2_TAB = 512;          // LDS stage of c_n pi_n, c_n tau_n (n x angles)
+constexpr int MIE2_TAB = 768;          // LDS stage of c_n pi_n, c_n tau_n (n x angles)
 constexpr int MIE2_NL = (MAXNLEG + MIE2_WG - 1) / MIE2_WG;  // Legendre orders per thread
 
 struct Mie2Lds {
@@ -1061,13 +1061,26 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
     const int nrt = (nbin + 7) >> 3;
     for (int k = t; k < 3 * MAX_NQUAD; k += MIE2_WG) (&L.ps[0][0])[k] = 0.0;
     // c_n pi_n, c_n tau_n at the angles mu >= 0 into LDS when they fit
+    // the whole slice at mu >= 0 when it fits; else, for series up to
+    // MIE2_TAB / 16 terms, one 16-angle tile of it per pass of B
     const bool tab_lds = nterms * nh2 <= MIE2_TAB;
+    const bool tab_tile = !tab_lds && nterms * 16 <= MIE2_TAB;
+    auto stage_tile = [&](int ct) {
+      for (int e = t; e < nterms * 16; e += MIE2_WG) {
+        const int n = e >> 4, hix = 16 * ct + (e & 15);
+        const bool ok = hix < nh2;
+        L.tab[e] = ok ? tpi[(size_t)n * nquad + j0 + hix] : 0.0;
+        L.tab[MIE2_TAB + e] = ok ? ttau[(size_t)n * nquad + j0 + hix] : 0.0;
+      }
+    };
     if (tab_lds)
       for (int e = t; e < nterms * nh2; e += MIE2_WG) {
         const int n = e / nh2, h = e - n * nh2;
         L.tab[e] = tpi[(size_t)n * nquad + j0 + h];
         L.tab[MIE2_TAB + e] = ttau[(size_t)n * nquad + j0 + h];
       }
+    else if (tab_tile)
+      stage_tile(0);
 
     // --- A: one thread per bin, the whole series (mie_scat_utilities.f90:8-116)
     double q0 = 0.0, q1 = 0.0, q2 = 0.0;
@@ -1173,10 +1186,19 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
     // parity-split over mu >= 0 as in the chunked kernel.  Waves own angle
     // tiles; with fewer tiles than waves, wpc waves share a tile, each over
     // every wpc-th row tile.  A row tile's own longest series sets its
-    // number of k-steps.
-    const int wpc = nct >= MIE2_NW ? 1 : MIE2_NW / nct;
-    for (int ctw = (wpc > 1 ? wv / wpc : wv); ctw < nct; ctw += (wpc > 1 ? nct : MIE2_NW)) {
-      const int sub = wpc > 1 ? wv % wpc : 0;
+    // number of k-steps.  With the table staged one angle tile at a time,
+    // the waves take the tiles together, one pass each.
+    const int npass = tab_tile ? nct : 1;
+    for (int pass = 0; pass < npass; pass++) {
+    if (pass > 0) {
+      stage_tile(pass);
+      __syncthreads();
+    }
+    const int wpc = tab_tile ? MIE2_NW : (nct >= MIE2_NW ? 1 : MIE2_NW / nct);
+    const int ct0 = tab_tile ? pass : (wpc > 1 ? wv / wpc : wv);
+    const int ct1 = tab_tile ? pass + 1 : nct;
+    for (int ctw = ct0; ctw < ct1; ctw += (wpc > 1 ? nct : MIE2_NW)) {
+      const int sub = tab_tile ? wv : (wpc > 1 ? wv % wpc : 0);
       const int hix = 16 * ctw + li;
       const bool ang_ok = hix < nh2;
       const int ang = j0 + hix, angm = nquad - 1 - ang;
@@ -1205,6 +1227,9 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
             if (tab_lds) {
               pv = L.tab[(n - 1) * nh2 + hix];
               tv = L.tab[MIE2_TAB + (n - 1) * nh2 + hix];
+            } else if (tab_tile) {
+              pv = L.tab[(n - 1) * 16 + li];
+              tv = L.tab[MIE2_TAB + (n - 1) * 16 + li];
             } else {
               pv = tpi[(size_t)(n - 1) * nquad + ang];
               tv = ttau[(size_t)(n - 1) * nquad + ang];
@@ -1292,13 +1317,16 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
       }
     }
     __syncthreads();
-    if (wpc > 1 && t < 16 * nct) {  // the shared tiles, waves in order
-      const int ctw = t >> 4, l16 = t & 15, hix = t;
+    // the shared tiles, waves in order (with the table staged by tile,
+    // the one tile of this pass)
+    if (wpc > 1 && t < (tab_tile ? 16 : 16 * nct)) {
+      const int ctw = tab_tile ? pass : t >> 4, l16 = t & 15;
+      const int hix = tab_tile ? 16 * pass + t : t;
       if (hix < nh2) {
         const int ang = j0 + hix, angm = nquad - 1 - ang;
         double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int q = 0; q < wpc; q++) {
-          const double *pw = L.pw[ctw * wpc + q];
+          const double *pw = L.pw[tab_tile ? q : ctw * wpc + q];
 #pragma unroll
           for (int k = 0; k < 6; k++) v[k] += pw[16 * k + l16];
         }
@@ -1312,6 +1340,7 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
         }
       }
     }
+    }  // pass
     // efficiency sums, waves in order
     double sumqe = 0.0, sumqs = 0.0, sumqb = 0.0;
     for (int q = 0; q < MIE2_NW; q++) {
