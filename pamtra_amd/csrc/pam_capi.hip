@@ -100,10 +100,11 @@ struct DevBuf {
 };
 
 // Mie angle tables of one device, shared by all its contexts and kept until
-// the process ends (66 MiB).
+// the process ends (66 MiB, and their 33 MiB FP32 copy).
 struct AngleTables {
-  DevBuf buf, pbuf;
+  DevBuf buf, pbuf, fbuf;
   const double *tab = nullptr;
+  const float *tab_f = nullptr;
   const int64_t *off = nullptr;
   int nmax = 0;
   const double *ptab = nullptr;  // phase-matrix tables (about 200 MiB)
@@ -128,11 +129,14 @@ static AngleTables *device_tables(int device) {
     double *tab = (double *)t->buf.p;
     int64_t *doff = (int64_t *)((char *)t->buf.p + tab_bytes);
     ok = hipMemcpy(doff, off, sizeof(int64_t) * (nmax + 1), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) ok = t->fbuf.ensure((size_t)n * 4) == hipSuccess;
     if (ok) {
       build_angle_tables(tab, doff, nmax, nullptr);
+      round_tables_f32(tab, (float *)t->fbuf.p, n, nullptr);
       ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
     }
     t->tab = tab;
+    t->tab_f = (const float *)t->fbuf.p;
     t->off = doff;
     t->nmax = nmax;
   }
@@ -160,6 +164,7 @@ static AngleTables *device_tables(int device) {
   if (!ok) {
     t->buf.release();
     t->pbuf.release();
+    t->fbuf.release();
     delete t;
     return nullptr;
   }
@@ -174,6 +179,7 @@ struct pam_context {
   DevBuf rtbuf;  // global scratch of the adding kernels at N = 64
   DevBuf miebuf; // U slots of the coefficient-first Mie kernel
   const double *d_atab = nullptr;
+  const float *d_atab_f = nullptr;
   const int64_t *d_atab_off = nullptr;
   int atab_nmax = 0;
   const double *d_ptab = nullptr;
@@ -367,6 +373,7 @@ extern "C" int pam_context_create(pam_context **out, int device) {
       return PAM_FATAL;
     }
     c->d_atab = t->tab;
+    c->d_atab_f = t->tab_f;
     c->d_atab_off = t->off;
     c->atab_nmax = t->nmax;
     c->d_ptab = t->ptab;
@@ -702,6 +709,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   {
     const char *mk = getenv("PAM_MIE");
     s.mie_chunked = (mk && strcmp(mk, "chunked") == 0) ? 1 : 0;
+    s.mie_fp64 = (mk && strcmp(mk, "fp64") == 0) ? 1 : 0;
     s.fault = FAULT_NONE;
     if (const char *ft = getenv("PAM_TEST_FAULT")) {
       char kind[16] = {0};
@@ -825,6 +833,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   w.mu = c->d_mu;
   w.qw = c->d_qw;
   w.atab = c->d_atab;
+  w.atab_f = c->d_atab_f;
   w.atab_off = c->d_atab_off;
   w.atab_nmax = c->atab_nmax;
   w.ptab = c->d_ptab;

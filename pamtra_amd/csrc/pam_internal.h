@@ -86,6 +86,8 @@ struct Settings {
   // otherwise calls without self-similar Rayleigh-Gans take the
   // coefficient-first kernel
   int mie_chunked;
+  // 1: the mixed-precision path keeps the FP64 Mie kernel (PAM_MIE=fp64)
+  int mie_fp64;
   // test mode (PAM_TEST_FAULT=<kind>:<column>:<channel>): the named (column,
   // channel) meets the failure the reference stops on, so that its isolation
   // can be tested; kind FAULT_NAN_COEF makes its Legendre coefficients NaN
@@ -156,6 +158,7 @@ struct Work {
   double *Sp, *Sm;                     // [nitems*nf][NN]
   // Mie angle tables per series length N (see build_angle_tables)
   const double *atab;
+  const float *atab_f;                 // the same tables rounded to FP32
   const int64_t *atab_off;             // [atab_nmax + 1]
   int atab_nmax;
   // phase-matrix tables per NUMPTS (see build_phase_tables) and the units
@@ -281,6 +284,8 @@ void launch_dsd(const Dims &d, const Settings &s, const Profile &p, const pam_hy
 // (doubles) and returns the total size; build_angle_tables fills the table.
 int64_t angle_table_layout(int64_t *off, int *nmax);
 void build_angle_tables(double *tab, const int64_t *d_off, int nmax, hipStream_t st);
+// dst[i] = (float)src[i], i < n: the FP32 copy of the angle tables
+void round_tables_f32(const double *tab, float *tab_f, int64_t n, hipStream_t st);
 // phase-matrix tables: off[0..PH_NBUCKET] (doubles), total size returned;
 // d_mu holds the NMU Lobatto nodes
 int64_t phase_table_layout(int64_t *off);

@@ -161,6 +161,16 @@ void build_angle_tables(double *tab, const int64_t *d_off, int nmax, hipStream_t
   hipLaunchKernelGGL(k_angle_tables, dim3((unsigned)nmax), dim3(MIE_WG), 0, st, tab, d_off);
 }
 
+__global__ void k_round_f32(const double *src, float *dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
+void round_tables_f32(const double *tab, float *tab_f, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_round_f32, dim3(1024), dim3(256), 0, st, tab, tab_f, n);
+}
+
 // Phase B as a matrix product on the FP64 matrix cores.  For a chunk of Bc
 // bins the coefficient matrix U (2 Bc rows: real and imaginary parts; Kp =
 // 2 Np columns: c_n a_n for n < Np, then c_n b_n) times
@@ -192,6 +202,7 @@ __device__ __forceinline__ cd cdiv_m(cd a, cd b) {
 }
 
 typedef double d4m_t __attribute__((ext_vector_type(4)));
+typedef float f4m_t __attribute__((ext_vector_type(4)));
 
 // LDS stage of the angle table (each of c_n pi_n and c_n tau_n, series
 // length x angles with mu >= 0); longer tables are read from memory.
@@ -978,11 +989,208 @@ constexpr int MIE2_NW = MIE2_WG / 64;
 constexpr int MIE2_TAB = 768;          // LDS stage of c_n pi_n, c_n tau_n (n x angles)
 constexpr int MIE2_NL = (MAXNLEG + MIE2_WG - 1) / MIE2_WG;  // Legendre orders per thread
 
+// Rows of the U slot.  FP64 (16x16x4 f64 MFMA, lane group k holds result
+// rows k + 4 r): per 16-row tile [re b0..b3, im b0..b3, re b4..b7, im
+// b4..b7].  FP32 (16x16x4 f32 MFMA, lane group k holds rows 4 k + r): per
+// tile [re b0, im b0, re b1, im b1, ...].  Either way a lane's four results
+// are the real and imaginary parts of two bins at one angle.
+template <typename R> struct Mie2Rows;
+template <> struct Mie2Rows<double> {
+  typedef d4m_t acc_t;
+  static constexpr int im_off = 4;
+  static __device__ __forceinline__ int row_re(int b) { return 16 * (b >> 3) + u_row_re(b & 7); }
+  static __device__ __forceinline__ int tile_bin(int r) { return (r & 3) + 4 * (r >> 3); }
+  // bin (within the tile) of result registers 2 h, 2 h + 1 of lane group k
+  static __device__ __forceinline__ int out_bin(int k, int h) { return 4 * h + k; }
+  static __device__ __forceinline__ acc_t mfma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mie2Rows<float> {
+  typedef f4m_t acc_t;
+  static constexpr int im_off = 1;
+  static __device__ __forceinline__ int row_re(int b) { return 16 * (b >> 3) + 2 * (b & 7); }
+  static __device__ __forceinline__ int tile_bin(int r) { return r >> 1; }
+  static __device__ __forceinline__ int out_bin(int k, int h) { return 2 * k + h; }
+  static __device__ __forceinline__ acc_t mfma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// miecalc + miecross for one bin (mie_scat_utilities.f90:8-116), FP64: the
+// D_n recursion, then psi_n, chi_n, a_n, b_n and the efficiency sums in
+// ascending n.  a_n, b_n go to the slot rows at ure (type R); D_n is parked
+// in the a-columns between the two passes.
+template <typename R>
+__device__ __forceinline__ void mie2_series_f64(double x, cd msp, int nmie, int nph, int ld, R *ure,
+                                                double &s1, double &s2, cd &s3) {
+  const cd mc = conj(msp);
+  const cd y = mc * x;
+  const cd yinv = cdiv_m(cd{1.0, 0.0}, y);
+  const double xinv = rcp_nr(x);
+  const cd mcinv = cdiv_m(cd{1.0, 0.0}, mc);
+  R *uim = ure + Mie2Rows<R>::im_off;
+  // downward log-derivative recursion from n = nmie + 15; D_n (n <= nmie)
+  // parked in the a-column n-1
+  cd dn = cd{0.0, 0.0};
+  for (int n = nmie + 15; n >= 2; n--) {
+    const cd ny = (double)n * yinv;
+    dn = ny - cdiv_m(cd{1.0, 0.0}, dn + ny);
+    if (n - 1 <= nmie) {
+      ure[(size_t)(n - 2) * ld] = (R)dn.re;
+      uim[(size_t)(n - 2) * ld] = (R)dn.im;
+    }
+  }
+  // upward Riccati-Bessel recursion, a_n, b_n and the efficiency sums in
+  // ascending n
+  double psim = cos(x), psin = sin(x), chim = -sin(x), chin = cos(x);
+  s1 = 0.0;
+  s2 = 0.0;
+  s3 = cd{0.0, 0.0};
+  // D_n is read back four terms ahead of its use (dq[] rotates), the
+  // loads issued before the stores of the terms between
+  cd dq[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    dq[q] = (q < nmie) ? cd{(double)ure[(size_t)q * ld], (double)uim[(size_t)q * ld]} : cd{0.0, 0.0};
+  for (int n0 = 1; n0 <= nmie; n0 += 4)
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int n = n0 + q;
+    if (n > nmie) break;
+    const cd Dn = dq[q];
+    if (n + 4 <= nmie) dq[q] = cd{(double)ure[(size_t)(n + 3) * ld], (double)uim[(size_t)(n + 3) * ld]};
+    const double fn = (double)(2 * n - 1) * xinv;
+    double tmp = psin;
+    psin = fn * psin - psim;
+    psim = tmp;
+    tmp = chin;
+    chin = fn * chin - chim;
+    chim = tmp;
+    const cd xin = cd{sp(psin), sp(-chin)};
+    const cd xim = cd{sp(psim), sp(-chim)};
+    const double nx = (double)n * xinv;
+    cd ct = Dn * mcinv + nx;
+    const cd an = cdiv_m(ct * psin - psim, ct * xin - xim);
+    ct = mc * Dn + nx;
+    const cd bn = cdiv_m(ct * psin - psim, ct * xin - xim);
+    ure[(size_t)(n - 1) * ld] = (R)an.re;
+    uim[(size_t)(n - 1) * ld] = (R)an.im;
+    ure[(size_t)(nph + n - 1) * ld] = (R)bn.re;
+    uim[(size_t)(nph + n - 1) * ld] = (R)bn.im;
+    const double wn = (double)(2 * n + 1);
+    s1 = s1 + wn * (an.re + bn.re);
+    s2 = s2 + wn * ((an.re * an.re + an.im * an.im) + (bn.re * bn.re + bn.im * bn.im));
+    const double ws = (n & 1) ? -wn : wn;
+    s3 = s3 + ws * cd{an.re - bn.re, an.im - bn.im};
+  }
+}
+
+// Single-precision complex arithmetic of the mixed-precision Mie series.
+struct cf {
+  float re, im;
+};
+__device__ __forceinline__ cf operator+(cf a, cf b) { return cf{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cf operator-(cf a, cf b) { return cf{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cf operator*(cf a, cf b) {
+  return cf{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cf operator*(float s, cf a) { return cf{s * a.re, s * a.im}; }
+__device__ __forceinline__ cf operator*(cf a, float s) { return cf{a.re * s, a.im * s}; }
+__device__ __forceinline__ cf operator+(cf a, float s) { return cf{a.re + s, a.im}; }
+__device__ __forceinline__ cf operator-(cf a, float s) { return cf{a.re - s, a.im}; }
+// 1/x from the hardware reciprocal and one Newton step
+__device__ __forceinline__ float rcp_nrf(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.f), r);
+}
+__device__ __forceinline__ cf cdiv_f(cf a, cf b) {
+  const float r = rcp_nrf(b.re * b.re + b.im * b.im);
+  return cf{(a.re * b.re + a.im * b.im) * r, (a.im * b.re - a.re * b.im) * r};
+}
+
+// The mixed-precision series of one bin: the complex work (the D_n
+// recursion, a_n, b_n) in FP32, the real Riccati-Bessel recursion in FP64
+// and the efficiency sums accumulated in FP64.
+//
+// psi_n and chi_n stay in double because the upward recursion cancels at
+// small x: psi_1 = sin x / x - cos x is O(x^2) from O(1) terms, so an FP32
+// psi_1 carries a relative error ~eps / x^2 (1.5e-2 at x = 2e-3), which
+// reaches a_1 through its numerator.  With psi_n, chi_n exact to FP64 and
+// rounded once, the remaining FP32 cancellation is in the numerator of b_1,
+// (m D_1 + 1/x) psi_1 - psi_0 = O(x^3) from O(x) terms: b_1 then has a
+// relative error ~eps / x^2, but |b_1| ~ x^2 |a_1|, so its absolute error
+// stays ~eps |a_1| in every sum it enters.  Bins with x < MIE2_X32 (where
+// chi_n^2 approaches the FP32 range) take the FP64 series.
+constexpr double MIE2_X32 = 1.0e-3;
+__device__ __forceinline__ void mie2_series_f32(double x, cd msp, int nmie, int nph, int ld,
+                                                float *ure, double &s1, double &s2, cd &s3) {
+  float *uim = ure + Mie2Rows<float>::im_off;
+  const cf mc = cf{(float)msp.re, -(float)msp.im};
+  const float xf = (float)x;
+  const cf y = mc * xf;
+  const cf yinv = cdiv_f(cf{1.f, 0.f}, y);
+  const double xinv = rcp_nr(x);
+  const cf mcinv = cdiv_f(cf{1.f, 0.f}, mc);
+  cf dn = cf{0.f, 0.f};
+  for (int n = nmie + 15; n >= 2; n--) {
+    const cf ny = (float)n * yinv;
+    dn = ny - cdiv_f(cf{1.f, 0.f}, dn + ny);
+    if (n - 1 <= nmie) {
+      ure[(size_t)(n - 2) * ld] = dn.re;
+      uim[(size_t)(n - 2) * ld] = dn.im;
+    }
+  }
+  double psim = cos(x), psin = sin(x), chim = -sin(x), chin = cos(x);
+  s1 = 0.0;
+  s2 = 0.0;
+  s3 = cd{0.0, 0.0};
+  cf dq[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    dq[q] = (q < nmie) ? cf{ure[(size_t)q * ld], uim[(size_t)q * ld]} : cf{0.f, 0.f};
+  for (int n0 = 1; n0 <= nmie; n0 += 4)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int n = n0 + q;
+      if (n > nmie) break;
+      const cf Dn = dq[q];
+      if (n + 4 <= nmie) dq[q] = cf{ure[(size_t)(n + 3) * ld], uim[(size_t)(n + 3) * ld]};
+      const double fn = (double)(2 * n - 1) * xinv;
+      double tmp = psin;
+      psin = fn * psin - psim;
+      psim = tmp;
+      tmp = chin;
+      chin = fn * chin - chim;
+      chim = tmp;
+      const cf xin = cf{(float)psin, (float)(-chin)};
+      const cf xim = cf{(float)psim, (float)(-chim)};
+      const float pn = (float)psin, pm = (float)psim;
+      const float nx = (float)((double)n * xinv);
+      cf ct = Dn * mcinv + nx;
+      const cf an = cdiv_f(ct * pn - cf{pm, 0.f}, ct * xin - xim);
+      ct = mc * Dn + nx;
+      const cf bn = cdiv_f(ct * pn - cf{pm, 0.f}, ct * xin - xim);
+      ure[(size_t)(n - 1) * ld] = an.re;
+      uim[(size_t)(n - 1) * ld] = an.im;
+      ure[(size_t)(nph + n - 1) * ld] = bn.re;
+      uim[(size_t)(nph + n - 1) * ld] = bn.im;
+      const double wn = (double)(2 * n + 1);
+      const double ar = an.re, ai = an.im, br = bn.re, bi = bn.im;
+      s1 = s1 + wn * (ar + br);
+      s2 = s2 + wn * ((ar * ar + ai * ai) + (br * br + bi * bi));
+      const double ws = (n & 1) ? -wn : wn;
+      s3 = s3 + ws * cd{ar - br, ai - bi};
+    }
+}
+
+// R = float: 19,748 B, within the 20 KB of eight workgroups per compute unit
+template <typename R>
 struct Mie2Lds {
   double ps[3][MAX_NQUAD];     // weighted phase sums per angle
-  double tab[2 * MIE2_TAB];    // angle table stage
+  R tab[2 * MIE2_TAB];         // angle table stage
   short nmb[MIE_MAXBIN];       // per bin: series length (0: skipped)
-  double pw[MIE2_NW][6 * 16];  // per-wave phase sums of shared angle tiles
+  R pw[MIE2_NW][6 * 16];       // per-wave phase sums of shared angle tiles
   double red[MIE2_NW][3];
   int redi[MIE2_NW][2];
   double hm[MAX_HYDRO][2];
@@ -992,14 +1200,20 @@ struct Mie2Lds {
   int slot;
 };
 
-// Rows of the U slot for bin b (re; im = re + 4): 8 bins per 16-row tile.
-__device__ __forceinline__ int mie2_row(int b) { return 16 * (b >> 3) + u_row_re(b & 7); }
-// bin of row r of a 16-row tile
-__device__ __forceinline__ int mie2_tile_bin(int r) { return (r & 3) + 4 * (r >> 3); }
-
+// R = double: the FP64 kernel.  R = float: the mixed-precision kernel (FP32
+// series and S1/S2 products with the slot, the angle tables and the MFMA in
+// FP32; the efficiency sums, the phase sums and the Legendre projection in
+// FP64).
+template <typename R>
 __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
                                           const pam_hydro_desc *hyd, const double *freqs,
-                                          const Work &w, int itf, double *U, Mie2Lds &L) {
+                                          const Work &w, int itf, R *U, Mie2Lds<R> &L) {
+  constexpr bool F32 = sizeof(R) == 4;
+  typedef Mie2Rows<R> M2;
+  typedef typename M2::acc_t acc_t;
+  // the angle-table stage holds TABN entries of each of c_n pi_n, c_n tau_n
+  constexpr int TABN = MIE2_TAB;
+  R *const tab = L.tab;
   const int t = threadIdx.x;
   const int wv = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
   const int li_item = itf / d.nf, f = itf % d.nf;
@@ -1054,30 +1268,32 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
     const int nph = max(4, (nterms + 3) & ~3);  // b-columns start here
     const int nlegen = mie_nlegen(nterms);
     const int nquad = mie_nquad(nterms);
-    const double *tmu = w.atab + w.atab_off[nterms], *twt = tmu + nquad, *tpi = twt + nquad,
-                 *ttau = tpi + (size_t)nterms * nquad, *tpl = ttau + (size_t)nterms * nquad;
+    const double *tmu = w.atab + w.atab_off[nterms], *twt = tmu + nquad,
+                 *tpl = twt + nquad + 2 * (size_t)nterms * nquad;
+    const R *tpi = (F32 ? (const R *)w.atab_f : (const R *)w.atab) + w.atab_off[nterms] + 2 * nquad;
+    const R *ttau = tpi + (size_t)nterms * nquad;
     const int nh2 = (nquad + 1) >> 1, j0 = nquad - nh2;
     const int nct = (nh2 + 15) >> 4;
     const int nrt = (nbin + 7) >> 3;
     for (int k = t; k < 3 * MAX_NQUAD; k += MIE2_WG) (&L.ps[0][0])[k] = 0.0;
     // c_n pi_n, c_n tau_n at the angles mu >= 0 into LDS when they fit
     // the whole slice at mu >= 0 when it fits; else, for series up to
-    // MIE2_TAB / 16 terms, one 16-angle tile of it per pass of B
-    const bool tab_lds = nterms * nh2 <= MIE2_TAB;
-    const bool tab_tile = !tab_lds && nterms * 16 <= MIE2_TAB;
+    // TABN / 16 terms, one 16-angle tile of it per pass of B
+    const bool tab_lds = nterms * nh2 <= TABN;
+    const bool tab_tile = !tab_lds && nterms * 16 <= TABN;
     auto stage_tile = [&](int ct) {
       for (int e = t; e < nterms * 16; e += MIE2_WG) {
         const int n = e >> 4, hix = 16 * ct + (e & 15);
         const bool ok = hix < nh2;
-        L.tab[e] = ok ? tpi[(size_t)n * nquad + j0 + hix] : 0.0;
-        L.tab[MIE2_TAB + e] = ok ? ttau[(size_t)n * nquad + j0 + hix] : 0.0;
+        tab[e] = ok ? tpi[(size_t)n * nquad + j0 + hix] : (R)0;
+        tab[TABN + e] = ok ? ttau[(size_t)n * nquad + j0 + hix] : (R)0;
       }
     };
     if (tab_lds)
       for (int e = t; e < nterms * nh2; e += MIE2_WG) {
         const int n = e / nh2, h = e - n * nh2;
-        L.tab[e] = tpi[(size_t)n * nquad + j0 + h];
-        L.tab[MIE2_TAB + e] = ttau[(size_t)n * nquad + j0 + h];
+        tab[e] = tpi[(size_t)n * nquad + j0 + h];
+        tab[TABN + e] = ttau[(size_t)n * nquad + j0 + h];
       }
     else if (tab_tile)
       stage_tile(0);
@@ -1098,64 +1314,15 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
       // (mie_spheres.f90:187-192), decided per bin
       const cd msp = (is_ice && rho[ir] != 917.0) ? eps_mix_air(m_ice, rho[ir]) : m_ice;
       const double x = PI * dia[ir] / wavelength;
-      const cd mc = conj(msp);
-      const cd y = mc * x;
-      const cd yinv = cdiv_m(cd{1.0, 0.0}, y);
-      const double xinv = rcp_nr(x);
-      const cd mcinv = cdiv_m(cd{1.0, 0.0}, mc);
-      double *ure = U + mie2_row(ir), *uim = ure + 4;
-      // downward log-derivative recursion from n = nmie + 15; D_n (n <= nmie)
-      // parked in the a-column n-1
-      cd dn = cd{0.0, 0.0};
-      for (int n = nmie + 15; n >= 2; n--) {
-        const cd ny = (double)n * yinv;
-        dn = ny - cdiv_m(cd{1.0, 0.0}, dn + ny);
-        if (n - 1 <= nmie) {
-          ure[(size_t)(n - 2) * ld] = dn.re;
-          uim[(size_t)(n - 2) * ld] = dn.im;
-        }
-      }
-      // upward Riccati-Bessel recursion, a_n, b_n and the efficiency sums in
-      // ascending n
-      double psim = cos(x), psin = sin(x), chim = -sin(x), chin = cos(x);
-      double s1 = 0.0, s2 = 0.0;
-      cd s3 = cd{0.0, 0.0};
-      // D_n is read back four terms ahead of its use (dq[] rotates), the
-      // loads issued before the stores of the terms between
-      cd dq[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-        dq[q] = (q < nmie) ? cd{ure[(size_t)q * ld], uim[(size_t)q * ld]} : cd{0.0, 0.0};
-      for (int n0 = 1; n0 <= nmie; n0 += 4)
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int n = n0 + q;
-        if (n > nmie) break;
-        const cd Dn = dq[q];
-        if (n + 4 <= nmie) dq[q] = cd{ure[(size_t)(n + 3) * ld], uim[(size_t)(n + 3) * ld]};
-        const double fn = (double)(2 * n - 1) * xinv;
-        double tmp = psin;
-        psin = fn * psin - psim;
-        psim = tmp;
-        tmp = chin;
-        chin = fn * chin - chim;
-        chim = tmp;
-        const cd xin = cd{sp(psin), sp(-chin)};
-        const cd xim = cd{sp(psim), sp(-chim)};
-        const double nx = (double)n * xinv;
-        cd ct = Dn * mcinv + nx;
-        const cd an = cdiv_m(ct * psin - psim, ct * xin - xim);
-        ct = mc * Dn + nx;
-        const cd bn = cdiv_m(ct * psin - psim, ct * xin - xim);
-        ure[(size_t)(n - 1) * ld] = an.re;
-        uim[(size_t)(n - 1) * ld] = an.im;
-        ure[(size_t)(nph + n - 1) * ld] = bn.re;
-        uim[(size_t)(nph + n - 1) * ld] = bn.im;
-        const double wn = (double)(2 * n + 1);
-        s1 = s1 + wn * (an.re + bn.re);
-        s2 = s2 + wn * ((an.re * an.re + an.im * an.im) + (bn.re * bn.re + bn.im * bn.im));
-        const double ws = (n & 1) ? -wn : wn;
-        s3 = s3 + ws * cd{an.re - bn.re, an.im - bn.im};
+      double s1, s2;
+      cd s3;
+      if constexpr (F32) {
+        if (x >= MIE2_X32)
+          mie2_series_f32(x, msp, nmie, nph, ld, U + M2::row_re(ir), s1, s2, s3);
+        else
+          mie2_series_f64(x, msp, nmie, nph, ld, U + M2::row_re(ir), s1, s2, s3);
+      } else {
+        mie2_series_f64(x, msp, nmie, nph, ld, U + M2::row_re(ir), s1, s2, s3);
       }
       const double rr = dia[ir] / 2.0;
       const double ar = (rr * rr) * PI;
@@ -1211,65 +1378,65 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
           const int b = 8 * rt + q;
           nmt = max(nmt, b < nbin ? (int)L.nmb[b] : 0);
         }
-        const int brow = 8 * rt + mie2_tile_bin(li);
+        const int brow = 8 * rt + M2::tile_bin(li);
         const int nrow = brow < nbin ? (int)L.nmb[brow] : 0;
         const int np = max(4, (nmt + 3) & ~3);
         const int nsk = np >> 2;
-        d4m_t x0 = {0.0, 0.0, 0.0, 0.0}, y0 = x0, x1 = x0, y1 = x0;
-        const double *urow = U + 16 * rt + li;
+        acc_t x0 = {(R)0, (R)0, (R)0, (R)0}, y0 = x0, x1 = x0, y1 = x0;
+        const R *urow = U + 16 * rt + li;
         // one k-step: both parity groups, operand a from the slot
-        auto kstep = [&](int sk, int g, double a) {
+        auto kstep = [&](int sk, int g, R a) {
           int cu, n;
           bool ap;
           mie_kmap(g, 4 * sk + lk, np, cu, n, ap);
-          double pv = 0.0, tv = 0.0;
+          R pv = (R)0, tv = (R)0;
           if (ang_ok && n <= nterms) {
             if (tab_lds) {
-              pv = L.tab[(n - 1) * nh2 + hix];
-              tv = L.tab[MIE2_TAB + (n - 1) * nh2 + hix];
+              pv = tab[(n - 1) * nh2 + hix];
+              tv = tab[TABN + (n - 1) * nh2 + hix];
             } else if (tab_tile) {
-              pv = L.tab[(n - 1) * 16 + li];
-              tv = L.tab[MIE2_TAB + (n - 1) * 16 + li];
+              pv = tab[(n - 1) * 16 + li];
+              tv = tab[TABN + (n - 1) * 16 + li];
             } else {
               pv = tpi[(size_t)(n - 1) * nquad + ang];
               tv = ttau[(size_t)(n - 1) * nquad + ang];
             }
           }
           if (g == 0) {
-            x0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? pv : tv, x0, 0, 0, 0);
-            y0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? tv : pv, y0, 0, 0, 0);
+            x0 = M2::mfma(a, ap ? pv : tv, x0);
+            y0 = M2::mfma(a, ap ? tv : pv, y0);
           } else {
-            x1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? pv : tv, x1, 0, 0, 0);
-            y1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ap ? tv : pv, y1, 0, 0, 0);
+            x1 = M2::mfma(a, ap ? pv : tv, x1);
+            y1 = M2::mfma(a, ap ? tv : pv, y1);
           }
         };
-        auto uload = [&](int sk, int g) -> double {
+        auto uload = [&](int sk, int g) -> R {
           int cu, n;
           bool ap;
           mie_kmap(g, 4 * sk + lk, np, cu, n, ap);
           const int cg = ap ? n - 1 : nph + n - 1;
-          return (n <= nrow) ? urow[(size_t)cg * ld] : 0.0;
+          return (n <= nrow) ? urow[(size_t)cg * ld] : (R)0;
         };
         // unrolled so that the operand loads of several k-steps are in
         // flight together (the tables come from memory for long series)
 #pragma unroll 4
         for (int sk = 0; sk < nsk; sk++) {
-          const double a0 = uload(sk, 0), a1 = uload(sk, 1);
+          const R a0 = uload(sk, 0), a1 = uload(sk, 1);
           kstep(sk, 0, a0);
           kstep(sk, 1, a1);
         }
 #pragma unroll
         for (int h2 = 0; h2 < 2; h2++) {
-          const int bw = 8 * rt + 4 * h2 + lk;
+          const int bw = 8 * rt + M2::out_bin(lk, h2);
           const double wgt = (bw < nbin && L.nmb[bw] > 0) ? nd[bw] * dd[bw] : 0.0;
           const int re = 2 * h2, im = 2 * h2 + 1;
           // +mu
-          double s1r = x0[re] + x1[re], s1i = x0[im] + x1[im];
-          double s2r = y1[re] + y0[re], s2i = y1[im] + y0[im];
-          double a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
-          p1 += (0.5 * (a2 + a1)) * wgt;
-          p2 += (0.5 * (a2 - a1)) * wgt;
-          p3 += (s2r * s1r + s2i * s1i) * wgt;
+          R s1r = x0[re] + x1[re], s1i = x0[im] + x1[im];
+          R s2r = y1[re] + y0[re], s2i = y1[im] + y0[im];
+          R a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
+          p1 += (double)((R)0.5 * (a2 + a1)) * wgt;
+          p2 += (double)((R)0.5 * (a2 - a1)) * wgt;
+          p3 += (double)(s2r * s1r + s2i * s1i) * wgt;
           // -mu
           s1r = x0[re] - x1[re];
           s1i = x0[im] - x1[im];
@@ -1277,9 +1444,9 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
           s2i = y1[im] - y0[im];
           a1 = s1r * s1r + s1i * s1i;
           a2 = s2r * s2r + s2i * s2i;
-          m1 += (0.5 * (a2 + a1)) * wgt;
-          m2 += (0.5 * (a2 - a1)) * wgt;
-          m3 += (s2r * s1r + s2i * s1i) * wgt;
+          m1 += (double)((R)0.5 * (a2 + a1)) * wgt;
+          m2 += (double)((R)0.5 * (a2 - a1)) * wgt;
+          m3 += (double)(s2r * s1r + s2i * s1i) * wgt;
         }
       }
       // the four lane groups hold different bins of the same angle
@@ -1306,7 +1473,7 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
             L.ps[2][angm] = m3;
           }
         } else {
-          double *pw = L.pw[wv];
+          R *pw = L.pw[wv];
           pw[li] = p1;
           pw[16 + li] = p2;
           pw[32 + li] = p3;
@@ -1326,7 +1493,7 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
         const int ang = j0 + hix, angm = nquad - 1 - ang;
         double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int q = 0; q < wpc; q++) {
-          const double *pw = L.pw[tab_tile ? q : ctw * wpc + q];
+          const R *pw = L.pw[tab_tile ? q : ctw * wpc + q];
 #pragma unroll
           for (int k = 0; k < 6; k++) v[k] += pw[16 * k + l16];
         }
@@ -1405,7 +1572,12 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
         g[0][j] = hl * c1;
         g[1][j] = hl * c2;
         g[2][j] = hl * c3;
-        if (g[0][j] > sp(1.0e-7)) atomicMax(&L.nlh, l);
+        // the reference keeps the orders with a coefficient above 1e-7
+        // (mie_spheres.f90:305-313); from FP32 S1/S2 the projection carries a
+        // noise of ~1e-7 of g_0 itself, so the mixed path keeps those above
+        // 1e-6 (the omitted orders change the phase function by < 1e-6 of
+        // its mean)
+        if (g[0][j] > (F32 ? 1.0e-6 : sp(1.0e-7))) atomicMax(&L.nlh, l);
         if (isnan(g[0][j]) || isnan(c2) || isnan(c3)) L.nan_ = 1;
       }
     }
@@ -1467,13 +1639,14 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
 
 // Persistent grid of the coefficient-first Mie: workgroup b owns slot b of
 // the U scratch and takes units from the work queue until it is drained.
-__global__ void __launch_bounds__(MIE2_WG) __attribute__((amdgpu_waves_per_eu(3))) k_mie2(Dims d, Settings s, const pam_hydro_desc *hyd,
+template <typename R>
+__global__ void __launch_bounds__(MIE2_WG) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 3))) k_mie2(Dims d, Settings s, const pam_hydro_desc *hyd,
                                                    const double *freqs, Work w) {
-  __shared__ Mie2Lds L;
-  double *U = w.mie_u + (size_t)blockIdx.x * w.mie_slot_doubles;
+  __shared__ Mie2Lds<R> L;
+  R *U = reinterpret_cast<R *>(w.mie_u + (size_t)blockIdx.x * w.mie_slot_doubles);
   const int n = d.nitems * d.nf;
   for (int itf = next_unit(&w.queue[0], &L.slot); itf < n; itf = next_unit(&w.queue[0], &L.slot))
-    mie2_unit(d, s, hyd, freqs, w, itf, U, L);
+    mie2_unit<R>(d, s, hyd, freqs, w, itf, U, L);
 }
 
 // One unit per workgroup, or (PERS) a persistent grid taking units from a
@@ -1501,7 +1674,10 @@ void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, con
   if (n == 0) return;
   if (!s.ssrg && !s.mie_chunked && w.mie_u && w.mie_slots > 0) {
     const unsigned grid = (unsigned)std::min<size_t>(n, (size_t)w.mie_slots);
-    hipLaunchKernelGGL(k_mie2, dim3(grid), dim3(MIE2_WG), 0, st, d, s, hyd, freqs, w);
+    if (s.precision == PAM_PREC_MIXED && !s.mie_fp64 && w.atab_f)
+      hipLaunchKernelGGL(k_mie2<float>, dim3(grid), dim3(MIE2_WG), 0, st, d, s, hyd, freqs, w);
+    else
+      hipLaunchKernelGGL(k_mie2<double>, dim3(grid), dim3(MIE2_WG), 0, st, d, s, hyd, freqs, w);
     return;
   }
   const dim3 grid((unsigned)persistent_grid(n, s.pers_mie, s.ncu));

@@ -52,6 +52,14 @@ def test_mixed_matches_fp64_per_channel(cfg, ncol):
     assert np.mean(nb[na >= 0]) < np.mean(na[na >= 0]) - 8
     per = _per_channel(a, b, freqs)
     assert max(per) <= MIXED_TOL
+    # the FP32 Mie series: extinction and single-scattering albedo per
+    # (layer, channel) against the FP64 series
+    ka, kb = a["kexttot"], b["kexttot"]
+    sc = ka > 0
+    np.testing.assert_array_equal(sc, kb > 0)
+    rk = np.abs(kb[sc] / ka[sc] - 1.0).max()
+    print(f"max relative dkext = {rk:.2e}", flush=True)
+    assert rk <= 1e-5
 
 
 def test_mixed_serving_mix_high_frequency():

@@ -65,15 +65,24 @@ def test_radar_matches_oracle(att):
     np.testing.assert_array_equal(g["ze"][..., 2, :], -9999.0)  # HV of spheres
 
 
-def test_radar_mixed_precision_is_the_fp64_radar():
-    """The radar outputs come from the FP64 Mie and extinction in both paths."""
+def test_radar_mixed_precision_vs_fp64_radar():
+    """The mixed path's radar outputs come from its FP32 Mie series (the
+    backscattering and extinction sums accumulated in FP64): Ze within 1e-4 dB
+    and Att_hydro within 1e-5 relative of the FP64 path; the gas attenuation
+    and the radar heights do not involve the Mie stage and are identical."""
     pam, freqs = workloads.config2(ncol=8, seed=72)
     nml = dict(pam.nmlSet)
     nml.update(active=True)
     a, st = _gpu(pam, freqs, nml, "fp64")
     b, st2 = _gpu(pam, freqs, nml, "mixed")
     assert st == st2 == 0
-    for k in ("ze", "att_hydro", "att_atmo", "radar_hgt"):
+    ok = a["ze"] > -9999.0
+    np.testing.assert_array_equal(ok, b["ze"] > -9999.0)
+    d = np.abs(a["ze"][ok] - b["ze"][ok]).max()
+    print(f"mixed vs fp64: max |dZe| = {d:.2e} dB", flush=True)
+    assert d <= 1e-4
+    np.testing.assert_allclose(b["att_hydro"], a["att_hydro"], rtol=1e-5, atol=1e-14)
+    for k in ("att_atmo", "radar_hgt"):
         np.testing.assert_array_equal(a[k], b[k])
 
 
