@@ -749,41 +749,16 @@ __device__ bool series_f32(const float *R, float *X, float *Gt, float *wmax, int
   return false;
 }
 
-// G = inv(I - X) for a given X (overwritten by its powers) by the product-
-// form series, truncated when max |X^(2^k)| < tol.  false: not converged
-// after NEU_MAXIT factors, or a NaN.
-__device__ bool series_inv_given(double *X, double *G, float *wmax, double tol, int &np) {
-  constexpr int U = KUNR_DBL;
-  const int t = threadIdx.x;
-  for (int e = t; e < NN2; e += RT_WG) {
-    const int i = e / NN, j = e % NN;
-    G[i * LD + j] = (i == j ? 1.0 : 0.0) + X[i * LD + j];
-  }
-  double m = mm_core<U>(X, X, nullptr, X, wmax);  // its barriers order the G writes
-  np++;
-  for (int it = 0; it < NEU_MAXIT; it++) {
-    if (!(m < 0x1p64)) return false;
-    if (m < tol) return true;
-    if ((double)NN * m * m < tol) {
-      mm_add<U>(G, G, X, G);
-      np++;
-      return true;
-    }
-    m = mm2<U>(G, X, G, G, X, X, nullptr, X, wmax);
-    np += 2;
-  }
-  return false;
-}
-
 // Start layer of the mixed-precision path.
 //
 // The reference doubles from a first-order layer of optical depth <= 1e-6
 // (radtran4.f90:323-345, INITIALIZE4 radintg4.f90:1-45), so ~16 doublings
 // per layer, and its result carries that start's truncation error (~1e-5
 // relative in R).  Here the start layer is the exact solution of the same
-// discrete-ordinate system over a sub-layer zdiff / 2^nd, computed in FP64
-// to ~1e-8, with nd the smallest count that makes ||H dz||_inf <= HS_THETA;
-// that removes about 13 doublings.
+// discrete-ordinate system over a sub-layer zdiff / 2^nd, with nd the
+// smallest count that makes ||H dz||_inf <= HS_THETA; that removes about 13
+// doublings.  It is formed in FP32, like the doublings that follow: its
+// truncation error (below) is at the FP32 resolution.
 //
 // With the first-order operators R1 = dz A, T1 = I - dz B (A, B per unit
 // length, from the phase matrix and the extinction) the intensities of the
@@ -800,18 +775,14 @@ __device__ bool series_inv_given(double *X, double *G, float *wmax, double tol, 
 // S+ = g+ - R g-, S- = -T g-.  Doubling from there with the reference's
 // linear-source recursion (linfac = f) gives the layer's R, T, S.
 // c and s are truncated after Z^2 and the particular sum after n = 7: the
-// start's relative error is ~theta^6/720.
+// start's relative error is ~theta^6/720 (3e-7).
 constexpr double HS_THETA = 0.25;
-constexpr double HS_TOL = 0x1p-40;  // series truncation of the start's inverse
 
 struct MixLds {
-  union {
-    double M[4][MSZ];  // FP64 start
-    struct {
-      float R[MSZ], T[MSZ], G[MSZ], X[MSZ];
-      float v[16][NN];  // S+, S-, C+, C-, then source-chain scratch
-    } f;                // FP32 doubling
-  } u;
+  // the start's matrices (dz S, dz D, X, Y and their sums), then the
+  // doubling's R, T~, G~ and scratch
+  float m[4][MSZ];
+  float v[16][NN];  // S+, S-, C+, C-, then source-chain scratch
   double g[4][NN];  // particular sums: constant p, m; ramp p, m
   double c[2][NN];  // chain vectors, then g- of the two sources
   double hrow[NN];
@@ -821,12 +792,55 @@ struct MixLds {
   int slot;  // unit index of a persistent workgroup
 };
 
+// Row i of A x for an FP32 matrix and an FP64 vector, accumulated in FP64
+// over the lane pair t, t^1 (as mv_pair).
+__device__ __forceinline__ double mvfd_pair(const float *A, const double *x, int i) {
+  const int h = (threadIdx.x & 1) * 8;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < NN / 2; k++) {
+    const int c = (k & 7) + h + 16 * (k >> 3);
+    s += (double)A[i * LD + c] * x[c];
+  }
+  return s + __shfl_xor(s, 1, 64);
+}
+static_assert(VL == 2, "the pair mapping of the mixed start");
+
+// G~ = inv(I - X) - I by the product-form series in FP32, X given (and
+// overwritten by its powers), truncated when max |X^(2^k)| < tol.  false:
+// not converged, or a NaN.
+__device__ bool series_inv_f32(float *X, float *Gt, float *wmax, float tol, int &np) {
+  const int t = threadIdx.x;
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int i = e / NN, j = e % NN;
+    Gt[i * LD + j] = X[i * LD + j];
+  }
+  float m = mmf(X, false, X, nullptr, X, wmax);  // its barriers order the Gt writes
+  np++;
+  for (int it = 0; it < NEU_MAXIT; it++) {
+    if (!(m < 0x1p64f)) return false;
+    if (m < tol) return true;
+    if ((float)NN * m * m < tol) {  // last factor: G~ <- G~ + (I + G~) X
+      mmf(Gt, true, X, Gt, Gt, nullptr);
+      np++;
+      return true;
+    }
+    m = mmf2(Gt, true, X, Gt, Gt, X, false, X, nullptr, X, wmax);
+    np += 2;
+  }
+  return false;
+}
+
 // One (scattering layer, channel) unit of the mixed-precision path; false
 // when an inverse series did not converge (nothing written; the unit is
-// recomputed in FP64 by the Gauss-Jordan pass).
+// recomputed in FP64 by the Gauss-Jordan pass).  The start layer is formed
+// in FP32 like the doublings: its operators are I plus terms of order
+// theta, and they are carried as those terms (c(X) - I, U - I, V - I), so
+// no FP32 difference cancels an identity.  CHECK_NORM4 and the choice of
+// nd read the FP64 phase matrix itself.
 __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double *freqs,
                                   const Work &w, int itf, MixLds &L) {
-  double *M0 = L.u.M[0], *M1 = L.u.M[1], *M2 = L.u.M[2], *M3 = L.u.M[3];
+  float *M0 = L.m[0], *M1 = L.m[1], *M2 = L.m[2], *M3 = L.m[3];
   double *smu = L.mu, *sqw = L.qw, *snorm = L.norm;
 
   const int t = threadIdx.x;
@@ -844,17 +858,20 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     salb = 0.0;
   }
   // the phase matrix, formed by k_phase in the unit's T and R slots:
-  // M0 = (+,+) quadrant, M1 = (-,+) quadrant
+  // P0 = (+,+) quadrant, P1 = (-,+) quadrant
+  // staged in FP64 over the LDS of the four FP32 matrices
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
   {
     const bool has = nleg > 0 && s.lhyd_scat;
-    const double *P0 = w.T + (size_t)itf * NN2, *P1 = w.R + (size_t)itf * NN2;
+    const double *G0 = w.T + (size_t)itf * NN2, *G1 = w.R + (size_t)itf * NN2;
+    double *D0 = reinterpret_cast<double *>(L.m[0]), *D1 = reinterpret_cast<double *>(L.m[2]);
     for (int e = t; e < NN2; e += RT_WG) {
-      const int i = e / NN, j = e % NN;
-      M0[i * LD + j] = has ? P0[e] : 0.0;
-      M1[i * LD + j] = has ? P1[e] : 0.0;
+      D0[e] = has ? G0[e] : 0.0;
+      D1[e] = has ? G1[e] : 0.0;
     }
   }
+  const double *P0 = reinterpret_cast<const double *>(L.m[0]);
+  const double *P1 = reinterpret_cast<const double *>(L.m[2]);
   __syncthreads();
   const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
   const double wl = C_LIGHT / (freqs[f] * 1.e3);
@@ -867,15 +884,17 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   const double C2 = 2.0 * 3.1415926535897932384;
   if (t < NMU) {  // CHECK_NORM4
     double sum = absorb - kext;
-    for (int j2 = 0; j2 < NMU; j2++)
-      sum = sum + 2.0 * PI * sqw[j2] * (M0[(NST * j2) * LD + NST * t] + M1[(NST * j2) * LD + NST * t]);
+    for (int j2 = 0; j2 < NMU; j2++) {
+      const int e = (NST * j2) * NN + NST * t;
+      sum = sum + 2.0 * PI * sqw[j2] * (P0[e] + P1[e]);
+    }
     snorm[t] = fabs(sum);
   } else if (t >= 64 && t < 64 + NN) {  // row r of |A| + |B|, per unit length
     const int r = t - 64;
     double sum = 0.0;
     for (int c = 0; c < NN; c++) {
       const double wc = C2 * sqw[c / NST];
-      sum += fabs(wc * M1[r * LD + c]) + fabs((c == r ? ext : 0.0) - wc * M0[r * LD + c]);
+      sum += fabs(wc * P1[r * NN + c]) + fabs((c == r ? ext : 0.0) - wc * P0[r * NN + c]);
     }
     L.hrow[r] = sum / smu[r / NST];
   }
@@ -891,14 +910,26 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   while (nd < 64 && hn * zdiff > ldexp(HS_THETA, nd)) nd++;
   const double dz = ldexp(zdiff, -nd);
   double linfac = (p0 == 0.0) ? 0.0 : ldexp(p1 / p0 - 1.0, -nd);
-  // dz S into M0, dz D into M1; dz e into the chain vector
-  for (int e = t; e < NN2; e += RT_WG) {
-    const int r = e / NN, c = e % NN;
-    const double sc = dz / smu[r / NST], wc = C2 * sqw[c / NST];
-    const double a = sc * wc * M1[r * LD + c];
-    const double b = sc * ((c == r ? ext : 0.0) - wc * M0[r * LD + c]);
-    M0[r * LD + c] = b + a;
-    M1[r * LD + c] = b - a;
+  // dz S into M0, dz D into M1 (formed in FP64, rounded once, written
+  // over the staged phase matrix); dz e into the chain vector
+  {
+    float vs[NN2 / RT_WG], vd[NN2 / RT_WG];
+#pragma unroll
+    for (int q = 0; q < NN2 / RT_WG; q++) {
+      const int e = t + q * RT_WG, r = e / NN, c = e % NN;
+      const double sc = dz / smu[r / NST], wc = C2 * sqw[c / NST];
+      const double a = sc * wc * P1[e];
+      const double b = sc * ((c == r ? ext : 0.0) - wc * P0[e]);
+      vs[q] = (float)(b + a);
+      vd[q] = (float)(b - a);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NN2 / RT_WG; q++) {
+      const int e = t + q * RT_WG, r = e / NN, c = e % NN;
+      M0[r * LD + c] = vs[q];
+      M1[r * LD + c] = vd[q];
+    }
   }
   double *cv = L.c[0], *cw = L.c[1];
   if (t < NN) cv[t] = (t % NST == 0) ? dz * (absorb + gas) / smu[t / NST] : 0.0;
@@ -918,7 +949,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
         const double v = cv[i];
         gcm += 2.0 * v * ifact[2 * k + 1];
         grm += 2.0 * v * ifact[2 * k + 2];
-        const double wk = mv_vl(M0, cv, i);
+        const double wk = mvfd_pair(M0, cv, i);
         gcp -= 2.0 * wk * ifact[2 * k + 2];
         grp -= 2.0 * wk * ifact[2 * k + 3];
         if (vl_lead()) cw[i] = wk;
@@ -926,7 +957,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
       __syncthreads();
       if (k < 3) {
         if (t < RT_WG / 4) {
-          const double v = mv_vl(M1, cw, i);
+          const double v = mvfd_pair(M1, cw, i);
           if (vl_lead()) cv[i] = v;
         }
         __syncthreads();
@@ -941,31 +972,31 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   }
   // (the next product's barriers order these writes)
   int nprod = 7;
-  mm<KUNR_DBL>(M0, M1, M2);  // X
-  mm<KUNR_DBL>(M2, M2, M3);  // X^2
+  mmf(M0, false, M1, nullptr, M2, nullptr);  // X
+  mmf(M2, false, M2, nullptr, M3, nullptr);  // X^2
   for (int e = t; e < NN2; e += RT_WG) {
     const int i = e / NN, j = e % NN;
-    const double id = (i == j) ? 1.0 : 0.0, x = M2[i * LD + j], x2 = M3[i * LD + j];
-    M2[i * LD + j] = id + x * (1.0 / 6.0) + x2 * (1.0 / 120.0);  // s(X)
-    M3[i * LD + j] = id + x * 0.5 + x2 * (1.0 / 24.0);           // c(X)
+    const float x = M2[i * LD + j], x2 = M3[i * LD + j];
+    M2[i * LD + j] = (i == j ? 1.f : 0.f) + (x * (1.f / 6.f) + x2 * (1.f / 120.f));  // s(X)
+    M3[i * LD + j] = x * 0.5f + x2 * (1.f / 24.f);                                  // c(X) - I
   }
   __syncthreads();
-  mm_add<KUNR_DBL>(M3, M2, M0, M3);  // U = c(X) + s(X) dz S
-  mm<KUNR_DBL>(M1, M0, M2);          // Y
-  mm<KUNR_DBL>(M2, M2, M0);          // Y^2
+  mmf(M2, false, M0, M3, M3, nullptr);  // U - I = c(X) - I + s(X) dz S
+  mmf(M1, false, M0, nullptr, M2, nullptr);  // Y
+  mmf(M2, false, M2, nullptr, M0, nullptr);  // Y^2
   for (int e = t; e < NN2; e += RT_WG) {
     const int i = e / NN, j = e % NN;
-    const double id = (i == j) ? 1.0 : 0.0, y = M2[i * LD + j], y2 = M0[i * LD + j];
-    M2[i * LD + j] = id + y * (1.0 / 6.0) + y2 * (1.0 / 120.0);  // s(Y)
-    M0[i * LD + j] = id + y * 0.5 + y2 * (1.0 / 24.0);           // c(Y)
+    const float y = M2[i * LD + j], y2 = M0[i * LD + j];
+    M2[i * LD + j] = (i == j ? 1.f : 0.f) + (y * (1.f / 6.f) + y2 * (1.f / 120.f));  // s(Y)
+    M0[i * LD + j] = y * 0.5f + y2 * (1.f / 24.f);                                  // c(Y) - I
   }
   __syncthreads();
-  mm_add<KUNR_DBL>(M0, M2, M1, M0);  // V = c(Y) + s(Y) dz D
+  mmf(M2, false, M1, M0, M0, nullptr);  // V - I = c(Y) - I + s(Y) dz D
   for (int e = t; e < NN2; e += RT_WG) {
     const int i = e / NN, j = e % NN;
-    const double u = M3[i * LD + j], v = M0[i * LD + j];
-    M1[i * LD + j] = (i == j ? 1.0 : 0.0) - 0.5 * (u + v);  // I - (U + V) / 2
-    M3[i * LD + j] = 0.5 * (u - v);
+    const float u = M3[i * LD + j], v = M0[i * LD + j];
+    M1[i * LD + j] = -0.5f * (u + v);  // I - (U + V) / 2
+    M3[i * LD + j] = 0.5f * (u - v);   // (U - V) / 2
   }
   if (t < NN) {  // g- of the linear and of the constant source
     const double gcmin = 0.5 * (L.g[0][t] - L.g[1][t]), grmin = 0.5 * (L.g[2][t] - L.g[3][t]);
@@ -973,36 +1004,22 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     cw[t] = p0 * gcmin;
   }
   __syncthreads();
-  if (!series_inv_given(M1, M2, L.wmax, HS_TOL, nprod)) return false;  // T in M2
-  mm<KUNR_DBL>(M3, M2, M0);                                             // R
+  // T = I + G~ with G~ = inv(I - M1) - I into M2; R = M3 T into M0
+  if (!series_inv_f32(M1, M2, L.wmax, NEU_TOL_F, nprod)) return false;
+  mmf(M3, false, M2, M3, M0, nullptr);
   nprod++;
   // S+ = g+ - R g-, S- = -T g-: linear source (groups 0, 1), constant (2, 3)
-  float src;
+  float *sR = M0, *sT = M2, *sG = M1, *sX = M3;
+  float(*sv)[NN] = L.v;
   {
     const int g = vl_group(), i = vl_row();
-    const double *A = (g & 1) ? M2 : M0;
-    const double mv = mv_vl(A, g < 2 ? cv : cw, i);
+    const double *gm = g < 2 ? cv : cw;
+    const double mv = (g & 1) ? gm[i] + mvfd_pair(sT, gm, i) : mvfd_pair(sR, gm, i);
     const double gcplus = 0.5 * (L.g[0][i] + L.g[1][i]), grplus = 0.5 * (L.g[2][i] + L.g[3][i]);
     const double gplus = p0 * (g < 2 ? gcplus + linfac * grplus : gcplus);
-    src = (float)((g & 1) ? -mv : gplus - mv);
+    const float src = (float)((g & 1) ? -mv : gplus - mv);
+    if (vl_lead()) sv[g][i] = src;
   }
-  float rr[NN2 / RT_WG], tt[NN2 / RT_WG];
-#pragma unroll
-  for (int q = 0; q < NN2 / RT_WG; q++) {
-    const int e = t + q * RT_WG, i = e / NN, j = e % NN;
-    rr[q] = (float)M0[i * LD + j];
-    tt[q] = (float)(M2[i * LD + j] - (i == j ? 1.0 : 0.0));
-  }
-  __syncthreads();
-  float *sR = L.u.f.R, *sT = L.u.f.T, *sG = L.u.f.G, *sX = L.u.f.X;
-  float(*sv)[NN] = L.u.f.v;
-#pragma unroll
-  for (int q = 0; q < NN2 / RT_WG; q++) {
-    const int e = t + q * RT_WG, i = e / NN, j = e % NN;
-    sR[i * LD + j] = rr[q];
-    sT[i * LD + j] = tt[q];
-  }
-  if (vl_lead()) sv[vl_group()][vl_row()] = src;
   __syncthreads();
   for (int it = 0; it < nd; it++) {
     if (!series_f32(sR, sX, sG, L.wmax, nprod)) return false;
@@ -1050,8 +1067,10 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   return true;
 }
 
+// At N = 32 one workgroup per unit fits seven per compute unit in LDS
+// (21 KB each); the register budget follows.
 template <bool PERS>
-__global__ void __launch_bounds__(RT_WG) k_doubling_mixed(Dims d, Settings s, const double *freqs,
+__global__ void __launch_bounds__(RT_WG) __attribute__((amdgpu_waves_per_eu((NN == 32 && !PERS) ? 7 : 1))) k_doubling_mixed(Dims d, Settings s, const double *freqs,
                                                           Work w) {
   __shared__ MixLds L;
   const int n = d.nitems * d.nf;
