@@ -56,6 +56,7 @@ constexpr int KUNR_DBL = (NN == 32) ? NN / 4 : 8;
 // ------------------------------------------------------------------
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
 
 // NaN-propagating maximum (a NaN anywhere wins)
 __device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
@@ -201,6 +202,114 @@ __device__ __forceinline__ double mv_row(const double *A, const double *x, int i
   return s;
 }
 
+// The same helpers in FP32 (the mixed-precision adding), on the FP32 matrix
+// cores.  Layout of the 16x16x4 f32 MFMA: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] and holds C[4*(l/16)+r][l%16], r = 0..3.
+template <int U = KUNR>
+__device__ __forceinline__ float mm_core(const float *A, const float *B, const float *D, float *C,
+                                         float *wmax) {
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int li = l & 15, lk = l >> 4;
+  f4_t acc[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[q][r] = D ? D[(I + 4 * lk + r) * LD + J + li] : 0.f;
+  }
+#pragma unroll U
+  for (int s = 0; s < NN / 4; s++) {
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+      const float a = A[(I + li) * LD + 4 * s + lk];
+      const float b = B[(4 * s + lk) * LD + J + li];
+      acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[q], 0, 0, 0);
+    }
+  }
+  __syncthreads();
+  float m = 0.f;
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) C[(I + 4 * lk + r) * LD + J + li] = acc[q][r];
+    m = max_nan(m, max_nan(max_nan(fabsf(acc[q][0]), fabsf(acc[q][1])),
+                           max_nan(fabsf(acc[q][2]), fabsf(acc[q][3]))));
+  }
+  if (wmax) {
+    const float mf = wave_max(m);
+    if (l == 0) wmax[wv] = mf;
+  }
+  __syncthreads();
+  return wmax ? wmax_all(wmax) : 0.f;
+}
+
+template <int U = KUNR>
+__device__ __forceinline__ float mm2(const float *A1, const float *B1, const float *D1, float *C1,
+                                     const float *A2, const float *B2, const float *D2, float *C2,
+                                     float *wmax) {
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int li = l & 15, lk = l >> 4;
+  f4_t acc1[TPW], acc2[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      acc1[q][r] = D1 ? D1[(I + 4 * lk + r) * LD + J + li] : 0.f;
+      acc2[q][r] = D2 ? D2[(I + 4 * lk + r) * LD + J + li] : 0.f;
+    }
+  }
+#pragma unroll U
+  for (int s = 0; s < NN / 4; s++) {
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+      const float a1 = A1[(I + li) * LD + 4 * s + lk], b1 = B1[(4 * s + lk) * LD + J + li];
+      const float a2 = A2[(I + li) * LD + 4 * s + lk], b2 = B2[(4 * s + lk) * LD + J + li];
+      acc1[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc1[q], 0, 0, 0);
+      acc2[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc2[q], 0, 0, 0);
+    }
+  }
+  __syncthreads();
+  float m = 0.f;
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      C1[(I + 4 * lk + r) * LD + J + li] = acc1[q][r];
+      C2[(I + 4 * lk + r) * LD + J + li] = acc2[q][r];
+    }
+    m = max_nan(m, max_nan(max_nan(fabsf(acc2[q][0]), fabsf(acc2[q][1])),
+                           max_nan(fabsf(acc2[q][2]), fabsf(acc2[q][3]))));
+  }
+  if (wmax) {
+    const float mf = wave_max(m);
+    if (l == 0) wmax[wv] = mf;
+  }
+  __syncthreads();
+  return wmax ? wmax_all(wmax) : 0.f;
+}
+
+template <int U = KUNR>
+__device__ __forceinline__ void mm(const float *A, const float *B, float *C) {
+  mm_core<U>(A, B, nullptr, C, nullptr);
+}
+
+template <int U = KUNR>
+__device__ __forceinline__ void mm_add(const float *D, const float *A, const float *B, float *C) {
+  mm_core<U>(A, B, D, C, nullptr);
+}
+
+__device__ __forceinline__ float mv_row(const float *A, const float *x, int i) {
+  float s = 0.f;
+#pragma unroll 8
+  for (int k = 0; k < NN; k++) s += A[i * LD + k] * x[k];
+  return s;
+}
+
 // Row i of A x for a 32-vector, split over the two lanes t and t^1 (lane h
 // sums the columns 8h..8h+7 and 16+8h..16+8h+7, so the two lanes of a pair
 // read LDS banks 16 apart); both lanes return the full sum.  No barrier.
@@ -331,27 +440,34 @@ constexpr int NEU_MAXIT = 20;
 // of the doubling, where X is ~1e-12).  Returns false if it has not
 // converged after NEU_MAXIT factors (spectral radius within ~1e-6 of one) or
 // X holds a NaN.  np counts the 32x32 products executed.
-template <int U = KUNR>
-__device__ __forceinline__ bool series_inv(const double *A, const double *B, double *X, double *G,
-                                           InvLds &sh, int &np) {
+// In FP32 (T = float, the mixed-precision adding) the series stops at
+// 2^-28, below the single-precision resolution of G.
+template <typename T> struct SeriesTol;
+template <> struct SeriesTol<double> { static constexpr double v = NEU_TOL; };
+template <> struct SeriesTol<float> { static constexpr float v = 0x1p-28f; };
+
+template <int U = KUNR, typename T>
+__device__ __forceinline__ bool series_inv(const T *A, const T *B, T *X, T *G, InvLds &sh,
+                                           int &np) {
+  constexpr T tol = SeriesTol<T>::v;
   const int t = threadIdx.x;
   np++;
-  double m = mm_core<U>(A, B, nullptr, X, sh.wmax);
+  T m = mm_core<U>(A, B, nullptr, X, sh.wmax);
   for (int e = t; e < NN2; e += RT_WG) {
     const int i = e / NN, j = e % NN;
-    G[i * LD + j] = (i == j ? 1.0 : 0.0) + X[i * LD + j];
+    G[i * LD + j] = (i == j ? (T)1.0 : (T)0.0) + X[i * LD + j];
   }
   __syncthreads();
   // G = I + P with P = X; then per factor: G <- G (I + P), P <- P^2, the
   // two products fused while the next factor still matters
-  if (!(m < 0x1p64)) return false;          // NaN or overflow
-  if ((double)NN * m * m < NEU_TOL) return true;  // bound on max |P^2|
+  if (!(m < (T)0x1p64)) return false;          // NaN or overflow
+  if ((T)NN * m * m < tol) return true;  // bound on max |P^2|
   m = mm_core<U>(X, X, nullptr, X, sh.wmax);
   np++;
   for (int it = 0; it < NEU_MAXIT; it++) {
-    if (!(m < 0x1p64)) return false;
-    if (m < NEU_TOL) return true;
-    if ((double)NN * m * m < NEU_TOL) {  // last factor
+    if (!(m < (T)0x1p64)) return false;
+    if (m < tol) return true;
+    if ((T)NN * m * m < tol) {  // last factor
       mm_add<U>(G, G, X, G);
       np++;
       return true;
@@ -367,9 +483,10 @@ __device__ __forceinline__ bool series_inv(const double *A, const double *B, dou
 // queue pass): the series, and Gauss-Jordan with partial pivoting where the
 // series does not converge (Gauss-Jordan only when sh.gj_only); false means
 // a singular matrix.
-template <bool GJ, int U = KUNR>
-__device__ __forceinline__ bool inv_i_minus_ab(const double *A, const double *B, double *X,
-                                               double *G, InvLds &sh, int &np) {
+template <bool GJ, int U = KUNR, typename T>
+__device__ __forceinline__ bool inv_i_minus_ab(const T *A, const T *B, T *X, T *G, InvLds &sh,
+                                               int &np) {
+  static_assert(!GJ || sizeof(T) == 8, "the Gauss-Jordan pass runs in FP64");
   if constexpr (GJ) {
     if (!sh.gj_only && series_inv<U>(A, B, X, G, sh, np)) return true;
     np++;
@@ -589,8 +706,6 @@ __global__ void __launch_bounds__(RT_WG) k_doubling_gj(Dims d, Settings s, const
 // the start layer (a higher-order start, see doubling_unit_f32) and the
 // doubling count are computed in FP64; the layer operators are written back
 // in FP64 for the adding kernel.
-
-typedef float f4_t __attribute__((ext_vector_type(4)));
 
 // C = D + (A + iA I) B on the FP32 matrix cores.  16x16x4 f32 layout: lane l
 // supplies A[l%16][l/16] and B[l/16][l%16] and holds D[4*(l/16)+r][l%16];
@@ -1091,18 +1206,21 @@ __global__ void __launch_bounds__(RT_WG) __attribute__((amdgpu_waves_per_eu((NN 
 
 // operator of the layer stack above the current level: R+, R-, T+, T- (LDS
 // matrices) and S+, S- (LDS vectors)
-struct Stack {
-  double *Rp, *Rm, *Tp, *Tm, *Sp, *Sm;
+template <typename T>
+struct StackT {
+  T *Rp, *Rm, *Tp, *Tm, *Sp, *Sm;
 };
+typedef StackT<double> Stack;
 
 // Row i = t/RPL of A x on all 256 threads: lane t%RPL sums the NN/RPL
 // columns from (NN/RPL)(t%RPL) on (4 at N = 32, conflict-free LDS banks; 16
 // at N = 64), the RPL lanes of a row combine by shuffles; all of them return
 // the full sum.  No barrier.
 constexpr int MVC = NN / RPL;
-__device__ __forceinline__ double mv_row8(const double *A, const double *x, int i) {
+template <typename T>
+__device__ __forceinline__ T mv_row8(const T *A, const T *x, int i) {
   const int c0 = MVC * (threadIdx.x % RPL);
-  double s = A[i * LD + c0] * x[c0];
+  T s = A[i * LD + c0] * x[c0];
 #pragma unroll
   for (int k = 1; k < MVC; k++) s += A[i * LD + c0 + k] * x[c0 + k];
 #pragma unroll
@@ -1114,76 +1232,12 @@ __device__ __forceinline__ bool r8_lead() { return (threadIdx.x % RPL) == 0; }
 
 // v <- A v on all threads, with barriers so that the shared vector may be
 // reused between stages
-__device__ __forceinline__ void mv_stage(const double *A, double *v) {
-  const double u = mv_row8(A, v, r8_row());
+template <typename T>
+__device__ __forceinline__ void mv_stage(const T *A, T *v) {
+  const T u = mv_row8(A, v, r8_row());
   __syncthreads();
   if (r8_lead()) v[r8_row()] = u;
   __syncthreads();
-}
-
-// Two independent products C = D + A B on the FP32 matrix cores with FP64
-// operands and results (the mixed-precision adding): operands are rounded to
-// single precision as they are loaded, products accumulate in FP32, and D is
-// added in FP64.  Layout of 16x16x4 f32: lane l holds C[4*(l/16)+r][l%16].
-// All operands are read before the first write (outputs may alias inputs).
-__device__ __forceinline__ void mm2x(const double *A1, const double *B1, const double *D1,
-                                     double *C1, const double *A2, const double *B2,
-                                     const double *D2, double *C2) {
-  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
-  const int li = l & 15, lk = l >> 4;
-  f4_t acc1[TPW], acc2[TPW];
-#pragma unroll
-  for (int q = 0; q < TPW; q++) {
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      acc1[q][r] = 0.f;
-      acc2[q][r] = 0.f;
-    }
-  }
-#pragma unroll KUNR
-  for (int s = 0; s < NN / 4; s++) {
-    const int kc = 4 * s + lk;
-#pragma unroll
-    for (int q = 0; q < TPW; q++) {
-      const int I = tile_i(wv, q), J = tile_j(wv, q);
-      const float a1 = (float)A1[(I + li) * LD + kc], a2 = (float)A2[(I + li) * LD + kc];
-      const float b1 = (float)B1[kc * LD + J + li], b2 = (float)B2[kc * LD + J + li];
-      acc1[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc1[q], 0, 0, 0);
-      acc2[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, b2, acc2[q], 0, 0, 0);
-    }
-  }
-  double d1[TPW][4], d2[TPW][4];
-#pragma unroll
-  for (int q = 0; q < TPW; q++) {
-    const int I = tile_i(wv, q), J = tile_j(wv, q);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      d1[q][r] = D1 ? D1[(I + 4 * lk + r) * LD + J + li] : 0.0;
-      d2[q][r] = D2 ? D2[(I + 4 * lk + r) * LD + J + li] : 0.0;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < TPW; q++) {
-    const int I = tile_i(wv, q), J = tile_j(wv, q);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      C1[(I + 4 * lk + r) * LD + J + li] = d1[q][r] + (double)acc1[q][r];
-      C2[(I + 4 * lk + r) * LD + J + li] = d2[q][r] + (double)acc2[q][r];
-    }
-  }
-  __syncthreads();
-}
-
-// mm2 in FP64, or mm2x in the mixed path (MX)
-template <bool MX>
-__device__ __forceinline__ void mm2p(const double *A1, const double *B1, const double *D1,
-                                     double *C1, const double *A2, const double *B2,
-                                     const double *D2, double *C2) {
-  if constexpr (MX)
-    mm2x(A1, B1, D1, C1, A2, B2, D2, C2);
-  else
-    mm2(A1, B1, D1, C1, A2, B2, D2, C2, nullptr);
 }
 
 // COMBINE_LAYERS4 (radintg4.f90:447-513): stack on top of a doubled layer
@@ -1192,31 +1246,31 @@ __device__ __forceinline__ void mm2p(const double *A1, const double *B1, const d
 //   A = T1m G,  R1m' = R1m + A (R2 T1p),  T1m' = A T2,  S1m' = S1m + A (...)
 //   B = T2 G',  R1p' = R2 + B (R1p T2),   T1p' = B T1p,  S1p' = S2p + B (...)
 // four products in two fused passes per side (the reference's order of the
-// products differs only by association).  MX: the mixed path's FP32
-// products (the inverses stay FP64).
-template <bool GJ, bool MX>
-__device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, const double *S2p,
-                             const double *S2m, double *G, double *X, double *vs, InvLds &sinv, int &np) {
+// products differs only by association).  T = float: the mixed path, every
+// operand and product in FP32.
+template <bool GJ, typename T>
+__device__ bool combine_scat(StackT<T> &s1, const T *R2, const T *T2, const T *S2p, const T *S2m,
+                             T *G, T *X, T *vs, InvLds &sinv, int &np) {
   const int t = threadIdx.x;
   if (!inv_i_minus_ab<GJ>(R2, s1.Rp, X, G, sinv, np)) return false;
   {
-    const double u = mv_row8(R2, s1.Sp, r8_row());
+    const T u = mv_row8(R2, s1.Sp, r8_row());
     if (r8_lead()) vs[r8_row()] = S2m[r8_row()] + u;  // S2m + R2m S1p
   }
-  mm2p<MX>(s1.Tm, G, nullptr, X, R2, s1.Tp, nullptr, G);  // X = A, G = R2 T1p
+  mm2(s1.Tm, G, nullptr, X, R2, s1.Tp, nullptr, G, nullptr);  // X = A, G = R2 T1p
   mv_stage(X, vs);
   if (t < NN) s1.Sm[t] = s1.Sm[t] + vs[t];                     // S1m + A (...)
-  mm2p<MX>(X, G, s1.Rm, s1.Rm, X, T2, nullptr, s1.Tm);      // R1m', T1m'
+  mm2(X, G, s1.Rm, s1.Rm, X, T2, nullptr, s1.Tm, nullptr);  // R1m', T1m'
   np += 4;
   if (!inv_i_minus_ab<GJ>(s1.Rp, R2, X, G, sinv, np)) return false;
   {
-    const double u = mv_row8(s1.Rp, S2m, r8_row());
+    const T u = mv_row8(s1.Rp, S2m, r8_row());
     if (r8_lead()) vs[r8_row()] = s1.Sp[r8_row()] + u;  // S1p + R1p S2m
   }
-  mm2p<MX>(T2, G, nullptr, X, s1.Rp, T2, nullptr, G);       // X = B, G = R1p T2
+  mm2(T2, G, nullptr, X, s1.Rp, T2, nullptr, G, nullptr);  // X = B, G = R1p T2
   mv_stage(X, vs);
   if (t < NN) s1.Sp[t] = S2p[t] + vs[t];                       // S2p + B (...)
-  mm2p<MX>(X, G, R2, s1.Rp, X, s1.Tp, nullptr, s1.Tp);      // R1p', T1p'
+  mm2(X, G, R2, s1.Rp, X, s1.Tp, nullptr, s1.Tp, nullptr);  // R1p', T1p'
   np += 4;
   __syncthreads();
   return true;
@@ -1225,10 +1279,11 @@ __device__ bool combine_scat(Stack &s1, const double *R2, const double *T2, cons
 // The same combine for a clear layer (NONSCATTER_LAYER4: R2 = 0,
 // T2 = diag(dg)).  With R2 = 0 both gammas are the identity and every
 // product reduces to a diagonal scaling or one matrix-vector product.
-__device__ void combine_clear(Stack &s1, const double *dg, const double *S2p, const double *S2m) {
+template <typename T>
+__device__ void combine_clear(StackT<T> &s1, const T *dg, const T *S2p, const T *S2m) {
   const int t = threadIdx.x, i8 = r8_row();
-  const double u = mv_row8(s1.Tm, S2m, i8);                // T1m S2m
-  const double u2 = s1.Sp[i8] + mv_row8(s1.Rp, S2m, i8);  // S1p + R1p S2m
+  const T u = mv_row8(s1.Tm, S2m, i8);                // T1m S2m
+  const T u2 = s1.Sp[i8] + mv_row8(s1.Rp, S2m, i8);  // S1p + R1p S2m
   __syncthreads();
   if (r8_lead()) {
     s1.Sm[i8] = s1.Sm[i8] + u;
@@ -1245,13 +1300,13 @@ __device__ void combine_clear(Stack &s1, const double *dg, const double *S2p, co
 
 // The combine with the surface "layer" of get_sfc_matrices
 // (sfc_matrices.f90:21-194): R2+ = 0, R2- = Rs, T2 = I, S2 = 0.
-template <bool GJ>
-__device__ bool combine_surface(Stack &s1, const double *Rs, double *G, double *X, double *vs,
-                                InvLds &sinv, int &np) {
+template <bool GJ, typename T>
+__device__ bool combine_surface(StackT<T> &s1, const T *Rs, T *G, T *X, T *vs, InvLds &sinv,
+                                int &np) {
   const int t = threadIdx.x;
   if (!inv_i_minus_ab<GJ>(Rs, s1.Rp, X, G, sinv, np)) return false;
   {
-    const double u = mv_row8(Rs, s1.Sp, r8_row());
+    const T u = mv_row8(Rs, s1.Sp, r8_row());
     if (r8_lead()) vs[r8_row()] = u;
   }
   __syncthreads();
@@ -1325,8 +1380,9 @@ __device__ __forceinline__ void fetch_layer(const Dims &d, const Work &w, int co
 // Stages a fetched layer: a doubled layer into (R2, T2, S2p, S2m), or the
 // closed form of a clear layer into (dg, S2p, S2m).  Returns true when the
 // layer scatters.
-__device__ bool stage_layer(const LayerFetch &F, double wl, const double *smu, double *R2,
-                            double *T2, double *dg, double *S2p, double *S2m, int *sbad) {
+template <typename T>
+__device__ bool stage_layer(const LayerFetch &F, double wl, const double *smu, T *R2, T *T2, T *dg,
+                            T *S2p, T *S2m, int *sbad) {
   const int t = threadIdx.x;
   if (F.itf >= 0) {
     if (t == 0 && (F.st & 0xff)) atomicMax(sbad, F.st);
@@ -1377,13 +1433,12 @@ __device__ bool stage_layer(const LayerFetch &F, double wl, const double *smu, d
 
 // INTERNAL_RADIANCE4 at the surface level: up = stack of all layers, down =
 // the surface (R- = Rs, T- = I, S- = 0)
-template <bool GJ>
-__device__ bool radiance_at_surface(const Stack &up, const double *Rs, const double *sky,
-                                    const double *gnd, double *G, double *X, double *vs,
-                                    double *ur, double *dr, InvLds &sinv, int &np) {
+template <bool GJ, typename T>
+__device__ bool radiance_at_surface(const StackT<T> &up, const T *Rs, const T *sky, const T *gnd,
+                                    T *G, T *X, T *vs, T *ur, T *dr, InvLds &sinv, int &np) {
   const int t = threadIdx.x;
   if (!inv_i_minus_ab<GJ>(up.Rp, Rs, X, G, sinv, np)) return false;
-  double s_ = 0.0;
+  T s_ = 0.0;
   if (t < NN) {
     s_ = mv_row(up.Rp, gnd, t);          // UPR (DNT INBOT)
     s_ = mv_row(up.Tp, sky, t) + s_;     // + UPT INTOP
@@ -1414,12 +1469,12 @@ __device__ bool radiance_at_surface(const Stack &up, const double *Rs, const dou
 
 // INTERNAL_RADIANCE4 at the top: up = nothing (R = 0, T = I, S = 0, both
 // gammas the identity), down = stack including the surface
-__device__ void radiance_at_top(const Stack &dn, const double *sky, const double *gnd,
-                                double *ur, double *dr) {
+template <typename T>
+__device__ void radiance_at_top(const StackT<T> &dn, const T *sky, const T *gnd, T *ur, T *dr) {
   const int t = threadIdx.x;
   if (t < NN) {
     dr[t] = sky[t];
-    double s_ = mv_row(dn.Rm, sky, t);
+    T s_ = mv_row(dn.Rm, sky, t);
     s_ = mv_row(dn.Tm, gnd, t) + s_;
     s_ = 0.0 + s_;
     ur[t] = dn.Sm[t] + s_;
@@ -1429,8 +1484,9 @@ __device__ void radiance_at_top(const Stack &dn, const double *sky, const double
 
 // CONVERT_OUTPUT + collect_output (rt_utilities.f90:687-737,
 // collect_output.f90:31-88) for one output level
+template <typename T>
 __device__ void write_tb(const Dims &d, const Settings &s, int col, int f, int io, double wl,
-                         const double *ur, const double *dr, double *tb) {
+                         const T *ur, const T *dr, double *tb) {
   const int t = threadIdx.x;
   if (t < NMU) {
     const int j = t;
@@ -1460,8 +1516,9 @@ __device__ void write_tb(const Dims &d, const Settings &s, int col, int f, int i
 // The surface's reflection matrix R- (get_sfc_matrices, sfc_matrices.f90:88-194):
 // specular/Fresnel 2x2 blocks on the diagonal, or the rank-one Lambertian
 // 2 r mu_j w_j on the I component
+template <typename T>
 __device__ void surface_matrix(char refl, const double *Rf, const double *smu, const double *sqw,
-                               double *Rs) {
+                               T *Rs) {
   for (int e = threadIdx.x; e < NN2; e += RT_WG) {
     const int r = e / NN, c = e % NN;
     const int i2 = r % NST, j2 = r / NST, i1 = c % NST, j1 = c / NST;
@@ -1481,8 +1538,9 @@ __device__ void surface_matrix(char refl, const double *Rf, const double *smu, c
 // The radiances entering the column: sky (blackbody at SKY_TEMP, I only)
 // at the top and the surface's emission at the bottom (thermal_radiance,
 // surface.f90:169-195; get_sfc_matrices for the specular/Lambertian emission)
+template <typename T>
 __device__ void boundary_radiances(char refl, const double *E, const double *Rf, double psky,
-                                   double pg, double *sky, double *gnd) {
+                                   double pg, T *sky, T *gnd) {
   const int t = threadIdx.x;
   if (t < NN) {
     const int i = t % NST, j = t / NST;
@@ -1507,28 +1565,38 @@ constexpr int ADD_LDS = (NN == 32) ? 8 : 4;   // of the 8 matrices of k_adding
 constexpr int LEV_LDS = (NN == 32) ? 10 : 4;  // of the 10 of k_adding_levels
 static_assert(8 - ADD_LDS <= RT_GMATS && 10 - LEV_LDS <= RT_GMATS, "scratch per slot");
 
-__device__ __forceinline__ double *add_mat(double (*lds)[MSZ], int nlds, const Work &w, int k) {
+template <typename T>
+__device__ __forceinline__ T *add_mat(T (*lds)[MSZ], int nlds, const Work &w, int k) {
   if (k < nlds) return lds[k];
-  return w.rt_scratch + ((size_t)blockIdx.x * RT_GMATS + (k - nlds)) * MSZ;
+  return reinterpret_cast<T *>(w.rt_scratch + ((size_t)blockIdx.x * RT_GMATS + (k - nlds)) * MSZ);
 }
 
-struct AddLds {
-  double M[ADD_LDS][MSZ];
-  double v[10][NN];
+// In FP32 (the mixed path) all eight matrices live in LDS at either N
+// (34 KB at N = 32, 133 KB at N = 64).
+template <typename T> constexpr int add_lds() { return sizeof(T) == 4 ? 8 : ADD_LDS; }
+
+template <typename T>
+struct AddLdsT {
+  T M[add_lds<T>()][MSZ];
+  T v[10][NN];
   InvLds inv;
   double mu[NMU], qw[NMU];
   int bad[1];
   int slot;  // unit index of a persistent workgroup
 };
+typedef AddLdsT<double> AddLds;
 
 // One (column, channel) unit of the chunk.  Returns false only in the
 // series variant (GJ = false) when an inverse did not converge; the unit is
-// then recomputed from scratch by the Gauss-Jordan pass (in FP64).  MX: the
-// combines' products on the FP32 matrix cores (mixed precision).
-template <bool GJ, bool MX = false>
+// then recomputed from scratch by the Gauss-Jordan pass (in FP64).
+// T = float: the mixed path's adding, with the stack, the layer operators,
+// the products, the inverse series and the radiances in FP32.
+template <bool GJ, typename T = double>
 __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freqs,
-                            const char *sfc_refl, const Work &w, double *tb, int cf, AddLds &LL) {
-  double(*sv)[NN] = LL.v;
+                            const char *sfc_refl, const Work &w, double *tb, int cf,
+                            AddLdsT<T> &LL) {
+  constexpr int NL = add_lds<T>();
+  T(*sv)[NN] = LL.v;
   InvLds &sinv = LL.inv;
   double *smu = LL.mu, *sqw = LL.qw;
   int *sbad = LL.bad;
@@ -1588,12 +1656,12 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
     }
     return true;
   }
-  Stack stk = {add_mat(LL.M, ADD_LDS, w, 0), add_mat(LL.M, ADD_LDS, w, 1),
-               add_mat(LL.M, ADD_LDS, w, 2), add_mat(LL.M, ADD_LDS, w, 3), sv[0], sv[1]};
-  double *R2 = add_mat(LL.M, ADD_LDS, w, 4), *T2 = add_mat(LL.M, ADD_LDS, w, 5);
-  double *G = add_mat(LL.M, ADD_LDS, w, 6), *X = add_mat(LL.M, ADD_LDS, w, 7);
-  double *S2p = sv[2], *S2m = sv[3], *dg = sv[4], *vs = sv[5];
-  double *sky = sv[6], *gnd = sv[7], *ur = sv[8], *dr = sv[9];
+  StackT<T> stk = {add_mat(LL.M, NL, w, 0), add_mat(LL.M, NL, w, 1), add_mat(LL.M, NL, w, 2),
+                   add_mat(LL.M, NL, w, 3), sv[0], sv[1]};
+  T *R2 = add_mat(LL.M, NL, w, 4), *T2 = add_mat(LL.M, NL, w, 5);
+  T *G = add_mat(LL.M, NL, w, 6), *X = add_mat(LL.M, NL, w, 7);
+  T *S2p = sv[2], *S2m = sv[3], *dg = sv[4], *vs = sv[5];
+  T *sky = sv[6], *gnd = sv[7], *ur = sv[8], *dr = sv[9];
   const double psky = planck(SKY_TEMP, wl);
   const double pg = planck(w.groundtemp[col], wl);
   const double *E = w.sfc_emis + ((size_t)col * d.nf + f) * 2 * NMU;
@@ -1703,7 +1771,7 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
       }
       __syncthreads();
     } else if (sc) {
-      if (!combine_scat<GJ, MX>(stk, R2, T2, S2p, S2m, G, X, vs, sinv, nprod)) {
+      if (!combine_scat<GJ>(stk, R2, T2, S2p, S2m, G, X, vs, sinv, nprod)) {
         if (!GJ) return false;
         fail(E_SINGULAR | PAM_FATAL);
         return true;
@@ -1750,16 +1818,18 @@ __device__ bool adding_unit(const Dims &d, const Settings &s, const double *freq
   return true;
 }
 
-template <bool PERS, bool MX>
-__global__ void __launch_bounds__(RT_WG) k_adding(Dims d, Settings s, const double *freqs,
+// T = float: the mixed path (at N = 32, 36 KB of LDS: four workgroups per
+// compute unit, and the register budget that follows)
+template <bool PERS, typename T>
+__global__ void __launch_bounds__(RT_WG) __attribute__((amdgpu_waves_per_eu((NN == 32 && sizeof(T) == 4) ? 4 : 1))) k_adding(Dims d, Settings s, const double *freqs,
                                                   const char *sfc_refl, Work w, double *tb) {
-  __shared__ AddLds LL;
+  __shared__ AddLdsT<T> LL;
   const int n = d.ncol_chunk * d.nf;
   int cf = PERS ? next_unit(&w.queue[2], &LL.slot) : (int)blockIdx.x;
   while (cf < n) {
     // inverse_gj == 2 (a test mode) queues every unit for the Gauss-Jordan pass
     const bool done =
-        s.inverse_gj != 2 && adding_unit<false, MX>(d, s, freqs, sfc_refl, w, tb, cf, LL);
+        s.inverse_gj != 2 && adding_unit<false, T>(d, s, freqs, sfc_refl, w, tb, cf, LL);
     if (!done && threadIdx.x == 0) {
       const unsigned q = atomicAdd(&w.redo_n[1], 1u);
       w.redo_add[q] = cf;
@@ -1852,7 +1922,7 @@ __device__ bool fold_layers(const Dims &d, const Work &w, int col, int f, int L,
       }
       __syncthreads();
     } else if (sc) {
-      if (!combine_scat<true, false>(stk, R2, T2, S2p, S2m, G, X, vs, sinv, np)) return false;
+      if (!combine_scat<true>(stk, R2, T2, S2p, S2m, G, X, vs, sinv, np)) return false;
       nscat++;
     } else {
       combine_clear(stk, dg, S2p, S2m);
@@ -2030,16 +2100,16 @@ void launch_adding(const Dims &d, const Settings &s, const double *freqs, const 
     if (RT_GMATS > 0) grid.x = (unsigned)std::min(n, slots);
     const bool pers = grid.x < n, mx = s.precision == PAM_PREC_MIXED;
     if (pers && mx)
-      hipLaunchKernelGGL((k_adding<true, true>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
+      hipLaunchKernelGGL((k_adding<true, float>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
                          tb);
     else if (pers)
-      hipLaunchKernelGGL((k_adding<true, false>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
+      hipLaunchKernelGGL((k_adding<true, double>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
                          tb);
     else if (mx)
-      hipLaunchKernelGGL((k_adding<false, true>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
+      hipLaunchKernelGGL((k_adding<false, float>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
                          tb);
     else
-      hipLaunchKernelGGL((k_adding<false, false>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
+      hipLaunchKernelGGL((k_adding<false, double>), grid, dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w,
                          tb);
     hipLaunchKernelGGL(k_adding_gj, dim3((unsigned)std::min<size_t>(n, std::min<size_t>(2048, slots))),
                        dim3(RT_WG), 0, st, d, s, freqs, sfc_refl, w, tb, 0);
