@@ -512,7 +512,8 @@ static size_t col_bytes(const Dims &d) {
 
 static size_t item_bytes_per(const Dims &d) {
   size_t per = 2 * 4 + (size_t)d.nh * d.nbmax * 4 * 8 + (size_t)d.nh * (4 + 8);
-  per += (size_t)d.nf * (3 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8 + 4 + 8);
+  per += (size_t)d.nf * (3 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8 + 4 + 8 +
+                         4 * 8);
   return per;
 }
 
@@ -542,6 +543,7 @@ static void carve_items(Work &w, void *base, const Dims &d, size_t nitems) {
   w.redo_dbl = cv.take<int32_t>(nif);
   w.ph_rank = cv.take<int32_t>(nif);
   w.ph_perm = cv.take<int32_t>(nif);
+  w.mie_m = cv.take<double>(nif * 4);
 }
 
 extern "C" int64_t pam_workspace_bytes(int64_t ncol, int32_t max_nlyrs, int32_t nhydro,

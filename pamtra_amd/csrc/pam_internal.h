@@ -159,6 +159,7 @@ struct Work {
   // Mie angle tables per series length N (see build_angle_tables)
   const double *atab;
   const float *atab_f;                 // the same tables rounded to FP32
+  double *mie_m;                       // per unit: water, ice refractive index (re, im)
   const int64_t *atab_off;             // [atab_nmax + 1]
   int atab_nmax;
   // phase-matrix tables per NUMPTS (see build_phase_tables) and the units

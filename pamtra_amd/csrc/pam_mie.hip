@@ -1219,7 +1219,6 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
   const int li_item = itf / d.nf, f = itf % d.nf;
   const int col = w.item_col[li_item], z = w.item_layer[li_item];
   int32_t *st = w.itf_status + itf;
-  const size_t oy = (size_t)col * d.ML + z;
   const double freq = freqs[f];
   const double wavelength = C_LIGHT / (freq * 1e9);
   const int ld = w.mie_ld;  // rows of the U slot (16 per 8 bins)
@@ -1227,16 +1226,14 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
   // per hydrometeor, one thread each: bins, refractive index
   // (scatProperties.f90:227-243), series length of the largest bin
   // (mie_spheres.f90:139-158)
+  // (the indices of water and ice come from k_mie_index)
   if (t < d.nh) {
     const int nbin = w.dsd_flag[(size_t)li_item * d.nh + t];
     int nterms = 0;
     cd m = cd{0.0, 0.0};
     if (nbin > 0) {
-      const double T = w.temp[oy];
-      if (hyd[t].liq_ice == 1)
-        m = csqrt_(eps_water_tkc(T - sp(273.15), freq));
-      else
-        m = csqrt_(eps_ice(T, freq));
+      const double *mi = w.mie_m + (size_t)itf * 4 + (hyd[t].liq_ice == 1 ? 0 : 2);
+      m = cd{mi[0], mi[1]};
       const double *dia = w.dsd_d + ((size_t)li_item * d.nh + t) * d.nbmax;
       nterms = mie_nstop(PI * dia[nbin - 1] / wavelength);
     }
@@ -1639,6 +1636,35 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
 
 // Persistent grid of the coefficient-first Mie: workgroup b owns slot b of
 // the U scratch and takes units from the work queue until it is drained.
+// Refractive indices of liquid water and ice (scatProperties.f90:227-243)
+// per (layer, channel) unit, for the phases present in it: one thread per
+// unit, so that their long serial evaluation overlaps across the whole grid
+// instead of holding a workgroup of the coefficient-first kernel.
+__global__ void k_mie_index(Dims d, const pam_hydro_desc *hyd, const double *freqs, Work w) {
+  const int n = d.nitems * d.nf;
+  for (int itf = blockIdx.x * blockDim.x + threadIdx.x; itf < n; itf += gridDim.x * blockDim.x) {
+    const int li_item = itf / d.nf, f = itf % d.nf;
+    const int col = w.item_col[li_item], z = w.item_layer[li_item];
+    bool nw = false, ni = false;
+    for (int h = 0; h < d.nh; h++)
+      if (w.dsd_flag[(size_t)li_item * d.nh + h] > 0) {
+        if (hyd[h].liq_ice == 1)
+          nw = true;
+        else
+          ni = true;
+      }
+    const double T = w.temp[(size_t)col * d.ML + z], freq = freqs[f];
+    cd mw = cd{0.0, 0.0}, mi = cd{0.0, 0.0};
+    if (nw) mw = csqrt_(eps_water_tkc(T - sp(273.15), freq));
+    if (ni) mi = csqrt_(eps_ice(T, freq));
+    double *o = w.mie_m + (size_t)itf * 4;
+    o[0] = mw.re;
+    o[1] = mw.im;
+    o[2] = mi.re;
+    o[3] = mi.im;
+  }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(MIE2_WG) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 3))) k_mie2(Dims d, Settings s, const pam_hydro_desc *hyd,
                                                    const double *freqs, Work w) {
@@ -1674,6 +1700,8 @@ void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, con
   if (n == 0) return;
   if (!s.ssrg && !s.mie_chunked && w.mie_u && w.mie_slots > 0) {
     const unsigned grid = (unsigned)std::min<size_t>(n, (size_t)w.mie_slots);
+    hipLaunchKernelGGL(k_mie_index, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)),
+                       dim3(256), 0, st, d, hyd, freqs, w);
     if (s.precision == PAM_PREC_MIXED && !s.mie_fp64 && w.atab_f)
       hipLaunchKernelGGL(k_mie2<float>, dim3(grid), dim3(MIE2_WG), 0, st, d, s, hyd, freqs, w);
     else
