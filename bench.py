@@ -210,6 +210,9 @@ def large_domain(args, torch):
                 "frac": round(ach / PEAK_FP64_TFLOPS, 4),
                 "mfma_executed_tflops": round(work[9] * 2 * 64 ** 3 / (stage["doubling"] * 1e-3)
                                               / 1e12, 3)}
+        else:
+            res[precision]["doubling_fp32_roofline"] = fp32_doubling_roofline(
+                list(work), stage["doubling"], 64)
     dch = np.max(np.abs(tbs["mixed"] - tbs["fp64"]), axis=(0, 1, 2, 3, 5))
     res["mixed"]["max_abs_dtb_vs_fp64_K"] = {f"{fr:g}": float(v) for fr, v in zip(freqs, dch)}
     res["mixed"]["speedup_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
@@ -219,6 +222,29 @@ def large_domain(args, torch):
     res["full_domain_s_fp64"] = round(1024 * 1024 * nf / res["fp64"]["value"], 1)
     log(f"large domain: {res}")
     return res
+
+
+def fp32_doubling_roofline(work, dbl_ms, n):
+    """The mixed doubling on the FP32 matrix cores: the N x N products it
+    executed (work[9]: start layer, series inverses and doublings), 2 N^3
+    flops each, against the 157.3 TFLOP/s FP32 matrix peak; beside it the
+    reference's count, 14 N^3 per doubling step of the path's own steps."""
+    ex = work[9] * 2 * n ** 3 / (dbl_ms * 1e-3) / 1e12
+    ref = work[3] * 14 * n ** 3 / (dbl_ms * 1e-3) / 1e12
+    return {"achieved": round(ex, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(ex / PEAK_FP32_TFLOPS, 4), "counted": "executed products",
+            "reference_count_tflops": round(ref, 3)}
+
+
+def mie_flops(work):
+    """The Mie stage's flops counted on the reference's algorithm
+    (mie_scat_utilities.f90:8-160, mie_spheres.f90:178-314): per bin the D_n
+    recursion from nt+15 (20 flops a step), 76 flops per term for a_n, b_n
+    and the efficiencies, 28 per term and angle for S1/S2 and the phase sums;
+    per (layer, hydrometeor) 13 per angle and order for the Legendre
+    projection.  Counters: [6] bins, [7] sum of series lengths, [12] sum of
+    nquad * series length, [13] sum of nquad * (nlegen + 1)."""
+    return 20 * (work[7] + 15 * work[6]) + 76 * work[7] + 28 * work[12] + 13 * work[13]
 
 
 def config3_leg(args, torch):
@@ -271,12 +297,8 @@ def config3_leg(args, torch):
                                        for n, v in zip(names, acc)},
                           "status_nonzero": int((status.cpu().numpy() != 0).sum())}
         if precision == "mixed":
-            # FP32 matrix-core rate of the mixed doubling: 14 N^3 flops a step
-            dbl = res[precision]["stage_ms"]["doubling"]
-            ach = work[3] * 14 * NN ** 3 / (dbl * 1e-3) / 1e12
-            res[precision]["doubling_fp32_roofline"] = {
-                "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_FP32_TFLOPS, 4)}
+            res[precision]["doubling_fp32_roofline"] = fp32_doubling_roofline(
+                work, res[precision]["stage_ms"]["doubling"], NN)
     dch = np.max(np.abs(tbs["mixed"] - tbs["fp64"]), axis=(0, 1, 2, 3, 5))
     res["mixed"]["max_abs_dtb_vs_fp64_K"] = {f"{fr:g}": float(v) for fr, v in zip(freqs, dch)}
     res["mixed"]["speedup_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
@@ -507,13 +529,8 @@ def main():
                                          if dom_name in executed else None)}
 
     # The Mie stage against the FP64 peak, counted on the reference's
-    # algorithm (mie_scat_utilities.f90:8-160, mie_spheres.f90:178-314): per
-    # bin the D_n recursion from nt+15 (20 flops a step), 76 flops per term
-    # for a_n, b_n and the efficiencies, 28 per term and angle for S1/S2 and
-    # the phase sums; per (layer, hydrometeor) 13 per angle and order for the
-    # Legendre projection.  Counters: [6] bins, [7] sum of series lengths,
-    # [12] sum of nquad * series length, [13] sum of nquad * (nlegen + 1).
-    f_mie = 20 * (work[7] + 15 * work[6]) + 76 * work[7] + 28 * work[12] + 13 * work[13]
+    # algorithm (mie_flops)
+    f_mie = mie_flops(work)
     mie_ms = stage_ms[names.index("mie")]
     ach_mie = f_mie / (mie_ms * 1e-3) / 1e12
     mie_roof = {"bound": "fp64", "kernel": "mie", "achieved": round(ach_mie, 3),
@@ -555,14 +572,27 @@ def main():
             sms += np.array(ms[:])
         torch.cuda.synchronize()
         dtm = (time.perf_counter() - t0) / args.steps
+        workm = (C.c_int64 * 14)()
+        lib.pam_last_work(ctx, workm, 14)
+        workm = list(workm)
         tbm = tb.cpu().numpy()
         dch = np.max(np.abs(tbm - tb_h), axis=(0, 1, 2, 3, 5))
+        sm = {n: round(v / args.steps, 3) for n, v in zip(names, sms)}
+        f_miem = mie_flops(workm)
         mixed = {"value": round(nx * ny * nf / dtm, 2), "ms_per_step": round(dtm * 1e3, 3),
-                 "stage_ms": {n: round(v / args.steps, 3) for n, v in zip(names, sms)},
+                 "stage_ms": sm,
                  "speedup_vs_fp64": round(dt / dtm, 3),
                  "max_abs_dtb_vs_fp64_K": {f"{fr:g}": float(v) for fr, v in zip(freqs, dch)},
-                 "dtype": "f32 products in the doubling and adding (f64: start layer, adding "
-                          "inverses, Mie, phase matrix)"}
+                 "doubling_fp32_roofline": fp32_doubling_roofline(workm, sm["doubling"], NN),
+                 "mie_roofline": {"bound": "fp32", "achieved": round(
+                     f_miem / (sm["mie"] * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(
+                         f_miem / (sm["mie"] * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                     "flops": f_miem},
+                 "dtype": "f32: Mie series (D_n, a_n, b_n) and S1/S2 products, the doubling "
+                          "(start layer included) and the adding; f64: psi/chi recursion, "
+                          "efficiency and phase sums, Legendre projection, phase matrix, "
+                          "CHECK_NORM4 and the choice of the doubling count"}
         log(f"mixed: {mixed}")
 
     host_rate = None
