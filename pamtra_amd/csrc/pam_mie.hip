@@ -1380,48 +1380,41 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
         const int np = max(4, (nmt + 3) & ~3);
         const int nsk = np >> 2;
         acc_t x0 = {(R)0, (R)0, (R)0, (R)0}, y0 = x0, x1 = x0, y1 = x0;
-        const R *urow = U + 16 * rt + li;
-        // one k-step: both parity groups, operand a from the slot
-        auto kstep = [&](int sk, int g, R a) {
-          int cu, n;
-          bool ap;
-          mie_kmap(g, 4 * sk + lk, np, cu, n, ap);
-          R pv = (R)0, tv = (R)0;
-          if (ang_ok && n <= nterms) {
-            if (tab_lds) {
-              pv = tab[(n - 1) * nh2 + hix];
-              tv = tab[TABN + (n - 1) * nh2 + hix];
-            } else if (tab_tile) {
-              pv = tab[(n - 1) * 16 + li];
-              tv = tab[TABN + (n - 1) * 16 + li];
-            } else {
-              pv = tpi[(size_t)(n - 1) * nquad + ang];
-              tv = ttau[(size_t)(n - 1) * nquad + ang];
-            }
-          }
-          if (g == 0) {
-            x0 = M2::mfma(a, ap ? pv : tv, x0);
-            y0 = M2::mfma(a, ap ? tv : pv, y0);
-          } else {
-            x1 = M2::mfma(a, ap ? pv : tv, x1);
-            y1 = M2::mfma(a, ap ? tv : pv, y1);
-          }
-        };
-        auto uload = [&](int sk, int g) -> R {
-          int cu, n;
-          bool ap;
-          mie_kmap(g, 4 * sk + lk, np, cu, n, ap);
-          const int cg = ap ? n - 1 : nph + n - 1;
-          return (n <= nrow) ? urow[(size_t)cg * ld] : (R)0;
-        };
-        // unrolled so that the operand loads of several k-steps are in
-        // flight together (the tables come from memory for long series)
+        // k-step sk of parity group g: the lanes lk < 2 take a_n, the lanes
+        // lk >= 2 take b_n, each at its own n = 4 sk + c_g (group 0: a_n of
+        // odd n, b_n of even n; group 1 the other way), so that a lane's role
+        // and strides are fixed and the operands of a step are plain loads
+        const bool ap = lk < 2;
+        const int c0 = 2 * (lk & 1) + (ap ? 1 : 2), c1 = 2 * (lk & 1) + (ap ? 2 : 1);
+        const int ub = ap ? 0 : nph;
+        const R *u0 = U + 16 * rt + li + (size_t)(ub + c0 - 1) * ld;
+        const R *u1 = U + 16 * rt + li + (size_t)(ub + c1 - 1) * ld;
+        // pp, pt: c_n pi_n and c_n tau_n of this lane's angle at n = 1, rows
+        // tstride apart; x takes pi_n for a_n and tau_n for b_n, y the other
+        auto run = [&](const R *pp, const R *pt, int tstride) {
+          const R *px = ap ? pp : pt, *py = ap ? pt : pp;
 #pragma unroll 4
-        for (int sk = 0; sk < nsk; sk++) {
-          const R a0 = uload(sk, 0), a1 = uload(sk, 1);
-          kstep(sk, 0, a0);
-          kstep(sk, 1, a1);
-        }
+          for (int sk = 0; sk < nsk; sk++) {
+            const int n0 = 4 * sk + c0, n1 = 4 * sk + c1;
+            const R a0 = n0 <= nrow ? u0[(size_t)(4 * sk) * ld] : (R)0;
+            const R a1 = n1 <= nrow ? u1[(size_t)(4 * sk) * ld] : (R)0;
+            const bool t0 = ang_ok && n0 <= nterms, t1 = ang_ok && n1 <= nterms;
+            const R tx0 = t0 ? px[(n0 - 1) * tstride] : (R)0;
+            const R ty0 = t0 ? py[(n0 - 1) * tstride] : (R)0;
+            const R tx1 = t1 ? px[(n1 - 1) * tstride] : (R)0;
+            const R ty1 = t1 ? py[(n1 - 1) * tstride] : (R)0;
+            x0 = M2::mfma(a0, tx0, x0);
+            y0 = M2::mfma(a0, ty0, y0);
+            x1 = M2::mfma(a1, tx1, x1);
+            y1 = M2::mfma(a1, ty1, y1);
+          }
+        };
+        if (tab_lds)
+          run(tab + hix, tab + TABN + hix, nh2);
+        else if (tab_tile)
+          run(tab + li, tab + TABN + li, 16);
+        else
+          run(tpi + ang, ttau + ang, nquad);
 #pragma unroll
         for (int h2 = 0; h2 < 2; h2++) {
           const int bw = 8 * rt + M2::out_bin(lk, h2);
