@@ -1210,6 +1210,7 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
                                           const Work &w, int itf, R *U, Mie2Lds<R> &L) {
   constexpr bool F32 = sizeof(R) == 4;
   typedef Mie2Rows<R> M2;
+  typedef R PA;  // per-lane phase-sum accumulators
   typedef typename M2::acc_t acc_t;
   // the angle-table stage holds TABN entries of each of c_n pi_n, c_n tau_n
   constexpr int TABN = MIE2_TAB;
@@ -1366,7 +1367,9 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
       const int hix = 16 * ctw + li;
       const bool ang_ok = hix < nh2;
       const int ang = j0 + hix, angm = nquad - 1 - ang;
-      double p1 = 0.0, p2 = 0.0, p3 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+      // the phase sums of a lane (FP32 in the mixed kernel: at most a few
+      // tens of positive-weighted terms per lane before the FP64 sums)
+      PA p1 = 0.0, p2 = 0.0, p3 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
       for (int rt = sub; rt < nrt; rt += wpc) {
         // longest series of the tile's 8 bins, and of this lane's row
         int nmt = 0;
@@ -1418,15 +1421,15 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
 #pragma unroll
         for (int h2 = 0; h2 < 2; h2++) {
           const int bw = 8 * rt + M2::out_bin(lk, h2);
-          const double wgt = (bw < nbin && L.nmb[bw] > 0) ? nd[bw] * dd[bw] : 0.0;
+          const PA wgt = (bw < nbin && L.nmb[bw] > 0) ? (PA)(nd[bw] * dd[bw]) : (PA)0.0;
           const int re = 2 * h2, im = 2 * h2 + 1;
           // +mu
           R s1r = x0[re] + x1[re], s1i = x0[im] + x1[im];
           R s2r = y1[re] + y0[re], s2i = y1[im] + y0[im];
           R a1 = s1r * s1r + s1i * s1i, a2 = s2r * s2r + s2i * s2i;
-          p1 += (double)((R)0.5 * (a2 + a1)) * wgt;
-          p2 += (double)((R)0.5 * (a2 - a1)) * wgt;
-          p3 += (double)(s2r * s1r + s2i * s1i) * wgt;
+          p1 += (PA)((R)0.5 * (a2 + a1)) * wgt;
+          p2 += (PA)((R)0.5 * (a2 - a1)) * wgt;
+          p3 += (PA)(s2r * s1r + s2i * s1i) * wgt;
           // -mu
           s1r = x0[re] - x1[re];
           s1i = x0[im] - x1[im];
@@ -1434,9 +1437,9 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
           s2i = y1[im] - y0[im];
           a1 = s1r * s1r + s1i * s1i;
           a2 = s2r * s2r + s2i * s2i;
-          m1 += (double)((R)0.5 * (a2 + a1)) * wgt;
-          m2 += (double)((R)0.5 * (a2 - a1)) * wgt;
-          m3 += (double)(s2r * s1r + s2i * s1i) * wgt;
+          m1 += (PA)((R)0.5 * (a2 + a1)) * wgt;
+          m2 += (PA)((R)0.5 * (a2 - a1)) * wgt;
+          m3 += (PA)(s2r * s1r + s2i * s1i) * wgt;
         }
       }
       // the four lane groups hold different bins of the same angle
