@@ -319,7 +319,13 @@ int pam_context_trim(pam_context *ctx);
  *     workgroup per unit; k in 1..64 launches k workgroups per compute unit
  *     that take units from a work queue;
  *   graph: 1 captures the kernels of a single-chunk call in a HIP graph and
- *     replays it while the launch arguments are unchanged.
+ *     replays it while the launch arguments are unchanged;
+ *   pipeline: k in 2..16 (the default 0: off) splits a call of at least 8192 k
+ *     units into at least k column chunks run on two streams (the context's
+ *     second stream beside the call's), each chunk's DSD and Mie beside the
+ *     previous chunk's phase matrix, doubling and adding; 0 or 1 runs the
+ *     call's chunks in order on its one stream.  The stage times of a
+ *     pipelined call are the sums of each stream's segments and overlap.
  * pam_context_set_tuning returns PAM_FATAL for an unknown key or value;
  * pam_context_get_tuning returns -1 for an unknown key.  No counterpart in
  * the reference (its Fortran loops have no launch shape). */

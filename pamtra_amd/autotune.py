@@ -11,7 +11,9 @@ spread of cost), so the choice is measured:
 * persistent grids: every candidate value is applied to the three kernels
   at once and each kernel keeps the value that gave its own stage the
   shortest time (the stage times come from HIP events, pam_last_stage_times);
-* graph replay: timed on the whole call, on and off.
+* graph replay and pipelining (the call's column chunks on two streams,
+  each chunk's DSD and Mie beside the previous chunk's radiative transfer):
+  timed on the whole call, each against the plain single-stream launch.
 
 Every choice gives bitwise the same results; only time changes.  Choices are
 cached per device name and workload signature in a JSON file.
@@ -29,6 +31,7 @@ PERSISTENT_KEYS = ("mie_persistent", "doubling_persistent", "adding_persistent")
 STAGE_OF_KEY = {"mie_persistent": "mie", "doubling_persistent": "doubling",
                 "adding_persistent": "adding"}
 PERSISTENT_CANDIDATES = (0, 1, 2, 4)
+PIPELINE = 4  # column chunks of a pipelined call
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "pamtra_amd", "tuning.json")
 
 
@@ -60,7 +63,8 @@ def tune(lib, ctx, run_once: Callable[[], None], repeats: int = 2,
     workload on `ctx`), applies the fastest and returns it (with the
     measured times under "_ms")."""
     stage_times = stage_times or (lambda: _stage_times(lib, ctx))
-    apply(lib, ctx, {"graph": 0})
+    # the stage times of a pipelined call overlap: shapes are chosen on one stream
+    apply(lib, ctx, {"graph": 0, "pipeline": 0})
     run_once()  # warm: workspace, tables
     best: Dict[str, tuple] = {}
     table: Dict[str, Dict[int, float]] = {k: {} for k in PERSISTENT_KEYS}
@@ -79,19 +83,20 @@ def tune(lib, ctx, run_once: Callable[[], None], repeats: int = 2,
     choice = {k: best[k][0] for k in PERSISTENT_KEYS}
     apply(lib, ctx, choice)
     whole = {}
-    for g in (0, 1):
-        apply(lib, ctx, {"graph": g})
+    for g, pl in ((0, 0), (1, 0), (0, PIPELINE)):
+        apply(lib, ctx, {"graph": g, "pipeline": pl})
         run_once()  # capture (graph) or warm
         t0 = timer()
         for _ in range(repeats):
             run_once()
-        whole[g] = (timer() - t0) / repeats * 1e3
-    choice["graph"] = min(whole, key=whole.get)
-    apply(lib, ctx, {"graph": choice["graph"]})
+        whole[(g, pl)] = (timer() - t0) / repeats * 1e3
+    choice["graph"], choice["pipeline"] = min(whole, key=whole.get)
+    apply(lib, ctx, {"graph": choice["graph"], "pipeline": choice["pipeline"]})
     out: Dict[str, object] = dict(choice)
     out["_ms"] = {"persistent": {k: {str(v): round(t, 3) for v, t in table[k].items()}
                                  for k in PERSISTENT_KEYS},
-                  "graph": {str(g): round(t, 3) for g, t in whole.items()}}
+                  "whole": {f"graph={g},pipeline={pl}": round(t, 3)
+                            for (g, pl), t in whole.items()}}
     return out
 
 

@@ -76,7 +76,11 @@ struct Arena {
   }
 };
 constexpr int NSTAGE = 6;
-constexpr int PAM_NTUNE = 4;  // entries of k_tuning_keys
+constexpr int PAM_NTUNE = 5;  // entries of k_tuning_keys
+// pipelined calls (tuning key "pipeline" = k >= 2): at least k column chunks
+// of at least PIPE_MIN_UNITS (column, channel) or (layer, channel) units
+constexpr int PIPE_MAX = 16, PIPE_DEFAULT = 0;
+constexpr int64_t PIPE_MIN_UNITS = 8192;
 
 // A device allocation that only grows.
 struct DevBuf {
@@ -175,6 +179,7 @@ static AngleTables *device_tables(int device) {
 struct pam_context {
   int device = 0;
   hipStream_t stream = nullptr;  // the stream of pam_run (non-blocking)
+  hipStream_t stream2 = nullptr; // the second stream of a pipelined call
   DevBuf colbuf, itembuf, iobuf, smallbuf;
   DevBuf rtbuf;  // global scratch of the adding kernels at N = 64
   DevBuf miebuf; // U slots of the coefficient-first Mie kernel
@@ -198,7 +203,8 @@ struct pam_context {
   void *scratch = nullptr;        // device address of this device's `scratch`
 };
 
-static const char *k_tuning_keys = "mie_persistent,doubling_persistent,adding_persistent,graph";
+static const char *k_tuning_keys =
+    "mie_persistent,doubling_persistent,adding_persistent,graph,pipeline";
 
 static int tuning_index(const char *key) {
   if (!key) return -1;
@@ -372,6 +378,12 @@ extern "C" int pam_context_create(pam_context **out, int device) {
       delete c;
       return PAM_FATAL;
     }
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipStreamDestroy(c->stream);
+      (void)hipFree(c->d_mu);
+      delete c;
+      return PAM_FATAL;
+    }
     c->d_atab = t->tab;
     c->d_atab_f = t->tab_f;
     c->d_atab_off = t->off;
@@ -388,8 +400,10 @@ extern "C" int pam_context_create(pam_context **out, int device) {
   c->budget_bytes = (size_t)(gb * (1ull << 30));
   if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     c->ncu = 0;
+  c->tune[4] = PIPE_DEFAULT;
   c->scratch = scratch_address(device);
   if (!c->scratch) {
+    (void)hipStreamDestroy(c->stream2);
     (void)hipStreamDestroy(c->stream);
     (void)hipFree(c->d_mu);
     delete c;
@@ -410,6 +424,7 @@ extern "C" int pam_context_destroy(pam_context *c) {
   c->miebuf.release();
   drop_graph(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->d_mu) (void)hipFree(c->d_mu);
   delete c;
   return PAM_SUCCESS;
@@ -435,6 +450,7 @@ extern "C" int pam_context_set_tuning(pam_context *c, const char *key, int64_t v
   const int i = tuning_index(key);
   if (!c || i < 0 || value < 0 || value > 64) return PAM_FATAL;
   if (i == 3 && value > 1) return PAM_FATAL;
+  if (i == 4 && value > PIPE_MAX) return PAM_FATAL;
   c->tune[i] = value;
   if (i == 3 && !value) drop_graph(c);
   return PAM_SUCCESS;
@@ -502,6 +518,8 @@ static size_t col_bytes(const Dims &d) {
   add(ncol * ML * 4);
   add(ncol * nf * ML * 8);
   add(2 * ncol * nf * 2 * NMU * 8);
+  add(ncol * nf * 4);
+  add(ncol * nf * 4);
   add(ncol * nf * 4);
   add(ncol * nf * 4);
   add(ncol * nf * 4);
@@ -779,17 +797,18 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
 
   // small buffers: hydro table, frequencies, counters
   size_t small = 4096 + sizeof(pam_hydro_desc) * std::max(1, d.nh) + 16 * (size_t)nfreq + 256 + 64 * 8 +
-                 3 * 4 * (PH_NBUCKET + 1) + 256;
+                 2 * 3 * 4 * (PH_NBUCKET + 1) + 1024;
   void *small_p = region(c->smallbuf, small);
   if (!small_p) return no_room("the call's tables");
   Carver sc(small_p);
   pam_hydro_desc *d_hyd = sc.take<pam_hydro_desc>(std::max(1, d.nh));
   double *d_freqs = sc.take<double>(nfreq);
   unsigned long long *d_cnt = sc.take<unsigned long long>(PAM_NWORK);
-  uint32_t *d_redo_n = sc.take<uint32_t>(3);
-  uint32_t *d_queue = sc.take<uint32_t>(4);
+  // per stream of a pipelined call: redo counts, work queues, phase buckets
+  uint32_t *d_redo_n = sc.take<uint32_t>(2 * 3);
+  uint32_t *d_queue = sc.take<uint32_t>(2 * 4);
   double *d_k2 = sc.take<double>(nfreq);
-  uint32_t *d_ph = sc.take<uint32_t>(3 * (PH_NBUCKET + 1));
+  uint32_t *d_ph = sc.take<uint32_t>(2 * 3 * (PH_NBUCKET + 1));
   if (d.nh > 0)
     HIP_OK(hipMemcpyAsync(d_hyd, hydros, sizeof(pam_hydro_desc) * d.nh, hipMemcpyHostToDevice,
                           stream));
@@ -804,6 +823,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   if (!col_p) return no_room("the per-column workspace");
   Work w;
   memset(&w, 0, sizeof(w));
+  int32_t *redo_add2 = nullptr, *redo_lev2 = nullptr;
   {
     Carver cv(col_p);
     size_t ncol = d.ncol, ML = d.ML, nf = d.nf;
@@ -829,6 +849,8 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     w.cf_status = cv.take<int32_t>(ncol * nf);
     w.redo_add = cv.take<int32_t>(ncol * nf);
     w.redo_lev = cv.take<int32_t>(ncol * nf);
+    redo_add2 = cv.take<int32_t>(ncol * nf);
+    redo_lev2 = cv.take<int32_t>(ncol * nf);
     launch_fill_u32((uint32_t *)w.col_status, ncol, 0, stream);
     launch_fill_u32((uint32_t *)w.cf_status, ncol * nf, 0, stream);
   }
@@ -845,23 +867,6 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   w.ph_toff = d_ph + 2 * (PH_NBUCKET + 1);
   w.counters = d_cnt;
   w.redo_n = d_redo_n;
-  if (RT_SLOT_DOUBLES > 0) {
-    w.rt_scratch = (double *)region(c->rtbuf, RT_SLOT_DOUBLES * 8 * (size_t)std::max(1, c->ncu));
-    if (!w.rt_scratch) return no_room("the adding kernels' scratch");
-  }
-  if (!s.ssrg && !s.mie_chunked) {
-    // one U slot per workgroup of the Mie kernel's persistent grid; within
-    // the fixed scratch of the allocation-free entry, at most a quarter of it
-    const int per_cu = s.pers_mie > 0 ? s.pers_mie : MIE2_PER_CU;
-    w.mie_ld = mie_slot_rows(d.nbmax);
-    w.mie_slot_doubles = mie_slot_doubles(d.nbmax);
-    size_t slots = (size_t)std::max(1, c->ncu) * per_cu;
-    if (arena) slots = std::min(slots, arena->size / 4 / (w.mie_slot_doubles * 8));
-    if (slots == 0) return no_room("one Mie workgroup's slot");
-    w.mie_slots = (int)slots;
-    w.mie_u = (double *)region(c->miebuf, slots * w.mie_slot_doubles * 8);
-    if (!w.mie_u) return no_room("the Mie kernel's slots");
-  }
   w.queue = d_queue;
   w.radar_k2 = (st->active && st->radar_k2) ? d_k2 : nullptr;
 
@@ -904,10 +909,20 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   }
   HIP_OK(hipMemcpyAsync(w.col_item0, off.data(), 4 * (size_t)d.ncol, hipMemcpyHostToDevice,
                         stream));
+  // Pipelining (tuning key "pipeline" = k >= 2, not in the allocation-free
+  // entry): at least k column chunks on two streams with a workspace each;
+  // chunk c+1 starts once chunk c's Mie stage is done, so that its DSD and
+  // Mie run beside chunk c's phase matrix, doubling and adding.
+  const int64_t total_units = total * d.nf;
+  int64_t npipe = 1;
+  if (!arena && c->tune[4] >= 2)
+    npipe = std::min<int64_t>(c->tune[4], total_units / PIPE_MIN_UNITS);
+  const bool pipelined = npipe >= 2;
+  const int nlane = pipelined ? 2 : 1;
   const size_t per_item = item_bytes_per(d);
   size_t budget = c->budget_bytes;
   if (const char *e = getenv("PAM_WORKSPACE_GB")) budget = (size_t)(atof(e) * (1ull << 30));
-  size_t max_items = std::max<size_t>(1, budget / per_item);
+  size_t max_items = std::max<size_t>(1, budget / per_item / nlane);
   if (arena) {
     const size_t room = arena->left();
     max_items = room > 64 * 256 ? (room - 64 * 256) / per_item : 0;
@@ -917,7 +932,8 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   // chunks of whole columns with about equal item counts (no small tail chunk)
   std::vector<std::pair<int, int>> chunks;
   {
-    const int64_t nchunk = std::max<int64_t>(1, (total + (int64_t)max_items - 1) / (int64_t)max_items);
+    const int64_t nchunk = std::max<int64_t>(
+        npipe, std::max<int64_t>(1, (total + (int64_t)max_items - 1) / (int64_t)max_items));
     const int64_t target = (total + nchunk - 1) / nchunk;
     int c0 = 0;
     int64_t acc = 0;
@@ -938,43 +954,85 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     chunk_items_max = std::max<size_t>(chunk_items_max, (size_t)n);
   }
   if (arena && chunk_items_max > max_items) return no_room("the optics of one column");
-  void *item_p = region(c->itembuf, std::max<size_t>(1, chunk_items_max) * per_item + 64 * 256);
+  const size_t item_region =
+      (std::max<size_t>(1, chunk_items_max) * per_item + 64 * 256 + 255) & ~(size_t)255;
+  void *item_p = region(c->itembuf, nlane * item_region);
   if (!item_p) return no_room("the per-layer optics");
+  if (RT_SLOT_DOUBLES > 0) {
+    w.rt_scratch = (double *)region(
+        c->rtbuf, nlane * RT_SLOT_DOUBLES * 8 * (size_t)std::max(1, c->ncu));
+    if (!w.rt_scratch) return no_room("the adding kernels' scratch");
+  }
+  if (!s.ssrg && !s.mie_chunked) {
+    // one U slot per workgroup of the Mie kernel's persistent grid; within
+    // the fixed scratch of the allocation-free entry, at most a quarter of it
+    const int per_cu = s.pers_mie > 0 ? s.pers_mie : MIE2_PER_CU;
+    w.mie_ld = mie_slot_rows(d.nbmax);
+    w.mie_slot_doubles = mie_slot_doubles(d.nbmax);
+    size_t slots = (size_t)std::max(1, c->ncu) * per_cu;
+    if (arena) slots = std::min(slots, arena->size / 4 / (w.mie_slot_doubles * 8));
+    if (slots == 0) return no_room("one Mie workgroup's slot");
+    w.mie_slots = (int)slots;
+    w.mie_u = (double *)region(c->miebuf, nlane * slots * w.mie_slot_doubles * 8);
+    if (!w.mie_u) return no_room("the Mie kernel's slots");
+  }
   c->work[0] = d.ncol;
   c->work[1] = d.nf;
   c->work[2] = total * d.nf;
+  // the second stream's copy of the workspace description: its own items,
+  // queues, redo lists, phase buckets, Mie slots and adding scratch
+  Work w2 = w;
+  hipStream_t stream2 = pipelined ? c->stream2 : stream;
+  if (pipelined) {
+    w2.queue = d_queue + 4;
+    w2.redo_n = d_redo_n + 3;
+    w2.ph_cnt = d_ph + 3 * (PH_NBUCKET + 1);
+    w2.ph_off = w2.ph_cnt + (PH_NBUCKET + 1);
+    w2.ph_toff = w2.ph_cnt + 2 * (PH_NBUCKET + 1);
+    w2.redo_add = redo_add2;
+    w2.redo_lev = redo_lev2;
+    if (w.mie_u) w2.mie_u = w.mie_u + (size_t)w.mie_slots * w.mie_slot_doubles;
+    if (RT_SLOT_DOUBLES > 0)
+      w2.rt_scratch = w.rt_scratch + RT_SLOT_DOUBLES * (size_t)std::max(1, c->ncu);
+    // stream2 starts after the columns and the surface on the call's stream
+    hipEvent_t e;
+    HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(e, stream));
+    HIP_OK(hipStreamWaitEvent(stream2, e, 0));
+    (void)hipEventDestroy(e);
+  }
 
   // The kernels of one chunk in five segments, one per timed stage (dsd,
   // mie, phase, doubling, adding).  Counters and status words are cleared by
   // a fill kernel (not a memset), so that each segment can be captured in a
   // graph.
-  auto enqueue_segment = [&](int seg, const Dims &dc, Work &wc) -> hipError_t {
+  auto enqueue_segment = [&](int seg, const Dims &dc, Work &wc, hipStream_t sx) -> hipError_t {
     switch (seg) {
       case 1:
-        launch_fill_u32(wc.queue, 4, 0, stream);
-        launch_fill_u32(wc.redo_n, 3, 0, stream);
+        launch_fill_u32(wc.queue, 4, 0, sx);
+        launch_fill_u32(wc.redo_n, 3, 0, sx);
         if (dc.nitems > 0)
-          launch_fill_u32((uint32_t *)wc.itf_status, (size_t)dc.nitems * d.nf, 0, stream);
-        launch_items(dc, wc, stream);
-        launch_dsd(dc, s, p, d_hyd, h4, wc, stream);
+          launch_fill_u32((uint32_t *)wc.itf_status, (size_t)dc.nitems * d.nf, 0, sx);
+        launch_items(dc, wc, sx);
+        launch_dsd(dc, s, p, d_hyd, h4, wc, sx);
         break;
       case 2:
-        launch_mie(dc, s, d_hyd, h4, p, d_freqs, wc, stream);
+        launch_mie(dc, s, d_hyd, h4, p, d_freqs, wc, sx);
         break;
       case 3:
-        launch_phase(dc, s, wc, stream);
+        launch_phase(dc, s, wc, sx);
         break;
       case 4:
-        launch_doubling(dc, s, d_freqs, wc, stream);
+        launch_doubling(dc, s, d_freqs, wc, sx);
         break;
       default:
-        launch_adding(dc, s, d_freqs, pin->sfc_refl, wc, res->tb, stream);
+        launch_adding(dc, s, d_freqs, pin->sfc_refl, wc, res->tb, sx);
         if (s.active)
           launch_radar(dc, s, d_freqs, wc, res->ze, res->att_hydro, res->att_atmo, res->radar_hgt,
-                       stream);
+                       sx);
         if (res->kextatmo || res->kexttot || res->salbedo || res->nlegen || res->ndoubles) {
           size_t nd = (size_t)dc.ncol_chunk * d.ML * d.nf;
-          hipLaunchKernelGGL(k_diag, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, stream, dc,
+          hipLaunchKernelGGL(k_diag, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, sx, dc,
                              wc, res->kextatmo, res->kexttot, res->salbedo, res->nlegen,
                              res->ndoubles);
         }
@@ -983,7 +1041,28 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     return hipGetLastError();
   };
   const bool use_graph = c->tune[3] && chunks.size() == 1 && !arena;
-  for (auto &ch : chunks) {
+  // stage times of a pipelined call: each segment from the previous event
+  // on its own stream (the segments of the two streams overlap in time)
+  int last_ev[2] = {(int)evs.size() - 1, -1};
+  std::vector<int> ev_from(evs.size());
+  for (size_t i = 0; i < ev_from.size(); i++) ev_from[i] = (int)i - 1;
+  auto mark_on = [&](int stage, int lane, hipStream_t sx) -> hipError_t {
+    if (arena) return hipSuccess;
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    evs.push_back(e);
+    ev_stage.push_back(stage);
+    ev_from.push_back(last_ev[lane]);
+    last_ev[lane] = (int)evs.size() - 1;
+    return hipEventRecord(e, sx);
+  };
+  hipEvent_t mie_done = nullptr;  // the previous chunk's Mie stage (pipelined)
+  for (size_t ci = 0; ci < chunks.size(); ci++) {
+    auto &ch = chunks[ci];
+    const int lane = pipelined ? (int)(ci & 1) : 0;
+    Work &wc = lane ? w2 : w;
+    hipStream_t sx = lane ? stream2 : stream;
     Dims dc;
     memcpy(&dc, &d, sizeof(d));
     dc.col0 = ch.first;
@@ -992,7 +1071,15 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     int64_t n = 0;
     for (int i = ch.first; i < ch.second; i++) n += cnt[i];
     dc.nitems = (int)n;
-    carve_items(w, item_p, dc, std::max<size_t>(1, (size_t)n));
+    carve_items(wc, (char *)item_p + (size_t)lane * item_region, dc, std::max<size_t>(1, (size_t)n));
+    if (pipelined) {
+      if (mie_done) {
+        HIP_OK(hipStreamWaitEvent(sx, mie_done, 0));
+        (void)hipEventDestroy(mie_done);
+        mie_done = nullptr;
+      }
+      if (last_ev[lane] < 0 || ci >= 2) HIP_OK(mark_on(-1, lane, sx));
+    }
     if (use_graph) {
       // the launch arguments the graphs bake in: shapes, settings,
       // workspace and the caller's device pointers
@@ -1022,7 +1109,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
           hipGraph_t g = nullptr;
           ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
           if (!ok) break;
-          const hipError_t e1 = enqueue_segment(seg, dc, w);
+          const hipError_t e1 = enqueue_segment(seg, dc, w, stream);
           const hipError_t e2 = hipStreamEndCapture(stream, &g);
           ok = e1 == hipSuccess && e2 == hipSuccess && g;
           if (ok) ok = hipGraphInstantiate(&c->gexec[seg - 1], g, nullptr, nullptr, 0) == hipSuccess;
@@ -1039,9 +1126,20 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
       if (use_graph && c->gexec[0])
         HIP_OK(hipGraphLaunch(c->gexec[seg - 1], stream));
       else
-        HIP_OK(enqueue_segment(seg, dc, w));
-      HIP_OK(mark(seg));
+        HIP_OK(enqueue_segment(seg, dc, wc, sx));
+      HIP_OK(mark_on(seg, lane, sx));
+      if (pipelined && seg == 2 && ci + 1 < chunks.size()) {
+        HIP_OK(hipEventCreateWithFlags(&mie_done, hipEventDisableTiming));
+        HIP_OK(hipEventRecord(mie_done, sx));
+      }
     }
+  }
+  if (pipelined) {  // the call's stream waits for the second stream
+    hipEvent_t e;
+    HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_OK(hipEventRecord(e, stream2));
+    HIP_OK(hipStreamWaitEvent(stream, e, 0));
+    (void)hipEventDestroy(e);
   }
   hipLaunchKernelGGL(k_status, dim3((unsigned)((d.ncol * d.nf + 255) / 256)), dim3(256), 0,
                      stream, d, w, res->status);
@@ -1054,7 +1152,9 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   // stage times
   for (size_t i = 1; i < evs.size(); i++) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, evs[i - 1], evs[i]) == hipSuccess && ev_stage[i] >= 0)
+    const int from = i < ev_from.size() ? ev_from[i] : (int)i - 1;
+    if (from >= 0 && ev_stage[i] >= 0 &&
+        hipEventElapsedTime(&ms, evs[from], evs[i]) == hipSuccess)
       c->stage_ms[ev_stage[i]] += ms;
   }
   free_events();

@@ -10,7 +10,7 @@ class FakeLib:
     """pam_tuning_* and pam_last_stage_times of a context whose stage times
     follow a given cost model of the tuning values."""
 
-    KEYS = b"mie_persistent,doubling_persistent,adding_persistent,graph"
+    KEYS = b"mie_persistent,doubling_persistent,adding_persistent,graph,pipeline"
 
     def __init__(self, cost):
         self.tune = {k: 0 for k in self.KEYS.decode().split(",")}
@@ -22,7 +22,8 @@ class FakeLib:
 
     def pam_context_set_tuning(self, ctx, key, value):
         k = key.decode()
-        if k not in self.tune or value < 0 or value > 64 or (k == "graph" and value > 1):
+        if k not in self.tune or value < 0 or value > 64 or (k == "graph" and value > 1) or (
+                k == "pipeline" and value > 16):
             return 2
         self.tune[k] = value
         return 0
@@ -40,7 +41,9 @@ def _setup(cost):
 
     def run_once():
         lib.calls += 1
-        clock[0] += sum(lib.stage_times().values()) * 1e-3 + (0.0 if lib.tune["graph"] else 0.002)
+        # graph replay saves launch time; pipelining overlaps 30 % of the stages
+        t = sum(lib.stage_times().values()) * 1e-3 * (0.7 if lib.tune["pipeline"] else 1.0)
+        clock[0] += t + (0.0 if lib.tune["graph"] else 0.002)
 
     return lib, clock, run_once
 
@@ -59,7 +62,7 @@ def test_picks_fastest_per_stage_and_graph():
     assert res["mie_persistent"] == 2
     assert res["doubling_persistent"] == 4
     assert res["adding_persistent"] == 0
-    assert res["graph"] == 1
+    assert res["graph"] == 0 and res["pipeline"] == autotune.PIPELINE
     # the choice is left applied on the context
     assert autotune.get_tuning(lib, None) == {k: res[k] for k in lib.tune}
 
@@ -86,3 +89,10 @@ def test_rejected_value_raises():
     except ValueError:
         return
     raise AssertionError("an invalid value must raise")
+
+
+def test_graph_when_pipelining_does_not_pay():
+    lib, clock, run_once = _setup(lambda st, tune: 0.001)
+    res = autotune.tune(lib, None, run_once, repeats=1, timer=lambda: clock[0],
+                        stage_times=lib.stage_times)
+    assert res["graph"] == 1 and res["pipeline"] == 0
