@@ -923,8 +923,27 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   size_t budget = c->budget_bytes;
   if (const char *e = getenv("PAM_WORKSPACE_GB")) budget = (size_t)(atof(e) * (1ull << 30));
   size_t max_items = std::max<size_t>(1, budget / per_item / nlane);
+  // the Mie kernel's U slots: one per workgroup of its persistent grid;
+  // within the fixed scratch of the allocation-free entry, at most a quarter
+  // of it
+  const bool mie_slotted = !s.ssrg && !s.mie_chunked;
+  size_t mie_nslots = 0;
+  if (mie_slotted) {
+    const int per_cu = s.pers_mie > 0 ? s.pers_mie : MIE2_PER_CU;
+    w.mie_ld = mie_slot_rows(d.nbmax);
+    w.mie_slot_doubles = mie_slot_doubles(d.nbmax);
+    mie_nslots = (size_t)std::max(1, c->ncu) * per_cu;
+    if (arena) mie_nslots = std::min(mie_nslots, arena->size / 4 / (w.mie_slot_doubles * 8));
+    if (mie_nslots == 0) return no_room("one Mie workgroup's slot");
+  }
+  const size_t rt_bytes = nlane * RT_SLOT_DOUBLES * 8 * (size_t)std::max(1, c->ncu);
+  const size_t mie_bytes = nlane * mie_nslots * (mie_slotted ? w.mie_slot_doubles * 8 : 0);
   if (arena) {
-    const size_t room = arena->left();
+    // what the optics may take: the arena less the adding scratch and the
+    // Mie slots carved after them (each region rounded up to 256 bytes)
+    size_t room = arena->left();
+    const size_t fixed = (rt_bytes + 255) / 256 * 256 + (mie_bytes + 255) / 256 * 256 + 512;
+    room = room > fixed ? room - fixed : 0;
     max_items = room > 64 * 256 ? (room - 64 * 256) / per_item : 0;
     if (max_items == 0 && total > 0) return no_room("one scattering layer's optics");
     max_items = std::max<size_t>(1, max_items);
@@ -959,21 +978,12 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   void *item_p = region(c->itembuf, nlane * item_region);
   if (!item_p) return no_room("the per-layer optics");
   if (RT_SLOT_DOUBLES > 0) {
-    w.rt_scratch = (double *)region(
-        c->rtbuf, nlane * RT_SLOT_DOUBLES * 8 * (size_t)std::max(1, c->ncu));
+    w.rt_scratch = (double *)region(c->rtbuf, rt_bytes);
     if (!w.rt_scratch) return no_room("the adding kernels' scratch");
   }
-  if (!s.ssrg && !s.mie_chunked) {
-    // one U slot per workgroup of the Mie kernel's persistent grid; within
-    // the fixed scratch of the allocation-free entry, at most a quarter of it
-    const int per_cu = s.pers_mie > 0 ? s.pers_mie : MIE2_PER_CU;
-    w.mie_ld = mie_slot_rows(d.nbmax);
-    w.mie_slot_doubles = mie_slot_doubles(d.nbmax);
-    size_t slots = (size_t)std::max(1, c->ncu) * per_cu;
-    if (arena) slots = std::min(slots, arena->size / 4 / (w.mie_slot_doubles * 8));
-    if (slots == 0) return no_room("one Mie workgroup's slot");
-    w.mie_slots = (int)slots;
-    w.mie_u = (double *)region(c->miebuf, nlane * slots * w.mie_slot_doubles * 8);
+  if (mie_slotted) {
+    w.mie_slots = (int)mie_nslots;
+    w.mie_u = (double *)region(c->miebuf, mie_bytes);
     if (!w.mie_u) return no_room("the Mie kernel's slots");
   }
   c->work[0] = d.ncol;
