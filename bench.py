@@ -388,6 +388,9 @@ def main():
                     help="only the SSRGA producer timing (needs --aggregates-cols)")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the large-domain (32-angle) sample and print its summary")
+    ap.add_argument("--config3-only", action="store_true",
+                    help="run only the four-category configuration (FP64 and mixed) and print "
+                         "its summary")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the columns are "
@@ -400,6 +403,9 @@ def main():
 
     if args.large_only:
         print(json.dumps({"large_domain": large_domain(args, torch)}))
+        return
+    if args.config3_only:
+        print(json.dumps({"config3": config3_leg(args, torch)}))
         return
     if args.aggregates_only:
         print(json.dumps({"aggregates": aggregates(args)}))
