@@ -224,6 +224,81 @@ def large_domain(args, torch):
     return res
 
 
+def large_domain_stream(args, torch):
+    """The large-domain configuration at its full size, streamed: the 1024 x
+    1024 columns are generated and run in slices of --large-slice columns,
+    one pam_run_device call per slice with the slice's inputs already in HBM
+    (generation and the host-to-device copies are outside the timed calls).
+    The mixed path runs every slice; FP64 runs the first --large-fp64-slices
+    slices, on which the per-channel TB difference is taken."""
+    from pamtra_amd import _capi, _lib, workloads
+    lib = _lib.library(32)
+    ctx = _lib.get_context(0, 32)
+    names = lib.pam_stage_names().decode().split(",")
+    total, size = args.large_total, args.large_slice
+    nslice = (total + size - 1) // size
+    res = {p: {"seconds": 0.0, "columns": 0, "stage_ms": np.zeros(len(names)),
+               "status_nonzero": 0} for p in ("fp64", "mixed")}
+    dmax = None
+    for k in range(nslice):
+        n = min(size, total - k * size)
+        pam, freqs = workloads.config5(ncol=n, seed=1005 + 7919 * k)
+        prof = to_device(pam, torch)
+        d4 = to_device_4d(pam, torch)
+        nf = len(freqs)
+        f, fp = _capi.freqs_array(freqs)
+        tbs = {}
+        for precision in (("fp64", "mixed") if k < args.large_fp64_slices else ("mixed",)):
+            m = _capi.marshal(dict(pam.nmlSet), dict(pam.set, _nangles=32, precision=precision),
+                              pam.df.data, d4, prof)
+            tb = torch.zeros((m.nx, m.ny, m.noutlevels, 64, nf, 2), dtype=torch.float64,
+                             device="cuda")
+            status = torch.zeros((m.nx, m.ny, nf), dtype=torch.int32, device="cuda")
+            r = _capi.PamResults()
+            r.tb = C.cast(C.c_void_p(tb.data_ptr()), C.POINTER(C.c_double))
+            r.status = C.cast(C.c_void_p(status.data_ptr()), C.POINTER(C.c_int32))
+            msg = C.create_string_buffer(512)
+            r.msg = C.cast(msg, C.c_char_p)
+            r.msglen = 512
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if lib.pam_run_device(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),
+                                  C.byref(m.profile), fp, nf, C.byref(r), None) == _capi.PAM_FATAL:
+                raise RuntimeError("pam_run_device (large domain): " + msg.value.decode())
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            ms = (C.c_double * len(names))()
+            lib.pam_last_stage_times(ctx, ms, len(names))
+            q = res[precision]
+            q["seconds"] += dt
+            q["columns"] += n
+            q["stage_ms"] += np.array(ms[:])
+            q["status_nonzero"] += int((status.cpu().numpy() != 0).sum())
+            if k < args.large_fp64_slices:
+                tbs[precision] = tb.cpu().numpy()
+            log(f"large domain slice {k + 1}/{nslice} ({n} columns) {precision}: {dt:.2f} s")
+        if len(tbs) == 2:
+            d = np.max(np.abs(tbs["mixed"] - tbs["fp64"]), axis=(0, 1, 2, 3, 5))
+            dmax = d if dmax is None else np.maximum(dmax, d)
+        del prof, d4
+    out = {}
+    for p, q in res.items():
+        if q["columns"] == 0:
+            continue
+        out[p] = {"columns": q["columns"], "channels": nf, "seconds": round(q["seconds"], 2),
+                  "value": round(q["columns"] * nf / q["seconds"], 2),
+                  "stage_ms": {n: round(v, 1) for n, v in zip(names, q["stage_ms"])},
+                  "status_nonzero": q["status_nonzero"]}
+    out["mixed"]["max_abs_dtb_vs_fp64_K"] = {f"{fr:g}": float(v) for fr, v in zip(freqs, dmax)}
+    out["mixed"]["speedup_vs_fp64"] = round(out["mixed"]["value"] / out["fp64"]["value"], 3)
+    out["workload"] = (f"large domain, {total} columns generated and run in slices of {size} "
+                       "(one call each), 150 layers, 12 channels 10.65-325 GHz, "
+                       "cloud/ice/snow/graupel 100 bins, NSTOKES=2, 32 angles; FP64 on the first "
+                       f"{args.large_fp64_slices} slices")
+    log(f"large domain (streamed): {out}")
+    return out
+
+
 def fp32_doubling_roofline(work, dbl_ms, n):
     """The mixed doubling on the FP32 matrix cores: the N x N products it
     executed (work[9]: start layer, series inverses and doublings), 2 N^3
@@ -388,6 +463,11 @@ def main():
                     help="only the SSRGA producer timing (needs --aggregates-cols)")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the large-domain (32-angle) sample and print its summary")
+    ap.add_argument("--large-stream", action="store_true",
+                    help="run only the large domain at full size, streamed in column slices")
+    ap.add_argument("--large-total", type=int, default=1024 * 1024)
+    ap.add_argument("--large-slice", type=int, default=65536)
+    ap.add_argument("--large-fp64-slices", type=int, default=2)
     ap.add_argument("--config3-only", action="store_true",
                     help="run only the four-category configuration (FP64 and mixed) and print "
                          "its summary")
@@ -403,6 +483,9 @@ def main():
 
     if args.large_only:
         print(json.dumps({"large_domain": large_domain(args, torch)}))
+        return
+    if args.large_stream:
+        print(json.dumps({"large_domain_full": large_domain_stream(args, torch)}))
         return
     if args.config3_only:
         print(json.dumps({"config3": config3_leg(args, torch)}))

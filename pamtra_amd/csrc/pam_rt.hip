@@ -4,7 +4,9 @@
 //    m=0 phase matrix that k_phase (pam_phase.hip) formed in the unit's
 //    R/T slots (scatcnv.f90:103-192), checks its normalisation
 //    (radscat4.f90:87-136), initialises the thin layer (radintg4.f90:1-73)
-//    and doubles it up (radintg4.f90:174-309).  Only the + quadrants are
+//    and doubles it up (radintg4.f90:174-309) on three LDS matrices (see
+//    DblLds3); the Gauss-Jordan queue pass keeps the four-matrix step
+//    (doubling_unit).  Only the + quadrants are
 //    formed: for randomly oriented particles with NSTOKES=2 the (-,-) and
 //    (+,-) quadrants equal (+,+) and (-,+) exactly, so R- = R+ and T- = T+
 //    throughout the symmetric doubling.
@@ -50,6 +52,9 @@ constexpr int RPL = RT_WG / NN;
 // no room for more)
 constexpr int KUNR = (NN == 32) ? NN / 4 : 2;
 constexpr int KUNR_DBL = (NN == 32) ? NN / 4 : 8;
+// k-loop unrolling of the hot doubling step: by 4 at N = 32, which keeps it at
+// 88 VGPRs, within the five waves per SIMD its LDS allows (by 8 it needs 116)
+constexpr int DBL_STEP_UNR = (NN == 32) ? 4 : 8;
 
 // ------------------------------------------------------------------
 // LDS dense helpers; every thread of the workgroup calls them
@@ -655,21 +660,347 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
   return true;
 }
 
+// ---- the hot kernel's unit on three LDS matrices
+//
+// The layer doubling needs R, T and a scratch matrix in LDS when R's own
+// output tile and V = R T are kept in registers across a step and the inverse
+// series is applied to T directly:
+//   Z = T G = T (I + X)(I + X^2)(I + X^4)...,  X = R R,
+// the factors in the order of series_inv (Z_0 = T + T X, then Z <- Z + Z P,
+// P <- P P).  Slot A holds R (then Z, then R'), B holds T, C holds X, the
+// powers P and then V.  At 30 KB of LDS five workgroups share a compute unit
+// (the four-matrix layout admits four).
+struct DblLds3 {
+  double A[MSZ], B[MSZ], C[MSZ];
+  double v[12][NN];  // S+, S-, C+, C-, then source-chain scratch
+  double norm[NMU];
+  float wmax[NWAVE];
+  double mu[NMU], qw[NMU];
+  int slot;  // unit index of a persistent workgroup
+};
+
+// Row i of A (x + f y) in the four-vector mapping (y may be null); the lane
+// split of mv_pair at N = 32, one lane per row at N = 64.  No barrier.
+__device__ __forceinline__ double mv_vl_f(const double *A, const double *x, const double *y,
+                                          double f, int i) {
+  double s = 0.0;
+  if constexpr (VL == 2) {
+    const int h = (threadIdx.x & 1) * 8;
+    // unrolled by 4 only: the passes around it keep their own operands live
+#pragma unroll 4
+    for (int k = 0; k < NN / 2; k++) {
+      const int c = (k & 7) + h + 16 * (k >> 3);
+      s += A[i * LD + c] * (y ? x[c] + f * y[c] : x[c]);
+    }
+    return s + __shfl_xor(s, 1, 64);
+  } else {
+#pragma unroll 8
+    for (int k = 0; k < NN; k++) s += A[i * LD + k] * (y ? x[k] + f * y[k] : x[k]);
+    return s;
+  }
+}
+
+// One step of DOUBLING_INTEGRATION4 (radintg4.f90:174-309) on the slots of
+// DblLds3 (R in A, T in B on entry and on exit).  With V = R T:
+//   R' = R + Z V,  T' = Z T,
+//   S+' = (S+ + Z (S+ + R (S- + f C-))) + f C+,  C+' = C+ + Z (C+ + R C-),
+//   S-' = S- + Z ((S- + f C-) + R S+),           C-' = C- + Z (C- + R C+).
+// Returns false when the series has not converged (the unit is queued for
+// the Gauss-Jordan pass, nothing of it stored).
+template <int U>
+__device__ __forceinline__ bool dbl_step3(double *sA, double *sB, double *sC,
+                                          double (*sv)[NN], double linfac, float *wmax,
+                                          int &np) {
+  const int t = threadIdx.x, wv = t >> 6, l = t & 63;
+  const int li = l & 15, lk = l >> 4;
+  const int g = vl_group(), i = vl_row();
+  const bool lead = vl_lead();
+  // ---- X = R R into C; V = R T and R's own tile kept in registers; the
+  // source chains' first stage beside them
+  d4_t ax[TPW], av[TPW], rd[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      ax[q][r] = av[q][r] = 0.0;
+      rd[q][r] = sA[(I + 4 * r + lk) * LD + J + li];
+    }
+  }
+#pragma unroll U
+  for (int s = 0; s < NN / 4; s++) {
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+      const double a = sA[(I + li) * LD + 4 * s + lk];
+      const double bx = sA[(4 * s + lk) * LD + J + li];
+      const double bv = sB[(4 * s + lk) * LD + J + li];
+      ax[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bx, ax[q], 0, 0, 0);
+      av[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, av[q], 0, 0, 0);
+    }
+  }
+  double u;
+  {
+    // g = 0: S+ + R (S- + f C-); 1: C+ + R C-; 2: (S- + f C-) + R S+; 3: C- + R C+
+    const double *x = g == 0 ? sv[1] : g == 1 ? sv[3] : g == 2 ? sv[0] : sv[2];
+    const double *y = g == 0 ? sv[3] : nullptr;
+    const double b = g == 0 ? sv[0][i] : g == 1 ? sv[2][i] : g == 2 ? sv[1][i] + linfac * sv[3][i]
+                                                                    : sv[3][i];
+    u = b + mv_vl_f(sA, x, y, linfac, i);
+  }
+  __syncthreads();
+  double m = 0.0;
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) sC[(I + 4 * r + lk) * LD + J + li] = ax[q][r];
+    m = max_nan(m, max_nan(max_nan(fabs(ax[q][0]), fabs(ax[q][1])),
+                           max_nan(fabs(ax[q][2]), fabs(ax[q][3]))));
+  }
+  if (lead) sv[8 + g][i] = u;
+  {
+    const float mf = wave_max((float)m);
+    if (l == 0) wmax[wv] = mf;
+  }
+  __syncthreads();
+  m = (double)wmax_all(wmax);
+  np += 2;
+  if (!(m < 0x1p64)) return false;  // NaN or overflow
+  if ((double)NN * m * m < NEU_TOL) {
+    // ---- thin layer (G = I + X): Z = T + T X into A, V into C
+    d4_t az[TPW];
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+      for (int r = 0; r < 4; r++) az[q][r] = sB[(I + 4 * r + lk) * LD + J + li];
+    }
+#pragma unroll U
+    for (int s = 0; s < NN / 4; s++) {
+#pragma unroll
+      for (int q = 0; q < TPW; q++) {
+        const int I = tile_i(wv, q), J = tile_j(wv, q);
+        const double a = sB[(I + li) * LD + 4 * s + lk];
+        const double b = sC[(4 * s + lk) * LD + J + li];
+        az[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, az[q], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int e = (I + 4 * r + lk) * LD + J + li;
+        sA[e] = az[q][r];
+        sC[e] = av[q][r];
+      }
+    }
+    __syncthreads();
+    np += 1;
+  } else {
+    // ---- thick layer: Z = T + T X and P = X X in one pass, then the
+    // remaining factors Z <- Z + Z P, P <- P P; then V into C
+    m = mm2<U>(sB, sC, sB, sA, sC, sC, nullptr, sC, wmax);
+    np += 2;
+    bool conv = false;
+    for (int it = 0; it < NEU_MAXIT; it++) {
+      if (!(m < 0x1p64)) return false;
+      if (m < NEU_TOL) {
+        conv = true;
+        break;
+      }
+      if ((double)NN * m * m < NEU_TOL) {  // last factor
+        mm_add<U>(sA, sA, sC, sA);
+        np++;
+        conv = true;
+        break;
+      }
+      m = mm2<U>(sA, sC, sA, sA, sC, sC, nullptr, sC, wmax);
+      np += 2;
+    }
+    if (!conv) return false;
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+      for (int r = 0; r < 4; r++) sC[(I + 4 * r + lk) * LD + J + li] = av[q][r];
+    }
+    __syncthreads();
+  }
+  // ---- T' = Z T, R' = R + Z V from the same Z operand loads; the chains'
+  // second stage Z u beside them
+  d4_t at[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; q++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) at[q][r] = 0.0;
+#pragma unroll U
+  for (int s = 0; s < NN / 4; s++) {
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+      const int I = tile_i(wv, q), J = tile_j(wv, q);
+      const double a = sA[(I + li) * LD + 4 * s + lk];
+      const double bt = sB[(4 * s + lk) * LD + J + li];
+      const double bv = sC[(4 * s + lk) * LD + J + li];
+      at[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bt, at[q], 0, 0, 0);
+      rd[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, rd[q], 0, 0, 0);
+    }
+  }
+  double nv;
+  {
+    const double z = mv_vl_f(sA, sv[8 + g], nullptr, 0.0, i);
+    nv = g == 0 ? (sv[0][i] + z) + linfac * sv[2][i]
+                : (g == 1 ? sv[2][i] : g == 2 ? sv[1][i] : sv[3][i]) + z;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < TPW; q++) {
+    const int I = tile_i(wv, q), J = tile_j(wv, q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int e = (I + 4 * r + lk) * LD + J + li;
+      sB[e] = at[q][r];
+      sA[e] = rd[q][r];
+    }
+  }
+  if (lead) sv[g == 0 ? 0 : (g == 1 ? 2 : (g == 2 ? 1 : 3))][i] = nv;
+  __syncthreads();
+  np += 2;
+  return true;
+}
+
+// One (scattering layer, channel) unit of the hot kernel: as doubling_unit
+// (scatcnv's phase matrix from k_phase, CHECK_NORM4, the doubling count,
+// INITIALIZE4 / INITIAL_SOURCE4, the doublings) on DblLds3.  Returns false
+// when an inverse series did not converge; nothing is written for the unit.
+__device__ bool doubling_unit3(const Dims &d, const Settings &s, const double *freqs,
+                               const Work &w, int itf, DblLds3 &L) {
+  double *sA = L.A, *sB = L.B, *sC = L.C;
+  double(*sv)[NN] = L.v;
+  double *snorm = L.norm, *smu = L.mu, *sqw = L.qw;
+  const int t = threadIdx.x;
+  const int li = itf / d.nf, f = itf % d.nf;
+  const int col = w.item_col[li], z = w.item_layer[li];
+  if (t < NMU) {
+    smu[t] = w.mu[t];
+    sqw[t] = w.qw[t];
+  }
+  if (w.itf_status[itf] & 0xff) return true;
+  const int nleg = w.nleg[itf];
+  double kext = w.kext[itf], salb = w.salb[itf];
+  if (nleg <= 0) {  // no Legendre series: no scatcnv, zero optics
+    kext = 0.0;
+    salb = 0.0;
+  }
+  // ---- scatcnv: the phase matrix, formed by k_phase in the unit's T
+  // ((+,+) quadrant, into B) and R ((-,+) quadrant, into A) slots
+  const int numpts = 2 * ((nleg + 1) / 2) + 4;
+  {
+    const bool has = nleg > 0 && s.lhyd_scat;
+    const double *P0 = w.T + (size_t)itf * NN2, *P1 = w.R + (size_t)itf * NN2;
+    for (int e = t; e < NN2; e += RT_WG) {
+      const int i = e / NN, j = e % NN;
+      sB[i * LD + j] = has ? P0[e] : 0.0;
+      sA[i * LD + j] = has ? P1[e] : 0.0;
+    }
+  }
+  __syncthreads();
+  const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
+  // ---- CHECK_NORM4
+  if (t < NMU) {
+    double sum = absorb - kext;
+    for (int j2 = 0; j2 < NMU; j2++)
+      sum = sum + 2.0 * PI * sqw[j2] * (sB[(NST * j2) * LD + NST * t] + sA[(NST * j2) * LD + NST * t]);
+    snorm[t] = fabs(sum);
+  }
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int j = 0; j < NMU; j++) mx = fmax(mx, snorm[j]);
+    if (mx > 5.0e-4) set_err(&w.itf_status[itf], E_NORM);
+  }
+  // ---- layer thickness, doubling count (radtran4.f90:311-329)
+  const double wl = C_LIGHT / (freqs[f] * 1.e3);
+  const double zdiff = fabs(w.hgt_lev[(size_t)col * (d.ML + 1) + z + 1] -
+                            w.hgt_lev[(size_t)col * (d.ML + 1) + z]);
+  const double gas = fmax(w.kgas[((size_t)col * d.nf + f) * d.ML + z], 0.0);
+  const double p0 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z + 1], wl);
+  const double p1 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z], wl);
+  const double extinct = kext + gas;
+  const double fnd = log(fmax(extinct * zdiff, 1.0e-7) / 1.0e-6) / (double)logf(2.0f);
+  const int nd = (fnd > 0.0) ? (int)fnd + 1 : 0;
+  const double nsub = (double)ldexpf(1.0f, nd);
+  const double dz = zdiff / nsub;
+  double linfac = (p0 == 0.0) ? 0.0 : (p1 / p0 - 1.0) / nsub;
+  // ---- INITIALIZE4 / INITIAL_SOURCE4, in place: R from the (-,+) quadrant
+  // in A, T from the (+,+) quadrant in B
+  const double C2 = 2.0 * 3.1415926535897932384;
+  for (int e = t; e < NN2; e += RT_WG) {
+    const int r = e / NN, c = e % NN;
+    const int i2 = r % NST, j2 = r / NST, i1 = c % NST, j1 = c / NST;
+    const double tmp = dz / smu[j2];
+    const double gext = (i1 == i2) ? gas : 0.0;
+    const double diag = (r == c) ? 1.0 : 0.0;
+    const double ext = (j1 == j2) ? ((i1 == i2) ? kext : 0.0) + gext : 0.0;
+    sA[r * LD + c] = C2 * tmp * sqw[j1] * sA[r * LD + c];
+    sB[r * LD + c] = diag - tmp * (ext - C2 * sqw[j1] * sB[r * LD + c]);
+  }
+  if (t < NN) {
+    const int i = t % NST, j = t / NST;
+    const double e = (i == 0) ? gas : 0.0;
+    const double tmp = p0 * dz / smu[j];
+    const double v = tmp * ((i == 0 ? absorb : 0.0) + e);
+    sv[0][t] = v;  // S+
+    sv[1][t] = v;  // S-
+    sv[2][t] = v;  // C+
+    sv[3][t] = v;  // C-
+  }
+  __syncthreads();
+  // ---- doubling (DOUBLING_INTEGRATION4, SYMMETRIC)
+  int nprod = 0;
+  for (int it = 0; it < nd; it++) {
+    if (!dbl_step3<DBL_STEP_UNR>(sA, sB, sC, sv, linfac, L.wmax, nprod)) return false;
+    linfac = 2.0 * linfac;
+  }
+  // ---- store R, T, S+, S-
+  double *R = w.R + (size_t)itf * NN2, *T = w.T + (size_t)itf * NN2;
+  for (int e = t; e < NN2; e += RT_WG) {
+    int i = e / NN, j = e % NN;
+    R[e] = sA[i * LD + j];
+    T[e] = sB[i * LD + j];
+  }
+  if (t < NN) {
+    w.Sp[(size_t)itf * NN + t] = sv[0][t];
+    w.Sm[(size_t)itf * NN + t] = sv[1][t];
+  }
+  if (t == 0) {
+    w.ndbl[itf] = nd;
+    atomicAdd(&w.counters[3], (unsigned long long)nd);
+    if (nleg > 0 && s.lhyd_scat)
+      atomicAdd(&w.counters[8], (unsigned long long)(numpts / 2 + 1) * (nleg + 1));
+    atomicAdd(&w.counters[9], (unsigned long long)nprod);
+  }
+  return true;
+}
+
 // One unit per workgroup, or (PERS) a persistent grid taking units from a
 // work queue.  The two are separate instantiations, so that the loop does
 // not cost the one-unit kernel registers.
-// four waves per SIMD at N = 32; at N = 64 the 141 KB of LDS admit one
-// workgroup per CU, and the registers are left to the larger tiles
-constexpr int DBL_WAVES = (NN == 32) ? 4 : 2;
+// five waves per SIMD at N = 32 (DblLds3, 28.8 KB: five workgroups per CU);
+// at N = 64 its 107 KB admit one workgroup per CU, and the registers are left
+// to the larger tiles
+constexpr int DBL_WAVES = (NN == 32) ? 5 : 2;
 template <bool PERS>
 __global__ void __launch_bounds__(RT_WG) __attribute__((amdgpu_waves_per_eu(DBL_WAVES))) k_doubling(Dims d, Settings s, const double *freqs,
                                                     Work w) {
-  __shared__ DblLds L;
+  __shared__ DblLds3 L;
   const int n = d.nitems * d.nf;
   int itf = PERS ? next_unit(&w.queue[1], &L.slot) : (int)blockIdx.x;
   while (itf < n) {
     // inverse_gj == 2 (a test mode) queues every unit for the Gauss-Jordan pass
-    const bool done = s.inverse_gj != 2 && doubling_unit<false>(d, s, freqs, w, itf, L);
+    const bool done = s.inverse_gj != 2 && doubling_unit3(d, s, freqs, w, itf, L);
     if (!done && threadIdx.x == 0) {
       const unsigned q = atomicAdd(&w.redo_n[0], 1u);
       w.redo_dbl[q] = itf;

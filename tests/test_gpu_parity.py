@@ -125,12 +125,17 @@ def test_gauss_jordan_and_series_inverses_agree(monkeypatch):
     _compare(b, c)
     # every unit through the queue of the Gauss-Jordan pass (the path of a
     # unit with an inverse whose series does not converge): the pass tries
-    # the series per inverse, so it reproduces the series result
+    # the same series per inverse, with G formed explicitly where the hot
+    # kernel applies the factors to T, so it agrees to the rounding of that
+    # association
     monkeypatch.setenv("PAM_INVERSE", "queue-all")
     q, st3 = _gpu(pam, freqs)
     monkeypatch.delenv("PAM_INVERSE")
     assert st3 == 0
-    np.testing.assert_allclose(q["tb"], a["tb"], rtol=0, atol=1e-9)
+    dq = np.max(np.abs(q["tb"] - a["tb"]))
+    print(f"queue pass vs hot kernel max |dTB| = {dq:.3e} K", flush=True)
+    assert dq <= 1e-6
+    _compare(q, c)
 
 
 def test_optically_thick_scattering_vs_oracle():
