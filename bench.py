@@ -144,6 +144,32 @@ def serve(args, precision="fp64", tuning=None, rate=50.0):
     return summ
 
 
+def plain_dropin_rate(args):
+    """The plain FP64 drop-in on the serving mix: each request through
+    pyPamtra.runPamtra in turn, as a user of the reference would call it
+    (host arrays staged over PCIe, one call per request, no batching across
+    requests).  The serving legs are compared with this rate."""
+    from pamtra_amd import workloads
+    mix = workloads.serving_requests(args.serve_requests, seed=1004)
+    # checkData=False: the padded levels above a column's own layers hold
+    # values the range check refuses (the serving path reads only the column's
+    # layers and skips the host check too)
+    mix[0][0].runPamtra(mix[0][1], checkData=False)  # warm the context and the tables
+    work = 0
+    t0 = time.perf_counter()
+    for i, (p, f) in enumerate(mix):
+        p.set["precision"] = "fp64"
+        p.runPamtra(f, checkData=False)
+        work += p.p["ngridx"] * p.p["ngridy"] * len(f)
+        if i % 20 == 19:
+            log(f"plain drop-in: {i + 1} requests done")
+    dt = time.perf_counter() - t0
+    rate = work / dt
+    log(f"plain FP64 drop-in on the serving mix: {rate:.0f} col*ch/s ({dt:.1f} s)")
+    return {"value": round(rate, 2), "seconds": round(dt, 3), "columns_channels": int(work),
+            "path": "pyPamtra.runPamtra per request, FP64, host arrays, one request at a time"}
+
+
 def large_domain(args, torch):
     """The large-domain configuration (150 layers, 12 channels 10.65-325 GHz,
     four categories of 100 bins, 32 angles per hemisphere: 64x64 operators)
@@ -495,8 +521,13 @@ def main():
         return
     rates = [float(x) for x in str(args.serve_rate).split(",") if x.strip()]
     if args.serve_only:
-        print(json.dumps({"serving": {"fp64": [serve(args, rate=r) for r in rates],
-                                      "mixed": [serve(args, "mixed", rate=r) for r in rates]}}))
+        plain = plain_dropin_rate(args)
+        res = {"plain_fp64_dropin": plain, "fp64": [serve(args, rate=r) for r in rates],
+               "mixed": [serve(args, "mixed", rate=r) for r in rates]}
+        for p in ("fp64", "mixed"):
+            for e in res[p]:
+                e["vs_plain_fp64_dropin"] = round(e["throughput"] / plain["value"], 3)
+        print(json.dumps({"serving": res}))
         return
 
     t0 = time.time()
@@ -729,11 +760,16 @@ def main():
     }
 
     if args.serve_requests > 0:
-        out["serving"] = {"fp64": [serve(args, tuning=tuning and tuning.get("fp64"), rate=r)
+        plain = plain_dropin_rate(args)
+        out["serving"] = {"plain_fp64_dropin": plain,
+                          "fp64": [serve(args, tuning=tuning and tuning.get("fp64"), rate=r)
                                    for r in rates]}
         if not args.no_mixed:
             out["serving"]["mixed"] = [serve(args, "mixed", tuning=tuning and tuning.get("mixed"),
                                              rate=r) for r in rates]
+        for p in ("fp64", "mixed"):
+            for e in out["serving"].get(p, []):
+                e["vs_plain_fp64_dropin"] = round(e["throughput"] / plain["value"], 3)
 
     if args.config3_cols > 0:
         out["config3"] = config3_leg(args, torch)
