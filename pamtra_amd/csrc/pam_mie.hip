@@ -987,6 +987,9 @@ __device__ __forceinline__ void mie_unit(const Dims &d, const Settings &s,
 constexpr int MIE2_WG = 128;
 constexpr int MIE2_NW = MIE2_WG / 64;
 constexpr int MIE2_TAB = 768;          // LDS stage of c_n pi_n, c_n tau_n (n x angles)
+// extinction [1/m] below which the mixed path skips a size bin: over a 20 km
+// column a layer of such bins changes the optical depth by < 1e-8 per bin
+constexpr double MIE2_CULL_EXT = 1e-13;
 constexpr int MIE2_NL = (MAXNLEG + MIE2_WG - 1) / MIE2_WG;  // Legendre orders per thread
 
 // Rows of the U slot.  FP64 (16x16x4 f64 MFMA, lane group k holds result
@@ -1305,6 +1308,16 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
         const int ns = mie_nstop(PI * dia[ir] / wavelength);
         if (ns > nterms) set_err(st, E_MIE_TERMS);
         nmie = min(ns, nterms);
+        // The mixed path of a passive run skips a bin whose extinction is
+        // bounded below MIE2_CULL_EXT (its particles' extinction cross
+        // section taken as at most 5 pi r^2): these are the large bins far in
+        // an exponential tail, whose long series set the length of this
+        // phase.  The hydrometeor's series length, quadrature and Legendre
+        // orders still follow its largest bin, as in the FP64 path.
+        if constexpr (F32) {
+          const double r = 0.5 * dia[ir];
+          if (!s.active && nd[ir] * dd[ir] * (5.0 * PI * r * r) < MIE2_CULL_EXT) nmie = 0;
+        }
       }
       L.nmb[ir] = (short)nmie;
       if (nmie <= 0) continue;
@@ -1378,6 +1391,7 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
           const int b = 8 * rt + q;
           nmt = max(nmt, b < nbin ? (int)L.nmb[b] : 0);
         }
+        if (nmt == 0) continue;  // no bin of the tile scatters: its terms are zero
         const int brow = 8 * rt + M2::tile_bin(li);
         const int nrow = brow < nbin ? (int)L.nmb[brow] : 0;
         const int np = max(4, (nmt + 3) & ~3);
