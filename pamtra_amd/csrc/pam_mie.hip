@@ -990,6 +990,12 @@ constexpr int MIE2_TAB = 768;          // LDS stage of c_n pi_n, c_n tau_n (n x 
 // extinction [1/m] below which the mixed path skips a size bin: over a 20 km
 // column a layer of such bins changes the optical depth by < 1e-8 per bin
 constexpr double MIE2_CULL_EXT = 1e-13;
+// a bin of number density nd [1/m^4], width dd and diameter dia [m] whose
+// extinction is bounded below MIE2_CULL_EXT (cross section <= 5 pi r^2)
+__device__ __forceinline__ bool mie2_negligible(double nd, double dd, double dia) {
+  const double r = 0.5 * dia;
+  return nd * dd * (5.0 * PI * r * r) < MIE2_CULL_EXT;
+}
 constexpr int MIE2_NL = (MAXNLEG + MIE2_WG - 1) / MIE2_WG;  // Legendre orders per thread
 
 // Rows of the U slot.  FP64 (16x16x4 f64 MFMA, lane group k holds result
@@ -1198,6 +1204,8 @@ struct Mie2Lds {
   int redi[MIE2_NW][2];
   double hm[MAX_HYDRO][2];
   int hnb[MAX_HYDRO], hnt[MAX_HYDRO];
+  int hkeep[MAX_HYDRO];  // mixed, passive: largest bin kept (-1: none skipped)
+  int hscan[MAX_HYDRO];  // 1: the largest bin is negligible, search the largest kept
   double lc[NCOEF][MAXNLEG + 1];  // Legendre sums over hydrometeors (thread l owns order l)
   int nlh, nan_;
   int slot;
@@ -1243,11 +1251,50 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
     }
     L.hnb[t] = nbin;
     L.hnt[t] = nterms;
+    // culling (below): -2 marks a hydrometeor whose largest bin is negligible
+    // and whose largest kept bin is searched; -1: no bin skipped
+    int keep = -1;
+    if constexpr (F32) {
+      if (!s.active && nbin > 0) {
+        const size_t o = ((size_t)li_item * d.nh + t) * d.nbmax + nbin - 1;
+        keep = mie2_negligible(w.dsd_nd[o], w.dsd_dd[o], w.dsd_d[o]) ? -2 : nbin - 1;
+      }
+    }
+    L.hkeep[t] = keep;
+    L.hscan[t] = keep == -2;
     L.hm[t][0] = m.re;
     L.hm[t][1] = m.im;
   }
   for (int k = t; k < NCOEF * (MAXNLEG + 1); k += MIE2_WG) (&L.lc[0][0])[k] = 0.0;
   __syncthreads();
+  // The mixed path of a passive run skips the bins whose extinction is
+  // bounded below MIE2_CULL_EXT (their particles' extinction cross section
+  // taken as at most 5 pi r^2): the large bins far in an exponential tail.
+  // A hydrometeor's series length, and with it its Gauss quadrature and
+  // Legendre orders (mie_spheres.f90:139-158), then follow its largest kept
+  // bin.  A hydrometeor none of whose bins would be kept keeps them all.
+  bool cull = false;
+  if constexpr (F32) cull = !s.active;
+  bool scan = false;
+  for (int hh = 0; hh < d.nh; hh++) scan = scan || L.hscan[hh];
+  if (cull && scan) {
+    for (int e = t; e < d.nh * d.nbmax; e += MIE2_WG) {
+      const int hh = e / d.nbmax, ir = e - hh * d.nbmax;
+      if (L.hscan[hh] && ir < L.hnb[hh]) {
+        const size_t o = ((size_t)li_item * d.nh + hh) * d.nbmax + ir;
+        if (!mie2_negligible(w.dsd_nd[o], w.dsd_dd[o], w.dsd_d[o])) atomicMax(&L.hkeep[hh], ir);
+      }
+    }
+    __syncthreads();
+    if (t < d.nh) {
+      if (L.hkeep[t] >= 0)
+        L.hnt[t] = mie_nstop(PI * w.dsd_d[((size_t)li_item * d.nh + t) * d.nbmax + L.hkeep[t]] /
+                             wavelength);
+      else
+        L.hkeep[t] = -1;  // no bin kept: none skipped
+    }
+    __syncthreads();
+  }
   double kexttot = 0.0, salbedo = 0.0, backtot = 0.0;
   int nleg_coef = 0;
   long long work_bins = 0, work_series = 0, work_ang = 0, work_proj = 0;
@@ -1261,6 +1308,7 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
     const cd m_ice = cd{L.hm[hh][0], -L.hm[hh][1]};
     const bool is_ice = (hyd[hh].liq_ice == -1);
     const int nterms = L.hnt[hh];
+    const bool hcull = cull && L.hkeep[hh] >= 0;
     if (nterms < 1 || nterms > w.atab_nmax || nterms > MAX_NTERMS || nbin > MIE_MAXBIN ||
         16 * ((nbin + 7) >> 3) > ld) {
       if (t == 0) set_err(st, E_MIE_TERMS);
@@ -1304,20 +1352,10 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
     int nbw = 0, nsw = 0;
     for (int ir = t; ir < nbin; ir += MIE2_WG) {
       int nmie = 0;
-      if (nd[ir] > 0.0) {
+      if (nd[ir] > 0.0 && !(hcull && mie2_negligible(nd[ir], dd[ir], dia[ir]))) {
         const int ns = mie_nstop(PI * dia[ir] / wavelength);
         if (ns > nterms) set_err(st, E_MIE_TERMS);
         nmie = min(ns, nterms);
-        // The mixed path of a passive run skips a bin whose extinction is
-        // bounded below MIE2_CULL_EXT (its particles' extinction cross
-        // section taken as at most 5 pi r^2): these are the large bins far in
-        // an exponential tail, whose long series set the length of this
-        // phase.  The hydrometeor's series length, quadrature and Legendre
-        // orders still follow its largest bin, as in the FP64 path.
-        if constexpr (F32) {
-          const double r = 0.5 * dia[ir];
-          if (!s.active && nd[ir] * dd[ir] * (5.0 * PI * r * r) < MIE2_CULL_EXT) nmie = 0;
-        }
       }
       L.nmb[ir] = (short)nmie;
       if (nmie <= 0) continue;
