@@ -106,12 +106,13 @@ struct DevBuf {
 // Mie angle tables of one device, shared by all its contexts and kept until
 // the process ends (66 MiB, and their 33 MiB FP32 copy).
 struct AngleTables {
-  DevBuf buf, pbuf, fbuf;
+  DevBuf buf, pbuf, fbuf, pfbuf;
   const double *tab = nullptr;
   const float *tab_f = nullptr;
   const int64_t *off = nullptr;
   int nmax = 0;
   const double *ptab = nullptr;  // phase-matrix tables (about 200 MiB)
+  const float *ptab_f = nullptr;  // their FP32 copy (the mixed path)
   const int64_t *poff = nullptr;
 };
 static void lobatto(int num, double *mu, double *w);
@@ -157,11 +158,14 @@ static AngleTables *device_tables(int device) {
       lobatto(NMU, mu, qw);
       ok = hipMemcpy(dpoff, poff, sizeof(poff), hipMemcpyHostToDevice) == hipSuccess &&
            hipMemcpy(dmu, mu, sizeof(mu), hipMemcpyHostToDevice) == hipSuccess;
+      if (ok) ok = t->pfbuf.ensure((size_t)pn * 4) == hipSuccess;
       if (ok) {
         build_phase_tables(ptab, dpoff, dmu, nullptr);
+        round_tables_f32(ptab, (float *)t->pfbuf.p, pn, nullptr);
         ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
       }
       t->ptab = ptab;
+      t->ptab_f = (const float *)t->pfbuf.p;
       t->poff = dpoff;
     }
   }
@@ -169,6 +173,7 @@ static AngleTables *device_tables(int device) {
     t->buf.release();
     t->pbuf.release();
     t->fbuf.release();
+    t->pfbuf.release();
     delete t;
     return nullptr;
   }
@@ -188,6 +193,7 @@ struct pam_context {
   const int64_t *d_atab_off = nullptr;
   int atab_nmax = 0;
   const double *d_ptab = nullptr;
+  const float *d_ptab_f = nullptr;
   const int64_t *d_ptab_off = nullptr;
   double *d_mu = nullptr, *d_qw = nullptr;
   double mu[NMU], qw[NMU];
@@ -389,6 +395,7 @@ extern "C" int pam_context_create(pam_context **out, int device) {
     c->d_atab_off = t->off;
     c->atab_nmax = t->nmax;
     c->d_ptab = t->ptab;
+    c->d_ptab_f = t->ptab_f;
     c->d_ptab_off = t->poff;
   }
   // per-call workspace budget: half of the free HBM, at most 96 GiB
@@ -861,6 +868,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   w.atab_off = c->d_atab_off;
   w.atab_nmax = c->atab_nmax;
   w.ptab = c->d_ptab;
+  w.ptab_f = c->d_ptab_f;
   w.ptab_off = c->d_ptab_off;
   w.ph_cnt = d_ph;
   w.ph_off = d_ph + (PH_NBUCKET + 1);

@@ -166,6 +166,7 @@ struct Work {
   // of the chunk gathered by NUMPTS
   const double *ptab;
   const int64_t *ptab_off;             // [PH_NBUCKET + 1]
+  const float *ptab_f;                 // the same tables rounded to FP32 (mixed path)
   uint32_t *ph_cnt, *ph_off, *ph_toff; // [PH_NBUCKET + 1]
   int32_t *ph_rank, *ph_perm;          // [nitems*nf]
   const double *mu, *qw;               // Lobatto nodes/weights [NMU]
@@ -193,6 +194,15 @@ struct Work {
 // ------------------------------------------------------------------
 
 __device__ __forceinline__ double sp(double x) { return (double)(float)x; }
+// The phase matrix k_phase leaves in a unit's R and T slots: FP32 in the
+// mixed path (formed from the FP32 copy of its tables), FP64 otherwise.
+__device__ __forceinline__ bool phase_in_f32(const Settings &s, const Work &w) {
+  return s.precision == PAM_PREC_MIXED && w.ptab_f != nullptr;
+}
+// entry e of the quadrant stored at `slot`
+__device__ __forceinline__ double phase_at(bool f32, const double *slot, int e) {
+  return f32 ? (double)reinterpret_cast<const float *>(slot)[e] : slot[e];
+}
 
 struct cd {
   double re, im;

@@ -1204,8 +1204,10 @@ struct Mie2Lds {
   int redi[MIE2_NW][2];
   double hm[MAX_HYDRO][2];
   int hnb[MAX_HYDRO], hnt[MAX_HYDRO];
-  int hkeep[MAX_HYDRO];  // mixed, passive: largest bin kept (-1: none skipped)
-  int hscan[MAX_HYDRO];  // 1: the largest bin is negligible, search the largest kept
+  // FP32 instance only (the FP64 one is at its LDS limit of six workgroups per
+  // CU): the largest bin kept (-1: none skipped), and whether it is searched
+  static constexpr int NHK = sizeof(R) == 4 ? MAX_HYDRO : 1;
+  int hkeep[NHK], hscan[NHK];
   double lc[NCOEF][MAXNLEG + 1];  // Legendre sums over hydrometeors (thread l owns order l)
   int nlh, nan_;
   int slot;
@@ -1253,15 +1255,15 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
     L.hnt[t] = nterms;
     // culling (below): -2 marks a hydrometeor whose largest bin is negligible
     // and whose largest kept bin is searched; -1: no bin skipped
-    int keep = -1;
     if constexpr (F32) {
+      int keep = -1;
       if (!s.active && nbin > 0) {
         const size_t o = ((size_t)li_item * d.nh + t) * d.nbmax + nbin - 1;
         keep = mie2_negligible(w.dsd_nd[o], w.dsd_dd[o], w.dsd_d[o]) ? -2 : nbin - 1;
       }
+      L.hkeep[t] = keep;
+      L.hscan[t] = keep == -2;
     }
-    L.hkeep[t] = keep;
-    L.hscan[t] = keep == -2;
     L.hm[t][0] = m.re;
     L.hm[t][1] = m.im;
   }
@@ -1273,11 +1275,12 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
   // A hydrometeor's series length, and with it its Gauss quadrature and
   // Legendre orders (mie_spheres.f90:139-158), then follow its largest kept
   // bin.  A hydrometeor none of whose bins would be kept keeps them all.
-  bool cull = false;
-  if constexpr (F32) cull = !s.active;
-  bool scan = false;
-  for (int hh = 0; hh < d.nh; hh++) scan = scan || L.hscan[hh];
-  if (cull && scan) {
+  bool cull = false, scan = false;
+  if constexpr (F32) {
+    cull = !s.active;
+    for (int hh = 0; hh < d.nh; hh++) scan = scan || L.hscan[hh];
+  }
+  if constexpr (F32) if (cull && scan) {
     for (int e = t; e < d.nh * d.nbmax; e += MIE2_WG) {
       const int hh = e / d.nbmax, ir = e - hh * d.nbmax;
       if (L.hscan[hh] && ir < L.hnb[hh]) {
@@ -1308,7 +1311,8 @@ __device__ __forceinline__ void mie2_unit(const Dims &d, const Settings &s,
     const cd m_ice = cd{L.hm[hh][0], -L.hm[hh][1]};
     const bool is_ice = (hyd[hh].liq_ice == -1);
     const int nterms = L.hnt[hh];
-    const bool hcull = cull && L.hkeep[hh] >= 0;
+    bool hcull = false;
+    if constexpr (F32) hcull = cull && L.hkeep[hh] >= 0;
     if (nterms < 1 || nterms > w.atab_nmax || nterms > MAX_NTERMS || nbin > MIE_MAXBIN ||
         16 * ((nbin + 7) >> 3) > ld) {
       if (t == 0) set_err(st, E_MIE_TERMS);

@@ -162,13 +162,38 @@ __global__ void k_phase_scatter(int nif, Settings s, Work w) {
 }
 
 typedef double d4p_t __attribute__((ext_vector_type(4)));
+typedef float f4p_t __attribute__((ext_vector_type(4)));
+
+// The matrix-core step of k_phase per element type.  FP64 (16x16x4 f64):
+// lane l supplies A[l%16][l/16], B[l/16][l%16] and holds D[4*r+l/16][l%16],
+// so its result r belongs to the tile's unit 4 r + l/16.  FP32 (16x16x4
+// f32): lane l holds D[4*(l/16)+r][l%16], unit 4 (l/16) + r.
+template <typename R> struct PhMfma;
+template <> struct PhMfma<double> {
+  typedef d4p_t acc_t;
+  static __device__ __forceinline__ acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int unit(int lk, int r) { return 4 * r + lk; }
+};
+template <> struct PhMfma<float> {
+  typedef f4p_t acc_t;
+  static __device__ __forceinline__ acc_t step(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int unit(int lk, int r) { return 4 * lk + r; }
+};
 
 // One tile: up to 16 units of one NUMPTS.  Each wave computes 16 of the 64
-// output tile pairs (16 angle pairs, entries e11/e12 or e21/e22).  16x16x4 f64
-// MFMA layout: lane l supplies A[l%16][l/16], B[l/16][l%16] and holds
-// D[4*r+l/16][l%16].
+// output tile pairs (16 angle pairs, entries e11/e12 or e21/e22).  R = double:
+// the FP64 tables and phase matrix; R = float (the mixed path): the FP32 copy
+// of the tables, FP32 products and an FP32 phase matrix, half the bytes read
+// and written.
+template <typename R>
 __global__ void __launch_bounds__(PH_WG) k_phase(Work w) {
-  __shared__ double sA[3][PH_TILE][PH_L1MAX + 3];  // scaled series of the tile's units
+  typedef PhMfma<R> MF;
+  typedef typename MF::acc_t acc_t;
+  __shared__ R sA[3][PH_TILE][PH_L1MAX + 3];  // scaled series of the tile's units
   __shared__ int s_unit[PH_TILE];
   __shared__ int s_m, s_first, s_count;
   const int t = threadIdx.x, wv = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
@@ -203,41 +228,45 @@ __global__ void __launch_bounds__(PH_WG) k_phase(Work w) {
         v = cst * w.coef[(size_t)itf * NCOEF * MAXNLEG + c * MAXNLEG + l];
       }
     }
-    sA[c][i][l] = v;
+    sA[c][i][l] = (R)v;
   }
   __syncthreads();
-  const double *Q = w.ptab + w.ptab_off[m];
+  const R *Q = (sizeof(R) == 4 ? (const R *)w.ptab_f : (const R *)w.ptab) + w.ptab_off[m];
   // entries (e11, e12) and (e21, e22) of 16 pairs per iteration, so that a
   // lane stores two adjacent doubles of its row
   for (int T = wv; T < 2 * (PH_NPAIR / 16); T += 4) {
     const int hi = T / (PH_NPAIR / 16), pt = T % (PH_NPAIR / 16);
     const int pair = 16 * pt + li;
-    d4p_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    acc_t acc0 = {(R)0, (R)0, (R)0, (R)0}, acc1 = acc0;
     // hi = 0: e11 = c1 Q1, e12 = c2 Q12; hi = 1: e21 = c2 Q21, e22 = c1 Qa - c3 Qb
     const int ca0 = hi ? 1 : 0, qa0 = hi ? 2 : 0, ca1 = hi ? 0 : 1, qa1 = hi ? 3 : 1;
     for (int sk = 0; sk < L1p / 4; sk++) {
       const int l = 4 * sk + lk;
       const bool in = l < L1;
-      const double b0 = in ? Q[((size_t)qa0 * L1 + l) * PH_NPAIR + pair] : 0.0;
-      const double b1 = in ? Q[((size_t)qa1 * L1 + l) * PH_NPAIR + pair] : 0.0;
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[ca0][li][l], b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[ca1][li][l], b1, acc1, 0, 0, 0);
+      const R b0 = in ? Q[((size_t)qa0 * L1 + l) * PH_NPAIR + pair] : (R)0;
+      const R b1 = in ? Q[((size_t)qa1 * L1 + l) * PH_NPAIR + pair] : (R)0;
+      acc0 = MF::step(sA[ca0][li][l], b0, acc0);
+      acc1 = MF::step(sA[ca1][li][l], b1, acc1);
     }
     if (hi) {  // e22: - sum_l c3_l Qb_l
       for (int sk = 0; sk < L1p / 4; sk++) {
         const int l = 4 * sk + lk;
-        const double b = l < L1 ? -Q[((size_t)4 * L1 + l) * PH_NPAIR + pair] : 0.0;
-        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[2][li][l], b, acc1, 0, 0, 0);
+        const R b = l < L1 ? -Q[((size_t)4 * L1 + l) * PH_NPAIR + pair] : (R)0;
+        acc1 = MF::step(sA[2][li][l], b, acc1);
       }
     }
     const int q = pair / (NMU * NMU), j2 = (pair / NMU) % NMU, j1 = pair % NMU;
     const int r = NST * j2 + hi, c = NST * j1;
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
-      const int itf = s_unit[4 * rr + lk];
+      const int itf = s_unit[MF::unit(lk, rr)];
       if (itf >= 0) {
-        double2 *dst = reinterpret_cast<double2 *>((q ? w.R : w.T) + (size_t)itf * NN2 + r * NN + c);
-        *dst = make_double2(acc0[rr], acc1[rr]);
+        double *slot = (q ? w.R : w.T) + (size_t)itf * NN2;
+        if constexpr (sizeof(R) == 4)
+          *reinterpret_cast<float2 *>(reinterpret_cast<float *>(slot) + r * NN + c) =
+              make_float2(acc0[rr], acc1[rr]);
+        else
+          *reinterpret_cast<double2 *>(slot + r * NN + c) = make_double2(acc0[rr], acc1[rr]);
       }
     }
   }
@@ -252,7 +281,10 @@ void launch_phase(const Dims &d, const Settings &s, Work &w, hipStream_t st) {
   hipLaunchKernelGGL(k_phase_scan, dim3(1), dim3(64), 0, st, w);
   hipLaunchKernelGGL(k_phase_scatter, dim3(nb), dim3(256), 0, st, nif, s, w);
   const int ntile_max = nif / PH_TILE + PH_NBUCKET;
-  hipLaunchKernelGGL(k_phase, dim3(ntile_max), dim3(PH_WG), 0, st, w);
+  if (s.precision == PAM_PREC_MIXED && w.ptab_f)  // as phase_in_f32
+    hipLaunchKernelGGL(k_phase<float>, dim3(ntile_max), dim3(PH_WG), 0, st, w);
+  else
+    hipLaunchKernelGGL(k_phase<double>, dim3(ntile_max), dim3(PH_WG), 0, st, w);
 }
 
 }  // namespace pam

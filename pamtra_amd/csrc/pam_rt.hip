@@ -546,12 +546,12 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
   // ((+,+) quadrant) and R ((-,+) quadrant) slots
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
   {
-    const bool has = nleg > 0 && s.lhyd_scat;
+    const bool has = nleg > 0 && s.lhyd_scat, pf = phase_in_f32(s, w);
     const double *P0 = w.T + (size_t)itf * NN2, *P1 = w.R + (size_t)itf * NN2;
     for (int e = t; e < NN2; e += RT_WG) {
       const int i = e / NN, j = e % NN;
-      sX[i * LD + j] = has ? P0[e] : 0.0;
-      sY[i * LD + j] = has ? P1[e] : 0.0;
+      sX[i * LD + j] = has ? phase_at(pf, P0, e) : 0.0;
+      sY[i * LD + j] = has ? phase_at(pf, P1, e) : 0.0;
     }
   }
   __syncthreads();
@@ -898,12 +898,12 @@ __device__ bool doubling_unit3(const Dims &d, const Settings &s, const double *f
   // ((+,+) quadrant, into B) and R ((-,+) quadrant, into A) slots
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
   {
-    const bool has = nleg > 0 && s.lhyd_scat;
+    const bool has = nleg > 0 && s.lhyd_scat, pf = phase_in_f32(s, w);
     const double *P0 = w.T + (size_t)itf * NN2, *P1 = w.R + (size_t)itf * NN2;
     for (int e = t; e < NN2; e += RT_WG) {
       const int i = e / NN, j = e % NN;
-      sB[i * LD + j] = has ? P0[e] : 0.0;
-      sA[i * LD + j] = has ? P1[e] : 0.0;
+      sB[i * LD + j] = has ? phase_at(pf, P0, e) : 0.0;
+      sA[i * LD + j] = has ? phase_at(pf, P1, e) : 0.0;
     }
   }
   __syncthreads();
@@ -1308,12 +1308,12 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   // staged in FP64 over the LDS of the four FP32 matrices
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
   {
-    const bool has = nleg > 0 && s.lhyd_scat;
+    const bool has = nleg > 0 && s.lhyd_scat, pf = phase_in_f32(s, w);
     const double *G0 = w.T + (size_t)itf * NN2, *G1 = w.R + (size_t)itf * NN2;
     double *D0 = reinterpret_cast<double *>(L.m[0]), *D1 = reinterpret_cast<double *>(L.m[2]);
     for (int e = t; e < NN2; e += RT_WG) {
-      D0[e] = has ? G0[e] : 0.0;
-      D1[e] = has ? G1[e] : 0.0;
+      D0[e] = has ? phase_at(pf, G0, e) : 0.0;
+      D1[e] = has ? phase_at(pf, G1, e) : 0.0;
     }
   }
   const double *P0 = reinterpret_cast<const double *>(L.m[0]);
