@@ -55,6 +55,13 @@ def lib(nummu: int = 16) -> C.CDLL:
         L.pam_oracle_gasabs_parts.argtypes = [C.c_double] * 4 + [dp, dp]
         L.pam_oracle_fastem_parts.argtypes = [C.c_double] * 5 + [dp]
         L.pam_oracle_eps_mix.argtypes = [C.c_double] * 3 + [dp, dp]
+        fp = C.POINTER(C.c_float)
+        L.pam_oracle_tm_tmatrix.restype = C.c_int
+        L.pam_oracle_tm_tmatrix.argtypes = [C.c_double] * 5 + [fp, C.c_int]
+        L.pam_oracle_tm_single.restype = C.c_int
+        L.pam_oracle_tm_single.argtypes = [C.c_double] * 6 + [dp] * 4
+        L.pam_oracle_calc_tmatrix.restype = C.c_int
+        L.pam_oracle_calc_tmatrix.argtypes = [C.c_double] * 3 + [C.c_int, C.c_int] + [dp] * 10
         L.pam_oracle_mie_coefs.restype = C.c_int
         L.pam_oracle_mie_coefs.argtypes = [C.c_double] * 3 + [C.c_int, C.POINTER(C.c_int), dp, dp,
                                                               dp]
@@ -250,6 +257,33 @@ def ssrg(freq_hz, diameter, del_d, ndens, mass, as_ratio, canting, m, kappa, bet
                                m.real, m.imag, kappa, beta, gamma, zeta, _dp(out), _dp(legen),
                                C.byref(nl))
     return st, out[0], out[1], out[2], legen, nl.value
+
+
+def tm_tmatrix(lam, m, axi, as_ratio, cap=48):
+    """The stored (REAL*4) T matrix of one spheroid: (nmax, t[8][cap+1][cap][cap])
+    with components RT11, RT12, RT21, RT22, IT11, IT12, IT21, IT22 indexed
+    [m][n-1][n'-1].  m = n + i k (EBCM convention), lam and axi in metres."""
+    t = np.zeros((8, cap + 1, cap, cap), np.float32)
+    nm = lib().pam_oracle_tm_tmatrix(lam, m.real, abs(m.imag), axi, as_ratio,
+                                     t.ctypes.data_as(C.POINTER(C.c_float)), cap)
+    return nm, t
+
+
+def calc_tmatrix(freq_hz, m, phase, dmax, del_d, ndens, density, as_ratio, canting, nummu=16):
+    """calc_tmatrix over the bins of one hydrometeor.  m = n - i k, the
+    refractive index scatProperties passes.  Returns (status, smat, ext, emis,
+    sback) in the reference's Fortran shapes (2,nummu,2,nummu,2), (2,2,nummu),
+    (2,nummu), (2,2)."""
+    arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, float), (len(dmax),)))
+            for a in (dmax, del_d, ndens, density, as_ratio, canting)]
+    sm, ex = np.zeros(2 * nummu * 2 * nummu * 2), np.zeros(4 * nummu)
+    em, sb = np.zeros(2 * nummu), np.zeros(4)
+    st = lib(nummu).pam_oracle_calc_tmatrix(freq_hz, m.real, -m.imag, phase, len(dmax),
+                                            *[_dp(a) for a in arrs], _dp(sm), _dp(ex), _dp(em),
+                                            _dp(sb))
+    return (st, sm.reshape((2, nummu, 2, nummu, 2), order="F"),
+            ex.reshape((2, 2, nummu), order="F"), em.reshape((2, nummu), order="F"),
+            sb.reshape((2, 2), order="F"))
 
 
 def esat(t_k):

@@ -250,6 +250,96 @@ contains
     call scatcnv(e, int(nlegen), c, kext, salb, smat, ext, emis)
   end subroutine
 
+  ! calc_tmatrix (tmatrix.f90:14-444) for one hydrometeor of one layer, passive
+  ! only (active = .false.: every incidence angle), tmatrix_db = 'none', one
+  ! radar polarisation 'NN'.  freq in Hz; (mre, mim) the refractive index
+  ! m = mre - i mim the reference's scatProperties.f90:222 passes.
+  ! smat(2,16,2,16,2), ext(2,2,16), emis(2,16) as calc_tmatrix returns them;
+  ! sback(2,2) starts at zero (calc_tmatrix only adds to it).
+  function ref_calc_tmatrix(freq, mre, mim, phase, nbins, dmax, del_d, ndens, density, &
+                            as_ratio, canting, temp, smat, ext, emis, sback) &
+                            bind(C, name="ref_calc_tmatrix") result(st)
+    use tmatrix, only: calc_tmatrix
+    use settings, only: active, passive, tmatrix_db, radar_npol, radar_pol, &
+                        lhyd_absorption, nummu, nstokes
+    real(c_double), value :: freq, mre, mim, temp
+    integer(c_int), value :: phase, nbins
+    real(c_double), intent(in) :: dmax(nbins), del_d(nbins), ndens(nbins), density(nbins), &
+                                  as_ratio(nbins), canting(nbins)
+    real(c_double), intent(out) :: smat(nstokes, nummu, nstokes, nummu, 2), &
+                                   ext(nstokes, nstokes, nummu), emis(nstokes, nummu), sback(2, 2)
+    integer(c_int) :: st
+    integer(kind=long) :: e
+    real(kind=dbl) :: back_spec(1, nbins)
+    active = .false.
+    passive = .true.
+    tmatrix_db = 'none'
+    radar_npol = 1
+    radar_pol(1) = 'NN'
+    lhyd_absorption = .true.
+    sback = 0.d0
+    e = 0
+    call calc_tmatrix(e, freq, cmplx(mre, -mim, kind=dbl), int(phase), int(nbins), dmax, del_d, &
+                      ndens, density, as_ratio, canting, temp, smat, ext, emis, back_spec, sback)
+    st = int(e, c_int)
+  end function
+
+  ! tmatrix_amplq (tmatrix_amplq.lp.f:276-540) for one spheroid (NP = -1,
+  ! RAT = 1): nmax and the REAL*4 T matrix of COMMON /TMAT/ as
+  ! t(n', n, m, c) = t[c][m][n-1][n'-1] with the strides (cap+1, cap, cap) of a
+  ! C array, c = RT11, RT12, RT21, RT22, IT11, IT12, IT21, IT22.
+  function ref_tmatrix_amplq(lam, mrr, mri, axi, as_ratio, t, cap) &
+                             bind(C, name="ref_tmatrix_amplq") result(nmax)
+    real(c_double), value :: lam, mrr, mri, axi, as_ratio
+    integer(c_int), value :: cap
+    real(c_float), intent(out) :: t(cap, cap, 0:cap, 8)
+    integer(c_int) :: nmax
+    real*4 :: rt11(101, 100, 100), rt12(101, 100, 100), rt21(101, 100, 100), &
+              rt22(101, 100, 100), it11(101, 100, 100), it12(101, 100, 100), &
+              it21(101, 100, 100), it22(101, 100, 100)
+    common /tmat/ rt11, rt12, rt21, rt22, it11, it12, it21, it22
+    real*16 :: ql, qr, qi
+    real*8 :: ax, ar, rat
+    integer :: np, nm, m, n, nn
+    integer(kind=long) :: e
+    interface
+      subroutine tmatrix_amplq(errorstatus, lam, mrr, mri, axi, as_ratio, rat, np, nmax)
+        import :: long
+        integer(kind=long) :: errorstatus
+        real*16 :: lam, mrr, mri
+        real*8 :: axi, as_ratio, rat
+        integer :: np, nmax
+      end subroutine
+    end interface
+    ql = lam
+    qr = mrr
+    qi = mri
+    ax = axi
+    ar = as_ratio
+    rat = 1.d0
+    np = -1
+    e = 0
+    call tmatrix_amplq(e, ql, qr, qi, ax, ar, rat, np, nm)
+    nmax = 0
+    if (e /= 0) return
+    nmax = nm
+    if (nm > cap) return
+    do m = 0, nm
+      do n = 1, nm
+        do nn = 1, nm
+          t(nn, n, m, 1) = rt11(m + 1, n, nn)
+          t(nn, n, m, 2) = rt12(m + 1, n, nn)
+          t(nn, n, m, 3) = rt21(m + 1, n, nn)
+          t(nn, n, m, 4) = rt22(m + 1, n, nn)
+          t(nn, n, m, 5) = it11(m + 1, n, nn)
+          t(nn, n, m, 6) = it12(m + 1, n, nn)
+          t(nn, n, m, 7) = it21(m + 1, n, nn)
+          t(nn, n, m, 8) = it22(m + 1, n, nn)
+        end do
+      end do
+    end do
+  end function
+
   ! run_drop_size_dist (drop_size_dist.f90:107-218) for one (layer,
   ! hydrometeor), with the module inputs hydrometeor_extinction.f90:150-300
   ! sets.  dreal(1:14) = as_ratio, rho_ms, a_ms, b_ms, alpha_as, beta_as,

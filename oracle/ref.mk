@@ -28,8 +28,14 @@ UNITS := kinds constants vars_index report_module nan rt_utilities settings conv
   large_scale_correction_module small_scale_correction_module mie_scat_utilities \
   scat_utilities mie_spheres scatcnv rho_air make_mass_size make_dist_param make_dist \
   calc_moment check_print make_soft_spheroid
+# the T-matrix producer: calc_tmatrix (tmatrix.f90) over Mishchenko's
+# quad-precision EBCM code (tmatrix_amplq.lp.f, tmatrix_lpq.f).  zlib_stuff
+# (module fileio) declares the zlib routines of the tmatrix_db = 'file' cache;
+# the driver runs tmatrix_db = 'none', so they stay unresolved (lazy binding).
+TM_F77 := tmatrix_lpq tmatrix_amplq.lp
+TM_UNITS := zlib_stuff tmatrix
 
-OBJS := $(addprefix $(OUT)/,$(addsuffix .o,$(UNITS)))
+OBJS := $(addprefix $(OUT)/,$(addsuffix .o,$(UNITS) $(TM_F77) $(TM_UNITS)))
 
 all: $(OUT)/libpam_ref.so
 
@@ -38,6 +44,11 @@ $(OUT)/libpam_ref.so: $(HERE)pam_ref_driver.f90 $(HERE)ref.mk $(HERE)ref_exports
 	@set -e; for u in $(UNITS); do \
 	  $(REF_FC) $(FFLAGS) -module-dir $(OUT) -c $(REFSRC)/$$u.f90 -o $(OUT)/$$u.o; \
 	done
+	@set -e; for u in $(TM_F77); do \
+	  $(REF_FC) $(FFLAGS) -I$(REFSRC) -module-dir $(OUT) -c $(REFSRC)/$$u.f -o $(OUT)/$$u.o; \
+	done
+	$(REF_FC) $(FFLAGS) -cpp -module-dir $(OUT) -c $(REFSRC)/zlib_stuff.f90 -o $(OUT)/zlib_stuff.o
+	$(REF_FC) $(FFLAGS) -module-dir $(OUT) -c $(REFSRC)/tmatrix.f90 -o $(OUT)/tmatrix.o
 	$(REF_FC) $(FFLAGS) -module-dir $(OUT) -c $(HERE)pam_ref_driver.f90 -o $(OUT)/pam_ref_driver.o
 	$(REF_FC) -shared -fPIC $(LDFLAGS_REF) -o $@.tmp $(OBJS) $(OUT)/pam_ref_driver.o
 	mv $@.tmp $@

@@ -66,6 +66,11 @@ def lib() -> C.CDLL:
         L.ref_mie_spheres.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, dp, dp,
                                       C.c_double, C.c_double, dp, dp, dp, ip]
         L.ref_scatcnv.argtypes = [C.c_int, dp, C.c_double, C.c_double, dp, dp, dp]
+        L.ref_calc_tmatrix.restype = C.c_int
+        L.ref_calc_tmatrix.argtypes = ([C.c_double] * 3 + [C.c_int, C.c_int] + [dp] * 6 +
+                                       [C.c_double] + [dp] * 4)
+        L.ref_tmatrix_amplq.restype = C.c_int
+        L.ref_tmatrix_amplq.argtypes = [C.c_double] * 5 + [C.POINTER(C.c_float), C.c_int]
         L.ref_dsd.restype = C.c_int
         L.ref_dsd.argtypes = [C.c_char_p, dp, ip, dp, ip] + [dp] * 7
         _lib = L
@@ -158,6 +163,26 @@ def mie_spheres(freq_hz, t, liq_ice, diameter, del_d, ndens, density, m):
     st = lib().ref_mie_spheres(freq_hz, t, liq_ice, n, *[_dp(a) for a in arrs], m.real,
                                abs(m.imag), _dp(out), _dp(legen), _dp(bs), C.byref(nl))
     return st, out, legen, bs, nl.value
+
+
+def tmatrix_amplq(lam, m, axi, as_ratio, cap=48):
+    """tmatrix_amplq for one spheroid: (nmax, t) as oracle.tm_tmatrix."""
+    t = np.zeros((8, cap + 1, cap, cap), np.float32)
+    nm = lib().ref_tmatrix_amplq(lam, m.real, abs(m.imag), axi, as_ratio,
+                                 t.ctypes.data_as(C.POINTER(C.c_float)), cap)
+    return nm, t
+
+
+def calc_tmatrix(freq_hz, m, phase, dmax, del_d, ndens, density, as_ratio, canting, temp=273.15):
+    """calc_tmatrix (tmatrix_db 'none', passive) as oracle.calc_tmatrix."""
+    arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, float), (len(dmax),)))
+            for a in (dmax, del_d, ndens, density, as_ratio, canting)]
+    sm, ex, em, sb = np.zeros(2 * 16 * 2 * 16 * 2), np.zeros(64), np.zeros(32), np.zeros(4)
+    st = lib().ref_calc_tmatrix(freq_hz, m.real, -m.imag, phase, len(dmax),
+                                *[_dp(a) for a in arrs], temp, _dp(sm), _dp(ex), _dp(em),
+                                _dp(sb))
+    return (st, sm.reshape((2, 16, 2, 16, 2), order="F"), ex.reshape((2, 2, 16), order="F"),
+            em.reshape((2, 16), order="F"), sb.reshape((2, 2), order="F"))
 
 
 def scatcnv(nlegen, coef3, kext, salb, nummu=16):

@@ -1,0 +1,146 @@
+"""Pins of the T-matrix oracle (oracle/pam_tmatrix.c).
+
+- Against the reference's own Fortran (tmatrix.f90, tmatrix_amplq.lp.f,
+  tmatrix_lpq.f compiled unchanged by oracle/ref.mk): the stored REAL*4 T
+  matrix bit for bit, and calc_tmatrix's scattering matrix, extinction
+  matrix, emission vector and backscattering to 1e-12 relative (the
+  remainder is the libm of two runtimes in the amplitude matrix).
+- Without the reference: the sphere limit (as_ratio = 1), where the m = 0
+  T matrix is minus the Mie coefficients, b_n in T11 and a_n in T22, checked
+  against the textbook Riccati-Bessel formulas evaluated by scipy; the
+  energy balance the producer builds into its emission vector (emission =
+  extinction - integrated scattering); the forward-scattering (optical
+  theorem) extinction against the sphere's Mie extinction.
+"""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from oracle import refbind as R  # noqa: E402
+
+needs_ref = pytest.mark.skipif(not R.available(),
+                               reason="reference Fortran sources not present to build the pins")
+
+C_LIGHT = 299792458.0
+
+# (freq Hz, refractive index n - i k, phase, dmax m, as_ratio, canting deg, density)
+CASES = [
+    (35.5e9, 5.5 - 2.8j, 1, 1.0e-3, 0.8, 0.0, 1000.0),
+    (35.5e9, 5.5 - 2.8j, 1, 3.0e-3, 0.6, 0.0, 1000.0),
+    (94.0e9, 3.5 - 1.9j, 1, 4.0e-3, 0.7, 0.0, 1000.0),
+    (89.0e9, 1.78 - 0.003j, -1, 5.0e-3, 0.6, 0.0, 200.0),
+    (89.0e9, 1.78 - 0.003j, -1, 2.0e-3, 1.4, 0.0, 400.0),
+    (35.5e9, 5.5 - 2.8j, 1, 2.0e-3, 0.8, 20.0, 1000.0),
+    (10.65e9, 7.0 - 2.5j, 1, 1.0e-3, 1.0, 0.0, 1000.0),
+]
+
+
+def rel(a, b):
+    s = float(np.max(np.abs(b)))
+    return float(np.max(np.abs(a - b))) / s if s > 0 else float(np.max(np.abs(a - b)))
+
+
+@needs_ref
+@pytest.mark.parametrize("case", [(299792458 / 35.5e9, 5.5 + 2.8j, 0.5e-3 * 0.8 ** (1 / 3), 0.8),
+                                  (299792458 / 89e9, 1.78 + 0.003j, 2.5e-3 * 0.6 ** (1 / 3), 0.6),
+                                  (299792458 / 183.31e9, 1.78 + 0.004j, 1.5e-3, 1.5),
+                                  (299792458 / 10.65e9, 7.0 + 2.5j, 0.5e-3, 1.0)])
+def test_stored_tmatrix_bitwise_equal_to_reference(case):
+    lam, m, axi, ar = case
+    n1, a = R.tmatrix_amplq(lam, m, axi, ar)
+    n2, b = oracle.tm_tmatrix(lam, m, axi, ar)
+    assert n1 == n2 > 0
+    assert np.array_equal(a, b), int(np.count_nonzero(a != b))
+
+
+@needs_ref
+@pytest.mark.parametrize("case", CASES)
+def test_calc_tmatrix_matches_reference(case):
+    f, m, ph, d, ar, cant, dens = case
+    nd, dd = np.array([2.0e6, 5.0e5]), np.array([1.0e-4, 2.0e-4])
+    dm = np.array([d, 0.5 * d])
+    r = R.calc_tmatrix(f, m, ph, dm, dd, nd, dens, ar, cant)
+    o = oracle.calc_tmatrix(f, m, ph, dm, dd, nd, dens, ar, cant)
+    assert r[0] == 0 and o[0] == 0
+    for k, what in zip(range(1, 5), ("smat", "ext", "emis", "sback")):
+        assert rel(o[k], r[k]) < 1e-12, (what, rel(o[k], r[k]))
+
+
+def _mie_ab(x, m, nmax):
+    from scipy.special import spherical_jn as jn, spherical_yn as yn
+    out = []
+    for n in range(1, nmax + 1):
+        def psi(z):
+            return z * jn(n, z)
+
+        def dpsi(z):
+            return jn(n, z) + z * jn(n, z, derivative=True)
+
+        xi = x * (jn(n, x) + 1j * yn(n, x))
+        dxi = (jn(n, x) + 1j * yn(n, x)) + x * (jn(n, x, derivative=True) +
+                                               1j * yn(n, x, derivative=True))
+        a = (m * psi(m * x) * dpsi(x) - psi(x) * dpsi(m * x)) / (m * psi(m * x) * dxi - xi * dpsi(m * x))
+        b = (psi(m * x) * dpsi(x) - m * psi(x) * dpsi(m * x)) / (psi(m * x) * dxi - m * xi * dpsi(m * x))
+        out.append((a, b))
+    return out
+
+
+@pytest.mark.parametrize("case", [(299792458 / 10.65e9, 7.0 + 2.5j, 0.5e-3),
+                                  (299792458 / 35.5e9, 5.5 + 2.8j, 1.0e-3),
+                                  (299792458 / 89.0e9, 1.78 + 0.003j, 2.0e-3)])
+def test_sphere_limit_is_minus_the_mie_coefficients(case):
+    """as_ratio = 1: T11(m=0, n, n) = -b_n and T22(m=0, n, n) = -a_n.  The
+    reference rounds Q to single precision before inverting it (cmplx
+    without KIND, tmatrix_amplq.lp.f:2130) and stores T as REAL*4, so each
+    complex entry is within a few single-precision epsilons of |T|."""
+    lam, m, axi = case
+    x = 2 * np.pi * axi / lam
+    nm, t = oracle.tm_tmatrix(lam, m, axi, 1.0)
+    assert nm >= 4
+    for n, (a, b) in enumerate(_mie_ab(x, m, 3), start=1):
+        for got, want in ((complex(t[0, 0, n - 1, n - 1], t[4, 0, n - 1, n - 1]), -b),
+                          (complex(t[3, 0, n - 1, n - 1], t[7, 0, n - 1, n - 1]), -a)):
+            assert abs(got - want) <= 4 * 2.0 ** -24 * abs(want), (n, got, want)
+    # off-diagonal (n != n') entries of a sphere vanish to rounding
+    for n in range(nm):
+        for nn in range(nm):
+            if n != nn:
+                assert abs(t[0, 0, n, nn]) < 1e-6 * abs(t[0, 0, 0, 0]) + 1e-30
+
+
+def test_emission_is_extinction_minus_integrated_scattering():
+    """emis(i, jj) = ext(1, i, jj) - 2 pi sum_{kk, ll} w_ll Z(1, ll, i, jj, kk)
+    (tmatrix.f90:667-672), and for absorbing particles emission > 0."""
+    mu, w = oracle.lobatto(16)
+    st, sm, ex, em, sb = oracle.calc_tmatrix(94e9, 3.5 - 1.9j, 1, [3e-3], [1e-4], [1e6], 1000.0,
+                                             0.7, 0.0)
+    assert st == 0
+    for jj in range(16):
+        for i in range(2):
+            integ = sum(sm[0, ll, i, jj, kk] * w[ll] * 2 * np.pi for kk in range(2) for ll in range(16))
+            assert abs(em[i, jj] - (ex[0, i, jj] - integ)) <= 1e-12 * abs(ex[0, 0, jj])
+        assert em[0, jj] > 0
+
+
+def test_forward_extinction_of_a_sphere_matches_mie():
+    """Optical theorem: for a sphere the T-matrix extinction (forward
+    amplitude) equals the Mie extinction cross section, 1e-6 relative (the
+    REAL*4 storage of T)."""
+    f, m, d = 35.5e9, 5.5 - 2.8j, 2.0e-3
+    st, sm, ex, em, sb = oracle.calc_tmatrix(f, m, 1, [d], [1.0], [1.0], 1000.0, 1.0, 0.0)
+    assert st == 0
+    lam = C_LIGHT / f
+    x = np.pi * d / lam
+    ab = _mie_ab(x, m.conjugate(), 40)
+    qext = 2 / x ** 2 * sum((2 * n + 1) * (a.real + b.real) for n, (a, b) in enumerate(ab, start=1))
+    cext = qext * np.pi * (d / 2) ** 2
+    for jj in range(16):
+        assert abs(ex[0, 0, jj] - cext) < 1e-6 * cext, (jj, ex[0, 0, jj], cext)
+        assert abs(ex[0, 1, jj]) < 1e-6 * cext
