@@ -60,11 +60,22 @@ enum pam_dist_name {
   PAM_DIST_NORM_MGAMMA = 13     /* normalised modified gamma (make_dist_param.f90:602-624) */
 };
 
-/* descriptor file column "scat_name": "mie-sphere" (src/scatProperties.f90:722-769)
- * and "ssrg-rt3[_kappa_beta_gamma[_zeta]]", the self-similar Rayleigh-Gans
+/* descriptor file column "scat_name": "mie-sphere" (src/scatProperties.f90:722-769),
+ * "ssrg-rt3[_kappa_beta_gamma[_zeta]]", the self-similar Rayleigh-Gans
  * aggregate model in its Legendre-series form (src/scatProperties.f90:771-863,
- * src/rayleigh_gans.f90:467-872). */
-enum pam_scat_name { PAM_SCAT_DISABLED = 0, PAM_SCAT_MIE_SPHERE = 1, PAM_SCAT_SSRG_RT3 = 2 };
+ * src/rayleigh_gans.f90:467-872), and "tmatrix", spheroids of axis ratio
+ * as_ratio and canting angle canting by the extended-boundary-condition T
+ * matrix (src/scatProperties.f90:289-368, src/tmatrix.f90:14-684), which adds
+ * its scattering matrix, per-angle extinction matrix and emission vector to
+ * the layer directly.  A call whose hydrometeors mix "tmatrix" with the
+ * Legendre-series models is refused (PAM_FATAL): the reference's result
+ * there reads an unset albedo (src/scatProperties.f90:141,911). */
+enum pam_scat_name {
+  PAM_SCAT_DISABLED = 0,
+  PAM_SCAT_MIE_SPHERE = 1,
+  PAM_SCAT_SSRG_RT3 = 2,
+  PAM_SCAT_TMATRIX = 3
+};
 
 /* arithmetic of the device path */
 enum pam_precision { PAM_PREC_FP64 = 0, PAM_PREC_MIXED = 1 };

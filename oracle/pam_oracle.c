@@ -2078,7 +2078,50 @@ typedef struct {
   double back;       /* rt_back of the co-polar channels (scatProperties.f90:743-757, 922) */
   int nlegen;        /* -1 when no scatcnv */
   double coef[3][MAXNLEG + 1];
+  /* T-matrix hydrometeors: their sums, in calc_tmatrix's Fortran order
+   * smat(2,NMU,2,NMU,2), ext(2,2,NMU), emis(2,NMU) (scatProperties.f90:289-368) */
+  int tm;
+  double tm_smat[2 * NMU * 2 * NMU * 2], tm_ext[4 * NMU], tm_emis[2 * NMU];
 } layer_optics;
+
+int pam_oracle_calc_tmatrix(double freq_hz, double mre, double mim, int phase, int nbins,
+                            const double *dmax, const double *del_d, const double *ndens,
+                            const double *density, const double *as_ratio,
+                            const double *canting, double *smat, double *ext, double *emis,
+                            double *sback);
+
+/* One T-matrix hydrometeor into the layer sums: rt_scattermatrix,
+ * rt_extmatrix, rt_emisvec and rt_kexttot (scatProperties.f90:291-367,
+ * 920-927).  Its albedo contribution (salb_hydro * kext_hydro, :911) reads a
+ * variable the T-matrix branch never sets; it is taken as zero, which only
+ * the salbedo diagnostic sees (no Legendre series is formed from it: calls
+ * mixing T-matrix and series models are refused). */
+static int accumulate_tmatrix(layer_optics *lo, double freq, int liq_ice, int nb,
+                              const double *dia, const double *dd, const double *ndens,
+                              const double *rho, const double *ar, const double *cant, double nr,
+                              double ni, double *kexttot, char *msg) {
+  double *sm = (double *)malloc(sizeof(double) * (2 * NMU * 2 * NMU * 2 + 6 * NMU + 4));
+  double *ex = sm + 2 * NMU * 2 * NMU * 2, *em = ex + 4 * NMU, *sb = em + 2 * NMU;
+  int st = pam_oracle_calc_tmatrix(freq * 1e9, nr, ni, liq_ice, nb, dia, dd, ndens, rho, ar, cant,
+                                   sm, ex, em, sb);
+  if (st) {
+    free(sm);
+    snprintf(msg, 120, "error in calc_tmatrix!");
+    return PAM_FATAL;
+  }
+  lo->tm = 1;
+  for (int i = 0; i < 2 * NMU * 2 * NMU * 2; i++) lo->tm_smat[i] = lo->tm_smat[i] + sm[i];
+  for (int i = 0; i < 4 * NMU; i++) lo->tm_ext[i] = lo->tm_ext[i] + ex[i];
+  for (int i = 0; i < 2 * NMU; i++) lo->tm_emis[i] = lo->tm_emis[i] + em[i];
+  /* kext_hydro = extinct_matrix_hydro(1,1,16,1): the vertical angle */
+  *kexttot = *kexttot + ex[0 + 2 * (0 + 2 * (NMU - 1))];
+  free(sm);
+  if (*kexttot < 0.0 || isnan(*kexttot)) {
+    snprintf(msg, 120, "rt_kexttot smaller than zero or nan");
+    return PAM_FATAL;
+  }
+  return PAM_SUCCESS;
+}
 
 /* One hydrometeor's optics into the layer sums (scatProperties.f90:920-946). */
 static int accumulate_hydro(layer_optics *lo, const mie_result *mr, double *kexttot,
@@ -2171,6 +2214,20 @@ static int layer_hydrometeors(const pam_settings *s, const pam_hydro_desc *hyd,
         ref_ice(T, freq, &nr, &ni);
       mie_result mr;
       int st;
+      if (d->scat_name == PAM_SCAT_TMATRIX) {
+        double *ll = (double *)malloc(sizeof(double) * 2 * nb);
+        for (int i = 0; i < nb; i++) {
+          double a = h4->fs_as_ratio[b5 + i], cn = h4->fs_canting[b5 + i];
+          ll[i] = (a < 0 || isnan(a)) ? 0.0 : a;
+          ll[nb + i] = (cn < 0 || isnan(cn)) ? 0.0 : cn;
+        }
+        st = accumulate_tmatrix(lo, freq, hp.liq_ice, nb, h4->fs_d + b5, fdd, fnd, h4->fs_rho + b5,
+                                ll, ll + nb, nr, ni, &kexttot, msg);
+        free(ll);
+        free(fb);
+        if (st == PAM_FATAL) return st;
+        continue;
+      }
       if (d->scat_name == PAM_SCAT_SSRG_RT3) {
         double *ll = (double *)malloc(sizeof(double) * 2 * nb);
         for (int i = 0; i < nb; i++) {
@@ -2233,6 +2290,25 @@ static int layer_hydrometeors(const pam_settings *s, const pam_hydro_desc *hyd,
     else
       ref_ice(T, freq, &nr, &ni);
     mie_result mr;
+    if (d->scat_name == PAM_SCAT_TMATRIX) {
+      /* per-bin lists from the descriptor, negative or NaN read as 0
+       * (scatProperties.f90:276-287) */
+      double ar = (hp.as_ratio < 0 || isnan(hp.as_ratio)) ? 0.0 : hp.as_ratio;
+      double cant = (d->canting < 0 || isnan(d->canting)) ? 0.0 : d->canting;
+      double *ll = (double *)malloc(sizeof(double) * 2 * nb);
+      for (int i = 0; i < nb; i++) {
+        ll[i] = ar;
+        ll[nb + i] = cant;
+      }
+      st = accumulate_tmatrix(lo, freq, hp.liq_ice, nb, bins.diameter2scat, bins.delta_d_ds, ndens,
+                              bins.density2scat, ll, ll + nb, nr, ni, &kexttot, msg);
+      free(ll);
+      free(ndens);
+      free(bb);
+      if (st == PAM_FATAL) return st;
+      if (work) work[0] += nb;
+      continue;
+    }
     if (d->scat_name == PAM_SCAT_SSRG_RT3) {
       /* per-bin lists from the descriptor, negative or NaN read as 0
        * (scatProperties.f90:276-287); mass_ds (drop_size_dist.f90:162-165) */
@@ -2266,6 +2342,10 @@ static int layer_hydrometeors(const pam_settings *s, const pam_hydro_desc *hyd,
     }
     st = accumulate_hydro(lo, &mr, &kexttot, &salbedo, &nleg_coef, msg);
     if (st == PAM_FATAL) return st;
+  }
+  if (lo->tm && nleg_coef > 0) {
+    snprintf(msg, 120, "tmatrix mixed with Legendre-series scattering models");
+    return PAM_FATAL;
   }
   /* finalize_rt3_scatProperties */
   salbedo = salbedo / kexttot;
@@ -2401,6 +2481,28 @@ static int run_column_freq(const pam_settings *s, const pam_hydro_desc *hyd,
     memset(smat, 0, sizeof(double) * 4 * NN * NN);
     memset(ext, 0, sizeof(ext));
     memset(emis, 0, sizeof(emis));
+    if (lo.tm) {
+      /* T-matrix sums (scatProperties.f90:320-323, 925-927): quadrants 1
+       * and 4 from the downward-scattering half, 2 and 3 from the other;
+       * extinction and emission equal in both hemispheres */
+      for (int k = 0; k < 2; k++)
+        for (int j1 = 0; j1 < NMU; j1++)
+          for (int i1 = 0; i1 < NST; i1++)
+            for (int j2 = 0; j2 < NMU; j2++)
+              for (int i2 = 0; i2 < NST; i2++) {
+                double v = lo.tm_smat[i2 + 2 * (j2 + NMU * (i1 + 2 * (j1 + NMU * k)))];
+                int r = i2 + NST * j2, cc = i1 + NST * j1;
+                smat[(k == 0 ? 0 : 1) * NN * NN + r * NN + cc] += v;
+                smat[(k == 0 ? 3 : 2) * NN * NN + r * NN + cc] += v;
+              }
+      for (int l = 0; l < 2; l++)
+        for (int j = 0; j < NMU; j++) {
+          for (int a = 0; a < NST; a++)
+            for (int b = 0; b < NST; b++)
+              ext[((l * NMU + j) * NST + a) * NST + b] += lo.tm_ext[a + 2 * (b + 2 * j)];
+          for (int a = 0; a < NST; a++) emis[(l * NMU + j) * NST + a] += lo.tm_emis[a + 2 * j];
+        }
+    }
     if (lo.nlegen > 0) {
       double cst = lo.kexttot * lo.salbedo / (4.0 * 3.1415926535897932384);
       double coef[3][MAXNLEG + 1];
