@@ -286,6 +286,23 @@ int pam_run_device_noalloc(pam_context *ctx, const pam_settings *settings,
 
 /* Kernel-level timings of the last call (milliseconds, measured with HIP
  * events on the call's stream), in the order of pam_stage_names(). */
+/* calc_tmatrix (src/tmatrix.f90:14-444) for one hydrometeor on the device:
+ * the sums over nbins size bins of the T-matrix spheroid's scattering
+ * matrix, extinction matrix and emission vector times ndens * del_d, in the
+ * reference's Fortran order, smat (2,NUMMU,2,NUMMU,2), ext (2,2,NUMMU),
+ * emis (2,NUMMU), NUMMU = pam_nummu().  freq_hz in Hz; (mre, mim) the
+ * refractive index m = mre - i mim of scatProperties.f90:243; phase
+ * (liq_ice) -1 mixes ice of density below 917 kg/m3 with air; dmax, del_d
+ * [m], ndens [m^-4], density [kg/m3], as_ratio per bin; canting 0 (the
+ * reference's canting list must be 0).  Host arrays.  nmax (optional,
+ * nbins entries) receives each bin's T-matrix order.  Returns PAM_SUCCESS
+ * or PAM_FATAL (invalid bins, a particle beyond the device's limits: order
+ * 40, 100 Gauss points). */
+int pam_calc_tmatrix(pam_context *ctx, double freq_hz, double mre, double mim, int32_t phase,
+                     int32_t nbins, const double *dmax, const double *del_d,
+                     const double *ndens, const double *density, const double *as_ratio,
+                     double *smat, double *ext, double *emis, int32_t *nmax);
+
 int pam_last_stage_times(const pam_context *ctx, double *ms, int32_t n);
 const char *pam_stage_names(void);
 

@@ -43,7 +43,7 @@ DIST_NAMES = {
 # distributions the reference names but whose bins it never fills:
 # make_dist.f90:164-215 sets f_ds for neither, so its result is undefined
 UNDEFINED_DISTS = ("const", "const_cosmo_ice")
-SCAT_NAMES = {"disabled": 0, "mie-sphere": 1, "ssrg-rt3": 2}
+SCAT_NAMES = {"disabled": 0, "mie-sphere": 1, "ssrg-rt3": 2, "tmatrix": 3}
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
 _dp = C.POINTER(C.c_double)
@@ -338,7 +338,7 @@ def scattering_model(name: str):
     characters for the model and the rest from the tenth on; a name with
     fewer than three or more than four parameters leaves it reading an unset
     token, which is refused here."""
-    if name in ("disabled", "mie-sphere"):
+    if name in ("disabled", "mie-sphere", "tmatrix"):
         return name, (0.0, 0.0, 0.0, 0.0)
     if name[:8] == "ssrg-rt3":
         if len(name) == 8:
@@ -362,12 +362,40 @@ def scattering_model(name: str):
                "(scatProperties.f90:625-720); not built")
     elif name[:5] in ("liudb", "hongd"):
         why = "a DDA-database model that needs the reference's external data files; not built"
-    elif name == "tmatrix":
-        why = "the T-matrix producer is not built"
     else:
         why = "unknown scattering model"
-    raise NotImplementedError(f"scat_name {name!r}: {why}; 'mie-sphere' and 'ssrg-rt3' "
-                              "are implemented")
+    raise NotImplementedError(f"scat_name {name!r}: {why}; 'mie-sphere', 'ssrg-rt3' and "
+                              "'tmatrix' are implemented")
+
+
+def check_tmatrix(hydros, settings, nmlSets):
+    """The T-matrix producer's limits (scatProperties.f90:289-368).  Mixing it
+    with the Legendre-series models is refused because the reference's result
+    is then undefined: the T-matrix branch adds salb_hydro * kext_hydro to the
+    layer albedo without ever setting salb_hydro (scatProperties.f90:141,
+    911).  Canting angles other than 0, the radar simulator and full-spectrum
+    input are not implemented on the device.  tmatrix_db 'file' (the
+    reference's on-disk cache of the same results) computes them."""
+    names = [int(h.scat_name) for h in hydros]
+    if SCAT_NAMES["tmatrix"] not in names:
+        return
+    if any(n not in (SCAT_NAMES["tmatrix"], SCAT_NAMES["disabled"]) for n in names):
+        raise NotImplementedError(
+            "scat_name 'tmatrix' mixed with 'mie-sphere' or 'ssrg-rt3': the reference's layer "
+            "albedo then reads an unset value (scatProperties.f90:141,911), so that mix has no "
+            "defined result")
+    for h in hydros:
+        if int(h.scat_name) == SCAT_NAMES["tmatrix"] and float(h.canting) > 0.0:
+            raise NotImplementedError("scat_name 'tmatrix' with a canting angle: only canting 0 "
+                                      "is implemented")
+    if settings.active:
+        raise NotImplementedError("scat_name 'tmatrix' with active (radar) simulation is not "
+                                  "implemented")
+    if nmlSets.get("hydro_fullspec", False):
+        raise NotImplementedError("scat_name 'tmatrix' with hydro_fullspec is not implemented")
+    db = str(nmlSets.get("tmatrix_db", "none")).strip()
+    if db not in ("none", "file"):
+        raise ValueError(f"tmatrix_db {db!r} must be 'none' or 'file'")
 
 
 @dataclass
@@ -400,6 +428,7 @@ def marshal(nmlSets, sets, descriptorFile, descriptorFile4D, profile,
     settings = make_settings(nmlSets, sets)
     hydros = make_hydros(descriptorFile)
     keep.append(hydros)
+    check_tmatrix(hydros, settings, nmlSets)
     h4 = PamHydro4D()
     for key, arr in (descriptorFile4D or {}).items():
         if key == "nbin":
@@ -573,6 +602,9 @@ def declare_library(lib: C.CDLL) -> C.CDLL:
     lib.pam_context_set_tuning.argtypes = [C.c_void_p, C.c_char_p, C.c_int64]
     lib.pam_context_get_tuning.restype = C.c_int64
     lib.pam_context_get_tuning.argtypes = [C.c_void_p, C.c_char_p]
+    lib.pam_calc_tmatrix.restype = C.c_int
+    lib.pam_calc_tmatrix.argtypes = ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                      C.c_int32] + [_dp] * 8 + [_ip])
     return lib
 
 
@@ -581,7 +613,7 @@ EXPORTED_SYMBOLS = ["pam_version", "pam_device_count", "pam_context_create",
                     "pam_last_stage_times", "pam_stage_names", "pam_last_work",
                     "pam_workspace_bytes", "pam_context_set_workspace_limit", "pam_context_bytes",
                     "pam_context_trim", "pam_tuning_keys", "pam_context_set_tuning",
-                    "pam_context_get_tuning", "pam_nummu"]
+                    "pam_context_get_tuning", "pam_nummu", "pam_calc_tmatrix"]
 
 PACKAGE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIBRARY_PATH = os.path.join(PACKAGE_DIR, LIBRARIES[NUMMU])

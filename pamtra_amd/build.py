@@ -20,7 +20,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpamtra_amd.so")
 VARIANTS = {16: "libpamtra_amd.so", 32: "libpamtra_amd_n64.so"}
 SOURCES = ["pam_column.hip", "pam_dsd.hip", "pam_mie.hip", "pam_phase.hip", "pam_rt.hip",
-           "pam_capi.hip"]
+           "pam_tmatrix.hip", "pam_capi.hip"]
 ARCH = os.environ.get("PAM_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -106,7 +106,8 @@ struct DevBuf {
 // Mie angle tables of one device, shared by all its contexts and kept until
 // the process ends (66 MiB, and their 33 MiB FP32 copy).
 struct AngleTables {
-  DevBuf buf, pbuf, fbuf, pfbuf;
+  DevBuf buf, pbuf, fbuf, pfbuf, gbuf;
+  const double *tm_gauss = nullptr;  // Gauss nodes of the T-matrix producer
   const double *tab = nullptr;
   const float *tab_f = nullptr;
   const int64_t *off = nullptr;
@@ -169,11 +170,18 @@ static AngleTables *device_tables(int device) {
       t->poff = dpoff;
     }
   }
+  if (ok) {
+    const std::vector<double> g = tm_gauss_table();
+    ok = t->gbuf.ensure(g.size() * 8) == hipSuccess &&
+         hipMemcpy(t->gbuf.p, g.data(), g.size() * 8, hipMemcpyHostToDevice) == hipSuccess;
+    t->tm_gauss = (const double *)t->gbuf.p;
+  }
   if (!ok) {
     t->buf.release();
     t->pbuf.release();
     t->fbuf.release();
     t->pfbuf.release();
+    t->gbuf.release();
     delete t;
     return nullptr;
   }
@@ -188,6 +196,8 @@ struct pam_context {
   DevBuf colbuf, itembuf, iobuf, smallbuf;
   DevBuf rtbuf;  // global scratch of the adding kernels at N = 64
   DevBuf miebuf; // U slots of the coefficient-first Mie kernel
+  DevBuf tmbuf;  // workgroup slots and geometry of the T-matrix producer
+  const double *d_tm_gauss = nullptr;
   const double *d_atab = nullptr;
   const float *d_atab_f = nullptr;
   const int64_t *d_atab_off = nullptr;
@@ -295,6 +305,9 @@ static const char *err_text(int32_t code) {
     case E_SSRG_INPUT:
       return "calc_self_similar_rayleigh_gans_rt3: assertion error (liq_ice must be -1, canting 0 or 90, "
              "as_ratio > 0)";
+    case E_TMATRIX:
+      return "error in calc_tmatrix (as_ratio must be > 0; T-matrix order or Gauss points above the "
+             "device limits)";
     default: return "error";
   }
 }
@@ -397,6 +410,7 @@ extern "C" int pam_context_create(pam_context **out, int device) {
     c->d_ptab = t->ptab;
     c->d_ptab_f = t->ptab_f;
     c->d_ptab_off = t->poff;
+    c->d_tm_gauss = t->tm_gauss;
   }
   // per-call workspace budget: half of the free HBM, at most 96 GiB
   // (PAM_WORKSPACE_GB, read at each call, overrides it)
@@ -420,6 +434,55 @@ extern "C" int pam_context_create(pam_context **out, int device) {
   return PAM_SUCCESS;
 }
 
+extern "C" int pam_calc_tmatrix(pam_context *c, double freq_hz, double mre, double mim,
+                                int32_t phase, int32_t nbins, const double *dmax,
+                                const double *del_d, const double *ndens, const double *density,
+                                const double *as_ratio, double *smat, double *ext, double *emis,
+                                int32_t *nmax) {
+  if (!c || nbins <= 0 || !dmax || !del_d || !ndens || !density || !as_ratio || !smat || !ext ||
+      !emis || !(freq_hz > 0.0))
+    return PAM_FATAL;
+  if (hipSetDevice(c->device) != hipSuccess) return PAM_FATAL;
+  const size_t nacc = tm_nacc(), nb = (size_t)nbins;
+  const size_t bytes = (tm_slot_doubles() + tm_geo_doubles() + nacc + 5 * nb) * 8 + 8 + 4 * nb;
+  void *buf = nullptr;
+  if (hipMalloc(&buf, bytes) != hipSuccess) return PAM_FATAL;
+  double *scr = (double *)buf, *geo = scr + tm_slot_doubles(), *out = geo + tm_geo_doubles();
+  double *in = out + nacc;
+  int *st = (int *)(in + 5 * nb), *nm = st + 2;
+  int rc = PAM_FATAL;
+  const double *src[5] = {dmax, del_d, ndens, density, as_ratio};
+  bool ok = true;
+  for (int k = 0; k < 5 && ok; k++)
+    ok = hipMemcpyAsync(in + k * nb, src[k], nb * 8, hipMemcpyHostToDevice, c->stream) ==
+         hipSuccess;
+  std::vector<int> zeros(nb, 0);
+  if (ok) ok = hipMemcpyAsync(nm, zeros.data(), 4 * nb, hipMemcpyHostToDevice, c->stream) == hipSuccess;
+  if (ok) {
+    launch_tm_hydro(nbins, in, in + nb, in + 2 * nb, in + 3 * nb, in + 4 * nb, freq_hz, mre, mim,
+                    phase, c->d_mu, c->d_qw, c->d_tm_gauss, geo, scr, out, st, nm, c->stream);
+    std::vector<double> h(nacc);
+    int hst = -1;
+    std::vector<int> hnm(nb);
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(h.data(), out, nacc * 8, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+         hipMemcpyAsync(&hst, st, 4, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+         hipMemcpyAsync(hnm.data(), nm, 4 * nb, hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+         hipStreamSynchronize(c->stream) == hipSuccess;
+    if (ok && hst == 0) {
+      const size_t ns = (size_t)TM_NSMAT;
+      memcpy(smat, h.data(), ns * 8);
+      memcpy(ext, h.data() + ns, 4 * NMU * 8);
+      memcpy(emis, h.data() + ns + 4 * NMU, 2 * NMU * 8);
+      if (nmax)
+        for (size_t i = 0; i < nb; i++) nmax[i] = hnm[i];
+      rc = PAM_SUCCESS;
+    }
+  }
+  (void)hipFree(buf);
+  return rc;
+}
+
 extern "C" int pam_context_destroy(pam_context *c) {
   if (!c) return PAM_SUCCESS;
   (void)hipSetDevice(c->device);
@@ -429,6 +492,7 @@ extern "C" int pam_context_destroy(pam_context *c) {
   c->smallbuf.release();
   c->rtbuf.release();
   c->miebuf.release();
+  c->tmbuf.release();
   drop_graph(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
@@ -492,6 +556,7 @@ extern "C" int pam_context_trim(pam_context *c) {
   c->smallbuf.release();
   c->rtbuf.release();
   c->miebuf.release();
+  c->tmbuf.release();
   return PAM_SUCCESS;
 }
 
@@ -538,7 +603,7 @@ static size_t col_bytes(const Dims &d) {
 static size_t item_bytes_per(const Dims &d) {
   size_t per = 2 * 4 + (size_t)d.nh * d.nbmax * 4 * 8 + (size_t)d.nh * (4 + 8);
   per += (size_t)d.nf * (3 * 8 + 4 + NCOEF * MAXNLEG * 8 + 4 + 4 + 2 * NN2 * 8 + 2 * NN * 8 + 4 + 8 +
-                         4 * 8);
+                         4 * 8 + 4 * NMU * 8);
   return per;
 }
 
@@ -569,6 +634,8 @@ static void carve_items(Work &w, void *base, const Dims &d, size_t nitems) {
   w.ph_rank = cv.take<int32_t>(nif);
   w.ph_perm = cv.take<int32_t>(nif);
   w.mie_m = cv.take<double>(nif * 4);
+  w.tm_ext = cv.take<double>(nif * 2 * NMU);
+  w.tm_emis = cv.take<double>(nif * 2 * NMU);
 }
 
 extern "C" int64_t pam_workspace_bytes(int64_t ncol, int32_t max_nlyrs, int32_t nhydro,
@@ -699,6 +766,31 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     set_msg(res, "nbin: at most 4000 size bins per hydrometeor are supported");
     return PAM_FATAL;
   }
+  {
+    // the T-matrix producer: not beside the Legendre-series models (the
+    // reference's layer albedo then reads an unset value), not with a
+    // canting angle, a radar or full spectra on the device
+    bool tm = false, series = false;
+    for (int i = 0; i < d.nh; i++) {
+      if (hydros[i].scat_name == PAM_SCAT_TMATRIX) {
+        tm = true;
+        if (hydros[i].canting > 0.0) {
+          set_msg(res, "tmatrix: only canting 0 is implemented on the device");
+          return PAM_FATAL;
+        }
+      } else if (hydros[i].scat_name != PAM_SCAT_DISABLED) {
+        series = true;
+      }
+    }
+    if (tm && series) {
+      set_msg(res, "tmatrix cannot be mixed with mie-sphere or ssrg-rt3 hydrometeors");
+      return PAM_FATAL;
+    }
+    if (tm && (st->active || (h4in && h4in->fs_nbin > 0))) {
+      set_msg(res, "tmatrix: radar simulation and full-spectrum input are not implemented");
+      return PAM_FATAL;
+    }
+  }
   d.col0 = 0;
   d.ncol_chunk = d.ncol;
   d.nitems = 0;
@@ -731,8 +823,10 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   s.pers_dbl = (int)c->tune[1];
   s.pers_add = (int)c->tune[2];
   s.ncu = c->ncu;
-  for (int i = 0; i < d.nh; i++)
+  for (int i = 0; i < d.nh; i++) {
     if (hydros[i].scat_name == PAM_SCAT_SSRG_RT3) s.ssrg = 1;
+    if (hydros[i].scat_name == PAM_SCAT_TMATRIX) s.tmatrix = 1;
+  }
   {
     const char *mk = getenv("PAM_MIE");
     s.mie_chunked = (mk && strcmp(mk, "chunked") == 0) ? 1 : 0;
@@ -869,6 +963,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   w.atab_nmax = c->atab_nmax;
   w.ptab = c->d_ptab;
   w.ptab_f = c->d_ptab_f;
+  w.tm_gauss = c->d_tm_gauss;
   w.ptab_off = c->d_ptab_off;
   w.ph_cnt = d_ph;
   w.ph_off = d_ph + (PH_NBUCKET + 1);
@@ -946,11 +1041,21 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   }
   const size_t rt_bytes = nlane * RT_SLOT_DOUBLES * 8 * (size_t)std::max(1, c->ncu);
   const size_t mie_bytes = nlane * mie_nslots * (mie_slotted ? w.mie_slot_doubles * 8 : 0);
+  // the T-matrix producer's workgroup slots (two per CU) and geometry table
+  size_t tm_nslots = 0;
+  if (s.tmatrix) {
+    tm_nslots = (size_t)std::max(1, c->ncu) * 2;
+    if (arena) tm_nslots = std::min(tm_nslots, arena->size / 4 / (tm_slot_doubles() * 8));
+    if (tm_nslots == 0) return no_room("one T-matrix workgroup's slot");
+  }
+  const size_t tm_lane = (tm_nslots * tm_slot_doubles() + tm_geo_doubles() + 63) / 64 * 64;
+  const size_t tm_bytes = s.tmatrix ? nlane * tm_lane * 8 : 0;
   if (arena) {
     // what the optics may take: the arena less the adding scratch and the
     // Mie slots carved after them (each region rounded up to 256 bytes)
     size_t room = arena->left();
-    const size_t fixed = (rt_bytes + 255) / 256 * 256 + (mie_bytes + 255) / 256 * 256 + 512;
+    const size_t fixed = (rt_bytes + 255) / 256 * 256 + (mie_bytes + 255) / 256 * 256 +
+                         (tm_bytes + 255) / 256 * 256 + 768;
     room = room > fixed ? room - fixed : 0;
     max_items = room > 64 * 256 ? (room - 64 * 256) / per_item : 0;
     if (max_items == 0 && total > 0) return no_room("one scattering layer's optics");
@@ -994,6 +1099,12 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     w.mie_u = (double *)region(c->miebuf, mie_bytes);
     if (!w.mie_u) return no_room("the Mie kernel's slots");
   }
+  if (s.tmatrix) {
+    w.tm_slots = (int)tm_nslots;
+    w.tm_scr = (double *)region(c->tmbuf, tm_bytes);
+    if (!w.tm_scr) return no_room("the T-matrix producer's slots");
+    w.tm_geo = w.tm_scr + tm_nslots * tm_slot_doubles();
+  }
   c->work[0] = d.ncol;
   c->work[1] = d.nf;
   c->work[2] = total * d.nf;
@@ -1010,6 +1121,10 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
     w2.redo_add = redo_add2;
     w2.redo_lev = redo_lev2;
     if (w.mie_u) w2.mie_u = w.mie_u + (size_t)w.mie_slots * w.mie_slot_doubles;
+    if (w.tm_scr) {
+      w2.tm_scr = w.tm_scr + tm_lane;
+      w2.tm_geo = w2.tm_scr + tm_nslots * tm_slot_doubles();
+    }
     if (RT_SLOT_DOUBLES > 0)
       w2.rt_scratch = w.rt_scratch + RT_SLOT_DOUBLES * (size_t)std::max(1, c->ncu);
     // stream2 starts after the columns and the surface on the call's stream
@@ -1035,10 +1150,14 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
         launch_dsd(dc, s, p, d_hyd, h4, wc, sx);
         break;
       case 2:
-        launch_mie(dc, s, d_hyd, h4, p, d_freqs, wc, sx);
+        if (s.tmatrix)
+          launch_tmatrix(dc, s, d_hyd, d_freqs, wc, sx);
+        else
+          launch_mie(dc, s, d_hyd, h4, p, d_freqs, wc, sx);
         break;
       case 3:
-        launch_phase(dc, s, wc, sx);
+        // the T-matrix producer writes the phase matrix itself
+        if (!s.tmatrix) launch_phase(dc, s, wc, sx);
         break;
       case 4:
         launch_doubling(dc, s, d_freqs, wc, sx);

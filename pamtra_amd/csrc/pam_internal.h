@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/pamtra_amd.h"
 
 namespace pam {
@@ -24,6 +26,11 @@ constexpr int MAX_NTERMS = 160;      // Mie series length supported on device
 constexpr int MAX_HYDRO = 32;        // bits of the per-layer hydrometeor mask
 constexpr int MAX_NUMPTS = 204;      // 2*int((199+1)/2)+4
 constexpr int PH_NBUCKET = MAX_NUMPTS / 2 - 1;  // NUMPTS values 4, 6, ..., 204
+// T-matrix producer: largest order NMAX and number of Gauss points NGAUSS
+// (the reference allows 99 and 500); a particle that needs more is refused
+constexpr int TM_NCAP = 40;
+constexpr int TM_GCAP = 100;
+constexpr int TM_NSMAT = 2 * NMU * 2 * NMU * 2;  // scattering matrix (2,NMU,2,NMU,2)
 
 constexpr double PI = 3.141592653589793;      // constants.f90:16
 constexpr double C_LIGHT = 299792458.0;       // constants.f90:15
@@ -57,6 +64,7 @@ enum ErrCode : int32_t {
   E_REFL = 19 << 8,
   E_NAN_COEF = 20 << 8,
   E_SSRG_INPUT = 21 << 8,
+  E_TMATRIX = 22 << 8,
 };
 
 enum FaultKind : int { FAULT_NONE = 0, FAULT_NAN_COEF = 1, FAULT_SINGULAR = 2 };
@@ -86,6 +94,9 @@ struct Settings {
   // otherwise calls without self-similar Rayleigh-Gans take the
   // coefficient-first kernel
   int mie_chunked;
+  // some hydrometeor uses the T-matrix producer (then every scattering one
+  // does: calls mixing it with the Legendre-series models are refused)
+  int tmatrix;
   // 1: the mixed-precision path keeps the FP64 Mie kernel (PAM_MIE=fp64)
   int mie_fp64;
   // test mode (PAM_TEST_FAULT=<kind>:<column>:<channel>): the named (column,
@@ -182,6 +193,15 @@ struct Work {
   double *mie_u;
   size_t mie_slot_doubles;
   int mie_ld, mie_slots;
+  // T-matrix producer: per-angle extinction (k11, k12) and emission (e1, e2)
+  // of each unit [nitems*nf][NMU][2], null when no hydrometeor uses it; the
+  // producer's workgroup slots, the call's geometry table and the Gauss
+  // node table (see pam_tmatrix.hip)
+  double *tm_ext, *tm_emis;
+  double *tm_scr;
+  int tm_slots;
+  double *tm_geo;
+  const double *tm_gauss;
   // work counters (atomics)
   unsigned long long *counters;        // [PAM_NWORK], see pam_last_work
   // work-queue counters of the persistent grids: mie, doubling, adding
@@ -236,6 +256,20 @@ __device__ __forceinline__ cd csqrt_(cd z) {
   if (r == 0.0) return cd{0.0, 0.0};
   if (z.re >= 0.0) return cd{r, z.im / (2.0 * r)};
   return cd{fabs(z.im) / (2.0 * r), z.im >= 0.0 ? r : -r};
+}
+
+// Sihvola mixing with air applied to a refractive index (eps_mix.f90:1-48,
+// called as in mie_spheres.f90:187-192); single-precision result
+__device__ __forceinline__ cd eps_mix_air(cd e1, double rho) {
+  const double v = 0.85;
+  double f = rho / 917.0;
+  double r1 = e1.re;
+  double b = r1 + 2.0 - 2.0 * v - f * (r1 - 1.0) * (1.0 + v);
+  double a = (r1 + (2.0 - v) + f * (2.0 - v) * (r1 - 1.0));
+  double emr = (sqrt(b * b + 4.0 * v * a) - b) / (2.0 * v);
+  double emi = e1.im * (-(emr - 1.0) + f * (emr + 2.0 + v * (emr - 1.0))) /
+               ((r1 + 2.0 + 2.0 * v * (emr - 1.0)) - f * (1.0 + v) * (r1 - 1.0));
+  return cd{sp(emr), sp(emi)};
 }
 
 // Planck radiance [W m^-2 sr^-1 um^-1] (rt_utilities.f90:69-90, units 'r')
@@ -303,6 +337,19 @@ int64_t phase_table_layout(int64_t *off);
 void build_phase_tables(double *tab, const int64_t *d_off, const double *d_mu, hipStream_t st);
 // phase matrices of the chunk's scattering units into their R/T slots
 void launch_phase(const Dims &d, const Settings &s, Work &w, hipStream_t st);
+void launch_mie_index(const Dims &d, const pam_hydro_desc *hyd, const double *freqs, Work &w,
+                      hipStream_t st);
+void launch_tmatrix(const Dims &d, const Settings &s, const pam_hydro_desc *hyd,
+                    const double *freqs, Work &w, hipStream_t st);
+size_t tm_slot_doubles();
+size_t tm_geo_doubles();
+std::vector<double> tm_gauss_table();
+size_t tm_nacc();
+void launch_tm_hydro(int nbins, const double *dmax, const double *del_d, const double *ndens,
+                     const double *rho, const double *as_ratio, double freq_hz, double mre,
+                     double mim, int phase, const double *mu, const double *qw,
+                     const double *gtab, double *geo, double *scr, double *out, int *status,
+                     int *nmax_out, hipStream_t st);
 void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, const Hydro4D &h4,
                 const Profile &p, const double *freqs, Work &w, hipStream_t st);
 // rows and doubles of one U slot of the coefficient-first Mie kernel

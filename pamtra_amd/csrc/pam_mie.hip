@@ -65,20 +65,6 @@ __device__ cd eps_ice(double T, double f) {
   return cd{er, alpha / f + (beta_m + delta_beta) * f};
 }
 
-// Sihvola mixing with air applied to a refractive index (eps_mix.f90:1-48,
-// called as in mie_spheres.f90:187-192); single-precision result
-__device__ cd eps_mix_air(cd e1, double rho) {
-  const double v = 0.85;
-  double f = rho / 917.0;
-  double r1 = e1.re;
-  double b = r1 + 2.0 - 2.0 * v - f * (r1 - 1.0) * (1.0 + v);
-  double a = (r1 + (2.0 - v) + f * (2.0 - v) * (r1 - 1.0));
-  double emr = (sqrt(b * b + 4.0 * v * a) - b) / (2.0 * v);
-  double emi = e1.im * (-(emr - 1.0) + f * (emr + 2.0 + v * (emr - 1.0))) /
-               ((r1 + 2.0 + 2.0 * v * (emr - 1.0)) - f * (1.0 + v) * (r1 - 1.0));
-  return cd{sp(emr), sp(emi)};
-}
-
 __device__ __forceinline__ int mie_nstop(double x) {
   return (int)(x + 4.0 * pow(x, sp(0.3334)) + 2.0);
 }
@@ -1715,6 +1701,14 @@ __global__ void k_mie_index(Dims d, const pam_hydro_desc *hyd, const double *fre
     o[2] = mi.re;
     o[3] = mi.im;
   }
+}
+
+void launch_mie_index(const Dims &d, const pam_hydro_desc *hyd, const double *freqs, Work &w,
+                      hipStream_t st) {
+  const size_t n = (size_t)d.nitems * d.nf;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mie_index, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)),
+                     dim3(256), 0, st, d, hyd, freqs, w);
 }
 
 template <typename R>

@@ -546,7 +546,7 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
   // ((+,+) quadrant) and R ((-,+) quadrant) slots
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
   {
-    const bool has = nleg > 0 && s.lhyd_scat, pf = phase_in_f32(s, w);
+    const bool has = (nleg > 0 || s.tmatrix) && s.lhyd_scat, pf = phase_in_f32(s, w);
     const double *P0 = w.T + (size_t)itf * NN2, *P1 = w.R + (size_t)itf * NN2;
     for (int e = t; e < NN2; e += RT_WG) {
       const int i = e / NN, j = e % NN;
@@ -556,9 +556,14 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
   }
   __syncthreads();
   const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
+  // the T-matrix producer's per-angle optics (tmatrix.f90:640-672) in place
+  // of the isotropic kext and (1 - salb) kext: extinction [[k11, k12],
+  // [k12, k11]] and emission (e1, e2) of each angle
+  const double *tmx = s.tmatrix ? w.tm_ext + (size_t)itf * 2 * NMU : nullptr;
+  const double *tme = s.tmatrix ? w.tm_emis + (size_t)itf * 2 * NMU : nullptr;
   // ---- CHECK_NORM4
   if (t < NMU) {
-    double sum = absorb - kext;
+    double sum = tmx ? (s.lhyd_emis ? tme[2 * t] : 0.0) - tmx[2 * t] : absorb - kext;
     for (int j2 = 0; j2 < NMU; j2++)
       sum = sum + 2.0 * PI * sqw[j2] * (sX[(NST * j2) * LD + NST * t] + sY[(NST * j2) * LD + NST * t]);
     snorm[t] = fabs(sum);
@@ -576,7 +581,7 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
   const double gas = fmax(w.kgas[((size_t)col * d.nf + f) * d.ML + z], 0.0);
   const double p0 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z + 1], wl);
   const double p1 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z], wl);
-  const double extinct = kext + gas;
+  const double extinct = (tmx ? tmx[0] : kext) + gas;
   const double fnd = log(fmax(extinct * zdiff, 1.0e-7) / 1.0e-6) / (double)logf(2.0f);
   const int nd = (fnd > 0.0) ? (int)fnd + 1 : 0;
   const double nsub = (double)ldexpf(1.0f, nd);
@@ -590,7 +595,8 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
     const double tmp = dz / smu[j2];
     const double gext = (i1 == i2) ? gas : 0.0;
     const double diag = (r == c) ? 1.0 : 0.0;
-    const double ext = (j1 == j2) ? ((i1 == i2) ? kext : 0.0) + gext : 0.0;
+    const double ext =
+        (j1 == j2) ? (tmx ? tmx[2 * j2 + (i1 == i2 ? 0 : 1)] : ((i1 == i2) ? kext : 0.0)) + gext : 0.0;
     sR[r * LD + c] = C2 * tmp * sqw[j1] * sY[r * LD + c];
     sT[r * LD + c] = diag - tmp * (ext - C2 * sqw[j1] * sX[r * LD + c]);
   }
@@ -598,7 +604,8 @@ __device__ bool doubling_unit(const Dims &d, const Settings &s, const double *fr
     const int i = t % NST, j = t / NST;
     const double e = (i == 0) ? gas : 0.0;
     const double tmp = p0 * dz / smu[j];
-    const double v = tmp * ((i == 0 ? absorb : 0.0) + e);
+    const double v =
+        tmp * ((tmx ? (s.lhyd_emis ? tme[2 * j + i] : 0.0) : (i == 0 ? absorb : 0.0)) + e);
     sv[0][t] = v;  // S+
     sv[1][t] = v;  // S-
     sv[2][t] = v;  // C+
@@ -898,7 +905,7 @@ __device__ bool doubling_unit3(const Dims &d, const Settings &s, const double *f
   // ((+,+) quadrant, into B) and R ((-,+) quadrant, into A) slots
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
   {
-    const bool has = nleg > 0 && s.lhyd_scat, pf = phase_in_f32(s, w);
+    const bool has = (nleg > 0 || s.tmatrix) && s.lhyd_scat, pf = phase_in_f32(s, w);
     const double *P0 = w.T + (size_t)itf * NN2, *P1 = w.R + (size_t)itf * NN2;
     for (int e = t; e < NN2; e += RT_WG) {
       const int i = e / NN, j = e % NN;
@@ -908,9 +915,14 @@ __device__ bool doubling_unit3(const Dims &d, const Settings &s, const double *f
   }
   __syncthreads();
   const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
+  // the T-matrix producer's per-angle optics (tmatrix.f90:640-672) in place
+  // of the isotropic kext and (1 - salb) kext: extinction [[k11, k12],
+  // [k12, k11]] and emission (e1, e2) of each angle
+  const double *tmx = s.tmatrix ? w.tm_ext + (size_t)itf * 2 * NMU : nullptr;
+  const double *tme = s.tmatrix ? w.tm_emis + (size_t)itf * 2 * NMU : nullptr;
   // ---- CHECK_NORM4
   if (t < NMU) {
-    double sum = absorb - kext;
+    double sum = tmx ? (s.lhyd_emis ? tme[2 * t] : 0.0) - tmx[2 * t] : absorb - kext;
     for (int j2 = 0; j2 < NMU; j2++)
       sum = sum + 2.0 * PI * sqw[j2] * (sB[(NST * j2) * LD + NST * t] + sA[(NST * j2) * LD + NST * t]);
     snorm[t] = fabs(sum);
@@ -928,7 +940,7 @@ __device__ bool doubling_unit3(const Dims &d, const Settings &s, const double *f
   const double gas = fmax(w.kgas[((size_t)col * d.nf + f) * d.ML + z], 0.0);
   const double p0 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z + 1], wl);
   const double p1 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z], wl);
-  const double extinct = kext + gas;
+  const double extinct = (tmx ? tmx[0] : kext) + gas;
   const double fnd = log(fmax(extinct * zdiff, 1.0e-7) / 1.0e-6) / (double)logf(2.0f);
   const int nd = (fnd > 0.0) ? (int)fnd + 1 : 0;
   const double nsub = (double)ldexpf(1.0f, nd);
@@ -943,7 +955,8 @@ __device__ bool doubling_unit3(const Dims &d, const Settings &s, const double *f
     const double tmp = dz / smu[j2];
     const double gext = (i1 == i2) ? gas : 0.0;
     const double diag = (r == c) ? 1.0 : 0.0;
-    const double ext = (j1 == j2) ? ((i1 == i2) ? kext : 0.0) + gext : 0.0;
+    const double ext =
+        (j1 == j2) ? (tmx ? tmx[2 * j2 + (i1 == i2 ? 0 : 1)] : ((i1 == i2) ? kext : 0.0)) + gext : 0.0;
     sA[r * LD + c] = C2 * tmp * sqw[j1] * sA[r * LD + c];
     sB[r * LD + c] = diag - tmp * (ext - C2 * sqw[j1] * sB[r * LD + c]);
   }
@@ -951,7 +964,8 @@ __device__ bool doubling_unit3(const Dims &d, const Settings &s, const double *f
     const int i = t % NST, j = t / NST;
     const double e = (i == 0) ? gas : 0.0;
     const double tmp = p0 * dz / smu[j];
-    const double v = tmp * ((i == 0 ? absorb : 0.0) + e);
+    const double v =
+        tmp * ((tmx ? (s.lhyd_emis ? tme[2 * j + i] : 0.0) : (i == 0 ? absorb : 0.0)) + e);
     sv[0][t] = v;  // S+
     sv[1][t] = v;  // S-
     sv[2][t] = v;  // C+
@@ -1308,7 +1322,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   // staged in FP64 over the LDS of the four FP32 matrices
   const int numpts = 2 * ((nleg + 1) / 2) + 4;
   {
-    const bool has = nleg > 0 && s.lhyd_scat, pf = phase_in_f32(s, w);
+    const bool has = (nleg > 0 || s.tmatrix) && s.lhyd_scat, pf = phase_in_f32(s, w);
     const double *G0 = w.T + (size_t)itf * NN2, *G1 = w.R + (size_t)itf * NN2;
     double *D0 = reinterpret_cast<double *>(L.m[0]), *D1 = reinterpret_cast<double *>(L.m[2]);
     for (int e = t; e < NN2; e += RT_WG) {
@@ -1320,6 +1334,11 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   const double *P1 = reinterpret_cast<const double *>(L.m[2]);
   __syncthreads();
   const double absorb = s.lhyd_emis ? (1.0 - salb) * kext : 0.0;
+  // the T-matrix producer's per-angle optics (tmatrix.f90:640-672) in place
+  // of the isotropic kext and (1 - salb) kext: extinction [[k11, k12],
+  // [k12, k11]] and emission (e1, e2) of each angle
+  const double *tmx = s.tmatrix ? w.tm_ext + (size_t)itf * 2 * NMU : nullptr;
+  const double *tme = s.tmatrix ? w.tm_emis + (size_t)itf * 2 * NMU : nullptr;
   const double wl = C_LIGHT / (freqs[f] * 1.e3);
   const double zdiff = fabs(w.hgt_lev[(size_t)col * (d.ML + 1) + z + 1] -
                             w.hgt_lev[(size_t)col * (d.ML + 1) + z]);
@@ -1327,9 +1346,15 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   const double p0 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z + 1], wl);
   const double p1 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z], wl);
   const double ext = kext + gas;
+  // entry (r, c) of the extinction matrix (plus gas) of the layer
+  auto extrc = [&](int r, int c) -> double {
+    if (!tmx) return c == r ? ext : 0.0;
+    if (r / NST != c / NST) return 0.0;
+    return r == c ? tmx[2 * (r / NST)] + gas : tmx[2 * (r / NST) + 1];
+  };
   const double C2 = 2.0 * 3.1415926535897932384;
   if (t < NMU) {  // CHECK_NORM4
-    double sum = absorb - kext;
+    double sum = tmx ? (s.lhyd_emis ? tme[2 * t] : 0.0) - tmx[2 * t] : absorb - kext;
     for (int j2 = 0; j2 < NMU; j2++) {
       const int e = (NST * j2) * NN + NST * t;
       sum = sum + 2.0 * PI * sqw[j2] * (P0[e] + P1[e]);
@@ -1340,7 +1365,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     double sum = 0.0;
     for (int c = 0; c < NN; c++) {
       const double wc = C2 * sqw[c / NST];
-      sum += fabs(wc * P1[r * NN + c]) + fabs((c == r ? ext : 0.0) - wc * P0[r * NN + c]);
+      sum += fabs(wc * P1[r * NN + c]) + fabs(extrc(r, c) - wc * P0[r * NN + c]);
     }
     L.hrow[r] = sum / smu[r / NST];
   }
@@ -1365,7 +1390,7 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
       const int e = t + q * RT_WG, r = e / NN, c = e % NN;
       const double sc = dz / smu[r / NST], wc = C2 * sqw[c / NST];
       const double a = sc * wc * P1[e];
-      const double b = sc * ((c == r ? ext : 0.0) - wc * P0[e]);
+      const double b = sc * (extrc(r, c) - wc * P0[e]);
       vs[q] = (float)(b + a);
       vd[q] = (float)(b - a);
     }
@@ -1378,7 +1403,9 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     }
   }
   double *cv = L.c[0], *cw = L.c[1];
-  if (t < NN) cv[t] = (t % NST == 0) ? dz * (absorb + gas) / smu[t / NST] : 0.0;
+  if (t < NN)
+    cv[t] = tmx ? dz * ((t % NST == 0 ? gas : 0.0) + (s.lhyd_emis ? tme[t] : 0.0)) / smu[t / NST]
+                : (t % NST == 0) ? dz * (absorb + gas) / smu[t / NST] : 0.0;
   __syncthreads();
   // particular sums with v_0 = dz e, w_k = dz S v_k, v_k+1 = dz D w_k:
   //   constant: m = 2 sum v_k / (2k+1)!,  p = -2 sum w_k / (2k+2)!

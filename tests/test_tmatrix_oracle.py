@@ -179,3 +179,25 @@ def test_oblate_tmatrix_layers_polarise():
     tb = t["tb"][0, 0, 0, :16, :, :]  # top of atmosphere, upwelling, (angle, freq, pol)
     assert np.max(np.abs(tb[0, :, 0] - tb[0, :, 1])) < 1e-6
     assert np.max(np.abs(tb[:, :, 0] - tb[:, :, 1])) > 1.0
+
+
+def test_marshalling_refuses_unsupported_tmatrix_calls():
+    """Mixing 'tmatrix' with the Legendre-series models (no defined result in
+    the reference), canting angles and the radar are refused up front."""
+    from pamtra_amd import _capi
+    from _scenario import build_pamtra, passive_nml
+    pam = build_pamtra()
+    pam.df.data["as_ratio"][:] = 0.6
+    pam.df.data["scat_name"][0] = "tmatrix"
+    with pytest.raises(NotImplementedError, match="mixed"):
+        _capi.marshal(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p)
+    pam.df.data["scat_name"][:] = "tmatrix"
+    _capi.marshal(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p)
+    pam.df.data["canting"][:] = 10.0
+    with pytest.raises(NotImplementedError, match="canting"):
+        _capi.marshal(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p)
+    pam.df.data["canting"][:] = -99.0
+    nml = passive_nml(pam)
+    nml["active"] = True
+    with pytest.raises(NotImplementedError, match="radar"):
+        _capi.marshal(nml, pam.set, pam.df.data, pam.df.data4D, pam.p)
