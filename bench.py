@@ -410,6 +410,80 @@ def config3_leg(args, torch):
     return res
 
 
+def tmatrix_leg(args, torch):
+    """The T-matrix producer: tmatrix_config (the config-1 grid, cloud
+    droplets as T-matrix spheres, rain as oblate spheroids of axis ratio
+    0.7, 30 bins each, 3 channels), FP64, timed per call, with the solve
+    count and rate, and the oracle (binary128 restatement) on a few columns
+    as the CPU baseline and parity check."""
+    from pamtra_amd import _capi, _lib, workloads
+    pam, freqs = workloads.tmatrix_config(ncol=args.tmatrix_cols)
+    lib, ctx = _lib.library(), _lib.get_context(0)
+    prof = to_device(pam, torch)
+    nml = dict(pam.nmlSet)
+    nf = len(freqs)
+    f, fp = _capi.freqs_array(freqs)
+    names = lib.pam_stage_names().decode().split(",")
+    m = _capi.marshal(nml, pam.set, pam.df.data, pam.df.data4D, prof)
+    nx, ny, nout = m.nx, m.ny, m.noutlevels
+    tb = torch.zeros((nx, ny, nout, 32, nf, 2), dtype=torch.float64, device="cuda")
+    status = torch.zeros((nx, ny, nf), dtype=torch.int32, device="cuda")
+    r = _capi.PamResults()
+    r.tb = C.cast(C.c_void_p(tb.data_ptr()), C.POINTER(C.c_double))
+    r.status = C.cast(C.c_void_p(status.data_ptr()), C.POINTER(C.c_int32))
+
+    def call():
+        if lib.pam_run_device(ctx, C.byref(m.settings), m.hydros, C.byref(m.hydro4d),
+                              C.byref(m.profile), fp, nf, C.byref(r), None) == _capi.PAM_FATAL:
+            raise RuntimeError("pam_run_device (T-matrix) failed")
+
+    call()
+    torch.cuda.synchronize()
+    acc = np.zeros(len(names))
+    t0 = time.perf_counter()
+    for i in range(args.tmatrix_steps):
+        call()
+        ms = (C.c_double * len(names))()
+        lib.pam_last_stage_times(ctx, ms, len(names))
+        acc += np.array(ms[:])
+        log(f"tmatrix step {i} done")
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.tmatrix_steps
+    work = (C.c_int64 * 14)()
+    lib.pam_last_work(ctx, work, 14)
+    stage = {n: round(v / args.tmatrix_steps, 3) for n, v in zip(names, acc)}
+    particles = int(work[6])
+    res = {"value": round(nx * ny * nf / dt, 2), "unit": "columns·channels/s",
+           "ms_per_step": round(dt * 1e3, 3), "stage_ms": stage,
+           "producer_stage": "mie", "particles_per_call": particles,
+           "mean_tmatrix_order": round(work[7] / max(1, particles), 2),
+           "particles_per_s": round(particles / (stage["mie"] * 1e-3), 1),
+           "status_nonzero": int((status.cpu().numpy() != 0).sum()),
+           "workload": (f"{args.tmatrix_cols} columns x 30 layers (0-12 km, ocean), 3 channels "
+                        "22.235/35.5/89 GHz, cloud (T-matrix spheres) and rain (oblate "
+                        "spheroids, as_ratio 0.7), 30 bins each, canting 0, FP64")}
+    if args.tmatrix_cpu_cols > 0 and not args.no_cpu:
+        import oracle
+        n = min(args.tmatrix_cpu_cols, nx)
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (
+            os.cpu_count() or 1)
+        threads = args.cpu_threads or min(16, avail)
+        sub = slice_columns(pam, n)
+        t1 = time.perf_counter()
+        ref, st, _ = oracle.run(nml, pam.set, pam.df.data, pam.df.data4D, sub, freqs,
+                                nthreads=threads)
+        tc = time.perf_counter() - t1
+        tbh = tb.cpu().numpy()
+        ok = np.isfinite(ref["tb"])
+        res["max_abs_dtb_vs_oracle_K"] = float(np.max(np.abs(tbh[:n][ok] - ref["tb"][ok])))
+        res["cpu_baseline"] = {"value": round(n * nf / tc, 4), "unit": "columns·channels/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"first {n} of the {nx} columns x {nf} channels, "
+                                         f"{tc:.1f} s (binary128 T-matrix solves)"}
+    log(f"tmatrix: {res}")
+    return res
+
+
 def aggregates(args):
     """Cost of the self-similar Rayleigh-Gans producer: the four-category
     columns (config 3 shapes) with snow and graupel as Mie soft spheres and
@@ -494,6 +568,13 @@ def main():
     ap.add_argument("--large-total", type=int, default=1024 * 1024)
     ap.add_argument("--large-slice", type=int, default=65536)
     ap.add_argument("--large-fp64-slices", type=int, default=2)
+    ap.add_argument("--tmatrix-cols", type=int, default=1024,
+                    help="columns of the T-matrix leg (0: skip it)")
+    ap.add_argument("--tmatrix-steps", type=int, default=2)
+    ap.add_argument("--tmatrix-cpu-cols", type=int, default=2,
+                    help="columns of the T-matrix leg's oracle sample")
+    ap.add_argument("--tmatrix-only", action="store_true",
+                    help="run only the T-matrix leg and print its summary")
     ap.add_argument("--config3-only", action="store_true",
                     help="run only the four-category configuration (FP64 and mixed) and print "
                          "its summary")
@@ -515,6 +596,9 @@ def main():
         return
     if args.config3_only:
         print(json.dumps({"config3": config3_leg(args, torch)}))
+        return
+    if args.tmatrix_only:
+        print(json.dumps({"tmatrix": tmatrix_leg(args, torch)}))
         return
     if args.aggregates_only:
         print(json.dumps({"aggregates": aggregates(args)}))
@@ -777,6 +861,8 @@ def main():
         out["large_domain"] = large_domain(args, torch)
     if args.aggregates_cols > 0:
         out["aggregates"] = aggregates(args)
+    if args.tmatrix_cols > 0:
+        out["tmatrix"] = tmatrix_leg(args, torch)
 
     if not args.no_cpu:
         import oracle

@@ -16,6 +16,9 @@ device for ``pam_run_device``.
            channels 10.65-325 GHz, the config3 categories, 32 quadrature
            angles per hemisphere (64x64 operators); built for a column
            sample of any size.
+  tmatrix_config: the config1 grid (30 layers, 0-12 km, ocean) with cloud
+           droplets as T-matrix spheres and rain as oblate T-matrix
+           spheroids (as_ratio 0.7), 30 bins each, 3 channels 22.235-89 GHz.
 Random draws use numpy.random.default_rng(seed), fields in a fixed order.
 """
 
@@ -122,6 +125,21 @@ def config1(ncol=64, seed=1001, nbin=50):
                       wind10v=np.zeros(ncol), timestamp=0)
     _common_nml(pam)
     return pam, [89.0]
+
+
+TMATRIX_FREQS = [22.235, 35.5, 89.0]
+
+
+def tmatrix_config(ncol=64, seed=1006, nbin=30, freqs=None):
+    """Passive RT4 with the T-matrix producer: the config1 grid and
+    hydrometeor fields, cloud droplets as spheres and rain as oblate
+    spheroids (axis ratio 0.7), canting 0."""
+    pam, _ = config1(ncol=ncol, seed=seed, nbin=nbin)
+    pam.df.data["scat_name"][:] = "tmatrix"
+    pam.df.data["as_ratio"][0] = 1.0
+    pam.df.data["as_ratio"][1] = 0.7
+    pam.df.data["canting"][:] = 0.0
+    return pam, list(freqs or TMATRIX_FREQS)
 
 
 def config2(ncol=16384, seed=1002, nbin=80, freqs=None):
