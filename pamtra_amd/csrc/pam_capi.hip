@@ -1041,10 +1041,10 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   }
   const size_t rt_bytes = nlane * RT_SLOT_DOUBLES * 8 * (size_t)std::max(1, c->ncu);
   const size_t mie_bytes = nlane * mie_nslots * (mie_slotted ? w.mie_slot_doubles * 8 : 0);
-  // the T-matrix producer's workgroup slots (two per CU) and geometry table
+  // the T-matrix producer's workgroup slots and geometry table
   size_t tm_nslots = 0;
   if (s.tmatrix) {
-    tm_nslots = (size_t)std::max(1, c->ncu) * 2;
+    tm_nslots = (size_t)std::max(1, c->ncu) * TM_PER_CU;
     if (arena) tm_nslots = std::min(tm_nslots, arena->size / 4 / (tm_slot_doubles() * 8));
     if (tm_nslots == 0) return no_room("one T-matrix workgroup's slot");
   }

@@ -31,6 +31,7 @@ constexpr int PH_NBUCKET = MAX_NUMPTS / 2 - 1;  // NUMPTS values 4, 6, ..., 204
 constexpr int TM_NCAP = 40;
 constexpr int TM_GCAP = 100;
 constexpr int TM_NSMAT = 2 * NMU * 2 * NMU * 2;  // scattering matrix (2,NMU,2,NMU,2)
+constexpr int TM_PER_CU = 8;  // one-wave workgroups per CU of its persistent grid
 
 constexpr double PI = 3.141592653589793;      // constants.f90:16
 constexpr double C_LIGHT = 299792458.0;       // constants.f90:15

@@ -41,11 +41,11 @@
 
 namespace pam {
 
-constexpr int TM_WG = 256;
+constexpr int TM_WG = 64;
 constexpr int TM_NDIR = 2 * NMU;         // scattering directions (hemisphere, angle)
 constexpr int TM_NPAIR = NMU * TM_NDIR;  // (incidence angle, scattering direction)
 constexpr int TM_NAZ = 30;               // azimuth_num (tmatrix.f90:142)
-constexpr int TM_GRP = 16;               // pairs per group of the azimuth stage
+constexpr int TM_GRP = 2;                // pairs per group of the azimuth stage
 constexpr int TM_B = TM_NCAP + 1;        // leading dimension over orders
 constexpr int TM_X = 2 * TM_GCAP + 1;    // 1-based point arrays
 constexpr int TM_QN = 2 * TM_NCAP;       // largest Q order
@@ -66,7 +66,8 @@ constexpr size_t TM_O_TT = TM_O_RGQ + 2 * TM_QPL;           // TR1, TI1
 constexpr size_t TM_O_TM = TM_O_TT + 2 * TM_QPL;            // float [8][NCAP][NCAP]
 constexpr size_t TM_O_AMP = TM_O_TM + 4 * (size_t)TM_NCAP * TM_NCAP;  // [NPAIR][TM_B][8]
 constexpr size_t TM_O_E = TM_O_AMP + (size_t)TM_NPAIR * TM_B * 8;      // [NMU][TM_B][8]
-constexpr size_t TM_O_PART = TM_O_E + (size_t)NMU * TM_B * 8;
+constexpr size_t TM_O_DV = TM_O_E + (size_t)NMU * TM_B * 8;     // d functions [2][NDIR][TM_B]
+constexpr size_t TM_O_PART = TM_O_DV + 2 * (size_t)TM_NDIR * TM_B;
 constexpr size_t TM_O_HYD = TM_O_PART + TM_NACC;
 constexpr size_t TM_O_UNIT = TM_O_HYD + TM_NACC;
 constexpr size_t TM_SLOT = (TM_O_UNIT + TM_NACC + 63) / 64 * 64;
@@ -154,9 +155,6 @@ struct TmLds {
       int redi[TM_WG / 64];
       int perm[TM_QN];
     } j;
-    struct {  // amplitude forms of one m: d functions of every direction
-      double a[TM_NDIR][TM_B], b[TM_NDIR][TM_B];
-    } v;
     struct {  // azimuth stage of one group of pairs
       double amp[TM_GRP][TM_B][8];
       double c[TM_GRP][TM_NAZ][4];
@@ -186,10 +184,12 @@ __device__ __forceinline__ double tm_block_max(TmLds &L, double v) {
 __device__ void tm_const(double *S, const double *gtab, int ngauss, int nmax) {
   const int t = threadIdx.x, ng = 2 * ngauss;
   const double *gx = gtab + (size_t)ngauss * (ngauss - 1) * 2;
+  #pragma unroll 1
   for (int i = t; i < ng; i += TM_WG) {
     S[TM_O_X + i + 1] = gx[i];
     S[TM_O_W + i + 1] = gx[ng + i];
   }
+  #pragma unroll 1
   for (int k = t; k < TM_B * TM_B; k += TM_WG) {
     const int n = k / TM_B, n1 = k % TM_B;
     if (n < 1 || n1 < 1 || n > nmax || n1 > nmax) continue;
@@ -198,6 +198,7 @@ __device__ void tm_const(double *S, const double *gtab, int ngauss, int nmax) {
     S[TM_O_ANN + k] = (double)d * (double)d1 * 0.5;
   }
   __syncthreads();
+  #pragma unroll 1
   for (int i = t + 1; i <= ngauss; i += TM_WG) {
     double y = S[TM_O_X + i];
     y = 1.0 / (1.0 - y * y);
@@ -309,6 +310,7 @@ __device__ bool tm_vary(double *S, TmLds &L, TmPart &P, int ngauss, int nmax) {
   {
     const double a = P.a * pow(P.eps, 1.0 / 3.0);
     const double aa = a * a, ee = P.eps * P.eps, ee1 = ee - 1.0;
+    #pragma unroll 1
     for (int i = t + 1; i <= ngauss; i += TM_WG) {
       const double c = S[TM_O_X + i], cc = c * c, ss = 1.0 - cc, s = sqrt(ss);
       const double rr = 1.0 / (ss + ee * cc);
@@ -327,6 +329,7 @@ __device__ bool tm_vary(double *S, TmLds &L, TmPart &P, int ngauss, int nmax) {
   const double prr = P.mrr * v0, pri = -P.mri * v0;
   __syncthreads();
   double ta = 0.0;
+  #pragma unroll 1
   for (int i = t + 1; i <= ng; i += TM_WG) {
     double vv = sqrt(S[TM_O_R + i]);
     const double v = vv * pi;
@@ -344,6 +347,7 @@ __device__ bool tm_vary(double *S, TmLds &L, TmPart &P, int ngauss, int nmax) {
   int nnmax2 = (int)(tb + 4.0 * pow(tb, 0.33333) + 8.0 * sqrt(tb));
   nnmax2 = nnmax2 - nmax + 5;
   if (nmax + nnmax2 >= 1200) return false;
+  #pragma unroll 1
   for (int i = t + 1; i <= ngauss; i += TM_WG) {
     const double v = sqrt(S[TM_O_R + i]) * pi;
     tm_bessel(S, i, v, v * P.mrr, v * P.mri, nmax, nnmax1, nnmax2);
@@ -410,6 +414,7 @@ __device__ bool tm_invert(double *S, TmLds &L, int n) {
     // pivot: first row of the largest |re| + |im| in column k
     double best = -1.0;
     int bi = n;
+    #pragma unroll 1
     for (int i = k + t; i < n; i += TM_WG) {
       const size_t e = (size_t)i * TM_QN + k;
       const double v = fabs(qrh[e]) + fabs(qih[e]);
@@ -445,6 +450,7 @@ __device__ bool tm_invert(double *S, TmLds &L, int n) {
     const int p = L.u.j.perm[k];
     if (p < 0) return false;
     if (p != k)
+      #pragma unroll 1
       for (int j = t; j < n; j += TM_WG) {
         const ddc a = ld(k, j), b = ld(p, j);
         st(k, j, b);
@@ -458,6 +464,7 @@ __device__ bool tm_invert(double *S, TmLds &L, int n) {
       L.u.j.inv[2] = inv.im.hi;
       L.u.j.inv[3] = inv.im.lo;
     }
+    #pragma unroll 1
     for (int i = t; i < n; i += TM_WG) {
       const ddc f = ld(i, k);
       L.u.j.fac[i][0] = f.re.hi;
@@ -468,9 +475,11 @@ __device__ bool tm_invert(double *S, TmLds &L, int n) {
     __syncthreads();
     const ddc inv = ddc{dd{L.u.j.inv[0], L.u.j.inv[1]}, dd{L.u.j.inv[2], L.u.j.inv[3]}};
     // pivot row times the pivot's inverse (its own entry becomes the inverse)
+    #pragma unroll 1
     for (int j = t; j < n; j += TM_WG) st(k, j, j == k ? inv : ddc_mul(ld(k, j), inv));
     __syncthreads();
     // the other rows: A(i, j) -= f_i A(k, j), with A(i, k) taken as 0
+    #pragma unroll 1
     for (int e = t; e < n * n; e += TM_WG) {
       const int i = e / n, j = e % n;
       if (i == k) continue;
@@ -486,6 +495,7 @@ __device__ bool tm_invert(double *S, TmLds &L, int n) {
   for (int k = n - 1; k >= 0; k--) {
     const int p = L.u.j.perm[k];
     if (p != k)
+      #pragma unroll 1
       for (int i = t; i < n; i += TM_WG) {
         const ddc a = ld(i, k), b = ld(i, p);
         st(i, k, b);
@@ -502,6 +512,7 @@ __device__ bool tm_invert(double *S, TmLds &L, int n) {
 __device__ bool tm_tmatr(double *S, TmLds &L, const TmPart &P, int m, int ngauss, int nmax) {
   const int t = threadIdx.x;
   // d functions at the points (mirrored half, the one the sums read)
+  #pragma unroll 1
   for (int i = t + 1; i <= ngauss; i += TM_WG) {
     const int i1 = ngauss + i, i2 = ngauss - i + 1;
     double *d1 = S + TM_O_D1 + (size_t)i2 * TM_B, *d2 = d1 + TM_BPLANE;
@@ -513,6 +524,7 @@ __device__ bool tm_tmatr(double *S, TmLds &L, const TmPart &P, int m, int ngauss
     }
   }
   const double qm = m, qmm = qm * qm;
+  #pragma unroll 1
   for (int i = t + 1; i <= ngauss; i += TM_WG) {
     const double wr = S[TM_O_W + i] * S[TM_O_R + i];
     L.u.g.ds[i] = S[TM_O_S + i] * qm * wr;
@@ -527,6 +539,7 @@ __device__ bool tm_tmatr(double *S, TmLds &L, const TmPart &P, int m, int ngauss
   const double *d1p = S + TM_O_D1, *d2p = d1p + TM_BPLANE;
   double *qrh = S + TM_O_Q, *qrl = qrh + TM_QPL, *qih = qrl + TM_QPL, *qil = qih + TM_QPL;
   double *rgr = S + TM_O_RGQ, *rgi = rgr + TM_QPL;
+  #pragma unroll 1
   for (int it = t; it < nm * nm; it += TM_WG) {
     const int n1 = mm1 + it / nm, n2 = mm1 + it % nm;
     const double an1 = (double)(float)(n1 * (n1 + 1)), an2 = (double)(float)(n2 * (n2 + 1));
@@ -534,6 +547,7 @@ __device__ bool tm_tmatr(double *S, TmLds &L, const TmPart &P, int m, int ngauss
     double gr11 = 0, gr12 = 0, gr21 = 0, gr22 = 0, gi11 = 0, gi12 = 0, gi21 = 0, gi22 = 0;
     const bool even = ((n1 + n2) & 1) == 0;
     if (m > 0 || even) {
+#pragma unroll 1
       for (int i = 1; i <= ngauss; i++) {
         const size_t o1 = (size_t)i * TM_B + n1, o2 = (size_t)i * TM_B + n2;
         const double d1n1 = d1p[o1], d2n1 = d2p[o1], d1n2 = d1p[o2], d2n2 = d2p[o2];
@@ -631,6 +645,7 @@ __device__ bool tm_tmatr(double *S, TmLds &L, const TmPart &P, int m, int ngauss
   if (!tm_invert(S, L, nq)) return false;
   // T = -RG(Q) Q^-1, operands rounded to double, summed over k in order
   double *tr1 = S + TM_O_TT, *ti1 = tr1 + TM_QPL;
+  #pragma unroll 1
   for (int e = t; e < nq * nq; e += TM_WG) {
     const int i = e / nq, j = e % nq;
     double tr = 0.0, ti = 0.0;
@@ -655,6 +670,7 @@ __device__ void tm_store(double *S, int m, int nmax) {
   float *T = reinterpret_cast<float *>(S + TM_O_TM);
   const double *tr1 = S + TM_O_TT, *ti1 = tr1 + TM_QPL;
   const int mm1 = m > 0 ? m : 1, nm = nmax - mm1 + 1;
+  #pragma unroll 1
   for (int e = t; e < nm * nm; e += TM_WG) {
     const int a = e / nm, b = e % nm;  // 0-based within the block
     const int n = mm1 + a, nn = mm1 + b;
@@ -734,21 +750,26 @@ __device__ void tm_vigampl(double x, int nmax, int m, double *dv1, double *dv2) 
 __device__ void tm_amp_order(double *S, TmLds &L, const double *geo, int m, int nmax) {
   const int t = threadIdx.x;
   const int nmin = m > 1 ? m : 1;
+  // d functions of every direction: a = dv1, b = dv2 [dir][n]
+  double *va = S + TM_O_DV, *vb = va + (size_t)TM_NDIR * TM_B;
+#pragma unroll 1
   for (int dir = t; dir < TM_NDIR; dir += TM_WG)
-    tm_vigampl(geo[TM_G_COS + dir], nmax, m, L.u.v.a[dir], L.u.v.b[dir]);
+    tm_vigampl(geo[TM_G_COS + dir], nmax, m, va + (size_t)dir * TM_B, vb + (size_t)dir * TM_B);
   __syncthreads();
   const float *T = reinterpret_cast<const float *>(S + TM_O_TM);
   const size_t pl = (size_t)TM_NCAP * TM_NCAP;
   const double *rn = geo + TM_G_RN;
   double *E = S + TM_O_E;
   const double fm = (double)m;
+  #pragma unroll 1
   for (int it = t; it < NMU * nmax; it += TM_WG) {
     const int jj = it / nmax, n = 1 + it % nmax;
     double e[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (n >= nmin)
+#pragma unroll 1
       for (int nn = nmin; nn <= nmax; nn++) {
         const size_t o = (size_t)(n - 1) * TM_NCAP + (nn - 1);
-        const double c = fm * L.u.v.a[jj][nn], d = L.u.v.b[jj][nn];
+        const double c = fm * va[jj * TM_B + nn], d = vb[jj * TM_B + nn];
         // CAL = i^(nn - n - 1) rn(n, nn)
         const double r = rn[(size_t)n * TM_B + nn];
         const int q = ((nn - n - 1) % 4 + 4) % 4;
@@ -775,11 +796,13 @@ __device__ void tm_amp_order(double *S, TmLds &L, const double *geo, int m, int 
   }
   __syncthreads();
   double *A = S + TM_O_AMP;
+  #pragma unroll 1
   for (int pr = t; pr < TM_NPAIR; pr += TM_WG) {
     const int jj = pr / TM_NDIR, o = pr % TM_NDIR;
     double vv[2] = {0, 0}, vh[2] = {0, 0}, hv[2] = {0, 0}, hh[2] = {0, 0};
+#pragma unroll 1
     for (int n = nmin; n <= nmax; n++) {
-      const double a = fm * L.u.v.a[o][n], b = L.u.v.b[o][n];
+      const double a = fm * va[o * TM_B + n], b = vb[o * TM_B + n];
       const double *e = E + ((size_t)jj * TM_B + n) * 8;
       vv[0] += a * e[0] + b * e[2];
       vv[1] += a * e[1] + b * e[3];
@@ -816,14 +839,17 @@ __device__ void tm_azimuths(double *S, TmLds &L, const double *geo, int nmax, do
   const double fact_sca = 0.5 / (wave_num * wave_num);
   const double fext = (2.0 * PI) / pow(wave_num, 2.0);
   const double phi_w = 1.0 / 360.0 * (360.0 / TM_NAZ), phi0_w = 1.0 / 360.0 * (360.0 / 1);
+  #pragma unroll 1
   for (int k = t; k < 4 * NMU; k += TM_WG) pext[k] = 0.0;
   const double *A = S + TM_O_AMP;
   for (int g0 = 0; g0 < TM_NPAIR; g0 += TM_GRP) {
+    #pragma unroll 1
     for (int k = t; k < TM_GRP * (nmax + 1) * 8; k += TM_WG) {
       const int p = k / ((nmax + 1) * 8), r = k % ((nmax + 1) * 8);
       L.u.z.amp[p][r / 8][r % 8] = A[((size_t)(g0 + p) * TM_B + r / 8) * 8 + r % 8];
     }
     __syncthreads();
+    #pragma unroll 1
     for (int it = t; it < TM_GRP * TM_NAZ; it += TM_WG) {
       const int p = it / TM_NAZ, az = it % TM_NAZ;
       const int pr = g0 + p, jj = pr / TM_NDIR, o = pr % TM_NDIR;
@@ -832,8 +858,16 @@ __device__ void tm_azimuths(double *S, TmLds &L, const double *geo, int nmax, do
       const double ph = go[0] - gi[0];
       double vv[2] = {L.u.z.amp[p][0][0], L.u.z.amp[p][0][1]}, vh[2] = {0.0, 0.0},
              hv[2] = {0.0, 0.0}, hh[2] = {L.u.z.amp[p][0][6], L.u.z.amp[p][0][7]};
+      // cos(m dphi), sin(m dphi) by the rotation recurrence from m = 1
+      double s1, c1;
+      sincos(ph, &s1, &c1);
+      double cs = c1, sn = s1;
+#pragma unroll 1
       for (int m = 1; m <= nmax; m++) {
-        const double fc = 2.0 * cos(m * ph), fs = 2.0 * sin(m * ph);
+        const double fc = 2.0 * cs, fs = 2.0 * sn;
+        const double cn = cs * c1 - sn * s1;
+        sn = sn * c1 + cs * s1;
+        cs = cn;
         const double *a = L.u.z.amp[p][m];
         vv[0] += fc * a[0];
         vv[1] += fc * a[1];
@@ -1007,6 +1041,7 @@ __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *
   const int t = threadIdx.x;
   const int li = itf / d.nf, f = itf % d.nf;
   double *part = S + TM_O_PART, *hsum = S + TM_O_HYD, *usum = S + TM_O_UNIT;
+  #pragma unroll 1
   for (int k = t; k < TM_NACC; k += TM_WG) usum[k] = 0.0;
   if (t == 0) {
     L.kext = 0.0;
@@ -1022,6 +1057,7 @@ __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *
     const pam_hydro_desc &hd = hyd[h];
     const int nbin = w.dsd_flag[(size_t)li * d.nh + h];
     if (hd.scat_name != PAM_SCAT_TMATRIX || nbin <= 0) continue;
+    #pragma unroll 1
     for (int k = t; k < TM_NACC; k += TM_WG) hsum[k] = 0.0;
     const double *mi = w.mie_m + (size_t)itf * 4 + (hd.liq_ice == 1 ? 0 : 2);
     const cd ref = cd{mi[0], -mi[1]};  // m = n - i k (scatProperties.f90:243)
@@ -1053,10 +1089,12 @@ __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *
       }
       nbins++;
       nsum += (unsigned long long)nm;
+      #pragma unroll 1
       for (int k = t; k < TM_NACC; k += TM_WG) hsum[k] = hsum[k] + part[k] * nd * dd_;
       __syncthreads();
     }
     if (!ok) break;
+    #pragma unroll 1
     for (int k = t; k < TM_NACC; k += TM_WG) usum[k] = usum[k] + hsum[k];
     if (t == 0) L.kext = L.kext + hsum[TM_NSMAT + 0 + 2 * (0 + 2 * (NMU - 1))];
     __syncthreads();
@@ -1069,6 +1107,7 @@ __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *
   // slot entry (r, c) = smat(r, c) with r = (angle, Stokes) out, c = in
   const bool f32 = phase_in_f32(s, w);
   double *Tq = w.T + (size_t)itf * NN2, *Rq = w.R + (size_t)itf * NN2;
+  #pragma unroll 1
   for (int e = t; e < NN2; e += TM_WG) {
     const int r = e / NN, c = e % NN;
     const double tv = usum[r + NN * c], rv = usum[NN2 + r + NN * c];
@@ -1081,6 +1120,7 @@ __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *
     }
   }
   const double *uext = usum + TM_NSMAT, *uemis = uext + 4 * NMU;
+  #pragma unroll 1
   for (int jj = t; jj < NMU; jj += TM_WG) {
     w.tm_ext[((size_t)itf * NMU + jj) * 2 + 0] = uext[0 + 2 * (0 + 2 * jj)];
     w.tm_ext[((size_t)itf * NMU + jj) * 2 + 1] = uext[0 + 2 * (1 + 2 * jj)];
@@ -1094,7 +1134,7 @@ __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *
   }
 }
 
-__global__ void __launch_bounds__(TM_WG) k_tmatrix(Dims d, Settings s, const pam_hydro_desc *hyd,
+__global__ void __launch_bounds__(TM_WG) __attribute__((amdgpu_waves_per_eu(2))) k_tmatrix(Dims d, Settings s, const pam_hydro_desc *hyd,
                                                    const double *freqs, Work w) {
   __shared__ TmLds L;
   double *S = w.tm_scr + (size_t)blockIdx.x * TM_SLOT;
@@ -1118,6 +1158,7 @@ __global__ void __launch_bounds__(TM_WG) k_tm_hydro(int nbins, const double *dma
   __shared__ TmLds L;
   const int t = threadIdx.x;
   double *part = S + TM_O_PART, *hsum = S + TM_O_HYD;
+  #pragma unroll 1
   for (int k = t; k < TM_NACC; k += TM_WG) hsum[k] = 0.0;
   __syncthreads();
   const cd ref = cd{mre, -mim};
@@ -1139,9 +1180,11 @@ __global__ void __launch_bounds__(TM_WG) k_tm_hydro(int nbins, const double *dma
       break;
     }
     if (t == 0) nmax_out[ir] = nm;
+    #pragma unroll 1
     for (int k = t; k < TM_NACC; k += TM_WG) hsum[k] = hsum[k] + part[k] * nd * dd_;
     __syncthreads();
   }
+  #pragma unroll 1
   for (int k = t; k < TM_NACC; k += TM_WG) out[k] = hsum[k];
   if (t == 0) *status = st;
 }
