@@ -157,8 +157,9 @@ def test_parameters_from_the_scattering_name():
     for bad in ("ssrg-rt3_0.19", "ssrg-rt3_1_2_3_4_5", "ssrg-rt3_a_b_c"):
         with pytest.raises(ValueError):
             _capi.scattering_model(bad)
+    assert _capi.scattering_model("tmatrix")[0] == "tmatrix"
     with pytest.raises(NotImplementedError):
-        _capi.scattering_model("tmatrix")
+        _capi.scattering_model("ssrga")
 
 
 # ------------------------------------------------------------ device parity
@@ -304,7 +305,7 @@ def test_aggregates_at_32_angles_vs_oracle():
 def test_other_scattering_names_are_refused_with_the_reason():
     for name, word in (("ssrga_0.2", "unset"), ("ss-rayleigh-gans", "asserts"),
                        ("rayleigh", "backscatter-only"), ("rayleigh-gans", "backscatter-only"),
-                       ("tmatrix", "T-matrix"), ("liudb_8", "DDA-database"),
+                       ("liudb_8", "DDA-database"),
                        ("hongdb_3", "DDA-database"), ("nonsense", "unknown")):
         with pytest.raises(NotImplementedError, match=word):
             _capi.scattering_model(name)
