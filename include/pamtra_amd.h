@@ -293,15 +293,16 @@ int pam_run_device_noalloc(pam_context *ctx, const pam_settings *settings,
  * emis (2,NUMMU), NUMMU = pam_nummu().  freq_hz in Hz; (mre, mim) the
  * refractive index m = mre - i mim of scatProperties.f90:243; phase
  * (liq_ice) -1 mixes ice of density below 917 kg/m3 with air; dmax, del_d
- * [m], ndens [m^-4], density [kg/m3], as_ratio per bin; canting 0 (the
- * reference's canting list must be 0).  Host arrays.  nmax (optional,
+ * [m], ndens [m^-4], density [kg/m3], as_ratio and canting [deg] per bin.
+ * Host arrays.  nmax (optional,
  * nbins entries) receives each bin's T-matrix order.  Returns PAM_SUCCESS
  * or PAM_FATAL (invalid bins, a particle beyond the device's limits: order
  * 40, 100 Gauss points). */
 int pam_calc_tmatrix(pam_context *ctx, double freq_hz, double mre, double mim, int32_t phase,
                      int32_t nbins, const double *dmax, const double *del_d,
                      const double *ndens, const double *density, const double *as_ratio,
-                     double *smat, double *ext, double *emis, int32_t *nmax);
+                     const double *canting, double *smat, double *ext, double *emis,
+                     int32_t *nmax);
 
 int pam_last_stage_times(const pam_context *ctx, double *ms, int32_t n);
 const char *pam_stage_names(void);

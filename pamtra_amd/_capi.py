@@ -373,8 +373,8 @@ def check_tmatrix(hydros, settings, nmlSets):
     with the Legendre-series models is refused because the reference's result
     is then undefined: the T-matrix branch adds salb_hydro * kext_hydro to the
     layer albedo without ever setting salb_hydro (scatProperties.f90:141,
-    911).  Canting angles other than 0, the radar simulator and full-spectrum
-    input are not implemented on the device.  tmatrix_db 'file' (the
+    911).  The radar simulator and full-spectrum input are not implemented
+    with it on the device.  tmatrix_db 'file' (the
     reference's on-disk cache of the same results) computes them."""
     names = [int(h.scat_name) for h in hydros]
     if SCAT_NAMES["tmatrix"] not in names:
@@ -384,10 +384,6 @@ def check_tmatrix(hydros, settings, nmlSets):
             "scat_name 'tmatrix' mixed with 'mie-sphere' or 'ssrg-rt3': the reference's layer "
             "albedo then reads an unset value (scatProperties.f90:141,911), so that mix has no "
             "defined result")
-    for h in hydros:
-        if int(h.scat_name) == SCAT_NAMES["tmatrix"] and float(h.canting) > 0.0:
-            raise NotImplementedError("scat_name 'tmatrix' with a canting angle: only canting 0 "
-                                      "is implemented")
     if settings.active:
         raise NotImplementedError("scat_name 'tmatrix' with active (radar) simulation is not "
                                   "implemented")
@@ -604,7 +600,7 @@ def declare_library(lib: C.CDLL) -> C.CDLL:
     lib.pam_context_get_tuning.argtypes = [C.c_void_p, C.c_char_p]
     lib.pam_calc_tmatrix.restype = C.c_int
     lib.pam_calc_tmatrix.argtypes = ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32,
-                                      C.c_int32] + [_dp] * 8 + [_ip])
+                                      C.c_int32] + [_dp] * 9 + [_ip])
     return lib
 
 

@@ -437,30 +437,31 @@ extern "C" int pam_context_create(pam_context **out, int device) {
 extern "C" int pam_calc_tmatrix(pam_context *c, double freq_hz, double mre, double mim,
                                 int32_t phase, int32_t nbins, const double *dmax,
                                 const double *del_d, const double *ndens, const double *density,
-                                const double *as_ratio, double *smat, double *ext, double *emis,
-                                int32_t *nmax) {
-  if (!c || nbins <= 0 || !dmax || !del_d || !ndens || !density || !as_ratio || !smat || !ext ||
-      !emis || !(freq_hz > 0.0))
+                                const double *as_ratio, const double *canting, double *smat,
+                                double *ext, double *emis, int32_t *nmax) {
+  if (!c || nbins <= 0 || !dmax || !del_d || !ndens || !density || !as_ratio || !canting || !smat ||
+      !ext || !emis || !(freq_hz > 0.0))
     return PAM_FATAL;
   if (hipSetDevice(c->device) != hipSuccess) return PAM_FATAL;
   const size_t nacc = tm_nacc(), nb = (size_t)nbins;
-  const size_t bytes = (tm_slot_doubles() + tm_geo_doubles() + nacc + 5 * nb) * 8 + 8 + 4 * nb;
+  const size_t bytes = (tm_slot_doubles() + tm_geo_doubles() + nacc + 6 * nb) * 8 + 8 + 4 * nb;
   void *buf = nullptr;
   if (hipMalloc(&buf, bytes) != hipSuccess) return PAM_FATAL;
   double *scr = (double *)buf, *geo = scr + tm_slot_doubles(), *out = geo + tm_geo_doubles();
   double *in = out + nacc;
-  int *st = (int *)(in + 5 * nb), *nm = st + 2;
+  int *st = (int *)(in + 6 * nb), *nm = st + 2;
   int rc = PAM_FATAL;
-  const double *src[5] = {dmax, del_d, ndens, density, as_ratio};
+  const double *src[6] = {dmax, del_d, ndens, density, as_ratio, canting};
   bool ok = true;
-  for (int k = 0; k < 5 && ok; k++)
+  for (int k = 0; k < 6 && ok; k++)
     ok = hipMemcpyAsync(in + k * nb, src[k], nb * 8, hipMemcpyHostToDevice, c->stream) ==
          hipSuccess;
   std::vector<int> zeros(nb, 0);
   if (ok) ok = hipMemcpyAsync(nm, zeros.data(), 4 * nb, hipMemcpyHostToDevice, c->stream) == hipSuccess;
   if (ok) {
-    launch_tm_hydro(nbins, in, in + nb, in + 2 * nb, in + 3 * nb, in + 4 * nb, freq_hz, mre, mim,
-                    phase, c->d_mu, c->d_qw, c->d_tm_gauss, geo, scr, out, st, nm, c->stream);
+    launch_tm_hydro(nbins, in, in + nb, in + 2 * nb, in + 3 * nb, in + 4 * nb, in + 5 * nb, freq_hz,
+                    mre, mim, phase, c->d_mu, c->d_qw, c->d_tm_gauss, geo, scr, out, st, nm,
+                    c->stream);
     std::vector<double> h(nacc);
     int hst = -1;
     std::vector<int> hnm(nb);
@@ -768,16 +769,12 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   }
   {
     // the T-matrix producer: not beside the Legendre-series models (the
-    // reference's layer albedo then reads an unset value), not with a
-    // canting angle, a radar or full spectra on the device
+    // reference's layer albedo then reads an unset value), not with a radar
+    // or full spectra on the device
     bool tm = false, series = false;
     for (int i = 0; i < d.nh; i++) {
       if (hydros[i].scat_name == PAM_SCAT_TMATRIX) {
         tm = true;
-        if (hydros[i].canting > 0.0) {
-          set_msg(res, "tmatrix: only canting 0 is implemented on the device");
-          return PAM_FATAL;
-        }
       } else if (hydros[i].scat_name != PAM_SCAT_DISABLED) {
         series = true;
       }
@@ -825,7 +822,9 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   s.ncu = c->ncu;
   for (int i = 0; i < d.nh; i++) {
     if (hydros[i].scat_name == PAM_SCAT_SSRG_RT3) s.ssrg = 1;
-    if (hydros[i].scat_name == PAM_SCAT_TMATRIX) s.tmatrix = 1;
+    // 2: some T-matrix hydrometeor has a canting angle
+    if (hydros[i].scat_name == PAM_SCAT_TMATRIX)
+      s.tmatrix = std::max(s.tmatrix, hydros[i].canting > 0.0 ? 2 : 1);
   }
   {
     const char *mk = getenv("PAM_MIE");

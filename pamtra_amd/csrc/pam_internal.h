@@ -96,7 +96,8 @@ struct Settings {
   // coefficient-first kernel
   int mie_chunked;
   // some hydrometeor uses the T-matrix producer (then every scattering one
-  // does: calls mixing it with the Legendre-series models are refused)
+  // does: calls mixing it with the Legendre-series models are refused); 2
+  // when one of them has a canting angle
   int tmatrix;
   // 1: the mixed-precision path keeps the FP64 Mie kernel (PAM_MIE=fp64)
   int mie_fp64;
@@ -347,10 +348,10 @@ size_t tm_geo_doubles();
 std::vector<double> tm_gauss_table();
 size_t tm_nacc();
 void launch_tm_hydro(int nbins, const double *dmax, const double *del_d, const double *ndens,
-                     const double *rho, const double *as_ratio, double freq_hz, double mre,
-                     double mim, int phase, const double *mu, const double *qw,
-                     const double *gtab, double *geo, double *scr, double *out, int *status,
-                     int *nmax_out, hipStream_t st);
+                     const double *rho, const double *as_ratio, const double *canting,
+                     double freq_hz, double mre, double mim, int phase, const double *mu,
+                     const double *qw, const double *gtab, double *geo, double *scr, double *out,
+                     int *status, int *nmax_out, hipStream_t st);
 void launch_mie(const Dims &d, const Settings &s, const pam_hydro_desc *hyd, const Hydro4D &h4,
                 const Profile &p, const double *freqs, Work &w, hipStream_t st);
 // rows and doubles of one U slot of the coefficient-first Mie kernel

@@ -1,7 +1,8 @@
 // pam_tmatrix.hip -- the T-matrix producer for spheroids: calc_tmatrix and
 // calc_single_tmatrix (tmatrix.f90:14-684) over the extended-boundary-
 // condition code of tmatrix_amplq.lp.f, for passive runs (every incidence
-// angle), one particle orientation with canting 0, 30 scattering azimuths.
+// angle), one particle orientation (alpha 0, the descriptor's canting angle
+// as beta), 30 scattering azimuths.
 //
 // One workgroup per (layer, channel) unit takes the unit's T-matrix
 // hydrometeors and their size bins in the reference's order; per bin it
@@ -13,7 +14,8 @@
 //      reference inverts in binary128), T = -RG(Q) Q^-1 is FP64 and is
 //      stored in single precision (COMMON /TMAT/ is REAL*4);
 //   2. forms the amplitude matrices (tmatrix_amplq.lp.f:602-888) of the
-//      NMU x 2 NMU (incidence, scattering) angle pairs.  With canting 0 the
+//      NMU x 2 NMU (incidence, scattering) angle pairs.  With a canting angle
+//      the frames are tilted per azimuth (tm_azimuths_beta); with canting 0 the
 //      particle frame is the laboratory frame up to the reference's 1e-8
 //      angle offsets, so the azimuth enters only through cos(m dphi),
 //      sin(m dphi) and the frame rotations: for each m the sums over n, n'
@@ -64,10 +66,20 @@ constexpr size_t TM_O_Q = TM_O_ANN + (size_t)TM_B * TM_B;  // 4 planes: re hi, r
 constexpr size_t TM_O_RGQ = TM_O_Q + 4 * TM_QPL;            // 2 planes
 constexpr size_t TM_O_TT = TM_O_RGQ + 2 * TM_QPL;           // TR1, TI1
 constexpr size_t TM_O_TM = TM_O_TT + 2 * TM_QPL;            // float [8][NCAP][NCAP]
-constexpr size_t TM_O_AMP = TM_O_TM + 4 * (size_t)TM_NCAP * TM_NCAP;  // [NPAIR][TM_B][8]
-constexpr size_t TM_O_E = TM_O_AMP + (size_t)TM_NPAIR * TM_B * 8;      // [NMU][TM_B][8]
+// per (pair, m) amplitude sums at canting 0, or the incidence forms of every
+// order [m][NMU][TM_B][8] with a canting angle
+constexpr size_t TM_AMP_SIZE = (size_t)TM_NPAIR * TM_B * 8 > (size_t)TM_B * NMU * TM_B * 8
+                                   ? (size_t)TM_NPAIR * TM_B * 8
+                                   : (size_t)TM_B * NMU * TM_B * 8;
+constexpr size_t TM_O_AMP = TM_O_TM + 4 * (size_t)TM_NCAP * TM_NCAP;
+constexpr size_t TM_O_E = TM_O_AMP + TM_AMP_SIZE;                       // [NMU][TM_B][8]
 constexpr size_t TM_O_DV = TM_O_E + (size_t)NMU * TM_B * 8;     // d functions [2][NDIR][TM_B]
-constexpr size_t TM_O_PART = TM_O_DV + 2 * (size_t)TM_NDIR * TM_B;
+// with a canting angle: the frames of the incidence directions [NMU][6] and of
+// the scattering directions [NDIR][NAZ][6] (phi_p, R or R1^-1, cos theta_p),
+// and the per-azimuth contributions [NPAIR][NAZ][4]
+constexpr size_t TM_O_FR = TM_O_DV + 2 * (size_t)TM_NDIR * TM_B;
+constexpr size_t TM_O_CON = TM_O_FR + 6 * (size_t)(NMU + TM_NDIR * TM_NAZ);
+constexpr size_t TM_O_PART = TM_O_CON + 4 * (size_t)TM_NPAIR * TM_NAZ;
 constexpr size_t TM_O_HYD = TM_O_PART + TM_NACC;
 constexpr size_t TM_O_UNIT = TM_O_HYD + TM_NACC;
 constexpr size_t TM_SLOT = (TM_O_UNIT + TM_NACC + 63) / 64 * 64;
@@ -689,6 +701,52 @@ __device__ void tm_store(double *S, int m, int nmax) {
   __syncthreads();
 }
 
+// The frame of one direction (tmatrix_amplq.lp.f:630-745) with alpha = 0
+// and the canting angle beta [deg] (the reference's 1e-8 offsets included):
+// cos theta_p, phi_p and R (incidence) or R1 (scattering).
+__device__ void tm_frame(double tl, double pl, double beta, double &ctp, double &phip,
+                         double R[2][2]) {
+  const double pin = acos(-1.0), pin2 = pin * 0.5, pi = pin / 180.0;
+  const double alph = 0.0 * pi, eps = 1e-8;
+  double bet = beta * pi;
+  double thetl = tl * pi, phil = pl * pi;
+  if (thetl < pin2) thetl = thetl + eps;
+  if (thetl > pin2) thetl = thetl - eps;
+  if (phil < pin) phil = phil + eps;
+  if (phil > pin) phil = phil - eps;
+  if (bet <= pin2 && pin2 - bet <= eps) bet = bet - eps;
+  if (bet > pin2 && bet - pin2 <= eps) bet = bet + eps;
+  const double cb = cos(bet), sb = sin(bet), ct = cos(thetl), st = sin(thetl);
+  double cp = cos(phil - alph), sp = sin(phil - alph);
+  ctp = ct * cb + st * sb * cp;
+  const double thetp = acos(ctp);
+  const double cpp = cb * st * cp - sb * ct, spp = st * sp;
+  phip = atan(spp / cpp);
+  if (phip > 0.0 && sp < 0.0) phip = phip + pin;
+  if (phip < 0.0 && sp > 0.0) phip = phip + pin;
+  if (phip < 0.0) phip = phip + 2.0 * pin;
+  const double ca = cos(alph), sa = sin(alph);
+  const double B[3][3] = {{ca * cb, sa * cb, -sb}, {-sa, ca, 0.0}, {ca * sb, sa * sb, cb}};
+  cp = cos(phil);
+  sp = sin(phil);
+  const double AL[3][2] = {{ct * cp, -sp}, {ct * sp, cp}, {-st, 0.0}};
+  const double ct2 = ctp, st2 = sin(thetp), cp2 = cos(phip), sp2 = sin(phip);
+  const double AP[2][3] = {{ct2 * cp2, ct2 * sp2, -st2}, {-sp2, cp2, 0.0}};
+  double C[3][2];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 2; j++) {
+      double x = 0.0;
+      for (int k = 0; k < 3; k++) x = x + B[i][k] * AL[k][j];
+      C[i][j] = x;
+    }
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) {
+      double x = 0.0;
+      for (int k = 0; k < 3; k++) x = x + AP[i][k] * C[k][j];
+      R[i][j] = x;
+    }
+}
+
 // VIGAMPL (tmatrix_amplq.lp.f:905-960): d functions / sin and derivative
 __device__ void tm_vigampl(double x, int nmax, int m, double *dv1, double *dv2) {
   for (int n = 1; n <= nmax; n++) dv1[n] = dv2[n] = 0.0;
@@ -738,28 +796,15 @@ __device__ void tm_vigampl(double x, int nmax, int m, double *dv1, double *dv2) 
   }
 }
 
-// The amplitude sums of order m (tmatrix_amplq.lp.f:808-855), factored:
-//   E1(jj,n) = sum_n' CAL(n,n') (c T11 + d T12), E2 = sum CAL (c T21 + d T22),
-//   E3 = sum CAL (d T11 + c T12),               E4 = sum CAL (d T21 + c T22),
-// with c = m dv1(n'), d = dv2(n') of the incidence direction jj, then per
-// (jj, scattering direction o), with a = m dv1(n), b = dv2(n) of o:
-//   A_VV = sum_n a E1 + b E2,  A_VH = sum a E3 + b E4,
-//   A_HV = sum b E1 + a E2,    A_HH = sum b E3 + a E4,
-// so that VV = sum_m 2cos(m dphi) A_VV (A_VV alone for m = 0),
-// VH = sum 2sin A_VH, HV = -sum 2sin A_HV, HH = sum 2cos A_HH.
-__device__ void tm_amp_order(double *S, TmLds &L, const double *geo, int m, int nmax) {
+// The incidence-side forms of order m (see tm_amp_order) into E[jj][n][8],
+// from the stored T(m) and the d functions va = dv1, vb = dv2 [jj][n] of the
+// incidence directions.
+__device__ void tm_eforms(const double *S, const double *va, const double *vb, const double *rn,
+                          int m, int nmax, double *E) {
   const int t = threadIdx.x;
   const int nmin = m > 1 ? m : 1;
-  // d functions of every direction: a = dv1, b = dv2 [dir][n]
-  double *va = S + TM_O_DV, *vb = va + (size_t)TM_NDIR * TM_B;
-#pragma unroll 1
-  for (int dir = t; dir < TM_NDIR; dir += TM_WG)
-    tm_vigampl(geo[TM_G_COS + dir], nmax, m, va + (size_t)dir * TM_B, vb + (size_t)dir * TM_B);
-  __syncthreads();
   const float *T = reinterpret_cast<const float *>(S + TM_O_TM);
   const size_t pl = (size_t)TM_NCAP * TM_NCAP;
-  const double *rn = geo + TM_G_RN;
-  double *E = S + TM_O_E;
   const double fm = (double)m;
   #pragma unroll 1
   for (int it = t; it < NMU * nmax; it += TM_WG) {
@@ -795,6 +840,29 @@ __device__ void tm_amp_order(double *S, TmLds &L, const double *geo, int m, int 
     for (int k = 0; k < 8; k++) eo[k] = e[k];
   }
   __syncthreads();
+}
+
+// The amplitude sums of order m (tmatrix_amplq.lp.f:808-855), factored:
+//   E1(jj,n) = sum_n' CAL(n,n') (c T11 + d T12), E2 = sum CAL (c T21 + d T22),
+//   E3 = sum CAL (d T11 + c T12),               E4 = sum CAL (d T21 + c T22),
+// with c = m dv1(n'), d = dv2(n') of the incidence direction jj, then per
+// (jj, scattering direction o), with a = m dv1(n), b = dv2(n) of o:
+//   A_VV = sum_n a E1 + b E2,  A_VH = sum a E3 + b E4,
+//   A_HV = sum b E1 + a E2,    A_HH = sum b E3 + a E4,
+// so that VV = sum_m 2cos(m dphi) A_VV (A_VV alone for m = 0),
+// VH = sum 2sin A_VH, HV = -sum 2sin A_HV, HH = sum 2cos A_HH.
+__device__ void tm_amp_order(double *S, TmLds &L, const double *geo, int m, int nmax) {
+  const int t = threadIdx.x;
+  const int nmin = m > 1 ? m : 1;
+  // d functions of every direction: a = dv1, b = dv2 [dir][n]
+  double *va = S + TM_O_DV, *vb = va + (size_t)TM_NDIR * TM_B;
+#pragma unroll 1
+  for (int dir = t; dir < TM_NDIR; dir += TM_WG)
+    tm_vigampl(geo[TM_G_COS + dir], nmax, m, va + (size_t)dir * TM_B, vb + (size_t)dir * TM_B);
+  __syncthreads();
+  double *E = S + TM_O_E;
+  tm_eforms(S, va, vb, geo + TM_G_RN, m, nmax, E);
+  const double fm = (double)m;
   double *A = S + TM_O_AMP;
   #pragma unroll 1
   for (int pr = t; pr < TM_NPAIR; pr += TM_WG) {
@@ -824,6 +892,49 @@ __device__ void tm_amp_order(double *S, TmLds &L, const double *geo, int m, int 
     ao[7] = hh[1];
   }
   __syncthreads();
+}
+
+// One (pair, azimuth) of the azimuth sums (tmatrix_amplq.lp.f:857-869,
+// tmatrix.f90:616-639): the amplitude matrix from the order sums (before the
+// division by k), the frame rotations R (gi[1..4]) and R1^-1 (go[1..4]),
+// |S|^2 into the four contributions c, and the imaginary parts of the
+// amplitudes S11, S22 (the forward extinction).
+__device__ __forceinline__ void tm_item(double vv[2], double vh[2], double hv[2], double hh[2],
+                                        const double *gi, const double *go, double dk,
+                                        double wave_num, double fact_sca, double phi_w,
+                                        double c[4], double &im11, double &im22) {
+  for (int q = 0; q < 2; q++) {
+    vv[q] = vv[q] / dk;
+    vh[q] = vh[q] / dk;
+    hv[q] = hv[q] / dk;
+    hh[q] = hh[q] / dk;
+  }
+  // rotations R (incidence) and R1^-1 (scattering), tmatrix_amplq.lp.f:862-869
+  const double r11 = gi[1], r12 = gi[2], r21 = gi[3], r22 = gi[4];
+  const double s11 = go[1], s12 = go[2], s21 = go[3], s22 = go[4];
+  double cvv[2], cvh[2], chv[2], chh[2], o11[2], o12[2], o21[2], o22[2];
+  for (int q = 0; q < 2; q++) {
+    cvv[q] = vv[q] * r11 + vh[q] * r21;
+    cvh[q] = vv[q] * r12 + vh[q] * r22;
+    chv[q] = hv[q] * r11 + hh[q] * r21;
+    chh[q] = hv[q] * r12 + hh[q] * r22;
+  }
+  for (int q = 0; q < 2; q++) {
+    o11[q] = (s11 * cvv[q] + s12 * chv[q]) * wave_num;
+    o12[q] = (s11 * cvh[q] + s12 * chh[q]) * wave_num;
+    o21[q] = (s21 * cvv[q] + s22 * chv[q]) * wave_num;
+    o22[q] = (s21 * cvh[q] + s22 * chh[q]) * wave_num;
+  }
+  const double a11 = o11[0] * o11[0] - o11[1] * (-o11[1]);
+  const double a12 = o12[0] * o12[0] - o12[1] * (-o12[1]);
+  const double a21 = o21[0] * o21[0] - o21[1] * (-o21[1]);
+  const double a22 = o22[0] * o22[0] - o22[1] * (-o22[1]);
+  c[0] = (fact_sca * (((a11 + a12) + a21) + a22)) * phi_w;
+  c[1] = (fact_sca * (((a11 - a12) + a21) - a22)) * phi_w;
+  c[2] = (fact_sca * (((a11 + a12) - a21) - a22)) * phi_w;
+  c[3] = (fact_sca * (((a11 - a12) - a21) + a22)) * phi_w;
+  im11 = o11[1];
+  im22 = o22[1];
 }
 
 // The azimuth sums (tmatrix.f90:588-663) of every pair from the per-order
@@ -878,38 +989,10 @@ __device__ void tm_azimuths(double *S, TmLds &L, const double *geo, int nmax, do
         hh[0] += fc * a[6];
         hh[1] += fc * a[7];
       }
-      for (int q = 0; q < 2; q++) {
-        vv[q] = vv[q] / dk;
-        vh[q] = vh[q] / dk;
-        hv[q] = hv[q] / dk;
-        hh[q] = hh[q] / dk;
-      }
-      // rotations R (incidence) and R1^-1 (scattering), tmatrix_amplq.lp.f:862-869
-      const double r11 = gi[1], r12 = gi[2], r21 = gi[3], r22 = gi[4];
-      const double s11 = go[1], s12 = go[2], s21 = go[3], s22 = go[4];
-      double cvv[2], cvh[2], chv[2], chh[2], o11[2], o12[2], o21[2], o22[2];
-      for (int q = 0; q < 2; q++) {
-        cvv[q] = vv[q] * r11 + vh[q] * r21;
-        cvh[q] = vv[q] * r12 + vh[q] * r22;
-        chv[q] = hv[q] * r11 + hh[q] * r21;
-        chh[q] = hv[q] * r12 + hh[q] * r22;
-      }
-      for (int q = 0; q < 2; q++) {
-        o11[q] = (s11 * cvv[q] + s12 * chv[q]) * wave_num;
-        o12[q] = (s11 * cvh[q] + s12 * chh[q]) * wave_num;
-        o21[q] = (s21 * cvv[q] + s22 * chv[q]) * wave_num;
-        o22[q] = (s21 * cvh[q] + s22 * chh[q]) * wave_num;
-      }
-      const double a11 = o11[0] * o11[0] - o11[1] * (-o11[1]);
-      const double a12 = o12[0] * o12[0] - o12[1] * (-o12[1]);
-      const double a21 = o21[0] * o21[0] - o21[1] * (-o21[1]);
-      const double a22 = o22[0] * o22[0] - o22[1] * (-o22[1]);
-      L.u.z.c[p][az][0] = (fact_sca * (((a11 + a12) + a21) + a22)) * phi_w;
-      L.u.z.c[p][az][1] = (fact_sca * (((a11 - a12) + a21) - a22)) * phi_w;
-      L.u.z.c[p][az][2] = (fact_sca * (((a11 + a12) - a21) - a22)) * phi_w;
-      L.u.z.c[p][az][3] = (fact_sca * (((a11 - a12) - a21) + a22)) * phi_w;
+      double im11, im22;
+      tm_item(vv, vh, hv, hh, gi, go, dk, wave_num, fact_sca, phi_w, L.u.z.c[p][az], im11, im22);
       if (az == 0 && o == jj) {  // forward: thet0 == thet, phi0 == phi
-        const double ip = (o11[1] + o22[1]) * fext, imn = (o11[1] - o22[1]) * fext;
+        const double ip = (im11 + im22) * fext, imn = (im11 - im22) * fext;
         pext[0 + 2 * (0 + 2 * jj)] = 0.0 + phi0_w * ip;
         pext[0 + 2 * (1 + 2 * jj)] = 0.0 + phi0_w * imn;
         pext[1 + 2 * (0 + 2 * jj)] = 0.0 + phi0_w * imn;
@@ -947,13 +1030,226 @@ __device__ void tm_azimuths(double *S, TmLds &L, const double *geo, int nmax, do
   __syncthreads();
 }
 
+// ---- a canting angle: the particle frame turns with the scattering azimuth
+
+// Frames of every incidence direction (azimuth 0) and scattering direction
+// for the canting angle beta [deg] (tmatrix.f90:588-612 angles, the frame
+// of tmatrix_amplq.lp.f:630-760) into the frame region of the scratch.
+__device__ void tm_frames_beta(double *S, double beta, const double *mu) {
+  const float thet_max = 179.9999f;
+  double *fr = S + TM_O_FR;
+#pragma unroll 1
+  for (int k = threadIdx.x; k < NMU + TM_NDIR * TM_NAZ; k += TM_WG) {
+    double th, phi;
+    double *g;
+    bool inc = k < NMU;
+    if (inc) {
+      th = acos(mu[k] * 1.0) * 180.0 / PI;
+      phi = 0.0;
+      g = fr + 6 * k;
+    } else {
+      const int q = k - NMU, o = q / TM_NAZ, az = q % TM_NAZ, kk = o / NMU, ll = o % NMU;
+      th = acos(mu[ll] * (kk == 0 ? 1.0 : -1.0)) * 180.0 / PI;
+      phi = 360.0 / (double)(float)TM_NAZ * ((double)(float)(az + 1) - 1.0);
+      g = fr + 6 * ((size_t)NMU + (size_t)o * TM_NAZ + az);
+    }
+    if (th > (double)thet_max) th = 180.0;
+    double ctp, phip, R[2][2];
+    tm_frame(th, phi, beta, ctp, phip, R);
+    g[0] = phip;
+    if (inc) {
+      g[1] = R[0][0];
+      g[2] = R[0][1];
+      g[3] = R[1][0];
+      g[4] = R[1][1];
+    } else {
+      const double dt = 1.0 / (R[0][0] * R[1][1] - R[0][1] * R[1][0]);
+      g[1] = R[1][1] * dt;
+      g[2] = -R[0][1] * dt;
+      g[3] = -R[1][0] * dt;
+      g[4] = R[0][0] * dt;
+    }
+    g[5] = ctp;
+  }
+  __syncthreads();
+}
+
+// The incidence forms of order m at the tilted incidence frames, kept for
+// every m: E_all[m][jj][n][8] in the amplitude region.
+__device__ void tm_eforms_beta(double *S, const double *geo, int m, int nmax) {
+  double *va = S + TM_O_DV, *vb = va + (size_t)TM_NDIR * TM_B;
+#pragma unroll 1
+  for (int jj = threadIdx.x; jj < NMU; jj += TM_WG)
+    tm_vigampl(S[TM_O_FR + 6 * jj + 5], nmax, m, va + (size_t)jj * TM_B, vb + (size_t)jj * TM_B);
+  __syncthreads();
+  tm_eforms(S, va, vb, geo + TM_G_RN, m, nmax, S + TM_O_AMP + (size_t)m * NMU * TM_B * 8);
+}
+
+// The scattering-side sums of order m of one pair: VIGAMPL's recurrence at
+// the scattering direction's cos theta_p (tmatrix_amplq.lp.f:905-960), run
+// in n, accumulating A_VV, A_VH, A_HV, A_HH (see tm_amp_order) from the
+// incidence forms e[n][8] as the d functions come.
+__device__ __forceinline__ void tm_vig_sum(double x, int nmax, int m, const double *e,
+                                           double acc[8]) {
+  for (int k = 0; k < 8; k++) acc[k] = 0.0;
+  const int nmin = m > 1 ? m : 1;
+  const double fm = (double)m;
+  auto add = [&](int n, double dv1, double dv2) {
+    if (n < nmin) return;
+    const double a = fm * dv1, b = dv2;
+    const double *q = e + (size_t)n * 8;
+    acc[0] += a * q[0] + b * q[2];
+    acc[1] += a * q[1] + b * q[3];
+    acc[2] += a * q[4] + b * q[6];
+    acc[3] += a * q[5] + b * q[7];
+    acc[4] += b * q[0] + a * q[2];
+    acc[5] += b * q[1] + a * q[3];
+    acc[6] += b * q[4] + a * q[6];
+    acc[7] += b * q[5] + a * q[7];
+  };
+  if (fabs(1.0 - fabs(x)) <= 1e-10) {
+    if (m != 1) return;
+#pragma unroll 1
+    for (int n = 1; n <= nmax; n++) {
+      double dn = 0.5 * sqrt((double)(n * (n + 1)));
+      if (x < 0.0) dn = dn * (((n + 1) & 1) ? -1.0 : 1.0);
+      const double dv1 = dn;
+      if (x < 0.0) dn = -dn;
+      add(n, dv1, dn);
+    }
+    return;
+  }
+  const double qs = sqrt(1.0 - x * x), qs1 = 1.0 / qs, dsi = qs1;
+  if (m == 0) {
+    double d1 = 1.0, d2 = x;
+#pragma unroll 1
+    for (int n = 1; n <= nmax; n++) {
+      const double qn = n, qn1 = n + 1, qn2 = 2 * n + 1;
+      const double d3 = (qn2 * x * d2 - qn * d1) / qn1;
+      const double der = qs1 * (qn1 * qn / qn2) * (-d1 + d3);
+      add(n, d2 * dsi, der);
+      d1 = d2;
+      d2 = d3;
+    }
+    return;
+  }
+  double a = 1.0;
+  const double qmm = (double)(m * m);
+#pragma unroll 1
+  for (int i = 1; i <= m; i++) {
+    const int i2 = i * 2;
+    a = a * sqrt((double)(i2 - 1) / (double)i2) * qs;
+  }
+  double d1 = 0.0, d2 = a;
+#pragma unroll 1
+  for (int n = m; n <= nmax; n++) {
+    const double qn = n, qn2 = 2 * n + 1, qn1 = n + 1;
+    const double qnm = sqrt(qn * qn - qmm), qnm1 = sqrt(qn1 * qn1 - qmm);
+    const double d3 = (qn2 * x * d2 - qnm * d1) / qnm1;
+    const double der = qs1 * (-(qn1 * qnm * d1) + qn * qnm1 * d3) / qn2;
+    add(n, d2 * dsi, der);
+    d1 = d2;
+    d2 = d3;
+  }
+}
+
+// The azimuth sums with a canting angle: per (pair, azimuth) the sums over m
+// of the scattering-side forms at that azimuth's tilted frame, then as
+// tm_azimuths.
+__device__ void tm_azimuths_beta(double *S, int nmax, double lam, const double *qw) {
+  const int t = threadIdx.x;
+  double *part = S + TM_O_PART, *pext = part + TM_NSMAT, *pemis = pext + 4 * NMU;
+  const double wave_num = 2.0 * PI / lam;
+  const double dk = 2.0 * acos(-1.0) / lam;
+  const double fact_sca = 0.5 / (wave_num * wave_num);
+  const double fext = (2.0 * PI) / pow(wave_num, 2.0);
+  const double phi_w = 1.0 / 360.0 * (360.0 / TM_NAZ), phi0_w = 1.0 / 360.0 * (360.0 / 1);
+#pragma unroll 1
+  for (int k = t; k < 4 * NMU; k += TM_WG) pext[k] = 0.0;
+  __syncthreads();
+  const double *fr = S + TM_O_FR, *Eall = S + TM_O_AMP;
+  double *con = S + TM_O_CON;
+#pragma unroll 1
+  for (int it = t; it < TM_NPAIR * TM_NAZ; it += TM_WG) {
+    const int pr = it / TM_NAZ, az = it % TM_NAZ, jj = pr / TM_NDIR, o = pr % TM_NDIR;
+    const double *gi = fr + 6 * jj, *go = fr + 6 * ((size_t)NMU + (size_t)o * TM_NAZ + az);
+    const double ph = go[0] - gi[0];
+    double s1, c1;
+    sincos(ph, &s1, &c1);
+    double cs = c1, sn = s1;
+    double vv[2] = {0.0, 0.0}, vh[2] = {0.0, 0.0}, hv[2] = {0.0, 0.0}, hh[2] = {0.0, 0.0};
+    double acc[8];
+#pragma unroll 1
+    for (int m = 0; m <= nmax; m++) {
+      tm_vig_sum(go[5], nmax, m, Eall + ((size_t)m * NMU + jj) * TM_B * 8, acc);
+      if (m == 0) {
+        vv[0] += acc[0];
+        vv[1] += acc[1];
+        hh[0] += acc[6];
+        hh[1] += acc[7];
+        continue;
+      }
+      const double fc = 2.0 * cs, fs = 2.0 * sn;
+      const double cn = cs * c1 - sn * s1;
+      sn = sn * c1 + cs * s1;
+      cs = cn;
+      vv[0] += fc * acc[0];
+      vv[1] += fc * acc[1];
+      vh[0] += fs * acc[2];
+      vh[1] += fs * acc[3];
+      hv[0] -= fs * acc[4];
+      hv[1] -= fs * acc[5];
+      hh[0] += fc * acc[6];
+      hh[1] += fc * acc[7];
+    }
+    double im11, im22;
+    tm_item(vv, vh, hv, hh, gi, go, dk, wave_num, fact_sca, phi_w,
+            con + ((size_t)pr * TM_NAZ + az) * 4, im11, im22);
+    if (az == 0 && o == jj) {
+      const double ip = (im11 + im22) * fext, imn = (im11 - im22) * fext;
+      pext[0 + 2 * (0 + 2 * jj)] = 0.0 + phi0_w * ip;
+      pext[0 + 2 * (1 + 2 * jj)] = 0.0 + phi0_w * imn;
+      pext[1 + 2 * (0 + 2 * jj)] = 0.0 + phi0_w * imn;
+      pext[1 + 2 * (1 + 2 * jj)] = 0.0 + phi0_w * ip;
+    }
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int pr = t; pr < TM_NPAIR; pr += TM_WG) {
+    const int jj = pr / TM_NDIR, o = pr % TM_NDIR, kk = o / NMU, ll = o % NMU;
+    double sm[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int az = 0; az < TM_NAZ; az++)
+      for (int q = 0; q < 4; q++) sm[q] = sm[q] + con[((size_t)pr * TM_NAZ + az) * 4 + q];
+    auto at = [&](int a, int b) { return part + a + 2 * (ll + NMU * (b + 2 * (jj + NMU * kk))); };
+    *at(0, 0) = 0.0 + sm[0] * phi0_w;
+    *at(0, 1) = 0.0 + sm[1] * phi0_w;
+    *at(1, 0) = 0.0 + sm[2] * phi0_w;
+    *at(1, 1) = 0.0 + sm[3] * phi0_w;
+  }
+  __syncthreads();
+  if (t < NMU) {
+    const int jj = t;
+    double e11 = 0.0, e12 = 0.0;
+    for (int kk = 0; kk < 2; kk++)
+      for (int ll = 0; ll < NMU; ll++) {
+        const double z11 = part[0 + 2 * (ll + NMU * (0 + 2 * (jj + NMU * kk)))];
+        const double z12 = part[0 + 2 * (ll + NMU * (1 + 2 * (jj + NMU * kk)))];
+        e11 = e11 + z11 * qw[ll] * 2.0 * PI;
+        e12 = e12 + z12 * qw[ll] * 2.0 * PI;
+      }
+    pemis[0 + 2 * jj] = pext[0 + 2 * (0 + 2 * jj)] - e11;
+    pemis[1 + 2 * jj] = pext[0 + 2 * (1 + 2 * jj)] - e12;
+  }
+  __syncthreads();
+}
+
 // One particle: the T matrix and its phase matrix, extinction and emission
 // into the part accumulator.  Returns false (uniformly) on a failure the
 // reference reports as fatal, or where the T-matrix order or the Gauss
 // points would exceed the device limits.
 __device__ bool tm_particle(double *S, TmLds &L, const double *geo, const double *gtab,
                             double freq_hz, double mrr, double mri, double axi, double as_ratio,
-                            const double *mu, const double *qw, int *nmax_out) {
+                            double beta, const double *mu, const double *qw, int *nmax_out) {
   const int t = threadIdx.x;
   TmPart P;
   P.lam = C_LIGHT / freq_hz;
@@ -1021,14 +1317,26 @@ __device__ bool tm_particle(double *S, TmLds &L, const double *geo, const double
   // particle is refused rather than taken unconverged
   if (!conv) return false;
   *nmax_out = nmax;
+  // canting 0: the factored amplitude sums; otherwise the tilted frames
+  const bool tilt = beta != 0.0;
+  if (tilt) tm_frames_beta(S, beta, mu);
   tm_store(S, 0, nmax);
-  tm_amp_order(S, L, geo, 0, nmax);
+  if (tilt)
+    tm_eforms_beta(S, geo, 0, nmax);
+  else
+    tm_amp_order(S, L, geo, 0, nmax);
   for (int m = 1; m <= nmax; m++) {
     if (!tm_tmatr(S, L, P, m, ngauss, nmax)) return false;
     tm_store(S, m, nmax);
-    tm_amp_order(S, L, geo, m, nmax);
+    if (tilt)
+      tm_eforms_beta(S, geo, m, nmax);
+    else
+      tm_amp_order(S, L, geo, m, nmax);
   }
-  tm_azimuths(S, L, geo, nmax, P.lam, mu, qw);
+  if (tilt)
+    tm_azimuths_beta(S, nmax, P.lam, qw);
+  else
+    tm_azimuths(S, L, geo, nmax, P.lam, mu, qw);
   (void)t;
   return true;
 }
@@ -1063,7 +1371,7 @@ __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *
     const cd ref = cd{mi[0], -mi[1]};  // m = n - i k (scatProperties.f90:243)
     const double ar = w.dsd_ar[(size_t)li * d.nh + h];
     const double cant = (hd.canting < 0.0 || isnan(hd.canting)) ? 0.0 : hd.canting;
-    if (!(ar > 0.0) || cant != 0.0) {
+    if (!(ar > 0.0)) {
       ok = false;
       break;
     }
@@ -1083,7 +1391,7 @@ __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *
       int nm = 0;
       __syncthreads();
       if (!tm_particle(S, L, w.tm_geo, w.tm_gauss, freqs[f] * 1.e9, mmix.re, -mmix.im, axi, ar,
-                       w.mu, w.qw, &nm)) {
+                       cant, w.mu, w.qw, &nm)) {
         ok = false;
         break;
       }
@@ -1150,7 +1458,8 @@ __global__ void __launch_bounds__(TM_WG) __attribute__((amdgpu_waves_per_eu(2)))
 __global__ void __launch_bounds__(TM_WG) k_tm_hydro(int nbins, const double *dmax,
                                                     const double *del_d, const double *ndens,
                                                     const double *rho, const double *as_ratio,
-                                                    double freq_hz, double mre, double mim,
+                                                    const double *canting, double freq_hz,
+                                                    double mre, double mim,
                                                     int phase, const double *geo,
                                                     const double *gtab, const double *mu,
                                                     const double *qw, double *S, double *out,
@@ -1164,9 +1473,9 @@ __global__ void __launch_bounds__(TM_WG) k_tm_hydro(int nbins, const double *dma
   const cd ref = cd{mre, -mim};
   int st = 0;
   for (int ir = 0; ir < nbins; ir++) {
-    const double nd = ndens[ir], dd_ = del_d[ir], ar = as_ratio[ir];
+    const double nd = ndens[ir], dd_ = del_d[ir], ar = as_ratio[ir], cant = canting[ir];
     if (nd == 0.0) continue;
-    if (!(nd > 0.0) || !(dd_ > 0.0) || !(rho[ir] > 0.0) || !(ar > 0.0)) {
+    if (!(nd > 0.0) || !(dd_ > 0.0) || !(rho[ir] > 0.0) || !(ar > 0.0) || !(cant >= 0.0)) {
       st = E_TMATRIX;
       break;
     }
@@ -1175,7 +1484,7 @@ __global__ void __launch_bounds__(TM_WG) k_tm_hydro(int nbins, const double *dma
                                  : 0.5 * dmax[ir] / pow(ar, 2.0 / 3.0);
     int nm = 0;
     __syncthreads();
-    if (!tm_particle(S, L, geo, gtab, freq_hz, mmix.re, -mmix.im, axi, ar, mu, qw, &nm)) {
+    if (!tm_particle(S, L, geo, gtab, freq_hz, mmix.re, -mmix.im, axi, ar, cant, mu, qw, &nm)) {
       st = E_TMATRIX;
       break;
     }
@@ -1187,48 +1496,6 @@ __global__ void __launch_bounds__(TM_WG) k_tm_hydro(int nbins, const double *dma
   #pragma unroll 1
   for (int k = t; k < TM_NACC; k += TM_WG) out[k] = hsum[k];
   if (t == 0) *status = st;
-}
-
-// The frame of one direction (tmatrix_amplq.lp.f:630-745) with alpha = 0
-// and beta = 0 (the reference's 1e-8 offsets included): cos theta_p, phi_p
-// and R (incidence) or R1 (scattering).
-__device__ void tm_frame(double tl, double pl, double &ctp, double &phip, double R[2][2]) {
-  const double pin = acos(-1.0), pin2 = pin * 0.5, pi = pin / 180.0;
-  const double alph = 0.0 * pi, bet = 0.0 * pi, eps = 1e-8;
-  double thetl = tl * pi, phil = pl * pi;
-  if (thetl < pin2) thetl = thetl + eps;
-  if (thetl > pin2) thetl = thetl - eps;
-  if (phil < pin) phil = phil + eps;
-  if (phil > pin) phil = phil - eps;
-  const double cb = cos(bet), sb = sin(bet), ct = cos(thetl), st = sin(thetl);
-  double cp = cos(phil - alph), sp = sin(phil - alph);
-  ctp = ct * cb + st * sb * cp;
-  const double thetp = acos(ctp);
-  const double cpp = cb * st * cp - sb * ct, spp = st * sp;
-  phip = atan(spp / cpp);
-  if (phip > 0.0 && sp < 0.0) phip = phip + pin;
-  if (phip < 0.0 && sp > 0.0) phip = phip + pin;
-  if (phip < 0.0) phip = phip + 2.0 * pin;
-  const double ca = cos(alph), sa = sin(alph);
-  const double B[3][3] = {{ca * cb, sa * cb, -sb}, {-sa, ca, 0.0}, {ca * sb, sa * sb, cb}};
-  cp = cos(phil);
-  sp = sin(phil);
-  const double AL[3][2] = {{ct * cp, -sp}, {ct * sp, cp}, {-st, 0.0}};
-  const double ct2 = ctp, st2 = sin(thetp), cp2 = cos(phip), sp2 = sin(phip);
-  const double AP[2][3] = {{ct2 * cp2, ct2 * sp2, -st2}, {-sp2, cp2, 0.0}};
-  double C[3][2];
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 2; j++) {
-      double x = 0.0;
-      for (int k = 0; k < 3; k++) x = x + B[i][k] * AL[k][j];
-      C[i][j] = x;
-    }
-  for (int i = 0; i < 2; i++)
-    for (int j = 0; j < 2; j++) {
-      double x = 0.0;
-      for (int k = 0; k < 3; k++) x = x + AP[i][k] * C[k][j];
-      R[i][j] = x;
-    }
 }
 
 // Geometry of the call: frames of every incidence and scattering direction
@@ -1243,7 +1510,7 @@ __global__ void k_tm_geom(const double *mu, double *geo) {
       double th = acos(mu[k] * 1.0) * 180.0 / PI;
       if (th > (double)thet_max) th = 180.0;
       double ctp, phip, R[2][2];
-      tm_frame(th, 0.0, ctp, phip, R);
+      tm_frame(th, 0.0, 0.0, ctp, phip, R);
       double *g = geo + TM_G_INC + 5 * k;
       g[0] = phip;
       g[1] = R[0][0];
@@ -1256,7 +1523,7 @@ __global__ void k_tm_geom(const double *mu, double *geo) {
       if (th > (double)thet_max) th = 180.0;
       const double phi = 360.0 / (double)(float)TM_NAZ * ((double)(float)(az + 1) - 1.0);
       double ctp, phip, R[2][2];
-      tm_frame(th, phi, ctp, phip, R);
+      tm_frame(th, phi, 0.0, ctp, phip, R);
       const double dt = 1.0 / (R[0][0] * R[1][1] - R[0][1] * R[1][0]);
       double *g = geo + TM_G_OUT + 5 * ((size_t)o * TM_NAZ + az);
       g[0] = phip;
@@ -1343,14 +1610,14 @@ void launch_tmatrix(const Dims &d, const Settings &s, const pam_hydro_desc *hyd,
 size_t tm_nacc() { return TM_NACC; }
 
 void launch_tm_hydro(int nbins, const double *dmax, const double *del_d, const double *ndens,
-                     const double *rho, const double *as_ratio, double freq_hz, double mre,
-                     double mim, int phase, const double *mu, const double *qw,
-                     const double *gtab, double *geo, double *scr, double *out, int *status,
-                     int *nmax_out, hipStream_t st) {
+                     const double *rho, const double *as_ratio, const double *canting,
+                     double freq_hz, double mre, double mim, int phase, const double *mu,
+                     const double *qw, const double *gtab, double *geo, double *scr, double *out,
+                     int *status, int *nmax_out, hipStream_t st) {
   hipLaunchKernelGGL(k_tm_geom, dim3(8), dim3(256), 0, st, mu, geo);
   hipLaunchKernelGGL(k_tm_hydro, dim3(1), dim3(TM_WG), 0, st, nbins, dmax, del_d, ndens, rho,
-                     as_ratio, freq_hz, mre, mim, phase, geo, gtab, mu, qw, scr, out, status,
-                     nmax_out);
+                     as_ratio, canting, freq_hz, mre, mim, phase, geo, gtab, mu, qw, scr, out,
+                     status, nmax_out);
 }
 
 }  // namespace pam

@@ -26,13 +26,17 @@ pytestmark = pytest.mark.gpu
 
 NMU = 16
 
+# (freq, m, phase, dmax, as_ratio, density, canting)
 CASES = [
-    (35.5e9, 5.5 - 2.8j, 1, [1.0e-3, 2.0e-3], [0.8, 0.8], 1000.0),
-    (35.5e9, 5.5 - 2.8j, 1, [3.0e-3, 4.5e-3], [0.6, 0.6], 1000.0),
-    (94.0e9, 3.5 - 1.9j, 1, [4.0e-3, 1.0e-3], [0.7, 1.2], 1000.0),
-    (89.0e9, 1.78 - 0.003j, -1, [5.0e-3, 2.0e-3], [0.6, 1.4], 200.0),
-    (183.31e9, 1.78 - 0.004j, -1, [6.0e-3, 3.0e-3], [0.6, 0.6], 100.0),
-    (10.65e9, 7.0 - 2.5j, 1, [1.0e-3, 0.5e-3], [1.0, 1.0], 1000.0),
+    (35.5e9, 5.5 - 2.8j, 1, [1.0e-3, 2.0e-3], [0.8, 0.8], 1000.0, 0.0),
+    (35.5e9, 5.5 - 2.8j, 1, [3.0e-3, 4.5e-3], [0.6, 0.6], 1000.0, 0.0),
+    (94.0e9, 3.5 - 1.9j, 1, [4.0e-3, 1.0e-3], [0.7, 1.2], 1000.0, 0.0),
+    (89.0e9, 1.78 - 0.003j, -1, [5.0e-3, 2.0e-3], [0.6, 1.4], 200.0, 0.0),
+    (183.31e9, 1.78 - 0.004j, -1, [6.0e-3, 3.0e-3], [0.6, 0.6], 100.0, 0.0),
+    (10.65e9, 7.0 - 2.5j, 1, [1.0e-3, 0.5e-3], [1.0, 1.0], 1000.0, 0.0),
+    (35.5e9, 5.5 - 2.8j, 1, [2.0e-3, 3.0e-3], [0.8, 0.7], 1000.0, 20.0),
+    (89.0e9, 1.78 - 0.003j, -1, [4.0e-3, 2.0e-3], [0.6, 1.5], 200.0, 45.0),
+    (94.0e9, 3.5 - 1.9j, 1, [3.0e-3, 1.0e-3], [0.7, 0.9], 1000.0, 90.0),
 ]
 
 
@@ -40,16 +44,17 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def _device_calc_tmatrix(f, m, phase, dmax, ar, dens, nd, dd):
+def _device_calc_tmatrix(f, m, phase, dmax, ar, dens, nd, dd, cant=0.0):
     from pamtra_amd import _lib
     lib = _lib.library()
     ctx = _lib.get_context(0)
     n = len(dmax)
     arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, float), (n,)))
-            for a in (dmax, dd, nd, dens, ar)]
+            for a in (dmax, dd, nd, dens, ar, cant)]
     sm, ex, em = np.zeros(2 * NMU * 2 * NMU * 2), np.zeros(4 * NMU), np.zeros(2 * NMU)
     nm = np.zeros(n, np.int32)
-    print(f"pam_calc_tmatrix f={f / 1e9:.2f} GHz dmax={dmax} as_ratio={ar}", flush=True)
+    print(f"pam_calc_tmatrix f={f / 1e9:.2f} GHz dmax={dmax} as_ratio={ar} canting={cant}",
+          flush=True)
     st = lib.pam_calc_tmatrix(ctx, f, m.real, -m.imag, phase, n, *[_dp(a) for a in arrs],
                               _dp(sm), _dp(ex), _dp(em), nm.ctypes.data_as(C.POINTER(C.c_int32)))
     return (st, sm.reshape((2, NMU, 2, NMU, 2), order="F"), ex.reshape((2, 2, NMU), order="F"),
@@ -62,10 +67,10 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("case", CASES)
 def test_calc_tmatrix_matches_oracle(case):
-    f, m, phase, dmax, ar, dens = case
+    f, m, phase, dmax, ar, dens, cant = case
     nd, dd = [2.0e6, 5.0e5], [1.0e-4, 2.0e-4]
-    st, sm, ex, em, nm = _device_calc_tmatrix(f, m, phase, dmax, ar, dens, nd, dd)
-    o = oracle.calc_tmatrix(f, m, phase, dmax, dd, nd, dens, ar, 0.0)
+    st, sm, ex, em, nm = _device_calc_tmatrix(f, m, phase, dmax, ar, dens, nd, dd, cant)
+    o = oracle.calc_tmatrix(f, m, phase, dmax, dd, nd, dens, ar, cant)
     assert st == 0 and o[0] == 0
     assert np.all(nm >= 4)
     r = [_rel(sm, o[1]), _rel(ex, o[2]), _rel(em, o[3])]
@@ -80,11 +85,12 @@ def test_calc_tmatrix_refuses_bad_bins():
     assert st != 0
 
 
-def _scenario(as_ratio, precision="fp64"):
+def _scenario(as_ratio, precision="fp64", canting=-99.0):
     pam = build_pamtra()
     pam.df.data["scat_name"][:] = "tmatrix"
     pam.df.data["as_ratio"][:] = as_ratio
-    pam.nmlSet["precision"] = precision
+    pam.df.data["canting"][:] = canting
+    pam.set["precision"] = precision
     return pam
 
 
@@ -122,6 +128,17 @@ def test_golden_tmatrix_scenario_vs_oracle(as_ratio):
     np.testing.assert_array_equal(g["nlegen"], c["nlegen"])
     ok = np.isfinite(c["kexttot"])
     np.testing.assert_allclose(g["kexttot"][ok], c["kexttot"][ok], rtol=1e-6, atol=1e-300)
+
+
+def test_golden_tmatrix_scenario_with_canting_vs_oracle():
+    """Canted spheroids (30 degrees): the tilted-frame amplitude path."""
+    g, st = _gpu(_scenario(0.6, canting=30.0), FREQUENCIES)
+    c, st2 = _cpu(_scenario(0.6, canting=30.0), FREQUENCIES)
+    assert st == 0 and st2 == 0, g["msg"]
+    d = np.abs(g["tb"] - c["tb"])
+    print(f"canting 30: max |dTB| = {d.max():.3e} K", flush=True)
+    assert d.max() <= 1e-4
+    np.testing.assert_array_equal(g["ndoubles"], c["ndoubles"])
 
 
 def test_golden_tmatrix_scenario_mixed_precision():

@@ -183,7 +183,8 @@ def test_oblate_tmatrix_layers_polarise():
 
 def test_marshalling_refuses_unsupported_tmatrix_calls():
     """Mixing 'tmatrix' with the Legendre-series models (no defined result in
-    the reference), canting angles and the radar are refused up front."""
+    the reference) and the radar are refused up front; canting angles are
+    accepted."""
     from pamtra_amd import _capi
     from _scenario import build_pamtra, passive_nml
     pam = build_pamtra()
@@ -194,8 +195,7 @@ def test_marshalling_refuses_unsupported_tmatrix_calls():
     pam.df.data["scat_name"][:] = "tmatrix"
     _capi.marshal(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p)
     pam.df.data["canting"][:] = 10.0
-    with pytest.raises(NotImplementedError, match="canting"):
-        _capi.marshal(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p)
+    _capi.marshal(passive_nml(pam), pam.set, pam.df.data, pam.df.data4D, pam.p)
     pam.df.data["canting"][:] = -99.0
     nml = passive_nml(pam)
     nml["active"] = True
