@@ -373,8 +373,7 @@ def check_tmatrix(hydros, settings, nmlSets):
     with the Legendre-series models is refused because the reference's result
     is then undefined: the T-matrix branch adds salb_hydro * kext_hydro to the
     layer albedo without ever setting salb_hydro (scatProperties.f90:141,
-    911).  The radar simulator and full-spectrum input are not implemented
-    with it on the device.  tmatrix_db 'file' (the
+    911).  The radar simulator is not implemented with it on the device.  tmatrix_db 'file' (the
     reference's on-disk cache of the same results) computes them."""
     names = [int(h.scat_name) for h in hydros]
     if SCAT_NAMES["tmatrix"] not in names:
@@ -387,8 +386,6 @@ def check_tmatrix(hydros, settings, nmlSets):
     if settings.active:
         raise NotImplementedError("scat_name 'tmatrix' with active (radar) simulation is not "
                                   "implemented")
-    if nmlSets.get("hydro_fullspec", False):
-        raise NotImplementedError("scat_name 'tmatrix' with hydro_fullspec is not implemented")
     db = str(nmlSets.get("tmatrix_db", "none")).strip()
     if db not in ("none", "file"):
         raise ValueError(f"tmatrix_db {db!r} must be 'none' or 'file'")

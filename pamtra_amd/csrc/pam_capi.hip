@@ -769,8 +769,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
   }
   {
     // the T-matrix producer: not beside the Legendre-series models (the
-    // reference's layer albedo then reads an unset value), not with a radar
-    // or full spectra on the device
+    // reference's layer albedo then reads an unset value), not with the radar
     bool tm = false, series = false;
     for (int i = 0; i < d.nh; i++) {
       if (hydros[i].scat_name == PAM_SCAT_TMATRIX) {
@@ -783,8 +782,8 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
       set_msg(res, "tmatrix cannot be mixed with mie-sphere or ssrg-rt3 hydrometeors");
       return PAM_FATAL;
     }
-    if (tm && (st->active || (h4in && h4in->fs_nbin > 0))) {
-      set_msg(res, "tmatrix: radar simulation and full-spectrum input are not implemented");
+    if (tm && st->active) {
+      set_msg(res, "tmatrix: radar simulation is not implemented");
       return PAM_FATAL;
     }
   }
@@ -1150,7 +1149,7 @@ static int run_device(pam_context *c, const pam_settings *st, const pam_hydro_de
         break;
       case 2:
         if (s.tmatrix)
-          launch_tmatrix(dc, s, d_hyd, d_freqs, wc, sx);
+          launch_tmatrix(dc, s, d_hyd, h4, d_freqs, wc, sx);
         else
           launch_mie(dc, s, d_hyd, h4, p, d_freqs, wc, sx);
         break;

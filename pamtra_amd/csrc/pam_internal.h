@@ -342,7 +342,7 @@ void launch_phase(const Dims &d, const Settings &s, Work &w, hipStream_t st);
 void launch_mie_index(const Dims &d, const pam_hydro_desc *hyd, const double *freqs, Work &w,
                       hipStream_t st);
 void launch_tmatrix(const Dims &d, const Settings &s, const pam_hydro_desc *hyd,
-                    const double *freqs, Work &w, hipStream_t st);
+                    const Hydro4D &h4, const double *freqs, Work &w, hipStream_t st);
 size_t tm_slot_doubles();
 size_t tm_geo_doubles();
 std::vector<double> tm_gauss_table();

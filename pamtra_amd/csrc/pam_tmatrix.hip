@@ -1345,7 +1345,8 @@ __device__ bool tm_particle(double *S, TmLds &L, const double *geo, const double
 // their bins (tmatrix.f90:150-426, scatProperties.f90:925-927), written to
 // the unit's slots and per-angle arrays.
 __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *hyd,
-                        const double *freqs, const Work &w, int itf, double *S, TmLds &L) {
+                        const Hydro4D &h4, const double *freqs, const Work &w, int itf,
+                        double *S, TmLds &L) {
   const int t = threadIdx.x;
   const int li = itf / d.nf, f = itf % d.nf;
   double *part = S + TM_O_PART, *hsum = S + TM_O_HYD, *usum = S + TM_O_UNIT;
@@ -1369,18 +1370,27 @@ __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *
     for (int k = t; k < TM_NACC; k += TM_WG) hsum[k] = 0.0;
     const double *mi = w.mie_m + (size_t)itf * 4 + (hd.liq_ice == 1 ? 0 : 2);
     const cd ref = cd{mi[0], -mi[1]};  // m = n - i k (scatProperties.f90:243)
-    const double ar = w.dsd_ar[(size_t)li * d.nh + h];
-    const double cant = (hd.canting < 0.0 || isnan(hd.canting)) ? 0.0 : hd.canting;
-    if (!(ar > 0.0)) {
-      ok = false;
-      break;
-    }
+    // the descriptor's (or 4D) as_ratio and canting, or with full spectra the
+    // per-bin lists; negative or NaN read as 0 (scatProperties.f90:284-287)
+    const double ar0 = w.dsd_ar[(size_t)li * d.nh + h];
+    const double cant0 = (hd.canting < 0.0 || isnan(hd.canting)) ? 0.0 : hd.canting;
+    const bool fs = h4.fs_nbin > 0;
+    const size_t b5 =
+        fs ? (((size_t)w.item_col[li] * d.ML + w.item_layer[li]) * d.nh + h) * (size_t)h4.fs_nbin
+           : 0;
     const size_t ob = ((size_t)li * d.nh + h) * d.nbmax;
     for (int ir = 0; ir < nbin && ok; ir++) {
       const double nd = w.dsd_nd[ob + ir], dd_ = w.dsd_dd[ob + ir];
       const double rho = w.dsd_rho[ob + ir], dmax = w.dsd_d[ob + ir];
       if (nd == 0.0) continue;
-      if (!(nd > 0.0) || !(dd_ > 0.0) || !(rho > 0.0)) {
+      double ar = ar0, cant = cant0;
+      if (fs) {
+        ar = h4.fs_as_ratio[b5 + ir];
+        cant = h4.fs_canting[b5 + ir];
+        ar = (ar < 0.0 || isnan(ar)) ? 0.0 : ar;
+        cant = (cant < 0.0 || isnan(cant)) ? 0.0 : cant;
+      }
+      if (!(nd > 0.0) || !(dd_ > 0.0) || !(rho > 0.0) || !(ar > 0.0)) {
         ok = false;
         break;
       }
@@ -1443,12 +1453,12 @@ __device__ void tm_unit(const Dims &d, const Settings &s, const pam_hydro_desc *
 }
 
 __global__ void __launch_bounds__(TM_WG) __attribute__((amdgpu_waves_per_eu(2))) k_tmatrix(Dims d, Settings s, const pam_hydro_desc *hyd,
-                                                   const double *freqs, Work w) {
+                                                   Hydro4D h4, const double *freqs, Work w) {
   __shared__ TmLds L;
   double *S = w.tm_scr + (size_t)blockIdx.x * TM_SLOT;
   const int n = d.nitems * d.nf;
   for (int itf = next_unit(&w.queue[0], &L.slot); itf < n; itf = next_unit(&w.queue[0], &L.slot))
-    tm_unit(d, s, hyd, freqs, w, itf, S, L);
+    tm_unit(d, s, hyd, h4, freqs, w, itf, S, L);
 }
 
 // calc_tmatrix (tmatrix.f90:150-426) over the bins of one hydrometeor, per
@@ -1598,13 +1608,13 @@ std::vector<double> tm_gauss_table() {
 }
 
 void launch_tmatrix(const Dims &d, const Settings &s, const pam_hydro_desc *hyd,
-                    const double *freqs, Work &w, hipStream_t st) {
+                    const Hydro4D &h4, const double *freqs, Work &w, hipStream_t st) {
   const size_t n = (size_t)d.nitems * d.nf;
   if (n == 0) return;
   launch_mie_index(d, hyd, freqs, w, st);
   hipLaunchKernelGGL(k_tm_geom, dim3(8), dim3(256), 0, st, w.mu, w.tm_geo);
   const unsigned grid = (unsigned)std::min<size_t>(n, (size_t)w.tm_slots);
-  hipLaunchKernelGGL(k_tmatrix, dim3(grid), dim3(TM_WG), 0, st, d, s, hyd, freqs, w);
+  hipLaunchKernelGGL(k_tmatrix, dim3(grid), dim3(TM_WG), 0, st, d, s, hyd, h4, freqs, w);
 }
 
 size_t tm_nacc() { return TM_NACC; }

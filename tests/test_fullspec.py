@@ -144,3 +144,25 @@ def test_fullspec_with_varying_per_bin_parameters_vs_oracle():
     np.testing.assert_array_equal(g["nlegen"], c["nlegen"])
     mx = _gpu(pam, True, "mixed")
     assert np.abs(mx["tb"] - g["tb"]).max() <= 0.05
+
+
+@pytest.mark.gpu
+def test_fullspec_tmatrix_with_per_bin_axis_ratio_and_canting_vs_oracle():
+    """T-matrix hydrometeors from full spectra: per-bin as_ratio and canting
+    (scatProperties.f90:261-270), rain oblate, snow with canting in half its
+    bins; the device against the oracle (1e-4 K)."""
+    pam = _fullspec_from_dsd(_pam([RAIN, SNOW]))
+    pam.df.data["scat_name"][:] = "tmatrix"
+    fs = pam.df.dataFullSpec
+    nb = fs["d_ds"].shape[-1]
+    ramp = np.linspace(0.0, 1.0, nb)
+    fs["as_ratio"][..., 0, :] = 1.0 - 0.4 * ramp
+    fs["as_ratio"][..., 1, :] = 0.5 + 0.4 * ramp
+    fs["canting"][..., 1, nb // 2:] = 30.0
+    g = _gpu(pam, True)
+    c = _cpu(pam, True)
+    d = np.abs(g["tb"] - c["tb"]).max()
+    print(f"tmatrix fullspec: max |dTB| = {d:.3e} K", flush=True)
+    assert d <= 1e-4
+    np.testing.assert_array_equal(g["ndoubles"], c["ndoubles"])
+
