@@ -1346,12 +1346,6 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
   const double p0 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z + 1], wl);
   const double p1 = planck(w.temp_lev[(size_t)col * (d.ML + 1) + z], wl);
   const double ext = kext + gas;
-  // entry (r, c) of the extinction matrix (plus gas) of the layer
-  auto extrc = [&](int r, int c) -> double {
-    if (!tmx) return c == r ? ext : 0.0;
-    if (r / NST != c / NST) return 0.0;
-    return r == c ? tmx[2 * (r / NST)] + gas : tmx[2 * (r / NST) + 1];
-  };
   const double C2 = 2.0 * 3.1415926535897932384;
   if (t < NMU) {  // CHECK_NORM4
     double sum = tmx ? (s.lhyd_emis ? tme[2 * t] : 0.0) - tmx[2 * t] : absorb - kext;
@@ -1365,7 +1359,15 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
     double sum = 0.0;
     for (int c = 0; c < NN; c++) {
       const double wc = C2 * sqw[c / NST];
-      sum += fabs(wc * P1[r * NN + c]) + fabs(extrc(r, c) - wc * P0[r * NN + c]);
+      sum += fabs(wc * P1[r * NN + c]) + fabs((c == r ? ext : 0.0) - wc * P0[r * NN + c]);
+    }
+    if (tmx) {
+      // a T-matrix unit's per-angle extinction matrix: its diagonal entry and
+      // the same-angle Stokes partner (column r ^ 1) replace ext and 0
+      const int j = r / NST, c1 = r ^ 1;
+      const double w0 = C2 * sqw[j];
+      sum += fabs((tmx[2 * j] + gas) - w0 * P0[r * NN + r]) - fabs(ext - w0 * P0[r * NN + r]);
+      sum += fabs(tmx[2 * j + 1] - w0 * P0[r * NN + c1]) - fabs(0.0 - w0 * P0[r * NN + c1]);
     }
     L.hrow[r] = sum / smu[r / NST];
   }
@@ -1390,9 +1392,23 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
       const int e = t + q * RT_WG, r = e / NN, c = e % NN;
       const double sc = dz / smu[r / NST], wc = C2 * sqw[c / NST];
       const double a = sc * wc * P1[e];
-      const double b = sc * (extrc(r, c) - wc * P0[e]);
+      const double b = sc * ((c == r ? ext : 0.0) - wc * P0[e]);
       vs[q] = (float)(b + a);
       vd[q] = (float)(b - a);
+    }
+    // a T-matrix unit: the two entries of row t that its per-angle extinction
+    // matrix changes (diagonal, same-angle Stokes partner), patched below
+    float ts[2] = {0.0f, 0.0f}, td[2] = {0.0f, 0.0f};
+    if (tmx && t < NN) {
+      const int j = t / NST;
+      const double sc = dz / smu[j], wc = C2 * sqw[j];
+      for (int k = 0; k < 2; k++) {
+        const int c = k == 0 ? t : (t ^ 1), e = t * NN + c;
+        const double a = sc * wc * P1[e];
+        const double b = sc * ((k == 0 ? tmx[2 * j] + gas : tmx[2 * j + 1]) - wc * P0[e]);
+        ts[k] = (float)(b + a);
+        td[k] = (float)(b - a);
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -1400,6 +1416,15 @@ __device__ bool doubling_unit_f32(const Dims &d, const Settings &s, const double
       const int e = t + q * RT_WG, r = e / NN, c = e % NN;
       M0[r * LD + c] = vs[q];
       M1[r * LD + c] = vd[q];
+    }
+    if (tmx) {
+      __syncthreads();
+      if (t < NN) {
+        M0[t * LD + t] = ts[0];
+        M1[t * LD + t] = td[0];
+        M0[t * LD + (t ^ 1)] = ts[1];
+        M1[t * LD + (t ^ 1)] = td[1];
+      }
     }
   }
   double *cv = L.c[0], *cw = L.c[1];
